@@ -1,0 +1,133 @@
+// vh_advance.hip -- (b) the forward-Euler Maxwell memory update of every
+// (slot, Maxwell element): nodal strain U = sigma*xUn(l) + dUn (likewise V),
+// strain coefficients (src/vilma_viscoelastic.f90:329-359) and
+// tau <- (1-M) tau - 2 mu M eps for the three radial shape coefficients
+// (:577-584, driven by vilma_advance, src/vilma_response.f90:944-992).
+//
+// One streaming pass: tau_n is read from one buffer and tau_{n+1} written to
+// the other (the step controller's rollback is then free), and the same pass
+// produces the per-slot max |tau_{n+1}| that the next drift solve's skip gate
+// and the controller's memory-norm guard need (src/vilma_response.f90:719-728,
+// :1795-1811), so no separate scan of the memory is ever made.
+// Block = 64 slots (lanes, coalesced rows of tau) x 4 element groups.
+#include "vh_internal.h"
+#include "vh_radial_dev.h"
+
+struct advance_args {
+    int nr, nk, nem;
+    double dt;
+    vh_dev v;
+    const double* tau_in;
+    double* tau_out;
+    const double* sigma;   // complex degree-major
+    const double* dN;      // [mn][4][nk]
+    double* mnorm_out;
+    unsigned long long* maxbits;
+};
+
+__global__ void __launch_bounds__(256) k_advance(advance_args a) {
+    __shared__ double red[4][64];
+    const int lane = threadIdx.x & 63, eg = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + lane;
+    const int nk = a.nk;
+    double amax = 0.0;
+    if (k < nk) {
+        const int l = a.v.kdeg[k];
+        const double Jr = (double)l * (double)(l + 1);
+        const double sre = a.sigma[2 * (k + 1)], sim = a.sigma[2 * (k + 1) + 1];
+        const double* xU = a.v.xUn + (size_t)(l - 1) * a.nr;
+        const double* xV = a.v.xVn + (size_t)(l - 1) * a.nr;
+        for (int em = eg; em < a.nem; em += 4) {
+            const size_t base = (size_t)em * 24 * nk + k;
+            if (a.v.is3d[em]) {
+                for (int c = 0; c < 24; c++) amax = fmax(amax, fabs(a.tau_out[base + (size_t)c * nk]));
+                continue;
+            }
+            const int e = a.v.emap[em];
+            const int n0 = a.v.mnode[e], n1 = a.v.mnode[e + 1];
+            const double* d0 = a.dN + (size_t)n0 * 4 * nk + k;
+            const double* d1 = a.dN + (size_t)n1 * 4 * nk + k;
+            double U0r = sre * xU[e] + d0[0], U0i = sim * xU[e] + d0[(size_t)nk];
+            double V0r = sre * xV[e] + d0[(size_t)2 * nk], V0i = sim * xV[e] + d0[(size_t)3 * nk];
+            double U1r = sre * xU[e + 1] + d1[0], U1i = sim * xU[e + 1] + d1[(size_t)nk];
+            double V1r = sre * xV[e + 1] + d1[(size_t)2 * nk], V1i = sim * xV[e + 1] + d1[(size_t)3 * nk];
+            double s[2][3][4];
+            strain_coeffs(U0r, U1r, V0r, V1r, Jr, s[0][0], s[0][1], s[0][2]);
+            strain_coeffs(U0i, U1i, V0i, V1i, Jr, s[1][0], s[1][1], s[1][2]);
+            const double Me = a.v.MkPerDt[e] * a.dt;   // response_set_dt: Mk = MkPerDt*dt
+            const double om = 1.0 - Me, two_muM = 2.0 * a.v.mu[e] * Me;
+#pragma unroll
+            for (int sh = 0; sh < 3; sh++)
+#pragma unroll
+                for (int ri = 0; ri < 2; ri++)
+#pragma unroll
+                    for (int lam = 0; lam < 4; lam++) {
+                        size_t off = base + (size_t)((sh * 2 + ri) * 4 + lam) * nk;
+                        double t = om * a.tau_in[off] - two_muM * s[ri][sh][lam];
+                        a.tau_out[off] = t;
+                        amax = fmax(amax, fabs(t));
+                    }
+        }
+    }
+    red[eg][lane] = amax;
+    __syncthreads();
+    if (eg == 0) {
+        double m = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
+        if (k < nk) a.mnorm_out[k] = m;
+        red[0][lane] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x < 32) red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + 32]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int i = 0; i < 32; i++) m = fmax(m, red[0][i]);
+        // |tau| >= 0: the IEEE bit pattern orders like the value (NaN above Inf)
+        atomicMax(a.maxbits, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+void vh_launch_advance(hipStream_t st, const vh_dims& d, const vh_dev& v, double dt, const double* tau_in,
+                       double* tau_out, const double* sigma, const double* dN, double* mnorm_out,
+                       double* maxnorm_out) {
+    advance_args a;
+    a.nr = d.nr; a.nk = d.nk; a.nem = d.nem; a.dt = dt; a.v = v;
+    a.tau_in = tau_in; a.tau_out = tau_out; a.sigma = sigma; a.dN = dN;
+    a.mnorm_out = mnorm_out; a.maxbits = (unsigned long long*)maxnorm_out;
+    vh_fill(st, maxnorm_out, 0.0, 1);
+    int nb = (d.nk + 63) / 64;
+    k_advance<<<nb, 256, 0, st>>>(a);
+    HIPCHECK(hipGetLastError());
+}
+
+// Per-slot max |tau| and the global max of a memory state that did not come
+// out of an advance (a restart or a host-set memory).
+__global__ void __launch_bounds__(256) k_memnorm(int nk, int nem, const double* __restrict__ tau, double* mnorm,
+                                                 unsigned long long* maxbits) {
+    __shared__ double red[4][64];
+    const int lane = threadIdx.x & 63, eg = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + lane;
+    double amax = 0.0;
+    if (k < nk)
+        for (int em = eg; em < nem; em += 4)
+            for (int c = 0; c < 24; c++) amax = fmax(amax, fabs(tau[((size_t)em * 24 + c) * nk + k]));
+    red[eg][lane] = amax;
+    __syncthreads();
+    if (eg == 0) {
+        double m = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
+        if (k < nk) mnorm[k] = m;
+        red[0][lane] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int i = 0; i < 64; i++) m = fmax(m, red[0][i]);
+        atomicMax(maxbits, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+void vh_launch_memnorm(hipStream_t st, const vh_dims& d, const double* tau, double* mnorm, double* maxnorm) {
+    vh_fill(st, maxnorm, 0.0, 1);
+    k_memnorm<<<(d.nk + 63) / 64, 256, 0, st>>>(d.nk, d.nem, tau, mnorm, (unsigned long long*)maxnorm);
+    HIPCHECK(hipGetLastError());
+}

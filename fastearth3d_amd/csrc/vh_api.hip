@@ -1,0 +1,979 @@
+// vh_api.hip -- the C-ABI of include/vilma_hip.h and the host side of the
+// time step: model set-up, the forward-Euler sub-stepping controller, the
+// sea-level fixed point and the coupling step. All state lives in device
+// memory; the host only makes the discrete decisions (SLE exits, coastline
+// flips, the stability guard) from a few scalars read back per iteration.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "vh_internal.h"
+#include "vh_rot.h"
+#include "vh_sle.h"
+#include "../../include/vilma_hip.h"
+
+namespace {
+
+thread_local std::string g_create_error;
+
+template <class T>
+T* dalloc(size_t n) {
+    T* p = nullptr;
+    HIPCHECK(hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)));
+    return p;
+}
+template <class T>
+void upload(T* d, const T* h, size_t n, hipStream_t st) {
+    HIPCHECK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+// ---------------------------------------------------------- earth model
+struct layer { double r_bot, r_top, rho, mu, eta; int rheology; };
+enum { RHEOL_ELASTIC = 0, RHEOL_MAXWELL = 1, RHEOL_FLUID = 2 };
+
+// incompressible PREM (Dziewonski & Anderson 1981, Table I), as
+// src/vilma_earth_structure.f90:118-183 uses it
+void prem_rho_mu(double r, double& rho, double& mu) {
+    double x = r / 6371.0e3, vs;
+    if (r <= 1221.5e3) { rho = 13088.5 - 8838.1 * x * x; mu = 7.0363e10; return; }
+    if (r <= 3480.0e3) { rho = 12581.5 - 1263.8 * x - 3642.6 * x * x - 5528.1 * x * x * x; mu = 7.0363e10; return; }
+    if (r <= 3630.0e3) { vs = 6925.4 + 1467.2 * x - 2083.4 * x * x + 978.3 * x * x * x; rho = 7956.5 - 6476.1 * x + 5528.3 * x * x - 3080.7 * x * x * x; }
+    else if (r <= 5600.0e3) { vs = 11167.1 - 13781.8 * x + 17457.5 * x * x - 9277.7 * x * x * x; rho = 7956.5 - 6476.1 * x + 5528.3 * x * x - 3080.7 * x * x * x; }
+    else if (r <= 5701.0e3) { vs = 22345.9 - 17247.3 * x - 2083.4 * x * x + 978.3 * x * x * x; rho = 7956.5 - 6476.1 * x + 5528.3 * x * x - 3080.7 * x * x * x; }
+    else if (r <= 5771.0e3) { vs = 9983.9 - 4932.4 * x; rho = 5319.7 - 1483.6 * x; }
+    else if (r <= 5971.0e3) { vs = 22351.2 - 18585.6 * x; rho = 11249.4 - 8029.8 * x; }
+    else if (r <= 6151.0e3) { vs = 8949.6 - 4459.7 * x; rho = 7108.9 - 3804.5 * x; }
+    else if (r <= 6346.6e3) { vs = 2151.9 + 2348.1 * x; rho = 2691.0 + 692.4 * x; }
+    else if (r <= 6356.0e3) { vs = 3900.0; rho = 2900.0; }
+    else { vs = 3200.0; rho = 2600.0; }
+    mu = vs * vs * rho;
+}
+
+void prem_shell_avg(double rb, double rt, double& rho, double& mu) {
+    const int n = 200;
+    double h = (rt - rb) / n, nrho = 0, nmu = 0, den = 0;
+    for (int i = 0; i <= n; i++) {
+        double w = (i == 0 || i == n) ? 1.0 : ((i % 2) ? 4.0 : 2.0);
+        double r = rb + i * h, ww = w * r * r, ri, mi;
+        prem_rho_mu(r, ri, mi);
+        nrho += ww * ri; nmu += ww * mi; den += ww;
+    }
+    rho = nrho / den; mu = nmu / den;
+}
+
+// build_earth (src/vilma_earth_structure.f90:62-116, :220-238)
+std::vector<layer> build_earth(const vh_params& p, double& r_earth) {
+    std::vector<layer> L;
+    if (!strcmp(p.earth, "M3-L70-V01")) {
+        const double km = 1e3;
+        r_earth = 6371.0 * km;
+        L = {{6301.0 * km, 6371.0 * km, 3037.0, 5.0605e10, DBL_MAX, RHEOL_ELASTIC},
+             {5951.0 * km, 6301.0 * km, 3438.0, 7.0363e10, 1.0e21, RHEOL_MAXWELL},
+             {5701.0 * km, 5951.0 * km, 3871.0, 1.0549e11, 1.0e21, RHEOL_MAXWELL},
+             {3480.0 * km, 5701.0 * km, 4978.0, 2.2834e11, 2.0e21, RHEOL_MAXWELL},
+             {0.0, 3480.0 * km, 10750.0, 0.0, 0.0, RHEOL_FLUID}};
+        return L;
+    }
+    bool prem = !strcmp(p.earth, "PREM");
+    if (!prem && strcmp(p.earth, "custom")) throw vh_error(std::string("build_earth: unknown earth model \"") + p.earth + "\"");
+    if (p.n_layer < 1 || p.n_layer > VH_MAX_LAYER) throw vh_error("build_layered: n_layer out of range for a layered earth model");
+    r_earth = p.r_earth;
+    for (int k = 0; k < p.n_layer; k++) {
+        double rho = p.rho[k], mu = p.mu[k];
+        if (prem) { prem_shell_avg(p.r_bot[k], p.r_top[k], rho, mu); if (p.rheology[k] == RHEOL_FLUID) mu = 0.0; }
+        L.push_back({p.r_bot[k], p.r_top[k], rho, mu, p.eta[k], p.rheology[k]});
+    }
+    return L;
+}
+
+double gravity_at(const std::vector<layer>& L, double r) {
+    double m = 0.0;
+    for (auto& ly : L) {
+        double lo = ly.r_bot, hi = std::min(ly.r_top, r);
+        if (hi > lo) m += (4.0 / 3.0) * VH_PI * ly.rho * (hi * hi * hi - lo * lo * lo);
+    }
+    return r > 0.0 ? VH_G * m / (r * r) : 0.0;
+}
+
+// radial_mesh_build (src/vilma_radial_fe.f90:153-224) + per-layer counts
+void build_mesh(const std::vector<layer>& L, double r_earth, const int* nel, std::vector<double>& r,
+                std::vector<int>& lay) {
+    int n = (int)L.size();
+    if (std::fabs(L[0].r_top - r_earth) > 1.0) throw vh_error("radial_mesh_build: r_top(1) must equal r_earth (layers are surface-first)");
+    if (L[n - 1].r_bot != 0.0) throw vh_error("radial_mesh_build: the innermost layer must reach r = 0");
+    for (int i = 0; i < n; i++) {
+        if (L[i].r_top <= L[i].r_bot) throw vh_error("radial_mesh_build: every layer needs r_top > r_bot (surface-first ordering)");
+        if (i < n - 1 && std::fabs(L[i].r_bot - L[i + 1].r_top) > 1.0) throw vh_error("radial_mesh_build: layers must be contiguous, r_bot(k) = r_top(k+1)");
+    }
+    auto count = [&](int i) {
+        if (nel && nel[i] > 0) return nel[i];
+        double r0 = L[i].r_bot, r1 = L[i].r_top, depth = r_earth - 0.5 * (r0 + r1);
+        double dr = depth <= 70.0e3 ? 5.0e3 : (depth <= 670.0e3 ? 10.0e3 : 40.0e3);
+        return std::max(1, (int)std::ceil((r1 - r0) / dr));
+    };
+    int ne = 0;
+    for (int i = 0; i < n; i++) ne += count(i);
+    r.assign(ne + 1, 0.0);
+    lay.assign(ne, 0);
+    int off = 0;
+    for (int i = n - 1; i >= 0; i--) {
+        int nl = count(i);
+        double h = (L[i].r_top - L[i].r_bot) / (double)nl;
+        for (int k = 1; k <= nl; k++) { r[off + k] = L[i].r_bot + (double)k * h; lay[off + k - 1] = i; }
+        off += nl;
+    }
+    r[ne] = r_earth;
+}
+
+// Gauss-Legendre nodes (descending) and weights
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+    x.assign(n, 0.0); w.assign(n, 0.0);
+    for (int i = 0; i < (n + 1) / 2; i++) {
+        double z = std::cos(VH_PI * (i + 0.75) / (n + 0.5)), z1, pp = 0.0;
+        for (int it = 0; it < 100; it++) {
+            double p1 = 1.0, p2 = 0.0;
+            for (int k = 1; k <= n; k++) { double p3 = p2; p2 = p1; p1 = ((2.0 * k - 1.0) * z * p2 - (k - 1.0) * p3) / k; }
+            pp = n * (z * p1 - p2) / (z * z - 1.0);
+            z1 = z; z = z1 - p1 / pp;
+            if (std::fabs(z - z1) <= 1e-16) break;
+        }
+        double p1 = 1.0, p2 = 0.0;
+        for (int k = 1; k <= n; k++) { double p3 = p2; p2 = p1; p1 = ((2.0 * k - 1.0) * z * p2 - (k - 1.0) * p3) / k; }
+        pp = n * (z * p1 - p2) / (z * z - 1.0);
+        x[i] = z; x[n - 1 - i] = -z;
+        w[i] = 2.0 / ((1.0 - z * z) * pp * pp); w[n - 1 - i] = w[i];
+    }
+    if (n % 2) x[n / 2] = 0.0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ model
+struct vh_model {
+    vh_params par{};
+    std::string err;
+    vh_dims d{};
+    vh_sht s{};
+    vh_dev v{};
+    hipStream_t st = nullptr;
+    int kind = 2;            // 0 null, 1 elastic, 2 ve
+    int deg1_cm = 1;
+    double skip_tol = 1e-4;
+    double a = 0, g = 0, dt = 0, time_resp = 0, time_yr = 0, bsl = 0, worst_mass_resid = 0;
+    // host geometry
+    std::vector<double> r, mu, rho, MkPerDt, Rk, colat, lon;
+    std::vector<int> lay, emap, mnodes, mnode_of, kdeg;
+    double rate_max = 0.0;   // max Maxwell rate incl. lateral field
+    // device: radial
+    double *d_r = 0, *d_mu = 0, *d_rho = 0, *d_Rk = 0, *d_MkPerDt = 0;
+    int *d_emap = 0, *d_mnodes = 0, *d_mnode = 0, *d_kdeg = 0;
+    unsigned char* d_is3d = 0;
+    double *d_ab = 0, *d_dr = 0, *d_dc = 0, *d_ab1 = 0, *d_null1 = 0, *d_gu = 0, *d_gn = 0, *d_gv = 0;
+    double *d_xUn = 0, *d_xVn = 0;
+    int *d_ipiv = 0, *d_ipiv1 = 0, *d_fail = 0;
+    double *d_tau[2] = {0, 0}, *d_mnorm[2] = {0, 0}, *d_maxn = 0;   // d_maxn[2]
+    int cur = 0;
+    double maxn_cur = 0.0;
+    double *d_dN = 0, *d_dsurf = 0;
+    vh_drift_plan* dplan = nullptr;
+    // device: SHT
+    double *d_plm_lj = 0, *d_dplm_lj = 0, *d_plm_jl = 0, *d_dplm_jl = 0, *d_w = 0, *d_st = 0, *d_ct = 0;
+    int* d_mstart = 0;
+    double2* d_tw = 0;
+    // device: grid state
+    double *d_topo0 = 0, *d_heq = 0, *d_C0ref = 0, *d_hice = 0, *d_ice1 = 0, *d_rsl = 0, *d_rsl_n = 0;
+    double *d_C = 0, *d_Cn = 0, *d_dig = 0, *d_C0s = 0, *d_ice = 0, *d_dice = 0, *d_wcorr = 0;
+    double *d_sraw = 0, *d_srot = 0, *d_zbed = 0, *d_gtmp = 0;
+    double2 *d_F = 0, *d_F2 = 0, *d_sigma = 0, *d_lm = 0, *d_uN = 0;
+    double *d_part = 0, *d_sc = 0;
+    double* h_pin = nullptr;
+    // rotation
+    bool rot_on = false;
+    rot_args ra{};
+    double *d_sc2 = 0, *d_cphi = 0, *d_sphi = 0;
+    double dt_fe_max = DBL_MAX, rot_kTe = 0, rot_hTe = 0, rot_ks_fluid = 0;
+    // SLE knobs + counters
+    int n_outer = 3, n_inner = 20, fixed_ocean = 0, subgrid = 1;
+    double tol = 1e-7, cfl = 1, dt_min = 0, dt_max = DBL_MAX, guard = 1e3;
+    long n_accept = 0, n_reject = 0, n_solve = 0, n_inner_tot = 0, n_outer_tot = 0;
+    bool lat_visc = false;
+    int ne3d = 0;
+
+    size_t ng() const { return (size_t)d.nphi * d.nlat; }
+    void sync() { HIPCHECK(hipStreamSynchronize(st)); }
+    // read n device doubles (in-order after the stream's work)
+    void read(const double* dsrc, int n) {
+        HIPCHECK(hipMemcpyAsync(h_pin, dsrc, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        sync();
+    }
+    void refresh_dev() {
+        v.r = d_r; v.mu = d_mu; v.MkPerDt = d_MkPerDt; v.emap = d_emap; v.mnodes = d_mnodes; v.kdeg = d_kdeg;
+        v.mnode = d_mnode; v.is3d = d_is3d; v.ab = d_ab; v.ipiv = d_ipiv; v.dr = d_dr; v.dc = d_dc;
+        v.ab1 = d_ab1; v.ipiv1 = d_ipiv1; v.null1 = d_null1; v.gu = d_gu; v.gn = d_gn; v.gv = d_gv;
+        v.xUn = d_xUn; v.xVn = d_xVn;
+    }
+    ~vh_model();
+};
+
+vh_model::~vh_model() {
+    if (dplan) vh_drift_plan_destroy(dplan);
+    void* ptrs[] = {d_r, d_mu, d_rho, d_Rk, d_MkPerDt, d_emap, d_mnodes, d_mnode, d_kdeg, d_is3d, d_ab, d_dr, d_dc,
+                    d_ab1, d_null1, d_gu, d_gn, d_gv, d_xUn, d_xVn, d_ipiv, d_ipiv1, d_fail, d_tau[0], d_tau[1],
+                    d_mnorm[0], d_mnorm[1], d_maxn, d_dN, d_dsurf, d_plm_lj, d_dplm_lj, d_plm_jl, d_dplm_jl, d_w,
+                    d_st, d_ct, d_mstart, d_tw, d_topo0, d_heq, d_C0ref, d_hice, d_ice1, d_rsl, d_rsl_n, d_C,
+                    d_Cn, d_dig, d_C0s, d_ice, d_dice, d_wcorr, d_sraw, d_srot, d_zbed, d_gtmp, d_F, d_F2,
+                    d_sigma, d_lm, d_uN, d_part, d_sc, d_sc2, d_cphi, d_sphi,
+                    ra.rs.mem, ra.rs.MkPerDt, ra.rs.xU, ra.rs.xV, ra.rs.dN, ra.rs.sc};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (h_pin) (void)hipHostFree(h_pin);
+    if (st) (void)hipStreamDestroy(st);
+}
+
+namespace {
+
+void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_bed_eq, const double* h_ice_eq,
+           const double* h_ice_init) {
+    h->par = *p;
+    if (strcmp(p->scheme, "fe")) throw vh_error(std::string("scheme \"") + p->scheme + "\" not supported (fe only)");
+    if (p->l_toroidal) throw vh_error("l_toroidal = .true. is not supported");
+    if ((p->mmax >= 0 && p->mmax != p->lmax) || p->mres != 1) throw vh_error("mmax/mres other than lmax/1 not supported");
+    if (p->lmax < 2) throw vh_error("lmax must be >= 2");
+    int nl = p->nlat > 0 ? p->nlat : 2 * p->lmax + 2;
+    int np = p->nphi > 0 ? p->nphi : 4 * p->lmax;
+    nl = std::max(nl, 32);
+    if (nl != nlat || np != nphi) throw vh_error("grid does not match the model Gauss grid");
+    if (np <= 2 * p->lmax) throw vh_error("sht_grid_init: nphi must exceed 2*lmax");
+    if (nl <= p->lmax) throw vh_error("sht_grid_init: nlat must exceed lmax");
+    if (!strcmp(p->earth_response, "ve")) h->kind = 2;
+    else if (!strcmp(p->earth_response, "elastic")) h->kind = 1;
+    else if (!strcmp(p->earth_response, "null")) h->kind = 0;
+    else throw vh_error("solid_earth_init: unknown earth_response (use ve|elastic|null)");
+    h->deg1_cm = !strcmp(p->deg1_frame, "cm");
+    h->skip_tol = p->skip_tol;
+    HIPCHECK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+    hipStream_t st = h->st;
+    HIPCHECK(hipHostMalloc((void**)&h->h_pin, 64 * sizeof(double), hipHostMallocDefault));
+
+    // ---- earth, mesh, element tables (setup, host)
+    double r_earth = 0;
+    std::vector<layer> L = build_earth(*p, r_earth);
+    build_mesh(L, r_earth, p->n_elem_layer, h->r, h->lay);
+    const int nr = (int)h->r.size(), ne = nr - 1, Lmax = p->lmax;
+    h->a = r_earth;
+    h->g = gravity_at(L, r_earth);
+    h->mu.resize(ne); h->rho.resize(ne); h->MkPerDt.resize(ne); h->Rk.assign(ne, 0.0);
+    for (int e = 0; e < ne; e++) {
+        const layer& ly = L[h->lay[e]];
+        h->mu[e] = ly.mu; h->rho[e] = ly.rho;
+        h->MkPerDt[e] = ly.rheology == RHEOL_MAXWELL ? h->mu[e] / ly.eta : 0.0;
+    }
+    for (int k = 1; k < ne; k++)
+        h->Rk[k] = h->Rk[k - 1] + (h->rho[k - 1] - h->rho[k]) * h->r[k] * h->r[k] * h->r[k];
+    h->mnode_of.assign(nr, -1);
+    for (int e = 0; e < ne; e++)
+        if (h->MkPerDt[e] > 0.0) { h->emap.push_back(e); h->mnode_of[e] = 1; h->mnode_of[e + 1] = 1; }
+    for (int n = 0; n < nr; n++)
+        if (h->mnode_of[n] > 0) { h->mnode_of[n] = (int)h->mnodes.size(); h->mnodes.push_back(n); }
+    h->rate_max = 0.0;
+    for (int e = 0; e < ne; e++) h->rate_max = std::max(h->rate_max, h->MkPerDt[e]);
+
+    vh_dims& d = h->d;
+    d.L = Lmax; d.nlm = (Lmax + 1) * (Lmax + 2) / 2; d.nr = nr; d.ne = ne; d.nd = 4 * nr - 1;
+    d.nk = Lmax * (Lmax + 3) / 2; d.nem = (int)h->emap.size(); d.nmn = (int)h->mnodes.size();
+    d.nphi = np; d.nlat = nl; d.nh = (nl + 1) / 2;
+    d.ns1 = d.nd + 1; d.kl1 = d.nd; d.ldab1 = 3 * d.nd + 1;
+    h->kdeg.resize(d.nk);
+    for (int l = 1, k = 0; l <= Lmax; l++)
+        for (int m = 0; m <= l; m++) h->kdeg[k++] = l;
+    double dt0 = p->dt_init > 0.0 ? p->dt_init : 1000.0 * VH_SEC_PER_YEAR;
+    h->dt = dt0;
+
+    // ---- device radial data
+    h->d_r = dalloc<double>(nr); upload(h->d_r, h->r.data(), nr, st);
+    h->d_mu = dalloc<double>(ne); upload(h->d_mu, h->mu.data(), ne, st);
+    h->d_rho = dalloc<double>(ne); upload(h->d_rho, h->rho.data(), ne, st);
+    h->d_Rk = dalloc<double>(ne); upload(h->d_Rk, h->Rk.data(), ne, st);
+    h->d_MkPerDt = dalloc<double>(ne); upload(h->d_MkPerDt, h->MkPerDt.data(), ne, st);
+    h->d_emap = dalloc<int>(d.nem); upload(h->d_emap, h->emap.data(), d.nem, st);
+    h->d_mnodes = dalloc<int>(d.nmn); upload(h->d_mnodes, h->mnodes.data(), d.nmn, st);
+    h->d_mnode = dalloc<int>(nr); upload(h->d_mnode, h->mnode_of.data(), nr, st);
+    h->d_kdeg = dalloc<int>(d.nk); upload(h->d_kdeg, h->kdeg.data(), d.nk, st);
+    std::vector<unsigned char> zero3d(std::max(d.nem, 1), 0);
+    h->d_is3d = dalloc<unsigned char>(d.nem); upload(h->d_is3d, zero3d.data(), d.nem, st);
+    h->d_gu = dalloc<double>(Lmax + 1); h->d_gn = dalloc<double>(Lmax + 1); h->d_gv = dalloc<double>(Lmax + 1);
+    h->d_xUn = dalloc<double>((size_t)Lmax * nr); h->d_xVn = dalloc<double>((size_t)Lmax * nr);
+    h->d_fail = dalloc<int>(1);
+    HIPCHECK(hipMemsetAsync(h->d_fail, 0, sizeof(int), st));
+    {   // degree 0: no deformation, monopole geoid (src/vilma_response.f90:599-605)
+        double g0[3] = {0.0, 4.0 * VH_PI * VH_G * h->a / h->g, 0.0};
+        upload(h->d_gu, &g0[0], 1, st); upload(h->d_gn, &g0[1], 1, st); upload(h->d_gv, &g0[2], 1, st);
+    }
+
+    if (h->kind >= 1) {
+        // ---- per-degree operators, factors and unit-load response (GPU)
+        h->d_ab = dalloc<double>((size_t)(Lmax - 1) * VH_LDAB * d.nd);
+        h->d_ipiv = dalloc<int>((size_t)(Lmax - 1) * d.nd);
+        h->d_dr = dalloc<double>((size_t)Lmax * d.nd);
+        h->d_dc = dalloc<double>((size_t)Lmax * d.nd);
+        h->d_ab1 = dalloc<double>((size_t)d.ldab1 * d.ns1);
+        h->d_ipiv1 = dalloc<int>(d.ns1);
+        h->d_null1 = dalloc<double>(d.nd);
+        double* work1 = dalloc<double>((size_t)15 * d.nd + 2);
+        vh_launch_setup_band(st, d, 2, Lmax, h->d_r, h->d_mu, h->d_rho, h->d_Rk, h->a, h->g, 0, h->d_ab, h->d_ipiv,
+                             h->d_dr, h->d_dc, nullptr, h->d_xUn, h->d_xVn, h->d_gu, h->d_gn, h->d_gv, h->d_fail);
+        vh_launch_setup_deg1(st, d, h->d_r, h->d_mu, h->d_rho, h->d_Rk, h->a, h->g, h->deg1_cm, h->d_ab1, h->d_ipiv1,
+                             h->d_dr, h->d_dc, h->d_null1, h->d_xUn, h->d_xVn, h->d_gu, h->d_gn, h->d_gv, work1,
+                             h->d_fail);
+        HIPCHECK(hipStreamSynchronize(st));
+        (void)hipFree(work1);
+        int fail = 0;
+        HIPCHECK(hipMemcpy(&fail, h->d_fail, sizeof(int), hipMemcpyDeviceToHost));
+        if (fail) throw vh_error("bordered_band_factor: band LU factorization failed");
+    }
+    if (h->kind == 2) {
+        size_t nm = (size_t)24 * std::max(d.nem, 1) * d.nk;
+        for (int b = 0; b < 2; b++) {
+            h->d_tau[b] = dalloc<double>(nm);
+            h->d_mnorm[b] = dalloc<double>(d.nk);
+            vh_fill(st, h->d_tau[b], 0.0, nm);
+            vh_fill(st, h->d_mnorm[b], 0.0, d.nk);
+        }
+        h->d_maxn = dalloc<double>(2);
+        vh_fill(st, h->d_maxn, 0.0, 2);
+        h->d_dN = dalloc<double>((size_t)4 * std::max(d.nmn, 1) * d.nk);
+        h->d_dsurf = dalloc<double>((size_t)6 * d.nk);
+        vh_fill(st, h->d_dN, 0.0, (size_t)4 * std::max(d.nmn, 1) * d.nk);
+        vh_fill(st, h->d_dsurf, 0.0, (size_t)6 * d.nk);
+        h->dplan = vh_drift_plan_create(d);
+    }
+    h->refresh_dev();
+
+    // ---- SHT plan
+    std::vector<double> ct, w;
+    gauss_legendre(nl, ct, w);
+    std::vector<double> sn(nl);
+    h->colat.resize(nl);
+    for (int j = 0; j < nl; j++) { h->colat[j] = std::acos(ct[j]); sn[j] = std::sqrt((1.0 - ct[j]) * (1.0 + ct[j])); }
+    h->lon.resize(np);
+    for (int i = 0; i < np; i++) h->lon[i] = 2.0 * VH_PI * i / np;
+    std::vector<int> mstart(Lmax + 1);
+    for (int m = 0, off = 0; m <= Lmax; m++) { mstart[m] = off; off += Lmax - m + 1; }
+    std::vector<double2> tw(np);
+    for (int t = 0; t < np; t++) tw[t] = make_double2(std::cos(-2.0 * VH_PI * t / np), std::sin(-2.0 * VH_PI * t / np));
+    vh_sht& s = h->s;
+    s.L = Lmax; s.nlat = nl; s.nphi = np; s.nh = d.nh; s.nlm = d.nlm;
+    {
+        int n = np, nf = 0;
+        const int pref[5] = {4, 2, 3, 5, 7};
+        for (int q = 0; q < 5 && n > 1; q++)
+            while (n % pref[q] == 0) { s.fac[nf++] = pref[q]; n /= pref[q]; }
+        if (n != 1) throw vh_error("sht_grid_init: nphi must factor into 2, 3, 5, 7");
+        if (nf > 16) throw vh_error("sht_grid_init: nphi has too many factors");
+        s.nfac = nf;
+    }
+    h->d_w = dalloc<double>(nl); upload(h->d_w, w.data(), nl, st);
+    h->d_ct = dalloc<double>(nl); upload(h->d_ct, ct.data(), nl, st);
+    h->d_st = dalloc<double>(nl); upload(h->d_st, sn.data(), nl, st);
+    h->d_mstart = dalloc<int>(Lmax + 1); upload(h->d_mstart, mstart.data(), Lmax + 1, st);
+    h->d_tw = dalloc<double2>(np); upload(h->d_tw, tw.data(), np, st);
+    size_t ntab = (size_t)d.nlm * d.nh;
+    h->d_plm_lj = dalloc<double>(ntab); h->d_dplm_lj = dalloc<double>(ntab);
+    h->d_plm_jl = dalloc<double>(ntab); h->d_dplm_jl = dalloc<double>(ntab);
+    s.plm_lj = h->d_plm_lj; s.dplm_lj = h->d_dplm_lj; s.plm_jl = h->d_plm_jl; s.dplm_jl = h->d_dplm_jl;
+    s.mstart = h->d_mstart; s.w = h->d_w; s.st = h->d_st; s.ct = h->d_ct; s.tw = h->d_tw;
+    vh_setup_legendre_tables(st, s, h->d_ct, h->d_st, h->d_plm_lj, h->d_dplm_lj, h->d_plm_jl, h->d_dplm_jl);
+
+    // ---- grid state (solid_earth_init, src/vilma_coupling.f90:196-215)
+    size_t ng = h->ng();
+    double** grids[] = {&h->d_topo0, &h->d_heq, &h->d_C0ref, &h->d_hice, &h->d_ice1, &h->d_rsl, &h->d_rsl_n,
+                        &h->d_C, &h->d_Cn, &h->d_dig, &h->d_C0s, &h->d_ice, &h->d_dice, &h->d_wcorr, &h->d_sraw,
+                        &h->d_srot, &h->d_zbed, &h->d_gtmp};
+    for (double** g : grids) { *g = dalloc<double>(ng); vh_fill(st, *g, 0.0, ng); }
+    upload(h->d_topo0, z_bed_eq, ng, st);
+    upload(h->d_heq, h_ice_eq, ng, st);
+    upload(h->d_hice, h_ice_init ? h_ice_init : h_ice_eq, ng, st);
+    upload(h->d_zbed, z_bed_eq, ng, st);
+    grid_ocean(st, (int)ng, h->d_topo0, h->d_heq, h->d_C0ref);
+    HIPCHECK(hipMemcpyAsync(h->d_C, h->d_C0ref, ng * sizeof(double), hipMemcpyDeviceToDevice, st));
+    h->d_F = dalloc<double2>((size_t)nl * (Lmax + 1));
+    h->d_F2 = dalloc<double2>((size_t)2 * nl * (Lmax + 1));
+    h->d_sigma = dalloc<double2>(d.nlm); vh_fill(st, (double*)h->d_sigma, 0.0, 2 * (size_t)d.nlm);
+    h->d_lm = dalloc<double2>(d.nlm);
+    h->d_uN = dalloc<double2>((size_t)2 * d.nlm);
+    h->d_part = dalloc<double>((size_t)4 * nl + 64);
+    h->d_sc = dalloc<double>(64); vh_fill(st, h->d_sc, 0.0, 64);
+
+    // ---- SLE / stepper knobs (build_solver, src/vilma_coupling.f90:291-308)
+    h->n_outer = p->sle_n_outer; h->n_inner = p->sle_n_inner; h->tol = p->sle_tol;
+    h->fixed_ocean = p->sle_fixed_ocean; h->subgrid = p->sle_subgrid;
+    h->cfl = p->cfl; h->dt_min = p->dt_min; h->dt_max = p->dt_max;
+
+    // ---- rotation (rotation_init, src/vilma_rotation.f90:114-152)
+    if (p->rotation && h->kind >= 1) {
+        h->rot_on = true;
+        rot_state& rs = h->ra.rs;
+        rs.mem = dalloc<double>((size_t)2 * ne * 24); vh_fill(st, rs.mem, 0.0, (size_t)2 * ne * 24);
+        rs.MkPerDt = dalloc<double>((size_t)2 * ne);
+        rs.xU = dalloc<double>((size_t)2 * nr); rs.xV = dalloc<double>((size_t)2 * nr);
+        rs.dN = dalloc<double>((size_t)8 * nr); vh_fill(st, rs.dN, 0.0, (size_t)8 * nr);
+        rs.sc = dalloc<double>(RS_N); vh_fill(st, rs.sc, 0.0, RS_N);
+        std::vector<double> rk2(2 * ne);
+        double mx = 0.0;
+        for (int e = 0; e < ne; e++) {
+            double eta = L[h->lay[e]].eta;
+            double v = eta > 0.0 ? h->mu[e] / eta : 0.0;
+            rk2[e] = v; rk2[ne + e] = v;
+            mx = std::max(mx, v);
+        }
+        if (mx > 0.0) h->dt_fe_max = 1.0 / mx;
+        upload(rs.MkPerDt, rk2.data(), 2 * ne, st);
+        // unit responses: load and tidal on the degree-2 operator, tidal on the
+        // Maxwell-fluidised earth (fluid_tidal_k, :375-404)
+        std::vector<double> mu_f(ne);
+        for (int e = 0; e < ne; e++) mu_f[e] = L[h->lay[e]].rheology == RHEOL_MAXWELL ? 0.0 : h->mu[e];
+        double* d_muf = dalloc<double>(ne); upload(d_muf, mu_f.data(), ne, st);
+        double* sab = dalloc<double>((size_t)VH_LDAB * d.nd * 3);
+        int* sip = dalloc<int>((size_t)d.nd * 3);
+        double* sdr = dalloc<double>((size_t)3 * d.nd);   // index l-1 = 1 used
+        double* sdc = dalloc<double>((size_t)3 * d.nd);
+        double* xf = dalloc<double>((size_t)3 * d.nd);
+        vh_launch_setup_band(st, d, 2, 2, h->d_r, h->d_mu, h->d_rho, h->d_Rk, h->a, h->g, 0, sab, sip, sdr, sdc,
+                             xf, nullptr, nullptr, nullptr, nullptr, nullptr, h->d_fail);
+        vh_launch_setup_band(st, d, 2, 2, h->d_r, h->d_mu, h->d_rho, h->d_Rk, h->a, h->g, 1, sab, sip, sdr, sdc,
+                             xf + d.nd, nullptr, nullptr, nullptr, nullptr, nullptr, h->d_fail);
+        vh_launch_setup_band(st, d, 2, 2, h->d_r, d_muf, h->d_rho, h->d_Rk, h->a, h->g, 1, sab, sip, sdr, sdc,
+                             xf + 2 * d.nd, nullptr, nullptr, nullptr, nullptr, nullptr, h->d_fail);
+        std::vector<double> x3((size_t)3 * d.nd);
+        HIPCHECK(hipMemcpyAsync(x3.data(), xf, x3.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        h->sync();
+        (void)hipFree(d_muf); (void)hipFree(sab); (void)hipFree(sip); (void)hipFree(sdr); (void)hipFree(sdc); (void)hipFree(xf);
+        int fail = 0;
+        HIPCHECK(hipMemcpy(&fail, h->d_fail, sizeof(int), hipMemcpyDeviceToHost));
+        if (fail) throw vh_error("bordered_band_factor: band LU factorization failed");
+        const double* xl = x3.data();
+        const double* xt = x3.data() + d.nd;
+        const double* xfl = x3.data() + 2 * d.nd;
+        int sidx = nr - 1;
+        std::vector<double> xu(2 * nr), xv(2 * nr);
+        for (int n = 0; n < nr; n++) {
+            xu[n] = xl[4 * n]; xv[n] = xl[4 * n + 1];
+            xu[nr + n] = xt[4 * n]; xv[nr + n] = xt[4 * n + 1];
+        }
+        upload(rs.xU, xu.data(), 2 * nr, st);
+        upload(rs.xV, xv.data(), 2 * nr, st);
+        double Fe_tidal = xt[4 * sidx + 2];
+        h->rot_kTe = -Fe_tidal - 1.0;
+        h->rot_hTe = h->g * xt[4 * sidx];
+        h->rot_ks_fluid = -xfl[4 * sidx + 2] / 1.0 - 1.0;
+        double sc[RS_N] = {0};
+        sc[RS_FE_LOAD] = xl[4 * sidx + 2];
+        sc[RS_UE_TIDAL] = xt[4 * sidx];
+        sc[RS_KTE] = h->rot_kTe;
+        sc[RS_KS] = h->rot_ks_fluid;
+        sc[RS_A] = h->a; sc[RS_G] = h->g; sc[RS_CMA] = 2.63e35; sc[RS_OMEGA] = 7.292115e-5;
+        upload(rs.sc, sc, RS_N, st);
+        h->ra.nr = nr; h->ra.nd = d.nd; h->ra.r = h->d_r; h->ra.mu = h->d_mu;
+        h->ra.ab2 = h->d_ab; h->ra.ipiv2 = h->d_ipiv; h->ra.dr2 = h->d_dr + d.nd; h->ra.dc2 = h->d_dc + d.nd;
+        std::vector<double> sc2(nl), cph(np), sph(np);
+        for (int j = 0; j < nl; j++) sc2[j] = std::sin(h->colat[j]) * std::cos(h->colat[j]);
+        for (int i = 0; i < np; i++) { cph[i] = std::cos(h->lon[i]); sph[i] = std::sin(h->lon[i]); }
+        h->d_sc2 = dalloc<double>(nl); upload(h->d_sc2, sc2.data(), nl, st);
+        h->d_cphi = dalloc<double>(np); upload(h->d_cphi, cph.data(), np, st);
+        h->d_sphi = dalloc<double>(np); upload(h->d_sphi, sph.data(), np, st);
+    }
+    h->sync();
+}
+
+// ---------------------------------------------------------------- SLE solve
+struct sle_res { int n_outer_done = 0, n_inner_last = 0, n_coast_flip = 0; double mass_resid = 0, esl = 0; };
+
+// sle_solve (src/vilma_sle.f90:124-405), forward-Euler response branch.
+// The advance writes tau_{n+1} into the spare buffer; the caller commits it.
+sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double frac, int interp, bool report_only,
+                  const double* srot) {
+    hipStream_t st = h->st;
+    const vh_sht& s = h->s;
+    const int n = (int)h->ng();
+    sle_res res;
+    sle_ice(st, n, ice0, ice1, frac, interp, h->d_heq, h->d_topo0, h->d_ice, h->d_dice, h->d_C0s);
+    if (h->kind == 2)
+        vh_drift_run(st, h->dplan, h->d, h->v, h->d_tau[h->cur], h->d_mnorm[h->cur], h->d_maxn + h->cur, h->skip_tol,
+                     h->deg1_cm, h->d_dN, h->d_dsurf);
+    vh_fill(st, h->d_wcorr, 0.0, n);
+    double C_int = 0.0, ice_int = 0.0, zeta_int = 0.0;
+    for (int io = 1; io <= h->n_outer; io++) {
+        int first = 0;
+        if (h->fixed_ocean) {
+            if (io == 1) HIPCHECK(hipMemcpyAsync(h->d_Cn, h->d_C0s, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        } else if (io == 1) {
+            first = 1;
+        }
+        sle_outer(st, s, first, h->d_topo0, h->d_rsl, h->d_ice, h->d_dice, h->d_C0s, h->d_Cn, h->d_C, h->d_dig,
+                  h->d_part, h->d_sc);
+        h->read(h->d_sc, 3);
+        C_int = h->h_pin[0];
+        if (C_int <= 0.0) break;
+        ice_int = -(VH_RHO_ICE / VH_RHO_WATER) * h->h_pin[1];
+        zeta_int = h->subgrid ? h->h_pin[2] : 0.0;
+        sle_wcorr(st, n, h->subgrid, h->d_topo0, h->d_C, h->d_C0s, h->d_wcorr);
+        for (int ii = 1; ii <= h->n_inner; ii++) {
+            sle_load_analysis(st, s, h->d_dig, h->d_C, h->d_rsl, h->d_wcorr, h->d_F, h->d_sigma);
+            sle_apply_synth(st, s, h->d_sigma, h->d_gu, h->d_gn, h->d_dsurf, h->d.nk, h->g, h->kind, h->d_uN, h->d_F2,
+                            srot, h->d_sraw);
+            sle_cs(st, s, h->d_C, h->d_sraw, h->d_part, h->d_sc + 3);
+            sle_update(st, s, h->d_sraw, h->d_C, h->d_rsl, h->d_sc, h->subgrid, h->d_part);
+            h->read(h->d_sc + 4, 3);
+            res.esl = h->h_pin[0];
+            double dmax = h->h_pin[1], smax = h->h_pin[2];
+            res.n_inner_last = ii;
+            if (dmax <= h->tol * std::max(smax, DBL_MIN)) break;
+        }
+        res.n_outer_done = io;
+        h->n_outer_tot++;
+        h->n_inner_tot += res.n_inner_last;
+        if (h->fixed_ocean) break;
+        sle_flip(st, s, h->d_topo0, h->d_rsl, h->d_ice, h->d_C, h->d_Cn, h->d_part, h->d_sc + 7);
+        h->read(h->d_sc + 7, 1);
+        res.n_coast_flip = (int)h->h_pin[0];
+        if (res.n_coast_flip == 0) break;
+    }
+    // closing load -> sigma, then the memory advance
+    sle_load_analysis(st, s, h->d_dig, h->d_C, h->d_rsl, h->d_wcorr, h->d_F, h->d_sigma);
+    int nxt = h->cur ^ 1;
+    if (!report_only && h->kind == 2) {
+        vh_launch_advance(st, h->d, h->v, h->dt, h->d_tau[h->cur], h->d_tau[nxt], (const double*)h->d_sigma, h->d_dN,
+                          h->d_mnorm[nxt], h->d_maxn + nxt);
+    }
+    if (!report_only) h->time_resp += h->dt;
+    // mass diagnostic (+ the new memory norm, in the same read-back)
+    sle_cs(st, s, h->d_C, h->d_rsl, h->d_part, h->d_sc + 8);
+    if (h->kind == 2) HIPCHECK(hipMemcpyAsync(h->d_sc + 9, h->d_maxn + (report_only ? h->cur : nxt), sizeof(double),
+                                              hipMemcpyDeviceToDevice, st));
+    h->read(h->d_sc + 8, 2);
+    double Cs = h->h_pin[0] - zeta_int;
+    res.mass_resid = std::fabs(ice_int) > 0.0 ? std::fabs(Cs - ice_int) / std::fabs(ice_int) : std::fabs(Cs);
+    return res;
+}
+
+// stepper_advance, forward-Euler branch (src/vilma_timestep.f90:79-227)
+void stepper_advance(vh_model* h, double t0, double t1, const double* srot) {
+    const double span = t1 - t0;
+    const int n = (int)h->ng();
+    h->worst_mass_resid = 0.0;
+    if (span <= 0.0) {
+        sle_res r = sle_solve(h, h->d_hice, h->d_hice, 0.0, 0, true, srot);
+        h->worst_mass_resid = r.mass_resid;
+        return;
+    }
+    double rate = h->kind == 2 ? h->rate_max : 0.0;
+    double dt_stab = rate > 0.0 ? h->cfl / rate : span;
+    dt_stab = std::min(dt_stab, h->dt_max);
+    if (h->dt_min > 0.0) dt_stab = std::max(dt_stab, h->dt_min);
+    int n_sub = std::max(1, (int)std::ceil(span / dt_stab - 1.0e-9));
+    double dt = span / (double)n_sub, t = t0;
+    double tau_run = h->maxn_cur;
+    for (;;) {
+        if (t >= t1 - 1.0e-9 * span) break;
+        dt = std::min(dt, t1 - t);
+        HIPCHECK(hipMemcpyAsync(h->d_rsl_n, h->d_rsl, n * sizeof(double), hipMemcpyDeviceToDevice, h->st));
+        double tau0 = h->maxn_cur, time_s = h->time_resp;
+        h->dt = dt;
+        double frac = ((t + dt) - t0) / span;
+        h->n_solve++;
+        sle_res r = sle_solve(h, h->d_hice, h->d_ice1, frac, 1, false, srot);
+        h->worst_mass_resid = std::max(h->worst_mass_resid, r.mass_resid);
+        double err_inf = h->kind == 2 ? h->h_pin[1] : 0.0;
+        bool finite = err_inf <= DBL_MAX;
+        double scale = std::max(tau0, tau_run);
+        bool accept = scale <= 0.0 ? finite : (finite && err_inf <= h->guard * scale);
+        if (accept) {
+            if (h->kind == 2) { h->cur ^= 1; h->maxn_cur = err_inf; }
+            tau_run = std::max(tau_run, err_inf);
+            t = t + dt; h->n_accept++;
+            dt = std::min(span / (double)n_sub, 2.0 * dt);
+        } else {
+            HIPCHECK(hipMemcpyAsync(h->d_rsl, h->d_rsl_n, n * sizeof(double), hipMemcpyDeviceToDevice, h->st));
+            h->time_resp = time_s;
+            h->n_reject++;
+            dt = 0.5 * dt;
+            if (dt <= 1.0e-12 * span) throw vh_error("vilma_timestep: explicit guard sub-step collapsed (viscosity too stiff for FE)");
+        }
+    }
+}
+
+// solid_earth_update (src/vilma_coupling.f90:336-437), passthrough grid
+void update(vh_model* h, const double* h_ice, double dt_yr) {
+    hipStream_t st = h->st;
+    const int n = (int)h->ng();
+    double dt = dt_yr * VH_SEC_PER_YEAR;
+    double t0 = h->time_yr * VH_SEC_PER_YEAR, t1 = t0 + dt;
+    upload(h->d_ice1, h_ice, n, st);
+    if (h->rot_on) {
+        rot_begin(st, h->ra);
+        rot_srot(st, h->ra, h->d.nphi, h->d.nlat, h->d_sc2, h->d_cphi, h->d_sphi, h->d_srot);
+        stepper_advance(h, t0, t1, h->d_srot);
+        vh_legendre_synthesis(st, h->s, h->d_sigma, 1, h->d_F);
+        vh_fft_inverse(st, h->s, h->d_F, h->d.nlat, h->d_gtmp);
+        rot_inertia(st, h->ra, h->s, h->d_gtmp, h->d_sc2, h->d_cphi, h->d_sphi, h->d_part);
+        int n_sub = std::max(1, (int)std::ceil(dt / h->dt_fe_max));
+        double dt_sub = dt / (double)n_sub;
+        for (int k = 0; k < n_sub; k++) {
+            rot_begin(st, h->ra);
+            rot_solve_commit(st, h->ra, dt_sub);
+        }
+    } else {
+        stepper_advance(h, t0, t1, nullptr);
+    }
+    HIPCHECK(hipMemcpyAsync(h->d_hice, h->d_ice1, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    grid_sub(st, n, h->d_topo0, h->d_rsl, h->d_zbed);
+    h->time_yr += dt_yr;
+    // update_bsl (src/vilma_coupling.f90:572-597)
+    vh_surface_integral(st, h->s, h->d_C, h->d_part, h->d_sc + 12);
+    grid_bsl_integrand(st, n, h->d_hice, h->d_C, h->d_heq, h->d_C0ref, h->d_gtmp);
+    vh_surface_integral(st, h->s, h->d_gtmp, h->d_part + h->d.nlat, h->d_sc + 13);
+    h->read(h->d_sc + 12, 2);
+    double c_int = h->h_pin[0];
+    h->bsl = c_int > 0.0 ? -(VH_RHO_ICE / VH_RHO_WATER) * h->h_pin[1] / c_int : 0.0;
+}
+
+// degree-major (device) <-> SHTns LM order (API) for complex spectra
+inline int lm_shtns(int L, int l, int m) { return m * (2 * L + 1 - m) / 2 + l; }
+
+void to_lm(const vh_model* h, const double2* dm, double* out) {
+    int L = h->d.L;
+    for (int l = 0; l <= L; l++)
+        for (int m = 0; m <= l; m++) {
+            int a = l * (l + 1) / 2 + m, b = lm_shtns(L, l, m);
+            out[2 * b] = dm[a].x; out[2 * b + 1] = dm[a].y;
+        }
+}
+void from_lm(const vh_model* h, const double* in, double2* dm) {
+    int L = h->d.L;
+    for (int l = 0; l <= L; l++)
+        for (int m = 0; m <= l; m++) {
+            int a = l * (l + 1) / 2 + m, b = lm_shtns(L, l, m);
+            dm[a] = make_double2(in[2 * b], in[2 * b + 1]);
+        }
+}
+
+template <class F>
+int guarded(vh_model* h, F&& f) {
+    if (!h) return 2;
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        h->err = e.what();
+        return 1;
+    }
+}
+
+}  // namespace
+
+// ================================================================ C-ABI
+extern "C" {
+
+int vh_params_defaults(vh_params* p) {
+    memset(p, 0, sizeof(*p));
+    p->lmax = 64; p->nlat = 0; p->nphi = 0; p->mmax = -1; p->mres = 1; p->eps_polar = -1.0;
+    strcpy(p->earth, "M3-L70-V01");
+    p->n_layer = 5; p->r_earth = 6371.0e3; p->r_core = 3480.0e3;
+    const double rb[5] = {6301.0e3, 5951.0e3, 5701.0e3, 3480.0e3, 0.0};
+    const double rt[5] = {6371.0e3, 6301.0e3, 5951.0e3, 5701.0e3, 3480.0e3};
+    const double rho[5] = {3037.0, 3438.0, 3871.0, 4978.0, 10750.0};
+    const double mu[5] = {5.0605e10, 7.0363e10, 1.0549e11, 2.2834e11, 0.0};
+    const double eta[5] = {1.0e40, 1.0e21, 1.0e21, 2.0e21, 0.0};
+    const int rh[5] = {0, 1, 1, 1, 2};
+    for (int i = 0; i < VH_MAX_LAYER; i++) p->rheology[i] = 1;
+    for (int i = 0; i < 5; i++) {
+        p->r_bot[i] = rb[i]; p->r_top[i] = rt[i]; p->rho[i] = rho[i]; p->mu[i] = mu[i]; p->eta[i] = eta[i]; p->rheology[i] = rh[i];
+    }
+    strcpy(p->scheme, "fe"); p->max_couple_iter = 20; strcpy(p->earth_response, "ve");
+    p->sle_n_outer = 3; p->sle_n_inner = 20; p->sle_tol = 1.0e-7; p->sle_max_mem_iter = 20;
+    p->sle_fixed_ocean = 0; p->sle_subgrid = 1;
+    p->dt_init = 0.0; p->dt_min = 0.0; p->dt_max = 1.0e30 * VH_SEC_PER_YEAR;
+    p->rtol = 1.0e-4; p->atol = 1.0e-3; p->safety = 0.9; p->grow_max = 5.0; p->shrink_min = 0.2; p->cfl = 1.0;
+    p->rotation = 1; strcpy(p->deg1_frame, "cm");
+    p->l_visc_3d = 0; p->visc3d_tol = 1.0e-3; p->l_toroidal = 0;
+    p->visc_log10_min = 19.5; p->visc_log10_max = 30.0; p->visc3d_lid_depth = 0.0; p->visc3d_lid_log10max = 22.0;
+    p->skip_tol = 1.0e-4;
+    return 0;
+}
+
+int vh_create(const vh_params* p, int nphi, int nlat, const double* z_bed_eq, const double* h_ice_eq,
+              const double* h_ice_init, vh_model** out) {
+    *out = nullptr;
+    vh_model* h = nullptr;
+    try {
+        h = new vh_model();
+        setup(h, p, nphi, nlat, z_bed_eq, h_ice_eq, h_ice_init);
+        *out = h;
+        return 0;
+    } catch (const std::exception& e) {
+        g_create_error = std::string("solid_earth_init: ") + e.what();
+        delete h;
+        return 1;
+    }
+}
+
+int vh_update(vh_model* h, const double* h_ice, double dt_yr) {
+    return guarded(h, [&] { update(h, h_ice, dt_yr); });
+}
+
+int vh_get_field(vh_model* h, int field, double* out) {
+    return guarded(h, [&] {
+        const double* src = nullptr;
+        switch (field) {
+            case VH_F_RSL: src = h->d_rsl; break;
+            case VH_F_ZBED: src = h->d_zbed; break;
+            case VH_F_C: src = h->d_C; break;
+            case VH_F_C0: src = h->d_C0ref; break;
+            case VH_F_SROT: src = h->d_srot; break;
+            case VH_F_HICE: src = h->d_hice; break;
+            default: throw vh_error("vh_get_field: unknown field");
+        }
+        HIPCHECK(hipMemcpyAsync(out, src, h->ng() * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        h->sync();
+    });
+}
+
+int vh_get_scalar(vh_model* h, int which, double* out) {
+    return guarded(h, [&] {
+        switch (which) {
+            case VH_S_TIME: *out = h->time_yr; break;
+            case VH_S_BSL: *out = h->bsl; break;
+            case VH_S_WORST_MASS_RESID: *out = h->worst_mass_resid; break;
+            case VH_S_N_ACCEPT: *out = (double)h->n_accept; break;
+            case VH_S_N_REJECT: *out = (double)h->n_reject; break;
+            case VH_S_N_SOLVE: *out = (double)h->n_solve; break;
+            case VH_S_N_INNER_TOT: *out = (double)h->n_inner_tot; break;
+            case VH_S_N_OUTER_TOT: *out = (double)h->n_outer_tot; break;
+            case VH_S_POLAR_M_RE:
+            case VH_S_POLAR_M_IM:
+                if (!h->rot_on) { *out = 0.0; break; }
+                h->read(h->ra.rs.sc + RS_M, 2);
+                *out = h->h_pin[which == VH_S_POLAR_M_RE ? 0 : 1];
+                break;
+            default: throw vh_error("vh_get_scalar: unknown selector");
+        }
+    });
+}
+
+int vh_get_spectrum(vh_model* h, int which, double* out) {
+    return guarded(h, [&] {
+        std::vector<double2> tmp(h->d.nlm);
+        if (which == VH_SP_SIGMA) {
+            HIPCHECK(hipMemcpyAsync(tmp.data(), h->d_sigma, h->d.nlm * sizeof(double2), hipMemcpyDeviceToHost, h->st));
+            h->sync();
+            to_lm(h, tmp.data(), out);
+            return;
+        }
+        if (which != VH_SP_U && which != VH_SP_N) throw vh_error("vh_get_spectrum: unknown selector");
+        // response_apply at the current load and frozen drift
+        std::vector<double2> uN(2 * (size_t)h->d.nlm);
+        vh_apply_only(h->st, h->d.nlm, h->d_sigma, h->d_gu, h->d_gn, h->d_dsurf, h->d.nk, h->g, h->kind, h->d_uN);
+        HIPCHECK(hipMemcpyAsync(uN.data(), h->d_uN, uN.size() * sizeof(double2), hipMemcpyDeviceToHost, h->st));
+        h->sync();
+        to_lm(h, uN.data() + (which == VH_SP_U ? 0 : h->d.nlm), out);
+    });
+}
+
+int vh_get_dims(vh_model* h, int which, int* out) {
+    return guarded(h, [&] {
+        switch (which) {
+            case VH_D_LMAX: *out = h->d.L; break;
+            case VH_D_NLAT: *out = h->d.nlat; break;
+            case VH_D_NPHI: *out = h->d.nphi; break;
+            case VH_D_NLM: *out = h->d.nlm; break;
+            case VH_D_NR: *out = h->d.nr; break;
+            case VH_D_NE: *out = h->d.ne; break;
+            case VH_D_NK: *out = h->d.nk; break;
+            case VH_D_NE3D: *out = h->ne3d; break;
+            case VH_D_NLAM: *out = 4; break;
+            case VH_D_NEM: *out = h->d.nem; break;
+            default: throw vh_error("vh_get_dims: unknown selector");
+        }
+    });
+}
+
+// reference restart layout: 6 arrays (nlam, ne, nk) column-major
+int vh_get_memory(vh_model* h, double* tau) {
+    return guarded(h, [&] {
+        if (h->kind != 2) throw vh_error("vh_get_memory: no viscoelastic memory");
+        const vh_dims& d = h->d;
+        std::vector<double> dev((size_t)24 * d.nem * d.nk);
+        HIPCHECK(hipMemcpyAsync(dev.data(), h->d_tau[h->cur], dev.size() * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        h->sync();
+        size_t n1 = (size_t)4 * d.ne * d.nk;
+        memset(tau, 0, 6 * n1 * sizeof(double));
+        // array order Are, Aim, Bre, Bim, Cre, Cim = (shape, reim)
+        for (int em = 0; em < d.nem; em++) {
+            int e = h->emap[em];
+            for (int sh = 0; sh < 3; sh++)
+                for (int ri = 0; ri < 2; ri++)
+                    for (int lam = 0; lam < 4; lam++) {
+                        const double* src = dev.data() + ((size_t)em * 24 + (sh * 2 + ri) * 4 + lam) * d.nk;
+                        double* dst = tau + (size_t)(sh * 2 + ri) * n1;
+                        for (int k = 0; k < d.nk; k++) dst[((size_t)k * d.ne + e) * 4 + lam] = src[k];
+                    }
+        }
+    });
+}
+
+int vh_set_memory(vh_model* h, const double* tau) {
+    return guarded(h, [&] {
+        if (h->kind != 2) throw vh_error("vh_set_memory: no viscoelastic memory");
+        const vh_dims& d = h->d;
+        std::vector<double> dev((size_t)24 * d.nem * d.nk);
+        size_t n1 = (size_t)4 * d.ne * d.nk;
+        for (int em = 0; em < d.nem; em++) {
+            int e = h->emap[em];
+            for (int sh = 0; sh < 3; sh++)
+                for (int ri = 0; ri < 2; ri++)
+                    for (int lam = 0; lam < 4; lam++) {
+                        double* dst = dev.data() + ((size_t)em * 24 + (sh * 2 + ri) * 4 + lam) * d.nk;
+                        const double* src = tau + (size_t)(sh * 2 + ri) * n1;
+                        for (int k = 0; k < d.nk; k++) dst[k] = src[((size_t)k * d.ne + e) * 4 + lam];
+                    }
+        }
+        upload(h->d_tau[h->cur], dev.data(), dev.size(), h->st);
+        vh_launch_memnorm(h->st, d, h->d_tau[h->cur], h->d_mnorm[h->cur], h->d_maxn + h->cur);
+        h->read(h->d_maxn + h->cur, 1);
+        h->maxn_cur = h->h_pin[0];
+    });
+}
+
+int vh_destroy(vh_model* h) {
+    delete h;
+    return 0;
+}
+
+const char* vh_last_error(const vh_model* h) {
+    return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+int vh_enable_visc_3d(vh_model* h, const double* pert_elem) {
+    return guarded(h, [&] { (void)pert_elem; throw vh_error("lateral viscosity: not available in this build"); });
+}
+
+int vh_enable_visc_3d_nodes(vh_model* h, const double* visc_node, double lid_depth, double lid_log10max,
+                            double log10_cap) {
+    return guarded(h, [&] {
+        (void)visc_node; (void)lid_depth; (void)lid_log10max; (void)log10_cap;
+        throw vh_error("lateral viscosity: not available in this build");
+    });
+}
+
+// ---- probes
+int vh_resp_begin_step(vh_model* h) {
+    return guarded(h, [&] {
+        if (h->kind != 2) return;
+        vh_drift_run(h->st, h->dplan, h->d, h->v, h->d_tau[h->cur], h->d_mnorm[h->cur], h->d_maxn + h->cur,
+                     h->skip_tol, h->deg1_cm, h->d_dN, h->d_dsurf);
+        h->sync();
+    });
+}
+
+int vh_resp_apply(vh_model* h, const double* sigma_lm, double* u_lm, double* n_lm) {
+    return guarded(h, [&] {
+        std::vector<double2> sg(h->d.nlm), uN(2 * (size_t)h->d.nlm);
+        from_lm(h, sigma_lm, sg.data());
+        upload(h->d_lm, sg.data(), h->d.nlm, h->st);
+        vh_apply_only(h->st, h->d.nlm, h->d_lm, h->d_gu, h->d_gn, h->d_dsurf, h->d.nk, h->g, h->kind, h->d_uN);
+        HIPCHECK(hipMemcpyAsync(uN.data(), h->d_uN, uN.size() * sizeof(double2), hipMemcpyDeviceToHost, h->st));
+        h->sync();
+        to_lm(h, uN.data(), u_lm);
+        to_lm(h, uN.data() + h->d.nlm, n_lm);
+    });
+}
+
+int vh_resp_advance(vh_model* h, const double* sigma_lm) {
+    return guarded(h, [&] {
+        if (h->kind != 2) return;
+        std::vector<double2> sg(h->d.nlm);
+        from_lm(h, sigma_lm, sg.data());
+        upload(h->d_lm, sg.data(), h->d.nlm, h->st);
+        int nxt = h->cur ^ 1;
+        vh_launch_advance(h->st, h->d, h->v, h->dt, h->d_tau[h->cur], h->d_tau[nxt], (const double*)h->d_lm, h->d_dN,
+                          h->d_mnorm[nxt], h->d_maxn + nxt);
+        h->read(h->d_maxn + nxt, 1);
+        h->maxn_cur = h->h_pin[0];
+        h->cur = nxt;
+        h->time_resp += h->dt;
+    });
+}
+
+int vh_resp_set_dt(vh_model* h, double dt_s) {
+    return guarded(h, [&] { h->dt = dt_s; });
+}
+
+int vh_get_drift(vh_model* h, double* dsurf, double* dnodal) {
+    return guarded(h, [&] {
+        if (h->kind != 2) throw vh_error("vh_get_drift: no viscoelastic memory");
+        const vh_dims& d = h->d;
+        if (dsurf) {
+            HIPCHECK(hipMemcpyAsync(dsurf, h->d_dsurf, (size_t)6 * d.nk * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        }
+        if (dnodal) {
+            std::vector<double> dn((size_t)4 * d.nmn * d.nk);
+            HIPCHECK(hipMemcpyAsync(dn.data(), h->d_dN, dn.size() * sizeof(double), hipMemcpyDeviceToHost, h->st));
+            h->sync();
+            size_t nn = (size_t)d.nr * d.nk;
+            memset(dnodal, 0, 4 * nn * sizeof(double));
+            for (int q = 0; q < d.nmn; q++) {
+                int node = h->mnodes[q];
+                for (int c = 0; c < 4; c++)
+                    for (int k = 0; k < d.nk; k++) dnodal[c * nn + (size_t)k * d.nr + node] = dn[((size_t)q * 4 + c) * d.nk + k];
+            }
+        }
+        h->sync();
+    });
+}
+
+int vh_get_gains(vh_model* h, double* g3, double* xn) {
+    return guarded(h, [&] {
+        const vh_dims& d = h->d;
+        int L = d.L;
+        if (g3) {
+            HIPCHECK(hipMemcpyAsync(g3, h->d_gu, (L + 1) * sizeof(double), hipMemcpyDeviceToHost, h->st));
+            HIPCHECK(hipMemcpyAsync(g3 + L + 1, h->d_gn, (L + 1) * sizeof(double), hipMemcpyDeviceToHost, h->st));
+            HIPCHECK(hipMemcpyAsync(g3 + 2 * (L + 1), h->d_gv, (L + 1) * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        }
+        if (xn) {
+            HIPCHECK(hipMemcpyAsync(xn, h->d_xUn, (size_t)L * d.nr * sizeof(double), hipMemcpyDeviceToHost, h->st));
+            HIPCHECK(hipMemcpyAsync(xn + (size_t)L * d.nr, h->d_xVn, (size_t)L * d.nr * sizeof(double),
+                                    hipMemcpyDeviceToHost, h->st));
+        }
+        h->sync();
+    });
+}
+
+int vh_sht_analysis(vh_model* h, const double* grid, double* lm) {
+    return guarded(h, [&] {
+        upload(h->d_gtmp, grid, h->ng(), h->st);
+        vh_fft_forward(h->st, h->s, h->d_gtmp, h->d.nlat, h->d_F);
+        vh_legendre_analysis(h->st, h->s, h->d_F, 1, h->d_lm);
+        std::vector<double2> t(h->d.nlm);
+        HIPCHECK(hipMemcpyAsync(t.data(), h->d_lm, t.size() * sizeof(double2), hipMemcpyDeviceToHost, h->st));
+        h->sync();
+        to_lm(h, t.data(), lm);
+    });
+}
+
+int vh_sht_synthesis(vh_model* h, const double* lm, double* grid) {
+    return guarded(h, [&] {
+        std::vector<double2> t(h->d.nlm);
+        from_lm(h, lm, t.data());
+        upload(h->d_lm, t.data(), h->d.nlm, h->st);
+        vh_legendre_synthesis(h->st, h->s, h->d_lm, 1, h->d_F);
+        vh_fft_inverse(h->st, h->s, h->d_F, h->d.nlat, h->d_gtmp);
+        HIPCHECK(hipMemcpyAsync(grid, h->d_gtmp, h->ng() * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        h->sync();
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,132 @@
+// vh_internal.h -- device-side layout and kernel entry points of the
+// MI355X-native viscoelastic time step (shared by the .hip translation units).
+//
+// Layout in HBM (all FP64):
+//   * Maxwell memory tau[em][c][k]: only Maxwell elements em carry memory
+//     (elastic lithosphere and fluid core never do, src/vilma_response.f90:
+//     571-586); c = (shape*2 + reim)*4 + lambda with shape A,B,C and lambda
+//     the four spheroidal channels 1,2,5,6; the slot index k (degree-grouped,
+//     l ascending then m ascending, src/vilma_response.f90:629-648) is
+//     contiguous, so every per-slot kernel reads coalesced rows of nk values.
+//     Two copies (ping-pong): the advance writes tau_{n+1} into the other
+//     buffer, so the step controller's rollback needs no copy.
+//   * Spectra are complex in degree-major order idx(l,m) = l(l+1)/2 + m,
+//     whose l >= 1 part is exactly the slot order (k = idx - 1).
+//   * Grid fields are (nlat, nphi) with phi fastest, north-first rows.
+//   * Per-degree band LU factors (l >= 2): kl = ku = 6 (node-interleaved P1/P0
+//     dofs, src/vilma_radial_fe.f90:226-238), ldab = 19, column-major.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdexcept>
+#include <string>
+
+struct vh_error : std::runtime_error { using std::runtime_error::runtime_error; };
+
+#define VH_KL 6
+#define VH_KV 12
+#define VH_LDAB 19
+#define VH_PI 3.14159265358979323846264338327950288
+#define VH_G 6.67430e-11
+#define VH_RHO_ICE 931.0
+#define VH_RHO_WATER 1000.0
+#define VH_SEC_PER_YEAR 31556926.0
+
+#define HIPCHECK(x)                                                             \
+    do {                                                                        \
+        hipError_t e_ = (x);                                                    \
+        if (e_ != hipSuccess) throw vh_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct vh_dims {
+    int L, nlm, nr, ne, nd, nk, nem, nmn, nphi, nlat, nh;
+    int ns1, ldab1, kl1;   // degree-1 bordered system
+};
+
+// Everything a kernel needs about the radial discretisation and the memory.
+struct vh_dev {
+    // mesh / elements
+    const double* r;        // [nr]
+    const double* mu;       // [ne]
+    const double* MkPerDt;  // [ne]   Maxwell rate mu/eta of the 1-D path (M = rate*dt)
+    const int* emap;        // [nem]  Maxwell element -> mesh element
+    const int* mnodes;      // [nmn]  memory node -> mesh node
+    const int* kdeg;        // [nk]   degree of slot k
+    const int* mnode;       // [nr]   mesh node -> memory-node index or -1
+    const unsigned char* is3d;   // [nem] genuinely 3-D (advanced pseudo-spectrally)
+    // operators
+    const double* ab;       // [(L-1)][19*nd]  l >= 2
+    const int* ipiv;        // [(L-1)][nd]
+    const double* dr;       // [L][nd]  row equilibration, index l-1
+    const double* dc;       // [L][nd]
+    const double* ab1;      // degree 1, [ldab1*ns1]
+    const int* ipiv1;       // [ns1]
+    const double* null1;    // [nd]
+    const double* gu;       // [L+1]
+    const double* gn;
+    const double* gv;
+    const double* xUn;      // [L][nr], index l-1
+    const double* xVn;
+};
+
+// ---------------------------------------------------------------- setup
+void vh_launch_setup_band(hipStream_t st, const vh_dims& d, int lmin, int lmax, const double* r,
+                          const double* mu, const double* rho, const double* Rk, double r_earth,
+                          double g_surf, int rhs_kind, double* ab, int* ipiv, double* dr, double* dc,
+                          double* xfull /* [njob][nd] or null */, double* xUn, double* xVn,
+                          double* gu, double* gn, double* gv, int* fail);
+void vh_launch_setup_deg1(hipStream_t st, const vh_dims& d, const double* r, const double* mu,
+                          const double* rho, const double* Rk, double r_earth, double g_surf,
+                          int deg1_cm, double* ab1, int* ipiv1, double* dr1, double* dc1,
+                          double* null1, double* xUn, double* xVn, double* gu, double* gn,
+                          double* gv, double* work, int* fail);
+
+// ---------------------------------------------------------------- (b) advance
+void vh_launch_advance(hipStream_t st, const vh_dims& d, const vh_dev& v, double dt, const double* tau_in,
+                       double* tau_out, const double* sigma, const double* dN, double* mnorm_out,
+                       double* maxnorm_out);
+void vh_launch_memnorm(hipStream_t st, const vh_dims& d, const double* tau, double* mnorm, double* maxnorm);
+
+// ---------------------------------------------------------------- SHT
+struct vh_sht {
+    int L, nlat, nphi, nh, nlm;
+    const double* plm_lj;  // [m][l][j], per-m offset mstart[m]*nh (synthesis)
+    const double* dplm_lj;
+    const double* plm_jl;  // [m][j][l], same per-m offset (analysis)
+    const double* dplm_jl;
+    const int* mstart;     // [L+1] (device)
+    const double* w;       // [nlat]
+    const double* st;      // [nlat] sin(colat)
+    const double* ct;      // [nlat]
+    const double2* tw;     // [nphi] e^{-2 pi i t/nphi}
+    int nfac, fac[16];
+};
+void vh_setup_legendre_tables(hipStream_t st, const vh_sht& s, const double* ct, const double* sn,
+                              double* plm_lj, double* dplm_lj, double* plm_jl, double* dplm_jl);
+// rows of real grid -> row spectra F[row][m], m = 0..L, scaled by 2 pi/nphi
+void vh_fft_forward(hipStream_t st, const vh_sht& s, const double* f, int nrows, double2* F);
+void vh_fft_inverse(hipStream_t st, const vh_sht& s, const double2* F, int nrows, double* f);
+// batch of nb fields: F [nb][nlat][L+1] <-> q [nb][nlm] (degree-major)
+void vh_legendre_analysis(hipStream_t st, const vh_sht& s, const double2* F, int nb, double2* q);
+void vh_legendre_synthesis(hipStream_t st, const vh_sht& s, const double2* q, int nb, double2* F);
+// synthesis into rows f*fst + j*jst (fst = 1, jst = nb interleaves the fields per latitude)
+void vh_legendre_synthesis_rows(hipStream_t st, const vh_sht& s, const double2* q, int nb, double2* F,
+                                int fst, int jst);
+// spheroidal pair (vt, vp) row spectra <-> q, nb fields
+void vh_sph_synthesis_F(hipStream_t st, const vh_sht& s, const double2* q, int nb, double2* FT, double2* FP);
+void vh_sph_analysis_F(hipStream_t st, const vh_sht& s, const double2* FT, const double2* FP, int nb, double2* q);
+
+// ---------------------------------------------------------------- reductions
+// out[0] = sum_j w_j * (2 pi/nphi) * sum_i f[j][i] (deterministic two-pass)
+void vh_surface_integral(hipStream_t st, const vh_sht& s, const double* f, double* part, double* out);
+// out[c] = sum_j w_j * dlon * part[c*nlat + j], c < ncomp (ncomp <= 64)
+void vh_weighted_total(hipStream_t st, const vh_sht& s, const double* part, int ncomp, double* out);
+void vh_fill(hipStream_t st, double* p, double v, size_t n);
+
+// ---------------------------------------------------------------- drift plan
+struct vh_drift_plan;
+vh_drift_plan* vh_drift_plan_create(const vh_dims& d);
+void vh_drift_plan_destroy(vh_drift_plan* p);
+void vh_drift_run(hipStream_t st, const vh_drift_plan* p, const vh_dims& d, const vh_dev& v,
+                  const double* tau, const double* mnorm, const double* maxnorm, double skip_tol,
+                  int deg1_cm, double* dN, double* dsurf);

@@ -1,0 +1,41 @@
+// vh_rot.h -- device state and host wrappers of the rotation channels.
+#pragma once
+#include "vh_internal.h"
+
+// scalar slots of rot_state::sc (complex values take two slots: re, im)
+enum {
+    RS_M = 0, RS_CLOAD = 2,
+    RS_DF = 4,        // dF[load], dF[tidal]  (4 slots)
+    RS_DU = 8,        // dU[load], dU[tidal]  (4 slots)
+    RS_FE_LOAD = 12, RS_UE_TIDAL = 13, RS_KTE = 14, RS_KS = 15,
+    RS_A = 16, RS_G = 17, RS_CMA = 18, RS_OMEGA = 19,
+    RS_I13 = 20, RS_I23 = 21,
+    RS_N = 24
+};
+
+struct rot_state {
+    double* mem;       // [2][ne][24]
+    double* MkPerDt;   // [2][ne]
+    double* xU;        // [2][nr]
+    double* xV;        // [2][nr]
+    double* dN;        // [2][4][nr]
+    double* sc;        // [RS_N]
+};
+
+struct rot_args {
+    int nr, nd;
+    const double* r;
+    const double* mu;
+    const double* ab2;   // degree-2 band factor of the field driver
+    const int* ipiv2;
+    const double* dr2;
+    const double* dc2;
+    rot_state rs;
+};
+
+void rot_begin(hipStream_t st, const rot_args& a);
+void rot_srot(hipStream_t st, const rot_args& a, int nphi, int nlat, const double* sc2, const double* cphi,
+              const double* sphi, double* srot);
+void rot_inertia(hipStream_t st, const rot_args& a, const vh_sht& s, const double* load, const double* sc2,
+                 const double* cphi, const double* sphi, double* part);
+void rot_solve_commit(hipStream_t st, const rot_args& a, double dt);

@@ -1,0 +1,163 @@
+// vh_rot.hip -- rotational feedback on the GPU: the degree-2 Liouville polar
+// motion with two compact Maxwell channels (loading and tidal)
+// (src/vilma_rotation.f90:154-249, :335-363, :408-487).
+//
+// The channels use the same degree-2 operator as the field driver, so they
+// solve against its band factor. Device state (rot_state below) holds the
+// channel memory [ch][e][c] (c = (shape*2 + reim)*4 + lambda, every mesh
+// element, as the reference carries memory wherever eta > 0), the frozen
+// drift and the complex scalars, so a coupling interval needs no host round
+// trip for the rotation.
+#include "vh_internal.h"
+#include "vh_radial_dev.h"
+#include "vh_rot.h"
+
+// k_rot_begin: lanes 0..3 = (channel, re/im): dissipative RHS of the channel
+// memory, band solve, frozen nodal and surface drift.
+__global__ void __launch_bounds__(64) k_rot_begin(rot_args a) {
+    extern __shared__ double xs[];
+    const int lane = threadIdx.x, nd = a.nd, nr = a.nr;
+    if (lane >= 4) return;
+    const int ch = lane >> 1, reim = lane & 1;
+    double* x = xs + lane;   // stride 4
+    const int stride = 4;
+    for (int j = 0; j < nd; j++) x[j * stride] = 0.0;
+    const double Jr = 6.0;
+    for (int e = 0; e < nr - 1; e++) {
+        const double* mem = a.rs.mem + ((size_t)ch * (nr - 1) + e) * 24;
+        double A[4], B[4], Cc[4], floc[4];
+        for (int lam = 0; lam < 4; lam++) {
+            A[lam] = mem[(0 * 2 + reim) * 4 + lam];
+            B[lam] = mem[(1 * 2 + reim) * 4 + lam];
+            Cc[lam] = mem[(2 * 2 + reim) * 4 + lam];
+        }
+        dissipative_element(a.r[e], a.r[e + 1], Jr, A, B, Cc, floc);
+        x[(4 * e) * stride] += floc[0];
+        x[(4 * e + 4) * stride] += floc[1];
+        x[(4 * e + 1) * stride] += floc[2];
+        x[(4 * e + 5) * stride] += floc[3];
+    }
+    for (int j = 0; j < nd; j++) x[j * stride] = a.dr2[j] * x[j * stride];
+    lane_band_solve(nd, a.ab2, a.ipiv2, x, stride);
+    for (int n = 0; n < nr; n++) {
+        a.rs.dN[((size_t)ch * 4 + reim) * nr + n] = a.dc2[4 * n] * x[(4 * n) * stride];
+        a.rs.dN[((size_t)ch * 4 + 2 + reim) * nr + n] = a.dc2[4 * n + 1] * x[(4 * n + 1) * stride];
+    }
+    int s = nr - 1;
+    double Fa = a.dc2[4 * s + 2] * x[(4 * s + 2) * stride];
+    double Ua = a.dc2[4 * s] * x[(4 * s) * stride];
+    // scalars: dF[ch] at RS_DF + 2ch (+reim), dU[ch] at RS_DU + 2ch
+    a.rs.sc[RS_DF + 2 * ch + reim] = Fa;
+    a.rs.sc[RS_DU + 2 * ch + reim] = Ua;
+}
+
+// rotation_s_rot (src/vilma_rotation.f90:192-222)
+__global__ void k_rot_srot(rot_args a, int nphi, int nlat, const double* __restrict__ sc2,
+                           const double* __restrict__ cphi, const double* __restrict__ sphi, double* srot) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nphi * nlat) return;
+    const double* S = a.rs.sc;
+    double mr = S[RS_M], mi = S[RS_M + 1];
+    double kTe = S[RS_KTE];
+    double cNr = mr + (kTe * mr - S[RS_DF + 2]), cNi = mi + (kTe * mi - S[RS_DF + 3]);
+    double Ue = S[RS_UE_TIDAL];
+    double cUr = Ue * mr + S[RS_DU + 2], cUi = Ue * mi + S[RS_DU + 3];
+    double Om = S[RS_OMEGA], ar = S[RS_A], g = S[RS_G];
+    double kN = Om * Om * ar * ar / g, ku = Om * Om * ar * ar;
+    int j = i / nphi, ip = i % nphi;
+    double gam = sc2[j];
+    srot[i] = gam * (kN * (cNr * cphi[ip] + cNi * sphi[ip]) - ku * (cUr * cphi[ip] + cUi * sphi[ip]));
+}
+
+// per-row partial sums of load * sin cos * (cos phi, sin phi)
+__global__ void __launch_bounds__(256) k_rot_inertia_rows(int nphi, int nlat, const double* __restrict__ load,
+        const double* __restrict__ sc2, const double* __restrict__ cphi, const double* __restrict__ sphi,
+        double* part) {
+    __shared__ double r0[256], r1[256];
+    int j = blockIdx.x;
+    double a0 = 0.0, a1 = 0.0;
+    for (int i = threadIdx.x; i < nphi; i += blockDim.x) {
+        double v = load[(size_t)j * nphi + i];
+        a0 += v * sc2[j] * cphi[i];
+        a1 += v * sc2[j] * sphi[i];
+    }
+    r0[threadIdx.x] = a0; r1[threadIdx.x] = a1;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) { r0[threadIdx.x] += r0[threadIdx.x + s]; r1[threadIdx.x] += r1[threadIdx.x + s]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part[j] = r0[0]; part[nlat + j] = r1[0]; }
+}
+
+// solve_m + commit (src/vilma_rotation.f90:170-190, :224-233, :466-487)
+__global__ void __launch_bounds__(256) k_rot_solve_commit(rot_args a, double dt) {
+    __shared__ double coef[4];
+    double* S = a.rs.sc;
+    if (threadIdx.x == 0) {
+        double aa = S[RS_A];
+        double a4 = aa * aa * aa * aa;
+        double Ir = -a4 * S[RS_I13], Ii = -a4 * S[RS_I23];
+        double scl = 5.0 / (4.0 * VH_PI * VH_G * aa);
+        double clr = Ir * scl, cli = Ii * scl;
+        double Fe = S[RS_FE_LOAD];
+        double Itr = -(Fe * clr + S[RS_DF + 0]), Iti = -(Fe * cli + S[RS_DF + 1]);
+        double CA = S[RS_CMA];
+        double pr = Itr / CA, pi = Iti / CA;
+        double ks = S[RS_KS], kTe = S[RS_KTE];
+        double den = 1.0 - kTe / ks;
+        double mr = (pr - S[RS_DF + 2] / ks) / den, mi = (pi - S[RS_DF + 3] / ks) / den;
+        S[RS_CLOAD] = clr; S[RS_CLOAD + 1] = cli;
+        S[RS_M] = mr; S[RS_M + 1] = mi;
+        coef[0] = clr; coef[1] = cli; coef[2] = mr; coef[3] = mi;
+    }
+    __syncthreads();
+    const int ne = a.nr - 1, nr = a.nr;
+    for (int t = threadIdx.x; t < 2 * ne; t += blockDim.x) {
+        int ch = t / ne, e = t % ne;
+        double cr = coef[2 * ch], ci = coef[2 * ch + 1];
+        const double* xU = a.rs.xU + (size_t)ch * nr;
+        const double* xV = a.rs.xV + (size_t)ch * nr;
+        const double* dN = a.rs.dN + (size_t)ch * 4 * nr;
+        double U0r = cr * xU[e] + dN[0 * nr + e], U1r = cr * xU[e + 1] + dN[0 * nr + e + 1];
+        double U0i = ci * xU[e] + dN[1 * nr + e], U1i = ci * xU[e + 1] + dN[1 * nr + e + 1];
+        double V0r = cr * xV[e] + dN[2 * nr + e], V1r = cr * xV[e + 1] + dN[2 * nr + e + 1];
+        double V0i = ci * xV[e] + dN[3 * nr + e], V1i = ci * xV[e + 1] + dN[3 * nr + e + 1];
+        double s[2][3][4];
+        strain_coeffs(U0r, U1r, V0r, V1r, 6.0, s[0][0], s[0][1], s[0][2]);
+        strain_coeffs(U0i, U1i, V0i, V1i, 6.0, s[1][0], s[1][1], s[1][2]);
+        double Me = a.rs.MkPerDt[(size_t)ch * ne + e] * dt;
+        double om = 1.0 - Me, two_muM = 2.0 * a.mu[e] * Me;
+        double* mem = a.rs.mem + ((size_t)ch * ne + e) * 24;
+        for (int sh = 0; sh < 3; sh++)
+            for (int ri = 0; ri < 2; ri++)
+                for (int lam = 0; lam < 4; lam++) {
+                    int c = (sh * 2 + ri) * 4 + lam;
+                    mem[c] = om * mem[c] - two_muM * s[ri][sh][lam];
+                }
+    }
+}
+
+void rot_begin(hipStream_t st, const rot_args& a) {
+    size_t lds = (size_t)4 * a.nd * sizeof(double);
+    k_rot_begin<<<1, 64, lds, st>>>(a);
+    HIPCHECK(hipGetLastError());
+}
+
+void rot_srot(hipStream_t st, const rot_args& a, int nphi, int nlat, const double* sc2, const double* cphi,
+              const double* sphi, double* srot) {
+    int n = nphi * nlat;
+    k_rot_srot<<<(n + 255) / 256, 256, 0, st>>>(a, nphi, nlat, sc2, cphi, sphi, srot);
+    HIPCHECK(hipGetLastError());
+}
+
+void rot_inertia(hipStream_t st, const rot_args& a, const vh_sht& s, const double* load, const double* sc2,
+                 const double* cphi, const double* sphi, double* part) {
+    k_rot_inertia_rows<<<s.nlat, 256, 0, st>>>(s.nphi, s.nlat, load, sc2, cphi, sphi, part);
+    vh_weighted_total(st, s, part, 2, a.rs.sc + RS_I13);
+}
+
+void rot_solve_commit(hipStream_t st, const rot_args& a, double dt) {
+    k_rot_solve_commit<<<1, 256, 0, st>>>(a, dt);
+    HIPCHECK(hipGetLastError());
+}

@@ -1,0 +1,34 @@
+// vh_sle.h -- host wrappers of the sea-level-equation grid kernels (vh_sle.hip).
+#pragma once
+#include "vh_internal.h"
+
+void sle_ice(hipStream_t st, int n, const double* ice0, const double* ice1, double frac, int interp,
+             const double* ice_ref, const double* topo0, double* ice, double* dice, double* C0);
+// C (first pass: from topo0 - rsl; later: C_next), d_ice_g, and sc[0..2] =
+// (int C, int d_ice_g, int topo0 (C - C0))
+void sle_outer(hipStream_t st, const vh_sht& s, int first, const double* topo0, const double* rsl,
+               const double* ice, const double* dice, const double* C0, const double* Cnext, double* C,
+               double* dig, double* part, double* sc);
+void sle_wcorr(hipStream_t st, int n, int subgrid, const double* topo0, const double* C, const double* C0, double* w);
+// load -> load_lm (degree-major)
+void sle_load_analysis(hipStream_t st, const vh_sht& s, const double* dig, const double* C, const double* rsl,
+                       const double* wcorr, double2* F, double2* load_lm);
+// apply (kind 0 null / 1 elastic / 2 ve) + synthesis of (u, N) -> Sraw = N - u (+ s_rot)
+void sle_apply_synth(hipStream_t st, const vh_sht& s, const double2* sigma, const double* gu, const double* gn,
+                     const double* dsurf, int nk, double g, int kind, double2* uN, double2* F2,
+                     const double* srot, double* sraw);
+// response_apply only: uN = [u; N] (degree-major) from sigma
+void vh_apply_only(hipStream_t st, int nlm, const double2* sigma, const double* gu, const double* gn,
+                   const double* dsurf, int nk, double g, int kind, double2* uN);
+// out[0] = int C f
+void sle_cs(hipStream_t st, const vh_sht& s, const double* C, const double* f, double* part, double* out);
+// rsl update; sc[4] = dphi, sc[5] = dmax, sc[6] = smax
+void sle_update(hipStream_t st, const vh_sht& s, const double* sraw, const double* C, double* rsl, double* sc,
+                int subgrid, double* part);
+// C_next and out[0] = number of flipped cells
+void sle_flip(hipStream_t st, const vh_sht& s, const double* topo0, const double* rsl, const double* ice,
+              const double* C, double* Cnext, double* part, double* out);
+void grid_sub(hipStream_t st, int n, const double* a, const double* b, double* out);
+void grid_ocean(hipStream_t st, int n, const double* topo, const double* ice, double* C);
+void grid_bsl_integrand(hipStream_t st, int n, const double* h, const double* C, const double* heq,
+                        const double* C0, double* out);

@@ -1,0 +1,282 @@
+// vh_sle.hip -- (d) grid-space kernels of the sea-level equation
+// (src/vilma_sle.f90:124-405) and the ocean function (:407-425).
+//
+// The ocean/coastline mask must be bit-exact, so every expression that
+// feeds it (interpolated ice, ice - d_ice, topo0 - rsl) is evaluated with
+// floating-point contraction off, in the reference's operation order.
+// Reductions are deterministic two-pass (per-row partials, then one ordered
+// weighted sum) so a rerun gives identical bits.
+#pragma clang fp contract(off)
+#include "vh_internal.h"
+#include "vh_sht_dev.h"
+
+__device__ __forceinline__ double ocean(double topo, double ice) {
+    return (topo < 0.0 && VH_RHO_ICE * ice < -VH_RHO_WATER * topo) ? 1.0 : 0.0;
+}
+
+// ice(t) = ice0 + frac*(ice1 - ice0), d_ice = ice - ice_ref, C0 = O(topo0, ice - d_ice)
+// (src/vilma_timestep.f90:314-326, src/vilma_sle.f90:216)
+__global__ void k_sle_ice(int n, const double* __restrict__ ice0, const double* __restrict__ ice1, double frac,
+                          int interp, const double* __restrict__ ice_ref, const double* __restrict__ topo0,
+                          double* ice, double* dice, double* C0) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = interp ? ice0[i] + frac * (ice1[i] - ice0[i]) : ice0[i];
+    double d = v - ice_ref[i];
+    ice[i] = v;
+    dice[i] = d;
+    C0[i] = ocean(topo0[i], v - d);
+}
+
+// Coastline pass head (src/vilma_sle.f90:231-294): C, grounded-ice increment,
+// and per-row partial sums of C, d_ice_g, topo0*(C - C0).
+__global__ void __launch_bounds__(256) k_sle_outer(int nphi, int first, const double* __restrict__ topo0,
+        const double* __restrict__ rsl, const double* __restrict__ ice, const double* __restrict__ dice,
+        const double* __restrict__ C0, const double* __restrict__ Cnext, double* C, double* dig,
+        double* part /* [3][nlat] */, int nlat) {
+    __shared__ double red[3][256];
+    const int j = blockIdx.x;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int i = threadIdx.x; i < nphi; i += blockDim.x) {
+        size_t id = (size_t)j * nphi + i;
+        double c = first ? ocean(topo0[id] - rsl[id], ice[id]) : Cnext[id];
+        double g = ice[id] * (1.0 - c) - (ice[id] - dice[id]) * (1.0 - C0[id]);
+        C[id] = c;
+        dig[id] = g;
+        s0 += c; s1 += g; s2 += topo0[id] * (c - C0[id]);
+    }
+    red[0][threadIdx.x] = s0; red[1][threadIdx.x] = s1; red[2][threadIdx.x] = s2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int c = 0; c < 3; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) part[(size_t)threadIdx.x * nlat + j] = red[threadIdx.x][0];
+}
+
+// subgrid sloping-coast load (src/vilma_sle.f90:289-291)
+__global__ void k_sle_wcorr(int n, int subgrid, const double* __restrict__ topo0, const double* __restrict__ C,
+                            const double* __restrict__ C0, double* wcorr) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    wcorr[i] = subgrid ? -VH_RHO_WATER * topo0[i] * (C[i] - C0[i]) : 0.0;
+}
+
+// surface load (assemble_load, src/vilma_sle.f90:387-403), fed straight into
+// the forward longitude transform
+struct InLoad {
+    const double *dig, *C, *rsl, *wcorr;
+    int nphi;
+    __device__ double operator()(int row, int i) const {
+        size_t id = (size_t)row * nphi + i;
+        return VH_RHO_ICE * dig[id] + VH_RHO_WATER * (C[id] * rsl[id]) + wcorr[id];
+    }
+};
+
+// response_apply (src/vilma_response.f90:842-864) into two spectra [u; N]
+// (degree-major): u = gu*sigma + dUa, N = gn*sigma - dFa/g.
+__global__ void k_apply(int nlm, const double2* __restrict__ sigma, const double* __restrict__ gu,
+                        const double* __restrict__ gn, const double* __restrict__ dsurf, int nk, double g,
+                        int kind, double2* uN) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nlm) return;
+    int l = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+    while ((l + 1) * (l + 2) / 2 <= idx) l++;
+    while (l * (l + 1) / 2 > idx) l--;
+    double2 s = sigma[idx], u = make_double2(0.0, 0.0), N = make_double2(0.0, 0.0);
+    if (kind != 0) {   // 0 = null response
+        u = make_double2(gu[l] * s.x, gu[l] * s.y);
+        N = make_double2(gn[l] * s.x, gn[l] * s.y);
+        if (kind == 2 && l >= 1) {
+            int k = idx - 1;
+            const double2* ds = (const double2*)dsurf;
+            double2 dU = ds[k], dF = ds[nk + k];
+            u = make_double2(u.x + dU.x, u.y + dU.y);
+            N = make_double2(N.x - dF.x / g, N.y - dF.y / g);
+        }
+    }
+    uN[idx] = u;
+    uN[nlm + idx] = N;
+}
+
+// inverse transform epilogue: rows arrive as interleaved pairs (u_j, N_j);
+// Sraw = N - u (+ s_rot)
+struct OutSraw {
+    double* sraw;
+    const double* srot;
+    int nphi;
+    __device__ void operator()(int r0, int i, double v0, double v1, bool has1) const {
+        (void)has1;
+        int j = r0 >> 1;
+        size_t id = (size_t)j * nphi + i;
+        double s = v1 - v0;
+        if (srot) s = s + srot[id];
+        sraw[id] = s;
+    }
+};
+
+__global__ void __launch_bounds__(256) k_rowsum_prod(int nphi, const double* __restrict__ a,
+                                                     const double* __restrict__ b, double* part) {
+    __shared__ double red[256];
+    const int j = blockIdx.x;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nphi; i += blockDim.x) {
+        size_t id = (size_t)j * nphi + i;
+        acc += a[id] * b[id];
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[j] = red[0];
+}
+
+// dphi = (ice_int - Cs_int + zeta_int)/C_int ; rsl_new = Sraw + dphi ;
+// dmax = max|C (rsl_new - rsl)| ; rsl = rsl_new ; smax = max|C rsl|
+// sc: [0] C_int [1] int d_ice_g [2] zeta_int [3] Cs_int; writes sc[4] = dphi.
+__global__ void __launch_bounds__(256) k_sle_update(int nphi, const double* __restrict__ sraw,
+        const double* __restrict__ C, double* rsl, double* sc, int subgrid, double* part /*[2][nlat]*/, int nlat) {
+    __shared__ double r0[256], r1[256];
+    const int j = blockIdx.x;
+    const double rho_ratio = VH_RHO_ICE / VH_RHO_WATER;
+    double ice_int = -rho_ratio * sc[1];
+    double zeta = subgrid ? sc[2] : 0.0;
+    double dphi = (ice_int - sc[3] + zeta) / sc[0];
+    double dmax = 0.0, smax = 0.0;
+    for (int i = threadIdx.x; i < nphi; i += blockDim.x) {
+        size_t id = (size_t)j * nphi + i;
+        double rn = sraw[id] + dphi;
+        dmax = fmax(dmax, fabs(C[id] * (rn - rsl[id])));
+        rsl[id] = rn;
+        smax = fmax(smax, fabs(C[id] * rn));
+    }
+    r0[threadIdx.x] = dmax; r1[threadIdx.x] = smax;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            r0[threadIdx.x] = fmax(r0[threadIdx.x], r0[threadIdx.x + s]);
+            r1[threadIdx.x] = fmax(r1[threadIdx.x], r1[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[j] = r0[0];
+        part[nlat + j] = r1[0];
+        if (j == 0) sc[4] = dphi;
+    }
+}
+
+__global__ void k_max_final(const double* __restrict__ part, int nlat, int ncomp, double* out) {
+    int c = threadIdx.x;
+    if (c >= ncomp) return;
+    double m = 0.0;
+    for (int j = 0; j < nlat; j++) m = fmax(m, part[(size_t)c * nlat + j]);
+    out[c] = m;
+}
+
+// C_next = O(topo0 - rsl, ice); per-row count of flipped cells
+__global__ void __launch_bounds__(256) k_sle_flip(int nphi, const double* __restrict__ topo0,
+        const double* __restrict__ rsl, const double* __restrict__ ice, const double* __restrict__ C,
+        double* Cnext, double* part) {
+    __shared__ double red[256];
+    const int j = blockIdx.x;
+    double cnt = 0.0;
+    for (int i = threadIdx.x; i < nphi; i += blockDim.x) {
+        size_t id = (size_t)j * nphi + i;
+        double cn = ocean(topo0[id] - rsl[id], ice[id]);
+        Cnext[id] = cn;
+        cnt += (cn != C[id]) ? 1.0 : 0.0;
+    }
+    red[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[j] = red[0];
+}
+
+__global__ void k_sum_final(const double* __restrict__ part, int nlat, double* out) {
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int j = 0; j < nlat; j++) s += part[j];
+    out[0] = s;
+}
+
+// z_bed = z_bed_eq - rsl ; ocean function of an arbitrary (topo, ice) pair
+__global__ void k_sub(int n, const double* __restrict__ a, const double* __restrict__ b, double* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] - b[i];
+}
+__global__ void k_ocean(int n, const double* __restrict__ topo, const double* __restrict__ ice, double* C) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) C[i] = ocean(topo[i], ice[i]);
+}
+// bsl integrand h_ice (1 - C) - h_ice_eq (1 - C0) (src/vilma_coupling.f90:572-597)
+__global__ void k_bsl_integrand(int n, const double* __restrict__ h, const double* __restrict__ C,
+                                const double* __restrict__ heq, const double* __restrict__ C0, double* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = h[i] * (1.0 - C[i]) - heq[i] * (1.0 - C0[i]);
+}
+
+// ------------------------------------------------------------ host wrappers
+#include "vh_sle.h"
+
+static inline unsigned nb256(size_t n) { return (unsigned)((n + 255) / 256); }
+
+void sle_ice(hipStream_t st, int n, const double* ice0, const double* ice1, double frac, int interp,
+             const double* ice_ref, const double* topo0, double* ice, double* dice, double* C0) {
+    k_sle_ice<<<nb256(n), 256, 0, st>>>(n, ice0, ice1, frac, interp, ice_ref, topo0, ice, dice, C0);
+}
+void sle_outer(hipStream_t st, const vh_sht& s, int first, const double* topo0, const double* rsl,
+               const double* ice, const double* dice, const double* C0, const double* Cnext, double* C,
+               double* dig, double* part, double* sc) {
+    k_sle_outer<<<s.nlat, 256, 0, st>>>(s.nphi, first, topo0, rsl, ice, dice, C0, Cnext, C, dig, part, s.nlat);
+    vh_weighted_total(st, s, part, 3, sc);
+}
+void sle_wcorr(hipStream_t st, int n, int subgrid, const double* topo0, const double* C, const double* C0, double* w) {
+    k_sle_wcorr<<<nb256(n), 256, 0, st>>>(n, subgrid, topo0, C, C0, w);
+}
+void sle_load_analysis(hipStream_t st, const vh_sht& s, const double* dig, const double* C, const double* rsl,
+                       const double* wcorr, double2* F, double2* load_lm) {
+    vh_fft_forward_t(st, s, InLoad{dig, C, rsl, wcorr, s.nphi}, s.nlat, F);
+    vh_legendre_analysis(st, s, F, 1, load_lm);
+}
+void sle_apply_synth(hipStream_t st, const vh_sht& s, const double2* sigma, const double* gu, const double* gn,
+                     const double* dsurf, int nk, double g, int kind, double2* uN, double2* F2,
+                     const double* srot, double* sraw) {
+    k_apply<<<nb256(s.nlm), 256, 0, st>>>(s.nlm, sigma, gu, gn, dsurf, nk, g, kind, uN);
+    vh_legendre_synthesis_rows(st, s, uN, 2, F2, 1, 2);
+    vh_fft_inverse_pair_t(st, s, F2, 2 * s.nlat, OutSraw{sraw, srot, s.nphi});
+}
+void vh_apply_only(hipStream_t st, int nlm, const double2* sigma, const double* gu, const double* gn,
+                   const double* dsurf, int nk, double g, int kind, double2* uN) {
+    k_apply<<<nb256(nlm), 256, 0, st>>>(nlm, sigma, gu, gn, dsurf, nk, g, kind, uN);
+}
+void sle_cs(hipStream_t st, const vh_sht& s, const double* C, const double* f, double* part, double* out) {
+    k_rowsum_prod<<<s.nlat, 256, 0, st>>>(s.nphi, C, f, part);
+    vh_weighted_total(st, s, part, 1, out);
+}
+void sle_update(hipStream_t st, const vh_sht& s, const double* sraw, const double* C, double* rsl, double* sc,
+                int subgrid, double* part) {
+    k_sle_update<<<s.nlat, 256, 0, st>>>(s.nphi, sraw, C, rsl, sc, subgrid, part, s.nlat);
+    k_max_final<<<1, 64, 0, st>>>(part, s.nlat, 2, sc + 5);
+}
+void sle_flip(hipStream_t st, const vh_sht& s, const double* topo0, const double* rsl, const double* ice,
+              const double* C, double* Cnext, double* part, double* out) {
+    k_sle_flip<<<s.nlat, 256, 0, st>>>(s.nphi, topo0, rsl, ice, C, Cnext, part);
+    k_sum_final<<<1, 64, 0, st>>>(part, s.nlat, out);
+}
+void grid_sub(hipStream_t st, int n, const double* a, const double* b, double* out) {
+    k_sub<<<nb256(n), 256, 0, st>>>(n, a, b, out);
+}
+void grid_ocean(hipStream_t st, int n, const double* topo, const double* ice, double* C) {
+    k_ocean<<<nb256(n), 256, 0, st>>>(n, topo, ice, C);
+}
+void grid_bsl_integrand(hipStream_t st, int n, const double* h, const double* C, const double* heq,
+                        const double* C0, double* out) {
+    k_bsl_integrand<<<nb256(n), 256, 0, st>>>(n, h, C, heq, C0, out);
+}

@@ -1,0 +1,112 @@
+"""Loader of the HIP library and the Python `solid_earth` over it.
+
+The library is the product: if libvilma_hip.so is missing the import of this
+module's `load()` raises (there is no CPU fallback).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import VhParams, dptr
+from .model import ModelHandle, VilmaError
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libvilma_hip.so")
+HEADER = os.path.join(HERE, "..", "include", "vilma_hip.h")
+_lib = None
+
+
+def load():
+    """ctypes handle of libvilma_hip.so (built by fastearth3d_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise VilmaError("libvilma_hip.so is not built (run python -m fastearth3d_amd.build)")
+        L = C.CDLL(LIB)
+        _abi.declare_api(L, "vh_")
+        L.vh_create.restype = C.c_int
+        L.vh_create.argtypes = [C.POINTER(VhParams), C.c_int, C.c_int, C.POINTER(C.c_double),
+                                C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_void_p)]
+        L.vh_destroy.argtypes = [C.c_void_p]
+        L.vh_last_error.restype = C.c_char_p
+        L.vh_last_error.argtypes = [C.c_void_p]
+        L.vh_params_defaults.argtypes = [C.POINTER(VhParams)]
+        L.vh_resp_apply.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 3
+        L.vh_resp_advance.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.vh_resp_set_dt.argtypes = [C.c_void_p, C.c_double]
+        L.vh_sht_analysis.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.vh_sht_synthesis.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names of the entry points declared in include/vilma_hip.h."""
+    import re
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(vh_[a-z0-9_]+)\s*\(", txt)))
+
+
+def _cp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class SolidEarth(ModelHandle):
+    """solid_earth on the GPU: SolidEarth(par, z_bed_eq, h_ice_eq[, h_ice_init])
+    is solid_earth_init; .update(h_ice, dt_yr) is solid_earth_update;
+    .finalize() is solid_earth_finalize."""
+
+    def __init__(self, params, z_bed_eq, h_ice_eq, h_ice_init=None):
+        L = load()
+        nphi, nlat = params.grid()
+        z = np.ascontiguousarray(z_bed_eq, dtype=np.float64)
+        h = np.ascontiguousarray(h_ice_eq, dtype=np.float64)
+        hi = None if h_ice_init is None else np.ascontiguousarray(h_ice_init, dtype=np.float64)
+        for a in (z, h) + ((hi,) if hi is not None else ()):
+            if a.size != nphi * nlat:
+                raise VilmaError("solid_earth_init: grid fields must have nphi*nlat = %d values" % (nphi * nlat))
+        out = C.c_void_p()
+        rc = L.vh_create(C.byref(params), nphi, nlat, dptr(z), dptr(h), dptr(hi), C.byref(out))
+        if rc != 0:
+            raise VilmaError(L.vh_last_error(None).decode())
+        super().__init__(L, "vh_", out.value, params)
+
+    def sht_analysis(self, f):
+        q = np.empty(self.dims["nlm"], dtype=np.complex128)
+        self._check(self._lib.vh_sht_analysis(self._h, _cp(np.ascontiguousarray(f, dtype=np.float64)), _cp(q)),
+                    "sht_analysis")
+        return q
+
+    def sht_synthesis(self, q):
+        f = np.empty((self.nlat, self.nphi))
+        self._check(self._lib.vh_sht_synthesis(self._h, _cp(np.ascontiguousarray(q, dtype=np.complex128)), _cp(f)),
+                    "sht_synthesis")
+        return f
+
+    def apply(self, sigma):
+        u = np.empty(self.dims["nlm"], dtype=np.complex128)
+        n = np.empty_like(u)
+        self._check(self._lib.vh_resp_apply(self._h, _cp(np.ascontiguousarray(sigma, dtype=np.complex128)),
+                                            _cp(u), _cp(n)), "response_apply")
+        return u, n
+
+    def advance(self, sigma):
+        self._check(self._lib.vh_resp_advance(self._h, _cp(np.ascontiguousarray(sigma, dtype=np.complex128))),
+                    "response_advance_endpoint")
+
+    def set_dt(self, dt_s):
+        self._check(self._lib.vh_resp_set_dt(self._h, float(dt_s)), "response_set_dt")
+
+
+def solid_earth_init(params, z_bed_eq, h_ice_eq, h_ice_init=None):
+    return SolidEarth(params, z_bed_eq, h_ice_eq, h_ice_init)
+
+
+def solid_earth_update(se, h_ice, dt_yr):
+    se.update(h_ice, dt_yr)
+
+
+def solid_earth_finalize(se):
+    se.finalize()

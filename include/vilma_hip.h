@@ -1,0 +1,166 @@
+/*
+ * vilma_hip.h -- C-ABI of the MI355X-native VILMA v2 viscoelastic time step.
+ *
+ * This header is the drop-in boundary. A host (the Fortran shim module
+ * `vilma` in fastearth3d_amd/fortran/vilma.f90, the Python mirror in
+ * fastearth3d_amd/__init__.py, or any C/C++ caller) drives the model through
+ * plain pointers and sizes; no torch or HIP type appears in a signature.
+ *
+ * Conventions (all FP64):
+ *   - Grid fields are (nphi, nlat) column-major with phi fastest, latitude rows
+ *     north-first on the Gauss-Legendre nodes, longitudes 2*pi*(i-1)/nphi
+ *     (reference: src/vilma_sht.f90:11-15, :120-123; src/vilma_coupling.f90:678-681).
+ *   - Spectra are complex, interleaved (re, im), in the SHTns "LM" order of the
+ *     reference: m-major, l contiguous, 0-based index m*(2*lmax+1-m)/2 + l for
+ *     mres = 1 (src/vilma_sht.f90:349-354). Orthonormal harmonics without the
+ *     Condon-Shortley phase; a real field is Q_l0 Y_l0 + 2 Re sum_{m>0} Q_lm Y_lm.
+ *   - Times in the API are years where the reference API uses years
+ *     (solid_earth_update dt_yr), seconds inside vh_params (vilma_par_load
+ *     converts dt_* from years to seconds on load, src/vilma_params.f90:269-277).
+ *   - Every function returns 0 on success and a nonzero status on failure;
+ *     vh_last_error() returns the message. The reference aborts with
+ *     `error stop 'message'` instead (e.g. src/vilma_coupling.f90:248,279);
+ *     the Fortran shim turns a nonzero status back into `error stop message`.
+ *   - A handle is not thread-safe; distinct handles may be used concurrently.
+ */
+#ifndef VILMA_HIP_H
+#define VILMA_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VH_MAX_LAYER 16
+
+/* The physics/numerics record of the reference's `&vilma` group
+ * (vilma_param_class, src/vilma_params.f90:26-205), restricted to the fields
+ * the viscoelastic time step reads, plus two additive knobs marked below. */
+typedef struct vh_params {
+    int    lmax, nlat, nphi, mmax, mres;    /* nlat/nphi <= 0: 2*lmax+2 / 4*lmax   */
+    double eps_polar;                       /* accepted, ignored: exact Legendre    */
+    char   earth[64];                       /* "M3-L70-V01" | "custom" | "PREM"    */
+    int    n_layer;
+    double r_earth, r_core;
+    double r_bot[VH_MAX_LAYER], r_top[VH_MAX_LAYER];
+    double rho[VH_MAX_LAYER], mu[VH_MAX_LAYER], eta[VH_MAX_LAYER];
+    int    rheology[VH_MAX_LAYER];          /* 0 elastic, 1 Maxwell, 2 fluid       */
+    /* additive: explicit radial element count per layer (surface-first);
+     * 0 keeps the reference depth-target rule of src/vilma_radial_fe.f90:140-224 */
+    int    n_elem_layer[VH_MAX_LAYER];
+    char   scheme[8];                       /* "fe" (forward Euler memory)         */
+    int    max_couple_iter;
+    char   earth_response[8];               /* "ve" | "elastic" | "null"           */
+    int    sle_n_outer, sle_n_inner;
+    double sle_tol;
+    int    sle_max_mem_iter, sle_fixed_ocean, sle_subgrid;
+    double dt_init, dt_min, dt_max;         /* seconds                             */
+    double rtol, atol, safety, grow_max, shrink_min, cfl;
+    int    rotation;
+    char   deg1_frame[8];                   /* "cm" | "cf"                         */
+    int    l_visc_3d;
+    double visc3d_tol;
+    int    l_toroidal;
+    double visc_log10_min, visc_log10_max, visc3d_lid_depth, visc3d_lid_log10max;
+    /* additive: drift skip gate, the reference constant skip_tol of
+     * src/vilma_response.f90:93 (1e-4); 0 solves every slot */
+    double skip_tol;
+} vh_params;
+
+typedef struct vh_model vh_model;
+
+/* Field / scalar / spectrum selectors for the getters. */
+enum {
+    VH_F_RSL = 0,      /* se%rsl   (full relative sea-level change)          */
+    VH_F_ZBED = 1,     /* se%z_bed = z_bed_eq - rsl                          */
+    VH_F_C = 2,        /* se%gg%C  ocean function 0/1                        */
+    VH_F_C0 = 3,       /* se%gg%C0 reference ocean function                  */
+    VH_F_SROT = 4,     /* se%gg%s_rot rotational sea-level contribution      */
+    VH_F_HICE = 5      /* se%gg%h_ice                                        */
+};
+enum {
+    VH_S_TIME = 0, VH_S_BSL = 1, VH_S_WORST_MASS_RESID = 2,
+    VH_S_N_ACCEPT = 3, VH_S_N_REJECT = 4, VH_S_N_SOLVE = 5,
+    VH_S_N_INNER_TOT = 6, VH_S_N_OUTER_TOT = 7,
+    VH_S_POLAR_M_RE = 8, VH_S_POLAR_M_IM = 9
+};
+enum {
+    VH_SP_SIGMA = 0,   /* se%sigma_lm: converged surface load [kg m^-2]      */
+    VH_SP_U = 1,       /* u_lm = gu*sigma + dUa at the last report           */
+    VH_SP_N = 2        /* N_lm = gn*sigma - dFa/g                            */
+};
+enum {
+    VH_D_LMAX = 0, VH_D_NLAT = 1, VH_D_NPHI = 2, VH_D_NLM = 3, VH_D_NR = 4,
+    VH_D_NE = 5, VH_D_NK = 6, VH_D_NE3D = 7, VH_D_NLAM = 8, VH_D_NEM = 9
+};
+
+/* Defaults of vilma_param_class overlaid with input/vilma_defaults.nml. */
+int vh_params_defaults(vh_params* p);
+
+/* solid_earth_init (src/vilma_coupling.f90:144-232), passthrough grid
+ * (:172-176): fields already on the model's Gauss grid, (nphi, nlat).
+ * h_ice_init may be NULL (defaults to h_ice_eq). Calls
+ * solid_earth_enable_visc_3d only through vh_enable_visc_3d* below: the
+ * reference reads the lateral field from netCDF (:320-334), which the
+ * boundary replaces by an in-memory array. */
+int vh_create(const vh_params* p, int nphi, int nlat,
+              const double* z_bed_eq, const double* h_ice_eq,
+              const double* h_ice_init, vh_model** out);
+
+/* response_enable_lateral_visc (src/vilma_response.f90:994-1053):
+ * pert_elem is the log10-viscosity perturbation per element, (nphi, nlat, ne). */
+int vh_enable_visc_3d(vh_model* h, const double* pert_elem);
+
+/* response_enable_lateral_visc_from_nodes (src/vilma_response.f90:1120-1169):
+ * visc_node is absolute log10(eta), (nphi*nlat, nr); lid_depth <= 0 disables
+ * the lid rule. */
+int vh_enable_visc_3d_nodes(vh_model* h, const double* visc_node,
+                            double lid_depth, double lid_log10max, double log10_cap);
+
+/* solid_earth_update (src/vilma_coupling.f90:336-437). */
+int vh_update(vh_model* h, const double* h_ice, double dt_yr);
+
+/* Outputs the host reads from the solid_earth components (:71-140). */
+int vh_get_field(vh_model* h, int field, double* out);           /* (nphi,nlat) */
+int vh_get_scalar(vh_model* h, int which, double* out);
+int vh_get_spectrum(vh_model* h, int which, double* out);        /* 2*nlm       */
+int vh_get_dims(vh_model* h, int which, int* out);
+
+/* Maxwell memory in the reference restart layout: six arrays
+ * Are,Aim,Bre,Bim,Cre,Cim, each (nlam, ne, nk) column-major, concatenated
+ * (src/vilma_io.f90:495-568 reads/writes tau_* in this order). */
+int vh_get_memory(vh_model* h, double* tau);
+int vh_set_memory(vh_model* h, const double* tau);
+
+/* solid_earth_finalize (src/vilma_coupling.f90:599-626). */
+int vh_destroy(vh_model* h);
+
+/* Message of the last failure on this handle (or of the last vh_create). */
+const char* vh_last_error(const vh_model* h);
+
+/* ---- Parity probes: the response operator API the SLE and stepper use
+ * (src/vilma_response.f90:43-52, dispatch :241-319). ---------------------- */
+
+/* response_begin_step -> solve_drift (:694-809), skip gate included. */
+int vh_resp_begin_step(vh_model* h);
+/* response_apply (:842-864): u = gu*sigma + dUa, N = gn*sigma - dFa/g. */
+int vh_resp_apply(vh_model* h, const double* sigma_lm, double* u_lm, double* n_lm);
+/* response_advance_endpoint, forward-Euler branch (:1618-1633 -> :944-992). */
+int vh_resp_advance(vh_model* h, const double* sigma_lm);
+/* response_set_dt (:1665-1680), seconds. */
+int vh_resp_set_dt(vh_model* h, double dt_s);
+/* Frozen drift of the last begin_step: dUa,dFa,dVa as complex (2*nk each,
+ * slot order k2lm of :629-648) and nodal dUn_re,dUn_im,dVn_re,dVn_im as
+ * (nr, nk) column-major. Either pointer group may be NULL. */
+int vh_get_drift(vh_model* h, double* dsurf /* 3*2*nk */, double* dnodal /* 4*nr*nk */);
+/* Per-degree elastic gains gu, gn, gv (0..lmax) and unit-load nodal fields
+ * xUn, xVn (nr, lmax) of response_init_ve (:599-627). */
+int vh_get_gains(vh_model* h, double* g3 /* 3*(lmax+1) */, double* xn /* 2*nr*lmax */);
+/* Scalar transforms on the model grid (sht_grid_analysis/synthesis,
+ * src/vilma_sht.f90:204-226). */
+int vh_sht_analysis(vh_model* h, const double* grid, double* lm);
+int vh_sht_synthesis(vh_model* h, const double* lm, double* grid);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VILMA_HIP_H */

@@ -1,0 +1,229 @@
+/*
+ * orc.h -- internal declarations of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY. The oracle is a plain-C/OpenMP restatement of the
+ * reference's viscoelastic time step (VILMA v2), used as the correctness
+ * checker for the HIP product path and as the CPU baseline in bench.py. Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * Each routine cites the reference file:line it restates.
+ */
+#ifndef ORC_H
+#define ORC_H
+
+#include <complex.h>
+#include <stdint.h>
+#include "../include/vilma_hip.h"
+
+typedef double complex cplx;
+
+#define ORC_PI 3.14159265358979323846264338327950288
+#define ORC_G 6.67430e-11          /* src/vilma_constants.f90:18    */
+#define ORC_RHO_ICE 931.0          /* src/vilma_constants.f90:27    */
+#define ORC_RHO_WATER 1000.0       /* src/vilma_constants.f90:28    */
+#define ORC_SEC_PER_YEAR 31556926.0 /* src/vilma_constants.f90:34   */
+#define ORC_NLAM 4
+
+enum { RHEOL_ELASTIC = 0, RHEOL_MAXWELL = 1, RHEOL_FLUID = 2 };
+enum { RESP_NULL = 0, RESP_ELASTIC = 1, RESP_VE = 2 };
+
+/* ---------------------------------------------------------------- earth */
+typedef struct { double r_bot, r_top, rho, mu, eta; int rheology; } orc_layer;
+typedef struct { int n; double r_earth, r_core; orc_layer L[VH_MAX_LAYER]; } orc_earth;
+
+int    orc_build_earth(const vh_params* p, orc_earth* e, char* err);
+double orc_gravity_at(const orc_earth* e, double r);
+
+/* ----------------------------------------------------------------- mesh */
+typedef struct { int nr, ne; double* r; int* lay; } orc_mesh;
+int  orc_mesh_build(const orc_earth* e, const int* n_elem_layer, orc_mesh* m, char* err);
+void orc_mesh_free(orc_mesh* m);
+
+/* ------------------------------------------------------------ band LU */
+typedef struct { int n, kl, ku, kv, ldab; double* ab; int* ipiv; } orc_band;
+int  orc_band_build(orc_band* b, int n, int nz, const int* rows, const int* cols,
+                    const double* vals);          /* 0-based COO; 0 = ok */
+void orc_band_solve(const orc_band* b, const double* rhs, double* x);
+void orc_band_free(orc_band* b);
+
+typedef struct { int nd, nb, ns; orc_band band; double *dr, *dc, *w; } orc_bband;
+
+typedef struct {
+    int j, nr, ne, ndof;
+    double r_earth, g_surf;
+    orc_bband sys;
+    double* nullmode;     /* ndof, degree 1 only */
+} orc_radop;
+
+double* orc_build_dense_operator(const orc_earth* e, const orc_mesh* m, int j, int with_uniq);
+int  orc_radop_assemble(orc_radop* op, const orc_earth* e, const orc_mesh* m, int j, char* err);
+void orc_radop_solve(const orc_radop* op, const double* b, double* x, const double* border);
+void orc_radop_load_rhs(const orc_radop* op, double sigma, double* b);
+void orc_radop_tidal_rhs(const orc_radop* op, double phi_t, double* b);
+void orc_radop_free(orc_radop* op);
+
+static inline int idx_u(int k) { return 4 * k; }      /* node k, 0-based */
+static inline int idx_v(int k) { return 4 * k + 1; }
+static inline int idx_f(int k) { return 4 * k + 2; }
+static inline int idx_p(int e) { return 4 * e + 3; }  /* element e       */
+
+/* ------------------------------------------------------- Maxwell kernels */
+void orc_strain_coeffs(double u1, double u2, double v1, double v2, double Jr,
+                       double a[4], double b[4], double c[4]);
+void orc_strain_constants(double Jr, double norm[4], double sa[4][4], double sb[4][4], double sc[4][4]);
+void orc_dissipative_rhs(int ne, const double* r, double sa[4][4], double sb[4][4], double sc[4][4],
+                         const double norm[4], const double* A, const double* B, const double* C,
+                         double* f);
+void orc_advance_memory_fe(int ne, const double* mu, const double* Mk, const double* Un,
+                           const double* Vn, double Jr, double* A, double* B, double* C,
+                           const unsigned char* active);
+
+/* ------------------------------------------------------------------ SHT */
+typedef struct {
+    int lmax, mmax, nlat, nphi, nlm, nh;
+    double *colat, *ct, *st, *lon, *w;   /* w: Gauss weights, sum 2 */
+    double *plm, *dplm;                  /* [mstart(m) + (l-m)]*nh + j, j < nh */
+    int* mstart;                         /* offset of (m, l=m) in LM order */
+    /* FFT plan */
+    int nfac, fac[32];
+    cplx* tw;                            /* nphi twiddles e^{-2 pi i k/nphi} */
+} orc_sht;
+
+int  orc_sht_init(orc_sht* s, int lmax, int nlat, int nphi, char* err);
+void orc_sht_free(orc_sht* s);
+static inline int orc_lmidx(const orc_sht* s, int l, int m) { return s->mstart[m] + (l - m); }
+double orc_surface_integral(const orc_sht* s, const double* f);
+void orc_sht_analysis(const orc_sht* s, const double* f, cplx* q);
+void orc_sht_synthesis(const orc_sht* s, const cplx* q, double* f);
+void orc_sph_synthesis(const orc_sht* s, const cplx* q, double* vt, double* vp);
+void orc_sph_analysis(const orc_sht* s, const double* vt, const double* vp, cplx* q);
+void orc_fft_rows_forward(const orc_sht* s, const double* f, int nrows, cplx* F);  /* m=0..lmax, scaled 2pi/nphi */
+void orc_fft_rows_inverse(const orc_sht* s, const cplx* F, int nrows, double* f);
+double orc_eval_point(const orc_sht* s, const cplx* q, double colat, double lon);
+
+/* tensor spherical harmonics (src/vilma_tensor_sh.f90) */
+typedef struct {
+    int lmax, nlm, nphi, nlat;
+    int *ldeg, *mord;
+    double *llp1, *cott, *invsin, *n6;
+} orc_tsh;
+int  orc_tsh_init(orc_tsh* t, const orc_sht* s);
+void orc_tsh_free(orc_tsh* t);
+void orc_tsh_synth(const orc_tsh* t, const orc_sht* s, const cplx* c /* 4*nlm, [lam][lm] */, double* dyad /* 6 planes */);
+void orc_tsh_analysis(const orc_tsh* t, const orc_sht* s, const double* dyad, cplx* c);
+
+/* ------------------------------------------------------------- response */
+typedef struct {
+    int kind;
+    int lmax, nr, ne, ndof, nlm;
+    double g, a, dt, time;
+    double skip_tol;
+    int deg1_cm;
+    /* elastic response */
+    double *ugain, *ngain, *vgain;
+    /* VE */
+    orc_radop* ops;          /* [lmax], ops[l-1] */
+    double *r, *mu, *Mk, *MkPerDt;
+    unsigned char* active1d;
+    double* Jr;              /* [lmax+1] */
+    double (*nrmc)[4];
+    double (*sa)[4][4], (*sb)[4][4], (*sc)[4][4];
+    double *gu, *gn, *gv;    /* [lmax+1] */
+    double *xUn, *xVn;       /* [l-1][nr] */
+    int nk;
+    int *k2lm, *kdeg, *kbeg;
+    double *Are, *Aim, *Bre, *Bim, *Cre, *Cim;   /* [k][e][lam] */
+    cplx *dUa, *dFa, *dVa;
+    double *dUn_re, *dUn_im, *dVn_re, *dVn_im;   /* [k][node] */
+    double* mnorm;
+    /* lateral viscosity */
+    int lat_visc, ne3d, tsh_ready;
+    double visc3d_tol;
+    double *Mk3, *MkPerDt3;  /* [e][nlat][nphi] */
+    int* e3d;
+    orc_tsh tsh;
+    /* controller snapshot */
+    double *Are_s, *Aim_s, *Bre_s, *Bim_s, *Cre_s, *Cim_s;
+    double time_s;
+} orc_resp;
+
+int  orc_resp_init_ve(orc_resp* r, const orc_earth* e, const orc_mesh* m, const orc_sht* s, double dt, char* err);
+int  orc_resp_init_elastic(orc_resp* r, const orc_earth* e, const orc_mesh* m, int lmax, char* err);
+void orc_resp_free(orc_resp* r);
+void orc_resp_begin(orc_resp* r);
+void orc_resp_apply(const orc_resp* r, const orc_sht* s, const cplx* sigma, cplx* u, cplx* n);
+void orc_resp_advance(orc_resp* r, const orc_sht* s, const cplx* sigma);
+void orc_resp_set_dt(orc_resp* r, double dt);
+double orc_resp_max_rate(const orc_resp* r);
+double orc_resp_memory_norm(const orc_resp* r);
+void orc_resp_save(orc_resp* r);
+void orc_resp_restore(orc_resp* r);
+int  orc_resp_enable_lateral(orc_resp* r, const orc_sht* s, const double* pert, char* err);
+
+/* ------------------------------------------------------------------ SLE */
+typedef struct {
+    int n_outer, n_inner, max_mem_iter, fixed_ocean, subgrid, warm_start;
+    double tol;
+    long n_solve, n_outer_tot, n_inner_tot;
+} orc_sle;
+typedef struct {
+    int n_outer_done, n_couple_done, n_inner_last, n_coast_flip;
+    double resid, mass_resid, ocean_frac, esl;
+} orc_sle_result;
+
+void orc_ocean_function(int n, const double* topo, const double* ice, double* C);
+void orc_sle_solve(orc_sle* sle, const orc_sht* s, orc_resp* resp, const double* d_ice,
+                   const double* ice, const double* topo0, double* rsl, double* C,
+                   orc_sle_result* res, int report_only, cplx* sigma_out, const double* s_rot);
+
+/* ------------------------------------------------------------- rotation */
+typedef struct {
+    int nr, ne, ndof, tidal;
+    double Jr, dt;
+    orc_radop op;
+    double *r, *mu, *Mk, *MkPerDt;
+    double norm[4], sa[4][4], sb[4][4], sc[4][4];
+    double Fe, Ue;
+    double *xUn, *xVn;
+    double *Are, *Aim, *Bre, *Bim, *Cre, *Cim;   /* [e][lam] */
+    cplx dF, dU;
+    double *dUn_re, *dUn_im, *dVn_re, *dVn_im;
+} orc_ch2;
+
+typedef struct {
+    int enabled;
+    cplx m;
+    double time, a, g, CminusA, Omega, k_s, k_s_fluid, k_s_flat, kTe, hTe, dt_fe_max;
+    cplx cload;
+    orc_ch2 load_ch, tidal_ch;
+} orc_rot;
+
+int  orc_rot_init(orc_rot* rt, const orc_earth* e, const orc_mesh* m, double dt, char* err);
+void orc_rot_free(orc_rot* rt);
+void orc_rot_begin(orc_rot* rt, double dt);
+void orc_rot_solve_m(orc_rot* rt, const orc_sht* s, const double* load);
+void orc_rot_s_rot(const orc_rot* rt, const orc_sht* s, double* srot);
+void orc_rot_update(orc_rot* rt, const orc_sht* s, const double* load, double dt);
+
+/* ---------------------------------------------------------------- model */
+typedef struct orc_model {
+    vh_params par;
+    orc_sht sht;
+    orc_earth earth;
+    orc_mesh mesh;
+    orc_resp resp;
+    orc_sle sle;
+    orc_rot rot;
+    /* stepper (src/vilma_timestep.f90:38-75) */
+    double cfl, dt_min, dt_max, guard_growth;
+    long n_accept, n_reject, n_solve;
+    double worst_mass_resid;
+    /* gauss-grid state */
+    int np, nl;
+    double *z_bed_eq, *h_ice_eq, *h_ice, *rsl, *z_bed, *C, *C0, *s_rot;
+    cplx* sigma_lm;
+    cplx *u_lm, *n_lm;
+    double time, bsl;
+    char err[512];
+} orc_model;
+
+#endif

@@ -1,0 +1,399 @@
+/*
+ * orc_response.c -- oracle: the surface-load response operator (null,
+ * elastic, viscoelastic field driver with 1-D and pseudo-spectral 3-D
+ * Maxwell memory).
+ *
+ * TEST INFRASTRUCTURE ONLY (see orc.h). Restates src/vilma_response.f90:
+ *   :386-420 (deg1_to_cm), :422-489 (elastic), :542-677 (init_ve),
+ *   :694-826 (solve_drift + skip gate), :842-864 (apply), :944-992 (FE
+ *   advance), :994-1053 (enable lateral visc), :1171-1319 (3-D advance),
+ *   :1665-1811 (set_dt, save/restore, max_rate, memory_norm).
+ * Memory arrays keep the reference layout (nlam, ne, nk) column-major.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "orc.h"
+
+static void deg1_to_cm(const orc_radop* op, double* x) {
+    if (!op->nullmode) return;
+    double fa_n = op->nullmode[idx_f(op->nr - 1)];
+    if (fa_n == 0.0) return;
+    double c = -x[idx_f(op->nr - 1)] / fa_n;
+    for (int i = 0; i < op->ndof; i++) x[i] = x[i] + c * op->nullmode[i];
+}
+
+int orc_resp_init_elastic(orc_resp* r, const orc_earth* e, const orc_mesh* m, int lmax, char* err) {
+    int deg1_cm = r->deg1_cm;
+    memset(r, 0, sizeof(*r));
+    r->deg1_cm = deg1_cm;
+    r->kind = RESP_ELASTIC; r->lmax = lmax;
+    r->a = e->r_earth; r->g = orc_gravity_at(e, e->r_earth);
+    r->ugain = calloc(lmax + 1, 8); r->ngain = calloc(lmax + 1, 8); r->vgain = calloc(lmax + 1, 8);
+    r->ngain[0] = 4.0 * ORC_PI * ORC_G * r->a / r->g;
+    int nd = 4 * m->nr - 1, fail = 0;
+#pragma omp parallel for schedule(dynamic)
+    for (int l = 1; l <= lmax; l++) {
+        orc_radop op; char e2[256];
+        if (orc_radop_assemble(&op, e, m, l, e2)) { fail = 1; continue; }
+        double* b = malloc(nd * 8); double* x = malloc(nd * 8);
+        orc_radop_load_rhs(&op, 1.0, b);
+        orc_radop_solve(&op, b, x, NULL);
+        if (l == 1 && deg1_cm) deg1_to_cm(&op, x);
+        r->ugain[l] = x[idx_u(op.nr - 1)];
+        r->ngain[l] = -x[idx_f(op.nr - 1)] / r->g;
+        r->vgain[l] = x[idx_v(op.nr - 1)];
+        free(b); free(x); orc_radop_free(&op);
+    }
+    if (fail) { snprintf(err, 256, "bordered_band_factor: band LU factorization failed"); return 1; }
+    if (lmax >= 1 && !deg1_cm) r->ngain[1] = 0.0;
+    return 0;
+}
+
+int orc_resp_init_ve(orc_resp* r, const orc_earth* e, const orc_mesh* m, const orc_sht* s, double dt, char* err) {
+    int deg1_cm = r->deg1_cm;
+    double skip_tol = r->skip_tol, visc3d_tol = r->visc3d_tol;
+    memset(r, 0, sizeof(*r));
+    r->deg1_cm = deg1_cm; r->skip_tol = skip_tol; r->visc3d_tol = visc3d_tol;
+    r->kind = RESP_VE;
+    int L = s->lmax;
+    r->lmax = L; r->nlm = s->nlm; r->nr = m->nr; r->ne = m->ne; r->ndof = 4 * m->nr - 1;
+    r->dt = dt; r->time = 0.0;
+    r->a = e->r_earth; r->g = orc_gravity_at(e, e->r_earth);
+    int nr = r->nr, ne = r->ne, nd = r->ndof;
+    r->r = malloc(nr * 8); memcpy(r->r, m->r, nr * 8);
+    r->mu = malloc(ne * 8); r->Mk = malloc(ne * 8); r->MkPerDt = malloc(ne * 8);
+    r->active1d = malloc(ne);
+    for (int el = 0; el < ne; el++) {
+        const orc_layer* ly = &e->L[m->lay[el]];
+        r->mu[el] = ly->mu; r->active1d[el] = 1;
+        if (ly->rheology == RHEOL_MAXWELL) {
+            r->Mk[el] = r->mu[el] * dt / ly->eta;
+            r->MkPerDt[el] = r->mu[el] / ly->eta;
+        } else {
+            r->Mk[el] = 0.0; r->MkPerDt[el] = 0.0;
+        }
+    }
+    r->Jr = calloc(L + 1, 8);
+    r->nrmc = calloc(L + 1, sizeof(*r->nrmc));
+    r->sa = calloc(L + 1, sizeof(*r->sa)); r->sb = calloc(L + 1, sizeof(*r->sb)); r->sc = calloc(L + 1, sizeof(*r->sc));
+    r->gu = calloc(L + 1, 8); r->gn = calloc(L + 1, 8); r->gv = calloc(L + 1, 8);
+    r->xUn = calloc((size_t)L * nr, 8); r->xVn = calloc((size_t)L * nr, 8);
+    r->ops = calloc(L, sizeof(orc_radop));
+    r->gn[0] = 4.0 * ORC_PI * ORC_G * r->a / r->g;
+    int fail = 0;
+#pragma omp parallel for schedule(dynamic)
+    for (int l = 1; l <= L; l++) {
+        char e2[256];
+        r->Jr[l] = (double)l * (double)(l + 1);
+        orc_strain_constants(r->Jr[l], r->nrmc[l], r->sa[l], r->sb[l], r->sc[l]);
+        if (orc_radop_assemble(&r->ops[l - 1], e, m, l, e2)) { fail = 1; continue; }
+        double* b = malloc(nd * 8); double* x = malloc(nd * 8);
+        orc_radop_load_rhs(&r->ops[l - 1], 1.0, b);
+        orc_radop_solve(&r->ops[l - 1], b, x, NULL);
+        if (l == 1 && deg1_cm) deg1_to_cm(&r->ops[0], x);
+        r->gu[l] = x[idx_u(nr - 1)];
+        r->gn[l] = -x[idx_f(nr - 1)] / r->g;
+        r->gv[l] = x[idx_v(nr - 1)];
+        for (int k = 0; k < nr; k++) {
+            r->xUn[(size_t)(l - 1) * nr + k] = x[idx_u(k)];
+            r->xVn[(size_t)(l - 1) * nr + k] = x[idx_v(k)];
+        }
+        free(b); free(x);
+    }
+    if (fail) { snprintf(err, 256, "bordered_band_factor: band LU factorization failed"); return 1; }
+    if (L >= 1 && !deg1_cm) r->gn[1] = 0.0;
+    /* degree-grouped slot map (src/vilma_response.f90:629-648) */
+    r->nk = L * (L + 3) / 2;
+    r->k2lm = malloc(r->nk * sizeof(int)); r->kdeg = malloc(r->nk * sizeof(int));
+    r->kbeg = malloc((L + 2) * sizeof(int));
+    int k = 0;
+    for (int l = 1; l <= L; l++) {
+        r->kbeg[l] = k;
+        for (int mm = 0; mm <= l; mm++) { r->k2lm[k] = orc_lmidx(s, l, mm); r->kdeg[k] = l; k++; }
+    }
+    r->kbeg[L + 1] = k;
+    size_t nmem = (size_t)4 * ne * r->nk;
+    r->Are = calloc(nmem, 8); r->Aim = calloc(nmem, 8); r->Bre = calloc(nmem, 8);
+    r->Bim = calloc(nmem, 8); r->Cre = calloc(nmem, 8); r->Cim = calloc(nmem, 8);
+    r->dUa = calloc(r->nk, sizeof(cplx)); r->dFa = calloc(r->nk, sizeof(cplx)); r->dVa = calloc(r->nk, sizeof(cplx));
+    size_t nn = (size_t)nr * r->nk;
+    r->dUn_re = calloc(nn, 8); r->dUn_im = calloc(nn, 8); r->dVn_re = calloc(nn, 8); r->dVn_im = calloc(nn, 8);
+    r->mnorm = calloc(r->nk, 8);
+    return 0;
+}
+
+void orc_resp_free(orc_resp* r) {
+    free(r->ugain); free(r->ngain); free(r->vgain);
+    if (r->ops) for (int l = 1; l <= r->lmax; l++) orc_radop_free(&r->ops[l - 1]);
+    free(r->ops); free(r->r); free(r->mu); free(r->Mk); free(r->MkPerDt); free(r->active1d);
+    free(r->Jr); free(r->nrmc); free(r->sa); free(r->sb); free(r->sc);
+    free(r->gu); free(r->gn); free(r->gv); free(r->xUn); free(r->xVn);
+    free(r->k2lm); free(r->kdeg); free(r->kbeg);
+    free(r->Are); free(r->Aim); free(r->Bre); free(r->Bim); free(r->Cre); free(r->Cim);
+    free(r->dUa); free(r->dFa); free(r->dVa);
+    free(r->dUn_re); free(r->dUn_im); free(r->dVn_re); free(r->dVn_im); free(r->mnorm);
+    free(r->Mk3); free(r->MkPerDt3); free(r->e3d);
+    if (r->tsh_ready) orc_tsh_free(&r->tsh);
+    free(r->Are_s); free(r->Aim_s); free(r->Bre_s); free(r->Bim_s); free(r->Cre_s); free(r->Cim_s);
+    memset(r, 0, sizeof(*r));
+}
+
+static double absmax6(size_t n, const double* x1, const double* x2, const double* x3,
+                      const double* x4, const double* x5, const double* x6) {
+    double a = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        a = fmax(a, fabs(x1[i])); a = fmax(a, fabs(x2[i])); a = fmax(a, fabs(x3[i]));
+        a = fmax(a, fabs(x4[i])); a = fmax(a, fabs(x5[i])); a = fmax(a, fabs(x6[i]));
+    }
+    return a;
+}
+
+/* solve_drift (src/vilma_response.f90:694-809) */
+void orc_resp_begin(orc_resp* r) {
+    if (r->kind != RESP_VE) return;
+    int ne = r->ne, nr = r->nr, nd = r->ndof, L = r->lmax;
+    size_t nb = (size_t)4 * ne;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < r->nk; k++)
+        r->mnorm[k] = absmax6(nb, r->Are + k * nb, r->Bre + k * nb, r->Cre + k * nb,
+                              r->Aim + k * nb, r->Bim + k * nb, r->Cim + k * nb);
+    double mx = 0.0;
+    for (int k = 0; k < r->nk; k++) mx = fmax(mx, r->mnorm[k]);
+    double thr = r->skip_tol * mx;
+#pragma omp parallel
+    {
+        double *fre = malloc(nd * 8), *fim = malloc(nd * 8), *xre = malloc(nd * 8), *xim = malloc(nd * 8);
+#pragma omp for schedule(dynamic)
+        for (int l = 1; l <= L; l++) {
+            for (int k = r->kbeg[l]; k < r->kbeg[l + 1]; k++) {
+                double* Ur = r->dUn_re + (size_t)k * nr; double* Ui = r->dUn_im + (size_t)k * nr;
+                double* Vr = r->dVn_re + (size_t)k * nr; double* Vi = r->dVn_im + (size_t)k * nr;
+                if (r->mnorm[k] <= thr) {
+                    r->dUa[k] = 0.0; r->dFa[k] = 0.0; r->dVa[k] = 0.0;
+                    memset(Ur, 0, nr * 8); memset(Ui, 0, nr * 8); memset(Vr, 0, nr * 8); memset(Vi, 0, nr * 8);
+                    continue;
+                }
+                memset(fre, 0, nd * 8); memset(fim, 0, nd * 8);
+                orc_dissipative_rhs(ne, r->r, r->sa[l], r->sb[l], r->sc[l], r->nrmc[l],
+                                    r->Are + k * nb, r->Bre + k * nb, r->Cre + k * nb, fre);
+                orc_dissipative_rhs(ne, r->r, r->sa[l], r->sb[l], r->sc[l], r->nrmc[l],
+                                    r->Aim + k * nb, r->Bim + k * nb, r->Cim + k * nb, fim);
+                orc_radop_solve(&r->ops[l - 1], fre, xre, NULL);
+                orc_radop_solve(&r->ops[l - 1], fim, xim, NULL);
+                if (l == 1 && r->deg1_cm) { deg1_to_cm(&r->ops[0], xre); deg1_to_cm(&r->ops[0], xim); }
+                r->dUa[k] = xre[idx_u(nr - 1)] + I * xim[idx_u(nr - 1)];
+                r->dFa[k] = xre[idx_f(nr - 1)] + I * xim[idx_f(nr - 1)];
+                r->dVa[k] = xre[idx_v(nr - 1)] + I * xim[idx_v(nr - 1)];
+                if (l == 1 && !r->deg1_cm) r->dFa[k] = 0.0;
+                for (int n = 0; n < nr; n++) {
+                    Ur[n] = xre[idx_u(n)]; Ui[n] = xim[idx_u(n)];
+                    Vr[n] = xre[idx_v(n)]; Vi[n] = xim[idx_v(n)];
+                }
+            }
+        }
+        free(fre); free(fim); free(xre); free(xim);
+    }
+}
+
+/* response_apply for every kind (src/vilma_response.f90:241-252, :376-384, :491-511, :842-864) */
+void orc_resp_apply(const orc_resp* r, const orc_sht* s, const cplx* sigma, cplx* u, cplx* n) {
+    memset(u, 0, s->nlm * sizeof(cplx)); memset(n, 0, s->nlm * sizeof(cplx));
+    if (r->kind == RESP_NULL) return;
+    if (r->kind == RESP_ELASTIC) {
+        int lcap = r->lmax < s->lmax ? r->lmax : s->lmax;
+        for (int m = 0; m <= s->lmax; m++)
+            for (int l = m; l <= lcap; l++) {
+                int lm = orc_lmidx(s, l, m);
+                u[lm] = r->ugain[l] * sigma[lm]; n[lm] = r->ngain[l] * sigma[lm];
+            }
+        return;
+    }
+    int lm0 = orc_lmidx(s, 0, 0);
+    u[lm0] = r->gu[0] * sigma[lm0];
+    n[lm0] = r->gn[0] * sigma[lm0];
+    for (int k = 0; k < r->nk; k++) {
+        int l = r->kdeg[k], lm = r->k2lm[k];
+        u[lm] = r->gu[l] * sigma[lm] + r->dUa[k];
+        n[lm] = r->gn[l] * sigma[lm] - r->dFa[k] / r->g;
+    }
+}
+
+/* gather + advance_shape_tensor for one element (src/vilma_response.f90:1227-1319) */
+static void advance_element_3d(orc_resp* r, const orc_sht* s, const cplx* sigma, int e,
+                               cplx* cm, cplx* ce, cplx* cdel, double* dtau, double* deps) {
+    int nlm = s->nlm, nr = r->nr, ne = r->ne;
+    size_t nb = (size_t)4 * ne;
+    size_t ng = (size_t)s->nphi * s->nlat;
+    /* cm/ce: 3 shapes x [lam][lm] */
+    memset(cm, 0, 3 * 4 * nlm * sizeof(cplx)); memset(ce, 0, 3 * 4 * nlm * sizeof(cplx));
+    for (int k = 0; k < r->nk; k++) {
+        int l = r->kdeg[k], lm = r->k2lm[k];
+        double sre = creal(sigma[lm]), sim = cimag(sigma[lm]);
+        const double* xU = r->xUn + (size_t)(l - 1) * nr; const double* xV = r->xVn + (size_t)(l - 1) * nr;
+        size_t o = (size_t)k * nr;
+        double Ur = sre * xU[e] + r->dUn_re[o + e], Ui = sim * xU[e] + r->dUn_im[o + e];
+        double Ur1 = sre * xU[e + 1] + r->dUn_re[o + e + 1], Ui1 = sim * xU[e + 1] + r->dUn_im[o + e + 1];
+        double Vr = sre * xV[e] + r->dVn_re[o + e], Vi = sim * xV[e] + r->dVn_im[o + e];
+        double Vr1 = sre * xV[e + 1] + r->dVn_re[o + e + 1], Vi1 = sim * xV[e + 1] + r->dVn_im[o + e + 1];
+        double ar[4], br[4], cr[4], ai[4], bi[4], ci[4];
+        orc_strain_coeffs(Ur, Ur1, Vr, Vr1, r->Jr[l], ar, br, cr);
+        orc_strain_coeffs(Ui, Ui1, Vi, Vi1, r->Jr[l], ai, bi, ci);
+        size_t mo = k * nb + (size_t)e * 4;
+        for (int lam = 0; lam < 4; lam++) {
+            ce[(0 * 4 + lam) * nlm + lm] = ar[lam] + I * ai[lam];
+            ce[(1 * 4 + lam) * nlm + lm] = br[lam] + I * bi[lam];
+            ce[(2 * 4 + lam) * nlm + lm] = cr[lam] + I * ci[lam];
+            cm[(0 * 4 + lam) * nlm + lm] = r->Are[mo + lam] + I * r->Aim[mo + lam];
+            cm[(1 * 4 + lam) * nlm + lm] = r->Bre[mo + lam] + I * r->Bim[mo + lam];
+            cm[(2 * 4 + lam) * nlm + lm] = r->Cre[mo + lam] + I * r->Cim[mo + lam];
+        }
+    }
+    double twoMu = 2.0 * r->mu[e];
+    const double* M3 = r->Mk3 + (size_t)e * ng;
+    for (int sh = 0; sh < 3; sh++) {
+        cplx* c = cm + (size_t)sh * 4 * nlm;
+        orc_tsh_synth(&r->tsh, s, c, dtau);
+        orc_tsh_synth(&r->tsh, s, ce + (size_t)sh * 4 * nlm, deps);
+        for (int p = 0; p < 6; p++)
+            for (size_t id = 0; id < ng; id++)
+                dtau[p * ng + id] = -M3[id] * (dtau[p * ng + id] + twoMu * deps[p * ng + id]);
+        orc_tsh_analysis(&r->tsh, s, dtau, cdel);
+        for (int q = 0; q < 4 * nlm; q++) c[q] = c[q] + cdel[q];
+    }
+    for (int k = 0; k < r->nk; k++) {
+        int lm = r->k2lm[k];
+        size_t mo = k * nb + (size_t)e * 4;
+        for (int lam = 0; lam < 4; lam++) {
+            r->Are[mo + lam] = creal(cm[(0 * 4 + lam) * nlm + lm]); r->Aim[mo + lam] = cimag(cm[(0 * 4 + lam) * nlm + lm]);
+            r->Bre[mo + lam] = creal(cm[(1 * 4 + lam) * nlm + lm]); r->Bim[mo + lam] = cimag(cm[(1 * 4 + lam) * nlm + lm]);
+            r->Cre[mo + lam] = creal(cm[(2 * 4 + lam) * nlm + lm]); r->Cim[mo + lam] = cimag(cm[(2 * 4 + lam) * nlm + lm]);
+        }
+    }
+}
+
+/* vilma_advance (src/vilma_response.f90:944-992) with advance_memory_3d (:1171-1225) */
+void orc_resp_advance(orc_resp* r, const orc_sht* s, const cplx* sigma) {
+    if (r->kind != RESP_VE) return;
+    int nr = r->nr, ne = r->ne;
+    size_t nb = (size_t)4 * ne;
+    if (r->lat_visc && r->ne3d > 0) {
+        size_t ng = (size_t)s->nphi * s->nlat;
+        int nlm = s->nlm;
+#pragma omp parallel
+        {
+            cplx* cm = malloc((size_t)12 * nlm * sizeof(cplx));
+            cplx* ce = malloc((size_t)12 * nlm * sizeof(cplx));
+            cplx* cdel = malloc((size_t)4 * nlm * sizeof(cplx));
+            double* dtau = malloc(6 * ng * 8); double* deps = malloc(6 * ng * 8);
+#pragma omp for schedule(dynamic)
+            for (int ei = 0; ei < r->ne3d; ei++) advance_element_3d(r, s, sigma, r->e3d[ei], cm, ce, cdel, dtau, deps);
+            free(cm); free(ce); free(cdel); free(dtau); free(deps);
+        }
+    }
+#pragma omp parallel
+    {
+        double *Ure = malloc(nr * 8), *Uim = malloc(nr * 8), *Vre = malloc(nr * 8), *Vim = malloc(nr * 8);
+#pragma omp for schedule(static)
+        for (int k = 0; k < r->nk; k++) {
+            int l = r->kdeg[k], lm = r->k2lm[k];
+            double sre = creal(sigma[lm]), sim = cimag(sigma[lm]);
+            const double* xU = r->xUn + (size_t)(l - 1) * nr; const double* xV = r->xVn + (size_t)(l - 1) * nr;
+            size_t o = (size_t)k * nr;
+            for (int n = 0; n < nr; n++) {
+                Ure[n] = sre * xU[n] + r->dUn_re[o + n];
+                Uim[n] = sim * xU[n] + r->dUn_im[o + n];
+                Vre[n] = sre * xV[n] + r->dVn_re[o + n];
+                Vim[n] = sim * xV[n] + r->dVn_im[o + n];
+            }
+            orc_advance_memory_fe(ne, r->mu, r->Mk, Ure, Vre, r->Jr[l], r->Are + k * nb, r->Bre + k * nb, r->Cre + k * nb, r->active1d);
+            orc_advance_memory_fe(ne, r->mu, r->Mk, Uim, Vim, r->Jr[l], r->Aim + k * nb, r->Bim + k * nb, r->Cim + k * nb, r->active1d);
+        }
+        free(Ure); free(Uim); free(Vre); free(Vim);
+    }
+}
+
+void orc_resp_set_dt(orc_resp* r, double dt) {
+    r->dt = dt;
+    if (r->kind != RESP_VE) return;
+    for (int e = 0; e < r->ne; e++) r->Mk[e] = r->MkPerDt[e] * dt;
+    if (r->lat_visc) {
+        size_t n = (size_t)r->ne * r->tsh.nphi * r->tsh.nlat;
+        for (size_t i = 0; i < n; i++) r->Mk3[i] = r->MkPerDt3[i] * dt;
+    }
+}
+
+double orc_resp_max_rate(const orc_resp* r) {
+    double rate = 0.0;
+    if (r->kind != RESP_VE) return 0.0;
+    for (int e = 0; e < r->ne; e++) rate = fmax(rate, r->MkPerDt[e]);
+    if (r->lat_visc) {
+        size_t n = (size_t)r->ne * r->tsh.nphi * r->tsh.nlat;
+        for (size_t i = 0; i < n; i++) rate = fmax(rate, r->MkPerDt3[i]);
+    }
+    return rate;
+}
+
+double orc_resp_memory_norm(const orc_resp* r) {
+    if (r->kind != RESP_VE || !r->Are) return 0.0;
+    size_t nb = (size_t)4 * r->ne;
+    double nrm = 0.0;
+#pragma omp parallel for reduction(max : nrm) schedule(static)
+    for (int k = 0; k < r->nk; k++)
+        nrm = fmax(nrm, absmax6(nb, r->Are + k * nb, r->Aim + k * nb, r->Bre + k * nb,
+                                r->Bim + k * nb, r->Cre + k * nb, r->Cim + k * nb));
+    return nrm;
+}
+
+void orc_resp_save(orc_resp* r) {
+    if (r->kind != RESP_VE) { r->time_s = r->time; return; }
+    size_t n = (size_t)4 * r->ne * r->nk;
+    if (!r->Are_s) {
+        r->Are_s = malloc(n * 8); r->Aim_s = malloc(n * 8); r->Bre_s = malloc(n * 8);
+        r->Bim_s = malloc(n * 8); r->Cre_s = malloc(n * 8); r->Cim_s = malloc(n * 8);
+    }
+    memcpy(r->Are_s, r->Are, n * 8); memcpy(r->Aim_s, r->Aim, n * 8); memcpy(r->Bre_s, r->Bre, n * 8);
+    memcpy(r->Bim_s, r->Bim, n * 8); memcpy(r->Cre_s, r->Cre, n * 8); memcpy(r->Cim_s, r->Cim, n * 8);
+    r->time_s = r->time;
+}
+
+void orc_resp_restore(orc_resp* r) {
+    r->time = r->time_s;
+    if (r->kind != RESP_VE) return;
+    size_t n = (size_t)4 * r->ne * r->nk;
+    memcpy(r->Are, r->Are_s, n * 8); memcpy(r->Aim, r->Aim_s, n * 8); memcpy(r->Bre, r->Bre_s, n * 8);
+    memcpy(r->Bim, r->Bim_s, n * 8); memcpy(r->Cre, r->Cre_s, n * 8); memcpy(r->Cim, r->Cim_s, n * 8);
+}
+
+/* response_enable_lateral_visc (src/vilma_response.f90:994-1053); toroidal
+ * channels are not carried (l_toroidal = .false.). pert is [e][nlat][nphi]. */
+int orc_resp_enable_lateral(orc_resp* r, const orc_sht* s, const double* pert, char* err) {
+    if (r->kind != RESP_VE) { snprintf(err, 256, "solid_earth_enable_visc_3d: lateral viscosity needs earth_response=ve"); return 1; }
+    size_t ng = (size_t)s->nphi * s->nlat;
+    if (!r->tsh_ready) { orc_tsh_init(&r->tsh, s); r->tsh_ready = 1; }
+    free(r->Mk3); free(r->MkPerDt3); free(r->e3d);
+    r->MkPerDt3 = malloc(ng * r->ne * 8); r->Mk3 = malloc(ng * r->ne * 8);
+    for (int e = 0; e < r->ne; e++)
+        for (size_t i = 0; i < ng; i++) r->MkPerDt3[e * ng + i] = r->MkPerDt[e] * pow(10.0, -pert[e * ng + i]);
+    r->ne3d = 0;
+    for (int e = 0; e < r->ne; e++) {
+        r->active1d[e] = 1;
+        if (r->MkPerDt[e] == 0.0) continue;
+        double mx = -INFINITY, mn = INFINITY, sum = 0.0;
+        for (size_t i = 0; i < ng; i++) { double v = pert[e * ng + i]; mx = fmax(mx, v); mn = fmin(mn, v); }
+        if (mx - mn > r->visc3d_tol) { r->active1d[e] = 0; r->ne3d++; }
+        else {
+            /* Fortran sum(pert(:,:,e)) order: phi fastest, then latitude */
+            for (size_t i = 0; i < ng; i++) sum += pert[e * ng + i];
+            r->MkPerDt[e] = r->MkPerDt[e] * pow(10.0, -(sum / (double)ng));
+        }
+    }
+    for (int e = 0; e < r->ne; e++) r->Mk[e] = r->MkPerDt[e] * r->dt;
+    for (size_t i = 0; i < ng * r->ne; i++) r->Mk3[i] = r->MkPerDt3[i] * r->dt;
+    r->e3d = malloc((r->ne3d > 0 ? r->ne3d : 1) * sizeof(int));
+    int q = 0;
+    for (int e = 0; e < r->ne; e++) if (!r->active1d[e]) r->e3d[q++] = e;
+    r->lat_visc = 1;
+    return 0;
+}
