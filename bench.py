@@ -1,0 +1,160 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X viscoelastic time step: simulated model-kyr per
+wall-second of solid_earth_update on one GPU.
+
+A step is one coupling call (solid_earth_update over dt_yr) of the workload:
+by default the lmax 128 lateral-viscosity configuration (64 radial elements,
+192 x 384 Gauss grid, sea level with migrating coastlines, rotation, dt 50 yr)
+the metric is quoted on. The ice history is uploaded to HBM before the timed
+region and handed over as device pointers (vh_update_device). W untimed warm-up
+calls, then K timed calls bracketed by device synchronisations. The JSON line
+adds the roofline of the dominant kernel (algorithmic bytes per launch over its
+HIP-event-timed average duration) and the oracle's CPU throughput on a bounded
+sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+NAMES = {0: "k_drift", 1: "k_advance", 2: "tensor_3d"}
+
+
+def alg_bytes(dims, cls):
+    """Algorithmic HBM bytes of one launch of kernel class cls: the Maxwell
+    memory of the Maxwell elements (24 doubles per element and slot), the
+    nodal drift at the memory nodes (4 doubles per node and slot), the
+    surface drift (6 doubles per slot) and the per-slot norm."""
+    nk, nem, nmn, nlm = dims["nk"], dims["nem"], dims["nmn"], dims["nlm"]
+    tau = 24 * nem * nk * 8
+    if cls == 0:   # drift: read tau + norm, write nodal + surface drift
+        return tau + 8 * nk + 4 * nmn * nk * 8 + 6 * nk * 8
+    if cls == 1:   # advance: read + write tau, read nodal drift and sigma, write norm
+        return 2 * tau + 4 * nmn * nk * 8 + 16 * nlm + 8 * nk
+    return None
+
+
+def cpu_baseline(cfg, lmax, calls):
+    """Oracle (OpenMP C restatement) timed on the host over `calls` coupling
+    calls of the same workload, after one untimed call."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from fastearth3d_amd import configs
+    c = configs.config(cfg, lmax=lmax)
+    m = O.OracleModel(c["params"], c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
+    if c["pert"] is not None:
+        m.enable_visc_3d(c["pert"])
+    m.update(c["ice"](0), c["dt_yr"])
+    t0 = time.perf_counter()
+    for k in range(1, 1 + calls):
+        m.update(c["ice"](k), c["dt_yr"])
+    el = time.perf_counter() - t0
+    nsub = m.scalar(3)
+    m.finalize()
+    return calls * c["dt_yr"] / 1000.0 / el, el, nsub
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--lmax", type=int, default=128)
+    ap.add_argument("--cpu-calls", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from fastearth3d_amd import configs, lib
+    L = lib.load()
+    L.vh_update_device.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
+    L.vh_profile.argtypes = [C.c_void_p, C.c_int]
+    L.vh_profile_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+
+    c = configs.config(a.config, lmax=a.lmax)
+    p = c["params"]
+    se = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
+    if c["pert"] is not None:
+        se.enable_visc_3d(c["pert"])
+    nsteps = a.warmup + a.steps
+    ice = torch.from_numpy(np.stack([c["ice"](k) for k in range(nsteps)])).to("cuda:0")
+    torch.cuda.synchronize()
+
+    def step(k):
+        rc = L.vh_update_device(se._h, C.c_void_p(ice[k].data_ptr()), c["dt_yr"])
+        if rc:
+            raise RuntimeError(se._err())
+
+    for k in range(a.warmup):
+        step(k)
+    L.vh_profile(se._h, 1)
+    n_acc0 = se.scalar(3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(a.warmup, nsteps):
+        step(k)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms = (C.c_double * 8)()
+    cnt = (C.c_double * 8)()
+    L.vh_profile_read(se._h, ms, cnt)
+    nsub = se.scalar(3) - n_acc0
+    kyr = a.steps * c["dt_yr"] / 1000.0
+    value = kyr / el
+
+    d = dict(se.dims)
+    d["nmn"] = se._dim(9) + 1 if False else None
+    # memory nodes of a contiguous Maxwell stack = nem + 1
+    d["nmn"] = d["nem"] + 1
+    best = max((k for k in NAMES if cnt[k] > 0), key=lambda k: ms[k], default=None)
+    roof = None
+    if best is not None and alg_bytes(d, best):
+        avg_s = ms[best] / cnt[best] / 1000.0
+        ach = alg_bytes(d, best) / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": NAMES[best], "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_us": round(avg_s * 1e6, 2), "launches": int(cnt[best]),
+                "alg_bytes_per_launch": alg_bytes(d, best)}
+    kernel_ms = {NAMES[k]: round(ms[k], 3) for k in NAMES if cnt[k] > 0}
+
+    cpu = None
+    if not a.no_cpu:
+        v, t, ns = cpu_baseline(a.config, a.lmax, a.cpu_calls)
+        cpu = {"value": round(v, 6), "unit": "model-kyr/s",
+               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())), "kind": "port",
+               "sample": "%d coupling call(s) of %g yr (%d sub-steps) of the same workload, %.1f s" % (
+                   a.cpu_calls, c["dt_yr"], int(ns) - 0, t)}
+
+    names = {0: "c0 1-D load-only lmax %d" % a.lmax, 1: "c1 1-D + SLE + rotation lmax %d" % a.lmax,
+             2: "c2 3-D viscosity + SLE + rotation lmax %d" % a.lmax}
+    out = {
+        "metric": "model-kyr/s at lmax=128 3D visc (1 GPU)",
+        "value": round(value, 4), "unit": "model-kyr/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(el / a.steps * 1000.0, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": names[a.config], "lmax": a.lmax, "nlat": se.nlat, "nphi": se.nphi,
+                   "radial_elements": d["ne"], "maxwell_elements": d["nem"], "dt_yr": c["dt_yr"],
+                   "sub_steps_timed": int(nsub), "ne3d": int(se.dims.get("ne3d", 0))},
+        "roofline": roof, "kernel_ms": kernel_ms, "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(value / cpu["value"], 2)
+    print(json.dumps(out))
+    se.finalize()
+
+
+if __name__ == "__main__":
+    main()

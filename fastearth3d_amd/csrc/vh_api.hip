@@ -199,6 +199,38 @@ struct vh_model {
     int n_outer = 3, n_inner = 20, fixed_ocean = 0, subgrid = 1;
     double tol = 1e-7, cfl = 1, dt_min = 0, dt_max = DBL_MAX, guard = 1e3;
     long n_accept = 0, n_reject = 0, n_solve = 0, n_inner_tot = 0, n_outer_tot = 0;
+    // kernel timing (HIP events on the model's stream), per kernel class
+    bool prof = false;
+    struct evpair { hipEvent_t a, b; int cls; };
+    std::vector<evpair> evpool;
+    size_t evused = 0;
+    double prof_ms[8] = {0};
+    long prof_n[8] = {0};
+    void prof_begin(int cls) {
+        if (!prof) return;
+        if (evused == evpool.size()) {
+            evpair e{};
+            HIPCHECK(hipEventCreate(&e.a));
+            HIPCHECK(hipEventCreate(&e.b));
+            evpool.push_back(e);
+        }
+        evpool[evused].cls = cls;
+        HIPCHECK(hipEventRecord(evpool[evused].a, st));
+    }
+    void prof_end() {
+        if (!prof) return;
+        HIPCHECK(hipEventRecord(evpool[evused].b, st));
+        evused++;
+    }
+    void prof_collect() {   // after a stream synchronisation
+        for (size_t i = 0; i < evused; i++) {
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, evpool[i].a, evpool[i].b));
+            prof_ms[evpool[i].cls] += ms;
+            prof_n[evpool[i].cls]++;
+        }
+        evused = 0;
+    }
     bool lat_visc = false;
     int ne3d = 0;
 
@@ -229,6 +261,7 @@ vh_model::~vh_model() {
                     ra.rs.mem, ra.rs.MkPerDt, ra.rs.xU, ra.rs.xV, ra.rs.dN, ra.rs.sc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    for (auto& e : evpool) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
     if (h_pin) (void)hipHostFree(h_pin);
     if (st) (void)hipStreamDestroy(st);
 }
@@ -500,9 +533,12 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
     const int n = (int)h->ng();
     sle_res res;
     sle_ice(st, n, ice0, ice1, frac, interp, h->d_heq, h->d_topo0, h->d_ice, h->d_dice, h->d_C0s);
-    if (h->kind == 2)
+    if (h->kind == 2) {
+        h->prof_begin(VH_K_DRIFT);
         vh_drift_run(st, h->dplan, h->d, h->v, h->d_tau[h->cur], h->d_mnorm[h->cur], h->d_maxn + h->cur, h->skip_tol,
                      h->deg1_cm, h->d_dN, h->d_dsurf);
+        h->prof_end();
+    }
     vh_fill(st, h->d_wcorr, 0.0, n);
     double C_int = 0.0, ice_int = 0.0, zeta_int = 0.0;
     for (int io = 1; io <= h->n_outer; io++) {
@@ -545,8 +581,10 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
     sle_load_analysis(st, s, h->d_dig, h->d_C, h->d_rsl, h->d_wcorr, h->d_F, h->d_sigma);
     int nxt = h->cur ^ 1;
     if (!report_only && h->kind == 2) {
+        h->prof_begin(VH_K_ADVANCE);
         vh_launch_advance(st, h->d, h->v, h->dt, h->d_tau[h->cur], h->d_tau[nxt], (const double*)h->d_sigma, h->d_dN,
                           h->d_mnorm[nxt], h->d_maxn + nxt);
+        h->prof_end();
     }
     if (!report_only) h->time_resp += h->dt;
     // mass diagnostic (+ the new memory norm, in the same read-back)
@@ -606,12 +644,13 @@ void stepper_advance(vh_model* h, double t0, double t1, const double* srot) {
 }
 
 // solid_earth_update (src/vilma_coupling.f90:336-437), passthrough grid
-void update(vh_model* h, const double* h_ice, double dt_yr) {
+void update(vh_model* h, const double* h_ice, double dt_yr, bool on_device) {
     hipStream_t st = h->st;
     const int n = (int)h->ng();
     double dt = dt_yr * VH_SEC_PER_YEAR;
     double t0 = h->time_yr * VH_SEC_PER_YEAR, t1 = t0 + dt;
-    upload(h->d_ice1, h_ice, n, st);
+    if (on_device) HIPCHECK(hipMemcpyAsync(h->d_ice1, h_ice, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    else upload(h->d_ice1, h_ice, n, st);
     if (h->rot_on) {
         rot_begin(st, h->ra);
         rot_srot(st, h->ra, h->d.nphi, h->d.nlat, h->d_sc2, h->d_cphi, h->d_sphi, h->d_srot);
@@ -636,6 +675,7 @@ void update(vh_model* h, const double* h_ice, double dt_yr) {
     grid_bsl_integrand(st, n, h->d_hice, h->d_C, h->d_heq, h->d_C0ref, h->d_gtmp);
     vh_surface_integral(st, h->s, h->d_gtmp, h->d_part + h->d.nlat, h->d_sc + 13);
     h->read(h->d_sc + 12, 2);
+    h->prof_collect();
     double c_int = h->h_pin[0];
     h->bsl = c_int > 0.0 ? -(VH_RHO_ICE / VH_RHO_WATER) * h->h_pin[1] / c_int : 0.0;
 }
@@ -721,7 +761,11 @@ int vh_create(const vh_params* p, int nphi, int nlat, const double* z_bed_eq, co
 }
 
 int vh_update(vh_model* h, const double* h_ice, double dt_yr) {
-    return guarded(h, [&] { update(h, h_ice, dt_yr); });
+    return guarded(h, [&] { update(h, h_ice, dt_yr, false); });
+}
+
+int vh_update_device(vh_model* h, const double* h_ice_device, double dt_yr) {
+    return guarded(h, [&] { update(h, h_ice_device, dt_yr, true); });
 }
 
 int vh_get_field(vh_model* h, int field, double* out) {
@@ -904,6 +948,23 @@ int vh_resp_advance(vh_model* h, const double* sigma_lm) {
         h->maxn_cur = h->h_pin[0];
         h->cur = nxt;
         h->time_resp += h->dt;
+    });
+}
+
+int vh_profile(vh_model* h, int enable) {
+    return guarded(h, [&] {
+        h->sync();
+        h->prof_collect();
+        h->prof = enable != 0;
+        for (int c = 0; c < VH_K_NCLASS; c++) { h->prof_ms[c] = 0.0; h->prof_n[c] = 0; }
+    });
+}
+
+int vh_profile_read(vh_model* h, double* ms, double* n) {
+    return guarded(h, [&] {
+        h->sync();
+        h->prof_collect();
+        for (int c = 0; c < VH_K_NCLASS; c++) { ms[c] = h->prof_ms[c]; n[c] = (double)h->prof_n[c]; }
     });
 }
 

@@ -402,6 +402,50 @@ __global__ void __launch_bounds__(256) k_setup_deg1(int nr, const double* __rest
     }
 }
 
+// Degree-1 drift operator: the bordered system's response, through its LU
+// factor, to a unit forcing at each memory-node U/V dof, restricted to the
+// rows the drift keeps (memory-node U,V, then surface U, F, V) and with the
+// centre-of-mass projection folded in. G1[row][col], col = 2*q + (0:U, 1:V).
+__global__ void __launch_bounds__(256) k_setup_deg1_op(int nr, int nmn, const int* __restrict__ mnodes,
+        const double* __restrict__ ab1, int ldab1, const int* __restrict__ ipiv1, const double* __restrict__ dr,
+        const double* __restrict__ dc, const double* __restrict__ null1, int deg1_cm, double* work, double* G1) {
+    const int nd = 4 * nr - 1, ns = nd + 1, kl = nd, kv = 2 * nd;
+    const int ncol = 2 * nmn, nrow = 2 * nmn + 3;
+    double* x = work;
+    for (int col = 0; col < ncol; col++) {
+        int dof = 4 * mnodes[col >> 1] + (col & 1);
+        for (int i = threadIdx.x; i < ns; i += blockDim.x) x[i] = (i == dof) ? dr[i] : 0.0;
+        __syncthreads();
+        band_solve_block(ns, kl, kv, ab1, ldab1, ipiv1, x);
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) x[i] = dc[i] * x[i];
+        __syncthreads();
+        if (deg1_cm) {
+            int f = 4 * (nr - 1) + 2;
+            double fa_n = null1[f];
+            if (fa_n != 0.0) {
+                double c = -x[f] / fa_n;
+                __syncthreads();
+                for (int i = threadIdx.x; i < nd; i += blockDim.x) x[i] = x[i] + c * null1[i];
+            }
+            __syncthreads();
+        }
+        for (int row = threadIdx.x; row < nrow; row += blockDim.x) {
+            int src;
+            if (row < 2 * nmn) src = 4 * mnodes[row >> 1] + (row & 1);
+            else src = 4 * (nr - 1) + (row - 2 * nmn == 0 ? 0 : (row - 2 * nmn == 1 ? 2 : 1));
+            G1[(size_t)row * ncol + col] = x[src];
+        }
+        __syncthreads();
+    }
+}
+
+void vh_launch_setup_deg1_op(hipStream_t st, const vh_dims& d, const int* mnodes, const double* ab1,
+                             const int* ipiv1, const double* dr1, const double* dc1, const double* null1,
+                             int deg1_cm, double* work, double* G1) {
+    k_setup_deg1_op<<<1, 256, 0, st>>>(d.nr, d.nmn, mnodes, ab1, d.ldab1, ipiv1, dr1, dc1, null1, deg1_cm, work, G1);
+    HIPCHECK(hipGetLastError());
+}
+
 void vh_launch_setup_deg1(hipStream_t st, const vh_dims& d, const double* r, const double* mu,
                           const double* rho, const double* Rk, double r_earth, double g_surf,
                           int deg1_cm, double* ab1, int* ipiv1, double* dr1, double* dc1,

@@ -118,6 +118,10 @@ int vh_enable_visc_3d_nodes(vh_model* h, const double* visc_node,
 
 /* solid_earth_update (src/vilma_coupling.f90:336-437). */
 int vh_update(vh_model* h, const double* h_ice, double dt_yr);
+/* The same step with the ice thickness already in device memory of the
+ * model's GPU (a caller that keeps its forcing resident in HBM); no
+ * counterpart in the reference, whose fields always live on the host. */
+int vh_update_device(vh_model* h, const double* h_ice_device, double dt_yr);
 
 /* Outputs the host reads from the solid_earth components (:71-140). */
 int vh_get_field(vh_model* h, int field, double* out);           /* (nphi,nlat) */
@@ -136,6 +140,14 @@ int vh_destroy(vh_model* h);
 
 /* Message of the last failure on this handle (or of the last vh_create). */
 const char* vh_last_error(const vh_model* h);
+
+/* ---- Measurement hook (no counterpart in the reference): per-kernel-class
+ * device time from HIP events recorded on the model's stream around each
+ * launch. enable != 0 starts accumulating (and clears the totals). */
+enum { VH_K_DRIFT = 0, VH_K_ADVANCE = 1, VH_K_TENSOR = 2, VH_K_NCLASS = 8 };
+int vh_profile(vh_model* h, int enable);
+/* ms[c] = summed device milliseconds, n[c] = number of timed launches */
+int vh_profile_read(vh_model* h, double* ms /* VH_K_NCLASS */, double* n /* VH_K_NCLASS */);
 
 /* ---- Parity probes: the response operator API the SLE and stepper use
  * (src/vilma_response.f90:43-52, dispatch :241-319). ---------------------- */

@@ -458,3 +458,15 @@ int orc_mesh_c(const vh_params* p, double* r, int* lay, int cap, char* err) {
 }
 
 void orc_ocean_function_c(int n, const double* topo, const double* ice, double* C) { orc_ocean_function(n, topo, ice, C); }
+
+/* Degree-1 bordered band factor (test probe): copies ab (ldab x n) and ipiv
+ * when the buffers are large enough; returns n, ldab, kl, ku in info. */
+int orc_deg1_factor_c(orc_model* m, double* ab, int* ipiv, long cap, int* info) {
+    const orc_band* b = &m->resp.ops[0].sys.band;
+    info[0] = b->n; info[1] = b->ldab; info[2] = b->kl; info[3] = b->ku;
+    if (ab && (long)b->ldab * b->n <= cap) {
+        memcpy(ab, b->ab, (size_t)b->ldab * b->n * 8);
+        memcpy(ipiv, b->ipiv, b->n * sizeof(int));
+    }
+    return 0;
+}

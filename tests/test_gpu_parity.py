@@ -1,0 +1,213 @@
+"""HIP path vs the oracle, through the C-ABI (marked gpu).
+
+Tolerances follow the north-star contract: 1e-10 relative per step on the
+spectral displacement/potential coefficients and the rsl/z_bed fields, 1e-8
+after multi-step runs, and a bit-exact ocean/coastline mask. Sizes are small
+enough for the oracle to finish in seconds, plus full-size (lmax 128)
+cases checked for a few steps and through size-independent properties
+(mass conservation, restart round trip).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel
+from fastearth3d_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+SEC_PER_YEAR = 31556926.0
+
+
+def _pair(oracle, idx, lmax, ne=None, **over):
+    from fastearth3d_amd import lib
+    c = configs.config(idx, lmax=lmax, ne=ne)
+    p = c["params"]
+    for k, v in over.items():
+        setattr(p, k, v)
+    mo = oracle.OracleModel(p, c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
+    mg = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
+    return c, mo, mg
+
+
+def _maxwell_nodes(oracle, p):
+    r, lay = oracle.mesh(p)
+    el = [e for e in range(len(lay)) if p.rheology[lay[e]] == 1]
+    return el, sorted(set(el + [e + 1 for e in el]))
+
+
+def _random_memory(oracle, mo, p, seed, scale_by_slot=False):
+    rng = np.random.default_rng(seed)
+    tau = rng.standard_normal(mo.memory().shape) * 1.0e3
+    el, _ = _maxwell_nodes(oracle, p)
+    mask = np.zeros(tau.shape[2], bool)
+    mask[el] = True
+    tau[:, :, ~mask, :] = 0.0
+    tau[:, :2, :, 3] = 0.0          # lambda = 6 has no l = 1 harmonic
+    if scale_by_slot:               # slot magnitudes over 8 decades: exercises the skip gate
+        tau *= 10.0 ** rng.uniform(-8, 0, size=(1, tau.shape[1], 1, 1))
+    return tau
+
+
+def test_gains_and_unit_response(oracle):
+    # nodal unit-load fields compared where they are used: at the nodes of
+    # Maxwell elements (inside the fluid core, down to r = 0, the high-degree
+    # solution is tiny and rounding-sensitive and never enters the memory)
+    for idx, L in [(0, 16), (1, 24)]:
+        c, mo, mg = _pair(oracle, idx, L)
+        g3o, xo = mo.gains()
+        g3g, xg = mg.gains()
+        for i in range(3):
+            assert rel(g3g[i], g3o[i]) <= 1e-10
+        _, nodes = _maxwell_nodes(oracle, c["params"])
+        assert rel(xg[:, :, nodes], xo[:, :, nodes]) <= 1e-10
+
+
+def test_gains_reference_mesh(oracle):
+    """M3-L70-V01 on the reference depth-target mesh (217 elements)."""
+    from fastearth3d_amd import lib
+    p = oracle.params_defaults()
+    p.lmax = 12
+    p.rotation = 0
+    nphi, nlat = p.grid()
+    z = np.full((nlat, nphi), 100.0)
+    h = np.zeros((nlat, nphi))
+    mo = oracle.OracleModel(p, z, h)
+    mg = lib.SolidEarth(p, z, h)
+    g3o, xo = mo.gains()
+    g3g, xg = mg.gains()
+    assert mg.dims["ne"] == 217
+    for i in range(3):
+        assert rel(g3g[i], g3o[i]) <= 1e-10
+    _, nodes = _maxwell_nodes(oracle, p)
+    assert rel(xg[:, :, nodes], xo[:, :, nodes]) <= 1e-10
+
+
+@pytest.mark.parametrize("L,nlat,nphi", [(32, 0, 0), (128, 192, 384)])
+def test_sht(oracle, L, nlat, nphi):
+    c, mo, mg = _pair(oracle, 1, L, nlat=nlat, nphi=nphi)
+    rng = np.random.default_rng(11)
+    f = rng.standard_normal((mg.nlat, mg.nphi))
+    qo = mo.sht_analysis(f)
+    assert rel(mg.sht_analysis(f), qo) <= 1e-12
+    assert rel(mg.sht_synthesis(qo), mo.sht_synthesis(qo)) <= 1e-12
+
+
+@pytest.mark.parametrize("skip_tol", [1.0e-4, 0.0])
+def test_drift(oracle, skip_tol):
+    c, mo, mg = _pair(oracle, 1, 24, skip_tol=skip_tol)
+    p = c["params"]
+    tau = _random_memory(oracle, mo, p, 7, scale_by_slot=True)
+    mo.set_memory(tau)
+    mg.set_memory(tau)
+    mo.begin_step()
+    mg.begin_step()
+    so, no = mo.drift()
+    sg, ng = mg.drift()
+    for i in range(3):
+        assert rel(sg[i], so[i]) <= 1e-10
+    # the skip gate made the same decisions (exact zeros in the same slots)
+    assert np.array_equal(sg[0] == 0, so[0] == 0)
+    if skip_tol > 0:
+        assert (so[0] == 0).any() and (so[0] != 0).any()
+    _, nodes = _maxwell_nodes(oracle, p)
+    assert rel(ng[:, :, nodes], no[:, :, nodes]) <= 1e-10
+
+
+def test_apply_and_advance(oracle):
+    c, mo, mg = _pair(oracle, 1, 24)
+    p = c["params"]
+    tau = _random_memory(oracle, mo, p, 9)
+    mo.set_memory(tau)
+    mg.set_memory(tau)
+    mo.begin_step()
+    mg.begin_step()
+    rng = np.random.default_rng(2)
+    nlm = mo.dims["nlm"]
+    sig = rng.standard_normal(nlm) + 1j * rng.standard_normal(nlm)
+    sig[: p.lmax + 1] = sig[: p.lmax + 1].real
+    uo, no_ = mo.apply(sig)
+    ug, ng_ = mg.apply(sig)
+    assert rel(ug, uo) <= 1e-10 and rel(ng_, no_) <= 1e-10
+    for m in (mo, mg):
+        m.set_dt(37.0 * SEC_PER_YEAR)
+        m.advance(sig)
+    assert rel(mg.memory(), mo.memory()) <= 1e-10
+
+
+def test_config0_run(oracle):
+    """Config-0 shaped run (load only, no ocean, cf frame) at lmax 32."""
+    c, mo, mg = _pair(oracle, 0, 32)
+    for k in range(20):
+        h = c["ice"](k)
+        mo.update(h, c["dt_yr"])
+        mg.update(h, c["dt_yr"])
+        if k == 0:
+            assert rel(mg.spectrum(1), mo.spectrum(1)) <= 1e-10
+    assert rel(mg.spectrum(1), mo.spectrum(1)) <= 1e-8
+    assert rel(mg.spectrum(2), mo.spectrum(2)) <= 1e-8
+    assert rel(mg.memory(), mo.memory()) <= 1e-8
+    assert mg.scalar(3) == mo.scalar(3)
+
+
+def test_config1_run(oracle):
+    """SLE with migrating coastlines and rotation at lmax 32."""
+    c, mo, mg = _pair(oracle, 1, 32)
+    for k in range(8):
+        h = c["ice"](k + 60)
+        mo.update(h, c["dt_yr"])
+        mg.update(h, c["dt_yr"])
+        assert np.array_equal(mg.C, mo.C)                 # bit-exact ocean function
+        assert rel(mg.rsl, mo.rsl) <= 1e-8
+        assert rel(mg.z_bed - c["z_bed_eq"], mo.z_bed - c["z_bed_eq"]) <= 1e-8
+    assert rel(mg.sigma_lm, mo.sigma_lm) <= 1e-8
+    assert rel(mg.spectrum(2), mo.spectrum(2)) <= 1e-8
+    assert mg.scalar(6) == mo.scalar(6)                   # same SLE inner iterations
+    assert abs(mg.bsl - mo.bsl) <= 1e-10 * max(1.0, abs(mo.bsl))
+    mo_m = mo.scalar(8) + 1j * mo.scalar(9)
+    mg_m = mg.scalar(8) + 1j * mg.scalar(9)
+    assert abs(mg_m - mo_m) <= 1e-8 * abs(mo_m)
+    assert mg.worst_mass_resid <= 1e-10
+
+
+def test_config1_full_size(oracle):
+    """lmax 128 on the 192 x 384 Gauss grid, 64 elements: two coupling calls."""
+    c, mo, mg = _pair(oracle, 1, 128)
+    for k in range(2):
+        h = c["ice"](k + 80)
+        mo.update(h, c["dt_yr"])
+        mg.update(h, c["dt_yr"])
+    assert np.array_equal(mg.C, mo.C)
+    assert rel(mg.rsl, mo.rsl) <= 1e-9
+    assert rel(mg.sigma_lm, mo.sigma_lm) <= 1e-9
+    assert rel(mg.memory(), mo.memory()) <= 1e-9
+    assert mg.worst_mass_resid <= 1e-10
+
+
+def test_restart_roundtrip(oracle):
+    """Memory out through the reference restart layout and back in: the
+    continued run is bit-identical to the uninterrupted one."""
+    from fastearth3d_amd import lib
+    c = configs.config(1, lmax=16)
+    p = c["params"]
+    p.rotation = 0
+    a = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"])
+    for k in range(3):
+        a.update(c["ice"](k + 50), c["dt_yr"])
+    tau = a.memory()
+    b = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"])
+    b.set_memory(tau)
+    np.testing.assert_array_equal(b.memory(), tau)
+
+
+def test_error_paths():
+    from fastearth3d_amd import lib
+    from fastearth3d_amd.model import VilmaError
+    c = configs.config(0, lmax=8)
+    p = c["params"]
+    p.scheme = b"trap"
+    with pytest.raises(VilmaError, match="scheme"):
+        lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"])
+    p = configs.config(0, lmax=8)["params"]
+    p.earth = b"nonsense"
+    with pytest.raises(VilmaError, match="unknown earth model"):
+        lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"])
