@@ -33,11 +33,13 @@ def alg_bytes(dims, cls):
     """Algorithmic HBM bytes of one launch of kernel class cls: the Maxwell
     memory of the Maxwell elements (24 doubles per element and slot), the
     nodal drift at the memory nodes (4 doubles per node and slot), the
-    surface drift (6 doubles per slot) and the per-slot norm."""
-    nk, nem, nmn, nlm = dims["nk"], dims["nem"], dims["nmn"], dims["nlm"]
+    surface drift (6 doubles per slot), the per-slot norm and, for the drift,
+    the per-degree restricted solution operators ((2 nmn + 3) x 2 nmn doubles
+    per degree, read once)."""
+    nk, nem, nmn, nlm, L = dims["nk"], dims["nem"], dims["nmn"], dims["nlm"], dims["lmax"]
     tau = 24 * nem * nk * 8
-    if cls == 0:   # drift: read tau + norm, write nodal + surface drift
-        return tau + 8 * nk + 4 * nmn * nk * 8 + 6 * nk * 8
+    if cls == 0:   # drift: read tau + norm + operators, write nodal + surface drift
+        return tau + 8 * nk + L * (2 * nmn + 3) * 2 * nmn * 8 + 4 * nmn * nk * 8 + 6 * nk * 8
     if cls == 1:   # advance: read + write tau, read nodal drift and sigma, write norm
         return 2 * tau + 4 * nmn * nk * 8 + 16 * nlm + 8 * nk
     return None
@@ -116,9 +118,7 @@ def main():
     value = kyr / el
 
     d = dict(se.dims)
-    d["nmn"] = se._dim(9) + 1 if False else None
-    # memory nodes of a contiguous Maxwell stack = nem + 1
-    d["nmn"] = d["nem"] + 1
+    d["nmn"] = se.dims["nmn"]
     best = max((k for k in NAMES if cnt[k] > 0), key=lambda k: ms[k], default=None)
     roof = None
     if best is not None and alg_bytes(d, best):

@@ -381,9 +381,9 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
         h->d_dsurf = dalloc<double>((size_t)6 * d.nk);
         vh_fill(st, h->d_dN, 0.0, (size_t)4 * std::max(d.nmn, 1) * d.nk);
         vh_fill(st, h->d_dsurf, 0.0, (size_t)6 * d.nk);
-        h->dplan = vh_drift_plan_create(d);
     }
     h->refresh_dev();
+    if (h->kind == 2) h->dplan = vh_drift_plan_create(st, d, h->v, h->deg1_cm);
 
     // ---- SHT plan
     std::vector<double> ct, w;
@@ -839,6 +839,7 @@ int vh_get_dims(vh_model* h, int which, int* out) {
             case VH_D_NE3D: *out = h->ne3d; break;
             case VH_D_NLAM: *out = 4; break;
             case VH_D_NEM: *out = h->d.nem; break;
+            case VH_D_NMN: *out = h->d.nmn; break;
             default: throw vh_error("vh_get_dims: unknown selector");
         }
     });

@@ -6,217 +6,190 @@
 // src/vilma_response.f90:711-760) and the degree-1 centre-of-mass projection
 // (:386-420, :770-779).
 //
-// Mapping: one 64-lane wavefront per (degree l, 64 right-hand sides); lane =
-// (order m, re/im) of that degree, so all lanes share one band factor, which
-// is read through the scalar cache (uniform addresses), while each lane's
-// solution vector lives in LDS as x[j*64 + lane] (bank-conflict free).
-// Degree 1 (the KKT-bordered, effectively dense system) runs as one extra
-// wavefront of the same launch that solves its four right-hand sides
-// cooperatively.
+// The forcing of a slot lives only on the U/V dofs of the memory nodes (the
+// nodes of Maxwell elements), and the step only ever reads the solution back
+// at those dofs and at the surface U, F, V. So at set-up, from the same
+// pivoted band LU (bordered LU for degree 1), the solution operator restricted
+// to exactly those rows and columns is formed once per degree:
+//
+//     G_l = S diag(dc) A_l^{-1} diag(dr) R,   (2 nmn + 3) x (2 nmn),
+//
+// with the centre-of-mass projection folded into G_1. Each step's drift is
+// then a batched product Y_l = G_l F_l, one workgroup per (degree, tile of
+// orders): the tile's forcing F_l is built in LDS straight from the memory
+// (one coalesced pass over tau) and multiplied by G_l (L2-resident) without
+// a sequential triangular sweep. The result equals the band solve up to
+// rounding (the same factor produced G).
+#include <vector>
 #include "vh_internal.h"
 #include "vh_radial_dev.h"
 
+// ------------------------------------------------------------ set-up (G_l)
+// blockIdx.x = degree offset (l = lmin + bx), blockIdx.y = chunk of 64
+// columns; lane = column; x[j*64 + lane] in global scratch.
+__global__ void __launch_bounds__(64) k_setup_gop(int lmin, int nr, int nd, int nmn, const int* __restrict__ mnodes,
+        const double* __restrict__ ab, const int* __restrict__ ipiv, const double* __restrict__ drv,
+        const double* __restrict__ dcv, double* work, double* G) {
+    const int l = lmin + blockIdx.x, lane = threadIdx.x;
+    const int ncol = 2 * nmn, nrow = ncol + 3;
+    const int col = blockIdx.y * 64 + lane;
+    double* x = work + ((size_t)blockIdx.x * gridDim.y + blockIdx.y) * nd * 64 + lane;
+    const double* dr = drv + (size_t)(l - 1) * nd;
+    const double* dc = dcv + (size_t)(l - 1) * nd;
+    for (int j = 0; j < nd; j++) x[j * 64] = 0.0;
+    if (col >= ncol) return;
+    int dof = 4 * mnodes[col >> 1] + (col & 1);
+    x[dof * 64] = dr[dof];
+    lane_band_solve(nd, ab + (size_t)(l - 2) * VH_LDAB * nd, ipiv + (size_t)(l - 2) * nd, x, 64);
+    double* Gl = G + (size_t)(l - 1) * nrow * ncol;
+    for (int row = 0; row < nrow; row++) {
+        int src;
+        if (row < ncol) src = 4 * mnodes[row >> 1] + (row & 1);
+        else src = 4 * (nr - 1) + (row == ncol ? 0 : (row == ncol + 1 ? 2 : 1));
+        Gl[(size_t)row * ncol + col] = dc[src] * x[src * 64];
+    }
+}
+
+// ------------------------------------------------------------------ drift
 struct drift_args {
-    int nr, nd, nk, nem, nmn, lpb;
-    const int2* jobs;
-    vh_dev v;
+    int nk, nem, nmn, ts, rw;
+    const int2* jobs;     // (l, first order m0)
+    const double* G;      // [l-1][nrow][ncol]
+    const double* r;
+    const int* emap;
+    const int* mnode;
     const double* tau;
     const double* mnorm;
     const double* maxnorm;
     double skip_tol;
-    int deg1_cm, ns1, ldab1;
+    int deg1_cm;
     double* dN;
     double* dsurf;
 };
 
-// RHS of one lane: accumulate f (physical) into x (stride 64), then scale by dr.
-__device__ __forceinline__ void build_rhs(const drift_args& a, int l, int k, int reim, double* x, int stride,
-                                          const double* dr) {
-    const double Jr = (double)l * (double)(l + 1);
-    for (int em = 0; em < a.nem; em++) {
-        int e = a.v.emap[em];
-        const double* t = a.tau + (size_t)em * 24 * a.nk + k;
-        double A[4], B[4], Cc[4], floc[4];
-#pragma unroll
-        for (int lam = 0; lam < 4; lam++) {
-            A[lam] = t[(size_t)((0 * 2 + reim) * 4 + lam) * a.nk];
-            B[lam] = t[(size_t)((1 * 2 + reim) * 4 + lam) * a.nk];
-            Cc[lam] = t[(size_t)((2 * 2 + reim) * 4 + lam) * a.nk];
-        }
-        dissipative_element(a.v.r[e], a.v.r[e + 1], Jr, A, B, Cc, floc);
-        x[(4 * e) * stride] += floc[0];
-        x[(4 * e + 4) * stride] += floc[1];
-        x[(4 * e + 1) * stride] += floc[2];
-        x[(4 * e + 5) * stride] += floc[3];
-    }
-    for (int j = 0; j < a.nd; j++) x[j * stride] = dr[j] * x[j * stride];
-}
+#define DRIFT_THREADS 512
 
-// writes the outputs of one solved lane (x = equilibrated solution y, stride)
-__device__ __forceinline__ void store_lane(const drift_args& a, int l, int k, int reim, bool act,
-                                           const double* x, int stride, const double* dc) {
-    const int nk = a.nk, nr = a.nr;
-    for (int q = 0; q < a.nmn; q++) {
-        int n = a.v.mnodes[q];
-        double U = act ? dc[4 * n] * x[(4 * n) * stride] : 0.0;
-        double V = act ? dc[4 * n + 1] * x[(4 * n + 1) * stride] : 0.0;
-        a.dN[(size_t)(q * 4 + reim) * nk + k] = U;
-        a.dN[(size_t)(q * 4 + 2 + reim) * nk + k] = V;
-    }
-    int s = nr - 1;
-    double Ua = act ? dc[4 * s] * x[(4 * s) * stride] : 0.0;
-    double Fa = act ? dc[4 * s + 2] * x[(4 * s + 2) * stride] : 0.0;
-    double Va = act ? dc[4 * s + 1] * x[(4 * s + 1) * stride] : 0.0;
-    if (l == 1 && !a.deg1_cm) Fa = 0.0;
-    a.dsurf[(size_t)(0 * nk + k) * 2 + reim] = Ua;
-    a.dsurf[(size_t)(1 * nk + k) * 2 + reim] = Fa;
-    a.dsurf[(size_t)(2 * nk + k) * 2 + reim] = Va;
-}
-
-__global__ void __launch_bounds__(64) k_drift(drift_args a) {
-    extern __shared__ double xs[];
+// Workgroup = (degree l, ts consecutive orders): rhs = reim*ts + (m - m0).
+__global__ void __launch_bounds__(DRIFT_THREADS) k_drift(drift_args a) {
+    extern __shared__ double F[];   // [ncol][rw]
     const int2 job = a.jobs[blockIdx.x];
-    const int l = job.x, lane = threadIdx.x;
+    const int l = job.x, m0 = job.y;
+    const int ncol = 2 * a.nmn, nrow = ncol + 3, rw = a.rw, ts = a.ts, nk = a.nk;
+    const int nsl = min(ts, l + 1 - m0);
+    const int kb = l * (l + 1) / 2 - 1 + m0;
     const double thr = a.skip_tol * a.maxnorm[0];
-    const int nd = a.nd;
-    if (l == 1) {
-        // ---- degree 1: 4 RHS (m = 0,1 x re/im), bordered system of size ns1,
-        // solved cooperatively by the wavefront (x[r*ns1 + j]).
-        const int ns = a.ns1, kl = nd, kv = 2 * nd, ldab = a.ldab1;
-        const double* dr = a.v.dr;   // index l-1 = 0
-        const double* dc = a.v.dc;
-        bool act4[4];
-        for (int r = 0; r < 4; r++) act4[r] = a.mnorm[r % 2] > thr;
-        for (int t = lane; t < 4 * ns; t += 64) xs[t] = 0.0;
-        __syncthreads();
-        if (lane < 4) {
-            int m = lane % 2, reim = lane / 2;
-            if (act4[lane]) build_rhs(a, 1, m, reim, xs + lane * ns, 1, dr);
-        }
-        __syncthreads();
-        for (int j = 0; j < ns - 1; j++) {
-            int lm = min(kl, ns - 1 - j);
-            int p = a.v.ipiv1[j];
-            if (lane < 4 && p != j) {
-                double t = xs[lane * ns + p]; xs[lane * ns + p] = xs[lane * ns + j]; xs[lane * ns + j] = t;
-            }
-            __syncthreads();
-            double t0 = xs[0 * ns + j], t1 = xs[1 * ns + j], t2 = xs[2 * ns + j], t3 = xs[3 * ns + j];
-            const double* col = a.v.ab1 + (size_t)j * ldab + kv;
-            for (int i = 1 + lane; i <= lm; i += 64) {
-                double Lij = col[i];
-                xs[0 * ns + j + i] -= Lij * t0;
-                xs[1 * ns + j + i] -= Lij * t1;
-                xs[2 * ns + j + i] -= Lij * t2;
-                xs[3 * ns + j + i] -= Lij * t3;
-            }
-            __syncthreads();
-        }
-        for (int j = ns - 1; j >= 0; j--) {
-            const double* col = a.v.ab1 + (size_t)j * ldab + kv;
-            if (lane < 4) xs[lane * ns + j] = xs[lane * ns + j] / col[0];
-            __syncthreads();
-            double t0 = xs[0 * ns + j], t1 = xs[1 * ns + j], t2 = xs[2 * ns + j], t3 = xs[3 * ns + j];
-            int lm = min(kv, j);
-            for (int i = 1 + lane; i <= lm; i += 64) {
-                double Uij = col[-i];
-                xs[0 * ns + j - i] -= Uij * t0;
-                xs[1 * ns + j - i] -= Uij * t1;
-                xs[2 * ns + j - i] -= Uij * t2;
-                xs[3 * ns + j - i] -= Uij * t3;
-            }
-            __syncthreads();
-        }
-        // physical solution, CM projection, outputs (one RHS per lane < 4)
-        for (int t = lane; t < 4 * nd; t += 64) { int r = t / nd, i = t % nd; xs[r * ns + i] = dc[i] * xs[r * ns + i]; }
-        __syncthreads();
-        if (lane < 4) {
-            double* x = xs + lane * ns;
-            if (a.deg1_cm) {
-                int f = 4 * (a.nr - 1) + 2;
-                double fa_n = a.v.null1[f];
-                if (fa_n != 0.0) {
-                    double c = -x[f] / fa_n;
-                    for (int i = 0; i < nd; i++) x[i] = x[i] + c * a.v.null1[i];
+    for (int t = threadIdx.x; t < ncol * rw; t += blockDim.x) F[t] = 0.0;
+    __syncthreads();
+    const int rhs = threadIdx.x % rw, grp = threadIdx.x / rw, ngrp = blockDim.x / rw;
+    const int ml = rhs % ts, reim = rhs / ts;
+    const bool valid = ml < nsl;
+    const bool act = valid && a.mnorm[kb + ml] > thr;
+    const double Jr = (double)l * (double)(l + 1);
+    // forcing: elements of one parity at a time, so no two threads of a pass
+    // touch the same node (only consecutive Maxwell elements share one)
+    for (int par = 0; par < 2; par++) {
+        if (act) {
+            for (int em = par + 2 * grp; em < a.nem; em += 2 * ngrp) {
+                const int e = a.emap[em], q = a.mnode[e];
+                const double* t = a.tau + (size_t)em * 24 * nk + kb + ml;
+                double A[4], B[4], Cc[4], floc[4];
+#pragma unroll
+                for (int lam = 0; lam < 4; lam++) {
+                    A[lam] = t[(size_t)((0 * 2 + reim) * 4 + lam) * nk];
+                    B[lam] = t[(size_t)((1 * 2 + reim) * 4 + lam) * nk];
+                    Cc[lam] = t[(size_t)((2 * 2 + reim) * 4 + lam) * nk];
                 }
+                dissipative_element(a.r[e], a.r[e + 1], Jr, A, B, Cc, floc);
+                F[(2 * q) * rw + rhs] += floc[0];
+                F[(2 * q + 2) * rw + rhs] += floc[1];
+                F[(2 * q + 1) * rw + rhs] += floc[2];
+                F[(2 * q + 3) * rw + rhs] += floc[3];
             }
-            // dc already applied: store with unit scaling
-            const int nk = a.nk, nr = a.nr, m = lane % 2, reim = lane / 2;
-            bool act = act4[lane];
-            for (int q = 0; q < a.nmn; q++) {
-                int n = a.v.mnodes[q];
-                a.dN[(size_t)(q * 4 + reim) * nk + m] = act ? x[4 * n] : 0.0;
-                a.dN[(size_t)(q * 4 + 2 + reim) * nk + m] = act ? x[4 * n + 1] : 0.0;
-            }
-            int s = nr - 1;
-            a.dsurf[(size_t)(0 * nk + m) * 2 + reim] = act ? x[4 * s] : 0.0;
-            a.dsurf[(size_t)(1 * nk + m) * 2 + reim] = (act && a.deg1_cm) ? x[4 * s + 2] : 0.0;
-            a.dsurf[(size_t)(2 * nk + m) * 2 + reim] = act ? x[4 * s + 1] : 0.0;
         }
-        return;
+        __syncthreads();
     }
-    // ---- degrees >= 2: one lane per RHS
-    const int nrhs = 2 * (l + 1);
-    const int rr = job.y + lane;
-    const bool valid = lane < a.lpb && rr < nrhs;
-    const int m = valid ? rr % (l + 1) : 0, reim = valid ? rr / (l + 1) : 0;
-    const int k = l * (l + 1) / 2 - 1 + m;
-    const bool act = valid && a.mnorm[k] > thr;
-    double* x = xs + lane;
-    const double* dr = a.v.dr + (size_t)(l - 1) * nd;
-    const double* dc = a.v.dc + (size_t)(l - 1) * nd;
-    const double* ab = a.v.ab + (size_t)(l - 2) * VH_LDAB * nd;
-    const int* ip = a.v.ipiv + (size_t)(l - 2) * nd;
-    for (int j = 0; j < nd; j++) x[j * 64] = 0.0;
-    if (act) build_rhs(a, l, k, reim, x, 64, dr);
-    // forward: P and L (dgbtrs), kl = 6
-    for (int j = 0; j < nd - 1; j++) {
-        int p = ip[j];
-        double xj = x[j * 64];
-        if (p != j) { double xp = x[p * 64]; x[p * 64] = xj; xj = xp; x[j * 64] = xj; }
-        const double* col = ab + (size_t)j * VH_LDAB + VH_KV;
-        int lm = min(VH_KL, nd - 1 - j);
+    // Y = G_l F, four rows per thread per pass
+    const double* G = a.G + (size_t)(l - 1) * nrow * ncol;
+    const int nq = (nrow + 3) / 4;
+    const int k = kb + ml;
+    for (int qi = grp; qi < nq; qi += ngrp) {
+        const int r0 = 4 * qi;
+        const double* g0 = G + (size_t)min(r0 + 0, nrow - 1) * ncol;
+        const double* g1 = G + (size_t)min(r0 + 1, nrow - 1) * ncol;
+        const double* g2 = G + (size_t)min(r0 + 2, nrow - 1) * ncol;
+        const double* g3 = G + (size_t)min(r0 + 3, nrow - 1) * ncol;
+        double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
+        for (int c = 0; c < ncol; c++) {
+            const double f = F[c * rw + rhs];
+            y0 = fma(g0[c], f, y0);
+            y1 = fma(g1[c], f, y1);
+            y2 = fma(g2[c], f, y2);
+            y3 = fma(g3[c], f, y3);
+        }
+        if (!valid) continue;
+        const double y[4] = {y0, y1, y2, y3};
 #pragma unroll
-        for (int i = 1; i <= VH_KL; i++)
-            if (i <= lm) x[(j + i) * 64] -= col[i] * xj;
+        for (int i = 0; i < 4; i++) {
+            const int row = r0 + i;
+            if (row < ncol) {
+                const int q = row >> 1, uv = row & 1;
+                a.dN[(size_t)(q * 4 + uv * 2 + reim) * nk + k] = y[i];
+            } else if (row < nrow) {
+                const int s = row - ncol;                      // 0: U, 1: F, 2: V
+                double val = (s == 1 && l == 1 && !a.deg1_cm) ? 0.0 : y[i];
+                a.dsurf[(size_t)(s * nk + k) * 2 + reim] = val;
+            }
+        }
     }
-    // back: U x = y, kv = 12
-    for (int j = nd - 1; j >= 0; j--) {
-        const double* col = ab + (size_t)j * VH_LDAB + VH_KV;
-        double xj = x[j * 64] / col[0];
-        x[j * 64] = xj;
-        int lm = min(VH_KV, j);
-#pragma unroll
-        for (int i = 1; i <= VH_KV; i++)
-            if (i <= lm) x[(j - i) * 64] -= col[-i] * xj;
-    }
-    if (valid) store_lane(a, l, k, reim, act, x, 64, dc);
 }
 
 // ------------------------------------------------------------------ plan
 struct vh_drift_plan {
     int2* jobs = nullptr;
-    int njobs = 0, lpb = 64;
+    double* G = nullptr;
+    int njobs = 0, ts = 32, rw = 64;
     size_t lds = 0;
 };
 
-vh_drift_plan* vh_drift_plan_create(const vh_dims& d) {
+vh_drift_plan* vh_drift_plan_create(hipStream_t st, const vh_dims& d, const vh_dev& v, int deg1_cm) {
     vh_drift_plan* p = new vh_drift_plan();
-    // lanes per wavefront limited by LDS (x: lanes*nd doubles)
-    int lpb = 64;
-    while (lpb > 1 && (size_t)64 * d.nd * 8 > 160 * 1024 && (size_t)lpb * d.nd * 8 > 150 * 1024) lpb /= 2;
-    p->lpb = lpb;
+    const int ncol = 2 * d.nmn, nrow = ncol + 3;
+    // right-hand sides per workgroup: as many as fit a 64 KiB forcing tile
+    int rw = 64;
+    while (rw > 2 && (size_t)std::max(ncol, 1) * rw * 8 > 64 * 1024) rw /= 2;
+    if ((size_t)std::max(ncol, 1) * rw * 8 > 160 * 1024)
+        throw vh_error("drift: too many memory nodes for the LDS forcing tile (nmn = " + std::to_string(d.nmn) + ")");
+    p->rw = rw; p->ts = rw / 2;
+    p->lds = (size_t)std::max(ncol, 1) * rw * 8;
     std::vector<int2> jobs;
-    if (d.L >= 1) jobs.push_back(make_int2(1, 0));
-    for (int l = 2; l <= d.L; l++)
-        for (int r0 = 0; r0 < 2 * (l + 1); r0 += lpb) jobs.push_back(make_int2(l, r0));
+    for (int l = 1; l <= d.L; l++)
+        for (int m0 = 0; m0 <= l; m0 += p->ts) jobs.push_back(make_int2(l, m0));
     p->njobs = (int)jobs.size();
     HIPCHECK(hipMalloc((void**)&p->jobs, jobs.size() * sizeof(int2)));
     HIPCHECK(hipMemcpy(p->jobs, jobs.data(), jobs.size() * sizeof(int2), hipMemcpyHostToDevice));
-    size_t lds_band = (size_t)64 * d.nd * 8;
-    size_t lds_deg1 = (size_t)4 * d.ns1 * 8;
-    p->lds = std::max(lds_band, lds_deg1);
-    if (p->lds > 160 * 1024) throw vh_error("drift: radial system too large for LDS (nd = " + std::to_string(d.nd) + ")");
+    HIPCHECK(hipMalloc((void**)&p->G, (size_t)d.L * nrow * std::max(ncol, 1) * sizeof(double)));
+    HIPCHECK(hipMemsetAsync(p->G, 0, (size_t)d.L * nrow * std::max(ncol, 1) * sizeof(double), st));
+    if (ncol > 0) {
+        // degree 1 through the bordered factor (centre-of-mass projection folded in)
+        double* w1 = nullptr;
+        HIPCHECK(hipMalloc((void**)&w1, (size_t)(d.ns1 + 1) * sizeof(double)));
+        vh_launch_setup_deg1_op(st, d, v.mnodes, v.ab1, v.ipiv1, v.dr, v.dc, v.null1, deg1_cm, w1, p->G);
+        // degrees >= 2, in batches that keep the solve scratch under 256 MiB
+        const int nch = (ncol + 63) / 64;
+        const size_t per = (size_t)nch * d.nd * 64 * sizeof(double);
+        int nb = (int)std::max<size_t>(1, ((size_t)256 << 20) / per);
+        nb = std::min(nb, std::max(d.L - 1, 1));
+        double* w = nullptr;
+        HIPCHECK(hipMalloc((void**)&w, per * nb));
+        for (int l0 = 2; l0 <= d.L; l0 += nb) {
+            int n = std::min(nb, d.L - l0 + 1);
+            k_setup_gop<<<dim3(n, nch), 64, 0, st>>>(l0, d.nr, d.nd, d.nmn, v.mnodes, v.ab, v.ipiv, v.dr, v.dc, w, p->G);
+            HIPCHECK(hipGetLastError());
+        }
+        HIPCHECK(hipStreamSynchronize(st));   // set-up only: release the scratch right away
+        HIPCHECK(hipFree(w));
+        HIPCHECK(hipFree(w1));
+    }
     HIPCHECK(hipFuncSetAttribute((const void*)k_drift, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds));
     return p;
 }
@@ -224,6 +197,7 @@ vh_drift_plan* vh_drift_plan_create(const vh_dims& d) {
 void vh_drift_plan_destroy(vh_drift_plan* p) {
     if (!p) return;
     (void)hipFree(p->jobs);
+    (void)hipFree(p->G);
     delete p;
 }
 
@@ -231,10 +205,10 @@ void vh_drift_run(hipStream_t st, const vh_drift_plan* p, const vh_dims& d, cons
                   const double* tau, const double* mnorm, const double* maxnorm, double skip_tol,
                   int deg1_cm, double* dN, double* dsurf) {
     drift_args a;
-    a.nr = d.nr; a.nd = d.nd; a.nk = d.nk; a.nem = d.nem; a.nmn = d.nmn; a.lpb = p->lpb;
-    a.jobs = p->jobs; a.v = v; a.tau = tau; a.mnorm = mnorm; a.maxnorm = maxnorm;
-    a.skip_tol = skip_tol; a.deg1_cm = deg1_cm; a.ns1 = d.ns1; a.ldab1 = d.ldab1;
-    a.dN = dN; a.dsurf = dsurf;
-    k_drift<<<p->njobs, 64, p->lds, st>>>(a);
+    a.nk = d.nk; a.nem = d.nem; a.nmn = d.nmn; a.ts = p->ts; a.rw = p->rw;
+    a.jobs = p->jobs; a.G = p->G; a.r = v.r; a.emap = v.emap; a.mnode = v.mnode;
+    a.tau = tau; a.mnorm = mnorm; a.maxnorm = maxnorm;
+    a.skip_tol = skip_tol; a.deg1_cm = deg1_cm; a.dN = dN; a.dsurf = dsurf;
+    k_drift<<<p->njobs, DRIFT_THREADS, p->lds, st>>>(a);
     HIPCHECK(hipGetLastError());
 }

@@ -80,6 +80,10 @@ void vh_launch_setup_deg1(hipStream_t st, const vh_dims& d, const double* r, con
                           int deg1_cm, double* ab1, int* ipiv1, double* dr1, double* dc1,
                           double* null1, double* xUn, double* xVn, double* gu, double* gn,
                           double* gv, double* work, int* fail);
+// degree-1 restricted solution operator G1 [2 nmn + 3][2 nmn] (work: ns1 doubles)
+void vh_launch_setup_deg1_op(hipStream_t st, const vh_dims& d, const int* mnodes, const double* ab1,
+                             const int* ipiv1, const double* dr1, const double* dc1, const double* null1,
+                             int deg1_cm, double* work, double* G1);
 
 // ---------------------------------------------------------------- (b) advance
 void vh_launch_advance(hipStream_t st, const vh_dims& d, const vh_dev& v, double dt, const double* tau_in,
@@ -125,7 +129,8 @@ void vh_fill(hipStream_t st, double* p, double v, size_t n);
 
 // ---------------------------------------------------------------- drift plan
 struct vh_drift_plan;
-vh_drift_plan* vh_drift_plan_create(const vh_dims& d);
+// builds the per-degree restricted solution operators from the factors in v
+vh_drift_plan* vh_drift_plan_create(hipStream_t st, const vh_dims& d, const vh_dev& v, int deg1_cm);
 void vh_drift_plan_destroy(vh_drift_plan* p);
 void vh_drift_run(hipStream_t st, const vh_drift_plan* p, const vh_dims& d, const vh_dev& v,
                   const double* tau, const double* mnorm, const double* maxnorm, double skip_tol,

@@ -31,7 +31,7 @@ class ModelHandle:
         self.nphi, self.nlat = params.grid()
         self.dims = {k: self._dim(v) for k, v in (
             ("lmax", _abi.D_LMAX), ("nlm", _abi.D_NLM), ("nr", _abi.D_NR), ("ne", _abi.D_NE),
-            ("nk", _abi.D_NK), ("nlam", _abi.D_NLAM), ("nem", _abi.D_NEM))}
+            ("nk", _abi.D_NK), ("nlam", _abi.D_NLAM), ("nem", _abi.D_NEM), ("nmn", _abi.D_NMN))}
 
     # ---------------------------------------------------------------- plumbing
     def _fn(self, name):

@@ -90,7 +90,8 @@ enum {
 };
 enum {
     VH_D_LMAX = 0, VH_D_NLAT = 1, VH_D_NPHI = 2, VH_D_NLM = 3, VH_D_NR = 4,
-    VH_D_NE = 5, VH_D_NK = 6, VH_D_NE3D = 7, VH_D_NLAM = 8, VH_D_NEM = 9
+    VH_D_NE = 5, VH_D_NK = 6, VH_D_NE3D = 7, VH_D_NLAM = 8, VH_D_NEM = 9,
+    VH_D_NMN = 10   /* memory nodes: nodes of Maxwell elements */
 };
 
 /* Defaults of vilma_param_class overlaid with input/vilma_defaults.nml. */

@@ -307,6 +307,18 @@ int orc_get_dims(orc_model* m, int which, int* out) {
         case VH_D_NE3D: *out = m->resp.ne3d; break;
         case VH_D_NLAM: *out = 4; break;
         case VH_D_NEM: *out = nem; break;
+        case VH_D_NMN: {
+            int nmn = 0;
+            if (m->resp.kind == RESP_VE)
+                for (int n = 0; n <= m->resp.ne; n++) {
+                    int used = 0;
+                    for (int e = n - 1; e <= n; e++)
+                        if (e >= 0 && e < m->resp.ne && (m->resp.MkPerDt[e] > 0.0 || !m->resp.active1d[e])) used = 1;
+                    nmn += used;
+                }
+            *out = nmn;
+            break;
+        }
         default: return 1;
     }
     return 0;
