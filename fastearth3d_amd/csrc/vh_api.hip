@@ -179,6 +179,8 @@ struct vh_model {
     double maxn_cur = 0.0;
     double *d_dN = 0, *d_dsurf = 0;
     vh_drift_plan* dplan = nullptr;
+    int *d_emap2 = 0, *d_mnodes2 = 0, *d_mnode2 = 0;   // rotation channel memory
+    double* d_G2 = 0;
     // device: SHT
     double *d_plm_lj = 0, *d_dplm_lj = 0, *d_plm_jl = 0, *d_dplm_jl = 0, *d_w = 0, *d_st = 0, *d_ct = 0;
     int* d_mstart = 0;
@@ -258,7 +260,8 @@ vh_model::~vh_model() {
                     d_st, d_ct, d_mstart, d_tw, d_topo0, d_heq, d_C0ref, d_hice, d_ice1, d_rsl, d_rsl_n, d_C,
                     d_Cn, d_dig, d_C0s, d_ice, d_dice, d_wcorr, d_sraw, d_srot, d_zbed, d_gtmp, d_F, d_F2,
                     d_sigma, d_lm, d_uN, d_part, d_sc, d_sc2, d_cphi, d_sphi,
-                    ra.rs.mem, ra.rs.MkPerDt, ra.rs.xU, ra.rs.xV, ra.rs.dN, ra.rs.sc};
+                    ra.rs.mem, ra.rs.MkPerDt, ra.rs.xU, ra.rs.xV, ra.rs.dN, ra.rs.sc, d_emap2, d_mnodes2,
+                    d_mnode2, d_G2};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : evpool) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
@@ -510,7 +513,23 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
         sc[RS_A] = h->a; sc[RS_G] = h->g; sc[RS_CMA] = 2.63e35; sc[RS_OMEGA] = 7.292115e-5;
         upload(rs.sc, sc, RS_N, st);
         h->ra.nr = nr; h->ra.nd = d.nd; h->ra.r = h->d_r; h->ra.mu = h->d_mu;
-        h->ra.ab2 = h->d_ab; h->ra.ipiv2 = h->d_ipiv; h->ra.dr2 = h->d_dr + d.nd; h->ra.dc2 = h->d_dc + d.nd;
+        {   // channel-memory elements and nodes, restricted degree-2 operator
+            std::vector<int> em2, mn2, mnode2(nr, -1);
+            for (int e = 0; e < ne; e++)
+                if (rk2[e] > 0.0 || rk2[ne + e] > 0.0) { em2.push_back(e); mnode2[e] = 1; mnode2[e + 1] = 1; }
+            for (int n = 0; n < nr; n++)
+                if (mnode2[n] > 0) { mnode2[n] = (int)mn2.size(); mn2.push_back(n); }
+            h->ra.nem2 = (int)em2.size(); h->ra.nmn2 = (int)mn2.size();
+            h->d_emap2 = dalloc<int>(std::max<size_t>(em2.size(), 1)); upload(h->d_emap2, em2.data(), em2.size(), st);
+            h->d_mnodes2 = dalloc<int>(std::max<size_t>(mn2.size(), 1)); upload(h->d_mnodes2, mn2.data(), mn2.size(), st);
+            h->d_mnode2 = dalloc<int>(nr); upload(h->d_mnode2, mnode2.data(), nr, st);
+            const size_t ng2 = (size_t)(2 * mn2.size() + 3) * std::max<size_t>(2 * mn2.size(), 1);
+            h->d_G2 = dalloc<double>(ng2);
+            vh_fill(st, h->d_G2, 0.0, ng2);
+            vh_setup_restricted_ops(st, d, h->v, 2, 2, h->ra.nmn2, h->d_mnodes2, h->d_G2);
+            h->ra.emap2 = h->d_emap2; h->ra.mnodes2 = h->d_mnodes2; h->ra.mnode2 = h->d_mnode2; h->ra.G2 = h->d_G2;
+            h->sync();   // the host lists above go out of scope
+        }
         std::vector<double> sc2(nl), cph(np), sph(np);
         for (int j = 0; j < nl; j++) sc2[j] = std::sin(h->colat[j]) * std::cos(h->colat[j]);
         for (int i = 0; i < np; i++) { cph[i] = std::cos(h->lon[i]); sph[i] = std::sin(h->lon[i]); }

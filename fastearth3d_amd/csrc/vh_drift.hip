@@ -26,7 +26,8 @@
 
 // ------------------------------------------------------------ set-up (G_l)
 // blockIdx.x = degree offset (l = lmin + bx), blockIdx.y = chunk of 64
-// columns; lane = column; x[j*64 + lane] in global scratch.
+// columns; lane = column; x[j*64 + lane] in global scratch. G holds one
+// (2 nmn + 3) x (2 nmn) operator per block row, starting at degree lmin.
 __global__ void __launch_bounds__(64) k_setup_gop(int lmin, int nr, int nd, int nmn, const int* __restrict__ mnodes,
         const double* __restrict__ ab, const int* __restrict__ ipiv, const double* __restrict__ drv,
         const double* __restrict__ dcv, double* work, double* G) {
@@ -41,13 +42,34 @@ __global__ void __launch_bounds__(64) k_setup_gop(int lmin, int nr, int nd, int 
     int dof = 4 * mnodes[col >> 1] + (col & 1);
     x[dof * 64] = dr[dof];
     lane_band_solve(nd, ab + (size_t)(l - 2) * VH_LDAB * nd, ipiv + (size_t)(l - 2) * nd, x, 64);
-    double* Gl = G + (size_t)(l - 1) * nrow * ncol;
+    double* Gl = G + (size_t)blockIdx.x * nrow * ncol;
     for (int row = 0; row < nrow; row++) {
         int src;
         if (row < ncol) src = 4 * mnodes[row >> 1] + (row & 1);
         else src = 4 * (nr - 1) + (row == ncol ? 0 : (row == ncol + 1 ? 2 : 1));
         Gl[(size_t)row * ncol + col] = dc[src] * x[src * 64];
     }
+}
+
+void vh_setup_restricted_ops(hipStream_t st, const vh_dims& d, const vh_dev& v, int lmin, int lmax, int nmn,
+                             const int* mnodes, double* G) {
+    const int ncol = 2 * nmn, nrow = ncol + 3;
+    if (ncol == 0 || lmax < lmin) return;
+    if (lmin < 2) throw vh_error("restricted operator: band factors start at degree 2");
+    const int nch = (ncol + 63) / 64;
+    const size_t per = (size_t)nch * d.nd * 64 * sizeof(double);
+    int nb = (int)std::max<size_t>(1, ((size_t)256 << 20) / per);   // scratch under 256 MiB per batch
+    nb = std::min(nb, lmax - lmin + 1);
+    double* w = nullptr;
+    HIPCHECK(hipMalloc((void**)&w, per * nb));
+    for (int l0 = lmin; l0 <= lmax; l0 += nb) {
+        int n = std::min(nb, lmax - l0 + 1);
+        k_setup_gop<<<dim3(n, nch), 64, 0, st>>>(l0, d.nr, d.nd, nmn, mnodes, v.ab, v.ipiv, v.dr, v.dc, w,
+                                                 G + (size_t)(l0 - lmin) * nrow * ncol);
+        HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipStreamSynchronize(st));   // set-up only: release the scratch right away
+    HIPCHECK(hipFree(w));
 }
 
 // ------------------------------------------------------------------ drift
@@ -174,20 +196,8 @@ vh_drift_plan* vh_drift_plan_create(hipStream_t st, const vh_dims& d, const vh_d
         double* w1 = nullptr;
         HIPCHECK(hipMalloc((void**)&w1, (size_t)(d.ns1 + 1) * sizeof(double)));
         vh_launch_setup_deg1_op(st, d, v.mnodes, v.ab1, v.ipiv1, v.dr, v.dc, v.null1, deg1_cm, w1, p->G);
-        // degrees >= 2, in batches that keep the solve scratch under 256 MiB
-        const int nch = (ncol + 63) / 64;
-        const size_t per = (size_t)nch * d.nd * 64 * sizeof(double);
-        int nb = (int)std::max<size_t>(1, ((size_t)256 << 20) / per);
-        nb = std::min(nb, std::max(d.L - 1, 1));
-        double* w = nullptr;
-        HIPCHECK(hipMalloc((void**)&w, per * nb));
-        for (int l0 = 2; l0 <= d.L; l0 += nb) {
-            int n = std::min(nb, d.L - l0 + 1);
-            k_setup_gop<<<dim3(n, nch), 64, 0, st>>>(l0, d.nr, d.nd, d.nmn, v.mnodes, v.ab, v.ipiv, v.dr, v.dc, w, p->G);
-            HIPCHECK(hipGetLastError());
-        }
-        HIPCHECK(hipStreamSynchronize(st));   // set-up only: release the scratch right away
-        HIPCHECK(hipFree(w));
+        vh_setup_restricted_ops(st, d, v, 2, d.L, d.nmn, v.mnodes, p->G + (size_t)nrow * ncol);
+        HIPCHECK(hipStreamSynchronize(st));
         HIPCHECK(hipFree(w1));
     }
     HIPCHECK(hipFuncSetAttribute((const void*)k_drift, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds));

@@ -129,6 +129,11 @@ void vh_fill(hipStream_t st, double* p, double v, size_t n);
 
 // ---------------------------------------------------------------- drift plan
 struct vh_drift_plan;
+// Restricted solution operators G_l (l = lmin..lmax, l >= 2) of the band
+// factors in v: unit forcing at the U/V dofs of the nmn listed nodes ->
+// solution at those dofs and at the surface U, F, V; [l-lmin][2nmn+3][2nmn].
+void vh_setup_restricted_ops(hipStream_t st, const vh_dims& d, const vh_dev& v, int lmin, int lmax, int nmn,
+                             const int* mnodes, double* G);
 // builds the per-degree restricted solution operators from the factors in v
 vh_drift_plan* vh_drift_plan_create(hipStream_t st, const vh_dims& d, const vh_dev& v, int deg1_cm);
 void vh_drift_plan_destroy(vh_drift_plan* p);

@@ -26,10 +26,14 @@ struct rot_args {
     int nr, nd;
     const double* r;
     const double* mu;
-    const double* ab2;   // degree-2 band factor of the field driver
-    const int* ipiv2;
-    const double* dr2;
-    const double* dc2;
+    // elements that carry channel memory (mu/eta > 0) and their nodes; G2 is
+    // the degree-2 solution operator restricted to those nodes' U/V dofs plus
+    // the surface U, F, V rows: [2 nmn2 + 3][2 nmn2]
+    int nem2, nmn2;
+    const int* emap2;    // [nem2]  -> mesh element
+    const int* mnode2;   // [nr]    mesh node -> index in mnodes2 or -1
+    const int* mnodes2;  // [nmn2]
+    const double* G2;
     rot_state rs;
 };
 

@@ -2,53 +2,61 @@
 // motion with two compact Maxwell channels (loading and tidal)
 // (src/vilma_rotation.f90:154-249, :335-363, :408-487).
 //
-// The channels use the same degree-2 operator as the field driver, so they
-// solve against its band factor. Device state (rot_state below) holds the
+// The channels use the same degree-2 operator as the field driver; their
+// frozen drift goes through that operator's solution restricted to the nodes
+// of the elements that carry channel memory (formed once from its band LU). Device state (rot_state below) holds the
 // channel memory [ch][e][c] (c = (shape*2 + reim)*4 + lambda, every mesh
 // element, as the reference carries memory wherever eta > 0), the frozen
 // drift and the complex scalars, so a coupling interval needs no host round
 // trip for the rotation.
+#include <algorithm>
 #include "vh_internal.h"
 #include "vh_radial_dev.h"
 #include "vh_rot.h"
 
-// k_rot_begin: lanes 0..3 = (channel, re/im): dissipative RHS of the channel
-// memory, band solve, frozen nodal and surface drift.
-__global__ void __launch_bounds__(64) k_rot_begin(rot_args a) {
-    extern __shared__ double xs[];
-    const int lane = threadIdx.x, nd = a.nd, nr = a.nr;
-    if (lane >= 4) return;
-    const int ch = lane >> 1, reim = lane & 1;
-    double* x = xs + lane;   // stride 4
-    const int stride = 4;
-    for (int j = 0; j < nd; j++) x[j * stride] = 0.0;
+// k_rot_begin: dissipative forcing of the two channel memories (rhs =
+// channel*2 + re/im) built in LDS, then the frozen nodal and surface drift
+// through the restricted degree-2 operator G2.
+__global__ void __launch_bounds__(256) k_rot_begin(rot_args a) {
+    extern __shared__ double F[];   // [2 nmn2][4]
+    const int nr = a.nr, ne = nr - 1, ncol = 2 * a.nmn2, nrow = ncol + 3;
+    for (int t = threadIdx.x; t < ncol * 4; t += blockDim.x) F[t] = 0.0;
+    __syncthreads();
+    const int rhs = threadIdx.x & 3, grp = threadIdx.x >> 2, ngrp = blockDim.x >> 2;
+    const int ch = rhs >> 1, reim = rhs & 1;
     const double Jr = 6.0;
-    for (int e = 0; e < nr - 1; e++) {
-        const double* mem = a.rs.mem + ((size_t)ch * (nr - 1) + e) * 24;
-        double A[4], B[4], Cc[4], floc[4];
-        for (int lam = 0; lam < 4; lam++) {
-            A[lam] = mem[(0 * 2 + reim) * 4 + lam];
-            B[lam] = mem[(1 * 2 + reim) * 4 + lam];
-            Cc[lam] = mem[(2 * 2 + reim) * 4 + lam];
+    for (int par = 0; par < 2; par++) {   // consecutive elements share a node: one parity per pass
+        for (int em = par + 2 * grp; em < a.nem2; em += 2 * ngrp) {
+            const int e = a.emap2[em], q = a.mnode2[e];
+            const double* mem = a.rs.mem + ((size_t)ch * ne + e) * 24;
+            double A[4], B[4], Cc[4], floc[4];
+#pragma unroll
+            for (int lam = 0; lam < 4; lam++) {
+                A[lam] = mem[(0 * 2 + reim) * 4 + lam];
+                B[lam] = mem[(1 * 2 + reim) * 4 + lam];
+                Cc[lam] = mem[(2 * 2 + reim) * 4 + lam];
+            }
+            dissipative_element(a.r[e], a.r[e + 1], Jr, A, B, Cc, floc);
+            F[(2 * q) * 4 + rhs] += floc[0];
+            F[(2 * q + 2) * 4 + rhs] += floc[1];
+            F[(2 * q + 1) * 4 + rhs] += floc[2];
+            F[(2 * q + 3) * 4 + rhs] += floc[3];
         }
-        dissipative_element(a.r[e], a.r[e + 1], Jr, A, B, Cc, floc);
-        x[(4 * e) * stride] += floc[0];
-        x[(4 * e + 4) * stride] += floc[1];
-        x[(4 * e + 1) * stride] += floc[2];
-        x[(4 * e + 5) * stride] += floc[3];
+        __syncthreads();
     }
-    for (int j = 0; j < nd; j++) x[j * stride] = a.dr2[j] * x[j * stride];
-    lane_band_solve(nd, a.ab2, a.ipiv2, x, stride);
-    for (int n = 0; n < nr; n++) {
-        a.rs.dN[((size_t)ch * 4 + reim) * nr + n] = a.dc2[4 * n] * x[(4 * n) * stride];
-        a.rs.dN[((size_t)ch * 4 + 2 + reim) * nr + n] = a.dc2[4 * n + 1] * x[(4 * n + 1) * stride];
+    for (int row = grp; row < nrow - 1; row += ngrp) {   // the surface V row is not needed
+        const double* g = a.G2 + (size_t)row * ncol;
+        double y = 0.0;
+        for (int c = 0; c < ncol; c++) y = fma(g[c], F[c * 4 + rhs], y);
+        if (row < ncol) {
+            const int n = a.mnodes2[row >> 1], uv = row & 1;
+            a.rs.dN[((size_t)ch * 4 + uv * 2 + reim) * nr + n] = y;
+        } else if (row == ncol) {
+            a.rs.sc[RS_DU + 2 * ch + reim] = y;
+        } else {
+            a.rs.sc[RS_DF + 2 * ch + reim] = y;
+        }
     }
-    int s = nr - 1;
-    double Fa = a.dc2[4 * s + 2] * x[(4 * s + 2) * stride];
-    double Ua = a.dc2[4 * s] * x[(4 * s) * stride];
-    // scalars: dF[ch] at RS_DF + 2ch (+reim), dU[ch] at RS_DU + 2ch
-    a.rs.sc[RS_DF + 2 * ch + reim] = Fa;
-    a.rs.sc[RS_DU + 2 * ch + reim] = Ua;
 }
 
 // rotation_s_rot (src/vilma_rotation.f90:192-222)
@@ -139,8 +147,8 @@ __global__ void __launch_bounds__(256) k_rot_solve_commit(rot_args a, double dt)
 }
 
 void rot_begin(hipStream_t st, const rot_args& a) {
-    size_t lds = (size_t)4 * a.nd * sizeof(double);
-    k_rot_begin<<<1, 64, lds, st>>>(a);
+    size_t lds = (size_t)4 * std::max(2 * a.nmn2, 1) * sizeof(double);
+    k_rot_begin<<<1, 256, lds, st>>>(a);
     HIPCHECK(hipGetLastError());
 }
 

@@ -256,23 +256,32 @@ __global__ void __launch_bounds__(256) k_rowsum(const double* __restrict__ f, in
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__global__ void k_weighted_total(const double* __restrict__ part, const double* __restrict__ w, int nlat,
-                                 double dlon, int ncomp, double* out) {
-    int c = threadIdx.x;
-    if (c >= ncomp) return;
+// One block per component: the terms are staged in LDS by the whole block,
+// then summed in latitude order by one lane (the order fixes the rounding).
+__global__ void __launch_bounds__(256) k_weighted_total(const double* __restrict__ part,
+        const double* __restrict__ w, int nlat, double dlon, double* out) {
+    extern __shared__ double sv[];   // [2][nlat]
+    const int c = blockIdx.x;
+    for (int j = threadIdx.x; j < nlat; j += blockDim.x) {
+        sv[j] = w[j];
+        sv[nlat + j] = part[(size_t)c * nlat + j];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     double total = 0.0;
-    for (int j = 0; j < nlat; j++) total += w[j] * part[(size_t)c * nlat + j] * dlon;
+    for (int j = 0; j < nlat; j++) total += sv[j] * sv[nlat + j] * dlon;
     out[c] = total;
 }
 
 void vh_surface_integral(hipStream_t st, const vh_sht& s, const double* f, double* part, double* out) {
     k_rowsum<<<s.nlat, 256, 0, st>>>(f, s.nphi, part);
-    k_weighted_total<<<1, 64, 0, st>>>(part, s.w, s.nlat, 2.0 * VH_PI / s.nphi, 1, out);
+    k_weighted_total<<<1, 256, 2 * s.nlat * sizeof(double), st>>>(part, s.w, s.nlat, 2.0 * VH_PI / s.nphi, out);
     HIPCHECK(hipGetLastError());
 }
 
 void vh_weighted_total(hipStream_t st, const vh_sht& s, const double* part, int ncomp, double* out) {
-    k_weighted_total<<<1, 64, 0, st>>>(part, s.w, s.nlat, 2.0 * VH_PI / s.nphi, ncomp, out);
+    if (ncomp <= 0) return;
+    k_weighted_total<<<ncomp, 256, 2 * s.nlat * sizeof(double), st>>>(part, s.w, s.nlat, 2.0 * VH_PI / s.nphi, out);
     HIPCHECK(hipGetLastError());
 }
 

@@ -28,45 +28,79 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 
-// twiddle e^{-+2 pi i t/N}
-__device__ __forceinline__ double2 twid(const double2* tw, int t, int N, bool inv) {
-    double2 w = tw[t % N];
+// twiddle e^{-+2 pi i t/N}, 0 <= t < N
+__device__ __forceinline__ double2 twid(const double2* tw, int t, bool inv) {
+    double2 w = tw[t];
     return inv ? make_double2(w.x, -w.y) : w;
+}
+
+// Size-R DFT of v (in registers; compile-time R so nothing spills).
+// w[q] = e^{-+2 pi i q/R}, q = 1..R-1.
+template <int R>
+__device__ __forceinline__ void dft_small(double2 (&v)[R], double2 (&V)[R], const double2 (&w)[R], bool inv) {
+    if constexpr (R == 2) {
+        V[0] = cadd(v[0], v[1]); V[1] = csub(v[0], v[1]);
+    } else if constexpr (R == 4) {
+        double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        double2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+        // forward: W4 = -i ; -i*z = (z.y, -z.x); inverse: +i*z = (-z.y, z.x)
+        double2 jd13 = inv ? make_double2(-d13.y, d13.x) : make_double2(d13.y, -d13.x);
+        V[0] = cadd(s02, s13); V[2] = csub(s02, s13);
+        V[1] = cadd(d02, jd13); V[3] = csub(d02, jd13);
+    } else {
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+            double2 acc = v[0];
+#pragma unroll
+            for (int q = 1; q < R; q++) {
+                const int e = (q * u) % R;
+                acc = cadd(acc, e == 0 ? v[q] : cmul(v[q], w[e]));
+            }
+            V[u] = acc;
+        }
+    }
+}
+
+// One Stockham radix-R pass over N points by nthr threads (thread index t).
+template <int R>
+__device__ __forceinline__ void fft_pass(const double2* src, double2* dst, int N, int Ns, const double2* tw,
+                                         bool inv, int t, int nthr) {
+    const int NR = N / R, tstep = N / (Ns * R);
+    double2 w[R];
+#pragma unroll
+    for (int q = 1; q < R; q++) w[q] = twid(tw, q * NR, inv);
+    w[0] = make_double2(1.0, 0.0);
+    for (int j = t; j < NR; j += nthr) {
+        const int k = j % Ns;
+        double2 v[R], V[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = src[j + r * NR];
+#pragma unroll
+        for (int r = 1; r < R; r++) v[r] = cmul(v[r], twid(tw, k * r * tstep, inv));
+        dft_small<R>(v, V, w, inv);
+        const int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) dst[idxD + r * Ns] = V[r];
+    }
 }
 
 // One complex FFT of length N by the calling wavefront (lane 0..63). a, b:
 // LDS buffers of N; returns the buffer that holds the result. Natural order
-// in and out (Stockham autosort). fac: radices (2..8), product N.
+// in and out (Stockham autosort). fac: radices (2, 3, 4, 5, 7), product N.
+// Every wavefront of the block must call it with the same N (the passes are
+// separated by block barriers).
 __device__ double2* fft_wave(double2* a, double2* b, int N, const int* fac, int nfac, const double2* tw,
                              bool inv, int lane) {
     int Ns = 1;
     double2 *src = a, *dst = b;
     for (int s = 0; s < nfac; s++) {
-        const int R = fac[s], NR = N / R, tstep = N / (Ns * R);
-        for (int j = lane; j < NR; j += 64) {
-            double2 v[8], V[8];
-            const int k = j % Ns;
-            for (int r = 0; r < R; r++) v[r] = src[j + r * NR];
-            for (int r = 1; r < R; r++) v[r] = cmul(v[r], twid(tw, k * r * tstep, N, inv));
-            if (R == 2) {
-                V[0] = cadd(v[0], v[1]); V[1] = csub(v[0], v[1]);
-            } else if (R == 4) {
-                double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-                double2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
-                // forward: W4 = -i ; -i*z = (z.y, -z.x); inverse: +i*z = (-z.y, z.x)
-                double2 jd13 = inv ? make_double2(-d13.y, d13.x) : make_double2(d13.y, -d13.x);
-                V[0] = cadd(s02, s13); V[2] = csub(s02, s13);
-                V[1] = cadd(d02, jd13); V[3] = csub(d02, jd13);
-            } else {
-                const int step = N / R;
-                for (int u = 0; u < R; u++) {
-                    double2 acc = v[0];
-                    for (int q = 1; q < R; q++) acc = cadd(acc, cmul(v[q], twid(tw, (q * u % R) * step, N, inv)));
-                    V[u] = acc;
-                }
-            }
-            const int idxD = (j / Ns) * Ns * R + k;
-            for (int r = 0; r < R; r++) dst[idxD + r * Ns] = V[r];
+        const int R = fac[s];
+        switch (R) {
+            case 2: fft_pass<2>(src, dst, N, Ns, tw, inv, lane, 64); break;
+            case 3: fft_pass<3>(src, dst, N, Ns, tw, inv, lane, 64); break;
+            case 4: fft_pass<4>(src, dst, N, Ns, tw, inv, lane, 64); break;
+            case 5: fft_pass<5>(src, dst, N, Ns, tw, inv, lane, 64); break;
+            default: fft_pass<7>(src, dst, N, Ns, tw, inv, lane, 64); break;
         }
         __syncthreads();
         double2* t = src; src = dst; dst = t;
@@ -75,15 +109,15 @@ __device__ double2* fft_wave(double2* a, double2* b, int N, const int* fac, int 
     return src;
 }
 
-// Real rows -> F[row][m] (m = 0..L) scaled by `scale`; 4 row pairs per
-// 256-thread block, LDS 2 buffers x N per wavefront.
+// Real rows -> F[row][m] (m = 0..L) scaled by `scale`; one row pair per
+// wavefront, LDS 2 buffers x N per wavefront.
 template <class In>
 __global__ void __launch_bounds__(256) k_fft_fwd(vh_sht s, In in, int nrows, double2* __restrict__ F, double scale) {
     extern __shared__ double2 lds[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, N = s.nphi, L = s.L;
     double2* a = lds + (size_t)wave * 2 * N;
     double2* b = a + N;
-    const int pair = blockIdx.x * 4 + wave;
+    const int pair = blockIdx.x * (blockDim.x >> 6) + wave;
     const int r0 = 2 * pair, r1 = 2 * pair + 1;
     const bool has0 = r0 < nrows, has1 = r1 < nrows;
     for (int i = lane; i < N; i += 64)
@@ -110,7 +144,7 @@ __global__ void __launch_bounds__(256) k_fft_inv(vh_sht s, const double2* __rest
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, N = s.nphi, L = s.L;
     double2* a = lds + (size_t)wave * 2 * N;
     double2* b = a + N;
-    const int pair = blockIdx.x * 4 + wave;
+    const int pair = blockIdx.x * (blockDim.x >> 6) + wave;
     const int r0 = 2 * pair, r1 = 2 * pair + 1;
     const bool has0 = r0 < nrows, has1 = r1 < nrows;
     for (int i = lane; i < N; i += 64) a[i] = make_double2(0.0, 0.0);
@@ -132,16 +166,16 @@ __global__ void __launch_bounds__(256) k_fft_inv(vh_sht s, const double2* __rest
 
 template <class In>
 void vh_fft_forward_t(hipStream_t st, const vh_sht& s, In in, int nrows, double2* F) {
-    int npairs = (nrows + 1) / 2, nb = (npairs + 3) / 4;
-    size_t lds = (size_t)4 * 2 * s.nphi * sizeof(double2);
-    k_fft_fwd<In><<<nb, 256, lds, st>>>(s, in, nrows, F, 2.0 * VH_PI / s.nphi);
+    int npairs = (nrows + 1) / 2;
+    size_t lds = (size_t)2 * s.nphi * sizeof(double2);
+    k_fft_fwd<In><<<npairs, 64, lds, st>>>(s, in, nrows, F, 2.0 * VH_PI / s.nphi);
     HIPCHECK(hipGetLastError());
 }
 
 template <class Out>
 void vh_fft_inverse_pair_t(hipStream_t st, const vh_sht& s, const double2* F, int nrows, Out out) {
-    int npairs = (nrows + 1) / 2, nb = (npairs + 3) / 4;
-    size_t lds = (size_t)4 * 2 * s.nphi * sizeof(double2);
-    k_fft_inv<Out><<<nb, 256, lds, st>>>(s, F, nrows, out);
+    int npairs = (nrows + 1) / 2;
+    size_t lds = (size_t)2 * s.nphi * sizeof(double2);
+    k_fft_inv<Out><<<npairs, 64, lds, st>>>(s, F, nrows, out);
     HIPCHECK(hipGetLastError());
 }

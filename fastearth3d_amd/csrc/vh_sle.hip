@@ -169,12 +169,19 @@ __global__ void __launch_bounds__(256) k_sle_update(int nphi, const double* __re
     }
 }
 
-__global__ void k_max_final(const double* __restrict__ part, int nlat, int ncomp, double* out) {
-    int c = threadIdx.x;
-    if (c >= ncomp) return;
+// one block per component; max is exact in any order
+__global__ void __launch_bounds__(256) k_max_final(const double* __restrict__ part, int nlat, double* out) {
+    __shared__ double red[256];
+    const int c = blockIdx.x;
     double m = 0.0;
-    for (int j = 0; j < nlat; j++) m = fmax(m, part[(size_t)c * nlat + j]);
-    out[c] = m;
+    for (int j = threadIdx.x; j < nlat; j += blockDim.x) m = fmax(m, part[(size_t)c * nlat + j]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = red[0];
 }
 
 // C_next = O(topo0 - rsl, ice); per-row count of flipped cells
@@ -199,11 +206,18 @@ __global__ void __launch_bounds__(256) k_sle_flip(int nphi, const double* __rest
     if (threadIdx.x == 0) part[j] = red[0];
 }
 
-__global__ void k_sum_final(const double* __restrict__ part, int nlat, double* out) {
-    if (threadIdx.x != 0) return;
-    double s = 0.0;
-    for (int j = 0; j < nlat; j++) s += part[j];
-    out[0] = s;
+// per-row counts are whole numbers: their sum is exact in any order
+__global__ void __launch_bounds__(256) k_sum_final(const double* __restrict__ part, int nlat, double* out) {
+    __shared__ double red[256];
+    double a = 0.0;
+    for (int j = threadIdx.x; j < nlat; j += blockDim.x) a += part[j];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
 }
 
 // z_bed = z_bed_eq - rsl ; ocean function of an arbitrary (topo, ice) pair
@@ -263,12 +277,12 @@ void sle_cs(hipStream_t st, const vh_sht& s, const double* C, const double* f, d
 void sle_update(hipStream_t st, const vh_sht& s, const double* sraw, const double* C, double* rsl, double* sc,
                 int subgrid, double* part) {
     k_sle_update<<<s.nlat, 256, 0, st>>>(s.nphi, sraw, C, rsl, sc, subgrid, part, s.nlat);
-    k_max_final<<<1, 64, 0, st>>>(part, s.nlat, 2, sc + 5);
+    k_max_final<<<2, 256, 0, st>>>(part, s.nlat, sc + 5);
 }
 void sle_flip(hipStream_t st, const vh_sht& s, const double* topo0, const double* rsl, const double* ice,
               const double* C, double* Cnext, double* part, double* out) {
     k_sle_flip<<<s.nlat, 256, 0, st>>>(s.nphi, topo0, rsl, ice, C, Cnext, part);
-    k_sum_final<<<1, 64, 0, st>>>(part, s.nlat, out);
+    k_sum_final<<<1, 256, 0, st>>>(part, s.nlat, out);
 }
 void grid_sub(hipStream_t st, int n, const double* a, const double* b, double* out) {
     k_sub<<<nb256(n), 256, 0, st>>>(n, a, b, out);
