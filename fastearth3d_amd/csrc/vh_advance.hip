@@ -23,6 +23,9 @@ struct advance_args {
     const double* dN;      // [mn][4][nk]
     double* mnorm_out;
     unsigned long long* maxbits;
+    const int* e3of;       // [nem] -> 3-D element index or -1
+    const double2* cdel;   // [i3*3 + shape][lambda][nlm]
+    int nlm;
 };
 
 __global__ void __launch_bounds__(256) k_advance(advance_args a) {
@@ -39,8 +42,19 @@ __global__ void __launch_bounds__(256) k_advance(advance_args a) {
         const double* xV = a.v.xVn + (size_t)(l - 1) * a.nr;
         for (int em = eg; em < a.nem; em += 4) {
             const size_t base = (size_t)em * 24 * nk + k;
-            if (a.v.is3d[em]) {
-                for (int c = 0; c < 24; c++) amax = fmax(amax, fabs(a.tau_out[base + (size_t)c * nk]));
+            if (a.v.is3d[em]) {   // tau_{n+1} = tau_n + 3-D increment (increment form)
+                const double2* cd = a.cdel + (size_t)a.e3of[em] * 12 * a.nlm + k + 1;
+#pragma unroll
+                for (int sh = 0; sh < 3; sh++)
+#pragma unroll
+                    for (int lam = 0; lam < 4; lam++) {
+                        const double2 dv = cd[(size_t)(sh * 4 + lam) * a.nlm];
+                        const size_t o0 = base + (size_t)((sh * 2 + 0) * 4 + lam) * nk;
+                        const size_t o1 = base + (size_t)((sh * 2 + 1) * 4 + lam) * nk;
+                        const double t0 = a.tau_in[o0] + dv.x, t1 = a.tau_in[o1] + dv.y;
+                        a.tau_out[o0] = t0; a.tau_out[o1] = t1;
+                        amax = fmax(amax, fmax(fabs(t0), fabs(t1)));
+                    }
                 continue;
             }
             const int e = a.v.emap[em];
@@ -89,8 +103,9 @@ __global__ void __launch_bounds__(256) k_advance(advance_args a) {
 
 void vh_launch_advance(hipStream_t st, const vh_dims& d, const vh_dev& v, double dt, const double* tau_in,
                        double* tau_out, const double* sigma, const double* dN, double* mnorm_out,
-                       double* maxnorm_out) {
+                       double* maxnorm_out, const int* e3of, const double2* cdel) {
     advance_args a;
+    a.e3of = e3of; a.cdel = cdel; a.nlm = d.nlm;
     a.nr = d.nr; a.nk = d.nk; a.nem = d.nem; a.dt = dt; a.v = v;
     a.tau_in = tau_in; a.tau_out = tau_out; a.sigma = sigma; a.dN = dN;
     a.mnorm_out = mnorm_out; a.maxbits = (unsigned long long*)maxnorm_out;

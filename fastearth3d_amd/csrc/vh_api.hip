@@ -161,7 +161,7 @@ struct vh_model {
     hipStream_t st = nullptr;
     int kind = 2;            // 0 null, 1 elastic, 2 ve
     int deg1_cm = 1;
-    double skip_tol = 1e-4;
+    double skip_tol = 1e-4, visc3d_tol = 1e-3;
     double a = 0, g = 0, dt = 0, time_resp = 0, time_yr = 0, bsl = 0, worst_mass_resid = 0;
     // host geometry
     std::vector<double> r, mu, rho, MkPerDt, Rk, colat, lon;
@@ -181,6 +181,12 @@ struct vh_model {
     vh_drift_plan* dplan = nullptr;
     int *d_emap2 = 0, *d_mnodes2 = 0, *d_mnode2 = 0;   // rotation channel memory
     double* d_G2 = 0;
+    // lateral viscosity: 3-D elements (batch b = 3*i3 + shape), tensor transforms
+    int *d_e3em = 0, *d_e3of = 0;
+    double *d_rate3 = 0, *d_n6 = 0;
+    double2 *d_X = 0, *d_Fd = 0, *d_cdel = 0;
+    int nbat = 0, tchunk = 0;
+    bool tsh_ready = false;
     // device: SHT
     double *d_plm_lj = 0, *d_dplm_lj = 0, *d_plm_jl = 0, *d_dplm_jl = 0, *d_w = 0, *d_st = 0, *d_ct = 0;
     int* d_mstart = 0;
@@ -261,7 +267,7 @@ vh_model::~vh_model() {
                     d_Cn, d_dig, d_C0s, d_ice, d_dice, d_wcorr, d_sraw, d_srot, d_zbed, d_gtmp, d_F, d_F2,
                     d_sigma, d_lm, d_uN, d_part, d_sc, d_sc2, d_cphi, d_sphi,
                     ra.rs.mem, ra.rs.MkPerDt, ra.rs.xU, ra.rs.xV, ra.rs.dN, ra.rs.sc, d_emap2, d_mnodes2,
-                    d_mnode2, d_G2};
+                    d_mnode2, d_G2, d_e3em, d_e3of, d_rate3, d_n6, d_X, d_Fd, d_cdel};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : evpool) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
@@ -290,6 +296,7 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
     else throw vh_error("solid_earth_init: unknown earth_response (use ve|elastic|null)");
     h->deg1_cm = !strcmp(p->deg1_frame, "cm");
     h->skip_tol = p->skip_tol;
+    h->visc3d_tol = p->visc3d_tol;
     HIPCHECK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
     hipStream_t st = h->st;
     HIPCHECK(hipHostMalloc((void**)&h->h_pin, 64 * sizeof(double), hipHostMallocDefault));
@@ -540,6 +547,36 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
     h->sync();
 }
 
+// vilma_advance (src/vilma_response.f90:944-992): the 3-D elements first
+// (advance_memory_3d, :1171-1225, increments into h->d_cdel), then the one
+// streaming pass that applies them and advances every 1-D element.
+void advance_memory(vh_model* h, const double* sigma, int nxt) {
+    hipStream_t st = h->st;
+    const vh_dims& d = h->d;
+    if (h->lat_visc && h->ne3d > 0) {
+        const size_t ng = (size_t)d.nphi * d.nlat;
+        h->prof_begin(VH_K_T3_GATHER);
+        vh_t3_gather(st, d, h->v, h->d_e3em, h->ne3d, h->d_tau[h->cur], sigma, h->d_dN, h->d_X);
+        h->prof_end();
+        for (int b0 = 0; b0 < h->nbat; b0 += h->tchunk) {
+            const int nb = std::min(h->tchunk, h->nbat - b0);
+            h->prof_begin(VH_K_TSH_SYNTH);
+            vh_tsh_synth(st, h->s, h->d_X + (size_t)b0 * 4 * d.nlm, nb, h->d_Fd);
+            h->prof_end();
+            h->prof_begin(VH_K_T3_ROWS);
+            vh_t3_rows(st, h->s, h->d_Fd, nb, 0, h->d_rate3 + (size_t)(b0 / 3) * ng, 3, h->dt, nullptr);
+            h->prof_end();
+            h->prof_begin(VH_K_TSH_ANALYSIS);
+            vh_tsh_analysis(st, h->s, h->d_Fd, nb, h->d_n6, h->d_cdel + (size_t)b0 * 4 * d.nlm);
+            h->prof_end();
+        }
+    }
+    h->prof_begin(VH_K_ADVANCE);
+    vh_launch_advance(st, d, h->v, h->dt, h->d_tau[h->cur], h->d_tau[nxt], sigma, h->d_dN, h->d_mnorm[nxt],
+                      h->d_maxn + nxt, h->d_e3of, h->d_cdel);
+    h->prof_end();
+}
+
 // ---------------------------------------------------------------- SLE solve
 struct sle_res { int n_outer_done = 0, n_inner_last = 0, n_coast_flip = 0; double mass_resid = 0, esl = 0; };
 
@@ -599,12 +636,7 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
     // closing load -> sigma, then the memory advance
     sle_load_analysis(st, s, h->d_dig, h->d_C, h->d_rsl, h->d_wcorr, h->d_F, h->d_sigma);
     int nxt = h->cur ^ 1;
-    if (!report_only && h->kind == 2) {
-        h->prof_begin(VH_K_ADVANCE);
-        vh_launch_advance(st, h->d, h->v, h->dt, h->d_tau[h->cur], h->d_tau[nxt], (const double*)h->d_sigma, h->d_dN,
-                          h->d_mnorm[nxt], h->d_maxn + nxt);
-        h->prof_end();
-    }
+    if (!report_only && h->kind == 2) advance_memory(h, (const double*)h->d_sigma, nxt);
     if (!report_only) h->time_resp += h->dt;
     // mass diagnostic (+ the new memory norm, in the same read-back)
     sle_cs(st, s, h->d_C, h->d_rsl, h->d_part, h->d_sc + 8);
@@ -920,15 +952,137 @@ const char* vh_last_error(const vh_model* h) {
     return h ? h->err.c_str() : g_create_error.c_str();
 }
 
-int vh_enable_visc_3d(vh_model* h, const double* pert_elem) {
-    return guarded(h, [&] { (void)pert_elem; throw vh_error("lateral viscosity: not available in this build"); });
+// tensor_sh_init (src/vilma_tensor_sh.f90:76-133): the n6 calibration
+void tensor_init(vh_model* h) {
+    if (h->tsh_ready) return;
+    h->d_n6 = dalloc<double>(h->d.L + 1);
+    vh_tsh_calibrate_n6(h->st, h->s, h->d_n6);
+    h->tsh_ready = true;
 }
 
+// response_enable_lateral_visc (src/vilma_response.f90:994-1053); pert is
+// [e][nlat][nphi] (phi fastest), toroidal channels not carried.
+void enable_lateral(vh_model* h, const double* pert) {
+    if (h->kind != 2) throw vh_error("solid_earth_enable_visc_3d: lateral viscosity needs earth_response=ve");
+    const vh_dims& d = h->d;
+    const int ne = d.ne;
+    const size_t ng = (size_t)d.nphi * d.nlat;
+    tensor_init(h);
+    std::vector<double> rate3(ng * ne);
+    double r3max = 0.0;
+    for (int e = 0; e < ne; e++)
+        for (size_t i = 0; i < ng; i++) {
+            double v = h->MkPerDt[e] * std::pow(10.0, -pert[(size_t)e * ng + i]);
+            rate3[(size_t)e * ng + i] = v;
+            r3max = std::max(r3max, v);
+        }
+    std::vector<int> e3;
+    for (int e = 0; e < ne; e++) {
+        if (h->MkPerDt[e] == 0.0) continue;
+        double mx = -INFINITY, mn = INFINITY;
+        for (size_t i = 0; i < ng; i++) { double v = pert[(size_t)e * ng + i]; mx = std::max(mx, v); mn = std::min(mn, v); }
+        if (mx - mn > h->visc3d_tol) {
+            e3.push_back(e);
+        } else {
+            double sum = 0.0;   // phi fastest, then latitude
+            for (size_t i = 0; i < ng; i++) sum += pert[(size_t)e * ng + i];
+            h->MkPerDt[e] = h->MkPerDt[e] * std::pow(10.0, -(sum / (double)ng));
+        }
+    }
+    // Maxwell-element index of each 3-D element; per-element flags
+    std::vector<int> e3em(e3.size()), e3of(std::max(d.nem, 1), -1);
+    std::vector<unsigned char> is3d(std::max(d.nem, 1), 0);
+    for (size_t i = 0; i < e3.size(); i++) {
+        int em = (int)(std::find(h->emap.begin(), h->emap.end(), e3[i]) - h->emap.begin());
+        e3em[i] = em; e3of[em] = (int)i; is3d[em] = 1;
+    }
+    hipStream_t st = h->st;
+    for (void* p : {(void*)h->d_e3em, (void*)h->d_e3of, (void*)h->d_rate3, (void*)h->d_X, (void*)h->d_Fd,
+                    (void*)h->d_cdel})
+        if (p) HIPCHECK(hipFree(p));
+    h->ne3d = (int)e3.size();
+    h->nbat = 3 * h->ne3d;
+    h->tchunk = std::min(h->nbat, 48);
+    h->d_e3em = dalloc<int>(e3em.size()); upload(h->d_e3em, e3em.data(), e3em.size(), st);
+    h->d_e3of = dalloc<int>(e3of.size()); upload(h->d_e3of, e3of.data(), e3of.size(), st);
+    upload(h->d_is3d, is3d.data(), (size_t)d.nem, st);
+    h->d_rate3 = dalloc<double>(ng * h->ne3d);
+    for (int i = 0; i < h->ne3d; i++) upload(h->d_rate3 + (size_t)i * ng, rate3.data() + (size_t)e3[i] * ng, ng, st);
+    h->d_X = dalloc<double2>((size_t)h->nbat * 4 * d.nlm);
+    h->d_cdel = dalloc<double2>((size_t)h->nbat * 4 * d.nlm);
+    h->d_Fd = dalloc<double2>((size_t)std::max(h->tchunk, 1) * 6 * (d.L + 1) * d.nlat);
+    upload(h->d_MkPerDt, h->MkPerDt.data(), (size_t)ne, st);
+    h->rate_max = r3max;
+    for (int e = 0; e < ne; e++) h->rate_max = std::max(h->rate_max, h->MkPerDt[e]);
+    h->lat_visc = true;
+    h->sync();
+}
+
+int vh_enable_visc_3d(vh_model* h, const double* pert_elem) {
+    return guarded(h, [&] { enable_lateral(h, pert_elem); });
+}
+
+// response_enable_lateral_visc_from_nodes (src/vilma_response.f90:1120-1169):
+// element value = mean of its two nodes' log10 viscosity, capped in the lid.
 int vh_enable_visc_3d_nodes(vh_model* h, const double* visc_node, double lid_depth, double lid_log10max,
                             double log10_cap) {
     return guarded(h, [&] {
-        (void)visc_node; (void)lid_depth; (void)lid_log10max; (void)log10_cap;
-        throw vh_error("lateral viscosity: not available in this build");
+        if (h->kind != 2) throw vh_error("solid_earth_enable_visc_3d: lateral viscosity needs earth_response=ve");
+        const int ne = h->d.ne, nr = h->d.nr;
+        const size_t ng = (size_t)h->d.nphi * h->d.nlat;
+        double r_lid = DBL_MAX;
+        if (lid_depth > 0.0) r_lid = h->r[nr - 1] - lid_depth;
+        std::vector<double> pert(ng * ne, 0.0);
+        for (int e = 0; e < ne; e++) {
+            if (h->MkPerDt[e] == 0.0) continue;
+            double logeta_ref = std::log10(h->mu[e] / h->MkPerDt[e]);
+            bool in_lid = h->r[e] >= r_lid - 1.0e-6 * h->r[nr - 1];
+            for (size_t sp = 0; sp < ng; sp++) {
+                double ea = 0.5 * (visc_node[sp + (size_t)e * ng] + visc_node[sp + (size_t)(e + 1) * ng]);
+                if (in_lid && ea > lid_log10max) ea = log10_cap;
+                pert[(size_t)e * ng + sp] = ea - logeta_ref;
+            }
+        }
+        enable_lateral(h, pert.data());
+    });
+}
+
+// tensor_sh_synth / tensor_sh_analysis (src/vilma_tensor_sh.f90:168-322):
+// c is [4][nlm] complex in SHTns LM order (lambda = 1, 2, 5, 6), dyad is
+// [6][nlat][nphi] (rr, rt, rp, tt, tp, pp).
+int vh_tensor_synthesis(vh_model* h, const double* c, double* dyad) {
+    return guarded(h, [&] {
+        tensor_init(h);
+        const vh_dims& d = h->d;
+        const size_t ng = (size_t)d.nphi * d.nlat;
+        std::vector<double2> X((size_t)4 * d.nlm);
+        for (int lam = 0; lam < 4; lam++) from_lm(h, c + (size_t)2 * lam * d.nlm, X.data() + (size_t)lam * d.nlm);
+        double2 *dX = dalloc<double2>(X.size()), *dF = dalloc<double2>((size_t)6 * (d.L + 1) * d.nlat);
+        double* dg = dalloc<double>(6 * ng);
+        upload(dX, X.data(), X.size(), h->st);
+        vh_tsh_synth(h->st, h->s, dX, 1, dF);
+        vh_t3_rows(h->st, h->s, dF, 1, 1, nullptr, 1, 0.0, dg);
+        HIPCHECK(hipMemcpyAsync(dyad, dg, 6 * ng * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        h->sync();
+        (void)hipFree(dX); (void)hipFree(dF); (void)hipFree(dg);
+    });
+}
+
+int vh_tensor_analysis(vh_model* h, const double* dyad, double* c) {
+    return guarded(h, [&] {
+        tensor_init(h);
+        const vh_dims& d = h->d;
+        const size_t ng = (size_t)d.nphi * d.nlat;
+        double2 *dC = dalloc<double2>((size_t)4 * d.nlm), *dF = dalloc<double2>((size_t)6 * (d.L + 1) * d.nlat);
+        double* dg = dalloc<double>(6 * ng);
+        upload(dg, dyad, 6 * ng, h->st);
+        vh_t3_rows(h->st, h->s, dF, 1, 2, nullptr, 1, 0.0, dg);
+        vh_tsh_analysis(h->st, h->s, dF, 1, h->d_n6, dC);
+        std::vector<double2> C((size_t)4 * d.nlm);
+        HIPCHECK(hipMemcpyAsync(C.data(), dC, C.size() * sizeof(double2), hipMemcpyDeviceToHost, h->st));
+        h->sync();
+        for (int lam = 0; lam < 4; lam++) to_lm(h, C.data() + (size_t)lam * d.nlm, c + (size_t)2 * lam * d.nlm);
+        (void)hipFree(dC); (void)hipFree(dF); (void)hipFree(dg);
     });
 }
 
@@ -962,8 +1116,7 @@ int vh_resp_advance(vh_model* h, const double* sigma_lm) {
         from_lm(h, sigma_lm, sg.data());
         upload(h->d_lm, sg.data(), h->d.nlm, h->st);
         int nxt = h->cur ^ 1;
-        vh_launch_advance(h->st, h->d, h->v, h->dt, h->d_tau[h->cur], h->d_tau[nxt], (const double*)h->d_lm, h->d_dN,
-                          h->d_mnorm[nxt], h->d_maxn + nxt);
+        advance_memory(h, (const double*)h->d_lm, nxt);
         h->read(h->d_maxn + nxt, 1);
         h->maxn_cur = h->h_pin[0];
         h->cur = nxt;

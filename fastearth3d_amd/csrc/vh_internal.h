@@ -86,9 +86,11 @@ void vh_launch_setup_deg1_op(hipStream_t st, const vh_dims& d, const int* mnodes
                              int deg1_cm, double* work, double* G1);
 
 // ---------------------------------------------------------------- (b) advance
+// 3-D elements (v.is3d) take tau_in + their increment from cdel
+// ([i3*3 + shape][lambda][nlm] complex, e3of[em] = i3) instead of the 1-D update.
 void vh_launch_advance(hipStream_t st, const vh_dims& d, const vh_dev& v, double dt, const double* tau_in,
                        double* tau_out, const double* sigma, const double* dN, double* mnorm_out,
-                       double* maxnorm_out);
+                       double* maxnorm_out, const int* e3of, const double2* cdel);
 void vh_launch_memnorm(hipStream_t st, const vh_dims& d, const double* tau, double* mnorm, double* maxnorm);
 
 // ---------------------------------------------------------------- SHT
@@ -126,6 +128,19 @@ void vh_surface_integral(hipStream_t st, const vh_sht& s, const double* f, doubl
 // out[c] = sum_j w_j * dlon * part[c*nlat + j], c < ncomp (ncomp <= 64)
 void vh_weighted_total(hipStream_t st, const vh_sht& s, const double* part, int ncomp, double* out);
 void vh_fill(hipStream_t st, double* p, double v, size_t n);
+
+// ---------------------------------------------------------------- tensor (3-D)
+// X [nb][4][nlm] tensor coefficients -> Fd [nb][6][L+1][nlat] dyad row spectra
+void vh_tsh_synth(hipStream_t st, const vh_sht& s, const double2* X, int nb, double2* Fd);
+// Fd -> C [nb][4][nlm]; n6 == null leaves the raw spin-2 adjoint in channel 3
+void vh_tsh_analysis(hipStream_t st, const vh_sht& s, const double2* Fd, int nb, const double* n6, double2* C);
+// mode 0: Fd <- FFT(-rate*dt * IFFT(Fd)), rate [nb/bpe][nlat][nphi];
+// mode 1: grid [nb][6][nlat][nphi] <- IFFT(Fd); mode 2: Fd <- FFT(grid)
+void vh_t3_rows(hipStream_t st, const vh_sht& s, double2* Fd, int nb, int mode, const double* rate, int bpe,
+                double dt, double* grid);
+void vh_t3_gather(hipStream_t st, const vh_dims& d, const vh_dev& v, const int* e3em, int ne3d, const double* tau,
+                  const double* sigma, const double* dN, double2* X);
+void vh_tsh_calibrate_n6(hipStream_t st, const vh_sht& s, double* n6_dev);
 
 // ---------------------------------------------------------------- drift plan
 struct vh_drift_plan;

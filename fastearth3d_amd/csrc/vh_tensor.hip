@@ -1,2 +1,381 @@
-// vh_tensor.hip -- placeholder until the lateral-viscosity path lands.
+// vh_tensor.hip -- (c) the lateral-viscosity memory update of the genuinely
+// 3-D elements: for each such element e and radial shape s in {A, B, C}
+//
+//     c_s <- c_s + analysis( -Mk3(:,:,e) * synth(c_s + 2 mu_e eps_s) )
+//
+// (increment form, src/vilma_response.f90:1171-1225, :1227-1319), where the
+// tensor transforms map the four spheroidal channels lambda = 1, 2, 5, 6 to
+// the six dyads rr, rt, rp, tt, tp, pp (src/vilma_tensor_sh.f90:168-322).
+//
+// The reference composes each tensor transform from 3 + 3 (synthesis) and
+// 3 + 5 (analysis) scalar/vector SHTs plus grid-space products with cot and
+// 1/sin. All of those factors depend on latitude only and commute with the
+// longitude FFT, so here a tensor transform is one Legendre pass in (m, j)
+// space producing/consuming all six dyads at once, from the P and dP tables:
+//
+//   rr = sum c1 P            rt = sum c2 dP          rp = (i m/s) sum c2 P
+//   tr = sum (-J c5) P       J = l(l+1)
+//   Sg = sum c6 [-J P - 2 cot dP + 2 m^2/s^2 P]      Sh = (i m/s) sum c6 (dP - cot P)
+//   tt = tr + Sg,  pp = tr - Sg,  tp = 4 Sh
+//
+// and the analysis is its quadrature adjoint with the reference's
+// normalisations (c2 / J, c5 / (-2J), c6 / n6(l), n6 calibrated at m = l as
+// in src/vilma_tensor_sh.f90:99-133). The pointwise product with the grid
+// field -Mk3 needs real space: one row kernel per (batch, latitude) does
+// inverse FFT -> multiply -> forward FFT for the six dyads in LDS, so the
+// grid never reaches HBM.
+//
+// Layouts: coefficients X[b][lam][idx] (complex, degree-major idx =
+// l(l+1)/2 + m); dyad row spectra Fd[b][dy][m][j] (complex, latitude
+// contiguous, so the Legendre kernels read and write coalesced rows and the
+// row kernel's column access is shared through L2 by an XCD-aware block
+// order); batch b = 3*i3 + s.
+#include <vector>
 #include "vh_internal.h"
+#include "vh_radial_dev.h"
+#include "vh_sht_dev.h"
+
+// Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous
+// range of logical block ids so that concurrently resident blocks on one XCD
+// work on neighbouring rows (shared L2 lines). Grid padded to a multiple of 8.
+__device__ __forceinline__ int xcd_block(int nblk) {
+    const int cnt = (nblk + 7) / 8;
+    return (int)(blockIdx.x % 8) * cnt + (int)(blockIdx.x / 8);
+}
+static inline unsigned xcd_grid(int nblk) { return (unsigned)(((nblk + 7) / 8) * 8); }
+
+// ---------------------------------------------------------- synthesis
+// Block = (batch b, order m); thread = north latitude j.
+__global__ void __launch_bounds__(128) k_tsh_synth(vh_sht s, const double2* __restrict__ X, int nb,
+                                                   double2* __restrict__ Fd) {
+    extern __shared__ double2 cs[];   // [5][nl]: c1, c2, -J c5, c6, -J c6
+    const int L = s.L, nh = s.nh, nlat = s.nlat, nlm = s.nlm;
+    const int lb = xcd_block(nb * (L + 1));
+    if (lb >= nb * (L + 1)) return;
+    const int b = lb / (L + 1), m = lb % (L + 1), nl = L - m + 1;
+    for (int t = threadIdx.x; t < 4 * nl; t += blockDim.x) {
+        const int lam = t / nl, il = t % nl, l = m + il;
+        double2 v = X[((size_t)b * 4 + lam) * nlm + l * (l + 1) / 2 + m];
+        const double J = (double)l * (double)(l + 1);
+        if (lam == 2) v = make_double2(-J * v.x, -J * v.y);
+        cs[lam * nl + il] = v;
+        if (lam == 3) cs[4 * nl + il] = make_double2(-J * v.x, -J * v.y);
+    }
+    __syncthreads();
+    const int j = threadIdx.x;
+    if (j >= nh) return;
+    const double* P = s.plm_lj + (size_t)s.mstart[m] * nh;
+    const double* D = s.dplm_lj + (size_t)s.mstart[m] * nh;
+    // [parity][sum]: A1, A2, D2, A5, A6, L6, D6
+    double2 acc[2][7];
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int q = 0; q < 7; q++) acc[p][q] = make_double2(0.0, 0.0);
+    auto step = [&](int il, double2 (&a)[7]) {
+        const double pv = P[(size_t)il * nh + j], dv = D[(size_t)il * nh + j];
+        const double2 c1 = cs[il], c2 = cs[nl + il], c5 = cs[2 * nl + il], c6 = cs[3 * nl + il], l6 = cs[4 * nl + il];
+        a[0].x = fma(c1.x, pv, a[0].x); a[0].y = fma(c1.y, pv, a[0].y);
+        a[1].x = fma(c2.x, pv, a[1].x); a[1].y = fma(c2.y, pv, a[1].y);
+        a[2].x = fma(c2.x, dv, a[2].x); a[2].y = fma(c2.y, dv, a[2].y);
+        a[3].x = fma(c5.x, pv, a[3].x); a[3].y = fma(c5.y, pv, a[3].y);
+        a[4].x = fma(c6.x, pv, a[4].x); a[4].y = fma(c6.y, pv, a[4].y);
+        a[5].x = fma(l6.x, pv, a[5].x); a[5].y = fma(l6.y, pv, a[5].y);
+        a[6].x = fma(c6.x, dv, a[6].x); a[6].y = fma(c6.y, dv, a[6].y);
+    };
+    int il = 0;
+    for (; il + 1 < nl; il += 2) { step(il, acc[0]); step(il + 1, acc[1]); }
+    if (il < nl) step(il, acc[0]);
+    const double ms = (double)m;
+#pragma unroll
+    for (int hemi = 0; hemi < 2; hemi++) {
+        const int row = hemi == 0 ? j : nlat - 1 - j;
+        if (hemi == 1 && row == j) break;   // equator row of an odd grid
+        // full sums at this row: P-type E +- O, dP-type +-(E - O) / (E + O)
+        const double sp = hemi == 0 ? 1.0 : -1.0;
+        double2 S[7];
+#pragma unroll
+        for (int q = 0; q < 7; q++) {
+            const bool dtype = (q == 2 || q == 6);
+            double2 e = acc[0][q], o = acc[1][q];
+            if (!dtype) S[q] = make_double2(e.x + sp * o.x, e.y + sp * o.y);
+            else S[q] = hemi == 0 ? make_double2(e.x + o.x, e.y + o.y) : make_double2(o.x - e.x, o.y - e.y);
+        }
+        const double st = s.st[row], cot = s.ct[row] / st, is = 1.0 / st;
+        const double mis = ms * is;
+        double2 out[6];
+        out[0] = S[0];                                             // rr
+        out[1] = S[2];                                             // rt
+        out[2] = make_double2(-mis * S[1].y, mis * S[1].x);        // rp = i (m/s) A2
+        const double2 tr = S[3];
+        const double g2 = 2.0 * mis * mis;
+        const double2 Sg = make_double2(S[5].x - 2.0 * cot * S[6].x + g2 * S[4].x,
+                                        S[5].y - 2.0 * cot * S[6].y + g2 * S[4].y);
+        const double2 hd = make_double2(S[6].x - cot * S[4].x, S[6].y - cot * S[4].y);
+        const double2 Sh = make_double2(-mis * hd.y, mis * hd.x); // i (m/s) (D6 - cot A6)
+        out[3] = make_double2(tr.x + Sg.x, tr.y + Sg.y);           // tt
+        out[4] = make_double2(4.0 * Sh.x, 4.0 * Sh.y);             // tp
+        out[5] = make_double2(tr.x - Sg.x, tr.y - Sg.y);           // pp
+#pragma unroll
+        for (int dy = 0; dy < 6; dy++) Fd[(((size_t)b * 6 + dy) * (L + 1) + m) * nlat + row] = out[dy];
+    }
+}
+
+// ----------------------------------------------------------- analysis
+// Block = (batch b, order m). Stage 1 folds the six dyads of the latitude
+// pair (j, nlat-1-j) into the seven weighted partial fields (even/odd l+m);
+// stage 2: thread = degree, quadrature over the north latitudes. n6 == null
+// returns the raw spin-2 adjoint (calibration).
+__global__ void __launch_bounds__(128) k_tsh_analysis(vh_sht s, const double2* __restrict__ Fd, int nb,
+                                                      const double* __restrict__ n6, double2* __restrict__ C) {
+    extern __shared__ double2 fs[];   // [7][2][nh]: f1, f2P, f5, fD, fb (P-type), f2D, fd (dP-type)
+    const int L = s.L, nh = s.nh, nlat = s.nlat, nlm = s.nlm;
+    const int lb = xcd_block(nb * (L + 1));
+    if (lb >= nb * (L + 1)) return;
+    const int b = lb / (L + 1), m = lb % (L + 1), nl = L - m + 1;
+    const double ms = (double)m;
+    for (int j = threadIdx.x; j < nh; j += blockDim.x) {
+        double2 f[2][7];
+#pragma unroll
+        for (int hemi = 0; hemi < 2; hemi++) {
+            const int row = hemi == 0 ? j : nlat - 1 - j;
+            if (hemi == 1 && row == j) {
+#pragma unroll
+                for (int q = 0; q < 7; q++) f[1][q] = make_double2(0.0, 0.0);
+                break;
+            }
+            double2 d[6];
+#pragma unroll
+            for (int dy = 0; dy < 6; dy++) d[dy] = Fd[(((size_t)b * 6 + dy) * (L + 1) + m) * nlat + row];
+            const double w = s.w[row], st = s.st[row], cot = s.ct[row] / st, is = 1.0 / st;
+            const double mis = ms * is;
+            const double2 Dv = make_double2(d[3].x - d[5].x, d[3].y - d[5].y);     // tt - pp
+            const double2 tp = d[4];
+            f[hemi][0] = make_double2(w * d[0].x, w * d[0].y);                      // rr
+            f[hemi][1] = make_double2(w * (mis * d[2].y), w * (-mis * d[2].x));      // (-i m/s) rp
+            f[hemi][2] = make_double2(w * (d[3].x + d[5].x), w * (d[3].y + d[5].y)); // tt + pp
+            f[hemi][3] = make_double2(w * Dv.x, w * Dv.y);
+            // 2 m^2/s^2 D + 2 i m cot/s tp
+            const double g2 = 2.0 * mis * mis, hc = 2.0 * mis * cot;
+            f[hemi][4] = make_double2(w * (g2 * Dv.x - hc * tp.y), w * (g2 * Dv.y + hc * tp.x));
+            f[hemi][5] = make_double2(w * d[1].x, w * d[1].y);                       // rt
+            // -2 cot D - 2 i m/s tp
+            f[hemi][6] = make_double2(w * (-2.0 * cot * Dv.x + 2.0 * mis * tp.y),
+                                      w * (-2.0 * cot * Dv.y - 2.0 * mis * tp.x));
+        }
+#pragma unroll
+        for (int q = 0; q < 7; q++) {
+            const bool dtype = q >= 5;
+            const double2 a = f[0][q], c = f[1][q];
+            const double2 plus = make_double2(a.x + c.x, a.y + c.y), minus = make_double2(a.x - c.x, a.y - c.y);
+            fs[(q * 2 + 0) * nh + j] = dtype ? minus : plus;    // even l+m
+            fs[(q * 2 + 1) * nh + j] = dtype ? plus : minus;    // odd l+m
+        }
+    }
+    __syncthreads();
+    const double* P = s.plm_jl + (size_t)s.mstart[m] * nh;
+    const double* D = s.dplm_jl + (size_t)s.mstart[m] * nh;
+    for (int il = threadIdx.x; il < nl; il += blockDim.x) {
+        const int l = m + il, p = il & 1;
+        const double2* f1 = fs + (0 * 2 + p) * nh;
+        const double2* f2P = fs + (1 * 2 + p) * nh;
+        const double2* f5 = fs + (2 * 2 + p) * nh;
+        const double2* fD = fs + (3 * 2 + p) * nh;
+        const double2* fb = fs + (4 * 2 + p) * nh;
+        const double2* f2D = fs + (5 * 2 + p) * nh;
+        const double2* fd = fs + (6 * 2 + p) * nh;
+        double2 s1 = make_double2(0, 0), s2 = s1, s5 = s1, sD = s1, sb = s1, sd = s1;
+        for (int j = 0; j < nh; j++) {
+            const double pv = P[(size_t)j * nl + il], dv = D[(size_t)j * nl + il];
+            s1.x = fma(pv, f1[j].x, s1.x); s1.y = fma(pv, f1[j].y, s1.y);
+            s2.x = fma(dv, f2D[j].x, fma(pv, f2P[j].x, s2.x)); s2.y = fma(dv, f2D[j].y, fma(pv, f2P[j].y, s2.y));
+            s5.x = fma(pv, f5[j].x, s5.x); s5.y = fma(pv, f5[j].y, s5.y);
+            sD.x = fma(pv, fD[j].x, sD.x); sD.y = fma(pv, fD[j].y, sD.y);
+            sb.x = fma(pv, fb[j].x, sb.x); sb.y = fma(pv, fb[j].y, sb.y);
+            sd.x = fma(dv, fd[j].x, sd.x); sd.y = fma(dv, fd[j].y, sd.y);
+        }
+        const double J = (double)l * (double)(l + 1);
+        const size_t idx = (size_t)l * (l + 1) / 2 + m;
+        double2* Cb = C + (size_t)b * 4 * nlm;
+        Cb[idx] = s1;
+        Cb[nlm + idx] = J > 0.0 ? make_double2(s2.x / J, s2.y / J) : make_double2(0.0, 0.0);
+        Cb[2 * nlm + idx] = J > 0.0 ? make_double2(s5.x / (-2.0 * J), s5.y / (-2.0 * J)) : make_double2(0.0, 0.0);
+        const double2 raw = make_double2(-J * sD.x + sb.x + sd.x, -J * sD.y + sb.y + sd.y);
+        double2 c6 = raw;
+        if (n6) {
+            const double n = n6[l];
+            c6 = n != 0.0 ? make_double2(raw.x / n, raw.y / n) : make_double2(0.0, 0.0);
+        }
+        Cb[3 * nlm + idx] = c6;
+    }
+}
+
+// ------------------------------------------------------------- row pass
+// Block = (batch b, latitude row j), three wavefronts, one per dyad pair
+// (rr, rt), (rp, tt), (tp, pp) packed as one complex FFT of two real rows.
+//   mode 0: inverse FFT, multiply by -rate*dt (rate row of the batch's
+//           element), forward FFT, in place in Fd
+//   mode 1: inverse FFT to grid[b][dy][j][i]
+//   mode 2: forward FFT of grid[b][dy][j][i] into Fd
+__global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, double2* __restrict__ Fd, int nb, int mode,
+                                                 const double* __restrict__ rate, int bpe, double dt,
+                                                 double* __restrict__ grid) {
+    extern __shared__ double2 lds[];
+    const int L = s.L, N = s.nphi, nlat = s.nlat;
+    const int lb = xcd_block(nb * nlat);
+    if (lb >= nb * nlat) return;
+    const int b = lb / nlat, j = lb % nlat;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double2* a = lds + (size_t)wave * 2 * N;
+    double2* c = a + N;
+    double* Mrow = (double*)(lds + (size_t)3 * 2 * N);
+    const int d0 = 2 * wave, d1 = 2 * wave + 1;
+    double2* F0 = Fd + ((size_t)b * 6 + d0) * (L + 1) * nlat + j;
+    double2* F1 = Fd + ((size_t)b * 6 + d1) * (L + 1) * nlat + j;
+    if (mode == 0) {
+        const double* rr = rate + ((size_t)(b / bpe) * nlat + j) * N;
+        for (int i = threadIdx.x; i < N; i += blockDim.x) Mrow[i] = rr[i] * dt;
+    }
+    double2* z;
+    if (mode != 2) {
+        for (int i = lane; i < N; i += 64) a[i] = make_double2(0.0, 0.0);
+        __syncthreads();
+        for (int m = lane; m <= L; m += 64) {
+            double2 A = F0[(size_t)m * nlat], B = F1[(size_t)m * nlat];
+            if (m == 0) { A.y = 0.0; B.y = 0.0; }
+            a[m] = make_double2(A.x - B.y, A.y + B.x);
+            if (m > 0) a[N - m] = make_double2(A.x + B.y, -A.y + B.x);
+        }
+        __syncthreads();
+        z = fft_wave(a, c, N, s.fac, s.nfac, s.tw, true, lane);
+        if (mode == 1) {
+            double* g0 = grid + (((size_t)b * 6 + d0) * nlat + j) * N;
+            double* g1 = grid + (((size_t)b * 6 + d1) * nlat + j) * N;
+            for (int i = lane; i < N; i += 64) { g0[i] = z[i].x; g1[i] = z[i].y; }
+            return;
+        }
+        for (int i = lane; i < N; i += 64) {
+            const double M = Mrow[i];
+            z[i] = make_double2(-M * z[i].x, -M * z[i].y);
+        }
+        __syncthreads();
+    } else {
+        const double* g0 = grid + (((size_t)b * 6 + d0) * nlat + j) * N;
+        const double* g1 = grid + (((size_t)b * 6 + d1) * nlat + j) * N;
+        for (int i = lane; i < N; i += 64) a[i] = make_double2(g0[i], g1[i]);
+        __syncthreads();
+        z = a;
+    }
+    double2* other = (z == a) ? c : a;
+    double2* y = fft_wave(z, other, N, s.fac, s.nfac, s.tw, false, lane);
+    const double scale = 2.0 * VH_PI / N;
+    for (int m = lane; m <= L; m += 64) {
+        double2 zm = y[m], zc = y[(N - m) % N];
+        zc.y = -zc.y;
+        const double2 A = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+        const double2 Dd = csub(zm, zc);
+        const double2 B = make_double2(0.5 * Dd.y, -0.5 * Dd.x);
+        F0[(size_t)m * nlat] = make_double2(scale * A.x, scale * A.y);
+        F1[(size_t)m * nlat] = make_double2(scale * B.x, scale * B.y);
+    }
+}
+
+// ------------------------------------------------------------- gather
+// X[(i3*3 + s)*4 + lam][k+1] = tau_s + 2 mu eps_s of slot k, element e3em[i3].
+__global__ void __launch_bounds__(256) k_t3_gather(int nr, int nk, int nlm, vh_dev v, const int* __restrict__ e3em,
+        const double* __restrict__ tau, const double* __restrict__ sigma, const double* __restrict__ dN,
+        double2* __restrict__ X) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, i3 = blockIdx.y;
+    double2* Xb = X + (size_t)i3 * 12 * nlm;
+    if (k == 0)
+        for (int q = 0; q < 12; q++) Xb[(size_t)q * nlm] = make_double2(0.0, 0.0);
+    if (k >= nk) return;
+    const int em = e3em[i3], e = v.emap[em];
+    const int l = v.kdeg[k];
+    const double Jr = (double)l * (double)(l + 1);
+    const double sre = sigma[2 * (k + 1)], sim = sigma[2 * (k + 1) + 1];
+    const double* xU = v.xUn + (size_t)(l - 1) * nr;
+    const double* xV = v.xVn + (size_t)(l - 1) * nr;
+    const int n0 = v.mnode[e], n1 = v.mnode[e + 1];
+    const double* q0 = dN + (size_t)n0 * 4 * nk + k;
+    const double* q1 = dN + (size_t)n1 * 4 * nk + k;
+    double U0r = sre * xU[e] + q0[0], U0i = sim * xU[e] + q0[(size_t)nk];
+    double V0r = sre * xV[e] + q0[(size_t)2 * nk], V0i = sim * xV[e] + q0[(size_t)3 * nk];
+    double U1r = sre * xU[e + 1] + q1[0], U1i = sim * xU[e + 1] + q1[(size_t)nk];
+    double V1r = sre * xV[e + 1] + q1[(size_t)2 * nk], V1i = sim * xV[e + 1] + q1[(size_t)3 * nk];
+    double sc[2][3][4];
+    strain_coeffs(U0r, U1r, V0r, V1r, Jr, sc[0][0], sc[0][1], sc[0][2]);
+    strain_coeffs(U0i, U1i, V0i, V1i, Jr, sc[1][0], sc[1][1], sc[1][2]);
+    const double twoMu = 2.0 * v.mu[e];
+    const double* t = tau + (size_t)em * 24 * nk + k;
+#pragma unroll
+    for (int sh = 0; sh < 3; sh++)
+#pragma unroll
+        for (int lam = 0; lam < 4; lam++) {
+            double re = t[(size_t)((sh * 2 + 0) * 4 + lam) * nk] + twoMu * sc[0][sh][lam];
+            double im = t[(size_t)((sh * 2 + 1) * 4 + lam) * nk] + twoMu * sc[1][sh][lam];
+            Xb[(size_t)(sh * 4 + lam) * nlm + k + 1] = make_double2(re, im);
+        }
+}
+
+// ------------------------------------------------------------- host side
+static size_t synth_lds(const vh_sht& s) { return (size_t)5 * (s.L + 1) * sizeof(double2); }
+static size_t anal_lds(const vh_sht& s) { return (size_t)14 * s.nh * sizeof(double2); }
+static size_t rows_lds(const vh_sht& s) { return (size_t)6 * s.nphi * sizeof(double2) + (size_t)s.nphi * sizeof(double); }
+
+void vh_tsh_synth(hipStream_t st, const vh_sht& s, const double2* X, int nb, double2* Fd) {
+    if (nb <= 0) return;
+    if (s.nh > 128) throw vh_error("tensor transform: nlat above 256 is not supported");
+    k_tsh_synth<<<xcd_grid(nb * (s.L + 1)), 128, synth_lds(s), st>>>(s, X, nb, Fd);
+    HIPCHECK(hipGetLastError());
+}
+
+void vh_tsh_analysis(hipStream_t st, const vh_sht& s, const double2* Fd, int nb, const double* n6, double2* C) {
+    if (nb <= 0) return;
+    k_tsh_analysis<<<xcd_grid(nb * (s.L + 1)), 128, anal_lds(s), st>>>(s, Fd, nb, n6, C);
+    HIPCHECK(hipGetLastError());
+}
+
+void vh_t3_rows(hipStream_t st, const vh_sht& s, double2* Fd, int nb, int mode, const double* rate, int bpe,
+                double dt, double* grid) {
+    if (nb <= 0) return;
+    k_t3_rows<<<xcd_grid(nb * s.nlat), 192, rows_lds(s), st>>>(s, Fd, nb, mode, rate, bpe, dt, grid);
+    HIPCHECK(hipGetLastError());
+}
+
+void vh_t3_gather(hipStream_t st, const vh_dims& d, const vh_dev& v, const int* e3em, int ne3d, const double* tau,
+                  const double* sigma, const double* dN, double2* X) {
+    if (ne3d <= 0) return;
+    dim3 g((d.nk + 255) / 256, ne3d);
+    k_t3_gather<<<g, 256, 0, st>>>(d.nr, d.nk, d.nlm, v, e3em, tau, sigma, dN, X);
+    HIPCHECK(hipGetLastError());
+}
+
+// n6(l), l = 2..L: the raw spin-2 adjoint of the unit sectoral (l, m = l)
+// spin-2 synthesis (tt = Sg, pp = -Sg, tp = 4 Sh), one batch holding every
+// sectoral unit at once (each order only sees its own).
+void vh_tsh_calibrate_n6(hipStream_t st, const vh_sht& s, double* n6_dev) {
+    const int L = s.L, nlm = s.nlm;
+    std::vector<double2> X((size_t)4 * nlm, make_double2(0.0, 0.0));
+    for (int l = 2; l <= L; l++) X[(size_t)3 * nlm + l * (l + 1) / 2 + l] = make_double2(1.0, 0.0);
+    double2 *dX = nullptr, *dF = nullptr, *dC = nullptr;
+    double* dg = nullptr;
+    HIPCHECK(hipMalloc((void**)&dX, X.size() * sizeof(double2)));
+    HIPCHECK(hipMalloc((void**)&dF, (size_t)6 * (L + 1) * s.nlat * sizeof(double2)));
+    HIPCHECK(hipMalloc((void**)&dC, X.size() * sizeof(double2)));
+    HIPCHECK(hipMalloc((void**)&dg, (size_t)6 * s.nlat * s.nphi * sizeof(double)));
+    HIPCHECK(hipMemcpyAsync(dX, X.data(), X.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+    // through the grid, as the transforms are used: synthesis -> dyads -> analysis
+    vh_tsh_synth(st, s, dX, 1, dF);
+    vh_t3_rows(st, s, dF, 1, 1, nullptr, 1, 0.0, dg);
+    vh_t3_rows(st, s, dF, 1, 2, nullptr, 1, 0.0, dg);
+    vh_tsh_analysis(st, s, dF, 1, nullptr, dC);
+    std::vector<double2> C(X.size());
+    HIPCHECK(hipMemcpyAsync(C.data(), dC, C.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    (void)hipFree(dX); (void)hipFree(dF); (void)hipFree(dC); (void)hipFree(dg);
+    std::vector<double> n6(L + 1, 0.0);
+    for (int l = 2; l <= L; l++) n6[l] = C[(size_t)3 * nlm + l * (l + 1) / 2 + l].x;
+    HIPCHECK(hipMemcpyAsync(n6_dev, n6.data(), (L + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipStreamSynchronize(st));
+}

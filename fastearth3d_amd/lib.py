@@ -38,6 +38,18 @@ def load():
         L.vh_resp_set_dt.argtypes = [C.c_void_p, C.c_double]
         L.vh_sht_analysis.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.vh_sht_synthesis.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.vh_tensor_synthesis.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.vh_tensor_analysis.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.vh_update_device.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
+        L.vh_profile.argtypes = [C.c_void_p, C.c_int]
+        L.vh_profile_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        for name in exported_symbols():
+            fn = getattr(L, name)
+            if fn.argtypes is None:   # every entry point takes a handle or pointer: never call undeclared
+                raise VilmaError("libvilma_hip.so: %s has no ctypes signature" % name)
+            if fn.restype is C.c_int or name == "vh_last_error":
+                continue
+            fn.restype = C.c_int
         _lib = L
     return _lib
 
@@ -46,7 +58,7 @@ def exported_symbols():
     """Names of the entry points declared in include/vilma_hip.h."""
     import re
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(vh_[a-z0-9_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(vh_\w+)\s*\(", txt, re.M)))
 
 
 def _cp(a):
@@ -84,6 +96,20 @@ class SolidEarth(ModelHandle):
         self._check(self._lib.vh_sht_synthesis(self._h, _cp(np.ascontiguousarray(q, dtype=np.complex128)), _cp(f)),
                     "sht_synthesis")
         return f
+
+    def tensor_synth(self, c):
+        """tensor_sh_synth: c (4, nlm) complex LM order -> dyads (6, nlat, nphi)."""
+        d = np.empty((6, self.nlat, self.nphi))
+        self._check(self._lib.vh_tensor_synthesis(self._h, _cp(np.ascontiguousarray(c, dtype=np.complex128)),
+                                                  _cp(d)), "tensor_sh_synth")
+        return d
+
+    def tensor_analysis(self, d):
+        """tensor_sh_analysis: dyads (6, nlat, nphi) -> c (4, nlm) complex."""
+        c = np.empty((4, self.dims["nlm"]), dtype=np.complex128)
+        self._check(self._lib.vh_tensor_analysis(self._h, _cp(np.ascontiguousarray(d, dtype=np.float64)),
+                                                 _cp(c)), "tensor_sh_analysis")
+        return c
 
     def apply(self, sigma):
         u = np.empty(self.dims["nlm"], dtype=np.complex128)

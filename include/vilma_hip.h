@@ -145,7 +145,15 @@ const char* vh_last_error(const vh_model* h);
 /* ---- Measurement hook (no counterpart in the reference): per-kernel-class
  * device time from HIP events recorded on the model's stream around each
  * launch. enable != 0 starts accumulating (and clears the totals). */
-enum { VH_K_DRIFT = 0, VH_K_ADVANCE = 1, VH_K_TENSOR = 2, VH_K_NCLASS = 8 };
+enum {
+    VH_K_DRIFT = 0,          /* drift: forcing + restricted-operator product   */
+    VH_K_ADVANCE = 1,        /* memory advance (1-D update, 3-D increments)    */
+    VH_K_TSH_SYNTH = 2,      /* 3-D: tensor Legendre synthesis                 */
+    VH_K_T3_ROWS = 3,        /* 3-D: IFFT, -Mk3 product, FFT per latitude row  */
+    VH_K_TSH_ANALYSIS = 4,   /* 3-D: tensor Legendre analysis                  */
+    VH_K_T3_GATHER = 5,      /* 3-D: memory + 2 mu strain coefficients         */
+    VH_K_NCLASS = 8
+};
 int vh_profile(vh_model* h, int enable);
 /* ms[c] = summed device milliseconds, n[c] = number of timed launches */
 int vh_profile_read(vh_model* h, double* ms /* VH_K_NCLASS */, double* n /* VH_K_NCLASS */);
@@ -172,6 +180,12 @@ int vh_get_gains(vh_model* h, double* g3 /* 3*(lmax+1) */, double* xn /* 2*nr*lm
  * src/vilma_sht.f90:204-226). */
 int vh_sht_analysis(vh_model* h, const double* grid, double* lm);
 int vh_sht_synthesis(vh_model* h, const double* lm, double* grid);
+/* Tensor transforms on the model grid (tensor_sh_synth / tensor_sh_analysis,
+ * src/vilma_tensor_sh.f90:168-322): c is [4][nlm] complex in LM order for
+ * lambda = 1, 2, 5, 6; dyad is [6][nlat][nphi] for rr, rt, rp, tt, tp, pp.
+ * The first call calibrates n6 (tensor_sh_init, :99-133). */
+int vh_tensor_synthesis(vh_model* h, const double* c, double* dyad);
+int vh_tensor_analysis(vh_model* h, const double* dyad, double* c);
 
 #ifdef __cplusplus
 }

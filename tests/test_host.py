@@ -86,6 +86,12 @@ def test_library_exports_header_symbols():
     lib = ctypes.CDLL(so)
     for n in names:
         assert hasattr(lib, n), n
+    # the Python binding gives every entry point a ctypes signature (an
+    # undeclared one would truncate the 64-bit handle to a C int)
+    from fastearth3d_amd import lib as vlib
+    L = vlib.load()
+    for n in names:
+        assert getattr(L, n).argtypes is not None, n
 
 
 def test_workload_generators():
