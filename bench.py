@@ -26,22 +26,43 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
-NAMES = {0: "k_drift", 1: "k_advance", 2: "tensor_3d"}
+NAMES = {0: "k_drift", 1: "k_advance", 2: "k_tsh_synth", 3: "k_t3_rows", 4: "k_tsh_analysis", 5: "k_t3_gather"}
 
 
-def alg_bytes(dims, cls):
-    """Algorithmic HBM bytes of one launch of kernel class cls: the Maxwell
-    memory of the Maxwell elements (24 doubles per element and slot), the
-    nodal drift at the memory nodes (4 doubles per node and slot), the
-    surface drift (6 doubles per slot), the per-slot norm and, for the drift,
-    the per-degree restricted solution operators ((2 nmn + 3) x 2 nmn doubles
-    per degree, read once)."""
+def alg_bytes(dims, cls, chunk_batches=None):
+    """Algorithmic HBM bytes of one launch of kernel class cls.
+
+    drift / advance: the Maxwell memory of the Maxwell elements (24 doubles
+    per element and slot), the nodal drift at the memory nodes (4 doubles
+    per node and slot), the surface drift (6 doubles per slot), the per-slot
+    norm and, for the drift, the per-degree restricted solution operators
+    ((2 nmn + 3) x 2 nmn doubles per degree, read once); the advance also
+    reads the 3-D increments (12 complex per 3-D element and slot).
+    Tensor kernels (per launch over `chunk_batches` element-shape batches):
+    synthesis reads 4 complex coefficients per (l, m) and writes 6 complex
+    dyad row spectra per (m, latitude), plus the P and dP tables once;
+    the row pass reads and writes those spectra and reads one viscosity-rate
+    row per latitude; the analysis is the synthesis reversed; the gather
+    reads the 3-D elements' memory and writes 12 complex per element and slot.
+    """
     nk, nem, nmn, nlm, L = dims["nk"], dims["nem"], dims["nmn"], dims["nlm"], dims["lmax"]
+    ne3d, nlat, nphi = dims.get("ne3d", 0), dims["nlat"], dims["nphi"]
     tau = 24 * nem * nk * 8
+    tables = 2 * nlm * ((nlat + 1) // 2) * 8
+    spec = 6 * (L + 1) * nlat * 16
+    nb = chunk_batches
     if cls == 0:   # drift: read tau + norm + operators, write nodal + surface drift
         return tau + 8 * nk + L * (2 * nmn + 3) * 2 * nmn * 8 + 4 * nmn * nk * 8 + 6 * nk * 8
-    if cls == 1:   # advance: read + write tau, read nodal drift and sigma, write norm
-        return 2 * tau + 4 * nmn * nk * 8 + 16 * nlm + 8 * nk
+    if cls == 1:   # advance: read + write tau, read nodal drift, sigma and 3-D increments, write norm
+        return 2 * tau + 4 * nmn * nk * 8 + 16 * nlm + 8 * nk + 12 * ne3d * nk * 16
+    if cls == 2 and nb:
+        return nb * (4 * nlm * 16 + spec) + tables
+    if cls == 3 and nb:
+        return nb * 2 * spec + (nb // 3 + 1) * nlat * nphi * 8
+    if cls == 4 and nb:
+        return nb * (spec + 4 * nlm * 16) + tables
+    if cls == 5:
+        return ne3d * (24 * nk * 8 + 12 * nlm * 16) + 4 * nmn * nk * 8 + 16 * nlm
     return None
 
 
@@ -118,16 +139,20 @@ def main():
     value = kyr / el
 
     d = dict(se.dims)
-    d["nmn"] = se.dims["nmn"]
+    d["nlat"], d["nphi"] = se.nlat, se.nphi
+    nbat = 3 * d.get("ne3d", 0)
     best = max((k for k in NAMES if cnt[k] > 0), key=lambda k: ms[k], default=None)
     roof = None
-    if best is not None and alg_bytes(d, best):
+    # the tensor kernels run in chunks of up to 48 batches; the average launch
+    # covers nbat / (number of chunks) batches
+    nch = (nbat + 47) // 48 if nbat else 1
+    if best is not None and alg_bytes(d, best, nbat / nch if nbat else None):
         avg_s = ms[best] / cnt[best] / 1000.0
-        ach = alg_bytes(d, best) / avg_s / 1e9
+        ach = alg_bytes(d, best, nbat / nch if nbat else None) / avg_s / 1e9
         roof = {"bound": "hbm", "kernel": NAMES[best], "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches": int(cnt[best]),
-                "alg_bytes_per_launch": alg_bytes(d, best)}
+                "alg_bytes_per_launch": int(alg_bytes(d, best, nbat / nch if nbat else None))}
     kernel_ms = {NAMES[k]: round(ms[k], 3) for k in NAMES if cnt[k] > 0}
 
     cpu = None

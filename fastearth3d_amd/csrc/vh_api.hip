@@ -190,7 +190,8 @@ struct vh_model {
     // device: SHT
     double *d_plm_lj = 0, *d_dplm_lj = 0, *d_plm_jl = 0, *d_dplm_jl = 0, *d_w = 0, *d_st = 0, *d_ct = 0;
     int* d_mstart = 0;
-    double2* d_tw = 0;
+    double2 *d_tw = 0, *d_twp = 0;
+    int2* d_ljobs = 0;
     // device: grid state
     double *d_topo0 = 0, *d_heq = 0, *d_C0ref = 0, *d_hice = 0, *d_ice1 = 0, *d_rsl = 0, *d_rsl_n = 0;
     double *d_C = 0, *d_Cn = 0, *d_dig = 0, *d_C0s = 0, *d_ice = 0, *d_dice = 0, *d_wcorr = 0;
@@ -263,7 +264,7 @@ vh_model::~vh_model() {
     void* ptrs[] = {d_r, d_mu, d_rho, d_Rk, d_MkPerDt, d_emap, d_mnodes, d_mnode, d_kdeg, d_is3d, d_ab, d_dr, d_dc,
                     d_ab1, d_null1, d_gu, d_gn, d_gv, d_xUn, d_xVn, d_ipiv, d_ipiv1, d_fail, d_tau[0], d_tau[1],
                     d_mnorm[0], d_mnorm[1], d_maxn, d_dN, d_dsurf, d_plm_lj, d_dplm_lj, d_plm_jl, d_dplm_jl, d_w,
-                    d_st, d_ct, d_mstart, d_tw, d_topo0, d_heq, d_C0ref, d_hice, d_ice1, d_rsl, d_rsl_n, d_C,
+                    d_st, d_ct, d_mstart, d_tw, d_twp, d_ljobs, d_topo0, d_heq, d_C0ref, d_hice, d_ice1, d_rsl, d_rsl_n, d_C,
                     d_Cn, d_dig, d_C0s, d_ice, d_dice, d_wcorr, d_sraw, d_srot, d_zbed, d_gtmp, d_F, d_F2,
                     d_sigma, d_lm, d_uN, d_part, d_sc, d_sc2, d_cphi, d_sphi,
                     ra.rs.mem, ra.rs.MkPerDt, ra.rs.xU, ra.rs.xV, ra.rs.dN, ra.rs.sc, d_emap2, d_mnodes2,
@@ -411,11 +412,10 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
     s.L = Lmax; s.nlat = nl; s.nphi = np; s.nh = d.nh; s.nlm = d.nlm;
     {
         int n = np, nf = 0;
-        const int pref[5] = {4, 2, 3, 5, 7};
-        for (int q = 0; q < 5 && n > 1; q++)
-            while (n % pref[q] == 0) { s.fac[nf++] = pref[q]; n /= pref[q]; }
+        const int pref[7] = {8, 6, 4, 2, 3, 5, 7};   // fewest passes for 2^a 3^b
+        for (int q = 0; q < 7 && n > 1; q++)
+            while (n % pref[q] == 0 && nf < 16) { s.fac[nf++] = pref[q]; n /= pref[q]; }
         if (n != 1) throw vh_error("sht_grid_init: nphi must factor into 2, 3, 5, 7");
-        if (nf > 16) throw vh_error("sht_grid_init: nphi has too many factors");
         s.nfac = nf;
     }
     h->d_w = dalloc<double>(nl); upload(h->d_w, w.data(), nl, st);
@@ -423,11 +423,31 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
     h->d_st = dalloc<double>(nl); upload(h->d_st, sn.data(), nl, st);
     h->d_mstart = dalloc<int>(Lmax + 1); upload(h->d_mstart, mstart.data(), Lmax + 1, st);
     h->d_tw = dalloc<double2>(np); upload(h->d_tw, tw.data(), np, st);
+    {   // per-pass Stockham twiddles e^{-2 pi i (j mod Ns) r tstep / N}, [r-1][j]
+        std::vector<double2> twp;
+        for (int p = 0, Ns = 1; p < s.nfac; p++) {
+            const int R = s.fac[p], NR = np / R, tstep = np / (Ns * R);
+            s.twoff[p] = (int)twp.size();
+            for (int r = 1; r < R; r++)
+                for (int j = 0; j < NR; j++) twp.push_back(tw[(size_t)((j % Ns) * r * tstep) % np]);
+            Ns *= R;
+        }
+        h->d_twp = dalloc<double2>(std::max<size_t>(twp.size(), 1)); upload(h->d_twp, twp.data(), twp.size(), st);
+        h->sync();   // twp goes out of scope
+    }
+    {   // Legendre analysis blocks: every order, in chunks of 64 degrees
+        std::vector<int2> lj;
+        for (int m = 0; m <= Lmax; m++)
+            for (int i0 = 0; i0 < Lmax - m + 1; i0 += 64) lj.push_back(make_int2(m, i0));
+        h->d_ljobs = dalloc<int2>(lj.size()); upload(h->d_ljobs, lj.data(), lj.size(), st);
+        s.ljobs = h->d_ljobs; s.nljobs = (int)lj.size();
+        h->sync();
+    }
     size_t ntab = (size_t)d.nlm * d.nh;
     h->d_plm_lj = dalloc<double>(ntab); h->d_dplm_lj = dalloc<double>(ntab);
     h->d_plm_jl = dalloc<double>(ntab); h->d_dplm_jl = dalloc<double>(ntab);
     s.plm_lj = h->d_plm_lj; s.dplm_lj = h->d_dplm_lj; s.plm_jl = h->d_plm_jl; s.dplm_jl = h->d_dplm_jl;
-    s.mstart = h->d_mstart; s.w = h->d_w; s.st = h->d_st; s.ct = h->d_ct; s.tw = h->d_tw;
+    s.mstart = h->d_mstart; s.w = h->d_w; s.st = h->d_st; s.ct = h->d_ct; s.tw = h->d_tw; s.twp = h->d_twp;
     vh_setup_legendre_tables(st, s, h->d_ct, h->d_st, h->d_plm_lj, h->d_dplm_lj, h->d_plm_jl, h->d_dplm_jl);
 
     // ---- grid state (solid_earth_init, src/vilma_coupling.f90:196-215)

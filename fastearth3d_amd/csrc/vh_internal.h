@@ -105,7 +105,10 @@ struct vh_sht {
     const double* st;      // [nlat] sin(colat)
     const double* ct;      // [nlat]
     const double2* tw;     // [nphi] e^{-2 pi i t/nphi}
-    int nfac, fac[16];
+    const double2* twp;    // per-pass FFT twiddles, pass p at twoff[p]: [R-1][nphi/R]
+    int nfac, fac[16], twoff[16];
+    const int2* ljobs;     // Legendre analysis blocks: (m, first degree offset), 64 degrees each
+    int nljobs;
 };
 void vh_setup_legendre_tables(hipStream_t st, const vh_sht& s, const double* ct, const double* sn,
                               double* plm_lj, double* dplm_lj, double* plm_jl, double* dplm_jl);

@@ -28,81 +28,120 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 
-// twiddle e^{-+2 pi i t/N}, 0 <= t < N
-__device__ __forceinline__ double2 twid(const double2* tw, int t, bool inv) {
-    double2 w = tw[t];
-    return inv ? make_double2(w.x, -w.y) : w;
+// grid points per lane whose loads are issued together in the row kernels
+#define VH_FFT_U 8
+
+__device__ __forceinline__ double2 cconj_if(double2 w, bool inv) { return inv ? make_double2(w.x, -w.y) : w; }
+// -i z (forward) / +i z (inverse)
+__device__ __forceinline__ double2 mul_mi(double2 z, bool inv) {
+    return inv ? make_double2(-z.y, z.x) : make_double2(z.y, -z.x);
 }
 
-// Size-R DFT of v (in registers; compile-time R so nothing spills).
-// w[q] = e^{-+2 pi i q/R}, q = 1..R-1.
+// Size-R DFT of v in registers (compile-time R: nothing spills), sign -+ for
+// forward / inverse.
 template <int R>
-__device__ __forceinline__ void dft_small(double2 (&v)[R], double2 (&V)[R], const double2 (&w)[R], bool inv) {
+__device__ __forceinline__ void dft_small(double2 (&v)[R], double2 (&V)[R], bool inv) {
     if constexpr (R == 2) {
         V[0] = cadd(v[0], v[1]); V[1] = csub(v[0], v[1]);
     } else if constexpr (R == 4) {
-        double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-        double2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
-        // forward: W4 = -i ; -i*z = (z.y, -z.x); inverse: +i*z = (-z.y, z.x)
-        double2 jd13 = inv ? make_double2(-d13.y, d13.x) : make_double2(d13.y, -d13.x);
+        const double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        const double2 s13 = cadd(v[1], v[3]), d13 = mul_mi(csub(v[1], v[3]), inv);
         V[0] = cadd(s02, s13); V[2] = csub(s02, s13);
-        V[1] = cadd(d02, jd13); V[3] = csub(d02, jd13);
-    } else {
+        V[1] = cadd(d02, d13); V[3] = csub(d02, d13);
+    } else if constexpr (R == 3) {
+        const double c = -0.5, sn = 0.86602540378443864676;   // cos, sin of 2 pi/3
+        const double2 t1 = cadd(v[1], v[2]);
+        const double2 t2 = make_double2(v[0].x + c * t1.x, v[0].y + c * t1.y);
+        const double2 d = csub(v[1], v[2]);
+        const double2 t3 = mul_mi(make_double2(sn * d.x, sn * d.y), inv);
+        V[0] = cadd(v[0], t1); V[1] = cadd(t2, t3); V[2] = csub(t2, t3);
+    } else if constexpr (R == 6) {   // 2 x 3: even/odd halves, W6^k on the odd one
+        double2 e[3] = {v[0], v[2], v[4]}, o[3] = {v[1], v[3], v[5]}, E[3], O[3];
+        dft_small<3>(e, E, inv);
+        dft_small<3>(o, O, inv);
+        const double h = 0.5, sn = 0.86602540378443864676;
+        const double2 w1 = cconj_if(make_double2(h, -sn), inv), w2 = cconj_if(make_double2(-h, -sn), inv);
+        const double2 O1 = cmul(O[1], w1), O2 = cmul(O[2], w2);
+        V[0] = cadd(E[0], O[0]); V[3] = csub(E[0], O[0]);
+        V[1] = cadd(E[1], O1); V[4] = csub(E[1], O1);
+        V[2] = cadd(E[2], O2); V[5] = csub(E[2], O2);
+    } else if constexpr (R == 8) {   // 2 x 4
+        double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]}, E[4], O[4];
+        dft_small<4>(e, E, inv);
+        dft_small<4>(o, O, inv);
+        const double r2 = 0.70710678118654752440;
+        const double2 w1 = cconj_if(make_double2(r2, -r2), inv), w3 = cconj_if(make_double2(-r2, -r2), inv);
+        const double2 O1 = cmul(O[1], w1), O2 = mul_mi(O[2], inv), O3 = cmul(O[3], w3);
+        V[0] = cadd(E[0], O[0]); V[4] = csub(E[0], O[0]);
+        V[1] = cadd(E[1], O1); V[5] = csub(E[1], O1);
+        V[2] = cadd(E[2], O2); V[6] = csub(E[2], O2);
+        V[3] = cadd(E[3], O3); V[7] = csub(E[3], O3);
+    } else {   // 5, 7: direct sum with exact-index twiddles
 #pragma unroll
         for (int u = 0; u < R; u++) {
             double2 acc = v[0];
 #pragma unroll
             for (int q = 1; q < R; q++) {
                 const int e = (q * u) % R;
-                acc = cadd(acc, e == 0 ? v[q] : cmul(v[q], w[e]));
+                if (e == 0) { acc = cadd(acc, v[q]); continue; }
+                double sv, cv;
+                sincospi(2.0 * e / R, &sv, &cv);
+                acc = cadd(acc, cmul(v[q], cconj_if(make_double2(cv, -sv), inv)));
             }
             V[u] = acc;
         }
     }
 }
 
-// One Stockham radix-R pass over N points by nthr threads (thread index t).
+// LDS exchange between the passes of one wavefront's FFT: LDS operations of
+// a wavefront complete in order, so only compiler reordering must be fenced.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One Stockham radix-R pass over N points. tp: this pass's twiddles
+// [r-1][j] = e^{-2 pi i (j mod Ns) r tstep/N} (coalesced over j).
 template <int R>
-__device__ __forceinline__ void fft_pass(const double2* src, double2* dst, int N, int Ns, const double2* tw,
+__device__ __forceinline__ void fft_pass(const double2* src, double2* dst, int N, int Ns, const double2* tp,
                                          bool inv, int t, int nthr) {
-    const int NR = N / R, tstep = N / (Ns * R);
-    double2 w[R];
-#pragma unroll
-    for (int q = 1; q < R; q++) w[q] = twid(tw, q * NR, inv);
-    w[0] = make_double2(1.0, 0.0);
+    const int NR = N / R;
     for (int j = t; j < NR; j += nthr) {
         const int k = j % Ns;
         double2 v[R], V[R];
 #pragma unroll
         for (int r = 0; r < R; r++) v[r] = src[j + r * NR];
 #pragma unroll
-        for (int r = 1; r < R; r++) v[r] = cmul(v[r], twid(tw, k * r * tstep, inv));
-        dft_small<R>(v, V, w, inv);
+        for (int r = 1; r < R; r++) v[r] = cmul(v[r], cconj_if(tp[(r - 1) * NR + j], inv));
+        dft_small<R>(v, V, inv);
         const int idxD = (j / Ns) * Ns * R + k;
 #pragma unroll
         for (int r = 0; r < R; r++) dst[idxD + r * Ns] = V[r];
     }
 }
 
-// One complex FFT of length N by the calling wavefront (lane 0..63). a, b:
-// LDS buffers of N; returns the buffer that holds the result. Natural order
-// in and out (Stockham autosort). fac: radices (2, 3, 4, 5, 7), product N.
-// Every wavefront of the block must call it with the same N (the passes are
-// separated by block barriers).
-__device__ double2* fft_wave(double2* a, double2* b, int N, const int* fac, int nfac, const double2* tw,
-                             bool inv, int lane) {
+// One complex FFT of length s.nphi by the calling wavefront alone (lane
+// 0..63): a, b are this wavefront's LDS buffers of N points; returns the one
+// holding the result. Natural order in and out (Stockham autosort). The
+// caller makes a's contents visible to the whole wavefront first.
+__device__ double2* fft_wave(double2* a, double2* b, const vh_sht& s, bool inv, int lane) {
+    const int N = s.nphi;
     int Ns = 1;
     double2 *src = a, *dst = b;
-    for (int s = 0; s < nfac; s++) {
-        const int R = fac[s];
+    for (int p = 0; p < s.nfac; p++) {
+        const int R = s.fac[p];
+        const double2* tp = s.twp + s.twoff[p];
         switch (R) {
-            case 2: fft_pass<2>(src, dst, N, Ns, tw, inv, lane, 64); break;
-            case 3: fft_pass<3>(src, dst, N, Ns, tw, inv, lane, 64); break;
-            case 4: fft_pass<4>(src, dst, N, Ns, tw, inv, lane, 64); break;
-            case 5: fft_pass<5>(src, dst, N, Ns, tw, inv, lane, 64); break;
-            default: fft_pass<7>(src, dst, N, Ns, tw, inv, lane, 64); break;
+            case 8: fft_pass<8>(src, dst, N, Ns, tp, inv, lane, 64); break;
+            case 6: fft_pass<6>(src, dst, N, Ns, tp, inv, lane, 64); break;
+            case 4: fft_pass<4>(src, dst, N, Ns, tp, inv, lane, 64); break;
+            case 2: fft_pass<2>(src, dst, N, Ns, tp, inv, lane, 64); break;
+            case 3: fft_pass<3>(src, dst, N, Ns, tp, inv, lane, 64); break;
+            case 5: fft_pass<5>(src, dst, N, Ns, tp, inv, lane, 64); break;
+            default: fft_pass<7>(src, dst, N, Ns, tp, inv, lane, 64); break;
         }
-        __syncthreads();
+        wave_lds_sync();
         double2* t = src; src = dst; dst = t;
         Ns *= R;
     }
@@ -120,10 +159,23 @@ __global__ void __launch_bounds__(256) k_fft_fwd(vh_sht s, In in, int nrows, dou
     const int pair = blockIdx.x * (blockDim.x >> 6) + wave;
     const int r0 = 2 * pair, r1 = 2 * pair + 1;
     const bool has0 = r0 < nrows, has1 = r1 < nrows;
-    for (int i = lane; i < N; i += 64)
-        a[i] = make_double2(has0 ? in(r0, i) : 0.0, has1 ? in(r1, i) : 0.0);
-    __syncthreads();
-    double2* z = fft_wave(a, b, N, s.fac, s.nfac, s.tw, false, lane);
+    // all of a lane's grid loads in flight before the first LDS store
+    for (int i0 = 0; i0 < N; i0 += 64 * VH_FFT_U) {
+        double v0[VH_FFT_U], v1[VH_FFT_U];
+#pragma unroll
+        for (int u = 0; u < VH_FFT_U; u++) {
+            const int i = i0 + lane + 64 * u;
+            v0[u] = (has0 && i < N) ? in(r0, i) : 0.0;
+            v1[u] = (has1 && i < N) ? in(r1, i) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < VH_FFT_U; u++) {
+            const int i = i0 + lane + 64 * u;
+            if (i < N) a[i] = make_double2(v0[u], v1[u]);
+        }
+    }
+    wave_lds_sync();
+    double2* z = fft_wave(a, b, s, false, lane);
     if (has0) {
         for (int m = lane; m <= L; m += 64) {
             double2 zm = z[m], zc = z[(N - m) % N];
@@ -148,20 +200,36 @@ __global__ void __launch_bounds__(256) k_fft_inv(vh_sht s, const double2* __rest
     const int r0 = 2 * pair, r1 = 2 * pair + 1;
     const bool has0 = r0 < nrows, has1 = r1 < nrows;
     for (int i = lane; i < N; i += 64) a[i] = make_double2(0.0, 0.0);
-    __syncthreads();
+    wave_lds_sync();
     if (has0) {
-        for (int m = lane; m <= L; m += 64) {
-            double2 A = F[(size_t)r0 * (L + 1) + m];
-            double2 B = has1 ? F[(size_t)r1 * (L + 1) + m] : make_double2(0.0, 0.0);
-            if (m == 0) { A.y = 0.0; B.y = 0.0; }
-            a[m] = make_double2(A.x - B.y, A.y + B.x);
-            if (m > 0) a[N - m] = make_double2(A.x + B.y, -A.y + B.x);
+        for (int m0 = 0; m0 <= L; m0 += 64 * VH_FFT_U) {
+            double2 A[VH_FFT_U], B[VH_FFT_U];
+#pragma unroll
+            for (int u = 0; u < VH_FFT_U; u++) {
+                const int m = m0 + lane + 64 * u;
+                A[u] = m <= L ? F[(size_t)r0 * (L + 1) + m] : make_double2(0.0, 0.0);
+                B[u] = (has1 && m <= L) ? F[(size_t)r1 * (L + 1) + m] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < VH_FFT_U; u++) {
+                const int m = m0 + lane + 64 * u;
+                if (m > L) continue;
+                if (m == 0) { A[u].y = 0.0; B[u].y = 0.0; }
+                a[m] = make_double2(A[u].x - B[u].y, A[u].y + B[u].x);
+                if (m > 0) a[N - m] = make_double2(A[u].x + B[u].y, -A[u].y + B[u].x);
+            }
         }
     }
-    __syncthreads();
-    double2* z = fft_wave(a, b, N, s.fac, s.nfac, s.tw, true, lane);
+    wave_lds_sync();
+    double2* z = fft_wave(a, b, s, true, lane);
     if (has0)
-        for (int i = lane; i < N; i += 64) out(r0, i, z[i].x, z[i].y, has1);
+        for (int i0 = 0; i0 < N; i0 += 64 * VH_FFT_U) {
+#pragma unroll
+            for (int u = 0; u < VH_FFT_U; u++) {
+                const int i = i0 + lane + 64 * u;
+                if (i < N) out(r0, i, z[i].x, z[i].y, has1);
+            }
+        }
 }
 
 template <class In>

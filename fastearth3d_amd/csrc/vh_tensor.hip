@@ -45,70 +45,84 @@ __device__ __forceinline__ int xcd_block(int nblk) {
 static inline unsigned xcd_grid(int nblk) { return (unsigned)(((nblk + 7) / 8) * 8); }
 
 // ---------------------------------------------------------- synthesis
-// Block = (batch b, order m); thread = north latitude j.
-__global__ void __launch_bounds__(128) k_tsh_synth(vh_sht s, const double2* __restrict__ X, int nb,
+// Block = (order m, TB consecutive batches); lane pair (2j, 2j+1) = north
+// latitude j, the even / odd l+m half of the degree sum. The P/dP table rows
+// of the order are loaded once per block for all TB batches; the pair swaps
+// its partial sums and the even lane writes the north row, the odd lane the
+// south row.
+template <int TB>
+__global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __restrict__ X, int nb,
                                                    double2* __restrict__ Fd) {
-    extern __shared__ double2 cs[];   // [5][nl]: c1, c2, -J c5, c6, -J c6
+    extern __shared__ double2 cs[];   // [TB][5][nl]: c1, c2, -J c5, c6, -J c6
     const int L = s.L, nh = s.nh, nlat = s.nlat, nlm = s.nlm;
-    const int lb = xcd_block(nb * (L + 1));
-    if (lb >= nb * (L + 1)) return;
-    const int b = lb / (L + 1), m = lb % (L + 1), nl = L - m + 1;
-    for (int t = threadIdx.x; t < 4 * nl; t += blockDim.x) {
-        const int lam = t / nl, il = t % nl, l = m + il;
-        double2 v = X[((size_t)b * 4 + lam) * nlm + l * (l + 1) / 2 + m];
+    const int ngrp = (nb + TB - 1) / TB;
+    const int lb = xcd_block(ngrp * (L + 1));
+    if (lb >= ngrp * (L + 1)) return;
+    const int b0 = (lb / (L + 1)) * TB, m = lb % (L + 1), nl = L - m + 1;
+    const int nbb = min(TB, nb - b0);
+    for (int t = threadIdx.x; t < TB * 4 * nl; t += blockDim.x) {
+        const int bb = t / (4 * nl), r = t % (4 * nl), lam = r / nl, il = r % nl, l = m + il;
+        double2 v = make_double2(0.0, 0.0);
+        if (bb < nbb) v = X[((size_t)(b0 + bb) * 4 + lam) * nlm + l * (l + 1) / 2 + m];
         const double J = (double)l * (double)(l + 1);
         if (lam == 2) v = make_double2(-J * v.x, -J * v.y);
-        cs[lam * nl + il] = v;
-        if (lam == 3) cs[4 * nl + il] = make_double2(-J * v.x, -J * v.y);
+        double2* cb = cs + (size_t)bb * 5 * nl;
+        cb[lam * nl + il] = v;
+        if (lam == 3) cb[4 * nl + il] = make_double2(-J * v.x, -J * v.y);
     }
     __syncthreads();
-    const int j = threadIdx.x;
-    if (j >= nh) return;
+    const int j = threadIdx.x >> 1, par = threadIdx.x & 1;
+    const bool live = j < nh;   // both lanes of a pair agree
+    const int jj = live ? j : 0;
     const double* P = s.plm_lj + (size_t)s.mstart[m] * nh;
     const double* D = s.dplm_lj + (size_t)s.mstart[m] * nh;
-    // [parity][sum]: A1, A2, D2, A5, A6, L6, D6
-    double2 acc[2][7];
+    // [batch][sum]: A1, A2, D2, A5, A6, L6, D6 over this lane's parity
+    double2 acc[TB][7];
 #pragma unroll
-    for (int p = 0; p < 2; p++)
+    for (int bb = 0; bb < TB; bb++)
 #pragma unroll
-        for (int q = 0; q < 7; q++) acc[p][q] = make_double2(0.0, 0.0);
-    auto step = [&](int il, double2 (&a)[7]) {
-        const double pv = P[(size_t)il * nh + j], dv = D[(size_t)il * nh + j];
-        const double2 c1 = cs[il], c2 = cs[nl + il], c5 = cs[2 * nl + il], c6 = cs[3 * nl + il], l6 = cs[4 * nl + il];
-        a[0].x = fma(c1.x, pv, a[0].x); a[0].y = fma(c1.y, pv, a[0].y);
-        a[1].x = fma(c2.x, pv, a[1].x); a[1].y = fma(c2.y, pv, a[1].y);
-        a[2].x = fma(c2.x, dv, a[2].x); a[2].y = fma(c2.y, dv, a[2].y);
-        a[3].x = fma(c5.x, pv, a[3].x); a[3].y = fma(c5.y, pv, a[3].y);
-        a[4].x = fma(c6.x, pv, a[4].x); a[4].y = fma(c6.y, pv, a[4].y);
-        a[5].x = fma(l6.x, pv, a[5].x); a[5].y = fma(l6.y, pv, a[5].y);
-        a[6].x = fma(c6.x, dv, a[6].x); a[6].y = fma(c6.y, dv, a[6].y);
-    };
-    int il = 0;
-    for (; il + 1 < nl; il += 2) { step(il, acc[0]); step(il + 1, acc[1]); }
-    if (il < nl) step(il, acc[0]);
-    const double ms = (double)m;
+        for (int q = 0; q < 7; q++) acc[bb][q] = make_double2(0.0, 0.0);
+    for (int il = par; il < nl; il += 2) {
+        const double pv = P[(size_t)il * nh + jj], dv = D[(size_t)il * nh + jj];
 #pragma unroll
-    for (int hemi = 0; hemi < 2; hemi++) {
-        const int row = hemi == 0 ? j : nlat - 1 - j;
-        if (hemi == 1 && row == j) break;   // equator row of an odd grid
-        // full sums at this row: P-type E +- O, dP-type +-(E - O) / (E + O)
-        const double sp = hemi == 0 ? 1.0 : -1.0;
+        for (int bb = 0; bb < TB; bb++) {
+            const double2* cb = cs + (size_t)bb * 5 * nl;
+            const double2 c1 = cb[il], c2 = cb[nl + il], c5 = cb[2 * nl + il], c6 = cb[3 * nl + il],
+                          l6 = cb[4 * nl + il];
+            double2* a = acc[bb];
+            a[0].x = fma(c1.x, pv, a[0].x); a[0].y = fma(c1.y, pv, a[0].y);
+            a[1].x = fma(c2.x, pv, a[1].x); a[1].y = fma(c2.y, pv, a[1].y);
+            a[2].x = fma(c2.x, dv, a[2].x); a[2].y = fma(c2.y, dv, a[2].y);
+            a[3].x = fma(c5.x, pv, a[3].x); a[3].y = fma(c5.y, pv, a[3].y);
+            a[4].x = fma(c6.x, pv, a[4].x); a[4].y = fma(c6.y, pv, a[4].y);
+            a[5].x = fma(l6.x, pv, a[5].x); a[5].y = fma(l6.y, pv, a[5].y);
+            a[6].x = fma(c6.x, dv, a[6].x); a[6].y = fma(c6.y, dv, a[6].y);
+        }
+    }
+    // even lane: north row (E + O; dP-type E + O), odd lane: south row
+    // (P-type E - O; dP-type O - E); own = this lane's parity sum
+    const int row = par == 0 ? jj : nlat - 1 - jj;
+    const bool write = live && !(par == 1 && row == jj);   // no south twin on the equator of an odd grid
+    const double st = s.st[row], cot = s.ct[row] / st, is = 1.0 / st;
+    const double mis = (double)m * is, g2 = 2.0 * mis * mis;
+#pragma unroll
+    for (int bb = 0; bb < TB; bb++) {
         double2 S[7];
 #pragma unroll
         for (int q = 0; q < 7; q++) {
+            const double2 own = acc[bb][q];
+            const double2 oth = make_double2(__shfl_xor(own.x, 1), __shfl_xor(own.y, 1));
+            const double2 e = par == 0 ? own : oth, o = par == 0 ? oth : own;
             const bool dtype = (q == 2 || q == 6);
-            double2 e = acc[0][q], o = acc[1][q];
-            if (!dtype) S[q] = make_double2(e.x + sp * o.x, e.y + sp * o.y);
-            else S[q] = hemi == 0 ? make_double2(e.x + o.x, e.y + o.y) : make_double2(o.x - e.x, o.y - e.y);
+            if (par == 0) S[q] = make_double2(e.x + o.x, e.y + o.y);
+            else S[q] = dtype ? make_double2(o.x - e.x, o.y - e.y) : make_double2(e.x - o.x, e.y - o.y);
         }
-        const double st = s.st[row], cot = s.ct[row] / st, is = 1.0 / st;
-        const double mis = ms * is;
+        if (!write || bb >= nbb) continue;
         double2 out[6];
         out[0] = S[0];                                             // rr
         out[1] = S[2];                                             // rt
         out[2] = make_double2(-mis * S[1].y, mis * S[1].x);        // rp = i (m/s) A2
         const double2 tr = S[3];
-        const double g2 = 2.0 * mis * mis;
         const double2 Sg = make_double2(S[5].x - 2.0 * cot * S[6].x + g2 * S[4].x,
                                         S[5].y - 2.0 * cot * S[6].y + g2 * S[4].y);
         const double2 hd = make_double2(S[6].x - cot * S[4].x, S[6].y - cot * S[4].y);
@@ -117,36 +131,43 @@ __global__ void __launch_bounds__(128) k_tsh_synth(vh_sht s, const double2* __re
         out[4] = make_double2(4.0 * Sh.x, 4.0 * Sh.y);             // tp
         out[5] = make_double2(tr.x - Sg.x, tr.y - Sg.y);           // pp
 #pragma unroll
-        for (int dy = 0; dy < 6; dy++) Fd[(((size_t)b * 6 + dy) * (L + 1) + m) * nlat + row] = out[dy];
+        for (int dy = 0; dy < 6; dy++)
+            Fd[(((size_t)(b0 + bb) * 6 + dy) * (L + 1) + m) * nlat + row] = out[dy];
     }
 }
 
 // ----------------------------------------------------------- analysis
-// Block = (batch b, order m). Stage 1 folds the six dyads of the latitude
-// pair (j, nlat-1-j) into the seven weighted partial fields (even/odd l+m);
-// stage 2: thread = degree, quadrature over the north latitudes. n6 == null
+// Block = (order m, TB consecutive batches). Stage 1 folds the six dyads of
+// the latitude pair (j, nlat-1-j) into the seven weighted partial fields
+// (even/odd l+m); stage 2: thread = degree, quadrature over the north
+// latitudes, the table row loaded once for the TB batches. n6 == null
 // returns the raw spin-2 adjoint (calibration).
+template <int TB>
 __global__ void __launch_bounds__(128) k_tsh_analysis(vh_sht s, const double2* __restrict__ Fd, int nb,
                                                       const double* __restrict__ n6, double2* __restrict__ C) {
-    extern __shared__ double2 fs[];   // [7][2][nh]: f1, f2P, f5, fD, fb (P-type), f2D, fd (dP-type)
+    extern __shared__ double2 fs[];   // [TB][7][2][nh]: f1, f2P, f5, fD, fb (P-type), f2D, fd (dP-type)
     const int L = s.L, nh = s.nh, nlat = s.nlat, nlm = s.nlm;
-    const int lb = xcd_block(nb * (L + 1));
-    if (lb >= nb * (L + 1)) return;
-    const int b = lb / (L + 1), m = lb % (L + 1), nl = L - m + 1;
+    const int ngrp = (nb + TB - 1) / TB;
+    const int lb = xcd_block(ngrp * (L + 1));
+    if (lb >= ngrp * (L + 1)) return;
+    const int b0 = (lb / (L + 1)) * TB, m = lb % (L + 1), nl = L - m + 1;
+    const int nbb = min(TB, nb - b0);
     const double ms = (double)m;
-    for (int j = threadIdx.x; j < nh; j += blockDim.x) {
+    for (int t = threadIdx.x; t < TB * nh; t += blockDim.x) {
+        const int bb = t / nh, j = t % nh;
+        double2* fb_ = fs + (size_t)bb * 14 * nh;
         double2 f[2][7];
 #pragma unroll
         for (int hemi = 0; hemi < 2; hemi++) {
             const int row = hemi == 0 ? j : nlat - 1 - j;
-            if (hemi == 1 && row == j) {
+            if (bb >= nbb || (hemi == 1 && row == j)) {
 #pragma unroll
-                for (int q = 0; q < 7; q++) f[1][q] = make_double2(0.0, 0.0);
-                break;
+                for (int q = 0; q < 7; q++) f[hemi][q] = make_double2(0.0, 0.0);
+                continue;
             }
             double2 d[6];
 #pragma unroll
-            for (int dy = 0; dy < 6; dy++) d[dy] = Fd[(((size_t)b * 6 + dy) * (L + 1) + m) * nlat + row];
+            for (int dy = 0; dy < 6; dy++) d[dy] = Fd[(((size_t)(b0 + bb) * 6 + dy) * (L + 1) + m) * nlat + row];
             const double w = s.w[row], st = s.st[row], cot = s.ct[row] / st, is = 1.0 / st;
             const double mis = ms * is;
             const double2 Dv = make_double2(d[3].x - d[5].x, d[3].y - d[5].y);     // tt - pp
@@ -168,8 +189,8 @@ __global__ void __launch_bounds__(128) k_tsh_analysis(vh_sht s, const double2* _
             const bool dtype = q >= 5;
             const double2 a = f[0][q], c = f[1][q];
             const double2 plus = make_double2(a.x + c.x, a.y + c.y), minus = make_double2(a.x - c.x, a.y - c.y);
-            fs[(q * 2 + 0) * nh + j] = dtype ? minus : plus;    // even l+m
-            fs[(q * 2 + 1) * nh + j] = dtype ? plus : minus;    // odd l+m
+            fb_[(q * 2 + 0) * nh + j] = dtype ? minus : plus;    // even l+m
+            fb_[(q * 2 + 1) * nh + j] = dtype ? plus : minus;    // odd l+m
         }
     }
     __syncthreads();
@@ -177,36 +198,42 @@ __global__ void __launch_bounds__(128) k_tsh_analysis(vh_sht s, const double2* _
     const double* D = s.dplm_jl + (size_t)s.mstart[m] * nh;
     for (int il = threadIdx.x; il < nl; il += blockDim.x) {
         const int l = m + il, p = il & 1;
-        const double2* f1 = fs + (0 * 2 + p) * nh;
-        const double2* f2P = fs + (1 * 2 + p) * nh;
-        const double2* f5 = fs + (2 * 2 + p) * nh;
-        const double2* fD = fs + (3 * 2 + p) * nh;
-        const double2* fb = fs + (4 * 2 + p) * nh;
-        const double2* f2D = fs + (5 * 2 + p) * nh;
-        const double2* fd = fs + (6 * 2 + p) * nh;
-        double2 s1 = make_double2(0, 0), s2 = s1, s5 = s1, sD = s1, sb = s1, sd = s1;
+        double2 s1[TB], s2[TB], s5[TB], sD[TB], sb[TB], sd[TB];
+#pragma unroll
+        for (int bb = 0; bb < TB; bb++)
+            s1[bb] = s2[bb] = s5[bb] = sD[bb] = sb[bb] = sd[bb] = make_double2(0.0, 0.0);
         for (int j = 0; j < nh; j++) {
             const double pv = P[(size_t)j * nl + il], dv = D[(size_t)j * nl + il];
-            s1.x = fma(pv, f1[j].x, s1.x); s1.y = fma(pv, f1[j].y, s1.y);
-            s2.x = fma(dv, f2D[j].x, fma(pv, f2P[j].x, s2.x)); s2.y = fma(dv, f2D[j].y, fma(pv, f2P[j].y, s2.y));
-            s5.x = fma(pv, f5[j].x, s5.x); s5.y = fma(pv, f5[j].y, s5.y);
-            sD.x = fma(pv, fD[j].x, sD.x); sD.y = fma(pv, fD[j].y, sD.y);
-            sb.x = fma(pv, fb[j].x, sb.x); sb.y = fma(pv, fb[j].y, sb.y);
-            sd.x = fma(dv, fd[j].x, sd.x); sd.y = fma(dv, fd[j].y, sd.y);
+#pragma unroll
+            for (int bb = 0; bb < TB; bb++) {
+                const double2* f = fs + (size_t)bb * 14 * nh + p * nh + j;
+                const double2 f1 = f[0], f2P = f[2 * nh], f5 = f[4 * nh], fD = f[6 * nh], fb = f[8 * nh],
+                              f2D = f[10 * nh], fd = f[12 * nh];
+                s1[bb].x = fma(pv, f1.x, s1[bb].x); s1[bb].y = fma(pv, f1.y, s1[bb].y);
+                s2[bb].x = fma(dv, f2D.x, fma(pv, f2P.x, s2[bb].x));
+                s2[bb].y = fma(dv, f2D.y, fma(pv, f2P.y, s2[bb].y));
+                s5[bb].x = fma(pv, f5.x, s5[bb].x); s5[bb].y = fma(pv, f5.y, s5[bb].y);
+                sD[bb].x = fma(pv, fD.x, sD[bb].x); sD[bb].y = fma(pv, fD.y, sD[bb].y);
+                sb[bb].x = fma(pv, fb.x, sb[bb].x); sb[bb].y = fma(pv, fb.y, sb[bb].y);
+                sd[bb].x = fma(dv, fd.x, sd[bb].x); sd[bb].y = fma(dv, fd.y, sd[bb].y);
+            }
         }
         const double J = (double)l * (double)(l + 1);
         const size_t idx = (size_t)l * (l + 1) / 2 + m;
-        double2* Cb = C + (size_t)b * 4 * nlm;
-        Cb[idx] = s1;
-        Cb[nlm + idx] = J > 0.0 ? make_double2(s2.x / J, s2.y / J) : make_double2(0.0, 0.0);
-        Cb[2 * nlm + idx] = J > 0.0 ? make_double2(s5.x / (-2.0 * J), s5.y / (-2.0 * J)) : make_double2(0.0, 0.0);
-        const double2 raw = make_double2(-J * sD.x + sb.x + sd.x, -J * sD.y + sb.y + sd.y);
-        double2 c6 = raw;
-        if (n6) {
-            const double n = n6[l];
-            c6 = n != 0.0 ? make_double2(raw.x / n, raw.y / n) : make_double2(0.0, 0.0);
+        const double n6l = n6 ? n6[l] : 1.0;
+#pragma unroll
+        for (int bb = 0; bb < TB; bb++) {
+            if (bb >= nbb) break;
+            double2* Cb = C + (size_t)(b0 + bb) * 4 * nlm;
+            Cb[idx] = s1[bb];
+            Cb[nlm + idx] = J > 0.0 ? make_double2(s2[bb].x / J, s2[bb].y / J) : make_double2(0.0, 0.0);
+            Cb[2 * nlm + idx] = J > 0.0 ? make_double2(s5[bb].x / (-2.0 * J), s5[bb].y / (-2.0 * J))
+                                        : make_double2(0.0, 0.0);
+            const double2 raw = make_double2(-J * sD[bb].x + sb[bb].x + sd[bb].x, -J * sD[bb].y + sb[bb].y + sd[bb].y);
+            double2 c6 = raw;
+            if (n6) c6 = n6l != 0.0 ? make_double2(raw.x / n6l, raw.y / n6l) : make_double2(0.0, 0.0);
+            Cb[3 * nlm + idx] = c6;
         }
-        Cb[3 * nlm + idx] = c6;
     }
 }
 
@@ -239,35 +266,47 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, double2* __restrict__
     double2* z;
     if (mode != 2) {
         for (int i = lane; i < N; i += 64) a[i] = make_double2(0.0, 0.0);
-        __syncthreads();
-        for (int m = lane; m <= L; m += 64) {
-            double2 A = F0[(size_t)m * nlat], B = F1[(size_t)m * nlat];
-            if (m == 0) { A.y = 0.0; B.y = 0.0; }
-            a[m] = make_double2(A.x - B.y, A.y + B.x);
-            if (m > 0) a[N - m] = make_double2(A.x + B.y, -A.y + B.x);
+        wave_lds_sync();
+        for (int m0 = 0; m0 <= L; m0 += 64 * VH_FFT_U) {
+            double2 A[VH_FFT_U], B[VH_FFT_U];
+#pragma unroll
+            for (int u = 0; u < VH_FFT_U; u++) {
+                const int m = m0 + lane + 64 * u;
+                A[u] = m <= L ? F0[(size_t)m * nlat] : make_double2(0.0, 0.0);
+                B[u] = m <= L ? F1[(size_t)m * nlat] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < VH_FFT_U; u++) {
+                const int m = m0 + lane + 64 * u;
+                if (m > L) continue;
+                if (m == 0) { A[u].y = 0.0; B[u].y = 0.0; }
+                a[m] = make_double2(A[u].x - B[u].y, A[u].y + B[u].x);
+                if (m > 0) a[N - m] = make_double2(A[u].x + B[u].y, -A[u].y + B[u].x);
+            }
         }
-        __syncthreads();
-        z = fft_wave(a, c, N, s.fac, s.nfac, s.tw, true, lane);
+        wave_lds_sync();
+        z = fft_wave(a, c, s, true, lane);
         if (mode == 1) {
             double* g0 = grid + (((size_t)b * 6 + d0) * nlat + j) * N;
             double* g1 = grid + (((size_t)b * 6 + d1) * nlat + j) * N;
             for (int i = lane; i < N; i += 64) { g0[i] = z[i].x; g1[i] = z[i].y; }
             return;
         }
+        __syncthreads();   // the rate row is written by the whole block
         for (int i = lane; i < N; i += 64) {
             const double M = Mrow[i];
             z[i] = make_double2(-M * z[i].x, -M * z[i].y);
         }
-        __syncthreads();
+        wave_lds_sync();
     } else {
         const double* g0 = grid + (((size_t)b * 6 + d0) * nlat + j) * N;
         const double* g1 = grid + (((size_t)b * 6 + d1) * nlat + j) * N;
         for (int i = lane; i < N; i += 64) a[i] = make_double2(g0[i], g1[i]);
-        __syncthreads();
+        wave_lds_sync();
         z = a;
     }
     double2* other = (z == a) ? c : a;
-    double2* y = fft_wave(z, other, N, s.fac, s.nfac, s.tw, false, lane);
+    double2* y = fft_wave(z, other, s, false, lane);
     const double scale = 2.0 * VH_PI / N;
     for (int m = lane; m <= L; m += 64) {
         double2 zm = y[m], zc = y[(N - m) % N];
@@ -319,20 +358,30 @@ __global__ void __launch_bounds__(256) k_t3_gather(int nr, int nk, int nlm, vh_d
 }
 
 // ------------------------------------------------------------- host side
-static size_t synth_lds(const vh_sht& s) { return (size_t)5 * (s.L + 1) * sizeof(double2); }
-static size_t anal_lds(const vh_sht& s) { return (size_t)14 * s.nh * sizeof(double2); }
+// batches per block of the tensor Legendre kernels (table reuse vs registers)
+#ifndef VH_TSH_TB_SYNTH
+#define VH_TSH_TB_SYNTH 3
+#endif
+#ifndef VH_TSH_TB_ANAL
+#define VH_TSH_TB_ANAL 1
+#endif
+static size_t synth_lds(const vh_sht& s) { return (size_t)VH_TSH_TB_SYNTH * 5 * (s.L + 1) * sizeof(double2); }
+static size_t anal_lds(const vh_sht& s) { return (size_t)VH_TSH_TB_ANAL * 14 * s.nh * sizeof(double2); }
 static size_t rows_lds(const vh_sht& s) { return (size_t)6 * s.nphi * sizeof(double2) + (size_t)s.nphi * sizeof(double); }
 
 void vh_tsh_synth(hipStream_t st, const vh_sht& s, const double2* X, int nb, double2* Fd) {
     if (nb <= 0) return;
     if (s.nh > 128) throw vh_error("tensor transform: nlat above 256 is not supported");
-    k_tsh_synth<<<xcd_grid(nb * (s.L + 1)), 128, synth_lds(s), st>>>(s, X, nb, Fd);
+    const int ngrp = (nb + VH_TSH_TB_SYNTH - 1) / VH_TSH_TB_SYNTH;
+    const int nthr = ((2 * s.nh + 63) / 64) * 64;
+    k_tsh_synth<VH_TSH_TB_SYNTH><<<xcd_grid(ngrp * (s.L + 1)), nthr, synth_lds(s), st>>>(s, X, nb, Fd);
     HIPCHECK(hipGetLastError());
 }
 
 void vh_tsh_analysis(hipStream_t st, const vh_sht& s, const double2* Fd, int nb, const double* n6, double2* C) {
     if (nb <= 0) return;
-    k_tsh_analysis<<<xcd_grid(nb * (s.L + 1)), 128, anal_lds(s), st>>>(s, Fd, nb, n6, C);
+    const int ngrp = (nb + VH_TSH_TB_ANAL - 1) / VH_TSH_TB_ANAL;
+    k_tsh_analysis<VH_TSH_TB_ANAL><<<xcd_grid(ngrp * (s.L + 1)), 128, anal_lds(s), st>>>(s, Fd, nb, n6, C);
     HIPCHECK(hipGetLastError());
 }
 
