@@ -44,6 +44,18 @@ __device__ __forceinline__ int xcd_block(int nblk) {
 }
 static inline unsigned xcd_grid(int nblk) { return (unsigned)(((nblk + 7) / 8) * 8); }
 
+// (order m, batch group g) blocks of the tensor Legendre kernels: XCD x
+// takes the orders m = x, x + 8, ... (balanced work, since a block costs
+// ~ L - m + 1) and runs all batch groups of one order back to back, so the
+// order's P/dP table rows are fetched into that XCD's L2 once.
+__device__ __forceinline__ bool xcd_order_block(int ngrp, int L, int& m, int& g) {
+    const int x = (int)(blockIdx.x % 8), r = (int)(blockIdx.x / 8);
+    m = x + 8 * (r / ngrp);
+    g = r % ngrp;
+    return m <= L;
+}
+static inline unsigned xcd_order_grid(int ngrp, int L) { return (unsigned)(8 * ngrp * ((L + 1 + 7) / 8)); }
+
 // ---------------------------------------------------------- synthesis
 // Block = (order m, TB consecutive batches); lane pair (2j, 2j+1) = north
 // latitude j, the even / odd l+m half of the degree sum. The P/dP table rows
@@ -56,6 +68,8 @@ __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __re
     extern __shared__ double2 cs[];   // [TB][5][nl]: c1, c2, -J c5, c6, -J c6
     const int L = s.L, nh = s.nh, nlat = s.nlat, nlm = s.nlm;
     const int ngrp = (nb + TB - 1) / TB;
+    // orders fastest within a batch group: neighbouring orders share the
+    // degree-major coefficient lines in L2
     const int lb = xcd_block(ngrp * (L + 1));
     if (lb >= ngrp * (L + 1)) return;
     const int b0 = (lb / (L + 1)) * TB, m = lb % (L + 1), nl = L - m + 1;
@@ -148,9 +162,9 @@ __global__ void __launch_bounds__(128) k_tsh_analysis(vh_sht s, const double2* _
     extern __shared__ double2 fs[];   // [TB][7][2][nh]: f1, f2P, f5, fD, fb (P-type), f2D, fd (dP-type)
     const int L = s.L, nh = s.nh, nlat = s.nlat, nlm = s.nlm;
     const int ngrp = (nb + TB - 1) / TB;
-    const int lb = xcd_block(ngrp * (L + 1));
-    if (lb >= ngrp * (L + 1)) return;
-    const int b0 = (lb / (L + 1)) * TB, m = lb % (L + 1), nl = L - m + 1;
+    int m, g;
+    if (!xcd_order_block(ngrp, L, m, g)) return;
+    const int b0 = g * TB, nl = L - m + 1;
     const int nbb = min(TB, nb - b0);
     const double ms = (double)m;
     for (int t = threadIdx.x; t < TB * nh; t += blockDim.x) {
@@ -381,7 +395,7 @@ void vh_tsh_synth(hipStream_t st, const vh_sht& s, const double2* X, int nb, dou
 void vh_tsh_analysis(hipStream_t st, const vh_sht& s, const double2* Fd, int nb, const double* n6, double2* C) {
     if (nb <= 0) return;
     const int ngrp = (nb + VH_TSH_TB_ANAL - 1) / VH_TSH_TB_ANAL;
-    k_tsh_analysis<VH_TSH_TB_ANAL><<<xcd_grid(ngrp * (s.L + 1)), 128, anal_lds(s), st>>>(s, Fd, nb, n6, C);
+    k_tsh_analysis<VH_TSH_TB_ANAL><<<xcd_order_grid(ngrp, s.L), 128, anal_lds(s), st>>>(s, Fd, nb, n6, C);
     HIPCHECK(hipGetLastError());
 }
 
