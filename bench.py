@@ -166,6 +166,14 @@ def main():
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches": int(cnt[best]),
                 "alg_bytes_per_launch": int(alg_bytes(d, best, nbat / nch if nbat else None))}
+    if roof is not None:
+        # HBM bytes per launch of this kernel from the committed PMC summary
+        # (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration)
+        tf = os.path.join(ROOT, "profiles", "c%d_lmax%d_traffic.json" % (a.config, a.lmax))
+        if os.path.exists(tf):
+            t = json.load(open(tf))["kernels"].get(roof["kernel"])
+            if t:
+                roof["traffic"] = t["hbm_bytes_per_launch"]
     kernel_ms = {NAMES[k]: round(ms[k], 3) for k in NAMES if cnt[k] > 0}
 
     cpu = None

@@ -30,6 +30,35 @@ def _compile(src):
     return obj
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN = os.path.join(HERE, "fortran", "vilma.f90")
+DRIVER_SRC = os.path.join(HERE, "..", "tests", "fortran", "drive.f90")
+DRIVER = os.path.join(OBJ, "vilma_drive")
+
+
+def build_fortran(verbose=False):
+    """Compile the Fortran host module `vilma` (object + .mod in _build/) and
+    link the test driver against libvilma_hip.so (rpath relative to itself).
+    Skipped, with a message, where no flang is installed."""
+    if not os.path.exists(FLANG):
+        if verbose:
+            print("flang not found: Fortran module not built")
+        return None
+    os.makedirs(OBJ, exist_ok=True)
+    obj = os.path.join(OBJ, "vilma_mod.o")
+    cmds = [[FLANG, "-O2", "-fPIC", "-module-dir", OBJ, "-c", FORTRAN, "-o", obj]]
+    if os.path.exists(DRIVER_SRC):
+        cmds.append([FLANG, "-O2", "-I", OBJ, DRIVER_SRC, obj, "-L", HERE, "-lvilma_hip",
+                     "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER])
+    for cmd in cmds:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("flang failed:\n%s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
+    if verbose:
+        print("built", obj, DRIVER)
+    return DRIVER
+
+
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
@@ -43,6 +72,7 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_fortran(verbose)
     return LIB
 
 

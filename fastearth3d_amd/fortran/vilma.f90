@@ -1,0 +1,437 @@
+!! vilma.f90 -- Fortran host module `vilma` over the MI355X C-ABI (include/vilma_hip.h).
+!!
+!! A drop-in for the reference's `use vilma` (src/vilma.f90:6-19) on the path a
+!! host such as CLIMBER-X drives: the parameter record and its namelist loader
+!! (vilma_param_class / vilma_par_load, src/vilma_params.f90:26-205, :209-331),
+!! and the solid_earth type with init / update / finalize and the lateral
+!! viscosity switch (src/vilma_coupling.f90:71-140, :144, :320, :336, :599).
+!! Same names, argument order and output components; a nonzero status from the
+!! library becomes `error stop <message>` as in the reference.
+!!
+!! Differences at this boundary: the optional `grid` argument of
+!! solid_earth_init (coords::grid_class remapping) is not supported, only the
+!! passthrough case (fields already on the model's Gauss grid); the lateral
+!! viscosity comes in as an array (solid_earth_enable_visc_3d_field /
+!! _nodes) instead of being read from netCDF; vilma_par_load reads the group
+!! "vilma" with the intrinsic NAMELIST facility (overlay of `filename` on
+!! `defaults_file`, dt_* in years converted to seconds).
+module vilma
+   use, intrinsic :: iso_c_binding
+   implicit none
+   private
+
+   public :: wp, MAX_LAYER, vilma_param_class, vilma_par_load
+   public :: solid_earth, solid_earth_init, solid_earth_update, solid_earth_finalize
+   public :: solid_earth_enable_visc_3d_field, solid_earth_enable_visc_3d_nodes
+
+   integer, parameter :: wp = c_double
+   integer, parameter :: MAX_LAYER = 16
+   real(wp), parameter :: sec_per_year = 31556926.0_wp
+
+   !! the parameter record (field names and defaults of src/vilma_params.f90:26-205)
+   type :: vilma_param_class
+      character(len=16) :: solver = "v2"
+      integer  :: lmax = 0, nlat = 0, nphi = 0, mmax = -1, mres = 1
+      real(wp) :: eps_polar = -1.0_wp
+      character(len=64) :: earth = "M3-L70-V01"
+      integer  :: n_layer = 0
+      real(wp) :: r_earth = 6371.0e3_wp, r_core = 3480.0e3_wp
+      real(wp) :: r_bot(MAX_LAYER) = 0.0_wp, r_top(MAX_LAYER) = 0.0_wp
+      real(wp) :: rho(MAX_LAYER) = 0.0_wp, mu(MAX_LAYER) = 0.0_wp, eta(MAX_LAYER) = 0.0_wp
+      integer  :: rheology(MAX_LAYER) = 1
+      integer  :: n_elem_layer(MAX_LAYER) = 0       !! additive: explicit elements per layer
+      character(len=8) :: scheme = "fe"
+      integer  :: max_couple_iter = 20
+      character(len=8) :: earth_response = "ve"
+      integer  :: sle_n_outer = 3, sle_n_inner = 20
+      real(wp) :: sle_tol = 1.0e-7_wp
+      integer  :: sle_max_mem_iter = 20
+      logical  :: sle_fixed_ocean = .false., sle_subgrid = .true.
+      real(wp) :: dt_init = 0.0_wp, dt_min = 0.0_wp, dt_max = huge(1.0_wp)   !! seconds
+      real(wp) :: rtol = 1.0e-4_wp, atol = 1.0e-3_wp, safety = 0.9_wp
+      real(wp) :: grow_max = 5.0_wp, shrink_min = 0.2_wp, cfl = 1.0_wp
+      logical  :: rotation = .true.
+      real(wp) :: equil_time_max = 0.0_wp, equil_rate_tol = 1.0e-3_wp
+      logical  :: pre_spinup_1d = .false.
+      character(len=8) :: deg1_frame = "cm"
+      logical  :: l_visc_3d = .false.
+      character(len=512) :: visc_3d_file = ""
+      character(len=64) :: name_visc = "eta", name_visc_lon = "lon", name_visc_lat = "lat"
+      character(len=64) :: name_visc_r = "r", name_visc_sd = ""
+      real(wp) :: f_visc_sd = 0.0_wp, f_visc_rel = 0.1_wp
+      real(wp) :: visc_log10_min = 19.5_wp, visc_log10_max = 30.0_wp
+      real(wp) :: visc3d_lid_depth = 0.0_wp, visc3d_lid_log10max = 22.0_wp
+      real(wp) :: visc3d_tol = 1.0e-3_wp
+      logical  :: l_toroidal = .false.
+      real(wp) :: skip_tol = 1.0e-4_wp                 !! additive: drift skip gate
+      ! VILMA-v1 backend settings: accepted so reference namelists load, inert here
+      integer  :: vilma_v1_jmax = 170, vilma_v1_l_prem = 1, vilma_v1_nsub = 1
+      character(len=512) :: vilma_v1_input_dir = "input/vilma", vilma_v1_out_dir = "vilma_v1"
+      character(len=512) :: vilma_v1_grid_file = "input/vilma/vilma_grid.nc"
+      character(len=128) :: vilma_v1_visc_1d_file = "visko.inp", vilma_v1_visc_3d_file = "visc3d_Bagge2021.nc"
+   end type vilma_param_class
+
+   !! C mirror of vh_params (include/vilma_hip.h)
+   type, bind(c) :: vh_params_c
+      integer(c_int) :: lmax, nlat, nphi, mmax, mres
+      real(c_double) :: eps_polar
+      character(kind=c_char) :: earth(64)
+      integer(c_int) :: n_layer
+      real(c_double) :: r_earth, r_core
+      real(c_double) :: r_bot(MAX_LAYER), r_top(MAX_LAYER)
+      real(c_double) :: rho(MAX_LAYER), mu(MAX_LAYER), eta(MAX_LAYER)
+      integer(c_int) :: rheology(MAX_LAYER)
+      integer(c_int) :: n_elem_layer(MAX_LAYER)
+      character(kind=c_char) :: scheme(8)
+      integer(c_int) :: max_couple_iter
+      character(kind=c_char) :: earth_response(8)
+      integer(c_int) :: sle_n_outer, sle_n_inner
+      real(c_double) :: sle_tol
+      integer(c_int) :: sle_max_mem_iter, sle_fixed_ocean, sle_subgrid
+      real(c_double) :: dt_init, dt_min, dt_max
+      real(c_double) :: rtol, atol, safety, grow_max, shrink_min, cfl
+      integer(c_int) :: rotation
+      character(kind=c_char) :: deg1_frame(8)
+      integer(c_int) :: l_visc_3d
+      real(c_double) :: visc3d_tol
+      integer(c_int) :: l_toroidal
+      real(c_double) :: visc_log10_min, visc_log10_max, visc3d_lid_depth, visc3d_lid_log10max
+      real(c_double) :: skip_tol
+   end type vh_params_c
+
+   !! the Gauss-grid state the host reads (se%gg%..., src/vilma_coupling.f90:71-140)
+   type :: gauss_grid_fields
+      real(wp), allocatable :: rsl(:,:), z_bed(:,:), C(:,:), C0(:,:), h_ice(:,:), s_rot(:,:)
+   end type gauss_grid_fields
+
+   type :: solid_earth
+      type(vilma_param_class) :: par
+      real(wp), allocatable :: rsl(:,:), z_bed(:,:)
+      real(wp) :: bsl = 0.0_wp, time = 0.0_wp, worst_mass_resid = 0.0_wp
+      complex(wp), allocatable :: sigma_lm(:)
+      type(gauss_grid_fields) :: gg
+      type(c_ptr) :: h = c_null_ptr
+      integer :: nphi = 0, nlat = 0
+   end type solid_earth
+
+   interface
+      integer(c_int) function vh_create(p, nphi, nlat, z, hi, hi0, out) bind(c, name="vh_create")
+         import :: c_int, c_ptr, vh_params_c
+         type(vh_params_c), intent(in) :: p
+         integer(c_int), value :: nphi, nlat
+         type(c_ptr), value :: z, hi, hi0
+         type(c_ptr), intent(out) :: out
+      end function
+      integer(c_int) function vh_update(h, hi, dt_yr) bind(c, name="vh_update")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h, hi
+         real(c_double), value :: dt_yr
+      end function
+      integer(c_int) function vh_get_field(h, which, out) bind(c, name="vh_get_field")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h, out
+         integer(c_int), value :: which
+      end function
+      integer(c_int) function vh_get_scalar(h, which, out) bind(c, name="vh_get_scalar")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h
+         integer(c_int), value :: which
+         real(c_double), intent(out) :: out
+      end function
+      integer(c_int) function vh_get_spectrum(h, which, out) bind(c, name="vh_get_spectrum")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h, out
+         integer(c_int), value :: which
+      end function
+      integer(c_int) function vh_get_dims(h, which, out) bind(c, name="vh_get_dims")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+         integer(c_int), value :: which
+         integer(c_int), intent(out) :: out
+      end function
+      integer(c_int) function vh_enable_visc_3d(h, pert) bind(c, name="vh_enable_visc_3d")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h, pert
+      end function
+      integer(c_int) function vh_enable_visc_3d_nodes(h, visc, lid_depth, lid_max, cap) &
+            bind(c, name="vh_enable_visc_3d_nodes")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h, visc
+         real(c_double), value :: lid_depth, lid_max, cap
+      end function
+      integer(c_int) function vh_destroy(h) bind(c, name="vh_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h
+      end function
+      type(c_ptr) function vh_last_error(h) bind(c, name="vh_last_error")
+         import :: c_ptr
+         type(c_ptr), value :: h
+      end function
+      integer(c_size_t) function c_strlen(s) bind(c, name="strlen")
+         import :: c_size_t, c_ptr
+         type(c_ptr), value :: s
+      end function
+   end interface
+
+contains
+
+   !! vilma_par_load (src/vilma_params.f90:209-331): `filename`'s &vilma group
+   !! overlaid on `defaults_file`'s (each defaults to the other); dt_init,
+   !! dt_min, dt_max and equil_time_max are years in the file, seconds in p.
+   subroutine vilma_par_load(p, filename, defaults_file, group)
+      type(vilma_param_class), intent(inout) :: p
+      character(len=*), intent(in) :: filename
+      character(len=*), intent(in), optional :: defaults_file, group
+      character(len=16) :: solver
+      integer  :: lmax, nlat, nphi, mmax, mres, n_layer, rheology(MAX_LAYER), n_elem_layer(MAX_LAYER)
+      integer  :: max_couple_iter, sle_n_outer, sle_n_inner, sle_max_mem_iter
+      integer  :: vilma_v1_jmax, vilma_v1_l_prem, vilma_v1_nsub
+      real(wp) :: eps_polar, r_earth, r_core, r_bot(MAX_LAYER), r_top(MAX_LAYER)
+      real(wp) :: rho(MAX_LAYER), mu(MAX_LAYER), eta(MAX_LAYER), sle_tol
+      real(wp) :: dt_init, dt_min, dt_max, rtol, atol, safety, grow_max, shrink_min, cfl
+      real(wp) :: equil_time_max, equil_rate_tol, f_visc_sd, f_visc_rel, visc_log10_min, visc_log10_max
+      real(wp) :: visc3d_lid_depth, visc3d_lid_log10max, visc3d_tol, skip_tol
+      character(len=64) :: earth, name_visc, name_visc_lon, name_visc_lat, name_visc_r, name_visc_sd
+      character(len=8)  :: scheme, earth_response, deg1_frame
+      character(len=512) :: visc_3d_file, vilma_v1_input_dir, vilma_v1_out_dir, vilma_v1_grid_file
+      character(len=128) :: vilma_v1_visc_1d_file, vilma_v1_visc_3d_file
+      logical  :: sle_fixed_ocean, sle_subgrid, rotation, pre_spinup_1d, l_visc_3d, l_toroidal
+      namelist /vilma/ solver, lmax, nlat, nphi, mmax, mres, eps_polar, earth, n_layer, r_earth, r_core, &
+         r_bot, r_top, rho, mu, eta, rheology, n_elem_layer, scheme, max_couple_iter, earth_response, &
+         sle_n_outer, sle_n_inner, sle_tol, sle_max_mem_iter, sle_fixed_ocean, sle_subgrid, &
+         dt_init, dt_min, dt_max, rtol, atol, safety, grow_max, shrink_min, cfl, rotation, &
+         equil_time_max, equil_rate_tol, pre_spinup_1d, deg1_frame, l_visc_3d, visc_3d_file, &
+         name_visc, name_visc_lon, name_visc_lat, name_visc_r, name_visc_sd, f_visc_sd, f_visc_rel, &
+         visc_log10_min, visc_log10_max, visc3d_lid_depth, visc3d_lid_log10max, visc3d_tol, &
+         l_toroidal, skip_tol, vilma_v1_jmax, vilma_v1_input_dir, vilma_v1_out_dir, vilma_v1_grid_file, &
+         vilma_v1_visc_1d_file, vilma_v1_visc_3d_file, vilma_v1_l_prem, vilma_v1_nsub
+
+      if (present(group)) then
+         if (trim(group) /= "vilma") error stop 'vilma_par_load: only the namelist group "vilma" is supported'
+      end if
+      solver = p%solver; lmax = p%lmax; nlat = p%nlat; nphi = p%nphi; mmax = p%mmax; mres = p%mres
+      eps_polar = p%eps_polar; earth = p%earth; n_layer = p%n_layer; r_earth = p%r_earth; r_core = p%r_core
+      r_bot = p%r_bot; r_top = p%r_top; rho = p%rho; mu = p%mu; eta = p%eta; rheology = p%rheology
+      n_elem_layer = p%n_elem_layer; scheme = p%scheme; max_couple_iter = p%max_couple_iter
+      earth_response = p%earth_response; sle_n_outer = p%sle_n_outer; sle_n_inner = p%sle_n_inner
+      sle_tol = p%sle_tol; sle_max_mem_iter = p%sle_max_mem_iter; sle_fixed_ocean = p%sle_fixed_ocean
+      sle_subgrid = p%sle_subgrid
+      dt_init = p%dt_init / sec_per_year; dt_min = p%dt_min / sec_per_year; dt_max = p%dt_max / sec_per_year
+      rtol = p%rtol; atol = p%atol; safety = p%safety; grow_max = p%grow_max; shrink_min = p%shrink_min
+      cfl = p%cfl; rotation = p%rotation
+      equil_time_max = p%equil_time_max / sec_per_year; equil_rate_tol = p%equil_rate_tol
+      pre_spinup_1d = p%pre_spinup_1d; deg1_frame = p%deg1_frame; l_visc_3d = p%l_visc_3d
+      visc_3d_file = p%visc_3d_file; name_visc = p%name_visc; name_visc_lon = p%name_visc_lon
+      name_visc_lat = p%name_visc_lat; name_visc_r = p%name_visc_r; name_visc_sd = p%name_visc_sd
+      f_visc_sd = p%f_visc_sd; f_visc_rel = p%f_visc_rel; visc_log10_min = p%visc_log10_min
+      visc_log10_max = p%visc_log10_max; visc3d_lid_depth = p%visc3d_lid_depth
+      visc3d_lid_log10max = p%visc3d_lid_log10max; visc3d_tol = p%visc3d_tol; l_toroidal = p%l_toroidal
+      skip_tol = p%skip_tol; vilma_v1_jmax = p%vilma_v1_jmax; vilma_v1_input_dir = p%vilma_v1_input_dir
+      vilma_v1_out_dir = p%vilma_v1_out_dir; vilma_v1_grid_file = p%vilma_v1_grid_file
+      vilma_v1_visc_1d_file = p%vilma_v1_visc_1d_file; vilma_v1_visc_3d_file = p%vilma_v1_visc_3d_file
+      vilma_v1_l_prem = p%vilma_v1_l_prem; vilma_v1_nsub = p%vilma_v1_nsub
+
+      if (present(defaults_file)) call read_group(defaults_file)
+      call read_group(filename)
+
+      if (trim(deg1_frame) /= "cf" .and. trim(deg1_frame) /= "cm") &
+         error stop 'vilma_params: deg1_frame must be "cf" or "cm"'
+      if (vilma_v1_l_prem /= 0 .and. vilma_v1_l_prem /= 1) error stop 'vilma_params: vilma_v1_l_prem must be 0 or 1'
+      if (vilma_v1_nsub < 1) error stop 'vilma_params: vilma_v1_nsub must be >= 1'
+
+      p%solver = solver; p%lmax = lmax; p%nlat = nlat; p%nphi = nphi; p%mmax = mmax; p%mres = mres
+      p%eps_polar = eps_polar; p%earth = earth; p%n_layer = n_layer; p%r_earth = r_earth; p%r_core = r_core
+      p%r_bot = r_bot; p%r_top = r_top; p%rho = rho; p%mu = mu; p%eta = eta; p%rheology = rheology
+      p%n_elem_layer = n_elem_layer; p%scheme = scheme; p%max_couple_iter = max_couple_iter
+      p%earth_response = earth_response; p%sle_n_outer = sle_n_outer; p%sle_n_inner = sle_n_inner
+      p%sle_tol = sle_tol; p%sle_max_mem_iter = sle_max_mem_iter; p%sle_fixed_ocean = sle_fixed_ocean
+      p%sle_subgrid = sle_subgrid
+      p%dt_init = dt_init * sec_per_year; p%dt_min = dt_min * sec_per_year; p%dt_max = dt_max * sec_per_year
+      p%rtol = rtol; p%atol = atol; p%safety = safety; p%grow_max = grow_max; p%shrink_min = shrink_min
+      p%cfl = cfl; p%rotation = rotation
+      p%equil_time_max = equil_time_max * sec_per_year; p%equil_rate_tol = equil_rate_tol
+      p%pre_spinup_1d = pre_spinup_1d; p%deg1_frame = deg1_frame; p%l_visc_3d = l_visc_3d
+      p%visc_3d_file = visc_3d_file; p%name_visc = name_visc; p%name_visc_lon = name_visc_lon
+      p%name_visc_lat = name_visc_lat; p%name_visc_r = name_visc_r; p%name_visc_sd = name_visc_sd
+      p%f_visc_sd = f_visc_sd; p%f_visc_rel = f_visc_rel; p%visc_log10_min = visc_log10_min
+      p%visc_log10_max = visc_log10_max; p%visc3d_lid_depth = visc3d_lid_depth
+      p%visc3d_lid_log10max = visc3d_lid_log10max; p%visc3d_tol = visc3d_tol; p%l_toroidal = l_toroidal
+      p%skip_tol = skip_tol; p%vilma_v1_jmax = vilma_v1_jmax; p%vilma_v1_input_dir = vilma_v1_input_dir
+      p%vilma_v1_out_dir = vilma_v1_out_dir; p%vilma_v1_grid_file = vilma_v1_grid_file
+      p%vilma_v1_visc_1d_file = vilma_v1_visc_1d_file; p%vilma_v1_visc_3d_file = vilma_v1_visc_3d_file
+      p%vilma_v1_l_prem = vilma_v1_l_prem; p%vilma_v1_nsub = vilma_v1_nsub
+
+   contains
+
+      subroutine read_group(fname)
+         character(len=*), intent(in) :: fname
+         integer :: u, ios
+         character(len=256) :: msg
+         open(newunit=u, file=fname, status="old", action="read", iostat=ios, iomsg=msg)
+         if (ios /= 0) then
+            write(*, '(a)') 'vilma_par_load: cannot open ' // trim(fname) // ': ' // trim(msg)
+            error stop 'vilma_par_load: cannot open the parameter file'
+         end if
+         read(u, nml=vilma, iostat=ios, iomsg=msg)
+         close(u)
+         if (ios /= 0) then
+            write(*, '(a)') 'vilma_par_load: ' // trim(fname) // ': ' // trim(msg)
+            error stop 'vilma_par_load: bad &vilma group'
+         end if
+      end subroutine read_group
+
+   end subroutine vilma_par_load
+
+   subroutine put_str(dst, src)
+      character(kind=c_char), intent(out) :: dst(:)
+      character(len=*), intent(in) :: src
+      integer :: i, n
+      dst = c_null_char
+      n = min(len_trim(src), size(dst) - 1)
+      do i = 1, n
+         dst(i) = src(i:i)
+      end do
+   end subroutine put_str
+
+   function to_c(p) result(c)
+      type(vilma_param_class), intent(in) :: p
+      type(vh_params_c) :: c
+      c%lmax = p%lmax; c%nlat = p%nlat; c%nphi = p%nphi; c%mmax = p%mmax; c%mres = p%mres
+      c%eps_polar = p%eps_polar
+      call put_str(c%earth, p%earth)
+      c%n_layer = p%n_layer; c%r_earth = p%r_earth; c%r_core = p%r_core
+      c%r_bot = p%r_bot; c%r_top = p%r_top; c%rho = p%rho; c%mu = p%mu; c%eta = p%eta
+      c%rheology = p%rheology; c%n_elem_layer = p%n_elem_layer
+      call put_str(c%scheme, p%scheme)
+      c%max_couple_iter = p%max_couple_iter
+      call put_str(c%earth_response, p%earth_response)
+      c%sle_n_outer = p%sle_n_outer; c%sle_n_inner = p%sle_n_inner; c%sle_tol = p%sle_tol
+      c%sle_max_mem_iter = p%sle_max_mem_iter
+      c%sle_fixed_ocean = merge(1, 0, p%sle_fixed_ocean); c%sle_subgrid = merge(1, 0, p%sle_subgrid)
+      c%dt_init = p%dt_init; c%dt_min = p%dt_min; c%dt_max = p%dt_max
+      c%rtol = p%rtol; c%atol = p%atol; c%safety = p%safety; c%grow_max = p%grow_max
+      c%shrink_min = p%shrink_min; c%cfl = p%cfl
+      c%rotation = merge(1, 0, p%rotation)
+      call put_str(c%deg1_frame, p%deg1_frame)
+      c%l_visc_3d = merge(1, 0, p%l_visc_3d); c%visc3d_tol = p%visc3d_tol
+      c%l_toroidal = merge(1, 0, p%l_toroidal)
+      c%visc_log10_min = p%visc_log10_min; c%visc_log10_max = p%visc_log10_max
+      c%visc3d_lid_depth = p%visc3d_lid_depth; c%visc3d_lid_log10max = p%visc3d_lid_log10max
+      c%skip_tol = p%skip_tol
+   end function to_c
+
+   !! nonzero status -> error stop with the library's message (the reference's convention)
+   subroutine check(self, rc, what)
+      type(solid_earth), intent(in) :: self
+      integer(c_int), intent(in) :: rc
+      character(len=*), intent(in) :: what
+      type(c_ptr) :: m
+      character(kind=c_char), pointer :: s(:)
+      character(len=:), allocatable :: msg
+      integer :: i, n
+      if (rc == 0) return
+      m = vh_last_error(self%h)
+      n = int(c_strlen(m))
+      call c_f_pointer(m, s, [n])
+      allocate(character(len=n) :: msg)
+      do i = 1, n
+         msg(i:i) = s(i)
+      end do
+      write(*, '(a)') trim(what) // ': ' // msg
+      error stop 'vilma: ' // what // ' failed (message above)'
+   end subroutine check
+
+   subroutine refresh(self)
+      type(solid_earth), intent(inout), target :: self
+      real(wp) :: v
+      call check(self, vh_get_field(self%h, 0, c_loc(self%gg%rsl)), "solid_earth_update")
+      call check(self, vh_get_field(self%h, 1, c_loc(self%gg%z_bed)), "solid_earth_update")
+      call check(self, vh_get_field(self%h, 2, c_loc(self%gg%C)), "solid_earth_update")
+      call check(self, vh_get_field(self%h, 4, c_loc(self%gg%s_rot)), "solid_earth_update")
+      call check(self, vh_get_field(self%h, 5, c_loc(self%gg%h_ice)), "solid_earth_update")
+      call check(self, vh_get_spectrum(self%h, 0, c_loc(self%sigma_lm)), "solid_earth_update")
+      self%rsl = self%gg%rsl
+      self%z_bed = self%gg%z_bed
+      call check(self, vh_get_scalar(self%h, 0, v), "solid_earth_update"); self%time = v
+      call check(self, vh_get_scalar(self%h, 1, v), "solid_earth_update"); self%bsl = v
+      call check(self, vh_get_scalar(self%h, 2, v), "solid_earth_update"); self%worst_mass_resid = v
+   end subroutine refresh
+
+   !! solid_earth_init (src/vilma_coupling.f90:144-232), passthrough grid only;
+   !! the caller sets self%par (e.g. by vilma_par_load) first.
+   subroutine solid_earth_init(self, z_bed_eq, h_ice_eq, grid, h_ice_init)
+      type(solid_earth), intent(inout), target :: self
+      real(wp), intent(in), target :: z_bed_eq(:,:), h_ice_eq(:,:)
+      class(*), intent(in), optional :: grid
+      real(wp), intent(in), target, optional :: h_ice_init(:,:)
+      type(vh_params_c), target :: c
+      type(c_ptr) :: hi0
+      integer(c_int) :: nlm
+      real(wp), allocatable, target :: z(:,:), hi(:,:), h0(:,:)
+      if (present(grid)) error stop 'solid_earth_init: grid remapping is not supported (fields must be on the Gauss grid)'
+      self%nphi = size(z_bed_eq, 1); self%nlat = size(z_bed_eq, 2)
+      c = to_c(self%par)
+      z = z_bed_eq; hi = h_ice_eq
+      hi0 = c_null_ptr
+      if (present(h_ice_init)) then
+         h0 = h_ice_init
+         hi0 = c_loc(h0)
+      end if
+      self%h = c_null_ptr
+      call check(self, vh_create(c, int(self%nphi, c_int), int(self%nlat, c_int), c_loc(z), c_loc(hi), hi0, &
+                                 self%h), "solid_earth_init")
+      call check(self, vh_get_dims(self%h, 3, nlm), "solid_earth_init")
+      allocate(self%rsl(self%nphi, self%nlat), self%z_bed(self%nphi, self%nlat), self%sigma_lm(nlm))
+      allocate(self%gg%rsl(self%nphi, self%nlat), self%gg%z_bed(self%nphi, self%nlat), &
+               self%gg%C(self%nphi, self%nlat), self%gg%C0(self%nphi, self%nlat), &
+               self%gg%h_ice(self%nphi, self%nlat), self%gg%s_rot(self%nphi, self%nlat))
+      call check(self, vh_get_field(self%h, 3, c_loc(self%gg%C0)), "solid_earth_init")
+      call refresh(self)
+   end subroutine solid_earth_init
+
+   !! solid_earth_update (src/vilma_coupling.f90:336-437)
+   subroutine solid_earth_update(self, h_ice, dt_yr)
+      type(solid_earth), intent(inout), target :: self
+      real(wp), intent(in) :: h_ice(:,:)
+      real(wp), intent(in) :: dt_yr
+      real(wp), allocatable, target :: hi(:,:)
+      if (.not. c_associated(self%h)) error stop 'solid_earth_update: model not initialised'
+      if (size(h_ice, 1) /= self%nphi .or. size(h_ice, 2) /= self%nlat) &
+         error stop 'solid_earth_update: h_ice must be on the model grid (nphi, nlat)'
+      hi = h_ice
+      call check(self, vh_update(self%h, c_loc(hi), real(dt_yr, c_double)), "solid_earth_update")
+      call refresh(self)
+   end subroutine solid_earth_update
+
+   !! solid_earth_enable_visc_3d (src/vilma_coupling.f90:320-334) with the
+   !! lateral field given in memory: log10-viscosity perturbation per element,
+   !! (nphi, nlat, ne) (response_enable_lateral_visc, src/vilma_response.f90:994-1053)
+   subroutine solid_earth_enable_visc_3d_field(self, pert_elem)
+      type(solid_earth), intent(inout) :: self
+      real(wp), intent(in), target, contiguous :: pert_elem(:,:,:)
+      call check(self, vh_enable_visc_3d(self%h, c_loc(pert_elem)), "solid_earth_enable_visc_3d")
+   end subroutine solid_earth_enable_visc_3d_field
+
+   !! the same from absolute nodal log10(eta), (nphi*nlat, nr)
+   !! (response_enable_lateral_visc_from_nodes, src/vilma_response.f90:1120-1169)
+   subroutine solid_earth_enable_visc_3d_nodes(self, visc_node)
+      type(solid_earth), intent(inout) :: self
+      real(wp), intent(in), target, contiguous :: visc_node(:,:)
+      call check(self, vh_enable_visc_3d_nodes(self%h, c_loc(visc_node), self%par%visc3d_lid_depth, &
+                 self%par%visc3d_lid_log10max, self%par%visc_log10_max), "solid_earth_enable_visc_3d")
+   end subroutine solid_earth_enable_visc_3d_nodes
+
+   !! solid_earth_finalize (src/vilma_coupling.f90:599-626)
+   subroutine solid_earth_finalize(self)
+      type(solid_earth), intent(inout) :: self
+      integer(c_int) :: rc
+      if (c_associated(self%h)) then
+         rc = vh_destroy(self%h)
+         self%h = c_null_ptr
+      end if
+      if (allocated(self%rsl)) deallocate(self%rsl, self%z_bed, self%sigma_lm)
+      if (allocated(self%gg%rsl)) deallocate(self%gg%rsl, self%gg%z_bed, self%gg%C, self%gg%C0, &
+                                             self%gg%h_ice, self%gg%s_rot)
+   end subroutine solid_earth_finalize
+
+end module vilma

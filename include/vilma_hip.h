@@ -3,7 +3,7 @@
  *
  * This header is the drop-in boundary. A host (the Fortran shim module
  * `vilma` in fastearth3d_amd/fortran/vilma.f90, the Python mirror in
- * fastearth3d_amd/__init__.py, or any C/C++ caller) drives the model through
+ * fastearth3d_amd/lib.py, or any C/C++ caller) drives the model through
  * plain pointers and sizes; no torch or HIP type appears in a signature.
  *
  * Conventions (all FP64):
