@@ -94,7 +94,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--lmax", type=int, default=128)
-    ap.add_argument("--cpu-calls", type=int, default=1)
+    ap.add_argument("--cpu-calls", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 times): the reference "
