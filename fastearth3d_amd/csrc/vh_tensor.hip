@@ -151,103 +151,175 @@ __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __re
 }
 
 // ----------------------------------------------------------- analysis
-// Block = (order m, TB consecutive batches). Stage 1 folds the six dyads of
-// the latitude pair (j, nlat-1-j) into the seven weighted partial fields
-// (even/odd l+m); stage 2: thread = degree, quadrature over the north
-// latitudes, the table row loaded once for the TB batches. n6 == null
-// returns the raw spin-2 adjoint (calibration).
-template <int TB>
-__global__ void __launch_bounds__(128) k_tsh_analysis(vh_sht s, const double2* __restrict__ Fd, int nb,
+// FP64 MFMA form. For one order m and degree parity p the analysis is the
+// product  S[l][col] = sum_j T[l][j] B[j][col]  with T = P or dP over the north
+// latitudes and B the latitude pair's folded, weighted fields: per batch five
+// P-type complex fields (f1, f2P, f5, fD, fb: 10 real columns) and two dP-type
+// (f2D, fd: 4 columns). Block = (order m, 4 batches): P-type columns 0..39
+// (three 16-wide tiles, 40..47 zero), dP-type 48..63 (one tile). Latitudes
+// are taken 16 at a time: the block folds them into LDS (B), then every
+// wavefront runs its (parity, 16-degree) row tiles through
+// v_mfma_f64_16x16x4 (lane l: A[l%16][l/16], B[l/16][l%16], D[4r+l/16][l%16])
+// with the P/dP fragments read straight from the [m][j][l] tables. The
+// per-(degree, batch) normalisations follow from the accumulated tiles.
+typedef double vh_d4 __attribute__((ext_vector_type(4)));
+#define TA_B 4          // batches per block
+#ifndef TA_JC
+#define TA_JC 32        // latitudes per LDS chunk
+#endif
+#define TA_ITEMS 3      // row tiles per wavefront (4 wavefronts, <= 10 tiles)
+
+__global__ void __launch_bounds__(256) k_tsh_analysis(vh_sht s, const double2* __restrict__ Fd, int nb,
                                                       const double* __restrict__ n6, double2* __restrict__ C) {
-    extern __shared__ double2 fs[];   // [TB][7][2][nh]: f1, f2P, f5, fD, fb (P-type), f2D, fd (dP-type)
+    __shared__ double Bs[2][TA_JC][64];           // [parity][latitude in chunk][column]
+    __shared__ double Ds[4][16][65];              // per-wavefront staging of one row tile
     const int L = s.L, nh = s.nh, nlat = s.nlat, nlm = s.nlm;
-    const int ngrp = (nb + TB - 1) / TB;
+    const int ngrp = (nb + TA_B - 1) / TA_B;
     int m, g;
     if (!xcd_order_block(ngrp, L, m, g)) return;
-    const int b0 = g * TB, nl = L - m + 1;
-    const int nbb = min(TB, nb - b0);
+    const int b0 = g * TA_B, nl = L - m + 1, nbb = min(TA_B, nb - b0);
+    const int ne = (nl + 1) / 2, no = nl / 2;
+    const int nte = (ne + 15) / 16, nto = (no + 15) / 16, nitems = nte + nto;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const double ms = (double)m;
-    for (int t = threadIdx.x; t < TB * nh; t += blockDim.x) {
-        const int bb = t / nh, j = t % nh;
-        double2* fb_ = fs + (size_t)bb * 14 * nh;
-        double2 f[2][7];
-#pragma unroll
-        for (int hemi = 0; hemi < 2; hemi++) {
-            const int row = hemi == 0 ? j : nlat - 1 - j;
-            if (bb >= nbb || (hemi == 1 && row == j)) {
-#pragma unroll
-                for (int q = 0; q < 7; q++) f[hemi][q] = make_double2(0.0, 0.0);
-                continue;
-            }
-            double2 d[6];
-#pragma unroll
-            for (int dy = 0; dy < 6; dy++) d[dy] = Fd[(((size_t)(b0 + bb) * 6 + dy) * (L + 1) + m) * nlat + row];
-            const double w = s.w[row], st = s.st[row], cot = s.ct[row] / st, is = 1.0 / st;
-            const double mis = ms * is;
-            const double2 Dv = make_double2(d[3].x - d[5].x, d[3].y - d[5].y);     // tt - pp
-            const double2 tp = d[4];
-            f[hemi][0] = make_double2(w * d[0].x, w * d[0].y);                      // rr
-            f[hemi][1] = make_double2(w * (mis * d[2].y), w * (-mis * d[2].x));      // (-i m/s) rp
-            f[hemi][2] = make_double2(w * (d[3].x + d[5].x), w * (d[3].y + d[5].y)); // tt + pp
-            f[hemi][3] = make_double2(w * Dv.x, w * Dv.y);
-            // 2 m^2/s^2 D + 2 i m cot/s tp
-            const double g2 = 2.0 * mis * mis, hc = 2.0 * mis * cot;
-            f[hemi][4] = make_double2(w * (g2 * Dv.x - hc * tp.y), w * (g2 * Dv.y + hc * tp.x));
-            f[hemi][5] = make_double2(w * d[1].x, w * d[1].y);                       // rt
-            // -2 cot D - 2 i m/s tp
-            f[hemi][6] = make_double2(w * (-2.0 * cot * Dv.x + 2.0 * mis * tp.y),
-                                      w * (-2.0 * cot * Dv.y - 2.0 * mis * tp.x));
-        }
-#pragma unroll
-        for (int q = 0; q < 7; q++) {
-            const bool dtype = q >= 5;
-            const double2 a = f[0][q], c = f[1][q];
-            const double2 plus = make_double2(a.x + c.x, a.y + c.y), minus = make_double2(a.x - c.x, a.y - c.y);
-            fb_[(q * 2 + 0) * nh + j] = dtype ? minus : plus;    // even l+m
-            fb_[(q * 2 + 1) * nh + j] = dtype ? plus : minus;    // odd l+m
-        }
-    }
-    __syncthreads();
     const double* P = s.plm_jl + (size_t)s.mstart[m] * nh;
     const double* D = s.dplm_jl + (size_t)s.mstart[m] * nh;
-    for (int il = threadIdx.x; il < nl; il += blockDim.x) {
-        const int l = m + il, p = il & 1;
-        double2 s1[TB], s2[TB], s5[TB], sD[TB], sb[TB], sd[TB];
+    vh_d4 acc[TA_ITEMS][4];
 #pragma unroll
-        for (int bb = 0; bb < TB; bb++)
-            s1[bb] = s2[bb] = s5[bb] = sD[bb] = sb[bb] = sd[bb] = make_double2(0.0, 0.0);
-        for (int j = 0; j < nh; j++) {
-            const double pv = P[(size_t)j * nl + il], dv = D[(size_t)j * nl + il];
+    for (int it = 0; it < TA_ITEMS; it++)
 #pragma unroll
-            for (int bb = 0; bb < TB; bb++) {
-                const double2* f = fs + (size_t)bb * 14 * nh + p * nh + j;
-                const double2 f1 = f[0], f2P = f[2 * nh], f5 = f[4 * nh], fD = f[6 * nh], fb = f[8 * nh],
-                              f2D = f[10 * nh], fd = f[12 * nh];
-                s1[bb].x = fma(pv, f1.x, s1[bb].x); s1[bb].y = fma(pv, f1.y, s1[bb].y);
-                s2[bb].x = fma(dv, f2D.x, fma(pv, f2P.x, s2[bb].x));
-                s2[bb].y = fma(dv, f2D.y, fma(pv, f2P.y, s2[bb].y));
-                s5[bb].x = fma(pv, f5.x, s5[bb].x); s5[bb].y = fma(pv, f5.y, s5[bb].y);
-                sD[bb].x = fma(pv, fD.x, sD[bb].x); sD[bb].y = fma(pv, fD.y, sD[bb].y);
-                sb[bb].x = fma(pv, fb.x, sb[bb].x); sb[bb].y = fma(pv, fb.y, sb[bb].y);
-                sd[bb].x = fma(dv, fd.x, sd[bb].x); sd[bb].y = fma(dv, fd.y, sd[bb].y);
+        for (int t = 0; t < 4; t++) acc[it][t] = vh_d4{0.0, 0.0, 0.0, 0.0};
+    for (int j0 = 0; j0 < nh; j0 += TA_JC) {
+        __syncthreads();   // previous chunk's fragments consumed
+        // ---- fold the latitude pairs (j, nlat-1-j), j = j0 + jl, of 4 batches
+        for (int t = threadIdx.x; t < TA_B * TA_JC; t += blockDim.x) {
+            const int bb = t / TA_JC, jl = t % TA_JC, j = j0 + jl;
+            double2 f[2][7];
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int q = 0; q < 7; q++) f[h][q] = make_double2(0.0, 0.0);
+            if (bb < nbb && j < nh) {
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int row = h == 0 ? j : nlat - 1 - j;
+                    if (h == 1 && row == j) break;
+                    double2 d[6];
+#pragma unroll
+                    for (int dy = 0; dy < 6; dy++) d[dy] = Fd[(((size_t)(b0 + bb) * 6 + dy) * (L + 1) + m) * nlat + row];
+                    const double w = s.w[row], st = s.st[row], cot = s.ct[row] / st, is = 1.0 / st;
+                    const double mis = ms * is;
+                    const double2 Dv = make_double2(d[3].x - d[5].x, d[3].y - d[5].y);     // tt - pp
+                    const double2 tp = d[4];
+                    f[h][0] = make_double2(w * d[0].x, w * d[0].y);                      // rr
+                    f[h][1] = make_double2(w * (mis * d[2].y), w * (-mis * d[2].x));      // (-i m/s) rp
+                    f[h][2] = make_double2(w * (d[3].x + d[5].x), w * (d[3].y + d[5].y)); // tt + pp
+                    f[h][3] = make_double2(w * Dv.x, w * Dv.y);
+                    const double g2 = 2.0 * mis * mis, hc = 2.0 * mis * cot;               // 2 m^2/s^2 D + 2 i m cot/s tp
+                    f[h][4] = make_double2(w * (g2 * Dv.x - hc * tp.y), w * (g2 * Dv.y + hc * tp.x));
+                    f[h][5] = make_double2(w * d[1].x, w * d[1].y);                       // rt
+                    f[h][6] = make_double2(w * (-2.0 * cot * Dv.x + 2.0 * mis * tp.y),    // -2 cot D - 2 i m/s tp
+                                           w * (-2.0 * cot * Dv.y - 2.0 * mis * tp.x));
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 7; q++) {
+                const bool dtype = q >= 5;
+                const double2 a = f[0][q], c = f[1][q];
+                const double2 plus = make_double2(a.x + c.x, a.y + c.y), minus = make_double2(a.x - c.x, a.y - c.y);
+                const double2 ev = dtype ? minus : plus, od = dtype ? plus : minus;   // even / odd l+m
+                const int col = dtype ? 48 + bb * 4 + 2 * (q - 5) : bb * 10 + 2 * q;
+                Bs[0][jl][col] = ev.x; Bs[0][jl][col + 1] = ev.y;
+                Bs[1][jl][col] = od.x; Bs[1][jl][col + 1] = od.y;
+            }
+            if (bb == 0)
+                for (int c = 40; c < 48; c++) { Bs[0][jl][c] = 0.0; Bs[1][jl][c] = 0.0; }
+        }
+        __syncthreads();
+        // ---- MFMA over this chunk's latitudes for the wavefront's row tiles;
+        // the next k-step's P/dP fragments are in flight while this one runs
+        {
+            int il_[TA_ITEMS], par_[TA_ITEMS];
+            bool row_[TA_ITEMS];
+#pragma unroll
+            for (int it = 0; it < TA_ITEMS; it++) {
+                const int item = wave + 4 * it;
+                const int p = item < nte ? 0 : 1, rt = item < nte ? item : item - nte;
+                const int i = rt * 16 + (lane & 15);
+                par_[it] = p;
+                il_[it] = 2 * i + p;
+                row_[it] = item < nitems && i < (p ? no : ne);
+            }
+            auto frag = [&](int ks, double (&aP)[TA_ITEMS], double (&aD)[TA_ITEMS]) {
+                const int j = j0 + ks * 4 + (lane >> 4);
+#pragma unroll
+                for (int it = 0; it < TA_ITEMS; it++) {
+                    const bool ok = row_[it] && j < nh;
+                    aP[it] = ok ? P[(size_t)j * nl + il_[it]] : 0.0;
+                    aD[it] = ok ? D[(size_t)j * nl + il_[it]] : 0.0;
+                }
+            };
+            double aP[TA_ITEMS], aD[TA_ITEMS], nP[TA_ITEMS], nD[TA_ITEMS];
+            frag(0, aP, aD);
+#pragma unroll 1
+            for (int ks = 0; ks < TA_JC / 4; ks++) {
+                if (ks + 1 < TA_JC / 4) frag(ks + 1, nP, nD);
+                const int jl = ks * 4 + (lane >> 4);
+#pragma unroll
+                for (int it = 0; it < TA_ITEMS; it++) {
+                    if (wave + 4 * it >= nitems) break;
+                    const double* Bp = &Bs[par_[it]][jl][0];
+                    acc[it][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(aP[it], Bp[(lane & 15)], acc[it][0], 0, 0, 0);
+                    acc[it][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(aP[it], Bp[16 + (lane & 15)], acc[it][1], 0, 0, 0);
+                    acc[it][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(aP[it], Bp[32 + (lane & 15)], acc[it][2], 0, 0, 0);
+                    acc[it][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(aD[it], Bp[48 + (lane & 15)], acc[it][3], 0, 0, 0);
+                }
+#pragma unroll
+                for (int it = 0; it < TA_ITEMS; it++) { aP[it] = nP[it]; aD[it] = nD[it]; }
             }
         }
-        const double J = (double)l * (double)(l + 1);
-        const size_t idx = (size_t)l * (l + 1) / 2 + m;
-        const double n6l = n6 ? n6[l] : 1.0;
+    }
+    // ---- epilogue: stage each row tile, then lane = (degree row, batch)
 #pragma unroll
-        for (int bb = 0; bb < TB; bb++) {
-            if (bb >= nbb) break;
+    for (int it = 0; it < TA_ITEMS; it++) {
+        const int item = wave + 4 * it;
+        if (item >= nitems) break;
+        const int p = item < nte ? 0 : 1, rt = item < nte ? item : item - nte;
+        const int np_ = p ? no : ne;
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) Ds[wave][4 * r + (lane >> 4)][t * 16 + (lane & 15)] = acc[it][t][r];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int rr = lane & 15, bb = lane >> 4, i = rt * 16 + rr;
+        if (i < np_ && bb < nbb) {
+            const int l = m + 2 * i + p;
+            const double* row = &Ds[wave][rr][0];
+            const double2 s1 = make_double2(row[bb * 10 + 0], row[bb * 10 + 1]);
+            const double2 s2 = make_double2(row[bb * 10 + 2] + row[48 + bb * 4 + 0], row[bb * 10 + 3] + row[48 + bb * 4 + 1]);
+            const double2 s5 = make_double2(row[bb * 10 + 4], row[bb * 10 + 5]);
+            const double2 sD = make_double2(row[bb * 10 + 6], row[bb * 10 + 7]);
+            const double2 sb = make_double2(row[bb * 10 + 8], row[bb * 10 + 9]);
+            const double2 sd = make_double2(row[48 + bb * 4 + 2], row[48 + bb * 4 + 3]);
+            const double J = (double)l * (double)(l + 1);
+            const size_t idx = (size_t)l * (l + 1) / 2 + m;
             double2* Cb = C + (size_t)(b0 + bb) * 4 * nlm;
-            Cb[idx] = s1[bb];
-            Cb[nlm + idx] = J > 0.0 ? make_double2(s2[bb].x / J, s2[bb].y / J) : make_double2(0.0, 0.0);
-            Cb[2 * nlm + idx] = J > 0.0 ? make_double2(s5[bb].x / (-2.0 * J), s5[bb].y / (-2.0 * J))
-                                        : make_double2(0.0, 0.0);
-            const double2 raw = make_double2(-J * sD[bb].x + sb[bb].x + sd[bb].x, -J * sD[bb].y + sb[bb].y + sd[bb].y);
+            Cb[idx] = s1;
+            Cb[nlm + idx] = J > 0.0 ? make_double2(s2.x / J, s2.y / J) : make_double2(0.0, 0.0);
+            Cb[2 * nlm + idx] = J > 0.0 ? make_double2(s5.x / (-2.0 * J), s5.y / (-2.0 * J)) : make_double2(0.0, 0.0);
+            const double2 raw = make_double2(-J * sD.x + sb.x + sd.x, -J * sD.y + sb.y + sd.y);
             double2 c6 = raw;
-            if (n6) c6 = n6l != 0.0 ? make_double2(raw.x / n6l, raw.y / n6l) : make_double2(0.0, 0.0);
+            if (n6) {
+                const double n = n6[l];
+                c6 = n != 0.0 ? make_double2(raw.x / n, raw.y / n) : make_double2(0.0, 0.0);
+            }
             Cb[3 * nlm + idx] = c6;
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -376,11 +448,7 @@ __global__ void __launch_bounds__(256) k_t3_gather(int nr, int nk, int nlm, vh_d
 #ifndef VH_TSH_TB_SYNTH
 #define VH_TSH_TB_SYNTH 3
 #endif
-#ifndef VH_TSH_TB_ANAL
-#define VH_TSH_TB_ANAL 1
-#endif
 static size_t synth_lds(const vh_sht& s) { return (size_t)VH_TSH_TB_SYNTH * 5 * (s.L + 1) * sizeof(double2); }
-static size_t anal_lds(const vh_sht& s) { return (size_t)VH_TSH_TB_ANAL * 14 * s.nh * sizeof(double2); }
 static size_t rows_lds(const vh_sht& s) { return (size_t)6 * s.nphi * sizeof(double2) + (size_t)s.nphi * sizeof(double); }
 
 void vh_tsh_synth(hipStream_t st, const vh_sht& s, const double2* X, int nb, double2* Fd) {
@@ -394,8 +462,11 @@ void vh_tsh_synth(hipStream_t st, const vh_sht& s, const double2* X, int nb, dou
 
 void vh_tsh_analysis(hipStream_t st, const vh_sht& s, const double2* Fd, int nb, const double* n6, double2* C) {
     if (nb <= 0) return;
-    const int ngrp = (nb + VH_TSH_TB_ANAL - 1) / VH_TSH_TB_ANAL;
-    k_tsh_analysis<VH_TSH_TB_ANAL><<<xcd_order_grid(ngrp, s.L), 128, anal_lds(s), st>>>(s, Fd, nb, n6, C);
+    // row tiles of one order: ceil(ne/16) + ceil(no/16) <= 4 wavefronts x TA_ITEMS
+    if ((s.L + 2) / 2 > 16 * 5 || (s.L + 1) / 2 > 16 * 5)
+        throw vh_error("tensor analysis: lmax above 159 is not supported by the MFMA tiling");
+    const int ngrp = (nb + TA_B - 1) / TA_B;
+    k_tsh_analysis<<<xcd_order_grid(ngrp, s.L), 256, 0, st>>>(s, Fd, nb, n6, C);
     HIPCHECK(hipGetLastError());
 }
 
