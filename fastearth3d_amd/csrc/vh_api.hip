@@ -300,7 +300,8 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
     h->visc3d_tol = p->visc3d_tol;
     HIPCHECK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
     hipStream_t st = h->st;
-    HIPCHECK(hipHostMalloc((void**)&h->h_pin, 64 * sizeof(double), hipHostMallocDefault));
+    HIPCHECK(hipHostMalloc((void**)&h->h_pin, (2 * (size_t)std::max(nl, 32) + 64) * sizeof(double),
+                           hipHostMallocDefault));
 
     // ---- earth, mesh, element tables (setup, host)
     double r_earth = 0;
@@ -634,13 +635,18 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
         sle_wcorr(st, n, h->subgrid, h->d_topo0, h->d_C, h->d_C0s, h->d_wcorr);
         for (int ii = 1; ii <= h->n_inner; ii++) {
             sle_load_analysis(st, s, h->d_dig, h->d_C, h->d_rsl, h->d_wcorr, h->d_F, h->d_sigma);
-            sle_apply_synth(st, s, h->d_sigma, h->d_gu, h->d_gn, h->d_dsurf, h->d.nk, h->g, h->kind, h->d_uN, h->d_F2,
-                            srot, h->d_sraw);
-            sle_cs(st, s, h->d_C, h->d_sraw, h->d_part, h->d_sc + 3);
-            sle_update(st, s, h->d_sraw, h->d_C, h->d_rsl, h->d_sc, h->subgrid, h->d_part);
-            h->read(h->d_sc + 4, 3);
-            res.esl = h->h_pin[0];
-            double dmax = h->h_pin[1], smax = h->h_pin[2];
+            // part: [0, 2 nlat) row maxima + [2 nlat] dphi (read back), [3 nlat, 4 nlat) row sums of C Sraw
+            double* cs = h->d_part + 3 * (size_t)s.nlat;
+            sle_apply_synth(st, s, h->d_sigma, h->d_gu, h->d_gn, h->d_dsurf, h->d.nk, h->g, h->kind, h->d_F2, srot,
+                            h->d_C, h->d_sraw, cs);
+            sle_update(st, s, h->d_sraw, h->d_C, h->d_rsl, h->d_sc, h->subgrid, cs, h->d_part);
+            h->read(h->d_part, 2 * s.nlat + 1);
+            double dmax = 0.0, smax = 0.0;   // max is exact in any order
+            for (int q = 0; q < s.nlat; q++) {
+                dmax = std::max(dmax, h->h_pin[q]);
+                smax = std::max(smax, h->h_pin[s.nlat + q]);
+            }
+            res.esl = h->h_pin[2 * s.nlat];
             res.n_inner_last = ii;
             if (dmax <= h->tol * std::max(smax, DBL_MIN)) break;
         }

@@ -112,90 +112,23 @@ __global__ void __launch_bounds__(256) k_leg_analysis(vh_sht s, const double2* _
     }
 }
 
-// Synthesis: block = order m; thread = (north latitude j, quarter of the
-// degrees), every field per table load; quarter sums (even / odd l+m) added
-// in a fixed order. nb <= 4.
-__global__ void __launch_bounds__(512) k_leg_synthesis(vh_sht s, const double2* __restrict__ q, int nb,
-                                                       double2* __restrict__ F, int fst, int jst) {
-    extern __shared__ double2 sh[];
-    const int m = blockIdx.x, L = s.L, nh = s.nh, nl = L - m + 1;
-    double2* Q = sh;                        // [nb][nl]
-    double2* red = sh + nb * (L + 1);       // [4][nh][nb][2]
-    for (int t = threadIdx.x; t < nb * nl; t += blockDim.x) {
-        int f = t / nl, il = t % nl, l = m + il;
-        Q[f * nl + il] = q[(size_t)f * s.nlm + l * (l + 1) / 2 + m];
-    }
-    __syncthreads();
-    const int g = threadIdx.x / nh, j = threadIdx.x % nh;
-    if (g < 4) {
-        const int lq = ((nl + 7) / 8) * 2;      // even count per quarter keeps parity aligned
-        const int la = min(nl, g * lq), lb = min(nl, la + lq);
-        const double* P = s.plm_lj + (size_t)s.mstart[m] * nh;
-        double2 e[4], o[4];
-#pragma unroll
-        for (int f = 0; f < 4; f++) e[f] = o[f] = make_double2(0.0, 0.0);
-        for (int il = la; il < lb; il += 2) {
-            const double p = P[(size_t)il * nh + j];
-            const bool two = il + 1 < lb;
-            const double p1 = two ? P[(size_t)(il + 1) * nh + j] : 0.0;
-#pragma unroll
-            for (int f = 0; f < 4; f++) {
-                if (f >= nb) break;
-                const double2 c0 = Q[f * nl + il];
-                e[f].x = fma(c0.x, p, e[f].x); e[f].y = fma(c0.y, p, e[f].y);
-                if (two) {
-                    const double2 c1 = Q[f * nl + il + 1];
-                    o[f].x = fma(c1.x, p1, o[f].x); o[f].y = fma(c1.y, p1, o[f].y);
-                }
-            }
-        }
-#pragma unroll
-        for (int f = 0; f < 4; f++) {
-            if (f >= nb) break;
-            double2* r = red + (((size_t)g * nh + j) * nb + f) * 2;
-            r[0] = e[f];
-            r[1] = o[f];
-        }
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < nh * nb; t += blockDim.x) {
-        const int jj = t / nb, f = t % nb, js = s.nlat - 1 - jj;
-        double2 ev = make_double2(0.0, 0.0), od = ev;
-        for (int k = 0; k < 4; k++) {
-            const double2* r = red + (((size_t)k * nh + jj) * nb + f) * 2;
-            ev = make_double2(ev.x + r[0].x, ev.y + r[0].y);
-            od = make_double2(od.x + r[1].x, od.y + r[1].y);
-        }
-        double2* Ff = F + (size_t)f * fst * (L + 1);
-        Ff[(size_t)jj * jst * (L + 1) + m] = make_double2(ev.x + od.x, ev.y + od.y);
-        if (js != jj) Ff[(size_t)js * jst * (L + 1) + m] = make_double2(ev.x - od.x, ev.y - od.y);
-    }
-}
-
 static size_t leg_anal_lds(const vh_sht& s, int nb) { return ((size_t)2 * nb * s.nh + 4 * 64 * nb) * sizeof(double2); }
-static size_t leg_synth_lds(const vh_sht& s, int nb) {
-    return ((size_t)nb * (s.L + 1) + (size_t)4 * s.nh * nb * 2) * sizeof(double2);
-}
+
 
 void vh_legendre_analysis(hipStream_t st, const vh_sht& s, const double2* F, int nb, double2* q) {
     k_leg_analysis<<<s.nljobs, 256, leg_anal_lds(s, nb), st>>>(s, F, nb, q, s.nlat, 1);
     HIPCHECK(hipGetLastError());
 }
 
-static int leg_synth_threads(const vh_sht& s, int nb) {
-    if (nb > 4) throw vh_error("legendre synthesis: at most 4 fields per call");
-    if (s.nh > 128) throw vh_error("legendre synthesis: nlat above 256 is not supported");
-    return ((4 * s.nh + 63) / 64) * 64;
-}
 
 void vh_legendre_synthesis(hipStream_t st, const vh_sht& s, const double2* q, int nb, double2* F) {
-    k_leg_synthesis<<<s.L + 1, leg_synth_threads(s, nb), leg_synth_lds(s, nb), st>>>(s, q, nb, F, s.nlat, 1);
+    vh_legendre_synthesis_t(st, s, SpecSrc{q, s.nlm}, nb, F, s.nlat, 1);
     HIPCHECK(hipGetLastError());
 }
 
 void vh_legendre_synthesis_rows(hipStream_t st, const vh_sht& s, const double2* q, int nb, double2* F,
                                 int fst, int jst) {
-    k_leg_synthesis<<<s.L + 1, leg_synth_threads(s, nb), leg_synth_lds(s, nb), st>>>(s, q, nb, F, fst, jst);
+    vh_legendre_synthesis_t(st, s, SpecSrc{q, s.nlm}, nb, F, fst, jst);
     HIPCHECK(hipGetLastError());
 }
 

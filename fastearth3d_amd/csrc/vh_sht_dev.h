@@ -12,14 +12,18 @@ struct InGrid {
     __device__ double operator()(int row, int i) const { return f[(size_t)row * nphi + i]; }
 };
 // consumers of the inverse transform receive the two real rows of a pair
-// (r0, r0 + 1) at longitude i
+// (r0, r0 + 1) at longitude i; with kRowSum the values they return are summed
+// over the row (wavefront reduction) and handed to row_total(r0, sum)
 struct OutGrid {
+    static constexpr bool kRowSum = false;
     double* f;
     int nphi;
-    __device__ void operator()(int r0, int i, double v0, double v1, bool has1) const {
+    __device__ double operator()(int r0, int i, double v0, double v1, bool has1) const {
         f[(size_t)r0 * nphi + i] = v0;
         if (has1) f[(size_t)(r0 + 1) * nphi + i] = v1;
+        return 0.0;
     }
+    __device__ void row_total(int, double) const {}
 };
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
@@ -222,14 +226,20 @@ __global__ void __launch_bounds__(256) k_fft_inv(vh_sht s, const double2* __rest
     }
     wave_lds_sync();
     double2* z = fft_wave(a, b, s, true, lane);
+    double rs = 0.0;   // consumers may return a per-row term to be summed
     if (has0)
         for (int i0 = 0; i0 < N; i0 += 64 * VH_FFT_U) {
 #pragma unroll
             for (int u = 0; u < VH_FFT_U; u++) {
                 const int i = i0 + lane + 64 * u;
-                if (i < N) out(r0, i, z[i].x, z[i].y, has1);
+                if (i < N) rs += out(r0, i, z[i].x, z[i].y, has1);
             }
         }
+    if constexpr (Out::kRowSum) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) rs += __shfl_xor(rs, off);
+        if (lane == 0 && has0) out.row_total(r0, rs);
+    }
 }
 
 template <class In>
@@ -247,3 +257,83 @@ void vh_fft_inverse_pair_t(hipStream_t st, const vh_sht& s, const double2* F, in
     k_fft_inv<Out><<<npairs, 64, lds, st>>>(s, F, nrows, out);
     HIPCHECK(hipGetLastError());
 }
+
+// Legendre synthesis: block = order m; thread = (north latitude j, quarter
+// of the degrees), every field per table load; quarter sums (even / odd l+m)
+// added in a fixed order. nb <= 4 fields; src(f, l, m, idx) supplies the
+// coefficient of field f at degree-major index idx (a spectrum, or one
+// computed on the fly such as the response applied to a load).
+struct SpecSrc {
+    const double2* q;
+    int nlm;
+    __device__ double2 operator()(int f, int l, int m, int idx) const { (void)l; (void)m; return q[(size_t)f * nlm + idx]; }
+};
+
+template <class Src>
+__global__ void __launch_bounds__(512) k_leg_synthesis(vh_sht s, Src src, int nb, double2* __restrict__ F, int fst,
+                                                       int jst) {
+    extern __shared__ double2 sh[];
+    const int m = blockIdx.x, L = s.L, nh = s.nh, nl = L - m + 1;
+    double2* Q = sh;                        // [nb][nl]
+    double2* red = sh + nb * (L + 1);       // [4][nh][nb][2]
+    for (int t = threadIdx.x; t < nb * nl; t += blockDim.x) {
+        int f = t / nl, il = t % nl, l = m + il;
+        Q[f * nl + il] = src(f, l, m, l * (l + 1) / 2 + m);
+    }
+    __syncthreads();
+    const int g = threadIdx.x / nh, j = threadIdx.x % nh;
+    if (g < 4) {
+        const int lq = ((nl + 7) / 8) * 2;      // even count per quarter keeps parity aligned
+        const int la = min(nl, g * lq), lb = min(nl, la + lq);
+        const double* P = s.plm_lj + (size_t)s.mstart[m] * nh;
+        double2 e[4], o[4];
+#pragma unroll
+        for (int f = 0; f < 4; f++) e[f] = o[f] = make_double2(0.0, 0.0);
+        for (int il = la; il < lb; il += 2) {
+            const double p = P[(size_t)il * nh + j];
+            const bool two = il + 1 < lb;
+            const double p1 = two ? P[(size_t)(il + 1) * nh + j] : 0.0;
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                if (f >= nb) break;
+                const double2 c0 = Q[f * nl + il];
+                e[f].x = fma(c0.x, p, e[f].x); e[f].y = fma(c0.y, p, e[f].y);
+                if (two) {
+                    const double2 c1 = Q[f * nl + il + 1];
+                    o[f].x = fma(c1.x, p1, o[f].x); o[f].y = fma(c1.y, p1, o[f].y);
+                }
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            if (f >= nb) break;
+            double2* r = red + (((size_t)g * nh + j) * nb + f) * 2;
+            r[0] = e[f];
+            r[1] = o[f];
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nh * nb; t += blockDim.x) {
+        const int jj = t / nb, f = t % nb, js = s.nlat - 1 - jj;
+        double2 ev = make_double2(0.0, 0.0), od = ev;
+        for (int k = 0; k < 4; k++) {
+            const double2* r = red + (((size_t)k * nh + jj) * nb + f) * 2;
+            ev = make_double2(ev.x + r[0].x, ev.y + r[0].y);
+            od = make_double2(od.x + r[1].x, od.y + r[1].y);
+        }
+        double2* Ff = F + (size_t)f * fst * (L + 1);
+        Ff[(size_t)jj * jst * (L + 1) + m] = make_double2(ev.x + od.x, ev.y + od.y);
+        if (js != jj) Ff[(size_t)js * jst * (L + 1) + m] = make_double2(ev.x - od.x, ev.y - od.y);
+    }
+}
+
+template <class Src>
+void vh_legendre_synthesis_t(hipStream_t st, const vh_sht& s, Src src, int nb, double2* F, int fst, int jst) {
+    if (nb > 4) throw vh_error("legendre synthesis: at most 4 fields per call");
+    if (s.nh > 128) throw vh_error("legendre synthesis: nlat above 256 is not supported");
+    const int nthr = ((4 * s.nh + 63) / 64) * 64;
+    const size_t lds = ((size_t)nb * (s.L + 1) + (size_t)4 * s.nh * nb * 2) * sizeof(double2);
+    k_leg_synthesis<Src><<<s.L + 1, nthr, lds, st>>>(s, src, nb, F, fst, jst);
+    HIPCHECK(hipGetLastError());
+}
+

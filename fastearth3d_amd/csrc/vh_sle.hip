@@ -100,20 +100,56 @@ __global__ void k_apply(int nlm, const double2* __restrict__ sigma, const double
     uN[nlm + idx] = N;
 }
 
+// the response applied to the load as the synthesis reads it: field 0 is
+// u = gu*sigma + dUa, field 1 is N = gn*sigma - dFa/g (response_apply,
+// src/vilma_response.f90:842-864)
+struct ApplySrc {
+    const double2* sigma;
+    const double* gu;
+    const double* gn;
+    const double* dsurf;
+    int nk, kind;
+    double g;
+    __device__ double2 operator()(int f, int l, int m, int idx) const {
+        (void)m;
+        if (kind == 0) return make_double2(0.0, 0.0);   // null response
+        const double2 s = sigma[idx];
+        if (f == 0) {
+            double2 u = make_double2(gu[l] * s.x, gu[l] * s.y);
+            if (kind == 2 && l >= 1) {
+                const double2 dU = ((const double2*)dsurf)[idx - 1];
+                u = make_double2(u.x + dU.x, u.y + dU.y);
+            }
+            return u;
+        }
+        double2 N = make_double2(gn[l] * s.x, gn[l] * s.y);
+        if (kind == 2 && l >= 1) {
+            const double2 dF = ((const double2*)dsurf)[nk + idx - 1];
+            N = make_double2(N.x - dF.x / g, N.y - dF.y / g);
+        }
+        return N;
+    }
+};
+
 // inverse transform epilogue: rows arrive as interleaved pairs (u_j, N_j);
-// Sraw = N - u (+ s_rot)
+// Sraw = N - u (+ s_rot); the row's sum of C * Sraw goes to cs[j]
 struct OutSraw {
+    static constexpr bool kRowSum = true;
     double* sraw;
     const double* srot;
+    const double* C;
+    double* cs;
     int nphi;
-    __device__ void operator()(int r0, int i, double v0, double v1, bool has1) const {
+    __device__ double operator()(int r0, int i, double v0, double v1, bool has1) const {
         (void)has1;
         int j = r0 >> 1;
         size_t id = (size_t)j * nphi + i;
         double s = v1 - v0;
         if (srot) s = s + srot[id];
         sraw[id] = s;
+        return C[id] * s;
     }
+    __device__ void row_total(int r0, double v) const { cs[r0 >> 1] = v; }
 };
 
 __global__ void __launch_bounds__(256) k_rowsum_prod(int nphi, const double* __restrict__ a,
@@ -134,17 +170,31 @@ __global__ void __launch_bounds__(256) k_rowsum_prod(int nphi, const double* __r
     if (threadIdx.x == 0) part[j] = red[0];
 }
 
+// Cs_int = sum_j w_j dlon cs_j (latitude order, every block the same);
 // dphi = (ice_int - Cs_int + zeta_int)/C_int ; rsl_new = Sraw + dphi ;
-// dmax = max|C (rsl_new - rsl)| ; rsl = rsl_new ; smax = max|C rsl|
-// sc: [0] C_int [1] int d_ice_g [2] zeta_int [3] Cs_int; writes sc[4] = dphi.
+// per row: dmax = max|C (rsl_new - rsl)|, smax = max|C rsl_new| ; rsl = rsl_new.
+// sc: [0] C_int [1] int d_ice_g [2] zeta_int; writes sc[3] = Cs_int, sc[4] = dphi.
+// out: [0, nlat) row dmax, [nlat, 2 nlat) row smax, [2 nlat] dphi.
 __global__ void __launch_bounds__(256) k_sle_update(int nphi, const double* __restrict__ sraw,
-        const double* __restrict__ C, double* rsl, double* sc, int subgrid, double* part /*[2][nlat]*/, int nlat) {
+        const double* __restrict__ C, double* rsl, double* sc, int subgrid, const double* __restrict__ cs,
+        const double* __restrict__ w, double dlon, double* out, int nlat) {
     __shared__ double r0[256], r1[256];
+    __shared__ double sv[2][512];
+    __shared__ double dphi_s;
     const int j = blockIdx.x;
-    const double rho_ratio = VH_RHO_ICE / VH_RHO_WATER;
-    double ice_int = -rho_ratio * sc[1];
-    double zeta = subgrid ? sc[2] : 0.0;
-    double dphi = (ice_int - sc[3] + zeta) / sc[0];
+    for (int t = threadIdx.x; t < nlat; t += blockDim.x) { sv[0][t] = w[t]; sv[1][t] = cs[t]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double total = 0.0;
+        for (int q = 0; q < nlat; q++) total += sv[0][q] * sv[1][q] * dlon;
+        const double rho_ratio = VH_RHO_ICE / VH_RHO_WATER;
+        const double ice_int = -rho_ratio * sc[1];
+        const double zeta = subgrid ? sc[2] : 0.0;
+        dphi_s = (ice_int - total + zeta) / sc[0];
+        if (j == 0) { sc[3] = total; sc[4] = dphi_s; out[2 * nlat] = dphi_s; }
+    }
+    __syncthreads();
+    const double dphi = dphi_s;
     double dmax = 0.0, smax = 0.0;
     for (int i = threadIdx.x; i < nphi; i += blockDim.x) {
         size_t id = (size_t)j * nphi + i;
@@ -163,9 +213,8 @@ __global__ void __launch_bounds__(256) k_sle_update(int nphi, const double* __re
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        part[j] = r0[0];
-        part[nlat + j] = r1[0];
-        if (j == 0) sc[4] = dphi;
+        out[j] = r0[0];
+        out[nlat + j] = r1[0];
     }
 }
 
@@ -260,11 +309,10 @@ void sle_load_analysis(hipStream_t st, const vh_sht& s, const double* dig, const
     vh_legendre_analysis(st, s, F, 1, load_lm);
 }
 void sle_apply_synth(hipStream_t st, const vh_sht& s, const double2* sigma, const double* gu, const double* gn,
-                     const double* dsurf, int nk, double g, int kind, double2* uN, double2* F2,
-                     const double* srot, double* sraw) {
-    k_apply<<<nb256(s.nlm), 256, 0, st>>>(s.nlm, sigma, gu, gn, dsurf, nk, g, kind, uN);
-    vh_legendre_synthesis_rows(st, s, uN, 2, F2, 1, 2);
-    vh_fft_inverse_pair_t(st, s, F2, 2 * s.nlat, OutSraw{sraw, srot, s.nphi});
+                     const double* dsurf, int nk, double g, int kind, double2* F2, const double* srot,
+                     const double* C, double* sraw, double* cs) {
+    vh_legendre_synthesis_t(st, s, ApplySrc{sigma, gu, gn, dsurf, nk, kind, g}, 2, F2, 1, 2);
+    vh_fft_inverse_pair_t(st, s, F2, 2 * s.nlat, OutSraw{sraw, srot, C, cs, s.nphi});
 }
 void vh_apply_only(hipStream_t st, int nlm, const double2* sigma, const double* gu, const double* gn,
                    const double* dsurf, int nk, double g, int kind, double2* uN) {
@@ -275,9 +323,10 @@ void sle_cs(hipStream_t st, const vh_sht& s, const double* C, const double* f, d
     vh_weighted_total(st, s, part, 1, out);
 }
 void sle_update(hipStream_t st, const vh_sht& s, const double* sraw, const double* C, double* rsl, double* sc,
-                int subgrid, double* part) {
-    k_sle_update<<<s.nlat, 256, 0, st>>>(s.nphi, sraw, C, rsl, sc, subgrid, part, s.nlat);
-    k_max_final<<<2, 256, 0, st>>>(part, s.nlat, sc + 5);
+                int subgrid, const double* cs, double* out) {
+    if (s.nlat > 512) throw vh_error("sle_update: nlat above 512 is not supported");
+    k_sle_update<<<s.nlat, 256, 0, st>>>(s.nphi, sraw, C, rsl, sc, subgrid, cs, s.w, 2.0 * VH_PI / s.nphi, out,
+                                         s.nlat);
 }
 void sle_flip(hipStream_t st, const vh_sht& s, const double* topo0, const double* rsl, const double* ice,
               const double* C, double* Cnext, double* part, double* out) {
