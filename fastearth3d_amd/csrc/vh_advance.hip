@@ -9,7 +9,6 @@
 // produces the per-slot max |tau_{n+1}| that the next drift solve's skip gate
 // and the controller's memory-norm guard need (src/vilma_response.f90:719-728,
 // :1795-1811), so no separate scan of the memory is ever made.
-// Block = 64 slots (lanes, coalesced rows of tau) x 4 element groups.
 #include "vh_internal.h"
 #include "vh_radial_dev.h"
 
@@ -28,35 +27,39 @@ struct advance_args {
     int nlm;
 };
 
+// Grid: x = tile of 64 slots (lanes, coalesced rows of tau), y = group of
+// four (Maxwell element, shape) pairs (one per wavefront), so every thread
+// moves 8 memory values (re/im x 4 channels) and the launch has thousands of
+// blocks. The per-slot max |tau_{n+1}| is combined across blocks with an
+// integer atomicMax on the bit pattern (|x| >= 0 orders like its bits), into
+// mnorm_out / the global max, both cleared before the launch.
 __global__ void __launch_bounds__(256) k_advance(advance_args a) {
     __shared__ double red[4][64];
-    const int lane = threadIdx.x & 63, eg = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wg = threadIdx.x >> 6;
     const int k = blockIdx.x * 64 + lane;
+    const int pair = blockIdx.y * 4 + wg;             // (em, shape) = (pair / 3, pair % 3)
     const int nk = a.nk;
     double amax = 0.0;
-    if (k < nk) {
-        const int l = a.v.kdeg[k];
-        const double Jr = (double)l * (double)(l + 1);
-        const double sre = a.sigma[2 * (k + 1)], sim = a.sigma[2 * (k + 1) + 1];
-        const double* xU = a.v.xUn + (size_t)(l - 1) * a.nr;
-        const double* xV = a.v.xVn + (size_t)(l - 1) * a.nr;
-        for (int em = eg; em < a.nem; em += 4) {
-            const size_t base = (size_t)em * 24 * nk + k;
-            if (a.v.is3d[em]) {   // tau_{n+1} = tau_n + 3-D increment (increment form)
-                const double2* cd = a.cdel + (size_t)a.e3of[em] * 12 * a.nlm + k + 1;
+    if (k < nk && pair < 3 * a.nem) {
+        const int em = pair / 3, sh = pair % 3;
+        const size_t base = (size_t)em * 24 * nk + k;
+        if (a.v.is3d[em]) {   // tau_{n+1} = tau_n + 3-D increment (increment form)
+            const double2* cd = a.cdel + ((size_t)a.e3of[em] * 3 + sh) * 4 * a.nlm + k + 1;
 #pragma unroll
-                for (int sh = 0; sh < 3; sh++)
-#pragma unroll
-                    for (int lam = 0; lam < 4; lam++) {
-                        const double2 dv = cd[(size_t)(sh * 4 + lam) * a.nlm];
-                        const size_t o0 = base + (size_t)((sh * 2 + 0) * 4 + lam) * nk;
-                        const size_t o1 = base + (size_t)((sh * 2 + 1) * 4 + lam) * nk;
-                        const double t0 = a.tau_in[o0] + dv.x, t1 = a.tau_in[o1] + dv.y;
-                        a.tau_out[o0] = t0; a.tau_out[o1] = t1;
-                        amax = fmax(amax, fmax(fabs(t0), fabs(t1)));
-                    }
-                continue;
+            for (int lam = 0; lam < 4; lam++) {
+                const double2 dv = cd[(size_t)lam * a.nlm];
+                const size_t o0 = base + (size_t)((sh * 2 + 0) * 4 + lam) * nk;
+                const size_t o1 = base + (size_t)((sh * 2 + 1) * 4 + lam) * nk;
+                const double t0 = a.tau_in[o0] + dv.x, t1 = a.tau_in[o1] + dv.y;
+                a.tau_out[o0] = t0; a.tau_out[o1] = t1;
+                amax = fmax(amax, fmax(fabs(t0), fabs(t1)));
             }
+        } else {
+            const int l = a.v.kdeg[k];
+            const double Jr = (double)l * (double)(l + 1);
+            const double sre = a.sigma[2 * (k + 1)], sim = a.sigma[2 * (k + 1) + 1];
+            const double* xU = a.v.xUn + (size_t)(l - 1) * a.nr;
+            const double* xV = a.v.xVn + (size_t)(l - 1) * a.nr;
             const int e = a.v.emap[em];
             const int n0 = a.v.mnode[e], n1 = a.v.mnode[e + 1];
             const double* d0 = a.dN + (size_t)n0 * 4 * nk + k;
@@ -71,23 +74,22 @@ __global__ void __launch_bounds__(256) k_advance(advance_args a) {
             const double Me = a.v.MkPerDt[e] * a.dt;   // response_set_dt: Mk = MkPerDt*dt
             const double om = 1.0 - Me, two_muM = 2.0 * a.v.mu[e] * Me;
 #pragma unroll
-            for (int sh = 0; sh < 3; sh++)
+            for (int ri = 0; ri < 2; ri++)
 #pragma unroll
-                for (int ri = 0; ri < 2; ri++)
-#pragma unroll
-                    for (int lam = 0; lam < 4; lam++) {
-                        size_t off = base + (size_t)((sh * 2 + ri) * 4 + lam) * nk;
-                        double t = om * a.tau_in[off] - two_muM * s[ri][sh][lam];
-                        a.tau_out[off] = t;
-                        amax = fmax(amax, fabs(t));
-                    }
+                for (int lam = 0; lam < 4; lam++) {
+                    const size_t off = base + (size_t)((sh * 2 + ri) * 4 + lam) * nk;
+                    const double sv = sh == 0 ? s[ri][0][lam] : (sh == 1 ? s[ri][1][lam] : s[ri][2][lam]);
+                    const double t = om * a.tau_in[off] - two_muM * sv;
+                    a.tau_out[off] = t;
+                    amax = fmax(amax, fabs(t));
+                }
         }
     }
-    red[eg][lane] = amax;
+    red[wg][lane] = amax;
     __syncthreads();
-    if (eg == 0) {
-        double m = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
-        if (k < nk) a.mnorm_out[k] = m;
+    if (wg == 0) {
+        const double m = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
+        if (k < nk) atomicMax((unsigned long long*)a.mnorm_out + k, (unsigned long long)__double_as_longlong(m));
         red[0][lane] = m;
     }
     __syncthreads();
@@ -96,7 +98,6 @@ __global__ void __launch_bounds__(256) k_advance(advance_args a) {
     if (threadIdx.x == 0) {
         double m = 0.0;
         for (int i = 0; i < 32; i++) m = fmax(m, red[0][i]);
-        // |tau| >= 0: the IEEE bit pattern orders like the value (NaN above Inf)
         atomicMax(a.maxbits, (unsigned long long)__double_as_longlong(m));
     }
 }
@@ -109,9 +110,11 @@ void vh_launch_advance(hipStream_t st, const vh_dims& d, const vh_dev& v, double
     a.nr = d.nr; a.nk = d.nk; a.nem = d.nem; a.dt = dt; a.v = v;
     a.tau_in = tau_in; a.tau_out = tau_out; a.sigma = sigma; a.dN = dN;
     a.mnorm_out = mnorm_out; a.maxbits = (unsigned long long*)maxnorm_out;
+    vh_fill(st, mnorm_out, 0.0, d.nk);
     vh_fill(st, maxnorm_out, 0.0, 1);
-    int nb = (d.nk + 63) / 64;
-    k_advance<<<nb, 256, 0, st>>>(a);
+    if (d.nem == 0) return;
+    dim3 g((d.nk + 63) / 64, (3 * d.nem + 3) / 4);
+    k_advance<<<g, 256, 0, st>>>(a);
     HIPCHECK(hipGetLastError());
 }
 

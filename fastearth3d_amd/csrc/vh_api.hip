@@ -395,7 +395,7 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
         vh_fill(st, h->d_dsurf, 0.0, (size_t)6 * d.nk);
     }
     h->refresh_dev();
-    if (h->kind == 2) h->dplan = vh_drift_plan_create(st, d, h->v, h->deg1_cm);
+    if (h->kind == 2) h->dplan = vh_drift_plan_create(st, d, h->v, h->deg1_cm, h->emap, h->mnode_of);
 
     // ---- SHT plan
     std::vector<double> ct, w;

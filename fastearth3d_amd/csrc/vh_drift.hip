@@ -80,6 +80,9 @@ struct drift_args {
     const double* r;
     const int* emap;
     const int* mnode;
+    const int* kdeg;
+    const int2* node_el;  // [nmn] (Maxwell element below, above the node) or -1
+    double* Fe;           // [em][4][2][nk] element forcing (floc, re/im)
     const double* tau;
     const double* mnorm;
     const double* maxnorm;
@@ -91,56 +94,82 @@ struct drift_args {
 
 #define DRIFT_THREADS 512
 
-// Workgroup = (degree l, ts consecutive orders): rhs = reim*ts + (m - m0).
+// Dissipative forcing of every (slot, Maxwell element): Fe[em][c][reim][k] =
+// floc_c (test dofs U_e, U_e+1, V_e, V_e+1), for the slots the skip gate keeps.
+// Grid: x = 64 slots (coalesced), y = 4 elements.
+__global__ void __launch_bounds__(256) k_drift_force(drift_args a) {
+    const int lane = threadIdx.x & 63, em = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = blockIdx.x * 64 + lane, nk = a.nk;
+    if (k >= nk || em >= a.nem) return;
+    if (!(a.mnorm[k] > a.skip_tol * a.maxnorm[0])) return;
+    const int l = a.kdeg[k], e = a.emap[em];
+    const double Jr = (double)l * (double)(l + 1);
+    const double* t = a.tau + (size_t)em * 24 * nk + k;
+    double* o = a.Fe + (size_t)em * 8 * nk + k;
+#pragma unroll
+    for (int reim = 0; reim < 2; reim++) {
+        double A[4], B[4], Cc[4], floc[4];
+#pragma unroll
+        for (int lam = 0; lam < 4; lam++) {
+            A[lam] = t[(size_t)((0 * 2 + reim) * 4 + lam) * nk];
+            B[lam] = t[(size_t)((1 * 2 + reim) * 4 + lam) * nk];
+            Cc[lam] = t[(size_t)((2 * 2 + reim) * 4 + lam) * nk];
+        }
+        dissipative_element(a.r[e], a.r[e + 1], Jr, A, B, Cc, floc);
+#pragma unroll
+        for (int c = 0; c < 4; c++) o[(size_t)(c * 2 + reim) * nk] = floc[c];
+    }
+}
+
+// Workgroup = (degree l, ts consecutive orders): rhs = reim*ts + (m - m0);
+// RW = 2 ts right-hand sides (64: one wavefront per row group).
+template <int RW>
 __global__ void __launch_bounds__(DRIFT_THREADS) k_drift(drift_args a) {
     extern __shared__ double F[];   // [ncol][rw]
     const int2 job = a.jobs[blockIdx.x];
     const int l = job.x, m0 = job.y;
-    const int ncol = 2 * a.nmn, nrow = ncol + 3, rw = a.rw, ts = a.ts, nk = a.nk;
+    constexpr int rw = RW, ts = RW / 2;
+    const int ncol = 2 * a.nmn, nrow = ncol + 3, nk = a.nk;
     const int nsl = min(ts, l + 1 - m0);
     const int kb = l * (l + 1) / 2 - 1 + m0;
     const double thr = a.skip_tol * a.maxnorm[0];
-    for (int t = threadIdx.x; t < ncol * rw; t += blockDim.x) F[t] = 0.0;
-    __syncthreads();
-    const int rhs = threadIdx.x % rw, grp = threadIdx.x / rw, ngrp = blockDim.x / rw;
+    const int rhs = threadIdx.x % rw, ngrp = DRIFT_THREADS / rw;
+    // with RW = 64 the row group is the wavefront index: uniform, so the G rows
+    // below stream through scalar loads instead of wave-wide vector loads
+    const int grp = RW == 64 ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : threadIdx.x / rw;
     const int ml = rhs % ts, reim = rhs / ts;
     const bool valid = ml < nsl;
     const bool act = valid && a.mnorm[kb + ml] > thr;
-    const double Jr = (double)l * (double)(l + 1);
-    // forcing: elements of one parity at a time, so no two threads of a pass
-    // touch the same node (only consecutive Maxwell elements share one)
-    for (int par = 0; par < 2; par++) {
+    // forcing at node q, dof uv: the upper-node term of the element below it
+    // plus the lower-node term of the element above it (assembly order of
+    // the reference: the element below comes first, src/vilma_viscoelastic.f90:441-480)
+    for (int c = grp; c < ncol; c += ngrp) {
+        double f = 0.0;
         if (act) {
-            for (int em = par + 2 * grp; em < a.nem; em += 2 * ngrp) {
-                const int e = a.emap[em], q = a.mnode[e];
-                const double* t = a.tau + (size_t)em * 24 * nk + kb + ml;
-                double A[4], B[4], Cc[4], floc[4];
-#pragma unroll
-                for (int lam = 0; lam < 4; lam++) {
-                    A[lam] = t[(size_t)((0 * 2 + reim) * 4 + lam) * nk];
-                    B[lam] = t[(size_t)((1 * 2 + reim) * 4 + lam) * nk];
-                    Cc[lam] = t[(size_t)((2 * 2 + reim) * 4 + lam) * nk];
-                }
-                dissipative_element(a.r[e], a.r[e + 1], Jr, A, B, Cc, floc);
-                F[(2 * q) * rw + rhs] += floc[0];
-                F[(2 * q + 2) * rw + rhs] += floc[1];
-                F[(2 * q + 1) * rw + rhs] += floc[2];
-                F[(2 * q + 3) * rw + rhs] += floc[3];
-            }
+            const int q = c >> 1, uv = c & 1;
+            const int2 ne = a.node_el[q];
+            const size_t kk = (size_t)reim * nk + kb + ml;
+            if (ne.x >= 0) f = f + a.Fe[((size_t)ne.x * 4 + (uv ? 3 : 1)) * 2 * nk + kk];
+            if (ne.y >= 0) f = f + a.Fe[((size_t)ne.y * 4 + (uv ? 2 : 0)) * 2 * nk + kk];
         }
-        __syncthreads();
+        F[c * rw + rhs] = f;
     }
+    __syncthreads();
     // Y = G_l F, four rows per thread per pass
-    const double* G = a.G + (size_t)(l - 1) * nrow * ncol;
+    const double* __restrict__ G = a.G + (size_t)(l - 1) * nrow * ncol;
     const int nq = (nrow + 3) / 4;
     const int k = kb + ml;
+    // G is read-only for the whole launch: read it through the constant
+    // address space, so the wave-uniform rows become scalar loads
+    typedef const __attribute__((address_space(4))) double cdouble;
     for (int qi = grp; qi < nq; qi += ngrp) {
         const int r0 = 4 * qi;
-        const double* g0 = G + (size_t)min(r0 + 0, nrow - 1) * ncol;
-        const double* g1 = G + (size_t)min(r0 + 1, nrow - 1) * ncol;
-        const double* g2 = G + (size_t)min(r0 + 2, nrow - 1) * ncol;
-        const double* g3 = G + (size_t)min(r0 + 3, nrow - 1) * ncol;
+        cdouble* g0 = (cdouble*)(G + (size_t)min(r0 + 0, nrow - 1) * ncol);
+        cdouble* g1 = (cdouble*)(G + (size_t)min(r0 + 1, nrow - 1) * ncol);
+        cdouble* g2 = (cdouble*)(G + (size_t)min(r0 + 2, nrow - 1) * ncol);
+        cdouble* g3 = (cdouble*)(G + (size_t)min(r0 + 3, nrow - 1) * ncol);
         double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
+#pragma unroll 4
         for (int c = 0; c < ncol; c++) {
             const double f = F[c * rw + rhs];
             y0 = fma(g0[c], f, y0);
@@ -168,13 +197,27 @@ __global__ void __launch_bounds__(DRIFT_THREADS) k_drift(drift_args a) {
 // ------------------------------------------------------------------ plan
 struct vh_drift_plan {
     int2* jobs = nullptr;
+    int2* node_el = nullptr;
+    double* Fe = nullptr;
     double* G = nullptr;
     int njobs = 0, ts = 32, rw = 64;
     size_t lds = 0;
 };
 
-vh_drift_plan* vh_drift_plan_create(hipStream_t st, const vh_dims& d, const vh_dev& v, int deg1_cm) {
+vh_drift_plan* vh_drift_plan_create(hipStream_t st, const vh_dims& d, const vh_dev& v, int deg1_cm,
+                                    const std::vector<int>& emap, const std::vector<int>& mnode_of) {
     vh_drift_plan* p = new vh_drift_plan();
+    {   // per memory node: the Maxwell element below (upper node = q) and above (lower node = q)
+        std::vector<int2> ne(std::max(d.nmn, 1), make_int2(-1, -1));
+        for (int em = 0; em < d.nem; em++) {
+            const int qlo = mnode_of[emap[em]], qup = mnode_of[emap[em] + 1];
+            ne[qlo].y = em;
+            ne[qup].x = em;
+        }
+        HIPCHECK(hipMalloc((void**)&p->node_el, ne.size() * sizeof(int2)));
+        HIPCHECK(hipMemcpy(p->node_el, ne.data(), ne.size() * sizeof(int2), hipMemcpyHostToDevice));
+        HIPCHECK(hipMalloc((void**)&p->Fe, (size_t)std::max(d.nem, 1) * 8 * d.nk * sizeof(double)));
+    }
     const int ncol = 2 * d.nmn, nrow = ncol + 3;
     // right-hand sides per workgroup: as many as fit a 64 KiB forcing tile
     int rw = 64;
@@ -200,7 +243,14 @@ vh_drift_plan* vh_drift_plan_create(hipStream_t st, const vh_dims& d, const vh_d
         HIPCHECK(hipStreamSynchronize(st));
         HIPCHECK(hipFree(w1));
     }
-    HIPCHECK(hipFuncSetAttribute((const void*)k_drift, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds));
+    switch (p->rw) {
+        case 64: HIPCHECK(hipFuncSetAttribute((const void*)k_drift<64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds)); break;
+        case 32: HIPCHECK(hipFuncSetAttribute((const void*)k_drift<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds)); break;
+        case 16: HIPCHECK(hipFuncSetAttribute((const void*)k_drift<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds)); break;
+        case 8: HIPCHECK(hipFuncSetAttribute((const void*)k_drift<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds)); break;
+        case 4: HIPCHECK(hipFuncSetAttribute((const void*)k_drift<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds)); break;
+        default: HIPCHECK(hipFuncSetAttribute((const void*)k_drift<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds)); break;
+    }
     return p;
 }
 
@@ -208,6 +258,8 @@ void vh_drift_plan_destroy(vh_drift_plan* p) {
     if (!p) return;
     (void)hipFree(p->jobs);
     (void)hipFree(p->G);
+    (void)hipFree(p->node_el);
+    (void)hipFree(p->Fe);
     delete p;
 }
 
@@ -216,9 +268,22 @@ void vh_drift_run(hipStream_t st, const vh_drift_plan* p, const vh_dims& d, cons
                   int deg1_cm, double* dN, double* dsurf) {
     drift_args a;
     a.nk = d.nk; a.nem = d.nem; a.nmn = d.nmn; a.ts = p->ts; a.rw = p->rw;
-    a.jobs = p->jobs; a.G = p->G; a.r = v.r; a.emap = v.emap; a.mnode = v.mnode;
+    a.jobs = p->jobs; a.G = p->G; a.r = v.r; a.emap = v.emap; a.mnode = v.mnode; a.kdeg = v.kdeg;
+    a.node_el = p->node_el; a.Fe = p->Fe;
     a.tau = tau; a.mnorm = mnorm; a.maxnorm = maxnorm;
     a.skip_tol = skip_tol; a.deg1_cm = deg1_cm; a.dN = dN; a.dsurf = dsurf;
-    k_drift<<<p->njobs, DRIFT_THREADS, p->lds, st>>>(a);
+    if (d.nem > 0) {
+        dim3 g((d.nk + 63) / 64, (d.nem + 3) / 4);
+        k_drift_force<<<g, 256, 0, st>>>(a);
+        HIPCHECK(hipGetLastError());
+    }
+    switch (p->rw) {
+        case 64: k_drift<64><<<p->njobs, DRIFT_THREADS, p->lds, st>>>(a); break;
+        case 32: k_drift<32><<<p->njobs, DRIFT_THREADS, p->lds, st>>>(a); break;
+        case 16: k_drift<16><<<p->njobs, DRIFT_THREADS, p->lds, st>>>(a); break;
+        case 8: k_drift<8><<<p->njobs, DRIFT_THREADS, p->lds, st>>>(a); break;
+        case 4: k_drift<4><<<p->njobs, DRIFT_THREADS, p->lds, st>>>(a); break;
+        default: k_drift<2><<<p->njobs, DRIFT_THREADS, p->lds, st>>>(a); break;
+    }
     HIPCHECK(hipGetLastError());
 }

@@ -20,6 +20,7 @@
 #include <stdint.h>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 struct vh_error : std::runtime_error { using std::runtime_error::runtime_error; };
 
@@ -153,7 +154,8 @@ struct vh_drift_plan;
 void vh_setup_restricted_ops(hipStream_t st, const vh_dims& d, const vh_dev& v, int lmin, int lmax, int nmn,
                              const int* mnodes, double* G);
 // builds the per-degree restricted solution operators from the factors in v
-vh_drift_plan* vh_drift_plan_create(hipStream_t st, const vh_dims& d, const vh_dev& v, int deg1_cm);
+vh_drift_plan* vh_drift_plan_create(hipStream_t st, const vh_dims& d, const vh_dev& v, int deg1_cm,
+                                    const std::vector<int>& emap, const std::vector<int>& mnode_of);
 void vh_drift_plan_destroy(vh_drift_plan* p);
 void vh_drift_run(hipStream_t st, const vh_drift_plan* p, const vh_dims& d, const vh_dev& v,
                   const double* tau, const double* mnorm, const double* maxnorm, double skip_tol,
