@@ -198,6 +198,7 @@ struct vh_model {
     double *d_sraw = 0, *d_srot = 0, *d_zbed = 0, *d_gtmp = 0;
     double2 *d_F = 0, *d_F2 = 0, *d_sigma = 0, *d_lm = 0, *d_uN = 0;
     double *d_part = 0, *d_sc = 0;
+    unsigned* d_sbar = 0;   // grid-barrier counter + timeout flag of the fused SLE loop
     double* h_pin = nullptr;
     // rotation
     bool rot_on = false;
@@ -266,7 +267,7 @@ vh_model::~vh_model() {
                     d_mnorm[0], d_mnorm[1], d_maxn, d_dN, d_dsurf, d_plm_lj, d_dplm_lj, d_plm_jl, d_dplm_jl, d_w,
                     d_st, d_ct, d_mstart, d_tw, d_twp, d_ljobs, d_topo0, d_heq, d_C0ref, d_hice, d_ice1, d_rsl, d_rsl_n, d_C,
                     d_Cn, d_dig, d_C0s, d_ice, d_dice, d_wcorr, d_sraw, d_srot, d_zbed, d_gtmp, d_F, d_F2,
-                    d_sigma, d_lm, d_uN, d_part, d_sc, d_sc2, d_cphi, d_sphi,
+                    d_sigma, d_lm, d_uN, d_part, d_sc, d_sbar, d_sc2, d_cphi, d_sphi,
                     ra.rs.mem, ra.rs.MkPerDt, ra.rs.xU, ra.rs.xV, ra.rs.dN, ra.rs.sc, d_emap2, d_mnodes2,
                     d_mnode2, d_G2, d_e3em, d_e3of, d_rate3, d_n6, d_X, d_Fd, d_cdel};
     for (void* p : ptrs)
@@ -424,15 +425,17 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
     h->d_st = dalloc<double>(nl); upload(h->d_st, sn.data(), nl, st);
     h->d_mstart = dalloc<int>(Lmax + 1); upload(h->d_mstart, mstart.data(), Lmax + 1, st);
     h->d_tw = dalloc<double2>(np); upload(h->d_tw, tw.data(), np, st);
-    {   // per-pass Stockham twiddles e^{-2 pi i (j mod Ns) r tstep / N}, [r-1][j]
+    {   // per-pass Stockham twiddles e^{-2 pi i k r tstep / N}, [r-1][k], k < Ns
         std::vector<double2> twp;
         for (int p = 0, Ns = 1; p < s.nfac; p++) {
-            const int R = s.fac[p], NR = np / R, tstep = np / (Ns * R);
+            const int R = s.fac[p], tstep = np / (Ns * R);
             s.twoff[p] = (int)twp.size();
-            for (int r = 1; r < R; r++)
-                for (int j = 0; j < NR; j++) twp.push_back(tw[(size_t)((j % Ns) * r * tstep) % np]);
+            if (Ns > 1)
+                for (int r = 1; r < R; r++)
+                    for (int k = 0; k < Ns; k++) twp.push_back(tw[(size_t)(k * r * tstep) % np]);
             Ns *= R;
         }
+        s.ntw = (int)twp.size();
         h->d_twp = dalloc<double2>(std::max<size_t>(twp.size(), 1)); upload(h->d_twp, twp.data(), twp.size(), st);
         h->sync();   // twp goes out of scope
     }
@@ -470,6 +473,7 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
     h->d_uN = dalloc<double2>((size_t)2 * d.nlm);
     h->d_part = dalloc<double>((size_t)4 * nl + 64);
     h->d_sc = dalloc<double>(64); vh_fill(st, h->d_sc, 0.0, 64);
+    h->d_sbar = dalloc<unsigned>(2);
 
     // ---- SLE / stepper knobs (build_solver, src/vilma_coupling.f90:291-308)
     h->n_outer = p->sle_n_outer; h->n_inner = p->sle_n_inner; h->tol = p->sle_tol;
@@ -618,6 +622,9 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
     }
     vh_fill(st, h->d_wcorr, 0.0, n);
     double C_int = 0.0, ice_int = 0.0, zeta_int = 0.0;
+    // the inner loop leaves F = transform of the load of the final rsl under
+    // the current mask; coastline flips only take effect in a next pass
+    bool load_current = false;
     for (int io = 1; io <= h->n_outer; io++) {
         int first = 0;
         if (h->fixed_ocean) {
@@ -627,29 +634,30 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
         }
         sle_outer(st, s, first, h->d_topo0, h->d_rsl, h->d_ice, h->d_dice, h->d_C0s, h->d_Cn, h->d_C, h->d_dig,
                   h->d_part, h->d_sc);
+        load_current = false;
         h->read(h->d_sc, 3);
         C_int = h->h_pin[0];
         if (C_int <= 0.0) break;
         ice_int = -(VH_RHO_ICE / VH_RHO_WATER) * h->h_pin[1];
         zeta_int = h->subgrid ? h->h_pin[2] : 0.0;
         sle_wcorr(st, n, h->subgrid, h->d_topo0, h->d_C, h->d_C0s, h->d_wcorr);
-        for (int ii = 1; ii <= h->n_inner; ii++) {
-            sle_load_analysis(st, s, h->d_dig, h->d_C, h->d_rsl, h->d_wcorr, h->d_F, h->d_sigma);
-            // part: [0, 2 nlat) row maxima + [2 nlat] dphi (read back), [3 nlat, 4 nlat) row sums of C Sraw
-            double* cs = h->d_part + 3 * (size_t)s.nlat;
-            sle_apply_synth(st, s, h->d_sigma, h->d_gu, h->d_gn, h->d_dsurf, h->d.nk, h->g, h->kind, h->d_F2, srot,
-                            h->d_C, h->d_sraw, cs);
-            sle_update(st, s, h->d_sraw, h->d_C, h->d_rsl, h->d_sc, h->subgrid, cs, h->d_part);
-            h->read(h->d_part, 2 * s.nlat + 1);
-            double dmax = 0.0, smax = 0.0;   // max is exact in any order
-            for (int q = 0; q < s.nlat; q++) {
-                dmax = std::max(dmax, h->h_pin[q]);
-                smax = std::max(smax, h->h_pin[s.nlat + q]);
-            }
-            res.esl = h->h_pin[2 * s.nlat];
-            res.n_inner_last = ii;
-            if (dmax <= h->tol * std::max(smax, DBL_MIN)) break;
-        }
+        // the inner fixed point on the device; part: [0, 2 nlat) row maxima,
+        // [3 nlat, 4 nlat) row sums of C Sraw; sc[16..18] the loop's result
+        sle_inner_args ia{};
+        ia.s = s; ia.dig = h->d_dig; ia.C = h->d_C; ia.wcorr = h->d_wcorr; ia.srot = srot;
+        ia.gu = h->d_gu; ia.gn = h->d_gn; ia.dsurf = h->d_dsurf; ia.sc = h->d_sc;
+        ia.nk = h->d.nk; ia.kind = h->kind; ia.subgrid = h->subgrid; ia.n_inner = h->n_inner;
+        ia.g = h->g; ia.tol = h->tol; ia.rsl = h->d_rsl; ia.cs = h->d_part + 3 * (size_t)s.nlat;
+        ia.part = h->d_part; ia.res = h->d_sc + 16; ia.F = h->d_F; ia.F2 = h->d_F2; ia.sigma = h->d_sigma;
+        ia.bar = h->d_sbar;
+        h->prof_begin(VH_K_SLE);
+        sle_inner(st, ia);
+        h->prof_end();
+        h->read(h->d_sc + 16, 3);
+        if (h->h_pin[2] != 0.0) throw vh_error("sle_solve: a grid barrier of the fused inner loop timed out");
+        res.n_inner_last = (int)h->h_pin[0];
+        if (res.n_inner_last > 0) res.esl = h->h_pin[1];
+        load_current = h->n_inner > 0;
         res.n_outer_done = io;
         h->n_outer_tot++;
         h->n_inner_tot += res.n_inner_last;
@@ -660,7 +668,8 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
         if (res.n_coast_flip == 0) break;
     }
     // closing load -> sigma, then the memory advance
-    sle_load_analysis(st, s, h->d_dig, h->d_C, h->d_rsl, h->d_wcorr, h->d_F, h->d_sigma);
+    if (load_current) vh_legendre_analysis(st, s, h->d_F, 1, h->d_sigma);
+    else sle_load_analysis(st, s, h->d_dig, h->d_C, h->d_rsl, h->d_wcorr, h->d_F, h->d_sigma);
     int nxt = h->cur ^ 1;
     if (!report_only && h->kind == 2) advance_memory(h, (const double*)h->d_sigma, nxt);
     if (!report_only) h->time_resp += h->dt;

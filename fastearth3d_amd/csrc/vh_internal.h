@@ -106,8 +106,8 @@ struct vh_sht {
     const double* st;      // [nlat] sin(colat)
     const double* ct;      // [nlat]
     const double2* tw;     // [nphi] e^{-2 pi i t/nphi}
-    const double2* twp;    // per-pass FFT twiddles, pass p at twoff[p]: [R-1][nphi/R]
-    int nfac, fac[16], twoff[16];
+    const double2* twp;    // per-pass FFT twiddles, pass p at twoff[p]: [R-1][Ns] (Ns = product of earlier radices)
+    int nfac, fac[16], twoff[16], ntw;
     const int2* ljobs;     // Legendre analysis blocks: (m, first degree offset), 64 degrees each
     int nljobs;
 };

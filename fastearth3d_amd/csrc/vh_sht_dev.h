@@ -106,7 +106,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // One Stockham radix-R pass over N points. tp: this pass's twiddles
-// [r-1][j] = e^{-2 pi i (j mod Ns) r tstep/N} (coalesced over j).
+// [r-1][k] = e^{-2 pi i k r tstep/N}, k = j mod Ns < Ns (none when Ns = 1: the
+// first pass multiplies by 1).
 template <int R>
 __device__ __forceinline__ void fft_pass(const double2* src, double2* dst, int N, int Ns, const double2* tp,
                                          bool inv, int t, int nthr) {
@@ -116,8 +117,10 @@ __device__ __forceinline__ void fft_pass(const double2* src, double2* dst, int N
         double2 v[R], V[R];
 #pragma unroll
         for (int r = 0; r < R; r++) v[r] = src[j + r * NR];
+        if (Ns > 1) {
 #pragma unroll
-        for (int r = 1; r < R; r++) v[r] = cmul(v[r], cconj_if(tp[(r - 1) * NR + j], inv));
+            for (int r = 1; r < R; r++) v[r] = cmul(v[r], cconj_if(tp[(r - 1) * Ns + k], inv));
+        }
         dft_small<R>(v, V, inv);
         const int idxD = (j / Ns) * Ns * R + k;
 #pragma unroll
@@ -128,14 +131,16 @@ __device__ __forceinline__ void fft_pass(const double2* src, double2* dst, int N
 // One complex FFT of length s.nphi by the calling wavefront alone (lane
 // 0..63): a, b are this wavefront's LDS buffers of N points; returns the one
 // holding the result. Natural order in and out (Stockham autosort). The
-// caller makes a's contents visible to the whole wavefront first.
-__device__ double2* fft_wave(double2* a, double2* b, const vh_sht& s, bool inv, int lane) {
+// caller makes a's contents visible to the whole wavefront first. tw: a copy
+// of the s.ntw per-pass twiddles (e.g. in LDS), or null for s.twp.
+__device__ double2* fft_wave(double2* a, double2* b, const vh_sht& s, bool inv, int lane,
+                             const double2* tw = nullptr) {
     const int N = s.nphi;
     int Ns = 1;
     double2 *src = a, *dst = b;
     for (int p = 0; p < s.nfac; p++) {
         const int R = s.fac[p];
-        const double2* tp = s.twp + s.twoff[p];
+        const double2* tp = (tw ? tw : s.twp) + s.twoff[p];
         switch (R) {
             case 8: fft_pass<8>(src, dst, N, Ns, tp, inv, lane, 64); break;
             case 6: fft_pass<6>(src, dst, N, Ns, tp, inv, lane, 64); break;
@@ -150,6 +155,80 @@ __device__ double2* fft_wave(double2* a, double2* b, const vh_sht& s, bool inv, 
         Ns *= R;
     }
     return src;
+}
+
+// Loads and stores of data that other blocks of a running cooperative launch
+// read or wrote: relaxed atomics at agent scope, which the compiler issues
+// coherent across the XCDs' L2 caches.
+__device__ __forceinline__ double ld_coh(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double2 ld_coh(const double2* p) {
+    return make_double2(ld_coh((const double*)p), ld_coh((const double*)p + 1));
+}
+__device__ __forceinline__ void st_coh(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh(double2* p, double2 v) {
+    st_coh((double*)p, v.x);
+    st_coh((double*)p + 1, v.y);
+}
+template <bool Coh>
+__device__ __forceinline__ double2 ld_g(const double2* p) {
+    if constexpr (Coh) return ld_coh(p); else return *p;
+}
+template <bool Coh>
+__device__ __forceinline__ void st_g(double2* p, double2 v) {
+    if constexpr (Coh) st_coh(p, v); else *p = v;
+}
+
+// a <- the packed complex row Z = A + iB of two real rows given by their half
+// spectra A = F0[m], B = F1[m] (m = 0..L; imaginary parts at m = 0 dropped):
+// Z_m = A_m + i B_m, Z_{N-m} = conj(A_m) + i conj(B_m). A null F0 / F1 reads
+// zeros. Called by the whole wavefront; a is complete and visible on return.
+// Coh: F0/F1 are read with coherent loads (written by other running blocks).
+template <bool Coh = false>
+__device__ __forceinline__ void pair_spectra_to_lds(const double2* F0, const double2* F1, int L, int N, double2* a,
+                                                    int lane) {
+    for (int i = lane; i < N; i += 64) a[i] = make_double2(0.0, 0.0);
+    wave_lds_sync();
+    if (F0) {
+        for (int m0 = 0; m0 <= L; m0 += 64 * VH_FFT_U) {
+            double2 A[VH_FFT_U], B[VH_FFT_U];
+#pragma unroll
+            for (int u = 0; u < VH_FFT_U; u++) {
+                const int m = m0 + lane + 64 * u;
+                A[u] = m <= L ? ld_g<Coh>(F0 + m) : make_double2(0.0, 0.0);
+                B[u] = (F1 && m <= L) ? ld_g<Coh>(F1 + m) : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < VH_FFT_U; u++) {
+                const int m = m0 + lane + 64 * u;
+                if (m > L) continue;
+                if (m == 0) { A[u].y = 0.0; B[u].y = 0.0; }
+                a[m] = make_double2(A[u].x - B[u].y, A[u].y + B[u].x);
+                if (m > 0) a[N - m] = make_double2(A[u].x + B[u].y, -A[u].y + B[u].x);
+            }
+        }
+    }
+    wave_lds_sync();
+}
+
+// The FFT z of a packed pair of real rows split into their half spectra
+// (m = 0..L), scaled: F0[m] = scale (Z_m + conj Z_{N-m})/2 and
+// F1[m] = scale (Z_m - conj Z_{N-m})/(2i); F1 may be null. Coh: coherent stores.
+template <bool Coh = false>
+__device__ __forceinline__ void lds_to_pair_spectra(const double2* z, int L, int N, double scale, double2* F0,
+                                                    double2* F1, int lane) {
+    for (int m = lane; m <= L; m += 64) {
+        double2 zm = z[m], zc = z[(N - m) % N];
+        zc.y = -zc.y;
+        const double2 A = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+        const double2 D = csub(zm, zc);
+        const double2 B = make_double2(0.5 * D.y, -0.5 * D.x);
+        st_g<Coh>(F0 + m, make_double2(scale * A.x, scale * A.y));
+        if (F1) st_g<Coh>(F1 + m, make_double2(scale * B.x, scale * B.y));
+    }
 }
 
 // Real rows -> F[row][m] (m = 0..L) scaled by `scale`; one row pair per
@@ -180,17 +259,7 @@ __global__ void __launch_bounds__(256) k_fft_fwd(vh_sht s, In in, int nrows, dou
     }
     wave_lds_sync();
     double2* z = fft_wave(a, b, s, false, lane);
-    if (has0) {
-        for (int m = lane; m <= L; m += 64) {
-            double2 zm = z[m], zc = z[(N - m) % N];
-            zc.y = -zc.y;
-            double2 A = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
-            double2 D = csub(zm, zc);
-            double2 B = make_double2(0.5 * D.y, -0.5 * D.x);
-            F[(size_t)r0 * (L + 1) + m] = make_double2(scale * A.x, scale * A.y);
-            if (has1) F[(size_t)r1 * (L + 1) + m] = make_double2(scale * B.x, scale * B.y);
-        }
-    }
+    if (has0) lds_to_pair_spectra(z, L, N, scale, F + (size_t)r0 * (L + 1), has1 ? F + (size_t)r1 * (L + 1) : nullptr, lane);
 }
 
 // F[row][m] -> real rows f = Re F0 + 2 Re sum_{m>0} F_m e^{i m phi}.
@@ -203,28 +272,8 @@ __global__ void __launch_bounds__(256) k_fft_inv(vh_sht s, const double2* __rest
     const int pair = blockIdx.x * (blockDim.x >> 6) + wave;
     const int r0 = 2 * pair, r1 = 2 * pair + 1;
     const bool has0 = r0 < nrows, has1 = r1 < nrows;
-    for (int i = lane; i < N; i += 64) a[i] = make_double2(0.0, 0.0);
-    wave_lds_sync();
-    if (has0) {
-        for (int m0 = 0; m0 <= L; m0 += 64 * VH_FFT_U) {
-            double2 A[VH_FFT_U], B[VH_FFT_U];
-#pragma unroll
-            for (int u = 0; u < VH_FFT_U; u++) {
-                const int m = m0 + lane + 64 * u;
-                A[u] = m <= L ? F[(size_t)r0 * (L + 1) + m] : make_double2(0.0, 0.0);
-                B[u] = (has1 && m <= L) ? F[(size_t)r1 * (L + 1) + m] : make_double2(0.0, 0.0);
-            }
-#pragma unroll
-            for (int u = 0; u < VH_FFT_U; u++) {
-                const int m = m0 + lane + 64 * u;
-                if (m > L) continue;
-                if (m == 0) { A[u].y = 0.0; B[u].y = 0.0; }
-                a[m] = make_double2(A[u].x - B[u].y, A[u].y + B[u].x);
-                if (m > 0) a[N - m] = make_double2(A[u].x + B[u].y, -A[u].y + B[u].x);
-            }
-        }
-    }
-    wave_lds_sync();
+    pair_spectra_to_lds(has0 ? F + (size_t)r0 * (L + 1) : nullptr, has1 ? F + (size_t)r1 * (L + 1) : nullptr, L, N,
+                        a, lane);
     double2* z = fft_wave(a, b, s, true, lane);
     double rs = 0.0;   // consumers may return a per-row term to be summed
     if (has0)

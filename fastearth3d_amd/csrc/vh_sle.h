@@ -13,23 +13,31 @@ void sle_wcorr(hipStream_t st, int n, int subgrid, const double* topo0, const do
 // load -> load_lm (degree-major)
 void sle_load_analysis(hipStream_t st, const vh_sht& s, const double* dig, const double* C, const double* rsl,
                        const double* wcorr, double2* F, double2* load_lm);
-// apply (kind 0 null / 1 elastic / 2 ve) inside the synthesis of (u, N) ->
-// Sraw = N - u (+ s_rot), and cs[j] = sum_i C Sraw over row j
-void sle_apply_synth(hipStream_t st, const vh_sht& s, const double2* sigma, const double* gu, const double* gn,
-                     const double* dsurf, int nk, double g, int kind, double2* F2, const double* srot,
-                     const double* C, double* sraw, double* cs);
 // response_apply only: uN = [u; N] (degree-major) from sigma
 void vh_apply_only(hipStream_t st, int nlm, const double2* sigma, const double* gu, const double* gn,
                    const double* dsurf, int nk, double g, int kind, double2* uN);
 // out[0] = int C f
 void sle_cs(hipStream_t st, const vh_sht& s, const double* C, const double* f, double* part, double* out);
-// Cs_int from the row sums cs, dphi, rsl update; sc[3] = Cs_int, sc[4] = dphi;
-// out: per-row dmax [0, nlat), smax [nlat, 2 nlat), dphi at [2 nlat]
-void sle_update(hipStream_t st, const vh_sht& s, const double* sraw, const double* C, double* rsl, double* sc,
-                int subgrid, const double* cs, double* out);
 // C_next and out[0] = number of flipped cells
 void sle_flip(hipStream_t st, const vh_sht& s, const double* topo0, const double* rsl, const double* ice,
               const double* C, double* Cnext, double* part, double* out);
+// The inner fixed point of the sea-level equation (src/vilma_sle.f90:296-330)
+// as one cooperative launch: at most n_inner iterations of load analysis,
+// response, synthesis and rsl update, with the exit test on the device.
+// F must hold nothing on entry (it is rebuilt from rsl); on return rsl, sigma
+// (the last load) and F (the load of the final rsl) are current.
+// res[0] = iterations run, res[1] = dphi of the last one, res[2] = 1 if a grid
+// barrier wait timed out. bar: 2 unsigned of scratch.
+struct sle_inner_args {
+    vh_sht s;
+    const double *dig, *C, *wcorr, *srot, *gu, *gn, *dsurf, *sc;
+    int nk, kind, subgrid, n_inner;
+    double g, tol;
+    double *rsl, *cs, *part, *res;
+    double2 *F, *F2, *sigma;
+    unsigned* bar;
+};
+void sle_inner(hipStream_t st, const sle_inner_args& a);
 void grid_sub(hipStream_t st, int n, const double* a, const double* b, double* out);
 void grid_ocean(hipStream_t st, int n, const double* topo, const double* ice, double* C);
 void grid_bsl_integrand(hipStream_t st, int n, const double* h, const double* C, const double* heq,
