@@ -196,7 +196,7 @@ struct vh_model {
     double *d_topo0 = 0, *d_heq = 0, *d_C0ref = 0, *d_hice = 0, *d_ice1 = 0, *d_rsl = 0, *d_rsl_n = 0;
     double *d_C = 0, *d_Cn = 0, *d_dig = 0, *d_C0s = 0, *d_ice = 0, *d_dice = 0, *d_wcorr = 0;
     double *d_sraw = 0, *d_srot = 0, *d_zbed = 0, *d_gtmp = 0;
-    double2 *d_F = 0, *d_F2 = 0, *d_sigma = 0, *d_lm = 0, *d_uN = 0;
+    double2 *d_F = 0, *d_FC = 0, *d_F2 = 0, *d_sigma = 0, *d_lm = 0, *d_uN = 0;
     double *d_part = 0, *d_sc = 0;
     unsigned* d_sbar = 0;   // grid-barrier counter + timeout flag of the fused SLE loop
     double* h_pin = nullptr;
@@ -266,7 +266,7 @@ vh_model::~vh_model() {
                     d_ab1, d_null1, d_gu, d_gn, d_gv, d_xUn, d_xVn, d_ipiv, d_ipiv1, d_fail, d_tau[0], d_tau[1],
                     d_mnorm[0], d_mnorm[1], d_maxn, d_dN, d_dsurf, d_plm_lj, d_dplm_lj, d_plm_jl, d_dplm_jl, d_w,
                     d_st, d_ct, d_mstart, d_tw, d_twp, d_ljobs, d_topo0, d_heq, d_C0ref, d_hice, d_ice1, d_rsl, d_rsl_n, d_C,
-                    d_Cn, d_dig, d_C0s, d_ice, d_dice, d_wcorr, d_sraw, d_srot, d_zbed, d_gtmp, d_F, d_F2,
+                    d_Cn, d_dig, d_C0s, d_ice, d_dice, d_wcorr, d_sraw, d_srot, d_zbed, d_gtmp, d_F, d_FC, d_F2,
                     d_sigma, d_lm, d_uN, d_part, d_sc, d_sbar, d_sc2, d_cphi, d_sphi,
                     ra.rs.mem, ra.rs.MkPerDt, ra.rs.xU, ra.rs.xV, ra.rs.dN, ra.rs.sc, d_emap2, d_mnodes2,
                     d_mnode2, d_G2, d_e3em, d_e3of, d_rate3, d_n6, d_X, d_Fd, d_cdel};
@@ -467,7 +467,8 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
     grid_ocean(st, (int)ng, h->d_topo0, h->d_heq, h->d_C0ref);
     HIPCHECK(hipMemcpyAsync(h->d_C, h->d_C0ref, ng * sizeof(double), hipMemcpyDeviceToDevice, st));
     h->d_F = dalloc<double2>((size_t)nl * (Lmax + 1));
-    h->d_F2 = dalloc<double2>((size_t)2 * nl * (Lmax + 1));
+    h->d_FC = dalloc<double2>((size_t)nl * (Lmax + 1));
+    h->d_F2 = dalloc<double2>((size_t)4 * ((nl + 1) / 2) * (Lmax + 1));
     h->d_sigma = dalloc<double2>(d.nlm); vh_fill(st, (double*)h->d_sigma, 0.0, 2 * (size_t)d.nlm);
     h->d_lm = dalloc<double2>(d.nlm);
     h->d_uN = dalloc<double2>((size_t)2 * d.nlm);
@@ -622,8 +623,8 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
     }
     vh_fill(st, h->d_wcorr, 0.0, n);
     double C_int = 0.0, ice_int = 0.0, zeta_int = 0.0;
-    // the inner loop leaves F = transform of the load of the final rsl under
-    // the current mask; coastline flips only take effect in a next pass
+    // the inner loop leaves sigma = spectrum of the load of the final rsl
+    // under the current mask; coastline flips only take effect in a next pass
     bool load_current = false;
     for (int io = 1; io <= h->n_outer; io++) {
         int first = 0;
@@ -648,7 +649,7 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
         ia.gu = h->d_gu; ia.gn = h->d_gn; ia.dsurf = h->d_dsurf; ia.sc = h->d_sc;
         ia.nk = h->d.nk; ia.kind = h->kind; ia.subgrid = h->subgrid; ia.n_inner = h->n_inner;
         ia.g = h->g; ia.tol = h->tol; ia.rsl = h->d_rsl; ia.cs = h->d_part + 3 * (size_t)s.nlat;
-        ia.part = h->d_part; ia.res = h->d_sc + 16; ia.F = h->d_F; ia.F2 = h->d_F2; ia.sigma = h->d_sigma;
+        ia.part = h->d_part; ia.res = h->d_sc + 16; ia.F = h->d_F; ia.FC = h->d_FC; ia.F2 = h->d_F2; ia.sigma = h->d_sigma;
         ia.bar = h->d_sbar;
         h->prof_begin(VH_K_SLE);
         sle_inner(st, ia);
@@ -657,7 +658,7 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
         if (h->h_pin[2] != 0.0) throw vh_error("sle_solve: a grid barrier of the fused inner loop timed out");
         res.n_inner_last = (int)h->h_pin[0];
         if (res.n_inner_last > 0) res.esl = h->h_pin[1];
-        load_current = h->n_inner > 0;
+        load_current = true;
         res.n_outer_done = io;
         h->n_outer_tot++;
         h->n_inner_tot += res.n_inner_last;
@@ -668,8 +669,7 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
         if (res.n_coast_flip == 0) break;
     }
     // closing load -> sigma, then the memory advance
-    if (load_current) vh_legendre_analysis(st, s, h->d_F, 1, h->d_sigma);
-    else sle_load_analysis(st, s, h->d_dig, h->d_C, h->d_rsl, h->d_wcorr, h->d_F, h->d_sigma);
+    if (!load_current) sle_load_analysis(st, s, h->d_dig, h->d_C, h->d_rsl, h->d_wcorr, h->d_F, h->d_sigma);
     int nxt = h->cur ^ 1;
     if (!report_only && h->kind == 2) advance_memory(h, (const double*)h->d_sigma, nxt);
     if (!report_only) h->time_resp += h->dt;

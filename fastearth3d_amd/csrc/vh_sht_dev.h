@@ -187,9 +187,10 @@ __device__ __forceinline__ void st_g(double2* p, double2 v) {
 // Z_m = A_m + i B_m, Z_{N-m} = conj(A_m) + i conj(B_m). A null F0 / F1 reads
 // zeros. Called by the whole wavefront; a is complete and visible on return.
 // Coh: F0/F1 are read with coherent loads (written by other running blocks).
+// mstride: distance between consecutive orders in F0/F1.
 template <bool Coh = false>
 __device__ __forceinline__ void pair_spectra_to_lds(const double2* F0, const double2* F1, int L, int N, double2* a,
-                                                    int lane) {
+                                                    int lane, int mstride = 1) {
     for (int i = lane; i < N; i += 64) a[i] = make_double2(0.0, 0.0);
     wave_lds_sync();
     if (F0) {
@@ -198,8 +199,8 @@ __device__ __forceinline__ void pair_spectra_to_lds(const double2* F0, const dou
 #pragma unroll
             for (int u = 0; u < VH_FFT_U; u++) {
                 const int m = m0 + lane + 64 * u;
-                A[u] = m <= L ? ld_g<Coh>(F0 + m) : make_double2(0.0, 0.0);
-                B[u] = (F1 && m <= L) ? ld_g<Coh>(F1 + m) : make_double2(0.0, 0.0);
+                A[u] = m <= L ? ld_g<Coh>(F0 + (size_t)m * mstride) : make_double2(0.0, 0.0);
+                B[u] = (F1 && m <= L) ? ld_g<Coh>(F1 + (size_t)m * mstride) : make_double2(0.0, 0.0);
             }
 #pragma unroll
             for (int u = 0; u < VH_FFT_U; u++) {

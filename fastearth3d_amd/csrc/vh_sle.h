@@ -24,8 +24,9 @@ void sle_flip(hipStream_t st, const vh_sht& s, const double* topo0, const double
 // The inner fixed point of the sea-level equation (src/vilma_sle.f90:296-330)
 // as one cooperative launch: at most n_inner iterations of load analysis,
 // response, synthesis and rsl update, with the exit test on the device.
-// F must hold nothing on entry (it is rebuilt from rsl); on return rsl, sigma
-// (the last load) and F (the load of the final rsl) are current.
+// F, FC, F2 are scratch spectra ([nlat][L+1], [nlat][L+1], [(nlat+1)/2][L+1][4]).
+// On return rsl is updated and sigma is the spectrum of the load of the final
+// rsl under the current mask (the solve's closing analysis).
 // res[0] = iterations run, res[1] = dphi of the last one, res[2] = 1 if a grid
 // barrier wait timed out. bar: 2 unsigned of scratch.
 struct sle_inner_args {
@@ -34,7 +35,7 @@ struct sle_inner_args {
     int nk, kind, subgrid, n_inner;
     double g, tol;
     double *rsl, *cs, *part, *res;
-    double2 *F, *F2, *sigma;
+    double2 *F, *FC, *F2, *sigma;
     unsigned* bar;
 };
 void sle_inner(hipStream_t st, const sle_inner_args& a);

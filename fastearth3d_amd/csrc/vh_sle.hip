@@ -176,29 +176,37 @@ __global__ void k_bsl_integrand(int n, const double* __restrict__ h, const doubl
 
 // ---------------------------------------------------------------- fused inner loop
 // The inner fixed point of sle_solve (src/vilma_sle.f90:296-330) in one
-// cooperative launch, so no iteration waits on the host:
-//   per iteration  B: order m (block m): Legendre analysis of the load's row
-//                     spectra -> sigma_lm, response applied (u, N), Legendre
-//                     synthesis of u and N -> row spectra F2
-//                  C: row pair p (block p, waves 0/1 = rows 2p, 2p+1): inverse
-//                     FFT of (u_j, N_j) -> Sraw_j kept in LDS, cs_j = sum C Sraw
-//                  D: row pair p: Cs_int = sum_j w_j dlon cs_j (every block, same
-//                     order), dphi, rsl update, row maxima of |C drsl|, |C rsl|;
-//                     then the next load of rows 2p, 2p+1 and its forward FFT
-//   and every block takes the same exit decision from the row maxima.
-// Each stage's arithmetic is fixed by the sizes alone (partition of the
-// Legendre sums, order of the Cs_int butterfly), so every block reaches the
-// same values and a rerun gives identical bits.
-// Data handed between blocks (F, F2, cs, part) moves through coherent loads and
-// stores (ld_coh / st_coh), so a grid barrier needs no cache write-back or
-// invalidate: each wavefront waits for its stores, then one counter per launch
-// (bar[0], zeroed before the launch) is raised and awaited. The wait is bounded:
-// a block that never arrives raises bar[1] and res[2] instead of a hang.
+// cooperative launch, so no iteration waits on the host. Row blocks own a
+// latitude pair p (rows 2p, 2p+1), order blocks an order m.
+//   start  rows:   FC = FFT(C), F = FFT(load(rsl))            [barrier]
+//          orders: F -> sigma -> (u, N) -> F2                  [barrier]
+//   per iteration
+//          rows:   inverse FFT of (u_j, N_j) -> Sraw_j (kept in LDS),
+//                  cs_j = sum C Sraw; base = rho_i dI + rho_w C Sraw + wcorr,
+//                  Fb = FFT(base)                              [barrier]
+//          all:    Cs_int = sum_j w_j dlon cs_j, dphi
+//          rows:   rsl = Sraw + dphi, row maxima of |C drsl|, |C rsl|
+//          orders: the next load's spectrum Fb + rho_w dphi FC (the transform
+//                  of rho_i dI + rho_w C rsl + wcorr, by linearity) ->
+//                  sigma -> (u, N) -> F2                        [barrier]
+//          all:    the same exit decision from the row maxima.
+// On exit sigma is the spectrum of the load of the final rsl: the closing
+// analysis of the solve. Each stage's arithmetic is fixed by the sizes alone
+// (partition of the Legendre sums, order of the Cs_int butterfly), so every
+// block reaches the same values and a rerun gives identical bits.
+// Data handed between blocks (F, FC, F2, cs, part) moves through coherent
+// loads and stores (ld_coh / st_coh), so a grid barrier needs no cache
+// write-back or invalidate: each wavefront waits for its stores, then one
+// counter per launch (bar[0], zeroed before the launch) is raised and awaited.
+// The wait is bounded: a block that never arrives raises bar[1] and res[2]
+// instead of a hang.
 
 #define VH_SLE_SPIN_LIMIT 2000000u
 // P-table values a thread has in flight in the order stage
-#define VH_SLE_JB 16
-#define VH_SLE_LB 8
+#define VH_SLE_JB 8
+#define VH_SLE_LB 4
+// grid values per lane a row wavefront loads ahead of its transform
+#define VH_SLE_PU 6
 
 __device__ __forceinline__ void sle_grid_sync(unsigned* bar, unsigned& target, double* res) {
     target += gridDim.x;
@@ -221,18 +229,212 @@ __device__ __forceinline__ void sle_grid_sync(unsigned* bar, unsigned& target, d
     __syncthreads();
 }
 
-// LDS: [2][nphi] doubles Sraw of the block's row pair (kept from C to D);
-// with TabLds the block's Legendre table [j][l] (row stride (L+1)|1 doubles,
-// loaded once per launch); the FFT twiddles (s.ntw, loaded once per launch);
-// then a union of the row stages' FFT buffers
-// (2 wavefronts x 2 nphi complex, the update's copy of w_j and cs_j after the
-// first two) and the order stage's folds, partial sums and (sigma, u, N).
-__host__ __device__ static size_t sle_tab_doubles(const vh_sht& s) { return (((size_t)s.nh * ((s.L + 1) | 1)) + 1) & ~(size_t)1; }
+// LDS of a row block: Sraw [2][nphi] doubles (kept across a barrier), the FFT
+// twiddles (s.ntw complex) and three FFT buffer pairs (3 x 2 nphi complex).
+// Of an order block: with TabLds its Legendre
+// table [j][l] (row stride (L+1)|1 doubles, loaded once per launch), then the
+// folds, partial sums and (sigma, u, N).
+__host__ __device__ static size_t sle_tab_doubles(const vh_sht& s) {
+    return (((size_t)s.nh * ((s.L + 1) | 1)) + 1) & ~(size_t)1;
+}
 static size_t sle_inner_lds(const vh_sht& s, bool tab) {
-    const size_t rows = (size_t)2 * s.nphi + std::max((size_t)2 * s.nphi, (size_t)s.nlat);
-    const size_t order = (size_t)2 * s.nh + std::max(256, s.L + 1) + 3 * (s.L + 1) + 16 * (size_t)s.nh;
-    return (size_t)2 * s.nphi * sizeof(double) + (tab ? sle_tab_doubles(s) * sizeof(double) : 0) +
-           ((size_t)s.ntw + std::max(rows, order)) * sizeof(double2);
+    const size_t rows = (size_t)2 * s.nphi * sizeof(double) +
+                        ((size_t)s.ntw + (size_t)6 * s.nphi) * sizeof(double2);
+    const size_t order = (tab ? sle_tab_doubles(s) * sizeof(double) : 0) +
+                         ((size_t)2 * s.nh + std::max(256, s.L + 1) + 3 * (s.L + 1) + 16 * (size_t)s.nh) * sizeof(double2);
+    return std::max(rows, order);
+}
+
+// wavefront sum of v over the block's lanes 0..63 (butterfly: every lane the same bits)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// dphi = (ice_int - Cs_int + zeta) / C_int with Cs_int from the row sums cs
+// (wavefront 0; result in *sd for the block)
+__device__ __forceinline__ void sle_dphi(const sle_inner_args& a, int nlat, double dlon, int wave, int lane,
+                                         double* sd) {
+    if (wave == 0) {
+        double total = 0.0;
+        for (int q = lane; q < nlat; q += 64) total += a.s.w[q] * ld_coh(a.cs + q) * dlon;
+        total = wave_sum(total);
+        if (lane == 0) {
+            const double rho_ratio = VH_RHO_ICE / VH_RHO_WATER;
+            const double ice_int = -rho_ratio * a.sc[1];
+            const double zeta = a.subgrid ? a.sc[2] : 0.0;
+            *sd = (ice_int - total + zeta) / a.sc[0];
+        }
+    }
+}
+
+// Order stage of block m: fold of the load spectrum (F, plus kdp * FC when
+// kdp != 0) -> Legendre analysis -> sigma -> response -> Legendre synthesis of
+// (u, N) -> F2. fn/fs: this thread's north / south spectrum values (j = tid < nh)
+// already loaded by the caller.
+template <bool TabLds>
+__device__ __forceinline__ void sle_order_stage(const sle_inner_args& a, int m, const double* T, int tstride,
+                                                double2* U, double2 fn, double2 fs) {
+    const vh_sht& s = a.s;
+    const int L = s.L, nlat = s.nlat, nh = s.nh, nl = L - m + 1, tid = threadIdx.x;
+    (void)nlat;
+    double2* E = U;                       // [nh]
+    double2* O = E + nh;                  // [nh]
+    double2* red = O + nh;                // [max(256, L+1)]
+    double2* Q = red + max(256, L + 1);   // [3][L+1]: sigma, u, N
+    double2* red2 = Q + 3 * (L + 1);      // [4][nh][2][2]
+    if (tid < nh) {   // nh <= 256 (checked at launch)
+        const double w = s.w[tid];
+        E[tid] = make_double2(w * (fn.x + fs.x), w * (fn.y + fs.y));
+        O[tid] = make_double2(w * (fn.x - fs.x), w * (fn.y - fs.y));
+    }
+    __syncthreads();
+    // analysis: thread = (latitude part, degree), S parts of the north
+    // rows, S = min(4, 256 / nl) so small orders still fill the block;
+    // partial sums added in part order
+    const double* Pa = s.plm_jl + (size_t)s.mstart[m] * nh;
+    {
+        const int S = max(1, min(4, 256 / nl)), jq = (nh + S - 1) / S;
+        for (int t = tid; t < S * nl; t += 256) {
+            const int part = t / nl, il = t % nl;
+            const int ja = part * jq, jb = min(nh, ja + jq);
+            const double2* G = (il & 1) ? O : E;
+            double ax = 0.0, ay = 0.0;
+            int j = ja;
+            for (; j + VH_SLE_JB <= jb; j += VH_SLE_JB) {   // whole batches: loads without predicates
+                double pv[VH_SLE_JB];
+                double2 gv[VH_SLE_JB];
+#pragma unroll
+                for (int u = 0; u < VH_SLE_JB; u++) {
+                    if constexpr (TabLds) pv[u] = T[(j + u) * tstride + il];
+                    else pv[u] = Pa[(size_t)(j + u) * nl + il];
+                    gv[u] = G[j + u];
+                }
+#pragma unroll
+                for (int u = 0; u < VH_SLE_JB; u++) { ax = fma(pv[u], gv[u].x, ax); ay = fma(pv[u], gv[u].y, ay); }
+            }
+            for (; j < jb; j++) {
+                double pj;
+                if constexpr (TabLds) pj = T[j * tstride + il];
+                else pj = Pa[(size_t)j * nl + il];
+                ax = fma(pj, G[j].x, ax); ay = fma(pj, G[j].y, ay);
+            }
+            red[t] = make_double2(ax, ay);
+        }
+        __syncthreads();
+        for (int il = tid; il < nl; il += 256) {
+            const int l = m + il;
+            double2 r = red[il];
+            for (int k = 1; k < S; k++) r = make_double2(r.x + red[k * nl + il].x, r.y + red[k * nl + il].y);
+            a.sigma[(size_t)l * (l + 1) / 2 + m] = r;
+            Q[il] = r;
+        }
+        __syncthreads();
+    }
+    // response applied to the load (response_apply, src/vilma_response.f90:842-864):
+    // u = gu sigma + dUa, N = gn sigma - dFa/g
+    for (int il = tid; il < nl; il += 256) {
+        const int l = m + il, idx = l * (l + 1) / 2 + m;
+        double2 u = make_double2(0.0, 0.0), Nv = u;
+        if (a.kind != 0) {
+            const double2 sg = Q[il];
+            u = make_double2(a.gu[l] * sg.x, a.gu[l] * sg.y);
+            Nv = make_double2(a.gn[l] * sg.x, a.gn[l] * sg.y);
+            if (a.kind == 2 && l >= 1) {
+                const double2 dU = ((const double2*)a.dsurf)[idx - 1];
+                const double2 dF = ((const double2*)a.dsurf)[a.nk + idx - 1];
+                u = make_double2(u.x + dU.x, u.y + dU.y);
+                Nv = make_double2(Nv.x - dF.x / a.g, Nv.y - dF.y / a.g);
+            }
+        }
+        Q[(L + 1) + il] = u;
+        Q[2 * (L + 1) + il] = Nv;
+    }
+    __syncthreads();
+    // synthesis of u, N (two fields, rows interleaved as the inverse
+    // transform pairs them): thread = (degree part, north latitude),
+    // S parts of even length, S = min(4, 256 / nh); parity sums kept
+    // apart for the south rows, parts added in order
+    const double* Ps = s.plm_lj + (size_t)s.mstart[m] * nh;
+    const int S = max(1, min(4, 256 / nh));
+    const int lq = ((nl + 2 * S - 1) / (2 * S)) * 2;
+    for (int t = tid; t < S * nh; t += 256) {
+        const int gq = t / nh, j = t % nh;
+        const int la = min(nl, gq * lq), lb = min(nl, la + lq);
+        double2 e[2], o[2];
+        e[0] = e[1] = o[0] = o[1] = make_double2(0.0, 0.0);
+        int il = la;
+        for (; il + 2 * VH_SLE_LB <= lb; il += 2 * VH_SLE_LB) {   // whole batches: loads without predicates
+            double pe[VH_SLE_LB], po[VH_SLE_LB];
+            double2 qe[2][VH_SLE_LB], qo[2][VH_SLE_LB];
+#pragma unroll
+            for (int u = 0; u < VH_SLE_LB; u++) {
+                if constexpr (TabLds) {
+                    pe[u] = T[j * tstride + il + 2 * u];
+                    po[u] = T[j * tstride + il + 2 * u + 1];
+                } else {
+                    pe[u] = Ps[(size_t)(il + 2 * u) * nh + j];
+                    po[u] = Ps[(size_t)(il + 2 * u + 1) * nh + j];
+                }
+#pragma unroll
+                for (int f = 0; f < 2; f++) {
+                    qe[f][u] = Q[(f + 1) * (L + 1) + il + 2 * u];
+                    qo[f][u] = Q[(f + 1) * (L + 1) + il + 2 * u + 1];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < VH_SLE_LB; u++)
+#pragma unroll
+                for (int f = 0; f < 2; f++) {
+                    e[f].x = fma(qe[f][u].x, pe[u], e[f].x); e[f].y = fma(qe[f][u].y, pe[u], e[f].y);
+                    o[f].x = fma(qo[f][u].x, po[u], o[f].x); o[f].y = fma(qo[f][u].y, po[u], o[f].y);
+                }
+        }
+        for (; il < lb; il += 2) {
+            const bool two = il + 1 < lb;
+            double p0, p1 = 0.0;
+            if constexpr (TabLds) {
+                p0 = T[j * tstride + il];
+                if (two) p1 = T[j * tstride + il + 1];
+            } else {
+                p0 = Ps[(size_t)il * nh + j];
+                if (two) p1 = Ps[(size_t)(il + 1) * nh + j];
+            }
+#pragma unroll
+            for (int f = 0; f < 2; f++) {
+                const double2 c0 = Q[(f + 1) * (L + 1) + il];
+                e[f].x = fma(c0.x, p0, e[f].x); e[f].y = fma(c0.y, p0, e[f].y);
+                if (two) {
+                    const double2 c1 = Q[(f + 1) * (L + 1) + il + 1];
+                    o[f].x = fma(c1.x, p1, o[f].x); o[f].y = fma(c1.y, p1, o[f].y);
+                }
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < 2; f++) {
+            double2* r = red2 + (((size_t)gq * nh + j) * 2 + f) * 2;
+            r[0] = e[f];
+            r[1] = o[f];
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < 2 * nh; t += 256) {
+        const int jj = t / 2, f = t % 2, js = nlat - 1 - jj;
+        double2 ev = make_double2(0.0, 0.0), od = ev;
+        for (int k = 0; k < S; k++) {
+            const double2* r = red2 + (((size_t)k * nh + jj) * 2 + f) * 2;
+            ev = make_double2(ev.x + r[0].x, ev.y + r[0].y);
+            od = make_double2(od.x + r[1].x, od.y + r[1].y);
+        }
+        // F2 rows q = 2j + field in groups of 4 (one latitude pair), order-major
+        // inside a group: [q/4][m][q%4] (64 contiguous bytes per order written,
+        // one contiguous 64 (L+1) bytes per latitude pair read)
+        const int qn = 2 * jj + f, qs = 2 * js + f;
+        st_coh(a.F2 + ((size_t)(qn >> 2) * (L + 1) + m) * 4 + (qn & 3), make_double2(ev.x + od.x, ev.y + od.y));
+        if (js != jj)
+            st_coh(a.F2 + ((size_t)(qs >> 2) * (L + 1) + m) * 4 + (qs & 3), make_double2(ev.x - od.x, ev.y - od.y));
+    }
 }
 
 template <bool TabLds>
@@ -242,38 +444,48 @@ __global__ void __launch_bounds__(256) k_sle_inner(sle_inner_args a) {
     const int L = s.L, N = s.nphi, nlat = s.nlat, nh = s.nh;
     const int npairs = (nlat + 1) / 2, blk = blockIdx.x, tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    double* srow = (double*)lds;                          // [2][N]
-    double* T = srow + 2 * N;                             // [nh][tstride] (TabLds)
-    double2* TW = (double2*)(T + (TabLds ? sle_tab_doubles(s) : 0));   // [s.ntw]
-    double2* U = TW + s.ntw;                                            // union
-    const int tstride = blk <= L ? ((L - blk + 1) | 1) : 1;
-    if (TabLds && blk <= L) {
-        const int nl = L - blk + 1;
-        const double* Pa = s.plm_jl + (size_t)s.mstart[blk] * nh;
-        for (int t = tid; t < nh * nl; t += 256) T[(t / nl) * tstride + t % nl] = Pa[t];
-    }
-    if (blk < npairs)
-        for (int t = tid; t < s.ntw; t += 256) TW[t] = s.twp[t];
-    __syncthreads();
+    const bool is_row = blk < npairs;
+    const int p = blk, m = blk - npairs;
     __shared__ double redmax[4][4];
     __shared__ double sdphi;
     unsigned target = 0;
     const double scale = 2.0 * VH_PI / N, dlon = 2.0 * VH_PI / N;
-    const InLoad load{a.dig, a.C, a.rsl, a.wcorr, N};
+    // row-block LDS
+    double* srow = (double*)lds;                          // [2][N]
+    double2* TW = (double2*)(srow + 2 * N);               // [s.ntw]
+    double2* RB = TW + s.ntw;                             // [3][2N] FFT buffer pairs
+    // order-block LDS
+    double* T = (double*)lds;                             // [nh][tstride] (TabLds)
+    double2* U = (double2*)(T + (TabLds ? sle_tab_doubles(s) : 0));
+    const int tstride = is_row ? 1 : ((L - m + 1) | 1);
+    if (is_row) {
+        for (int t = tid; t < s.ntw; t += 256) TW[t] = s.twp[t];
+    } else if (TabLds) {
+        const int nl = L - m + 1;
+        const double* Pa = s.plm_jl + (size_t)s.mstart[m] * nh;
+        for (int t = tid; t < nh * nl; t += 256) T[(t / nl) * tstride + t % nl] = Pa[t];
+    }
+    __syncthreads();
 
-    // initial load of rows 2p, 2p+1 -> F (k_fft_fwd<InLoad>)
-    if (blk < npairs && wave == 0) {
-        const int r0 = 2 * blk, r1 = r0 + 1;
+    // ---- start: FC = FFT(C) (wave 0), F = FFT(load(rsl)) (wave 1)
+    if (is_row && wave < 2) {
+        const int r0 = 2 * p, r1 = r0 + 1;
         const bool has1 = r1 < nlat;
-        double2* fa = U;
-        double2* fb = U + N;
+        double2* fa = RB + (size_t)wave * 2 * N;
+        double2* fb = fa + N;
+        const InLoad load{a.dig, a.C, a.rsl, a.wcorr, N};
         for (int i0 = 0; i0 < N; i0 += 64 * VH_FFT_U) {
             double v0[VH_FFT_U], v1[VH_FFT_U];
 #pragma unroll
             for (int u = 0; u < VH_FFT_U; u++) {
                 const int i = i0 + lane + 64 * u;
-                v0[u] = i < N ? load(r0, i) : 0.0;
-                v1[u] = (has1 && i < N) ? load(r1, i) : 0.0;
+                if (wave == 0) {
+                    v0[u] = i < N ? a.C[(size_t)r0 * N + i] : 0.0;
+                    v1[u] = (has1 && i < N) ? a.C[(size_t)r1 * N + i] : 0.0;
+                } else {
+                    v0[u] = i < N ? load(r0, i) : 0.0;
+                    v1[u] = (has1 && i < N) ? load(r1, i) : 0.0;
+                }
             }
 #pragma unroll
             for (int u = 0; u < VH_FFT_U; u++) {
@@ -283,217 +495,92 @@ __global__ void __launch_bounds__(256) k_sle_inner(sle_inner_args a) {
         }
         wave_lds_sync();
         double2* z = fft_wave(fa, fb, s, false, lane, TW);
-        lds_to_pair_spectra<true>(z, L, N, scale, a.F + (size_t)r0 * (L + 1),
-                                  has1 ? a.F + (size_t)r1 * (L + 1) : nullptr, lane);
+        double2* dst = wave == 0 ? a.FC : a.F;
+        lds_to_pair_spectra<true>(z, L, N, scale, dst + (size_t)r0 * (L + 1),
+                                  has1 ? dst + (size_t)r1 * (L + 1) : nullptr, lane);
+    }
+    sle_grid_sync(a.bar, target, a.res);
+    if (!is_row) {
+        double2 fn = make_double2(0.0, 0.0), fs = fn;
+        if (tid < nh) {
+            const int js = nlat - 1 - tid;
+            fn = ld_coh(a.F + (size_t)tid * (L + 1) + m);
+            fs = js != tid ? ld_coh(a.F + (size_t)js * (L + 1) + m) : make_double2(0.0, 0.0);
+        }
+        sle_order_stage<TabLds>(a, m, T, tstride, U, fn, fs);
     }
     sle_grid_sync(a.bar, target, a.res);
 
     for (int ii = 1; ii <= a.n_inner; ii++) {
-        // ---- B: order m
-        if (blk <= L) {
-            const int m = blk, nl = L - m + 1;
-            double2* E = U;                       // [nh]
-            double2* O = E + nh;                  // [nh]
-            double2* red = O + nh;                // [max(256, L+1)]
-            double2* Q = red + max(256, L + 1);   // [3][L+1]: sigma, u, N
-            double2* red2 = Q + 3 * (L + 1);      // [4][nh][2][2]
-            for (int j = tid; j < nh; j += 256) {
-                const int js = nlat - 1 - j;
-                const double2 fn = ld_coh(a.F + (size_t)j * (L + 1) + m);
-                const double2 fs = js != j ? ld_coh(a.F + (size_t)js * (L + 1) + m) : make_double2(0.0, 0.0);
-                const double w = s.w[j];
-                E[j] = make_double2(w * (fn.x + fs.x), w * (fn.y + fs.y));
-                O[j] = make_double2(w * (fn.x - fs.x), w * (fn.y - fs.y));
-            }
-            __syncthreads();
-            // analysis: thread = (latitude part, degree), S parts of the north
-            // rows, S = min(4, 256 / nl) so small orders still fill the block;
-            // partial sums added in part order
-            const double* Pa = s.plm_jl + (size_t)s.mstart[m] * nh;
-            {
-                const int S = max(1, min(4, 256 / nl)), jq = (nh + S - 1) / S;
-                for (int t = tid; t < S * nl; t += 256) {
-                    const int part = t / nl, il = t % nl;
-                    const int ja = part * jq, jb = min(nh, ja + jq);
-                    const double2* G = (il & 1) ? O : E;
-                    double ax = 0.0, ay = 0.0;
-                    for (int j0 = ja; j0 < jb; j0 += VH_SLE_JB) {
-                        double pv[VH_SLE_JB];
+        // ---- rows: inverse FFT of (u_j, N_j) -> Sraw_j, cs_j, the base load and its FFT
+        if (is_row) {
+            double2* P = RB + (size_t)4 * N;   // base pair (rows 2p, 2p+1), then its FFT scratch
+            if (wave < 2) {
+                const int j = 2 * p + wave;
+                if (j < nlat) {
+                    double2* fa = RB + (size_t)wave * 2 * N;
+                    double2* fb = fa + N;
+                    // the row's mask, rotation term, grounded ice and subgrid term
+                    // in flight over the transform (longitudes below 64 VH_SLE_PU)
+                    double cpre[VH_SLE_PU], spre[VH_SLE_PU], gpre[VH_SLE_PU], wpre[VH_SLE_PU];
 #pragma unroll
-                        for (int u = 0; u < VH_SLE_JB; u++) {
-                            if constexpr (TabLds) pv[u] = j0 + u < jb ? T[(j0 + u) * tstride + il] : 0.0;
-                            else pv[u] = j0 + u < jb ? Pa[(size_t)(j0 + u) * nl + il] : 0.0;
-                        }
-#pragma unroll
-                        for (int u = 0; u < VH_SLE_JB; u++)
-                            if (j0 + u < jb) { ax = fma(pv[u], G[j0 + u].x, ax); ay = fma(pv[u], G[j0 + u].y, ay); }
-                    }
-                    red[t] = make_double2(ax, ay);
-                }
-                __syncthreads();
-                for (int il = tid; il < nl; il += 256) {
-                    const int l = m + il;
-                    double2 r = red[il];
-                    for (int k = 1; k < S; k++) r = make_double2(r.x + red[k * nl + il].x, r.y + red[k * nl + il].y);
-                    a.sigma[(size_t)l * (l + 1) / 2 + m] = r;
-                    Q[il] = r;
-                }
-                __syncthreads();
-            }
-            // response applied to the load (response_apply, src/vilma_response.f90:842-864):
-            // u = gu sigma + dUa, N = gn sigma - dFa/g
-            for (int il = tid; il < nl; il += 256) {
-                const int l = m + il, idx = l * (l + 1) / 2 + m;
-                double2 u = make_double2(0.0, 0.0), Nv = u;
-                if (a.kind != 0) {
-                    const double2 sg = Q[il];
-                    u = make_double2(a.gu[l] * sg.x, a.gu[l] * sg.y);
-                    Nv = make_double2(a.gn[l] * sg.x, a.gn[l] * sg.y);
-                    if (a.kind == 2 && l >= 1) {
-                        const double2 dU = ((const double2*)a.dsurf)[idx - 1];
-                        const double2 dF = ((const double2*)a.dsurf)[a.nk + idx - 1];
-                        u = make_double2(u.x + dU.x, u.y + dU.y);
-                        Nv = make_double2(Nv.x - dF.x / a.g, Nv.y - dF.y / a.g);
-                    }
-                }
-                Q[(L + 1) + il] = u;
-                Q[2 * (L + 1) + il] = Nv;
-            }
-            __syncthreads();
-            // synthesis of u, N (two fields, rows interleaved as the inverse
-            // transform pairs them): thread = (degree part, north latitude),
-            // S parts of even length, S = min(4, 256 / nh); parity sums kept
-            // apart for the south rows, parts added in order
-            const double* Ps = s.plm_lj + (size_t)s.mstart[m] * nh;
-            const int S = max(1, min(4, 256 / nh));
-            const int lq = ((nl + 2 * S - 1) / (2 * S)) * 2;
-            for (int t = tid; t < S * nh; t += 256) {
-                const int gq = t / nh, j = t % nh;
-                const int la = min(nl, gq * lq), lb = min(nl, la + lq);
-                double2 e[2], o[2];
-                e[0] = e[1] = o[0] = o[1] = make_double2(0.0, 0.0);
-                for (int l0 = la; l0 < lb; l0 += 2 * VH_SLE_LB) {
-                    double pe[VH_SLE_LB], po[VH_SLE_LB];
-#pragma unroll
-                    for (int u = 0; u < VH_SLE_LB; u++) {
-                        const int il = l0 + 2 * u;
-                        if constexpr (TabLds) {
-                            pe[u] = il < lb ? T[j * tstride + il] : 0.0;
-                            po[u] = il + 1 < lb ? T[j * tstride + il + 1] : 0.0;
-                        } else {
-                            pe[u] = il < lb ? Ps[(size_t)il * nh + j] : 0.0;
-                            po[u] = il + 1 < lb ? Ps[(size_t)(il + 1) * nh + j] : 0.0;
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < VH_SLE_LB; u++) {
-                        const int il = l0 + 2 * u;
-                        if (il >= lb) break;
-                        const bool two = il + 1 < lb;
-#pragma unroll
-                        for (int f = 0; f < 2; f++) {
-                            const double2 c0 = Q[(f + 1) * (L + 1) + il];
-                            e[f].x = fma(c0.x, pe[u], e[f].x); e[f].y = fma(c0.y, pe[u], e[f].y);
-                            if (two) {
-                                const double2 c1 = Q[(f + 1) * (L + 1) + il + 1];
-                                o[f].x = fma(c1.x, po[u], o[f].x); o[f].y = fma(c1.y, po[u], o[f].y);
-                            }
-                        }
-                    }
-                }
-#pragma unroll
-                for (int f = 0; f < 2; f++) {
-                    double2* r = red2 + (((size_t)gq * nh + j) * 2 + f) * 2;
-                    r[0] = e[f];
-                    r[1] = o[f];
-                }
-            }
-            __syncthreads();
-            for (int t = tid; t < 2 * nh; t += 256) {
-                const int jj = t / 2, f = t % 2, js = nlat - 1 - jj;
-                double2 ev = make_double2(0.0, 0.0), od = ev;
-                for (int k = 0; k < S; k++) {
-                    const double2* r = red2 + (((size_t)k * nh + jj) * 2 + f) * 2;
-                    ev = make_double2(ev.x + r[0].x, ev.y + r[0].y);
-                    od = make_double2(od.x + r[1].x, od.y + r[1].y);
-                }
-                st_coh(a.F2 + ((size_t)jj * 2 + f) * (L + 1) + m, make_double2(ev.x + od.x, ev.y + od.y));
-                if (js != jj) st_coh(a.F2 + ((size_t)js * 2 + f) * (L + 1) + m, make_double2(ev.x - od.x, ev.y - od.y));
-            }
-        }
-        sle_grid_sync(a.bar, target, a.res);
-
-        // ---- C: inverse FFT of (u_j, N_j) -> Sraw_j in LDS, cs_j
-        if (blk < npairs && wave < 2) {
-            const int j = 2 * blk + wave;
-            if (j < nlat) {
-                double2* fa = U + (size_t)wave * 2 * N;
-                double2* fb = fa + N;
-                // the row's mask and rotation term in flight over the transform
-                double cpre[VH_FFT_U], spre[VH_FFT_U];
-#pragma unroll
-                for (int u = 0; u < VH_FFT_U; u++) {
-                    const int i = lane + 64 * u;
-                    cpre[u] = i < N ? a.C[(size_t)j * N + i] : 0.0;
-                    spre[u] = (a.srot && i < N) ? a.srot[(size_t)j * N + i] : 0.0;
-                }
-                pair_spectra_to_lds<true>(a.F2 + (size_t)(2 * j) * (L + 1), a.F2 + (size_t)(2 * j + 1) * (L + 1), L, N,
-                                          fa, lane);
-                double2* z = fft_wave(fa, fb, s, true, lane, TW);
-                double rs = 0.0;
-                for (int i0 = 0; i0 < N; i0 += 64 * VH_FFT_U) {
-#pragma unroll
-                    for (int u = 0; u < VH_FFT_U; u++) {
-                        const int i = i0 + lane + 64 * u;
-                        if (i >= N) break;
+                    for (int u = 0; u < VH_SLE_PU; u++) {
+                        const int i = lane + 64 * u;
                         const size_t id = (size_t)j * N + i;
-                        double v = z[i].y - z[i].x;
-                        if (a.srot) v = v + (i0 == 0 ? spre[u] : a.srot[id]);
-                        srow[wave * N + i] = v;
-                        rs += (i0 == 0 ? cpre[u] : a.C[id]) * v;
+                        cpre[u] = i < N ? a.C[id] : 0.0;
+                        spre[u] = (a.srot && i < N) ? a.srot[id] : 0.0;
+                        gpre[u] = i < N ? a.dig[id] : 0.0;
+                        wpre[u] = i < N ? a.wcorr[id] : 0.0;
                     }
-                }
+                    const double2* F2p = a.F2 + (size_t)p * (L + 1) * 4 + 2 * wave;   // rows q = 2j, 2j + 1
+                    pair_spectra_to_lds<true>(F2p, F2p + 1, L, N, fa, lane, 4);
+                    double2* z = fft_wave(fa, fb, s, true, lane, TW);
+                    double rs = 0.0;
+                    for (int i0 = 0; i0 < N; i0 += 64 * VH_SLE_PU) {
 #pragma unroll
-                for (int off = 32; off > 0; off >>= 1) rs += __shfl_xor(rs, off);
-                if (lane == 0) st_coh(a.cs + j, rs);
+                        for (int u = 0; u < VH_SLE_PU; u++) {
+                            const int i = i0 + lane + 64 * u;
+                            if (i >= N) break;
+                            const size_t id = (size_t)j * N + i;
+                            const bool pre = i0 == 0;
+                            double v = z[i].y - z[i].x;
+                            if (a.srot) v = v + (pre ? spre[u] : a.srot[id]);
+                            srow[wave * N + i] = v;
+                            const double c = pre ? cpre[u] : a.C[id];
+                            rs += c * v;
+                            const double base = VH_RHO_ICE * (pre ? gpre[u] : a.dig[id]) + VH_RHO_WATER * (c * v) +
+                                                (pre ? wpre[u] : a.wcorr[id]);
+                            if (wave == 0) P[i].x = base; else P[i].y = base;
+                        }
+                    }
+                    rs = wave_sum(rs);
+                    if (lane == 0) st_coh(a.cs + j, rs);
+                } else {
+                    for (int i = lane; i < N; i += 64) P[i].y = 0.0;
+                }
             }
-        }
-        sle_grid_sync(a.bar, target, a.res);
-
-        // ---- D: dphi, rsl update, row maxima; the next load and its FFT
-        if (blk < npairs) {
-            const int r0 = 2 * blk;
-            double* wcs = (double*)(U + 2 * N);   // [nlat]: w_j cs_j dlon
-            for (int q = tid; q < nlat; q += 256) wcs[q] = s.w[q] * ld_coh(a.cs + q) * dlon;
             __syncthreads();
             if (wave == 0) {
-                // Cs_int: per-lane sums in latitude order, then a butterfly
-                // (every lane, every block: the same bits)
-                double total = 0.0;
-                for (int q = lane; q < nlat; q += 64) total += wcs[q];
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) total += __shfl_xor(total, off);
-                const double rho_ratio = VH_RHO_ICE / VH_RHO_WATER;
-                const double ice_int = -rho_ratio * a.sc[1];
-                const double zeta = a.subgrid ? a.sc[2] : 0.0;
-                if (lane == 0) {
-                    sdphi = (ice_int - total + zeta) / a.sc[0];
-                    if (blk == 0) { a.res[0] = (double)ii; a.res[1] = sdphi; }
-                }
+                const int r0 = 2 * p;
+                double2* z = fft_wave(P, P + N, s, false, lane, TW);
+                lds_to_pair_spectra<true>(z, L, N, scale, a.F + (size_t)r0 * (L + 1),
+                                          r0 + 1 < nlat ? a.F + (size_t)(r0 + 1) * (L + 1) : nullptr, lane);
             }
+        }
+        sle_grid_sync(a.bar, target, a.res);
+
+        // ---- dphi (every block); rows: rsl update and row maxima; orders: the next sigma and (u, N)
+        if (is_row) {
+            sle_dphi(a, nlat, dlon, wave, lane, &sdphi);
             __syncthreads();
             const double dphi = sdphi;
-            const bool has1 = r0 + 1 < nlat;
-            double2* fa = U;
-            double2* fb = U + N;
+            if (blk == 0 && tid == 0) { a.res[0] = (double)ii; a.res[1] = dphi; }
+            const int r0 = 2 * p;
             double dm[2] = {0.0, 0.0}, sm[2] = {0.0, 0.0};
 #pragma unroll
             for (int w = 0; w < 2; w++) {
                 const int j = r0 + w;
-                if (j >= nlat) {
-                    for (int i = tid; i < N; i += 256) fa[i].y = 0.0;
-                    continue;
-                }
+                if (j >= nlat) continue;
                 for (int i = tid; i < N; i += 256) {
                     const size_t id = (size_t)j * N + i;
                     const double rn = srow[w * N + i] + dphi;
@@ -501,8 +588,6 @@ __global__ void __launch_bounds__(256) k_sle_inner(sle_inner_args a) {
                     dm[w] = fmax(dm[w], fabs(c * (rn - a.rsl[id])));
                     a.rsl[id] = rn;
                     sm[w] = fmax(sm[w], fabs(c * rn));
-                    const double ld = VH_RHO_ICE * a.dig[id] + VH_RHO_WATER * (c * rn) + a.wcorr[id];
-                    if (w == 0) fa[i].x = ld; else fa[i].y = ld;
                 }
             }
 #pragma unroll
@@ -521,11 +606,24 @@ __global__ void __launch_bounds__(256) k_sle_inner(sle_inner_args a) {
                     st_coh(a.part + (tid < 2 ? 0 : nlat) + r0 + w, v);
                 }
             }
-            if (wave == 0) {
-                double2* z = fft_wave(fa, fb, s, false, lane, TW);
-                lds_to_pair_spectra<true>(z, L, N, scale, a.F + (size_t)r0 * (L + 1),
-                                          has1 ? a.F + (size_t)(r0 + 1) * (L + 1) : nullptr, lane);
+        } else {
+            // this thread's spectrum values in flight while dphi is formed
+            double2 bn = make_double2(0.0, 0.0), bs = bn, cn = bn, cs = bn;
+            const int js = nlat - 1 - tid;
+            if (tid < nh) {
+                bn = ld_coh(a.F + (size_t)tid * (L + 1) + m);
+                cn = ld_coh(a.FC + (size_t)tid * (L + 1) + m);
+                if (js != tid) {
+                    bs = ld_coh(a.F + (size_t)js * (L + 1) + m);
+                    cs = ld_coh(a.FC + (size_t)js * (L + 1) + m);
+                }
             }
+            sle_dphi(a, nlat, dlon, wave, lane, &sdphi);
+            __syncthreads();
+            const double k = VH_RHO_WATER * sdphi;
+            const double2 fn = make_double2(bn.x + k * cn.x, bn.y + k * cn.y);
+            const double2 fs = make_double2(bs.x + k * cs.x, bs.y + k * cs.y);
+            sle_order_stage<TabLds>(a, m, T, tstride, U, fn, fs);
         }
         sle_grid_sync(a.bar, target, a.res);
 
@@ -597,9 +695,9 @@ void grid_bsl_integrand(hipStream_t st, int n, const double* h, const double* C,
 
 void sle_inner(hipStream_t st, const sle_inner_args& a) {
     const vh_sht& s = a.s;
-    const int grid = std::max(s.L + 1, (s.nlat + 1) / 2);
+    const int grid = (s.nlat + 1) / 2 + s.L + 1;   // row blocks, then order blocks
     // the Legendre table of each order stays in LDS when it fits beside the rest
-    const bool tab = sle_inner_lds(s, true) <= (size_t)150 * 1024;
+    const bool tab = sle_inner_lds(s, true) <= (size_t)156 * 1024;
     const size_t lds = sle_inner_lds(s, tab);
     const void* fn = tab ? (const void*)k_sle_inner<true> : (const void*)k_sle_inner<false>;
     // resident-block capacity, checked once per (device, LDS size)
@@ -618,6 +716,7 @@ void sle_inner(hipStream_t st, const sle_inner_args& a) {
         HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         cap_dev = dev; cap_lds = lds; cap = (long)per_cu * ncu;
     }
+    if (s.nh > 256) throw vh_error("sle: the fused inner loop supports nlat <= 512");
     if (cap < grid)
         throw vh_error("sle: the fused inner loop needs " + std::to_string(grid) + " resident blocks, the device holds " +
                        std::to_string(cap));
