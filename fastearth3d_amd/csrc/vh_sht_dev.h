@@ -108,7 +108,14 @@ __device__ __forceinline__ void wave_lds_sync() {
 // One Stockham radix-R pass over N points. tp: this pass's twiddles
 // [r-1][k] = e^{-2 pi i k r tstep/N}, k = j mod Ns < Ns (none when Ns = 1: the
 // first pass multiplies by 1).
-template <int R>
+// swz(i): i with its low 3 bits xor-ed by bits 3..5, a permutation inside each
+// aligned group of 8 elements. The first pass stores with stride R across
+// lanes (all in one LDS bank group); swizzling that buffer spreads the lanes of
+// a group over 8 distinct 16-byte slots, and the second pass reads it back the
+// same way.
+__device__ __forceinline__ int swz(int i) { return i ^ ((i >> 3) & 7); }
+
+template <int R, bool SwS = false, bool SwD = false>
 __device__ __forceinline__ void fft_pass(const double2* src, double2* dst, int N, int Ns, const double2* tp,
                                          bool inv, int t, int nthr) {
     const int NR = N / R;
@@ -116,7 +123,7 @@ __device__ __forceinline__ void fft_pass(const double2* src, double2* dst, int N
         const int k = j % Ns;
         double2 v[R], V[R];
 #pragma unroll
-        for (int r = 0; r < R; r++) v[r] = src[j + r * NR];
+        for (int r = 0; r < R; r++) v[r] = src[SwS ? swz(j + r * NR) : j + r * NR];
         if (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < R; r++) v[r] = cmul(v[r], cconj_if(tp[(r - 1) * Ns + k], inv));
@@ -124,7 +131,7 @@ __device__ __forceinline__ void fft_pass(const double2* src, double2* dst, int N
         dft_small<R>(v, V, inv);
         const int idxD = (j / Ns) * Ns * R + k;
 #pragma unroll
-        for (int r = 0; r < R; r++) dst[idxD + r * Ns] = V[r];
+        for (int r = 0; r < R; r++) dst[SwD ? swz(idxD + r * Ns) : idxD + r * Ns] = V[r];
     }
 }
 
@@ -138,18 +145,32 @@ __device__ double2* fft_wave(double2* a, double2* b, const vh_sht& s, bool inv, 
     const int N = s.nphi;
     int Ns = 1;
     double2 *src = a, *dst = b;
+    // the buffer between the first and second pass is swizzled (swz)
+    const bool sw = s.nfac > 1 && (N & 7) == 0;
     for (int p = 0; p < s.nfac; p++) {
         const int R = s.fac[p];
         const double2* tp = (tw ? tw : s.twp) + s.twoff[p];
+        const int mode = !sw ? 0 : (p == 0 ? 1 : (p == 1 ? 2 : 0));   // 1: store swizzled, 2: load swizzled
+#define VH_FFT_CASE(RR)                                                                       \
+        case RR:                                                                              \
+            if (mode == 1) fft_pass<RR, false, true>(src, dst, N, Ns, tp, inv, lane, 64);     \
+            else if (mode == 2) fft_pass<RR, true, false>(src, dst, N, Ns, tp, inv, lane, 64); \
+            else fft_pass<RR>(src, dst, N, Ns, tp, inv, lane, 64);                            \
+            break;
         switch (R) {
-            case 8: fft_pass<8>(src, dst, N, Ns, tp, inv, lane, 64); break;
-            case 6: fft_pass<6>(src, dst, N, Ns, tp, inv, lane, 64); break;
-            case 4: fft_pass<4>(src, dst, N, Ns, tp, inv, lane, 64); break;
-            case 2: fft_pass<2>(src, dst, N, Ns, tp, inv, lane, 64); break;
-            case 3: fft_pass<3>(src, dst, N, Ns, tp, inv, lane, 64); break;
-            case 5: fft_pass<5>(src, dst, N, Ns, tp, inv, lane, 64); break;
-            default: fft_pass<7>(src, dst, N, Ns, tp, inv, lane, 64); break;
+            VH_FFT_CASE(8)
+            VH_FFT_CASE(6)
+            VH_FFT_CASE(4)
+            VH_FFT_CASE(2)
+            VH_FFT_CASE(3)
+            VH_FFT_CASE(5)
+            default:   // 7
+                if (mode == 1) fft_pass<7, false, true>(src, dst, N, Ns, tp, inv, lane, 64);
+                else if (mode == 2) fft_pass<7, true, false>(src, dst, N, Ns, tp, inv, lane, 64);
+                else fft_pass<7>(src, dst, N, Ns, tp, inv, lane, 64);
+                break;
         }
+#undef VH_FFT_CASE
         wave_lds_sync();
         double2* t = src; src = dst; dst = t;
         Ns *= R;

@@ -171,7 +171,7 @@ typedef double vh_d4 __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(256) k_tsh_analysis(vh_sht s, const double2* __restrict__ Fd, int nb,
                                                       const double* __restrict__ n6, double2* __restrict__ C) {
-    __shared__ double Bs[2][TA_JC][64];           // [parity][latitude in chunk][column]
+    __shared__ double Bs[2][TA_JC][65];           // [parity][latitude in chunk][column], rows padded: the fold stores stride over banks
     __shared__ double Ds[4][16][65];              // per-wavefront staging of one row tile
     const int L = s.L, nh = s.nh, nlat = s.nlat, nlm = s.nlm;
     const int ngrp = (nb + TA_B - 1) / TA_B;
