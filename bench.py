@@ -30,7 +30,7 @@ NAMES = {0: "k_drift", 1: "k_advance", 2: "k_tsh_synth", 3: "k_t3_rows", 4: "k_t
          6: "k_sle_inner"}
 
 
-def alg_bytes(dims, cls, chunk_batches=None):
+def alg_bytes(dims, cls, chunk_batches=None, sle_iters=None):
     """Algorithmic HBM bytes of one launch of kernel class cls.
 
     drift / advance: the Maxwell memory of the Maxwell elements (24 doubles
@@ -45,6 +45,8 @@ def alg_bytes(dims, cls, chunk_batches=None):
     the row pass reads and writes those spectra and reads one viscosity-rate
     row per latitude; the analysis is the synthesis reversed; the gather
     reads the 3-D elements' memory and writes 12 complex per element and slot.
+    The fused sea-level loop (per launch over `sle_iters` iterations on
+    average) reads and writes the grid fields of each iteration once.
     """
     nk, nem, nmn, nlm, L = dims["nk"], dims["nem"], dims["nmn"], dims["nlm"], dims["lmax"]
     ne3d, nlat, nphi = dims.get("ne3d", 0), dims["nlat"], dims["nphi"]
@@ -64,6 +66,12 @@ def alg_bytes(dims, cls, chunk_batches=None):
         return nb * (spec + 4 * nlm * 16) + tables
     if cls == 5:
         return ne3d * (24 * nk * 8 + 12 * nlm * 16) + 4 * nmn * nk * 8 + 16 * nlm
+    if cls == 6 and sle_iters is not None:
+        # sea-level loop: per iteration 8 grid fields and 3 spectra (load, C,
+        # rsl, ice increment, subgrid term, Sraw, s_rot, the updated rsl;
+        # sigma, u, N), plus the loop's start (mask and load transforms)
+        per_it = 8 * nlat * nphi * 8 + 3 * nlm * 16
+        return sle_iters * per_it + 2 * nlat * nphi * 8
     return None
 
 
@@ -139,6 +147,7 @@ def main():
         step(k)
     L.vh_profile(se._h, 1)
     n_acc0 = se.scalar(3)
+    n_in0 = se.scalar(6)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(a.warmup, nsteps):
@@ -149,32 +158,36 @@ def main():
     cnt = (C.c_double * 8)()
     L.vh_profile_read(se._h, ms, cnt)
     nsub = se.scalar(3) - n_acc0
+    n_inner = se.scalar(6) - n_in0
     kyr = a.steps * c["dt_yr"] / 1000.0
     value = kyr / el
 
     d = dict(se.dims)
     d["nlat"], d["nphi"] = se.nlat, se.nphi
     nbat = 3 * d.get("ne3d", 0)
-    best = max((k for k in NAMES if cnt[k] > 0), key=lambda k: ms[k], default=None)
-    roof = None
     # the tensor kernels run in chunks of up to 48 batches; the average launch
     # covers nbat / (number of chunks) batches
     nch = (nbat + 47) // 48 if nbat else 1
-    if best is not None and alg_bytes(d, best, nbat / nch if nbat else None):
-        avg_s = ms[best] / cnt[best] / 1000.0
-        ach = alg_bytes(d, best, nbat / nch if nbat else None) / avg_s / 1e9
-        roof = {"bound": "hbm", "kernel": NAMES[best], "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "avg_launch_us": round(avg_s * 1e6, 2), "launches": int(cnt[best]),
-                "alg_bytes_per_launch": int(alg_bytes(d, best, nbat / nch if nbat else None))}
-    if roof is not None:
-        # HBM bytes per launch of this kernel from the committed PMC summary
-        # (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration)
-        tf = os.path.join(ROOT, "profiles", "c%d_lmax%d_traffic.json" % (a.config, a.lmax))
-        if os.path.exists(tf):
-            t = json.load(open(tf))["kernels"].get(roof["kernel"])
-            if t:
-                roof["traffic"] = t["hbm_bytes_per_launch"]
+    tf = os.path.join(ROOT, "profiles", "c%d_lmax%d_traffic.json" % (a.config, a.lmax))
+    traffic = json.load(open(tf))["kernels"] if os.path.exists(tf) else {}
+    per_kernel = {}
+    for k in NAMES:
+        if cnt[k] <= 0:
+            continue
+        ab = alg_bytes(d, k, nbat / nch if nbat else None, n_inner / cnt[k] if k == 6 else None)
+        if not ab:
+            continue
+        avg_s = ms[k] / cnt[k] / 1000.0
+        ach = ab / avg_s / 1e9
+        t = traffic.get(NAMES[k])
+        per_kernel[NAMES[k]] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(ach / HBM_PEAK_GBS, 4),
+                                "traffic": t["hbm_bytes_per_launch"] if t else None,
+                                "avg_launch_us": round(avg_s * 1e6, 2), "launches": int(cnt[k]),
+                                "alg_bytes_per_launch": int(ab), "total_ms": round(ms[k], 3)}
+    # the dominant kernel by summed device time
+    best = max(per_kernel, key=lambda n: per_kernel[n]["total_ms"], default=None)
+    roof = dict(per_kernel[best], kernel=best) if best else None
     kernel_ms = {NAMES[k]: round(ms[k], 3) for k in NAMES if cnt[k] > 0}
 
     cpu = None
@@ -195,7 +208,7 @@ def main():
         "config": {"workload": names[a.config], "lmax": a.lmax, "nlat": se.nlat, "nphi": se.nphi,
                    "radial_elements": d["ne"], "maxwell_elements": d["nem"], "dt_yr": c["dt_yr"],
                    "sub_steps_timed": int(nsub), "ne3d": int(se.dims.get("ne3d", 0))},
-        "roofline": roof, "kernel_ms": kernel_ms, "cpu_baseline": cpu,
+        "roofline": roof, "kernel_roofline": per_kernel, "kernel_ms": kernel_ms, "cpu_baseline": cpu,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 2)

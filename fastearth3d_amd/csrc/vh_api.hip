@@ -199,6 +199,7 @@ struct vh_model {
     double2 *d_F = 0, *d_FC = 0, *d_F2 = 0, *d_sigma = 0, *d_lm = 0, *d_uN = 0;
     double *d_part = 0, *d_sc = 0;
     unsigned* d_sbar = 0;   // grid-barrier counter + timeout flag of the fused SLE loop
+    unsigned sbar_base = 0; // the counter's value (launches advance it by known amounts)
     double* h_pin = nullptr;
     // rotation
     bool rot_on = false;
@@ -475,6 +476,7 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
     h->d_part = dalloc<double>((size_t)4 * nl + 64);
     h->d_sc = dalloc<double>(64); vh_fill(st, h->d_sc, 0.0, 64);
     h->d_sbar = dalloc<unsigned>(2);
+    sle_bar_reset(st, h->d_sbar);
 
     // ---- SLE / stepper knobs (build_solver, src/vilma_coupling.f90:291-308)
     h->n_outer = p->sle_n_outer; h->n_inner = p->sle_n_inner; h->tol = p->sle_tol;
@@ -651,12 +653,23 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
         ia.g = h->g; ia.tol = h->tol; ia.rsl = h->d_rsl; ia.cs = h->d_part + 3 * (size_t)s.nlat;
         ia.part = h->d_part; ia.res = h->d_sc + 16; ia.F = h->d_F; ia.FC = h->d_FC; ia.F2 = h->d_F2; ia.sigma = h->d_sigma;
         ia.bar = h->d_sbar;
+        const unsigned grid = (unsigned)sle_inner_grid(s);
+        if ((unsigned long long)h->sbar_base + (unsigned long long)grid * (2 + 2 * (unsigned)h->n_inner) >= (1ull << 31)) {
+            sle_bar_reset(st, h->d_sbar);
+            h->sbar_base = 0;
+        }
+        ia.bar_base = h->sbar_base;
         h->prof_begin(VH_K_SLE);
         sle_inner(st, ia);
         h->prof_end();
         h->read(h->d_sc + 16, 3);
-        if (h->h_pin[2] != 0.0) throw vh_error("sle_solve: a grid barrier of the fused inner loop timed out");
+        if (h->h_pin[2] != 0.0) {
+            sle_bar_reset(st, h->d_sbar);
+            h->sbar_base = 0;
+            throw vh_error("sle_solve: a grid barrier of the fused inner loop timed out");
+        }
         res.n_inner_last = (int)h->h_pin[0];
+        h->sbar_base += grid * (2 + 2 * (unsigned)res.n_inner_last);
         if (res.n_inner_last > 0) res.esl = h->h_pin[1];
         load_current = true;
         res.n_outer_done = io;

@@ -28,7 +28,8 @@ void sle_flip(hipStream_t st, const vh_sht& s, const double* topo0, const double
 // On return rsl is updated and sigma is the spectrum of the load of the final
 // rsl under the current mask (the solve's closing analysis).
 // res[0] = iterations run, res[1] = dphi of the last one, res[2] = 1 if a grid
-// barrier wait timed out. bar: 2 unsigned of scratch.
+// barrier wait timed out. bar: [0] barrier counter, at bar_base on entry, raised
+// by the grid size at each of the launch's 2 + 2 res[0] barriers; [1] timeout flag.
 struct sle_inner_args {
     vh_sht s;
     const double *dig, *C, *wcorr, *srot, *gu, *gn, *dsurf, *sc;
@@ -37,8 +38,11 @@ struct sle_inner_args {
     double *rsl, *cs, *part, *res;
     double2 *F, *FC, *F2, *sigma;
     unsigned* bar;
+    unsigned bar_base;
 };
 void sle_inner(hipStream_t st, const sle_inner_args& a);
+int sle_inner_grid(const vh_sht& s);
+void sle_bar_reset(hipStream_t st, unsigned* bar);   // counter and flag to 0
 void grid_sub(hipStream_t st, int n, const double* a, const double* b, double* out);
 void grid_ocean(hipStream_t st, int n, const double* topo, const double* ice, double* C);
 void grid_bsl_integrand(hipStream_t st, int n, const double* h, const double* C, const double* heq,

@@ -196,8 +196,9 @@ __global__ void k_bsl_integrand(int n, const double* __restrict__ h, const doubl
 // block reaches the same values and a rerun gives identical bits.
 // Data handed between blocks (F, FC, F2, cs, part) moves through coherent
 // loads and stores (ld_coh / st_coh), so a grid barrier needs no cache
-// write-back or invalidate: each wavefront waits for its stores, then one
-// counter per launch (bar[0], zeroed before the launch) is raised and awaited.
+// write-back or invalidate: each wavefront waits for its stores, then the
+// counter bar[0] is raised and awaited (it runs on across launches; the host
+// passes the value it starts from).
 // The wait is bounded: a block that never arrives raises bar[1] and res[2]
 // instead of a hang.
 
@@ -448,7 +449,8 @@ __global__ void __launch_bounds__(256) k_sle_inner(sle_inner_args a) {
     const int p = blk, m = blk - npairs;
     __shared__ double redmax[4][4];
     __shared__ double sdphi;
-    unsigned target = 0;
+    unsigned target = a.bar_base;   // the counter keeps counting across launches
+    if (blk == 0 && tid == 0) { a.res[0] = 0.0; a.res[2] = 0.0; }
     const double scale = 2.0 * VH_PI / N, dlon = 2.0 * VH_PI / N;
     // row-block LDS
     double* srow = (double*)lds;                          // [2][N]
@@ -644,10 +646,16 @@ __global__ void __launch_bounds__(256) k_sle_inner(sle_inner_args a) {
     }
 }
 
-__global__ void k_sle_inner_prep(unsigned* bar, double* res) {
+__global__ void k_sle_bar_reset(unsigned* bar) {
     if (threadIdx.x < 2) bar[threadIdx.x] = 0u;
-    if (threadIdx.x < 3) res[threadIdx.x] = 0.0;
 }
+
+void sle_bar_reset(hipStream_t st, unsigned* bar) {
+    k_sle_bar_reset<<<1, 64, 0, st>>>(bar);
+    HIPCHECK(hipGetLastError());
+}
+
+int sle_inner_grid(const vh_sht& s) { return (s.nlat + 1) / 2 + s.L + 1; }
 
 static inline unsigned nb256(size_t n) { return (unsigned)((n + 255) / 256); }
 
@@ -695,7 +703,7 @@ void grid_bsl_integrand(hipStream_t st, int n, const double* h, const double* C,
 
 void sle_inner(hipStream_t st, const sle_inner_args& a) {
     const vh_sht& s = a.s;
-    const int grid = (s.nlat + 1) / 2 + s.L + 1;   // row blocks, then order blocks
+    const int grid = sle_inner_grid(s);   // row blocks, then order blocks
     // the Legendre table of each order stays in LDS when it fits beside the rest
     const bool tab = sle_inner_lds(s, true) <= (size_t)156 * 1024;
     const size_t lds = sle_inner_lds(s, tab);
@@ -720,8 +728,6 @@ void sle_inner(hipStream_t st, const sle_inner_args& a) {
     if (cap < grid)
         throw vh_error("sle: the fused inner loop needs " + std::to_string(grid) + " resident blocks, the device holds " +
                        std::to_string(cap));
-    k_sle_inner_prep<<<1, 64, 0, st>>>(a.bar, a.res);
-    HIPCHECK(hipGetLastError());
     sle_inner_args arg = a;
     void* params[] = {&arg};
     HIPCHECK(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(256), params, (unsigned)lds, st));
