@@ -55,6 +55,9 @@ def lib():
         L.orc_mesh_c.argtypes = [C.POINTER(VhParams), C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int, C.c_char_p]
         L.orc_elem_integrals.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double)]
         L.orc_ocean_function_c.argtypes = [C.c_int] + [C.POINTER(C.c_double)] * 3
+        L.orc_rotation_run.argtypes = [C.POINTER(VhParams), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                       C.c_double, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.c_char_p]
         _lib = L
     return _lib
 
@@ -167,6 +170,20 @@ def love(params, j, tidal=False, fluidize=False):
     if lib().orc_love(C.byref(params), j, int(tidal), int(fluidize), _cp(out), err):
         raise VilmaError(err.value.decode())
     return out
+
+
+def rotation_run(params, lmax, nlat, nphi, load, dt_s, nsteps):
+    """The rotation channels alone under a fixed load (the reference's
+    tests/test_rotation.f90:73-117 driving). Returns (m[nsteps+1] complex,
+    kTe, k_s, rigid (2,1) inertia complex)."""
+    load = np.ascontiguousarray(load, dtype=np.float64)
+    assert load.size == nlat * nphi
+    mo = np.empty(2 * (nsteps + 1))
+    info = np.empty(4)
+    err = C.create_string_buffer(512)
+    if lib().orc_rotation_run(C.byref(params), lmax, nlat, nphi, _cp(load), dt_s, nsteps, _cp(mo), _cp(info), err):
+        raise VilmaError(err.value.decode())
+    return mo[0::2] + 1j * mo[1::2], info[0], info[1], info[2] + 1j * info[3]
 
 
 def dense_operator(params, j):

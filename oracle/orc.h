@@ -203,6 +203,8 @@ void orc_rot_begin(orc_rot* rt, double dt);
 void orc_rot_solve_m(orc_rot* rt, const orc_sht* s, const double* load);
 void orc_rot_s_rot(const orc_rot* rt, const orc_sht* s, double* srot);
 void orc_rot_update(orc_rot* rt, const orc_sht* s, const double* load, double dt);
+int  orc_rotation_run(const vh_params* p, int lmax, int nlat, int nphi, const double* load, double dt,
+                      int nsteps, double* m_out, double* info, char* err);
 
 /* ---------------------------------------------------------------- model */
 typedef struct orc_model {

@@ -178,3 +178,28 @@ void orc_rot_update(orc_rot* rt, const orc_sht* s, const double* load, double dt
     ch_commit(&rt->tidal_ch, rt->m);
     rt->time += rt->load_ch.dt;
 }
+
+/* The rotation module on its own, as the reference's rotation test drives it
+ * (tests/test_rotation.f90:73-117): a fixed surface load on the grid, the
+ * channels of the earth in p on its own mesh, nsteps + 1 updates of dt
+ * seconds. m_out[2i], m_out[2i+1] = m after update i (the polar motion at
+ * t = i dt); info = kTe, k_s, and the rigid (2,1) inertia of the load as
+ * (re, im). Test helper: the oracle's rotation checked against the Spada et
+ * al. (2011) benchmark. */
+int orc_rotation_run(const vh_params* p, int lmax, int nlat, int nphi, const double* load, double dt,
+                     int nsteps, double* m_out, double* info, char* err) {
+    orc_earth e; orc_mesh m; orc_sht s; orc_rot rt;
+    if (orc_build_earth(p, &e, err)) return 1;
+    if (orc_mesh_build(&e, p->n_elem_layer, &m, err)) return 1;
+    if (orc_sht_init(&s, lmax, nlat, nphi, err)) { orc_mesh_free(&m); return 1; }
+    if (orc_rot_init(&rt, &e, &m, dt, err)) { orc_sht_free(&s); orc_mesh_free(&m); return 1; }
+    rt.enabled = 1;
+    cplx I21 = inertia21(&s, load, rt.a);
+    info[0] = rt.kTe; info[1] = rt.k_s; info[2] = creal(I21); info[3] = cimag(I21);
+    for (int i = 0; i <= nsteps; i++) {
+        orc_rot_update(&rt, &s, load, dt);
+        m_out[2 * i] = creal(rt.m); m_out[2 * i + 1] = cimag(rt.m);
+    }
+    orc_rot_free(&rt); orc_sht_free(&s); orc_mesh_free(&m);
+    return 0;
+}

@@ -5,8 +5,10 @@ analytic Love limits (tests/test_love.f90), the M3-L70-V01 benchmark Love
 table (tests/test_benchmark_love.f90, data/benchmarks/love_M3-L70-V01),
 SHT conventions (tests/test_sht.f90), tensor-SH identities
 (tests/test_tensor_sh.f90), flotation (tests/test_flotation.f90), SLE mass
-conservation (tests/test_sle.f90:118) and the 3-D == 1-D reduction
-(tests/test_response_3d.f90:224-246). CPU only."""
+conservation (tests/test_sle.f90:118), the 3-D == 1-D reduction
+(tests/test_response_3d.f90:224-246) and the rotation channels against the
+Spada et al. (2011) Test 3/2 polar-motion benchmark (tests/test_rotation.f90,
+data/benchmarks/rotation_spada2011). CPU only."""
 import json
 import os
 
@@ -119,6 +121,52 @@ def test_love_table_m3l70v01(oracle):
             hf, lf, kf = oracle.love(p, j, fluidize=True)
             for got, ref in ((hf, f["h"]), (lf, f["l"]), (kf, f["k"])):
                 assert abs(got - ref) <= 1e-2 * abs(ref), (j, got, ref)
+
+
+# -------------------------------------------------------------- rotation
+def _spada_load(name, h, colat, lon):
+    """Heaviside ice loads of Spada et al. (2011) Table 4, centroid colatitude
+    25 deg, longitude 75 deg, half-width 10 deg, rho_ice 931: the parabolic cap
+    sigma = rho h sqrt((cos g - cos a)/(1 - cos a)) and the uniform disc
+    sigma = rho h (as tests/test_rotation.f90:129-155 builds them)."""
+    deg = np.pi / 180.0
+    thc, lac, al = 25.0 * deg, 75.0 * deg, 10.0 * deg
+    cg = np.cos(thc) * np.cos(colat)[:, None] + np.sin(thc) * np.sin(colat)[:, None] * np.cos(lon[None, :] - lac)
+    out = np.zeros((colat.size, lon.size))
+    inside = cg >= np.cos(al)
+    if name == "cap":
+        out[inside] = 931.0 * h * np.sqrt(np.maximum((cg[inside] - np.cos(al)) / (1.0 - np.cos(al)), 0.0))
+    else:
+        out[inside] = 931.0 * h
+    return out
+
+
+@pytest.mark.parametrize("name,h", [("cap", 1500.0), ("disc", 1000.0)])
+def test_rotation_spada2011(oracle, name, h):
+    """Rotation channels (orc_rotation.c) vs the Spada et al. (2011) Test 3/2
+    benchmark, the checks of tests/test_rotation.f90: rigid (2,1) inertia of
+    the load / (C - A) vs the published G_sigma (2 %), 0 < k_T,elastic < k_s <
+    1.5 and 0.5 < k_s, and |m(t)| under the Heaviside load vs Table 14 with the
+    Chandler wobble excluded (the quasi-static column; 3 %) at t = 0, 1, 2, 5,
+    10, 20 kyr; M3-L70-V01 on the reference depth-target mesh, lmax 128 on a
+    256 x 512 Gauss grid, dt = 25 yr (Spada's year, 3.15576e7 s)."""
+    ref = json.load(open(os.path.join(GOLDEN, "spada2011_test3_2.json")))
+    p = oracle.params_defaults()
+    p.earth = b"M3-L70-V01"
+    lmax, nlat, nphi = 128, 256, 512
+    x, _ = np.polynomial.legendre.leggauss(nlat)
+    colat = np.arccos(x[::-1])                      # north first
+    lon = 2.0 * np.pi * np.arange(nphi) / nphi
+    yr = 3.15576e7
+    dt = 25.0 * yr
+    m, kTe, ks, I21 = oracle.rotation_run(p, lmax, nlat, nphi, _spada_load(name, h, colat, lon), dt, 800)
+    assert 0.5 < ks < 1.5 and 0.0 < kTe < ks, (kTe, ks)
+    G = complex(*ref["G_sigma"][name])
+    assert abs(I21 / 2.63e35 - G) <= 0.02 * abs(G), (I21 / 2.63e35, G)
+    mdeg = np.abs(m) * 180.0 / np.pi
+    for row in ref["table"][name]["excl"]:
+        i = int(round(row["t_kyr"] * 1000.0 * yr / dt))
+        assert abs(mdeg[i] - row["m_Gs_deg"]) <= 0.03 * row["m_Gs_deg"], (row, mdeg[i])
 
 
 # ------------------------------------------------------------------- SHT
