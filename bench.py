@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 NAMES = {0: "k_drift", 1: "k_advance", 2: "k_tsh_synth", 3: "k_t3_rows", 4: "k_tsh_analysis", 5: "k_t3_gather",
-         6: "k_sle_inner"}
+         6: "k_sle_solve"}
 
 
 def alg_bytes(dims, cls, chunk_batches=None, sle_iters=None):

@@ -199,7 +199,7 @@ struct vh_model {
     double2 *d_F = 0, *d_FC = 0, *d_F2 = 0, *d_sigma = 0, *d_lm = 0, *d_uN = 0;
     double *d_part = 0, *d_sc = 0;
     unsigned* d_sbar = 0;   // grid-barrier counter + timeout flag of the fused SLE loop
-    unsigned sbar_base = 0; // the counter's value (launches advance it by known amounts)
+    long sbar_budget = 0;   // most the counter can have advanced since its last reset
     double* h_pin = nullptr;
     // rotation
     bool rot_on = false;
@@ -473,7 +473,7 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
     h->d_sigma = dalloc<double2>(d.nlm); vh_fill(st, (double*)h->d_sigma, 0.0, 2 * (size_t)d.nlm);
     h->d_lm = dalloc<double2>(d.nlm);
     h->d_uN = dalloc<double2>((size_t)2 * d.nlm);
-    h->d_part = dalloc<double>((size_t)4 * nl + 64);
+    h->d_part = dalloc<double>((size_t)7 * nl + 64);
     h->d_sc = dalloc<double>(64); vh_fill(st, h->d_sc, 0.0, 64);
     h->d_sbar = dalloc<unsigned>(2);
     sle_bar_reset(st, h->d_sbar);
@@ -606,7 +606,7 @@ void advance_memory(vh_model* h, const double* sigma, int nxt) {
 }
 
 // ---------------------------------------------------------------- SLE solve
-struct sle_res { int n_outer_done = 0, n_inner_last = 0, n_coast_flip = 0; double mass_resid = 0, esl = 0; };
+struct sle_res { double mass_resid = 0, err_inf = 0; };
 
 // sle_solve (src/vilma_sle.f90:124-405), forward-Euler response branch.
 // The advance writes tau_{n+1} into the spare buffer; the caller commits it.
@@ -624,75 +624,50 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
         h->prof_end();
     }
     vh_fill(st, h->d_wcorr, 0.0, n);
-    double C_int = 0.0, ice_int = 0.0, zeta_int = 0.0;
-    // the inner loop leaves sigma = spectrum of the load of the final rsl
-    // under the current mask; coastline flips only take effect in a next pass
-    bool load_current = false;
-    for (int io = 1; io <= h->n_outer; io++) {
-        int first = 0;
-        if (h->fixed_ocean) {
-            if (io == 1) HIPCHECK(hipMemcpyAsync(h->d_Cn, h->d_C0s, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-        } else if (io == 1) {
-            first = 1;
-        }
-        sle_outer(st, s, first, h->d_topo0, h->d_rsl, h->d_ice, h->d_dice, h->d_C0s, h->d_Cn, h->d_C, h->d_dig,
-                  h->d_part, h->d_sc);
-        load_current = false;
-        h->read(h->d_sc, 3);
-        C_int = h->h_pin[0];
-        if (C_int <= 0.0) break;
-        ice_int = -(VH_RHO_ICE / VH_RHO_WATER) * h->h_pin[1];
-        zeta_int = h->subgrid ? h->h_pin[2] : 0.0;
-        sle_wcorr(st, n, h->subgrid, h->d_topo0, h->d_C, h->d_C0s, h->d_wcorr);
-        // the inner fixed point on the device; part: [0, 2 nlat) row maxima,
-        // [3 nlat, 4 nlat) row sums of C Sraw; sc[16..18] the loop's result
-        sle_inner_args ia{};
-        ia.s = s; ia.dig = h->d_dig; ia.C = h->d_C; ia.wcorr = h->d_wcorr; ia.srot = srot;
-        ia.gu = h->d_gu; ia.gn = h->d_gn; ia.dsurf = h->d_dsurf; ia.sc = h->d_sc;
-        ia.nk = h->d.nk; ia.kind = h->kind; ia.subgrid = h->subgrid; ia.n_inner = h->n_inner;
-        ia.g = h->g; ia.tol = h->tol; ia.rsl = h->d_rsl; ia.cs = h->d_part + 3 * (size_t)s.nlat;
-        ia.part = h->d_part; ia.res = h->d_sc + 16; ia.F = h->d_F; ia.FC = h->d_FC; ia.F2 = h->d_F2; ia.sigma = h->d_sigma;
-        ia.bar = h->d_sbar;
-        const unsigned grid = (unsigned)sle_inner_grid(s);
-        if ((unsigned long long)h->sbar_base + (unsigned long long)grid * (2 + 2 * (unsigned)h->n_inner) >= (1ull << 31)) {
-            sle_bar_reset(st, h->d_sbar);
-            h->sbar_base = 0;
-        }
-        ia.bar_base = h->sbar_base;
-        h->prof_begin(VH_K_SLE);
-        sle_inner(st, ia);
-        h->prof_end();
-        h->read(h->d_sc + 16, 3);
-        if (h->h_pin[2] != 0.0) {
-            sle_bar_reset(st, h->d_sbar);
-            h->sbar_base = 0;
-            throw vh_error("sle_solve: a grid barrier of the fused inner loop timed out");
-        }
-        res.n_inner_last = (int)h->h_pin[0];
-        h->sbar_base += grid * (2 + 2 * (unsigned)res.n_inner_last);
-        if (res.n_inner_last > 0) res.esl = h->h_pin[1];
-        load_current = true;
-        res.n_outer_done = io;
-        h->n_outer_tot++;
-        h->n_inner_tot += res.n_inner_last;
-        if (h->fixed_ocean) break;
-        sle_flip(st, s, h->d_topo0, h->d_rsl, h->d_ice, h->d_C, h->d_Cn, h->d_part, h->d_sc + 7);
-        h->read(h->d_sc + 7, 1);
-        res.n_coast_flip = (int)h->h_pin[0];
-        if (res.n_coast_flip == 0) break;
+    // the coastline passes and inner fixed point in one cooperative launch;
+    // sc[16..28): its results (zeroed here), then Cs and the new memory norm,
+    // all read back once at the end of the solve
+    double* r = h->d_sc + 16;
+    vh_fill(st, r, 0.0, 12);
+    sle_solve_args ia{};
+    ia.s = s; ia.topo0 = h->d_topo0; ia.ice = h->d_ice; ia.dice = h->d_dice; ia.C0 = h->d_C0s; ia.srot = srot;
+    ia.gu = h->d_gu; ia.gn = h->d_gn; ia.dsurf = h->d_dsurf;
+    ia.C = h->d_C; ia.Cn = h->d_Cn; ia.dig = h->d_dig; ia.wcorr = h->d_wcorr;
+    ia.nk = h->d.nk; ia.kind = h->kind; ia.subgrid = h->subgrid; ia.n_inner = h->n_inner; ia.n_outer = h->n_outer;
+    ia.fixed_ocean = h->fixed_ocean; ia.g = h->g; ia.tol = h->tol; ia.rsl = h->d_rsl;
+    ia.part = h->d_part; ia.cs = h->d_part + 3 * (size_t)s.nlat; ia.part3 = h->d_part + 4 * (size_t)s.nlat;
+    ia.res = r; ia.F = h->d_F; ia.FC = h->d_FC; ia.F2 = h->d_F2; ia.sigma = h->d_sigma; ia.bar = h->d_sbar;
+    const long step = sle_solve_max_barriers(s, h->n_outer, h->n_inner);
+    if (h->sbar_budget + step >= (1L << 31)) {   // keep the running counter far from wrap-around
+        sle_bar_reset(st, h->d_sbar);
+        h->sbar_budget = 0;
     }
-    // closing load -> sigma, then the memory advance
-    if (!load_current) sle_load_analysis(st, s, h->d_dig, h->d_C, h->d_rsl, h->d_wcorr, h->d_F, h->d_sigma);
+    h->sbar_budget += step;
+    h->prof_begin(VH_K_SLE);
+    sle_solve_launch(st, ia);
+    h->prof_end();
+    // sigma is the closing load transform; then the memory advance
     int nxt = h->cur ^ 1;
     if (!report_only && h->kind == 2) advance_memory(h, (const double*)h->d_sigma, nxt);
     if (!report_only) h->time_resp += h->dt;
     // mass diagnostic (+ the new memory norm, in the same read-back)
-    sle_cs(st, s, h->d_C, h->d_rsl, h->d_part, h->d_sc + 8);
-    if (h->kind == 2) HIPCHECK(hipMemcpyAsync(h->d_sc + 9, h->d_maxn + (report_only ? h->cur : nxt), sizeof(double),
+    sle_cs(st, s, h->d_C, h->d_rsl, h->d_part, r + 10);
+    if (h->kind == 2) HIPCHECK(hipMemcpyAsync(r + 11, h->d_maxn + (report_only ? h->cur : nxt), sizeof(double),
                                               hipMemcpyDeviceToDevice, st));
-    h->read(h->d_sc + 8, 2);
-    double Cs = h->h_pin[0] - zeta_int;
+    h->read(r, 12);
+    const double* hr = h->h_pin;
+    if (hr[2] != 0.0) {
+        sle_bar_reset(st, h->d_sbar);
+        h->sbar_budget = 0;
+        throw vh_error("sle_solve: a grid barrier of the fused sea-level loop timed out");
+    }
+    h->n_inner_tot += (long)hr[4];
+    h->n_outer_tot += (long)hr[5];
+    const double ice_int = -(VH_RHO_ICE / VH_RHO_WATER) * hr[6];
+    const double zeta_int = h->subgrid ? hr[7] : 0.0;
+    const double Cs = hr[10] - zeta_int;
     res.mass_resid = std::fabs(ice_int) > 0.0 ? std::fabs(Cs - ice_int) / std::fabs(ice_int) : std::fabs(Cs);
+    res.err_inf = hr[11];
     return res;
 }
 
@@ -723,7 +698,7 @@ void stepper_advance(vh_model* h, double t0, double t1, const double* srot) {
         h->n_solve++;
         sle_res r = sle_solve(h, h->d_hice, h->d_ice1, frac, 1, false, srot);
         h->worst_mass_resid = std::max(h->worst_mass_resid, r.mass_resid);
-        double err_inf = h->kind == 2 ? h->h_pin[1] : 0.0;
+        double err_inf = h->kind == 2 ? r.err_inf : 0.0;
         bool finite = err_inf <= DBL_MAX;
         double scale = std::max(tau0, tau_run);
         bool accept = scale <= 0.0 ? finite : (finite && err_inf <= h->guard * scale);

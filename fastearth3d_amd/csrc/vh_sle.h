@@ -4,45 +4,42 @@
 
 void sle_ice(hipStream_t st, int n, const double* ice0, const double* ice1, double frac, int interp,
              const double* ice_ref, const double* topo0, double* ice, double* dice, double* C0);
-// C (first pass: from topo0 - rsl; later: C_next), d_ice_g, and sc[0..2] =
-// (int C, int d_ice_g, int topo0 (C - C0))
-void sle_outer(hipStream_t st, const vh_sht& s, int first, const double* topo0, const double* rsl,
-               const double* ice, const double* dice, const double* C0, const double* Cnext, double* C,
-               double* dig, double* part, double* sc);
-void sle_wcorr(hipStream_t st, int n, int subgrid, const double* topo0, const double* C, const double* C0, double* w);
-// load -> load_lm (degree-major)
-void sle_load_analysis(hipStream_t st, const vh_sht& s, const double* dig, const double* C, const double* rsl,
-                       const double* wcorr, double2* F, double2* load_lm);
 // response_apply only: uN = [u; N] (degree-major) from sigma
 void vh_apply_only(hipStream_t st, int nlm, const double2* sigma, const double* gu, const double* gn,
                    const double* dsurf, int nk, double g, int kind, double2* uN);
 // out[0] = int C f
 void sle_cs(hipStream_t st, const vh_sht& s, const double* C, const double* f, double* part, double* out);
-// C_next and out[0] = number of flipped cells
-void sle_flip(hipStream_t st, const vh_sht& s, const double* topo0, const double* rsl, const double* ice,
-              const double* C, double* Cnext, double* part, double* out);
-// The inner fixed point of the sea-level equation (src/vilma_sle.f90:296-330)
-// as one cooperative launch: at most n_inner iterations of load analysis,
-// response, synthesis and rsl update, with the exit test on the device.
-// F, FC, F2 are scratch spectra ([nlat][L+1], [nlat][L+1], [(nlat+1)/2][L+1][4]).
-// On return rsl is updated and sigma is the spectrum of the load of the final
-// rsl under the current mask (the solve's closing analysis).
-// res[0] = iterations run, res[1] = dphi of the last one, res[2] = 1 if a grid
-// barrier wait timed out. bar: [0] barrier counter, at bar_base on entry, raised
-// by the grid size at each of the launch's 2 + 2 res[0] barriers; [1] timeout flag.
-struct sle_inner_args {
+// sle_solve's coastline passes and inner fixed point (src/vilma_sle.f90:231-355)
+// as one cooperative launch: per pass (at most n_outer) the mask and
+// grounded-ice increment with their integrals, the subgrid term, then at most
+// n_inner iterations of load analysis, response, synthesis and rsl update,
+// then the coastline test; every exit decision is taken on the device.
+// F, FC, F2 are scratch spectra ([nlat][L+1], [nlat][L+1], [(nlat+1)/2][L+1][4]);
+// part holds [2][nlat] row maxima, [nlat] flip counts, [nlat] row sums cs;
+// part3 [3][nlat] the pass integrand row sums. On return C, dig, wcorr, rsl
+// and Cn are those of the last pass and sigma is the spectrum of the load of
+// the final rsl (the solve's closing analysis).
+// res (zeroed by the caller): [0] iterations of the last pass, [1] its last
+// dphi, [2] 1 if a grid barrier wait timed out, [4] iterations over all
+// passes, [5] passes with ocean (C_int > 0), [6], [7] int d_ice_g and
+// int topo0 (C - C0) of the last such pass.
+// bar: [0] barrier counter (continues across launches), [1] timeout flag,
+// [2] the counter's value at the start of the next launch.
+struct sle_solve_args {
     vh_sht s;
-    const double *dig, *C, *wcorr, *srot, *gu, *gn, *dsurf, *sc;
-    int nk, kind, subgrid, n_inner;
+    const double *topo0, *ice, *dice, *C0, *srot, *gu, *gn, *dsurf;
+    double *C, *Cn, *dig, *wcorr;
+    int nk, kind, subgrid, n_inner, n_outer, fixed_ocean;
     double g, tol;
-    double *rsl, *cs, *part, *res;
+    double *rsl, *cs, *part, *part3, *res;
     double2 *F, *FC, *F2, *sigma;
     unsigned* bar;
-    unsigned bar_base;
 };
-void sle_inner(hipStream_t st, const sle_inner_args& a);
-int sle_inner_grid(const vh_sht& s);
-void sle_bar_reset(hipStream_t st, unsigned* bar);   // counter and flag to 0
+void sle_solve_launch(hipStream_t st, const sle_solve_args& a);
+int sle_solve_grid(const vh_sht& s);
+// most barrier-counter steps one launch can take
+long sle_solve_max_barriers(const vh_sht& s, int n_outer, int n_inner);
+void sle_bar_reset(hipStream_t st, unsigned* bar);   // counter, flag and start value to 0
 void grid_sub(hipStream_t st, int n, const double* a, const double* b, double* out);
 void grid_ocean(hipStream_t st, int n, const double* topo, const double* ice, double* C);
 void grid_bsl_integrand(hipStream_t st, int n, const double* h, const double* C, const double* heq,

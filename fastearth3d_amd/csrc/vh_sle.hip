@@ -31,43 +31,6 @@ __global__ void k_sle_ice(int n, const double* __restrict__ ice0, const double* 
     C0[i] = ocean(topo0[i], v - d);
 }
 
-// Coastline pass head (src/vilma_sle.f90:231-294): C, grounded-ice increment,
-// and per-row partial sums of C, d_ice_g, topo0*(C - C0).
-__global__ void __launch_bounds__(256) k_sle_outer(int nphi, int first, const double* __restrict__ topo0,
-        const double* __restrict__ rsl, const double* __restrict__ ice, const double* __restrict__ dice,
-        const double* __restrict__ C0, const double* __restrict__ Cnext, double* C, double* dig,
-        double* part /* [3][nlat] */, int nlat) {
-    __shared__ double red[3][256];
-    const int j = blockIdx.x;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int i = threadIdx.x; i < nphi; i += blockDim.x) {
-        size_t id = (size_t)j * nphi + i;
-        double c = first ? ocean(topo0[id] - rsl[id], ice[id]) : Cnext[id];
-        double g = ice[id] * (1.0 - c) - (ice[id] - dice[id]) * (1.0 - C0[id]);
-        C[id] = c;
-        dig[id] = g;
-        s0 += c; s1 += g; s2 += topo0[id] * (c - C0[id]);
-    }
-    red[0][threadIdx.x] = s0; red[1][threadIdx.x] = s1; red[2][threadIdx.x] = s2;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
-            for (int c = 0; c < 3; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x < 3) part[(size_t)threadIdx.x * nlat + j] = red[threadIdx.x][0];
-}
-
-// subgrid sloping-coast load (src/vilma_sle.f90:289-291)
-__global__ void k_sle_wcorr(int n, int subgrid, const double* __restrict__ topo0, const double* __restrict__ C,
-                            const double* __restrict__ C0, double* wcorr) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    wcorr[i] = subgrid ? -VH_RHO_WATER * topo0[i] * (C[i] - C0[i]) : 0.0;
-}
-
-// surface load (assemble_load, src/vilma_sle.f90:387-403), fed straight into
-// the forward longitude transform
 struct InLoad {
     const double *dig, *C, *rsl, *wcorr;
     int nphi;
@@ -121,42 +84,6 @@ __global__ void __launch_bounds__(256) k_rowsum_prod(int nphi, const double* __r
     if (threadIdx.x == 0) part[j] = red[0];
 }
 
-__global__ void __launch_bounds__(256) k_sle_flip(int nphi, const double* __restrict__ topo0,
-        const double* __restrict__ rsl, const double* __restrict__ ice, const double* __restrict__ C,
-        double* Cnext, double* part) {
-    __shared__ double red[256];
-    const int j = blockIdx.x;
-    double cnt = 0.0;
-    for (int i = threadIdx.x; i < nphi; i += blockDim.x) {
-        size_t id = (size_t)j * nphi + i;
-        double cn = ocean(topo0[id] - rsl[id], ice[id]);
-        Cnext[id] = cn;
-        cnt += (cn != C[id]) ? 1.0 : 0.0;
-    }
-    red[threadIdx.x] = cnt;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[j] = red[0];
-}
-
-// per-row counts are whole numbers: their sum is exact in any order
-__global__ void __launch_bounds__(256) k_sum_final(const double* __restrict__ part, int nlat, double* out) {
-    __shared__ double red[256];
-    double a = 0.0;
-    for (int j = threadIdx.x; j < nlat; j += blockDim.x) a += part[j];
-    red[threadIdx.x] = a;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[0] = red[0];
-}
-
-// z_bed = z_bed_eq - rsl ; ocean function of an arbitrary (topo, ice) pair
 __global__ void k_sub(int n, const double* __restrict__ a, const double* __restrict__ b, double* out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = a[i] - b[i];
@@ -231,16 +158,17 @@ __device__ __forceinline__ void sle_grid_sync(unsigned* bar, unsigned& target, d
 }
 
 // LDS of a row block: Sraw [2][nphi] doubles (kept across a barrier), the FFT
-// twiddles (s.ntw complex) and three FFT buffer pairs (3 x 2 nphi complex).
+// twiddles (s.ntw complex) and three FFT buffer pairs (3 x 2 nphi complex;
+// the pass head's row sums use the same space).
 // Of an order block: with TabLds its Legendre
 // table [j][l] (row stride (L+1)|1 doubles, loaded once per launch), then the
 // folds, partial sums and (sigma, u, N).
 __host__ __device__ static size_t sle_tab_doubles(const vh_sht& s) {
     return (((size_t)s.nh * ((s.L + 1) | 1)) + 1) & ~(size_t)1;
 }
-static size_t sle_inner_lds(const vh_sht& s, bool tab) {
-    const size_t rows = (size_t)2 * s.nphi * sizeof(double) +
-                        ((size_t)s.ntw + (size_t)6 * s.nphi) * sizeof(double2);
+static size_t sle_solve_lds(const vh_sht& s, bool tab) {
+    const size_t rows = (size_t)2 * s.nphi * sizeof(double) + (size_t)s.ntw * sizeof(double2) +
+                        std::max((size_t)6 * s.nphi * sizeof(double2), (size_t)6 * 256 * sizeof(double));
     const size_t order = (tab ? sle_tab_doubles(s) * sizeof(double) : 0) +
                          ((size_t)2 * s.nh + std::max(256, s.L + 1) + 3 * (s.L + 1) + 16 * (size_t)s.nh) * sizeof(double2);
     return std::max(rows, order);
@@ -255,17 +183,17 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // dphi = (ice_int - Cs_int + zeta) / C_int with Cs_int from the row sums cs
 // (wavefront 0; result in *sd for the block)
-__device__ __forceinline__ void sle_dphi(const sle_inner_args& a, int nlat, double dlon, int wave, int lane,
-                                         double* sd) {
+__device__ __forceinline__ void sle_dphi(const sle_solve_args& a, const double* tot, int nlat, double dlon, int wave,
+                                         int lane, double* sd) {
     if (wave == 0) {
         double total = 0.0;
         for (int q = lane; q < nlat; q += 64) total += a.s.w[q] * ld_coh(a.cs + q) * dlon;
         total = wave_sum(total);
         if (lane == 0) {
             const double rho_ratio = VH_RHO_ICE / VH_RHO_WATER;
-            const double ice_int = -rho_ratio * a.sc[1];
-            const double zeta = a.subgrid ? a.sc[2] : 0.0;
-            *sd = (ice_int - total + zeta) / a.sc[0];
+            const double ice_int = -rho_ratio * tot[1];
+            const double zeta = a.subgrid ? tot[2] : 0.0;
+            *sd = (ice_int - total + zeta) / tot[0];
         }
     }
 }
@@ -275,7 +203,7 @@ __device__ __forceinline__ void sle_dphi(const sle_inner_args& a, int nlat, doub
 // (u, N) -> F2. fn/fs: this thread's north / south spectrum values (j = tid < nh)
 // already loaded by the caller.
 template <bool TabLds>
-__device__ __forceinline__ void sle_order_stage(const sle_inner_args& a, int m, const double* T, int tstride,
+__device__ __forceinline__ void sle_order_stage(const sle_solve_args& a, int m, const double* T, int tstride,
                                                 double2* U, double2 fn, double2 fs) {
     const vh_sht& s = a.s;
     const int L = s.L, nlat = s.nlat, nh = s.nh, nl = L - m + 1, tid = threadIdx.x;
@@ -439,7 +367,7 @@ __device__ __forceinline__ void sle_order_stage(const sle_inner_args& a, int m, 
 }
 
 template <bool TabLds>
-__global__ void __launch_bounds__(256) k_sle_inner(sle_inner_args a) {
+__global__ void __launch_bounds__(256) k_sle_solve(sle_solve_args a) {
     extern __shared__ double2 lds[];
     const vh_sht& s = a.s;
     const int L = s.L, N = s.nphi, nlat = s.nlat, nh = s.nh;
@@ -449,9 +377,10 @@ __global__ void __launch_bounds__(256) k_sle_inner(sle_inner_args a) {
     const int p = blk, m = blk - npairs;
     __shared__ double redmax[4][4];
     __shared__ double sdphi;
-    unsigned target = a.bar_base;   // the counter keeps counting across launches
-    if (blk == 0 && tid == 0) { a.res[0] = 0.0; a.res[2] = 0.0; }
+    unsigned target = a.bar[2];   // the counter keeps counting across launches
     const double scale = 2.0 * VH_PI / N, dlon = 2.0 * VH_PI / N;
+    __shared__ double stot[4];    // pass totals: C_int, int d_ice_g, int topo0 (C - C0); flips
+    double n_iters = 0.0, n_passes = 0.0, last_ice = 0.0, last_zeta = 0.0;
     // row-block LDS
     double* srow = (double*)lds;                          // [2][N]
     double2* TW = (double2*)(srow + 2 * N);               // [s.ntw]
@@ -469,6 +398,67 @@ __global__ void __launch_bounds__(256) k_sle_inner(sle_inner_args a) {
     }
     __syncthreads();
 
+  for (int io = 1; io <= a.n_outer; io++) {
+    // ---- coastline pass head (src/vilma_sle.f90:231-294), rows: the mask C
+    // (first pass: from topo0 - rsl; later: the previous pass's C_next; fixed
+    // ocean: C0), the grounded-ice increment d_ice_g and, per row, the sums of
+    // C, d_ice_g and topo0 (C - C0) (a 256-thread tree per row)
+    if (is_row) {
+        double* red = (double*)RB;   // [6][256]: (quantity, row of the pair)
+        double sp[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int w = 0; w < 2; w++) {
+            const int j = 2 * p + w;
+            if (j >= nlat) continue;
+            for (int i = tid; i < N; i += 256) {
+                const size_t id = (size_t)j * N + i;
+                const double c = a.fixed_ocean ? a.C0[id]
+                                               : (io == 1 ? ocean(a.topo0[id] - a.rsl[id], a.ice[id]) : a.Cn[id]);
+                const double g = a.ice[id] * (1.0 - c) - (a.ice[id] - a.dice[id]) * (1.0 - a.C0[id]);
+                a.C[id] = c;
+                a.dig[id] = g;
+                sp[3 * w + 0] += c; sp[3 * w + 1] += g; sp[3 * w + 2] += a.topo0[id] * (c - a.C0[id]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 6; q++) red[q * 256 + tid] = sp[q];
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if (tid < st)
+#pragma unroll
+                for (int q = 0; q < 6; q++) red[q * 256 + tid] += red[q * 256 + tid + st];
+            __syncthreads();
+        }
+        if (tid < 6) {
+            const int w = tid / 3, q = tid % 3, j = 2 * p + w;
+            if (j < nlat) st_coh(a.part3 + (size_t)q * nlat + j, red[tid * 256]);
+        }
+    }
+    sle_grid_sync(a.bar, target, a.res);
+    // every block: the pass totals sum_j w_j dlon part_j (latitude order, as
+    // k_weighted_total), one lane per quantity
+    if (tid < 3) {
+        double total = 0.0;
+        for (int q = 0; q < nlat; q++) total += s.w[q] * ld_coh(a.part3 + (size_t)tid * nlat + q) * dlon;
+        stot[tid] = total;
+    }
+    __syncthreads();
+    const double C_int = stot[0];
+    if (C_int > 0.0) {
+        // subgrid sloping-coast load (src/vilma_sle.f90:289-291), own rows
+        if (is_row)
+            for (int w = 0; w < 2; w++) {
+                const int j = 2 * p + w;
+                if (j >= nlat) continue;
+                for (int i = tid; i < N; i += 256) {
+                    const size_t id = (size_t)j * N + i;
+                    a.wcorr[id] = a.subgrid ? -VH_RHO_WATER * a.topo0[id] * (a.C[id] - a.C0[id]) : 0.0;
+                }
+            }
+        n_passes += 1.0;
+        last_ice = stot[1];
+        last_zeta = stot[2];
+    }
     // ---- start: FC = FFT(C) (wave 0), F = FFT(load(rsl)) (wave 1)
     if (is_row && wave < 2) {
         const int r0 = 2 * p, r1 = r0 + 1;
@@ -512,8 +502,10 @@ __global__ void __launch_bounds__(256) k_sle_inner(sle_inner_args a) {
         sle_order_stage<TabLds>(a, m, T, tstride, U, fn, fs);
     }
     sle_grid_sync(a.bar, target, a.res);
+    if (C_int <= 0.0) break;   // no ocean: load only; sigma above is the closing analysis
 
     for (int ii = 1; ii <= a.n_inner; ii++) {
+        n_iters += 1.0;
         // ---- rows: inverse FFT of (u_j, N_j) -> Sraw_j, cs_j, the base load and its FFT
         if (is_row) {
             double2* P = RB + (size_t)4 * N;   // base pair (rows 2p, 2p+1), then its FFT scratch
@@ -573,7 +565,7 @@ __global__ void __launch_bounds__(256) k_sle_inner(sle_inner_args a) {
 
         // ---- dphi (every block); rows: rsl update and row maxima; orders: the next sigma and (u, N)
         if (is_row) {
-            sle_dphi(a, nlat, dlon, wave, lane, &sdphi);
+            sle_dphi(a, stot, nlat, dlon, wave, lane, &sdphi);
             __syncthreads();
             const double dphi = sdphi;
             if (blk == 0 && tid == 0) { a.res[0] = (double)ii; a.res[1] = dphi; }
@@ -620,7 +612,7 @@ __global__ void __launch_bounds__(256) k_sle_inner(sle_inner_args a) {
                     cs = ld_coh(a.FC + (size_t)js * (L + 1) + m);
                 }
             }
-            sle_dphi(a, nlat, dlon, wave, lane, &sdphi);
+            sle_dphi(a, stot, nlat, dlon, wave, lane, &sdphi);
             __syncthreads();
             const double k = VH_RHO_WATER * sdphi;
             const double2 fn = make_double2(bn.x + k * cn.x, bn.y + k * cn.y);
@@ -644,10 +636,48 @@ __global__ void __launch_bounds__(256) k_sle_inner(sle_inner_args a) {
         smx = fmax(fmax(redmax[0][1], redmax[1][1]), fmax(redmax[2][1], redmax[3][1]));
         if (dmx <= a.tol * fmax(smx, DBL_MIN)) break;
     }
+    if (a.fixed_ocean) break;
+    // ---- coastline fixed point (src/vilma_sle.f90:336-344): C_next from topo0 - rsl,
+    // flipped cells per row; every block stops when none flipped
+    if (is_row) {
+        double cnt[2] = {0.0, 0.0};
+#pragma unroll
+        for (int w = 0; w < 2; w++) {
+            const int j = 2 * p + w;
+            if (j >= nlat) continue;
+            for (int i = tid; i < N; i += 256) {
+                const size_t id = (size_t)j * N + i;
+                const double cn = ocean(a.topo0[id] - a.rsl[id], a.ice[id]);
+                a.Cn[id] = cn;
+                cnt[w] += (cn != a.C[id]) ? 1.0 : 0.0;
+            }
+        }
+#pragma unroll
+        for (int w = 0; w < 2; w++) cnt[w] = wave_sum(cnt[w]);
+        if (lane == 0) { redmax[wave][0] = cnt[0]; redmax[wave][1] = cnt[1]; }
+        __syncthreads();
+        if (tid < 2 && 2 * p + tid < nlat)
+            st_coh(a.part + 2 * nlat + 2 * p + tid, redmax[0][tid] + redmax[1][tid] + redmax[2][tid] + redmax[3][tid]);
+    }
+    sle_grid_sync(a.bar, target, a.res);
+    double nf = 0.0;   // whole numbers: exact in any order
+    for (int q = tid; q < nlat; q += 256) nf += ld_coh(a.part + 2 * nlat + q);
+    nf = wave_sum(nf);
+    __syncthreads();
+    if (lane == 0) redmax[wave][0] = nf;
+    __syncthreads();
+    nf = redmax[0][0] + redmax[1][0] + redmax[2][0] + redmax[3][0];
+    __syncthreads();
+    if (nf == 0.0) break;
+  }
+    if (blk == 0 && tid == 0) {
+        a.res[4] = n_iters; a.res[5] = n_passes; a.res[6] = last_ice; a.res[7] = last_zeta;
+        a.bar[2] = target;
+    }
 }
 
 __global__ void k_sle_bar_reset(unsigned* bar) {
-    if (threadIdx.x < 2) bar[threadIdx.x] = 0u;
+    if (threadIdx.x < 3) bar[threadIdx.x] = 0u;
 }
 
 void sle_bar_reset(hipStream_t st, unsigned* bar) {
@@ -655,27 +685,17 @@ void sle_bar_reset(hipStream_t st, unsigned* bar) {
     HIPCHECK(hipGetLastError());
 }
 
-int sle_inner_grid(const vh_sht& s) { return (s.nlat + 1) / 2 + s.L + 1; }
+int sle_solve_grid(const vh_sht& s) { return (s.nlat + 1) / 2 + s.L + 1; }
+
+long sle_solve_max_barriers(const vh_sht& s, int n_outer, int n_inner) {
+    return (long)sle_solve_grid(s) * n_outer * (4 + 2 * (long)n_inner);
+}
 
 static inline unsigned nb256(size_t n) { return (unsigned)((n + 255) / 256); }
 
 void sle_ice(hipStream_t st, int n, const double* ice0, const double* ice1, double frac, int interp,
              const double* ice_ref, const double* topo0, double* ice, double* dice, double* C0) {
     k_sle_ice<<<nb256(n), 256, 0, st>>>(n, ice0, ice1, frac, interp, ice_ref, topo0, ice, dice, C0);
-}
-void sle_outer(hipStream_t st, const vh_sht& s, int first, const double* topo0, const double* rsl,
-               const double* ice, const double* dice, const double* C0, const double* Cnext, double* C,
-               double* dig, double* part, double* sc) {
-    k_sle_outer<<<s.nlat, 256, 0, st>>>(s.nphi, first, topo0, rsl, ice, dice, C0, Cnext, C, dig, part, s.nlat);
-    vh_weighted_total(st, s, part, 3, sc);
-}
-void sle_wcorr(hipStream_t st, int n, int subgrid, const double* topo0, const double* C, const double* C0, double* w) {
-    k_sle_wcorr<<<nb256(n), 256, 0, st>>>(n, subgrid, topo0, C, C0, w);
-}
-void sle_load_analysis(hipStream_t st, const vh_sht& s, const double* dig, const double* C, const double* rsl,
-                       const double* wcorr, double2* F, double2* load_lm) {
-    vh_fft_forward_t(st, s, InLoad{dig, C, rsl, wcorr, s.nphi}, s.nlat, F);
-    vh_legendre_analysis(st, s, F, 1, load_lm);
 }
 void vh_apply_only(hipStream_t st, int nlm, const double2* sigma, const double* gu, const double* gn,
                    const double* dsurf, int nk, double g, int kind, double2* uN) {
@@ -684,11 +704,6 @@ void vh_apply_only(hipStream_t st, int nlm, const double2* sigma, const double* 
 void sle_cs(hipStream_t st, const vh_sht& s, const double* C, const double* f, double* part, double* out) {
     k_rowsum_prod<<<s.nlat, 256, 0, st>>>(s.nphi, C, f, part);
     vh_weighted_total(st, s, part, 1, out);
-}
-void sle_flip(hipStream_t st, const vh_sht& s, const double* topo0, const double* rsl, const double* ice,
-              const double* C, double* Cnext, double* part, double* out) {
-    k_sle_flip<<<s.nlat, 256, 0, st>>>(s.nphi, topo0, rsl, ice, C, Cnext, part);
-    k_sum_final<<<1, 256, 0, st>>>(part, s.nlat, out);
 }
 void grid_sub(hipStream_t st, int n, const double* a, const double* b, double* out) {
     k_sub<<<nb256(n), 256, 0, st>>>(n, a, b, out);
@@ -701,13 +716,13 @@ void grid_bsl_integrand(hipStream_t st, int n, const double* h, const double* C,
     k_bsl_integrand<<<nb256(n), 256, 0, st>>>(n, h, C, heq, C0, out);
 }
 
-void sle_inner(hipStream_t st, const sle_inner_args& a) {
+void sle_solve_launch(hipStream_t st, const sle_solve_args& a) {
     const vh_sht& s = a.s;
-    const int grid = sle_inner_grid(s);   // row blocks, then order blocks
+    const int grid = sle_solve_grid(s);   // row blocks, then order blocks
     // the Legendre table of each order stays in LDS when it fits beside the rest
-    const bool tab = sle_inner_lds(s, true) <= (size_t)156 * 1024;
-    const size_t lds = sle_inner_lds(s, tab);
-    const void* fn = tab ? (const void*)k_sle_inner<true> : (const void*)k_sle_inner<false>;
+    const bool tab = sle_solve_lds(s, true) <= (size_t)156 * 1024;
+    const size_t lds = sle_solve_lds(s, tab);
+    const void* fn = tab ? (const void*)k_sle_solve<true> : (const void*)k_sle_solve<false>;
     // resident-block capacity, checked once per (device, LDS size)
     static thread_local int cap_dev = -1;
     static thread_local size_t cap_lds = 0;
@@ -728,7 +743,7 @@ void sle_inner(hipStream_t st, const sle_inner_args& a) {
     if (cap < grid)
         throw vh_error("sle: the fused inner loop needs " + std::to_string(grid) + " resident blocks, the device holds " +
                        std::to_string(cap));
-    sle_inner_args arg = a;
+    sle_solve_args arg = a;
     void* params[] = {&arg};
     HIPCHECK(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(256), params, (unsigned)lds, st));
 }

@@ -152,7 +152,7 @@ enum {
     VH_K_T3_ROWS = 3,        /* 3-D: IFFT, -Mk3 product, FFT per latitude row  */
     VH_K_TSH_ANALYSIS = 4,   /* 3-D: tensor Legendre analysis                  */
     VH_K_T3_GATHER = 5,      /* 3-D: memory + 2 mu strain coefficients         */
-    VH_K_SLE = 6,            /* sea-level equation: fused inner fixed point    */
+    VH_K_SLE = 6,            /* sea-level equation: the fused solve (passes)   */
     VH_K_NCLASS = 8
 };
 int vh_profile(vh_model* h, int enable);
