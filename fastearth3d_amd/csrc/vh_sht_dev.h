@@ -352,8 +352,8 @@ __global__ void __launch_bounds__(512) k_leg_synthesis(vh_sht s, Src src, int nb
         Q[f * nl + il] = src(f, l, m, l * (l + 1) / 2 + m);
     }
     __syncthreads();
-    const int g = threadIdx.x / nh, j = threadIdx.x % nh;
-    if (g < 4) {
+    for (int t = threadIdx.x; t < 4 * nh; t += blockDim.x) {
+        const int g = t / nh, j = t % nh;
         const int lq = ((nl + 7) / 8) * 2;      // even count per quarter keeps parity aligned
         const int la = min(nl, g * lq), lb = min(nl, la + lq);
         const double* P = s.plm_lj + (size_t)s.mstart[m] * nh;
@@ -401,8 +401,7 @@ __global__ void __launch_bounds__(512) k_leg_synthesis(vh_sht s, Src src, int nb
 template <class Src>
 void vh_legendre_synthesis_t(hipStream_t st, const vh_sht& s, Src src, int nb, double2* F, int fst, int jst) {
     if (nb > 4) throw vh_error("legendre synthesis: at most 4 fields per call");
-    if (s.nh > 128) throw vh_error("legendre synthesis: nlat above 256 is not supported");
-    const int nthr = ((4 * s.nh + 63) / 64) * 64;
+    const int nthr = std::min(512, ((4 * s.nh + 63) / 64) * 64);   // (quarter, latitude) tasks, looped
     const size_t lds = ((size_t)nb * (s.L + 1) + (size_t)4 * s.nh * nb * 2) * sizeof(double2);
     k_leg_synthesis<Src><<<s.L + 1, nthr, lds, st>>>(s, src, nb, F, fst, jst);
     HIPCHECK(hipGetLastError());

@@ -158,8 +158,8 @@ __device__ __forceinline__ void sle_grid_sync(unsigned* bar, unsigned& target, d
 }
 
 // LDS of a row block: Sraw [2][nphi] doubles (kept across a barrier), the FFT
-// twiddles (s.ntw complex) and three FFT buffer pairs (3 x 2 nphi complex;
-// the pass head's row sums use the same space).
+// twiddles (s.ntw complex) and two FFT buffer pairs (2 x 2 nphi complex; the
+// pass head's row sums use the same space).
 // Of an order block: with TabLds its Legendre
 // table [j][l] (row stride (L+1)|1 doubles, loaded once per launch), then the
 // folds, partial sums and (sigma, u, N).
@@ -168,7 +168,7 @@ __host__ __device__ static size_t sle_tab_doubles(const vh_sht& s) {
 }
 static size_t sle_solve_lds(const vh_sht& s, bool tab) {
     const size_t rows = (size_t)2 * s.nphi * sizeof(double) + (size_t)s.ntw * sizeof(double2) +
-                        std::max((size_t)6 * s.nphi * sizeof(double2), (size_t)6 * 256 * sizeof(double));
+                        std::max((size_t)4 * s.nphi * sizeof(double2), (size_t)6 * 256 * sizeof(double));
     const size_t order = (tab ? sle_tab_doubles(s) * sizeof(double) : 0) +
                          ((size_t)2 * s.nh + std::max(256, s.L + 1) + 3 * (s.L + 1) + 16 * (size_t)s.nh) * sizeof(double2);
     return std::max(rows, order);
@@ -366,8 +366,11 @@ __device__ __forceinline__ void sle_order_stage(const sle_solve_args& a, int m, 
     }
 }
 
-template <bool TabLds>
-__global__ void __launch_bounds__(256) k_sle_solve(sle_solve_args a) {
+// MinW: wavefronts per SIMD the build must allow; 2 halves the registers
+// (with spills) so that two blocks share a CU when the grid (latitude pairs +
+// orders, 386 blocks at lmax 192) exceeds the CU count.
+template <bool TabLds, int MinW>
+__global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     extern __shared__ double2 lds[];
     const vh_sht& s = a.s;
     const int L = s.L, N = s.nphi, nlat = s.nlat, nh = s.nh;
@@ -384,7 +387,7 @@ __global__ void __launch_bounds__(256) k_sle_solve(sle_solve_args a) {
     // row-block LDS
     double* srow = (double*)lds;                          // [2][N]
     double2* TW = (double2*)(srow + 2 * N);               // [s.ntw]
-    double2* RB = TW + s.ntw;                             // [3][2N] FFT buffer pairs
+    double2* RB = TW + s.ntw;                             // [2][2N] FFT buffer pairs
     // order-block LDS
     double* T = (double*)lds;                             // [nh][tstride] (TabLds)
     double2* U = (double2*)(T + (TabLds ? sle_tab_doubles(s) : 0));
@@ -508,7 +511,9 @@ __global__ void __launch_bounds__(256) k_sle_solve(sle_solve_args a) {
         n_iters += 1.0;
         // ---- rows: inverse FFT of (u_j, N_j) -> Sraw_j, cs_j, the base load and its FFT
         if (is_row) {
-            double2* P = RB + (size_t)4 * N;   // base pair (rows 2p, 2p+1), then its FFT scratch
+            // after a wavefront's transform its result sits in buffer a when the
+            // pass count is even, else in b; the other one takes the row's base load
+            const size_t zoff = (s.nfac & 1) ? (size_t)N : 0, soff = (size_t)N - zoff;
             if (wave < 2) {
                 const int j = 2 * p + wave;
                 if (j < nlat) {
@@ -544,19 +549,25 @@ __global__ void __launch_bounds__(256) k_sle_solve(sle_solve_args a) {
                             rs += c * v;
                             const double base = VH_RHO_ICE * (pre ? gpre[u] : a.dig[id]) + VH_RHO_WATER * (c * v) +
                                                 (pre ? wpre[u] : a.wcorr[id]);
-                            if (wave == 0) P[i].x = base; else P[i].y = base;
+                            ((double*)(RB + (size_t)wave * 2 * N + soff))[i] = base;
                         }
                     }
                     rs = wave_sum(rs);
                     if (lane == 0) st_coh(a.cs + j, rs);
-                } else {
-                    for (int i = lane; i < N; i += 64) P[i].y = 0.0;
                 }
             }
             __syncthreads();
             if (wave == 0) {
+                // pack the pair's base loads (rows 2p, 2p+1) into wave 0's result
+                // buffer, transform with its other buffer as scratch
                 const int r0 = 2 * p;
-                double2* z = fft_wave(P, P + N, s, false, lane, TW);
+                const bool has1 = r0 + 1 < nlat;
+                double2* P = RB + zoff;
+                const double* b0 = (const double*)(RB + soff);
+                const double* b1 = (const double*)(RB + 2 * (size_t)N + soff);
+                for (int i = lane; i < N; i += 64) P[i] = make_double2(b0[i], has1 ? b1[i] : 0.0);
+                wave_lds_sync();
+                double2* z = fft_wave(P, RB + soff, s, false, lane, TW);
                 lds_to_pair_spectra<true>(z, L, N, scale, a.F + (size_t)r0 * (L + 1),
                                           r0 + 1 < nlat ? a.F + (size_t)(r0 + 1) * (L + 1) : nullptr, lane);
             }
@@ -718,31 +729,33 @@ void grid_bsl_integrand(hipStream_t st, int n, const double* h, const double* C,
 
 void sle_solve_launch(hipStream_t st, const sle_solve_args& a) {
     const vh_sht& s = a.s;
+    if (s.nh > 256) throw vh_error("sle: the fused sea-level loop supports nlat <= 512");
     const int grid = sle_solve_grid(s);   // row blocks, then order blocks
     // the Legendre table of each order stays in LDS when it fits beside the rest
     const bool tab = sle_solve_lds(s, true) <= (size_t)156 * 1024;
     const size_t lds = sle_solve_lds(s, tab);
-    const void* fn = tab ? (const void*)k_sle_solve<true> : (const void*)k_sle_solve<false>;
-    // resident-block capacity, checked once per (device, LDS size)
+    int dev = 0, ncu = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const bool two = grid > ncu;   // one block per CU does not hold the grid
+    const void* fn = tab ? (two ? (const void*)k_sle_solve<true, 2> : (const void*)k_sle_solve<true, 1>)
+                         : (two ? (const void*)k_sle_solve<false, 2> : (const void*)k_sle_solve<false, 1>);
+    // resident-block capacity, checked once per (device, kernel, LDS size)
     static thread_local int cap_dev = -1;
+    static thread_local const void* cap_fn = nullptr;
     static thread_local size_t cap_lds = 0;
     static thread_local long cap = 0;
-    int dev = 0;
-    HIPCHECK(hipGetDevice(&dev));
-    if (cap_dev != dev || cap_lds != lds) {
-        int coop = 0, per_cu = 0, ncu = 0;
+    if (cap_dev != dev || cap_fn != fn || cap_lds != lds) {
+        int coop = 0, per_cu = 0;
         HIPCHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
         if (!coop) throw vh_error("sle: the device does not support cooperative launches");
-        if (lds > 65536)
-            HIPCHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        if (lds > 65536) HIPCHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds));
-        HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        cap_dev = dev; cap_lds = lds; cap = (long)per_cu * ncu;
+        cap_dev = dev; cap_fn = fn; cap_lds = lds; cap = (long)per_cu * ncu;
     }
-    if (s.nh > 256) throw vh_error("sle: the fused inner loop supports nlat <= 512");
     if (cap < grid)
-        throw vh_error("sle: the fused inner loop needs " + std::to_string(grid) + " resident blocks, the device holds " +
-                       std::to_string(cap));
+        throw vh_error("sle: the fused sea-level loop needs " + std::to_string(grid) +
+                       " resident blocks, the device holds " + std::to_string(cap));
     sle_solve_args arg = a;
     void* params[] = {&arg};
     HIPCHECK(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(256), params, (unsigned)lds, st));

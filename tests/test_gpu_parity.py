@@ -183,6 +183,21 @@ def test_config1_full_size(oracle):
     assert mg.worst_mass_resid <= 1e-10
 
 
+def test_config1_lmax192(oracle):
+    """lmax 192 on its default 386 x 768 Gauss grid (a 4-pass 768-point FFT;
+    the sea-level loop's Legendre tables no longer fit in LDS and are read from
+    HBM), 40 elements: two coupling calls."""
+    c, mo, mg = _pair(oracle, 1, 192, ne=40)
+    for k in range(2):
+        h = c["ice"](k + 80)
+        mo.update(h, c["dt_yr"])
+        mg.update(h, c["dt_yr"])
+    assert np.array_equal(mg.C, mo.C)
+    assert rel(mg.rsl, mo.rsl) <= 1e-9
+    assert rel(mg.sigma_lm, mo.sigma_lm) <= 1e-9
+    assert mg.worst_mass_resid <= 1e-10
+
+
 def test_restart_roundtrip(oracle):
     """Memory out through the reference restart layout and back in: the
     continued run is bit-identical to the uninterrupted one."""
