@@ -96,6 +96,60 @@ def cpu_baseline(cfg, lmax, calls):
     return calls * c["dt_yr"] / 1000.0 / el, el, nsub
 
 
+def ensemble_bench(a, L):
+    """Configuration 4: an uneven ensemble (lmax 64/96/128/192, 32-64
+    elements, 16 of 64 members 3-D, sea level + rotation, dt 100 yr). Each
+    member is its own model, stepped one after another on one host thread
+    (the fused sea-level kernel of a member takes the whole device), every
+    member's ice history resident in HBM. value = timed member calls per
+    second / 200 (completed 200-call members per second)."""
+    import torch
+    from fastearth3d_amd import configs, lib
+    n = a.members
+    members = []
+    for i in range(n):
+        c = configs.ensemble_member(i, n)
+        se = lib.SolidEarth(c["params"], c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
+        if c["pert"] is not None:
+            se.enable_visc_3d(c["pert"])
+        ice = torch.from_numpy(np.stack([c["ice"](k) for k in range(a.warmup + a.steps)])).to("cuda:0")
+        members.append((se, ice, c))
+    torch.cuda.synchronize()
+
+    def run(i, k0, k1):
+        se, ice, c = members[i]
+        for k in range(k0, k1):
+            if L.vh_update_device(se._h, C.c_void_p(ice[k].data_ptr()), c["dt_yr"]):
+                raise RuntimeError(se._err())
+        return se.scalar(3)
+
+    for i in range(n):
+        run(i, 0, a.warmup)
+    torch.cuda.synchronize()
+    sub0 = sum(m[0].scalar(3) for m in members)
+    t0 = time.perf_counter()
+    for i in range(n):
+        run(i, a.warmup, a.warmup + a.steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    nsub = sum(m[0].scalar(3) for m in members) - sub0
+    calls = n * a.steps
+    kyr = sum(m[2]["dt_yr"] for m in members) * a.steps / 1000.0
+    mem = configs.ensemble_members(n)
+    out = {
+        "metric": "ens/s (64-member uneven ensemble, 1 GPU)", "value": round(calls / 200.0 / el, 4),
+        "unit": "ens/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(el / a.steps * 1000.0, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "c4 ensemble: %d members, lmax 64/96/128/192, 32-64 elements, %d 3-D, "
+                               "SLE + rotation, dt 100 yr" % (n, sum(x[2] for x in mem)),
+                   "members": n, "calls_per_member_timed": a.steps,
+                   "sub_steps_timed": int(nsub)},
+        "member_kyr_per_s": round(kyr / el, 4), "member_calls_per_s": round(calls / el, 2),
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +159,7 @@ def main():
     ap.add_argument("--lmax", type=int, default=128)
     ap.add_argument("--cpu-calls", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--members", type=int, default=64, help="ensemble size (--config 4)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 times): the reference "
                          "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -128,6 +183,9 @@ def main():
     L.vh_update_device.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
     L.vh_profile.argtypes = [C.c_void_p, C.c_int]
     L.vh_profile_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    if a.config == 4:
+        ensemble_bench(a, L)
+        return
 
     c = configs.config(a.config, lmax=a.lmax)
     p = c["params"]

@@ -160,9 +160,52 @@ def delta_visc(params, colat, lon, seed, lmax_delta=32, std=0.5):
     return s[:, None, None] * f[None, :, :]
 
 
+ENSEMBLE_LMAX = (64, 96, 128, 192)
+_topo_cache = {}
+
+
+def ensemble_members(n=64):
+    """The members of the ensemble workload: (lmax, ne, is_3d) per member.
+    lmax cycles through 64, 96, 128, 192; ne spreads over [32, 64]; 16 of the
+    members are laterally heterogeneous (3-D), taken among those with
+    lmax <= 128 (the tensor transforms are built for nlat <= 256)."""
+    cand = [i for i in range(n) if ENSEMBLE_LMAX[i % 4] != 192]
+    n3 = 16 * n // 64
+    three = set(cand[3 * k + k % 3] for k in range(n3) if 3 * k + k % 3 < len(cand))
+    return [(ENSEMBLE_LMAX[i % 4], 32 + (7 * i) % 33, i in three) for i in range(n)]
+
+
+def ensemble_member(i, n=64, seed=0):
+    """Workload of ensemble member i: the sea-level + rotation setting of
+    configuration 1 (migrating coastlines, subgrid term, dt 100 yr, 200
+    calls) at the member's lmax and element count, its own three ice domes,
+    and for 3-D members a lateral viscosity field of its own. The topography
+    is shared by the members of one lmax (its random field is generated up to
+    degree min(lmax, 64))."""
+    L, ne, is3d = ensemble_members(n)[i]
+    nlat, nphi = (192, 384) if L == 128 else (0, 0)
+    p = five_layer_params(L, ne, nlat, nphi)
+    p.rotation = 1
+    nphi, nlat = p.grid()
+    colat, lon = gauss_grid(nlat, nphi)
+    if L not in _topo_cache:
+        _topo_cache[L] = ocean_topography(colat, lon, min(L, 64), seed + 11)
+    z = _topo_cache[L]
+    domes = three_domes(colat, lon, seed + 101 + i)
+    pert = None
+    if is3d:
+        p.l_visc_3d = 1
+        pert = delta_visc(p, colat, lon, seed + 211 + i)
+    dt = 100.0
+    return dict(name="c4m%d" % i, params=p, z_bed_eq=z, h_ice_eq=np.zeros_like(z), h_ice_init=None,
+                ice=lambda k, dt=dt: domes * float(ice_history_factor((k + 1) * dt / 1000.0)),
+                dt_yr=dt, steps=200, pert=pert)
+
+
 def config(idx, lmax=None, ne=None, seed=0):
     """Workload of benchmark configuration idx (0, 1 or 2); lmax / ne may be
-    lowered for the oracle-sized parity cases."""
+    lowered for the oracle-sized parity cases. The ensemble (configuration 4)
+    is built member by member (ensemble_member)."""
     if idx == 0:
         L = lmax or 64
         p = five_layer_params(L, ne or 40)

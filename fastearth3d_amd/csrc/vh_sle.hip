@@ -9,6 +9,7 @@
 #pragma clang fp contract(off)
 #include <algorithm>
 #include <cfloat>
+#include <mutex>
 #include "vh_internal.h"
 #include "vh_sht_dev.h"
 #include "vh_sle.h"
@@ -121,11 +122,12 @@ __global__ void k_bsl_integrand(int n, const double* __restrict__ h, const doubl
 // analysis of the solve. Each stage's arithmetic is fixed by the sizes alone
 // (partition of the Legendre sums, order of the Cs_int butterfly), so every
 // block reaches the same values and a rerun gives identical bits.
-// Data handed between blocks (F, FC, F2, cs, part) moves through coherent
-// loads and stores (ld_coh / st_coh), so a grid barrier needs no cache
-// write-back or invalidate: each wavefront waits for its stores, then the
-// counter bar[0] is raised and awaited (it runs on across launches; the host
-// passes the value it starts from).
+// Data handed between blocks (F, FC, F2, cs, part, part3) moves through
+// agent-scope loads and stores (ld_coh / st_coh), which bypass the XCDs'
+// non-coherent caches, so a grid barrier needs no cache write-back or
+// invalidate: each wavefront waits for its stores, then the counter bar[0] is
+// raised and awaited (it runs on across launches; each launch starts from
+// bar[2]). Data from the kernels before the launch is acquired once at entry.
 // The wait is bounded: a block that never arrives raises bar[1] and res[2]
 // instead of a hang.
 
@@ -139,7 +141,7 @@ __global__ void k_bsl_integrand(int n, const double* __restrict__ h, const doubl
 __device__ __forceinline__ void sle_grid_sync(unsigned* bar, unsigned& target, double* res) {
     target += gridDim.x;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_s_waitcnt(0);   // this wavefront's coherent stores are complete
+    __builtin_amdgcn_s_waitcnt(0);   // this wavefront's stores are complete
     __syncthreads();
     if (threadIdx.x == 0) {
         __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -380,6 +382,11 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     const int p = blk, m = blk - npairs;
     __shared__ double redmax[4][4];
     __shared__ double sdphi;
+    // Inputs written by the kernels before this launch (ice, drift, rsl, ...)
+    // were written by other XCDs: an agent-scope acquire invalidates this
+    // CU's and XCD's cached copies before anything is read, whatever fences
+    // the launch itself carries; a release at the end publishes the outputs.
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     unsigned target = a.bar[2];   // the counter keeps counting across launches
     const double scale = 2.0 * VH_PI / N, dlon = 2.0 * VH_PI / N;
     __shared__ double stot[4];    // pass totals: C_int, int d_ice_g, int topo0 (C - C0); flips
@@ -685,6 +692,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
         a.res[4] = n_iters; a.res[5] = n_passes; a.res[6] = last_ice; a.res[7] = last_zeta;
         a.bar[2] = target;
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
 }
 
 __global__ void k_sle_bar_reset(unsigned* bar) {
@@ -697,6 +705,8 @@ void sle_bar_reset(hipStream_t st, unsigned* bar) {
 }
 
 int sle_solve_grid(const vh_sht& s) { return (s.nlat + 1) / 2 + s.L + 1; }
+
+
 
 long sle_solve_max_barriers(const vh_sht& s, int n_outer, int n_inner) {
     return (long)sle_solve_grid(s) * n_outer * (4 + 2 * (long)n_inner);
@@ -758,5 +768,16 @@ void sle_solve_launch(hipStream_t st, const sle_solve_args& a) {
                        " resident blocks, the device holds " + std::to_string(cap));
     sle_solve_args arg = a;
     void* params[] = {&arg};
+    // Cooperative launches of different models (streams) run one after the
+    // other on the device: two grids that each hold part of the CUs while
+    // waiting for the rest would never finish (the stream waits on the
+    // previous launch's completion event).
+    static std::mutex mu;
+    static hipEvent_t last_ev[64] = {};
+    std::lock_guard<std::mutex> lock(mu);
+    if (dev < 0 || dev >= 64) throw vh_error("sle: device index out of range");
+    if (!last_ev[dev]) HIPCHECK(hipEventCreateWithFlags(&last_ev[dev], hipEventDisableTiming));
+    else HIPCHECK(hipStreamWaitEvent(st, last_ev[dev], 0));
     HIPCHECK(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(256), params, (unsigned)lds, st));
+    HIPCHECK(hipEventRecord(last_ev[dev], st));
 }
