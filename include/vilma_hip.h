@@ -21,7 +21,12 @@
  *     vh_last_error() returns the message. The reference aborts with
  *     `error stop 'message'` instead (e.g. src/vilma_coupling.f90:248,279);
  *     the Fortran shim turns a nonzero status back into `error stop message`.
- *   - A handle is not thread-safe; distinct handles may be used concurrently.
+ *   - A handle is not thread-safe; distinct handles may be used from different
+ *     threads. Their sea-level solves (one cooperative kernel each, holding a
+ *     block on most CUs) run one after another on the device; a solve whose
+ *     blocks cannot all become resident within its bounded wait (a device
+ *     saturated by other handles' work) fails with an error rather than hang,
+ *     so throughput runs step the handles of one device from one thread.
  */
 #ifndef VILMA_HIP_H
 #define VILMA_HIP_H
