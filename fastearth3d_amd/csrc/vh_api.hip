@@ -13,6 +13,7 @@
 #include "vh_internal.h"
 #include "vh_rot.h"
 #include "vh_sle.h"
+#include "vh_adjoint.h"
 #include "../../include/vilma_hip.h"
 
 namespace {
@@ -152,6 +153,10 @@ void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
 }  // namespace
 
 // ------------------------------------------------------------------ model
+struct adj_ctx;
+void adj_free(adj_ctx* a);
+bool adj_recording(const adj_ctx* a);
+
 struct vh_model {
     vh_params par{};
     std::string err;
@@ -244,6 +249,8 @@ struct vh_model {
     }
     bool lat_visc = false;
     int ne3d = 0;
+    std::vector<int> e3mesh;   // mesh element of each 3-D element
+    adj_ctx* adj = nullptr;    // reverse-mode record (vh_forward_record / vh_backward)
 
     size_t ng() const { return (size_t)d.nphi * d.nlat; }
     void sync() { HIPCHECK(hipStreamSynchronize(st)); }
@@ -262,6 +269,7 @@ struct vh_model {
 };
 
 vh_model::~vh_model() {
+    if (adj) adj_free(adj);
     if (dplan) vh_drift_plan_destroy(dplan);
     void* ptrs[] = {d_r, d_mu, d_rho, d_Rk, d_MkPerDt, d_emap, d_mnodes, d_mnode, d_kdeg, d_is3d, d_ab, d_dr, d_dc,
                     d_ab1, d_null1, d_gu, d_gn, d_gv, d_xUn, d_xVn, d_ipiv, d_ipiv1, d_fail, d_tau[0], d_tau[1],
@@ -606,7 +614,8 @@ void advance_memory(vh_model* h, const double* sigma, int nxt) {
 }
 
 // ---------------------------------------------------------------- SLE solve
-struct sle_res { double mass_resid = 0, err_inf = 0; };
+struct sle_res { double mass_resid = 0, err_inf = 0, n_last = 0, c_int = 0; };
+void adj_stash_push(vh_model* h, double dt, double frac, const sle_res& r);
 
 // sle_solve (src/vilma_sle.f90:124-405), forward-Euler response branch.
 // The advance writes tau_{n+1} into the spare buffer; the caller commits it.
@@ -668,6 +677,8 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
     const double Cs = hr[10] - zeta_int;
     res.mass_resid = std::fabs(ice_int) > 0.0 ? std::fabs(Cs - ice_int) / std::fabs(ice_int) : std::fabs(Cs);
     res.err_inf = hr[11];
+    res.n_last = hr[0];
+    res.c_int = hr[3];
     return res;
 }
 
@@ -703,6 +714,7 @@ void stepper_advance(vh_model* h, double t0, double t1, const double* srot) {
         double scale = std::max(tau0, tau_run);
         bool accept = scale <= 0.0 ? finite : (finite && err_inf <= h->guard * scale);
         if (accept) {
+            if (h->adj && adj_recording(h->adj)) adj_stash_push(h, dt, frac, r);
             if (h->kind == 2) { h->cur ^= 1; h->maxn_cur = err_inf; }
             tau_run = std::max(tau_run, err_inf);
             t = t + dt; h->n_accept++;
@@ -752,6 +764,330 @@ void update(vh_model* h, const double* h_ice, double dt_yr, bool on_device) {
     h->prof_collect();
     double c_int = h->h_pin[0];
     h->bsl = c_int > 0.0 ? -(VH_RHO_ICE / VH_RHO_WATER) * h->h_pin[1] / c_int : 0.0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- adjoint
+// Reverse mode over a run of coupling calls: the forward record keeps the
+// model state every `every` calls; the backward sweep restores each
+// checkpoint in turn (last first), re-runs that segment's calls keeping every
+// accepted FE sub-step's inputs in HBM, and walks the sub-steps backwards
+// through the transposed kernels (vh_adjoint.hip). Discrete decisions (skip
+// gate, coastline mask, sub-step sizes, 3-D element set) are those of the
+// forward run; the sea-level solve's warm start is not differentiated (the
+// converged rsl does not depend on it).
+struct adj_sub {   // one accepted FE sub-step: tau_n, closing sigma, drift, last mask, norms
+    double *tau = 0, *sigma = 0, *dN = 0, *C = 0, *mnorm = 0;
+    double dt = 0, frac = 0, c_int = 0;
+    int call = 0, n_last = 0;
+};
+struct adj_ckpt {
+    double *tau = 0, *mnorm = 0, *rsl = 0, *hice = 0;
+    double maxn_cur = 0, time_yr = 0, time_resp = 0;
+};
+struct adj_ctx {
+    int ncalls = 0, every = 1;
+    double dt_yr = 0;
+    size_t ice_n = 0;
+    double* ice = nullptr;   // [ncalls][ng]
+    std::vector<adj_ckpt> ck;
+    std::vector<adj_sub> sub;
+    size_t nsub = 0;
+    bool recording = false, recorded = false;
+    int call = 0;
+    // reverse sweep
+    bool work = false;
+    size_t rh_n = 0, ih_n = 0;
+    double *tauh[2] = {0, 0};
+    int ah = 0;
+    double *Ue = 0, *mpart = 0, *Kh = 0, *dNh = 0, *dsurfh = 0, *sigh = 0, *Fh = 0;
+    double *rate_h = 0, *iceh = 0, *rh = 0, *pgrad = 0;
+    double2 *Xa = 0, *Fa = 0;
+    double *gx = 0, *gb = 0, *gt = 0, *gy = 0, *part = 0, *sc = 0;
+    double2 *F = 0, *q = 0, *q1 = 0;
+    int *e3of_mesh = 0, *em_of = 0;
+};
+
+bool adj_recording(const adj_ctx* a) { return a->recording; }
+
+void adj_free(adj_ctx* a) {
+    std::vector<void*> p = {a->ice, a->tauh[0], a->tauh[1], a->Ue, a->mpart, a->Kh, a->dNh, a->dsurfh, a->sigh,
+                            a->Fh, a->rate_h, a->iceh, a->rh, a->pgrad, a->Xa, a->Fa, a->gx, a->gb, a->gt, a->gy,
+                            a->part, a->sc, a->F, a->q, a->q1, a->e3of_mesh, a->em_of};
+    for (auto& c : a->ck) { p.push_back(c.tau); p.push_back(c.mnorm); p.push_back(c.rsl); p.push_back(c.hice); }
+    for (auto& u : a->sub) { p.push_back(u.tau); p.push_back(u.sigma); p.push_back(u.dN); p.push_back(u.C); p.push_back(u.mnorm); }
+    for (void* x : p)
+        if (x) (void)hipFree(x);
+    delete a;
+}
+
+namespace {
+size_t adj_ntau(const vh_model* h) { return (size_t)24 * std::max(h->d.nem, 1) * h->d.nk; }
+size_t adj_ndN(const vh_model* h) { return (size_t)4 * std::max(h->d.nmn, 1) * h->d.nk; }
+void d2d(double* dst, const double* src, size_t n, hipStream_t st) {
+    HIPCHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+}
+
+void adj_save(vh_model* h, adj_ckpt& c) {
+    const size_t ng = h->ng();
+    if (!c.tau) {
+        c.tau = dalloc<double>(adj_ntau(h)); c.mnorm = dalloc<double>(h->d.nk + 1);
+        c.rsl = dalloc<double>(ng); c.hice = dalloc<double>(ng);
+    }
+    d2d(c.tau, h->d_tau[h->cur], adj_ntau(h), h->st);
+    d2d(c.mnorm, h->d_mnorm[h->cur], h->d.nk, h->st);
+    d2d(c.mnorm + h->d.nk, h->d_maxn + h->cur, 1, h->st);
+    d2d(c.rsl, h->d_rsl, ng, h->st);
+    d2d(c.hice, h->d_hice, ng, h->st);
+    c.maxn_cur = h->maxn_cur; c.time_yr = h->time_yr; c.time_resp = h->time_resp;
+}
+
+void adj_restore(vh_model* h, const adj_ckpt& c) {
+    const size_t ng = h->ng();
+    d2d(h->d_tau[h->cur], c.tau, adj_ntau(h), h->st);
+    d2d(h->d_mnorm[h->cur], c.mnorm, h->d.nk, h->st);
+    d2d(h->d_maxn + h->cur, c.mnorm + h->d.nk, 1, h->st);
+    d2d(h->d_rsl, c.rsl, ng, h->st);
+    d2d(h->d_hice, c.hice, ng, h->st);
+    h->maxn_cur = c.maxn_cur; h->time_yr = c.time_yr; h->time_resp = c.time_resp;
+}
+
+void adj_stash_push(vh_model* h, double dt, double frac, const sle_res& r) {
+    adj_ctx* A = h->adj;
+    const vh_dims& d = h->d;
+    if (A->nsub == A->sub.size()) {
+        adj_sub u;
+        u.tau = dalloc<double>(adj_ntau(h)); u.sigma = dalloc<double>(2 * (size_t)d.nlm);
+        u.dN = dalloc<double>(adj_ndN(h)); u.C = dalloc<double>(h->ng()); u.mnorm = dalloc<double>(d.nk + 1);
+        A->sub.push_back(u);
+    }
+    adj_sub& u = A->sub[A->nsub++];
+    d2d(u.tau, h->d_tau[h->cur], adj_ntau(h), h->st);
+    d2d(u.sigma, (const double*)h->d_sigma, 2 * (size_t)d.nlm, h->st);
+    d2d(u.dN, h->d_dN, adj_ndN(h), h->st);
+    d2d(u.C, h->d_C, h->ng(), h->st);
+    d2d(u.mnorm, h->d_mnorm[h->cur], d.nk, h->st);
+    d2d(u.mnorm + d.nk, h->d_maxn + h->cur, 1, h->st);
+    u.dt = dt; u.frac = frac; u.c_int = r.c_int; u.n_last = (int)r.n_last; u.call = A->call;
+}
+
+void adj_check(const vh_model* h) {
+    if (h->kind != 2) throw vh_error("adjoint: needs earth_response=ve");
+    if (h->rot_on) throw vh_error("adjoint: rotation feedback is not differentiated (rotation must be off)");
+}
+
+void forward_record(vh_model* h, int ncalls, const double* ice_hist, bool dev, double dt_yr, int every,
+                    double* rsl_out, bool rsl_dev) {
+    adj_check(h);
+    if (ncalls <= 0) throw vh_error("vh_forward_record: ncalls must be positive");
+    if (!(dt_yr > 0.0)) throw vh_error("vh_forward_record: dt_yr must be positive");
+    if (!h->adj) h->adj = new adj_ctx();
+    adj_ctx* A = h->adj;
+    const size_t ng = h->ng();
+    A->ncalls = ncalls; A->every = std::max(1, every); A->dt_yr = dt_yr; A->recorded = false;
+    if (A->ice_n < (size_t)ncalls * ng) {
+        if (A->ice) HIPCHECK(hipFree(A->ice));
+        A->ice = dalloc<double>((size_t)ncalls * ng);
+        A->ice_n = (size_t)ncalls * ng;
+    }
+    HIPCHECK(hipMemcpyAsync(A->ice, ice_hist, (size_t)ncalls * ng * sizeof(double),
+                            dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->st));
+    const int nseg = (ncalls + A->every - 1) / A->every;
+    if ((int)A->ck.size() < nseg + 1) A->ck.resize(nseg + 1);
+    for (int c = 0; c < ncalls; c++) {
+        if (c % A->every == 0) adj_save(h, A->ck[c / A->every]);
+        update(h, A->ice + (size_t)c * ng, dt_yr, true);
+        if (rsl_out)
+            HIPCHECK(hipMemcpyAsync(rsl_out + (size_t)c * ng, h->d_rsl, ng * sizeof(double),
+                                    rsl_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, h->st));
+    }
+    adj_save(h, A->ck[nseg]);
+    h->sync();
+    A->recorded = true;
+}
+
+void adj_alloc_work(vh_model* h) {
+    adj_ctx* A = h->adj;
+    const vh_dims& d = h->d;
+    const size_t ng = h->ng();
+    if (!A->work) {
+        A->tauh[0] = dalloc<double>(adj_ntau(h)); A->tauh[1] = dalloc<double>(adj_ntau(h));
+        A->Ue = dalloc<double>((size_t)8 * std::max(d.nem, 1) * d.nk);
+        A->mpart = dalloc<double>((size_t)std::max(d.nem, 1) * vh_advance_adjoint_nbx(d));
+        A->Kh = dalloc<double>(std::max(d.nem, 1));
+        A->dNh = dalloc<double>(adj_ndN(h));
+        A->dsurfh = dalloc<double>((size_t)6 * d.nk);
+        A->sigh = dalloc<double>(2 * (size_t)d.nlm);
+        A->Fh = dalloc<double>((size_t)4 * std::max(d.nmn, 1) * d.nk);
+        A->gx = dalloc<double>(ng); A->gb = dalloc<double>(ng); A->gt = dalloc<double>(ng); A->gy = dalloc<double>(ng);
+        A->part = dalloc<double>(d.nlat); A->sc = dalloc<double>(4);
+        A->F = dalloc<double2>((size_t)d.nlat * (d.L + 1));
+        A->q = dalloc<double2>(d.nlm); A->q1 = dalloc<double2>(d.nlm);
+        A->pgrad = dalloc<double>((size_t)d.ne * ng);
+        A->e3of_mesh = dalloc<int>(d.ne); A->em_of = dalloc<int>(d.ne);
+        A->work = true;
+    }
+    // 3-D buffers follow the current lateral set
+    if (A->rate_h) { HIPCHECK(hipFree(A->rate_h)); A->rate_h = 0; }
+    if (A->Xa) { HIPCHECK(hipFree(A->Xa)); A->Xa = 0; }
+    if (A->Fa) { HIPCHECK(hipFree(A->Fa)); A->Fa = 0; }
+    if (h->lat_visc && h->ne3d > 0) {
+        A->rate_h = dalloc<double>((size_t)h->ne3d * ng);
+        A->Xa = dalloc<double2>((size_t)h->nbat * 4 * d.nlm);
+        A->Fa = dalloc<double2>((size_t)std::max(h->tchunk, 1) * 6 * (d.L + 1) * d.nlat);
+    }
+    if (A->rh_n < (size_t)A->ncalls * ng) {
+        if (A->rh) HIPCHECK(hipFree(A->rh));
+        A->rh = dalloc<double>((size_t)A->ncalls * ng);
+        A->rh_n = (size_t)A->ncalls * ng;
+    }
+    if (A->ih_n < (size_t)(A->ncalls + 1) * ng) {
+        if (A->iceh) HIPCHECK(hipFree(A->iceh));
+        A->iceh = dalloc<double>((size_t)(A->ncalls + 1) * ng);
+        A->ih_n = (size_t)(A->ncalls + 1) * ng;
+    }
+    std::vector<int> e3of(d.ne, -1), emof(d.ne, -1);
+    for (size_t i = 0; i < h->e3mesh.size() && h->lat_visc; i++) e3of[h->e3mesh[i]] = (int)i;
+    for (int em = 0; em < d.nem; em++) emof[h->emap[em]] = em;
+    upload(A->e3of_mesh, e3of.data(), d.ne, h->st);
+    upload(A->em_of, emof.data(), d.ne, h->st);
+}
+
+// Transposed sea-level solve of one sub-step (src/vilma_sle.f90:296-355 for
+// the forward): sigma_h (adjoint of the closing load transform) and rh (of
+// the output rsl, or null) -> dsurf_h and the ice adjoints of the call's two
+// ice fields. Grid adjoints are carried in density form (divided by W).
+void sle_adjoint(vh_model* h, const adj_sub& S, const double* rh) {
+    adj_ctx* A = h->adj;
+    hipStream_t st = h->st;
+    const vh_sht& s = h->s;
+    const vh_dims& d = h->d;
+    const size_t ng = h->ng();
+    auto syn = [&](const double2* q, double* g) {
+        vh_legendre_synthesis(st, s, q, 1, A->F);
+        vh_fft_inverse(st, s, A->F, d.nlat, g);
+    };
+    auto anl = [&](const double* g, double2* q) {
+        vh_fft_forward(st, s, g, d.nlat, A->F);
+        vh_legendre_analysis(st, s, A->F, 1, q);
+    };
+    sle_adj_grid_args g{};
+    g.n = (int)ng; g.nphi = d.nphi; g.dlon = 2.0 * VH_PI / d.nphi; g.w = s.w; g.C = S.C; g.frac = S.frac;
+    g.ih0 = A->iceh + (size_t)S.call * ng; g.ih1 = A->iceh + (size_t)(S.call + 1) * ng;
+    // q1 = sigma_h with m > 0 halved; t = Syn(q1) (so the sigma path's grid adjoint is W t)
+    vh_sle_adj_spec(st, d.nlm, d.nk, 0, (const double2*)A->sigh, A->q1, nullptr, h->d_gu, h->d_gn, h->g, nullptr);
+    syn(A->q1, A->gt);
+    const int nl = S.c_int > 0.0 ? S.n_last : 0;
+    if (nl > 0) {
+        g.inv_cint = 1.0 / S.c_int;
+        sle_adj_grid_args a = g;
+        a.mode = 0; a.rh = rh; a.t = A->gt; a.x = A->gx; a.b = A->gb;
+        vh_sle_adj_grid(st, a);
+        for (int i = 0; i < nl; i++) {
+            a = g; a.mode = 1; a.x = A->gx; a.b = A->gb;
+            vh_sle_adj_grid(st, a);
+            if (i == nl - 1) break;
+            // x <- rho_w C Syn((gn - gu) Anl(x - C I(x)/C_int))
+            vh_surface_integral(st, s, A->gx, A->part, A->sc);
+            a = g; a.mode = 2; a.x = A->gx; a.I = A->sc; a.y = A->gy;
+            vh_sle_adj_grid(st, a);
+            anl(A->gy, A->q);
+            vh_sle_adj_spec(st, d.nlm, d.nk, 1, nullptr, A->q, nullptr, h->d_gu, h->d_gn, h->g, nullptr);
+            syn(A->q, A->gt);
+            a = g; a.mode = 3; a.t = A->gt; a.x = A->gx;
+            vh_sle_adj_grid(st, a);
+        }
+        // y = b - C I(b)/C_int; q = Anl(y) -> dsurf_h; q <- (gn - gu) q + q1
+        vh_surface_integral(st, s, A->gb, A->part, A->sc + 1);
+        a = g; a.mode = 2; a.x = A->gb; a.I = A->sc + 1; a.y = A->gy;
+        vh_sle_adj_grid(st, a);
+        anl(A->gy, A->q);
+        vh_sle_adj_spec(st, d.nlm, d.nk, 2, nullptr, A->q, A->q1, h->d_gu, h->d_gn, h->g, A->dsurfh);
+        syn(A->q, A->gt);
+        a = g; a.mode = 4; a.t = A->gt; a.I = A->sc + 1;
+        vh_sle_adj_grid(st, a);
+    } else {
+        vh_fill(st, A->dsurfh, 0.0, (size_t)6 * d.nk);
+        sle_adj_grid_args a = g;
+        a.mode = 4; a.t = A->gt; a.I = nullptr; a.inv_cint = 0.0;
+        vh_sle_adj_grid(st, a);
+    }
+}
+
+void reverse_sub(vh_model* h, const adj_sub& S, const double* rh) {
+    adj_ctx* A = h->adj;
+    hipStream_t st = h->st;
+    const vh_dims& d = h->d;
+    const size_t ng = h->ng();
+    double* th_in = A->tauh[A->ah];
+    double* th_out = A->tauh[A->ah ^ 1];
+    if (h->lat_visc && h->ne3d > 0) {
+        vh_t3_gather(st, d, h->v, h->d_e3em, h->ne3d, S.tau, S.sigma, S.dN, h->d_X);
+        vh_t3_pack_adj(st, d, h->v, h->d_e3em, h->ne3d, h->d_n6, th_in, A->Xa);
+        for (int b0 = 0; b0 < h->nbat; b0 += h->tchunk) {
+            const int nb = std::min(h->tchunk, h->nbat - b0);
+            vh_tsh_synth(st, h->s, h->d_X + (size_t)b0 * 4 * d.nlm, nb, h->d_Fd);
+            vh_tsh_synth(st, h->s, A->Xa + (size_t)b0 * 4 * d.nlm, nb, A->Fa);
+            vh_t3_rows_adj(st, h->s, A->Fa, h->d_Fd, nb / 3, h->d_rate3 + (size_t)(b0 / 3) * ng, S.dt,
+                           A->rate_h + (size_t)(b0 / 3) * ng);
+            vh_tsh_analysis(st, h->s, A->Fa, nb, h->d_n6, A->Xa + (size_t)b0 * 4 * d.nlm);
+        }
+    }
+    adv_adj_args a{};
+    a.nr = d.nr; a.nk = d.nk; a.nem = d.nem; a.nlm = d.nlm; a.dt = S.dt; a.v = h->v;
+    a.tauh_in = th_in; a.tauh_out = th_out; a.tau = S.tau; a.sigma = S.sigma; a.dN = S.dN;
+    a.e3of = h->d_e3of; a.Xh = A->Xa; a.n6 = h->d_n6; a.Ue = A->Ue; a.mpart = A->mpart;
+    vh_advance_adjoint(st, d, a, h->d_mnodes, vh_drift_plan_node_el(h->dplan), A->dNh, A->sigh, A->Kh);
+    sle_adjoint(h, S, rh);
+    vh_drift_adjoint(st, h->dplan, d, h->v, S.mnorm, S.mnorm + d.nk, h->skip_tol, h->deg1_cm, A->dNh, A->dsurfh,
+                     A->Fh, th_out);
+    A->ah ^= 1;
+}
+
+void backward(vh_model* h, const double* dJ_drsl, bool dev, double* dJ_dpert, double* dJ_dice, bool out_dev) {
+    adj_ctx* A = h->adj;
+    if (!A || !A->recorded) throw vh_error("vh_backward: no forward record (call vh_forward_record first)");
+    adj_check(h);
+    hipStream_t st = h->st;
+    const vh_dims& d = h->d;
+    const size_t ng = h->ng();
+    adj_alloc_work(h);
+    HIPCHECK(hipMemcpyAsync(A->rh, dJ_drsl, (size_t)A->ncalls * ng * sizeof(double),
+                            dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    A->ah = 0;
+    vh_fill(st, A->tauh[0], 0.0, adj_ntau(h));
+    vh_fill(st, A->Kh, 0.0, std::max(d.nem, 1));
+    vh_fill(st, A->iceh, 0.0, (size_t)(A->ncalls + 1) * ng);
+    if (A->rate_h) vh_fill(st, A->rate_h, 0.0, (size_t)h->ne3d * ng);
+    const int nseg = (A->ncalls + A->every - 1) / A->every;
+    for (int seg = nseg - 1; seg >= 0; seg--) {
+        adj_restore(h, A->ck[seg]);
+        A->nsub = 0;
+        A->recording = true;
+        const int c1 = std::min(A->ncalls, (seg + 1) * A->every);
+        try {
+            for (int c = seg * A->every; c < c1; c++) {
+                A->call = c;
+                update(h, A->ice + (size_t)c * ng, A->dt_yr, true);
+            }
+        } catch (...) {
+            A->recording = false;
+            throw;
+        }
+        A->recording = false;
+        for (size_t i = A->nsub; i-- > 0;) {
+            const adj_sub& S = A->sub[i];
+            const bool last = i + 1 == A->nsub || A->sub[i + 1].call != S.call;
+            reverse_sub(h, S, last ? A->rh + (size_t)S.call * ng : nullptr);
+        }
+    }
+    adj_restore(h, A->ck[nseg]);
+    vh_pert_grad(st, d.ne, ng, A->e3of_mesh, A->em_of, h->d_MkPerDt, h->d_rate3, A->rate_h, A->Kh, A->pgrad);
+    const hipMemcpyKind k = out_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (dJ_dpert) HIPCHECK(hipMemcpyAsync(dJ_dpert, A->pgrad, (size_t)d.ne * ng * sizeof(double), k, st));
+    if (dJ_dice) HIPCHECK(hipMemcpyAsync(dJ_dice, A->iceh, (size_t)(A->ncalls + 1) * ng * sizeof(double), k, st));
+    h->sync();
 }
 
 // degree-major (device) <-> SHTns LM order (API) for complex spectra
@@ -1024,6 +1360,7 @@ void enable_lateral(vh_model* h, const double* pert) {
                     (void*)h->d_cdel})
         if (p) HIPCHECK(hipFree(p));
     h->ne3d = (int)e3.size();
+    h->e3mesh = e3;
     h->nbat = 3 * h->ne3d;
     h->tchunk = std::min(h->nbat, 48);
     h->d_e3em = dalloc<int>(e3em.size()); upload(h->d_e3em, e3em.data(), e3em.size(), st);
@@ -1207,6 +1544,24 @@ int vh_get_gains(vh_model* h, double* g3, double* xn) {
         }
         h->sync();
     });
+}
+
+// reverse mode (no counterpart in the reference: gradients of an rsl misfit)
+int vh_forward_record(vh_model* h, int ncalls, const double* h_ice_hist, double dt_yr, int ckpt_every,
+                      double* rsl_out) {
+    return guarded(h, [&] { forward_record(h, ncalls, h_ice_hist, false, dt_yr, ckpt_every, rsl_out, false); });
+}
+int vh_forward_record_device(vh_model* h, int ncalls, const double* h_ice_hist_device, double dt_yr, int ckpt_every,
+                             double* rsl_out_device) {
+    return guarded(h, [&] {
+        forward_record(h, ncalls, h_ice_hist_device, true, dt_yr, ckpt_every, rsl_out_device, true);
+    });
+}
+int vh_backward(vh_model* h, const double* dJ_drsl, double* dJ_dlogeta, double* dJ_dice) {
+    return guarded(h, [&] { backward(h, dJ_drsl, false, dJ_dlogeta, dJ_dice, false); });
+}
+int vh_backward_device(vh_model* h, const double* dJ_drsl_device, double* dJ_dlogeta_device, double* dJ_dice_device) {
+    return guarded(h, [&] { backward(h, dJ_drsl_device, true, dJ_dlogeta_device, dJ_dice_device, true); });
 }
 
 int vh_sht_analysis(vh_model* h, const double* grid, double* lm) {

@@ -194,6 +194,94 @@ __global__ void __launch_bounds__(DRIFT_THREADS) k_drift(drift_args a) {
     }
 }
 
+// ------------------------------------------------------------------ adjoint
+// Transpose of the two kernels above for the same skip-gate set (slots the
+// gate drops have a zero drift, so they pass no adjoint back).
+//   k_drift_gT:        Fh = G_l^T [dN_h; dsurf_h] per (slot, re/im),
+//                      Fh[(c*2 + reim)][nk], c = 2 q + uv the node dof;
+//   k_drift_force_adj: tau_h += (forcing)^T of the element's four test dofs.
+template <int RW>
+__global__ void __launch_bounds__(DRIFT_THREADS) k_drift_gT(drift_args a, const double* __restrict__ dNh,
+                                                           const double* __restrict__ dsurfh, double* Fh) {
+    extern __shared__ double Y[];   // [nrow][rw]
+    const int2 job = a.jobs[blockIdx.x];
+    const int l = job.x, m0 = job.y;
+    constexpr int rw = RW, ts = RW / 2;
+    const int ncol = 2 * a.nmn, nrow = ncol + 3, nk = a.nk;
+    const int nsl = min(ts, l + 1 - m0);
+    const int kb = l * (l + 1) / 2 - 1 + m0;
+    const double thr = a.skip_tol * a.maxnorm[0];
+    const int rhs = threadIdx.x % rw, ngrp = DRIFT_THREADS / rw;
+    const int grp = RW == 64 ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : threadIdx.x / rw;
+    const int ml = rhs % ts, reim = rhs / ts;
+    const bool valid = ml < nsl;
+    const bool act = valid && a.mnorm[kb + ml] > thr;
+    const int k = kb + ml;
+    for (int row = grp; row < nrow; row += ngrp) {
+        double y = 0.0;
+        if (act) {
+            if (row < ncol) {
+                const int q = row >> 1, uv = row & 1;
+                y = dNh[(size_t)(q * 4 + uv * 2 + reim) * nk + k];
+            } else {
+                const int sidx = row - ncol;   // 0: U, 1: F, 2: V
+                y = (sidx == 1 && l == 1 && !a.deg1_cm) ? 0.0 : dsurfh[(size_t)(sidx * nk + k) * 2 + reim];
+            }
+        }
+        Y[row * rw + rhs] = y;
+    }
+    __syncthreads();
+    const double* __restrict__ G = a.G + (size_t)(l - 1) * nrow * ncol;
+    typedef const __attribute__((address_space(4))) double cdouble;
+    const int nq = (ncol + 3) / 4;
+    for (int qi = grp; qi < nq; qi += ngrp) {
+        const int c0 = 4 * qi;
+        const int i0 = min(c0 + 0, ncol - 1), i1 = min(c0 + 1, ncol - 1), i2 = min(c0 + 2, ncol - 1),
+                  i3 = min(c0 + 3, ncol - 1);
+        double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0;
+#pragma unroll 4
+        for (int row = 0; row < nrow; row++) {
+            cdouble* g = (cdouble*)(G + (size_t)row * ncol);
+            const double y = Y[row * rw + rhs];
+            f0 = fma(g[i0], y, f0);
+            f1 = fma(g[i1], y, f1);
+            f2 = fma(g[i2], y, f2);
+            f3 = fma(g[i3], y, f3);
+        }
+        if (!valid) continue;
+        const double f[4] = {f0, f1, f2, f3};
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if (c0 + i < ncol) Fh[((size_t)(c0 + i) * 2 + reim) * nk + k] = f[i];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_drift_force_adj(drift_args a, const double* __restrict__ Fh,
+                                                         double* tauh) {
+    const int lane = threadIdx.x & 63, em = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = blockIdx.x * 64 + lane, nk = a.nk;
+    if (k >= nk || em >= a.nem) return;
+    if (!(a.mnorm[k] > a.skip_tol * a.maxnorm[0])) return;
+    const int l = a.kdeg[k], e = a.emap[em];
+    const double Jr = (double)l * (double)(l + 1);
+    const int qlo = a.mnode[e], qup = a.mnode[e + 1];
+    double* t = tauh + (size_t)em * 24 * nk + k;
+#pragma unroll
+    for (int reim = 0; reim < 2; reim++) {
+        // test dofs U_e, U_e+1, V_e, V_e+1 <- (lower, U), (upper, U), (lower, V), (upper, V)
+        const double fl[4] = {Fh[((size_t)(2 * qlo + 0) * 2 + reim) * nk + k], Fh[((size_t)(2 * qup + 0) * 2 + reim) * nk + k],
+                              Fh[((size_t)(2 * qlo + 1) * 2 + reim) * nk + k], Fh[((size_t)(2 * qup + 1) * 2 + reim) * nk + k]};
+        double A[4], B[4], Cc[4];
+        dissipative_element_adj(a.r[e], a.r[e + 1], Jr, fl, A, B, Cc);
+#pragma unroll
+        for (int lam = 0; lam < 4; lam++) {
+            t[(size_t)((0 * 2 + reim) * 4 + lam) * nk] += A[lam];
+            t[(size_t)((1 * 2 + reim) * 4 + lam) * nk] += B[lam];
+            t[(size_t)((2 * 2 + reim) * 4 + lam) * nk] += Cc[lam];
+        }
+    }
+}
+
 // ------------------------------------------------------------------ plan
 struct vh_drift_plan {
     int2* jobs = nullptr;
@@ -201,8 +289,10 @@ struct vh_drift_plan {
     double* Fe = nullptr;
     double* G = nullptr;
     int njobs = 0, ts = 32, rw = 64;
-    size_t lds = 0;
+    size_t lds = 0, lds_t = 0;
 };
+
+const int2* vh_drift_plan_node_el(const vh_drift_plan* p) { return p->node_el; }
 
 vh_drift_plan* vh_drift_plan_create(hipStream_t st, const vh_dims& d, const vh_dev& v, int deg1_cm,
                                     const std::vector<int>& emap, const std::vector<int>& mnode_of) {
@@ -242,6 +332,17 @@ vh_drift_plan* vh_drift_plan_create(hipStream_t st, const vh_dims& d, const vh_d
         vh_setup_restricted_ops(st, d, v, 2, d.L, d.nmn, v.mnodes, p->G + (size_t)nrow * ncol);
         HIPCHECK(hipStreamSynchronize(st));
         HIPCHECK(hipFree(w1));
+    }
+    p->lds_t = (size_t)(std::max(ncol, 1) + 3) * rw * 8;
+    if (p->lds_t > 160 * 1024)
+        throw vh_error("drift adjoint: too many memory nodes for the LDS tile (nmn = " + std::to_string(d.nmn) + ")");
+    switch (p->rw) {
+        case 64: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_gT<64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
+        case 32: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_gT<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
+        case 16: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_gT<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
+        case 8: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_gT<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
+        case 4: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_gT<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
+        default: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_gT<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
     }
     switch (p->rw) {
         case 64: HIPCHECK(hipFuncSetAttribute((const void*)k_drift<64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds)); break;
@@ -285,5 +386,29 @@ void vh_drift_run(hipStream_t st, const vh_drift_plan* p, const vh_dims& d, cons
         case 4: k_drift<4><<<p->njobs, DRIFT_THREADS, p->lds, st>>>(a); break;
         default: k_drift<2><<<p->njobs, DRIFT_THREADS, p->lds, st>>>(a); break;
     }
+    HIPCHECK(hipGetLastError());
+}
+
+void vh_drift_adjoint(hipStream_t st, const vh_drift_plan* p, const vh_dims& d, const vh_dev& v,
+                      const double* mnorm, const double* maxnorm, double skip_tol, int deg1_cm,
+                      const double* dNh, const double* dsurfh, double* Fh, double* tauh) {
+    if (d.nem == 0) return;
+    drift_args a;
+    a.nk = d.nk; a.nem = d.nem; a.nmn = d.nmn; a.ts = p->ts; a.rw = p->rw;
+    a.jobs = p->jobs; a.G = p->G; a.r = v.r; a.emap = v.emap; a.mnode = v.mnode; a.kdeg = v.kdeg;
+    a.node_el = p->node_el; a.Fe = p->Fe;
+    a.tau = nullptr; a.mnorm = mnorm; a.maxnorm = maxnorm;
+    a.skip_tol = skip_tol; a.deg1_cm = deg1_cm; a.dN = nullptr; a.dsurf = nullptr;
+    switch (p->rw) {
+        case 64: k_drift_gT<64><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, Fh); break;
+        case 32: k_drift_gT<32><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, Fh); break;
+        case 16: k_drift_gT<16><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, Fh); break;
+        case 8: k_drift_gT<8><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, Fh); break;
+        case 4: k_drift_gT<4><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, Fh); break;
+        default: k_drift_gT<2><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, Fh); break;
+    }
+    HIPCHECK(hipGetLastError());
+    dim3 g((d.nk + 63) / 64, (d.nem + 3) / 4);
+    k_drift_force_adj<<<g, 256, 0, st>>>(a, Fh, tauh);
     HIPCHECK(hipGetLastError());
 }

@@ -20,7 +20,8 @@ void sle_cs(hipStream_t st, const vh_sht& s, const double* C, const double* f, d
 // and Cn are those of the last pass and sigma is the spectrum of the load of
 // the final rsl (the solve's closing analysis).
 // res (zeroed by the caller): [0] iterations of the last pass, [1] its last
-// dphi, [2] 1 if a grid barrier wait timed out, [4] iterations over all
+// dphi, [2] 1 if a grid barrier wait timed out, [3] C_int of the last pass,
+// [4] iterations over all
 // passes, [5] passes with ocean (C_int > 0), [6], [7] int d_ice_g and
 // int topo0 (C - C0) of the last such pass.
 // bar: [0] barrier counter (continues across launches), [1] timeout flag,

@@ -454,6 +454,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     }
     __syncthreads();
     const double C_int = stot[0];
+    if (blk == 0 && tid == 0) a.res[3] = C_int;
     if (C_int > 0.0) {
         // subgrid sloping-coast load (src/vilma_sle.f90:289-291), own rows
         if (is_row)

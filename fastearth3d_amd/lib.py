@@ -43,6 +43,11 @@ def load():
         L.vh_update_device.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
         L.vh_profile.argtypes = [C.c_void_p, C.c_int]
         L.vh_profile_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.vh_forward_record.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_double, C.c_int,
+                                        C.POINTER(C.c_double)]
+        L.vh_forward_record_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_int, C.c_void_p]
+        L.vh_backward.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 3
+        L.vh_backward_device.argtypes = [C.c_void_p] + [C.c_void_p] * 3
         for name in exported_symbols():
             fn = getattr(L, name)
             if fn.argtypes is None:   # every entry point takes a handle or pointer: never call undeclared
@@ -124,6 +129,31 @@ class SolidEarth(ModelHandle):
 
     def set_dt(self, dt_s):
         self._check(self._lib.vh_resp_set_dt(self._h, float(dt_s)), "response_set_dt")
+
+    def forward_record(self, ice_hist, dt_yr, ckpt_every=10):
+        """Run len(ice_hist) coupling calls keeping checkpoints; returns the
+        rsl after each call, (ncalls, nlat, nphi)."""
+        ice = np.ascontiguousarray(ice_hist, dtype=np.float64)
+        n = ice.shape[0]
+        if ice.size != n * self.nphi * self.nlat:
+            raise VilmaError("forward_record: ice history must be (ncalls, nlat, nphi)")
+        rsl = np.empty((n, self.nlat, self.nphi))
+        self._check(self._lib.vh_forward_record(self._h, n, _cp(ice), float(dt_yr), int(ckpt_every), _cp(rsl)),
+                    "forward_record")
+        self._ncalls = n
+        return rsl
+
+    def backward(self, dJ_drsl):
+        """Adjoint sweep of the last forward_record: returns (dJ_dlogeta
+        (ne, nlat, nphi), dJ_dice (ncalls+1, nlat, nphi))."""
+        g = np.ascontiguousarray(dJ_drsl, dtype=np.float64)
+        n = getattr(self, "_ncalls", 0)
+        if g.size != n * self.nphi * self.nlat:
+            raise VilmaError("backward: dJ_drsl must be (ncalls, nlat, nphi) of the last forward_record")
+        dp = np.empty((self.dims["ne"], self.nlat, self.nphi))
+        di = np.empty((n + 1, self.nlat, self.nphi))
+        self._check(self._lib.vh_backward(self._h, _cp(g), _cp(dp), _cp(di)), "backward")
+        return dp, di
 
 
 def solid_earth_init(params, z_bed_eq, h_ice_eq, h_ice_init=None):

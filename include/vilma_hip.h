@@ -193,6 +193,31 @@ int vh_sht_synthesis(vh_model* h, const double* lm, double* grid);
 int vh_tensor_synthesis(vh_model* h, const double* c, double* dyad);
 int vh_tensor_analysis(vh_model* h, const double* dyad, double* c);
 
+/* ---- Reverse mode (no counterpart in the reference, which has no adjoint).
+ * Gradients of a misfit J(rsl_1..rsl_n) of the rsl after each of n coupling
+ * calls with respect to the lateral log10-viscosity perturbation (pert_elem
+ * of vh_enable_visc_3d) and the ice history, through hand-written transposes
+ * of the step's kernels. vh_forward_record runs ncalls solid_earth_update
+ * calls of dt_yr with h_ice_hist[c] (ncalls x (nphi,nlat)), keeps the model
+ * state every ckpt_every calls and writes rsl after each call to rsl_out
+ * (may be NULL). vh_backward takes dJ_drsl (ncalls x (nphi,nlat)), re-runs
+ * each checkpoint segment keeping its sub-steps' inputs in HBM and sweeps
+ * them backwards; it returns dJ_dlogeta (ne x (nphi,nlat), per mesh element
+ * as pert_elem: grid values for genuinely 3-D elements, the element-uniform
+ * share elsewhere) and dJ_dice ((ncalls+1) x (nphi,nlat): the ice before
+ * the first call, then each call's ice). The model is left in the state after
+ * the recorded calls. The derivative is the one of the recorded branch: skip
+ * gate, coastline masks, sub-step sizes and the 3-D element set are held;
+ * needs earth_response = "ve" and rotation off. The _device variants take
+ * device pointers. */
+int vh_forward_record(vh_model* h, int ncalls, const double* h_ice_hist, double dt_yr, int ckpt_every,
+                      double* rsl_out);
+int vh_forward_record_device(vh_model* h, int ncalls, const double* h_ice_hist_device, double dt_yr,
+                             int ckpt_every, double* rsl_out_device);
+int vh_backward(vh_model* h, const double* dJ_drsl, double* dJ_dlogeta, double* dJ_dice);
+int vh_backward_device(vh_model* h, const double* dJ_drsl_device, double* dJ_dlogeta_device,
+                       double* dJ_dice_device);
+
 #ifdef __cplusplus
 }
 #endif

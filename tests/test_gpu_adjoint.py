@@ -1,0 +1,176 @@
+"""Reverse mode on the GPU (vh_forward_record / vh_backward, through the C-ABI).
+
+The adjoint gradient of an rsl misfit with respect to the ice history and the
+lateral log-viscosity field is checked against central finite differences of
+the forward run: of the GPU forward (h = 1e-4 for the viscosity field, as
+the adjoint workload states), and of the oracle's forward, the CPU
+restatement of the reference step (src/vilma_coupling.f90:336-437), so the
+gradient is anchored to the reference's forward map and not only to this
+library's. The derivative is that of the recorded branch (skip gate,
+coastline masks, sub-step counts held), so the cases keep that branch fixed:
+skip gate off (skip_tol = 0), rotation off, a tight sea-level tolerance, no
+perturbation of elements whose lateral spread sits near the 3-D threshold,
+and the accepted sub-step count and 3-D element set are checked to be
+unchanged under the perturbation. The scalar-transform transposition identities that the
+sea-level adjoint is built on are checked through the public transforms.
+"""
+import numpy as np
+import pytest
+
+from fastearth3d_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(lmax, ncalls, seed=0):
+    c = configs.config(2, lmax=lmax)
+    p = c["params"]
+    p.rotation = 0
+    p.skip_tol = 0.0
+    p.sle_tol = 1e-12
+    ice = np.stack([c["ice"](k + 60) for k in range(ncalls)])
+    rng = np.random.default_rng(seed)
+    shape = ice.shape
+    wgt = rng.uniform(0.5, 1.5, shape)
+    tgt = rng.standard_normal(shape)
+    return c, p, ice, wgt, tgt
+
+
+def _misfit(rsl, wgt, tgt):
+    return 0.5 * float(np.sum(wgt * (rsl - tgt) ** 2))
+
+
+def _run(make, c, p, pert, ice):
+    m = make(p, c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
+    m.enable_visc_3d(pert)
+    out = []
+    for k in range(ice.shape[0]):
+        m.update(ice[k], c["dt_yr"])
+        out.append(m.rsl.copy())
+    branch = (m.scalar(3), m.dims["ne3d"])   # accepted sub-steps, genuinely 3-D elements
+    m.finalize()
+    return np.stack(out), branch
+
+
+def _directions(c, p, ice, seed):
+    rng = np.random.default_rng(seed)
+    land = (c["z_bed_eq"] > 0.0) & (ice.min(axis=0) > 10.0)   # grounded ice on land: no flotation flips
+    vi = rng.standard_normal(ice.shape) * land
+    # elements whose lateral spread lies near the 3-D threshold would change
+    # class under the perturbation (a different branch): leave them out
+    pert = c["pert"]
+    spread = pert.max(axis=(1, 2)) - pert.min(axis=(1, 2))
+    keep = np.abs(spread - p.visc3d_tol) > 0.5 * p.visc3d_tol
+    vp = rng.uniform(-1.0, 1.0, pert.shape) * keep[:, None, None]
+    return vi, vp
+
+
+def _gradients(c, p, ice, wgt, tgt, every):
+    from fastearth3d_amd import lib
+    m = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
+    m.enable_visc_3d(c["pert"])
+    rsl = m.forward_record(ice, c["dt_yr"], ckpt_every=every)
+    n_acc = (m.scalar(3), m.dims["ne3d"])
+    gp, gi = m.backward(wgt * (rsl - tgt))
+    rsl_after = m.rsl.copy()
+    m.finalize()
+    return rsl, n_acc, gp, gi, rsl_after
+
+
+def test_forward_record_matches_updates():
+    """The recorded forward is the plain forward, and backward leaves the model at its end state."""
+    from fastearth3d_amd import lib
+    c, p, ice, wgt, tgt = _case(16, 4)
+    rsl, n_acc, gp, gi, rsl_after = _gradients(c, p, ice, wgt, tgt, 2)
+    ref, n_ref = _run(lib.SolidEarth, c, p, c["pert"], ice)
+    assert np.array_equal(rsl, ref)
+    assert n_acc == n_ref
+    assert np.array_equal(rsl_after, ref[-1])
+    assert np.all(np.isfinite(gp)) and np.all(np.isfinite(gi))
+
+
+def test_checkpoint_interval_invariance():
+    """Segments re-run from checkpoints reproduce the same sweep: the gradient does not depend on the interval."""
+    c, p, ice, wgt, tgt = _case(16, 5)
+    a = _gradients(c, p, ice, wgt, tgt, 1)
+    b = _gradients(c, p, ice, wgt, tgt, 3)
+    d = _gradients(c, p, ice, wgt, tgt, 5)
+    for x in (b, d):
+        assert np.array_equal(x[2], a[2])
+        assert np.array_equal(x[3], a[3])
+
+
+def _fd_check(make, c, p, ice, wgt, tgt, gp, gi, n_acc, vi, vp, h_ice, h_pert, tol):
+    # ice history (the calls' ice; gi[0] is the ice before the first call)
+    jp, na = _run(make, c, p, c["pert"], ice + h_ice * vi)
+    jm, nb = _run(make, c, p, c["pert"], ice - h_ice * vi)
+    assert na == n_acc and nb == n_acc
+    fd = (_misfit(jp, wgt, tgt) - _misfit(jm, wgt, tgt)) / (2.0 * h_ice)
+    ad = float(np.sum(gi[1:] * vi))
+    assert abs(ad - fd) <= tol * abs(fd), (ad, fd)
+    # lateral log-viscosity field
+    jp, na = _run(make, c, p, c["pert"] + h_pert * vp, ice)
+    jm, nb = _run(make, c, p, c["pert"] - h_pert * vp, ice)
+    assert na == n_acc and nb == n_acc
+    fd = (_misfit(jp, wgt, tgt) - _misfit(jm, wgt, tgt)) / (2.0 * h_pert)
+    ad = float(np.sum(gp * vp))
+    assert abs(ad - fd) <= tol * abs(fd), (ad, fd)
+
+
+def test_gradient_vs_gpu_finite_differences():
+    from fastearth3d_amd import lib
+    c, p, ice, wgt, tgt = _case(16, 4)
+    rsl, n_acc, gp, gi, _ = _gradients(c, p, ice, wgt, tgt, 2)
+    vi, vp = _directions(c, p, ice, 5)
+    _fd_check(lib.SolidEarth, c, p, ice, wgt, tgt, gp, gi, n_acc, vi, vp, 0.1, 1e-4, 1e-6)
+
+
+def test_gradient_vs_oracle_finite_differences(oracle):
+    """Directional derivatives of the oracle's forward (the reference step restated) against the GPU adjoint."""
+    c, p, ice, wgt, tgt = _case(12, 3)
+    rsl, n_acc, gp, gi, _ = _gradients(c, p, ice, wgt, tgt, 2)
+    ref, n_ref = _run(oracle.OracleModel, c, p, c["pert"], ice)
+    assert n_ref == n_acc
+    assert np.abs(rsl - ref).max() <= 1e-8 * np.abs(ref).max()
+    vi, vp = _directions(c, p, ice, 6)
+    _fd_check(oracle.OracleModel, c, p, ice, wgt, tgt, gp, gi, n_acc, vi, vp, 0.1, 1e-4, 1e-6)
+
+
+def test_sht_transposes():
+    """Anl^T q = W Syn(q / c_m) and Syn^T f = c_m Anl(f / W) (c_0 = 1, c_m = 2; W = w_j 2 pi/nphi):
+    the identities the sea-level adjoint uses, on the stored real and imaginary parts."""
+    from fastearth3d_amd import lib
+    c = configs.config(1, lmax=24)
+    p = c["params"]
+    p.rotation = 0
+    m = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"])
+    L, nlat, nphi = 24, m.nlat, m.nphi
+    x, w = np.polynomial.legendre.leggauss(nlat)
+    W = np.repeat((w[::-1] * 2.0 * np.pi / nphi)[:, None], nphi, axis=1)
+    cm = np.array([1.0 if mm == 0 else 2.0 for mm in range(L + 1) for _ in range(mm, L + 1)])
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal((nlat, nphi))
+    q = rng.standard_normal(m.dims["nlm"]) + 1j * rng.standard_normal(m.dims["nlm"])
+    q[cm == 1.0] = q[cm == 1.0].real
+    dot = lambda a, b: float(np.sum(a.real * b.real + a.imag * b.imag))
+    lhs = dot(m.sht_analysis(f), q)
+    rhs = float(np.sum(f * W * m.sht_synthesis(q / cm)))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    lhs = float(np.sum(m.sht_synthesis(q) * f))
+    rhs = dot(q, cm * m.sht_analysis(f / W))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    m.finalize()
+
+
+def test_adjoint_rejects_rotation():
+    from fastearth3d_amd import lib
+    from fastearth3d_amd.model import VilmaError
+    c = configs.config(1, lmax=12)
+    p = c["params"]
+    p.rotation = 1
+    m = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"])
+    with pytest.raises(VilmaError, match="rotation"):
+        m.forward_record(np.stack([c["ice"](60)]), c["dt_yr"])
+    with pytest.raises(VilmaError, match="no forward record"):
+        m.backward(np.zeros((0, m.nlat, m.nphi)))
+    m.finalize()
