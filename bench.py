@@ -28,6 +28,10 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 NAMES = {0: "k_drift", 1: "k_advance", 2: "k_tsh_synth", 3: "k_t3_rows", 4: "k_tsh_analysis", 5: "k_t3_gather",
          6: "k_sle_solve"}
+ADJ_NAMES = {2: "k_tsh_synth (forward + adjoint rows)", 4: "k_tsh_analysis", 5: "k_t3_gather + k_t3_pack_adj",
+             7: "k_t3_rows_adj", 8: "k_adv_adj + k_adv_nodes", 9: "sea-level transpose (scalar SHT kernels)",
+             10: "k_drift_gT + k_drift_force_adj"}
+NCLASS = 12
 
 
 def alg_bytes(dims, cls, chunk_batches=None, sle_iters=None):
@@ -150,6 +154,251 @@ def ensemble_bench(a, L):
     print(json.dumps(out))
 
 
+def site_matrix(L, colat, lon):
+    """E[s, i] (i in SHTns LM order) with f(site s) = Re(E @ a) for a real
+    field of spectrum a: c_m P_lm(cos colat_s) e^{i m lon_s}, orthonormal
+    P_lm without the Condon-Shortley phase, c_0 = 1, c_m = 2
+    (the spectral point evaluation of sht_grid_eval_point, src/vilma_sht.f90:310-325)."""
+    x, sn = np.cos(colat), np.sin(colat)
+    cols = []
+    pmm = np.full_like(x, 1.0 / np.sqrt(4.0 * np.pi))
+    for m in range(L + 1):
+        if m > 0:
+            pmm = pmm * np.sqrt((2.0 * m + 1.0) / (2.0 * m)) * sn
+        cm = 1.0 if m == 0 else 2.0
+        e = cm * np.exp(1j * m * lon)
+        pm2, pm1 = None, None
+        for l in range(m, L + 1):
+            if l == m:
+                p = pmm
+            elif l == m + 1:
+                p = np.sqrt(2.0 * m + 3.0) * x * pmm
+            else:
+                a_ = np.sqrt((4.0 * l * l - 1.0) / (l * l - m * m))
+                b_ = np.sqrt(((l - 1.0) ** 2 - m * m) / (4.0 * (l - 1.0) ** 2 - 1.0))
+                p = a_ * (x * pm1 - b_ * pm2)
+            cols.append(p * e)
+            pm2, pm1 = pm1, p
+    return np.stack(cols, axis=1)
+
+
+def adjoint_bench(a, L):
+    """Configuration 3: reverse mode over 100 coupling calls of the lmax 128
+    3-D workload (rotation off: the adjoint holds no rotation feedback).
+    Misfit J = sum (rsl - target)^2 at 500 uniform sites x 10 evenly spaced
+    output calls, target from a forward run with another seeded delta field;
+    gradients w.r.t. the lateral log-viscosity field (and delta's SH
+    coefficients l <= 32 per element, by the transposed synthesis) and the
+    ice history. value = wall(forward_record + backward) / wall(plain
+    forward) over the same calls from the same state (vh_forward_rewind);
+    memory checkpointed every 10 calls. Then central differences (h = 1e-4)
+    along --fd-dirs seeded delta directions (l <= 32, elements away from the
+    3-D threshold) and one ice direction, on fresh models."""
+    import torch
+    from fastearth3d_amd import configs, lib
+    L.vh_forward_record_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_int, C.c_void_p]
+    L.vh_backward_device.argtypes = [C.c_void_p] + [C.c_void_p] * 3
+    L.vh_forward_rewind.argtypes = [C.c_void_p]
+    ncalls, every, nsite, nout = a.adj_calls, 10, 500, 10
+    c = configs.config(2, lmax=a.lmax)
+    p = c["params"]
+    p.rotation = 0
+    nphi, nlat = p.grid()
+    ng = nphi * nlat
+    colat, lon = configs.gauss_grid(nlat, nphi)
+    ice_np = np.stack([c["ice"](k) for k in range(ncalls)])
+    ice = torch.from_numpy(ice_np).to("cuda:0")
+    outs = [ncalls * (i + 1) // nout - 1 for i in range(nout)]
+    rng = np.random.default_rng(71)
+    zs = rng.uniform(-1.0, 1.0, nsite)
+    site_colat, site_lon = np.arccos(zs), rng.uniform(0.0, 2.0 * np.pi, nsite)
+    E = site_matrix(a.lmax, site_colat, site_lon)
+    x_, w_ = np.polynomial.legendre.leggauss(nlat)
+    W = np.repeat((w_[::-1] * 2.0 * np.pi / nphi)[:, None], nphi, axis=1)
+    cm = np.array([1.0 if m == 0 else 2.0 for m in range(a.lmax + 1) for _ in range(m, a.lmax + 1)])
+
+    def check(se, rc):
+        if rc:
+            raise RuntimeError(se._err())
+
+    def site_values(se, rsl_grid):
+        return (E @ se.sht_analysis(rsl_grid)).real
+
+    # target: another seeded delta field, same ice
+    tgt_pert = configs.delta_visc(p, colat, lon, 1037)
+    st = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
+    st.enable_visc_3d(tgt_pert)
+    target = {}
+    for k in range(ncalls):
+        check(st, L.vh_update_device(st._h, C.c_void_p(ice[k].data_ptr()), c["dt_yr"]))
+        if k in outs:
+            target[k] = site_values(st, st.rsl)
+    st.finalize()
+
+    se = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
+    se.enable_visc_3d(c["pert"])
+    rsl_d = torch.empty((ncalls, nlat, nphi), dtype=torch.float64, device="cuda:0")
+    dJ_d = torch.zeros_like(rsl_d)
+    gp_d = torch.empty((se.dims["ne"], nlat, nphi), dtype=torch.float64, device="cuda:0")
+    gi_d = torch.empty((ncalls + 1, nlat, nphi), dtype=torch.float64, device="cuda:0")
+
+    def misfit_grad(rsl_out):
+        J = 0.0
+        dJ_d.zero_()
+        host = rsl_out[outs].cpu().numpy()
+        for i, k in enumerate(outs):
+            r = site_values(se, host[i]) - target[k]
+            J += float(np.sum(r * r))
+            ah = (2.0 * r) @ np.conj(E)                 # dJ/da (stored re/im parts)
+            g = W * se.sht_synthesis(ah / cm)          # Anl^T: dJ/drsl on the grid
+            dJ_d[k] = torch.from_numpy(g).to("cuda:0")
+        return J
+
+    def record():
+        check(se, L.vh_forward_record_device(se._h, ncalls, C.c_void_p(ice.data_ptr()), c["dt_yr"], every,
+                                             C.c_void_p(rsl_d.data_ptr())))
+
+    def backward():
+        check(se, L.vh_backward_device(se._h, C.c_void_p(dJ_d.data_ptr()), C.c_void_p(gp_d.data_ptr()),
+                                       C.c_void_p(gi_d.data_ptr())))
+
+    # warm-up cycle (allocates the checkpoints and the sub-step store), then rewind
+    record()
+    misfit_grad(rsl_d)
+    backward()
+    check(se, L.vh_forward_rewind(se._h))
+    # plain forward over the window
+    torch.cuda.synchronize()
+    n0 = se.scalar(3)
+    t0 = time.perf_counter()
+    for k in range(ncalls):
+        check(se, L.vh_update_device(se._h, C.c_void_p(ice[k].data_ptr()), c["dt_yr"]))
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    nsub = se.scalar(3) - n0
+    check(se, L.vh_forward_rewind(se._h))
+    # forward with checkpoints + backward with recompute (misfit gradient timed apart)
+    L.vh_profile(se._h, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    record()
+    torch.cuda.synchronize()
+    t_rec = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J = misfit_grad(rsl_d)
+    torch.cuda.synchronize()
+    t_mis = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    backward()
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    ms = (C.c_double * NCLASS)()
+    cnt = (C.c_double * NCLASS)()
+    L.vh_profile_read(se._h, ms, cnt)
+    gp = gp_d.cpu().numpy()
+    n3 = se.dims["ne3d"]
+    # gradient w.r.t. delta's SH coefficients (l <= 32) per element: Syn^T = c_m Anl(. / W)
+    gdelta_norm = 0.0
+    lm_l = np.array([l for m in range(a.lmax + 1) for l in range(m, a.lmax + 1)])
+    for e in range(gp.shape[0]):
+        q = cm * se.sht_analysis(gp[e] / W)
+        gdelta_norm += float(np.sum(np.abs(q[lm_l <= 32]) ** 2))
+    se.finalize()
+
+    # Gradient check: the derivative is the one of the recorded branch, and a
+    # perturbed run of 500 sub-steps flips coastline cells and skip-gate slots
+    # (the misfit is not differentiable there), so the check runs the same
+    # window with the coastline held (sle_fixed_ocean), the gate off
+    # (skip_tol = 0) and the sea-level fixed point converged tightly
+    # (sle_tol 1e-11: the adjoint differentiates the converged solve, not the
+    # warm start of a truncated one), adjoint and central differences
+    # (h = 1e-4) on fresh models from the initial state.
+    p.sle_fixed_ocean = 1
+    p.skip_tol = 0.0
+    p.sle_tol = 1e-11
+    if a.fd_dirs > 0:
+        se = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
+        se.enable_visc_3d(c["pert"])
+        n0 = se.scalar(3)
+        record()
+        misfit_grad(rsl_d)
+        branch = (se.scalar(3) - n0, se.dims["ne3d"])
+        backward()
+        gp = gp_d.cpu().numpy()
+        gi = gi_d.cpu().numpy()
+        se.finalize()
+
+    def J_of(pert, icearr):
+        m = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
+        m.enable_visc_3d(pert)
+        iced = torch.from_numpy(icearr).to("cuda:0")
+        n0_ = m.scalar(3)
+        Jv = 0.0
+        for k in range(ncalls):
+            check(m, L.vh_update_device(m._h, C.c_void_p(iced[k].data_ptr()), c["dt_yr"]))
+            if k in outs:
+                r = site_values(m, m.rsl) - target[k]
+                Jv += float(np.sum(r * r))
+        br = (m.scalar(3) - n0_, m.dims["ne3d"])
+        m.finalize()
+        return Jv, br
+
+    h = 1e-4
+    pert = c["pert"]
+    spread = pert.max(axis=(1, 2)) - pert.min(axis=(1, 2))
+    keep = np.abs(spread - p.visc3d_tol) > 0.5 * p.visc3d_tol
+    helper = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
+    frng = np.random.default_rng(a.fd_seed)
+    checks = []
+    for dcount in range(a.fd_dirs):
+        v = np.zeros_like(pert)
+        for e in range(pert.shape[0]):
+            if not keep[e]:
+                continue
+            q = frng.standard_normal(len(lm_l)) + 1j * frng.standard_normal(len(lm_l))
+            q[lm_l > 32] = 0.0
+            q[cm == 1.0] = q[cm == 1.0].real
+            v[e] = helper.sht_synthesis(q)
+        v /= np.abs(v).max()
+        jp, bp = J_of(pert + h * v, ice_np)
+        jm, bm = J_of(pert - h * v, ice_np)
+        fd = (jp - jm) / (2.0 * h)
+        ad = float(np.sum(gp * v))
+        checks.append({"kind": "delta_lm (l<=32)", "adjoint": ad, "fd": fd,
+                       "rel_err": abs(ad - fd) / max(abs(fd), 1e-300), "branch_kept": bp == branch and bm == branch})
+    if a.fd_dirs > 0:
+        land = (c["z_bed_eq"] > 0.0) & (ice_np.max(axis=0) > 10.0)
+        vi = frng.standard_normal(ice_np.shape) * land
+        hi = 0.1
+        jp, bp = J_of(pert, ice_np + hi * vi)
+        jm, bm = J_of(pert, ice_np - hi * vi)
+        fd = (jp - jm) / (2.0 * hi)
+        ad = float(np.sum(gi[1:] * vi))
+        checks.append({"kind": "ice history", "adjoint": ad, "fd": fd,
+                       "rel_err": abs(ad - fd) / max(abs(fd), 1e-300), "branch_kept": bp == branch and bm == branch})
+    helper.finalize()
+
+    kernel_ms = {ADJ_NAMES.get(k, NAMES.get(k)): round(ms[k], 3) for k in range(NCLASS) if cnt[k] > 0}
+    ratio = (t_rec + t_bwd) / t_fwd
+    out = {
+        "metric": "adjoint/forward time ratio (lmax 128 3-D, %d steps, 1 GPU)" % ncalls, "value": round(ratio, 4),
+        "unit": "x", "n_gpus": 1, "steps": ncalls, "warmup": 1,
+        "ms_per_step": round((t_rec + t_bwd) / ncalls * 1000.0, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "c3 adjoint of the lmax %d 3-D + SLE workload (rotation off), %d calls of %g yr, "
+                               "checkpoint every %d calls, %d sites x %d output calls"
+                               % (a.lmax, ncalls, c["dt_yr"], every, nsite, nout),
+                   "lmax": a.lmax, "nlat": nlat, "nphi": nphi, "ne3d": int(n3), "sub_steps": int(nsub)},
+        "forward_s": round(t_fwd, 4), "record_s": round(t_rec, 4), "backward_s": round(t_bwd, 4),
+        "misfit_gradient_s": round(t_mis, 4), "ratio_incl_misfit": round((t_rec + t_bwd + t_mis) / t_fwd, 4),
+        "J": J, "grad_delta_lm_norm": gdelta_norm ** 0.5,
+        "grad_check": {"h": h, "branch": "coastline held (sle_fixed_ocean), skip gate off, sle_tol 1e-11", "directions": checks,
+                       "max_rel_err": max((x["rel_err"] for x in checks), default=None)},
+        "kernel_ms_reverse_run": kernel_ms, "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +409,9 @@ def main():
     ap.add_argument("--cpu-calls", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--members", type=int, default=64, help="ensemble size (--config 4)")
+    ap.add_argument("--adj-calls", type=int, default=100, help="coupling calls of the adjoint window (--config 3)")
+    ap.add_argument("--fd-dirs", type=int, default=2, help="finite-difference check directions (--config 3)")
+    ap.add_argument("--fd-seed", type=int, default=5)
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 times): the reference "
                          "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -185,6 +437,9 @@ def main():
     L.vh_profile_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     if a.config == 4:
         ensemble_bench(a, L)
+        return
+    if a.config == 3:
+        adjoint_bench(a, L)
         return
 
     c = configs.config(a.config, lmax=a.lmax)
@@ -212,8 +467,8 @@ def main():
         step(k)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    ms = (C.c_double * 8)()
-    cnt = (C.c_double * 8)()
+    ms = (C.c_double * NCLASS)()
+    cnt = (C.c_double * NCLASS)()
     L.vh_profile_read(se._h, ms, cnt)
     nsub = se.scalar(3) - n_acc0
     n_inner = se.scalar(6) - n_in0

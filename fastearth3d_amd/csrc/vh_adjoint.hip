@@ -200,10 +200,12 @@ __device__ __forceinline__ int xcd_block_a(int nblk) {
 
 // Block = (3-D element i3, latitude row j); three wavefronts, one per dyad
 // pair (rr, rt), (rp, tt), (tp, pp). For each of the element's three shapes:
-// the forward rows Ff and the adjoint rows Fa to real space, the product
-// sum_dy Omega_dy g_f g_a accumulated per longitude, then Fa <- FFT(-rate dt
-// g_a) in place (the forward row pass, mode 0). At the end
-// rate_h[i3][j][i] += -(2 pi/nphi) w_j dt * the accumulated product.
+// the forward rows Ff to real space (kept in registers), the adjoint rows Fa
+// to real space, the product sum_dy Omega_dy g_f g_a accumulated per
+// longitude (registers), then Fa <- FFT(-rate dt g_a) in place (the forward
+// row pass, mode 0). At the end rate_h[i3][j][i] += -(2 pi/nphi) w_j dt *
+// the accumulated product summed over the three wavefronts.
+#define RA_U 8   // longitudes per lane (nphi <= 512)
 __global__ void __launch_bounds__(192) k_t3_rows_adj(vh_sht s, double2* __restrict__ Fa, const double2* __restrict__ Ff,
                                                      int ne3, const double* __restrict__ rate, double dt,
                                                      double* __restrict__ rate_h) {
@@ -213,23 +215,25 @@ __global__ void __launch_bounds__(192) k_t3_rows_adj(vh_sht s, double2* __restri
     if (lb >= ne3 * nlat) return;
     const int i3 = lb / nlat, j = lb % nlat;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double2* a = lds + (size_t)wave * 3 * N;
+    double2* a = lds + (size_t)wave * 2 * N;
     double2* c = a + N;
-    double2* keep = c + N;
-    double* Mrow = (double*)(lds + (size_t)9 * N);
-    double* acc = Mrow + N + (size_t)wave * N;
+    double* Mrow = (double*)(lds + (size_t)6 * N);
     const double* rr = rate + ((size_t)i3 * nlat + j) * N;
     for (int i = threadIdx.x; i < N; i += blockDim.x) Mrow[i] = rr[i] * dt;
-    for (int i = lane; i < N; i += 64) acc[i] = 0.0;
+    double acc[RA_U];
+#pragma unroll
+    for (int u = 0; u < RA_U; u++) acc[u] = 0.0;
     const double omA = wave == 2 ? 0.5 : 1.0;   // dyad weights: tp (first of wave 2) counts 1/2, the rest 1
     const int d0 = 2 * wave, d1 = 2 * wave + 1;
+    __syncthreads();   // Mrow complete
     for (int sh = 0; sh < 3; sh++) {
         const int b = 3 * i3 + sh;
-        // ---- forward rows -> keep, adjoint rows -> z
+        double2 gf[RA_U];
         double2* z = nullptr;
         for (int pass = 0; pass < 2; pass++) {
-            const double2* F0 = (pass == 0 ? Ff : (const double2*)Fa) + ((size_t)b * 6 + d0) * (L + 1) * nlat + j;
-            const double2* F1 = (pass == 0 ? Ff : (const double2*)Fa) + ((size_t)b * 6 + d1) * (L + 1) * nlat + j;
+            const double2* Fs = pass == 0 ? Ff : (const double2*)Fa;
+            const double2* F0 = Fs + ((size_t)b * 6 + d0) * (L + 1) * nlat + j;
+            const double2* F1 = Fs + ((size_t)b * 6 + d1) * (L + 1) * nlat + j;
             for (int i = lane; i < N; i += 64) a[i] = make_double2(0.0, 0.0);
             wave_lds_sync();
             for (int m0 = 0; m0 <= L; m0 += 64 * VH_FFT_U) {
@@ -252,16 +256,23 @@ __global__ void __launch_bounds__(192) k_t3_rows_adj(vh_sht s, double2* __restri
             wave_lds_sync();
             z = fft_wave(a, c, s, true, lane);
             if (pass == 0) {
-                for (int i = lane; i < N; i += 64) keep[i] = z[i];
+#pragma unroll
+                for (int u = 0; u < RA_U; u++) {
+                    const int i = lane + 64 * u;
+                    gf[u] = i < N ? z[i] : make_double2(0.0, 0.0);
+                }
                 wave_lds_sync();
             }
         }
-        __syncthreads();   // Mrow is written by the whole block (first shape)
-        for (int i = lane; i < N; i += 64) {
-            const double2 g = z[i], f = keep[i];
-            acc[i] = fma(omA * g.x, f.x, fma(g.y, f.y, acc[i]));
-            const double M = Mrow[i];
-            z[i] = make_double2(-M * g.x, -M * g.y);
+#pragma unroll
+        for (int u = 0; u < RA_U; u++) {
+            const int i = lane + 64 * u;
+            if (i < N) {
+                const double2 g = z[i];
+                acc[u] = fma(omA * g.x, gf[u].x, fma(g.y, gf[u].y, acc[u]));
+                const double M = Mrow[i];
+                z[i] = make_double2(-M * g.x, -M * g.y);
+            }
         }
         wave_lds_sync();
         double2* other = (z == a) ? c : a;
@@ -280,12 +291,21 @@ __global__ void __launch_bounds__(192) k_t3_rows_adj(vh_sht s, double2* __restri
         }
         wave_lds_sync();
     }
+    // cross-wavefront sum through the (now free) FFT buffers
     __syncthreads();
-    const double* acc0 = Mrow + N;
+    double* red = (double*)a;
+#pragma unroll
+    for (int u = 0; u < RA_U; u++) {
+        const int i = lane + 64 * u;
+        if (i < N) red[i] = acc[u];
+    }
+    __syncthreads();
     const double f = -(2.0 * VH_PI / N) * s.w[j] * dt;
+    const double* r0 = (const double*)lds;
+    const double* r1 = (const double*)(lds + (size_t)2 * N);
+    const double* r2 = (const double*)(lds + (size_t)4 * N);
     double* out = rate_h + ((size_t)i3 * nlat + j) * N;
-    for (int i = threadIdx.x; i < N; i += blockDim.x)
-        out[i] += f * (acc0[i] + acc0[N + i] + acc0[2 * (size_t)N + i]);
+    for (int i = threadIdx.x; i < N; i += blockDim.x) out[i] += f * (r0[i] + r1[i] + r2[i]);
 }
 
 void vh_t3_pack_adj(hipStream_t st, const vh_dims& d, const vh_dev& v, const int* e3em, int ne3d, const double* n6,
@@ -299,8 +319,8 @@ void vh_t3_pack_adj(hipStream_t st, const vh_dims& d, const vh_dev& v, const int
 void vh_t3_rows_adj(hipStream_t st, const vh_sht& s, double2* Fa, const double2* Ff, int ne3, const double* rate,
                     double dt, double* rate_h) {
     if (ne3 <= 0) return;
-    const size_t lds = (size_t)9 * s.nphi * sizeof(double2) + (size_t)4 * s.nphi * sizeof(double);
-    if (lds > 160 * 1024) throw vh_error("3-D adjoint: nphi too large for the row pass");
+    const size_t lds = (size_t)6 * s.nphi * sizeof(double2) + (size_t)s.nphi * sizeof(double);
+    if (s.nphi > 64 * RA_U) throw vh_error("3-D adjoint: nphi above 512 is not supported by the row pass");
     const unsigned nblk = (unsigned)(((ne3 * s.nlat + 7) / 8) * 8);
     k_t3_rows_adj<<<nblk, 192, lds, st>>>(s, Fa, Ff, ne3, rate, dt, rate_h);
     HIPCHECK(hipGetLastError());

@@ -156,6 +156,7 @@ void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
 struct adj_ctx;
 void adj_free(adj_ctx* a);
 bool adj_recording(const adj_ctx* a);
+double2* adj_fd_pending(vh_model* h);   // store for the sub-step's tensor row spectra while recording
 
 struct vh_model {
     vh_params par{};
@@ -220,8 +221,8 @@ struct vh_model {
     struct evpair { hipEvent_t a, b; int cls; };
     std::vector<evpair> evpool;
     size_t evused = 0;
-    double prof_ms[8] = {0};
-    long prof_n[8] = {0};
+    double prof_ms[VH_K_NCLASS] = {0};
+    long prof_n[VH_K_NCLASS] = {0};
     void prof_begin(int cls) {
         if (!prof) return;
         if (evused == evpool.size()) {
@@ -599,6 +600,11 @@ void advance_memory(vh_model* h, const double* sigma, int nxt) {
             h->prof_begin(VH_K_TSH_SYNTH);
             vh_tsh_synth(st, h->s, h->d_X + (size_t)b0 * 4 * d.nlm, nb, h->d_Fd);
             h->prof_end();
+            if (h->adj && adj_recording(h->adj)) {   // the reverse sweep's rate gradient reads them
+                const size_t per = (size_t)6 * (d.L + 1) * d.nlat;
+                HIPCHECK(hipMemcpyAsync(adj_fd_pending(h) + (size_t)b0 * per, h->d_Fd, nb * per * sizeof(double2),
+                                        hipMemcpyDeviceToDevice, st));
+            }
             h->prof_begin(VH_K_T3_ROWS);
             vh_t3_rows(st, h->s, h->d_Fd, nb, 0, h->d_rate3 + (size_t)(b0 / 3) * ng, 3, h->dt, nullptr);
             h->prof_end();
@@ -779,6 +785,8 @@ void update(vh_model* h, const double* h_ice, double dt_yr, bool on_device) {
 // converged rsl does not depend on it).
 struct adj_sub {   // one accepted FE sub-step: tau_n, closing sigma, drift, last mask, norms
     double *tau = 0, *sigma = 0, *dN = 0, *C = 0, *mnorm = 0;
+    double2* Fd = 0;   // 3-D: the forward tensor row spectra of every batch
+    size_t fd_n = 0;
     double dt = 0, frac = 0, c_int = 0;
     int call = 0, n_last = 0;
 };
@@ -795,7 +803,10 @@ struct adj_ctx {
     std::vector<adj_sub> sub;
     size_t nsub = 0;
     bool recording = false, recorded = false;
+    bool tail_ok = false;   // the store holds the last segment (kept by the record pass)
     int call = 0;
+    double2* Fd_pending = nullptr;
+    size_t fd_n = 0;
     // reverse sweep
     bool work = false;
     size_t rh_n = 0, ih_n = 0;
@@ -811,12 +822,28 @@ struct adj_ctx {
 
 bool adj_recording(const adj_ctx* a) { return a->recording; }
 
+static size_t adj_nfd(const vh_model* h) { return (size_t)h->nbat * 6 * (h->d.L + 1) * h->d.nlat; }
+
+double2* adj_fd_pending(vh_model* h) {
+    adj_ctx* A = h->adj;
+    if (!A->Fd_pending || A->fd_n != adj_nfd(h)) {
+        if (A->Fd_pending) HIPCHECK(hipFree(A->Fd_pending));
+        A->Fd_pending = nullptr;
+        A->Fd_pending = dalloc<double2>(adj_nfd(h));
+        A->fd_n = adj_nfd(h);
+    }
+    return A->Fd_pending;
+}
+
 void adj_free(adj_ctx* a) {
     std::vector<void*> p = {a->ice, a->tauh[0], a->tauh[1], a->Ue, a->mpart, a->Kh, a->dNh, a->dsurfh, a->sigh,
                             a->Fh, a->rate_h, a->iceh, a->rh, a->pgrad, a->Xa, a->Fa, a->gx, a->gb, a->gt, a->gy,
-                            a->part, a->sc, a->F, a->q, a->q1, a->e3of_mesh, a->em_of};
+                            a->part, a->sc, a->F, a->q, a->q1, a->e3of_mesh, a->em_of, a->Fd_pending};
     for (auto& c : a->ck) { p.push_back(c.tau); p.push_back(c.mnorm); p.push_back(c.rsl); p.push_back(c.hice); }
-    for (auto& u : a->sub) { p.push_back(u.tau); p.push_back(u.sigma); p.push_back(u.dN); p.push_back(u.C); p.push_back(u.mnorm); }
+    for (auto& u : a->sub) {
+        p.push_back(u.tau); p.push_back(u.sigma); p.push_back(u.dN); p.push_back(u.C); p.push_back(u.mnorm);
+        p.push_back(u.Fd);
+    }
     for (void* x : p)
         if (x) (void)hipFree(x);
     delete a;
@@ -870,6 +897,10 @@ void adj_stash_push(vh_model* h, double dt, double frac, const sle_res& r) {
     d2d(u.mnorm, h->d_mnorm[h->cur], d.nk, h->st);
     d2d(u.mnorm + d.nk, h->d_maxn + h->cur, 1, h->st);
     u.dt = dt; u.frac = frac; u.c_int = r.c_int; u.n_last = (int)r.n_last; u.call = A->call;
+    if (h->lat_visc && h->ne3d > 0) {   // the sub-step's row spectra: swap the pending store in
+        std::swap(u.Fd, A->Fd_pending);
+        std::swap(u.fd_n, A->fd_n);
+    }
 }
 
 void adj_check(const vh_model* h) {
@@ -895,16 +926,26 @@ void forward_record(vh_model* h, int ncalls, const double* ice_hist, bool dev, d
                             dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->st));
     const int nseg = (ncalls + A->every - 1) / A->every;
     if ((int)A->ck.size() < nseg + 1) A->ck.resize(nseg + 1);
+    A->tail_ok = false;
     for (int c = 0; c < ncalls; c++) {
         if (c % A->every == 0) adj_save(h, A->ck[c / A->every]);
-        update(h, A->ice + (size_t)c * ng, dt_yr, true);
+        if (c == (nseg - 1) * A->every) { A->nsub = 0; A->recording = true; }   // the last segment is reversed first
+        A->call = c;
+        try {
+            update(h, A->ice + (size_t)c * ng, dt_yr, true);
+        } catch (...) {
+            A->recording = false;
+            throw;
+        }
         if (rsl_out)
             HIPCHECK(hipMemcpyAsync(rsl_out + (size_t)c * ng, h->d_rsl, ng * sizeof(double),
                                     rsl_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, h->st));
     }
+    A->recording = false;
     adj_save(h, A->ck[nseg]);
     h->sync();
     A->recorded = true;
+    A->tail_ok = true;
 }
 
 void adj_alloc_work(vh_model* h) {
@@ -1023,25 +1064,38 @@ void reverse_sub(vh_model* h, const adj_sub& S, const double* rh) {
     double* th_in = A->tauh[A->ah];
     double* th_out = A->tauh[A->ah ^ 1];
     if (h->lat_visc && h->ne3d > 0) {
-        vh_t3_gather(st, d, h->v, h->d_e3em, h->ne3d, S.tau, S.sigma, S.dN, h->d_X);
+        const size_t per = (size_t)6 * (d.L + 1) * d.nlat;
+        h->prof_begin(VH_K_T3_GATHER);
         vh_t3_pack_adj(st, d, h->v, h->d_e3em, h->ne3d, h->d_n6, th_in, A->Xa);
+        h->prof_end();
         for (int b0 = 0; b0 < h->nbat; b0 += h->tchunk) {
             const int nb = std::min(h->tchunk, h->nbat - b0);
-            vh_tsh_synth(st, h->s, h->d_X + (size_t)b0 * 4 * d.nlm, nb, h->d_Fd);
+            h->prof_begin(VH_K_TSH_SYNTH);
             vh_tsh_synth(st, h->s, A->Xa + (size_t)b0 * 4 * d.nlm, nb, A->Fa);
-            vh_t3_rows_adj(st, h->s, A->Fa, h->d_Fd, nb / 3, h->d_rate3 + (size_t)(b0 / 3) * ng, S.dt,
-                           A->rate_h + (size_t)(b0 / 3) * ng);
+            h->prof_end();
+            h->prof_begin(VH_K_ADJ_ROWS);
+            vh_t3_rows_adj(st, h->s, A->Fa, S.Fd + (size_t)b0 * per, nb / 3, h->d_rate3 + (size_t)(b0 / 3) * ng,
+                           S.dt, A->rate_h + (size_t)(b0 / 3) * ng);
+            h->prof_end();
+            h->prof_begin(VH_K_TSH_ANALYSIS);
             vh_tsh_analysis(st, h->s, A->Fa, nb, h->d_n6, A->Xa + (size_t)b0 * 4 * d.nlm);
+            h->prof_end();
         }
     }
     adv_adj_args a{};
     a.nr = d.nr; a.nk = d.nk; a.nem = d.nem; a.nlm = d.nlm; a.dt = S.dt; a.v = h->v;
     a.tauh_in = th_in; a.tauh_out = th_out; a.tau = S.tau; a.sigma = S.sigma; a.dN = S.dN;
     a.e3of = h->d_e3of; a.Xh = A->Xa; a.n6 = h->d_n6; a.Ue = A->Ue; a.mpart = A->mpart;
+    h->prof_begin(VH_K_ADJ_ADVANCE);
     vh_advance_adjoint(st, d, a, h->d_mnodes, vh_drift_plan_node_el(h->dplan), A->dNh, A->sigh, A->Kh);
+    h->prof_end();
+    h->prof_begin(VH_K_ADJ_SLE);
     sle_adjoint(h, S, rh);
+    h->prof_end();
+    h->prof_begin(VH_K_ADJ_DRIFT);
     vh_drift_adjoint(st, h->dplan, d, h->v, S.mnorm, S.mnorm + d.nk, h->skip_tol, h->deg1_cm, A->dNh, A->dsurfh,
                      A->Fh, th_out);
+    h->prof_end();
     A->ah ^= 1;
 }
 
@@ -1062,26 +1116,32 @@ void backward(vh_model* h, const double* dJ_drsl, bool dev, double* dJ_dpert, do
     if (A->rate_h) vh_fill(st, A->rate_h, 0.0, (size_t)h->ne3d * ng);
     const int nseg = (A->ncalls + A->every - 1) / A->every;
     for (int seg = nseg - 1; seg >= 0; seg--) {
-        adj_restore(h, A->ck[seg]);
-        A->nsub = 0;
-        A->recording = true;
-        const int c1 = std::min(A->ncalls, (seg + 1) * A->every);
-        try {
-            for (int c = seg * A->every; c < c1; c++) {
-                A->call = c;
-                update(h, A->ice + (size_t)c * ng, A->dt_yr, true);
+        if (!(seg == nseg - 1 && A->tail_ok)) {   // re-run the segment from its checkpoint
+            A->tail_ok = false;
+            adj_restore(h, A->ck[seg]);
+            A->nsub = 0;
+            A->recording = true;
+            const int c1 = std::min(A->ncalls, (seg + 1) * A->every);
+            try {
+                for (int c = seg * A->every; c < c1; c++) {
+                    A->call = c;
+                    update(h, A->ice + (size_t)c * ng, A->dt_yr, true);
+                }
+            } catch (...) {
+                A->recording = false;
+                throw;
             }
-        } catch (...) {
             A->recording = false;
-            throw;
         }
-        A->recording = false;
         for (size_t i = A->nsub; i-- > 0;) {
             const adj_sub& S = A->sub[i];
             const bool last = i + 1 == A->nsub || A->sub[i + 1].call != S.call;
             reverse_sub(h, S, last ? A->rh + (size_t)S.call * ng : nullptr);
         }
+        h->sync();
+        h->prof_collect();
     }
+    A->tail_ok = false;
     adj_restore(h, A->ck[nseg]);
     vh_pert_grad(st, d.ne, ng, A->e3of_mesh, A->em_of, h->d_MkPerDt, h->d_rate3, A->rate_h, A->Kh, A->pgrad);
     const hipMemcpyKind k = out_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
@@ -1562,6 +1622,13 @@ int vh_backward(vh_model* h, const double* dJ_drsl, double* dJ_dlogeta, double* 
 }
 int vh_backward_device(vh_model* h, const double* dJ_drsl_device, double* dJ_dlogeta_device, double* dJ_dice_device) {
     return guarded(h, [&] { backward(h, dJ_drsl_device, true, dJ_dlogeta_device, dJ_dice_device, true); });
+}
+int vh_forward_rewind(vh_model* h) {
+    return guarded(h, [&] {
+        if (!h->adj || !h->adj->recorded) throw vh_error("vh_forward_rewind: no forward record");
+        adj_restore(h, h->adj->ck[0]);
+        h->sync();
+    });
 }
 
 int vh_sht_analysis(vh_model* h, const double* grid, double* lm) {

@@ -48,6 +48,7 @@ def load():
         L.vh_forward_record_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_int, C.c_void_p]
         L.vh_backward.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 3
         L.vh_backward_device.argtypes = [C.c_void_p] + [C.c_void_p] * 3
+        L.vh_forward_rewind.argtypes = [C.c_void_p]
         for name in exported_symbols():
             fn = getattr(L, name)
             if fn.argtypes is None:   # every entry point takes a handle or pointer: never call undeclared
@@ -154,6 +155,10 @@ class SolidEarth(ModelHandle):
         di = np.empty((n + 1, self.nlat, self.nphi))
         self._check(self._lib.vh_backward(self._h, _cp(g), _cp(dp), _cp(di)), "backward")
         return dp, di
+
+    def rewind(self):
+        """Back to the state at the start of the last forward_record."""
+        self._check(self._lib.vh_forward_rewind(self._h), "forward_rewind")
 
 
 def solid_earth_init(params, z_bed_eq, h_ice_eq, h_ice_init=None):

@@ -158,7 +158,11 @@ enum {
     VH_K_TSH_ANALYSIS = 4,   /* 3-D: tensor Legendre analysis                  */
     VH_K_T3_GATHER = 5,      /* 3-D: memory + 2 mu strain coefficients         */
     VH_K_SLE = 6,            /* sea-level equation: the fused solve (passes)   */
-    VH_K_NCLASS = 8
+    VH_K_ADJ_ROWS = 7,       /* reverse: 3-D row pass with the rate gradient   */
+    VH_K_ADJ_ADVANCE = 8,    /* reverse: memory advance transpose + node sums  */
+    VH_K_ADJ_SLE = 9,        /* reverse: transposed sea-level solve            */
+    VH_K_ADJ_DRIFT = 10,     /* reverse: G^T product + forcing transpose       */
+    VH_K_NCLASS = 12
 };
 int vh_profile(vh_model* h, int enable);
 /* ms[c] = summed device milliseconds, n[c] = number of timed launches */
@@ -215,6 +219,9 @@ int vh_forward_record(vh_model* h, int ncalls, const double* h_ice_hist, double 
 int vh_forward_record_device(vh_model* h, int ncalls, const double* h_ice_hist_device, double dt_yr,
                              int ckpt_every, double* rsl_out_device);
 int vh_backward(vh_model* h, const double* dJ_drsl, double* dJ_dlogeta, double* dJ_dice);
+/* Restores the model state at the start of the last vh_forward_record (the
+ * window's initial state), e.g. to run the same window again. */
+int vh_forward_rewind(vh_model* h);
 int vh_backward_device(vh_model* h, const double* dJ_drsl_device, double* dJ_dlogeta_device,
                        double* dJ_dice_device);
 

@@ -83,9 +83,12 @@ def test_forward_record_matches_updates():
     c, p, ice, wgt, tgt = _case(16, 4)
     rsl, n_acc, gp, gi, rsl_after = _gradients(c, p, ice, wgt, tgt, 2)
     ref, n_ref = _run(lib.SolidEarth, c, p, c["pert"], ice)
-    assert np.array_equal(rsl, ref)
+    # the first model run of a process can differ from later ones in the last
+    # bit (7e-15 on 17 m here); the record itself adds nothing
+    scale = np.abs(ref).max()
+    assert np.abs(rsl - ref).max() <= 1e-13 * scale
     assert n_acc == n_ref
-    assert np.array_equal(rsl_after, ref[-1])
+    assert np.abs(rsl_after - ref[-1]).max() <= 1e-13 * scale
     assert np.all(np.isfinite(gp)) and np.all(np.isfinite(gi))
 
 
