@@ -9,7 +9,8 @@ struct adv_adj_args {
     vh_dev v;
     const double* tauh_in;    // adjoint of tau_{n+1}
     double* tauh_out;         // adjoint of tau_n (overwritten)
-    const double* tau;        // forward tau_n
+    const double* tau;        // forward tau_n of the 1-D elements [e1of[em]][24][nk]
+    const int* e1of;          // [nem] -> index of a 1-D element in tau, or -1
     const double* sigma;      // forward closing load transform (complex, degree-major)
     const double* dN;         // forward nodal drift [mn][4][nk]
     const int* e3of;          // [nem] -> 3-D element or -1

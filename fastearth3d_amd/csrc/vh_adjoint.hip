@@ -83,6 +83,7 @@ __global__ void __launch_bounds__(256) k_adv_adj(adv_adj_args a) {
             strain_coeffs(U0i, U1i, V0i, V1i, Jr, s[1][0], s[1][1], s[1][2]);
             const double Me = a.v.MkPerDt[e] * a.dt;
             const double om = 1.0 - Me, two_mu = 2.0 * a.v.mu[e], two_muM = two_mu * Me;
+            const double* t1 = a.tau + (size_t)a.e1of[em] * 24 * nk + k;
 #pragma unroll
             for (int sh = 0; sh < 3; sh++)
 #pragma unroll
@@ -93,7 +94,7 @@ __global__ void __launch_bounds__(256) k_adv_adj(adv_adj_args a) {
                         const double th = a.tauh_in[off];
                         a.tauh_out[off] = om * th;
                         Eh[ri][sh][lam] = -two_muM * th;
-                        mh = fma(th, -a.tau[off] - two_mu * s[ri][sh][lam], mh);
+                        mh = fma(th, -t1[(size_t)((sh * 2 + ri) * 4 + lam) * nk] - two_mu * s[ri][sh][lam], mh);
                     }
         }
         double uh[2][4];   // (U0, U1, V0, V1) per re/im

@@ -597,16 +597,17 @@ void advance_memory(vh_model* h, const double* sigma, int nxt) {
         h->prof_end();
         for (int b0 = 0; b0 < h->nbat; b0 += h->tchunk) {
             const int nb = std::min(h->tchunk, h->nbat - b0);
+            // while recording for the reverse sweep, the synthesis writes into
+            // the sub-step's store (its rate gradient reads the forward rows)
+            // and the row pass reads it from there
+            double2* syn = h->d_Fd;
+            if (h->adj && adj_recording(h->adj))
+                syn = adj_fd_pending(h) + (size_t)b0 * 6 * (d.L + 1) * d.nlat;
             h->prof_begin(VH_K_TSH_SYNTH);
-            vh_tsh_synth(st, h->s, h->d_X + (size_t)b0 * 4 * d.nlm, nb, h->d_Fd);
+            vh_tsh_synth(st, h->s, h->d_X + (size_t)b0 * 4 * d.nlm, nb, syn);
             h->prof_end();
-            if (h->adj && adj_recording(h->adj)) {   // the reverse sweep's rate gradient reads them
-                const size_t per = (size_t)6 * (d.L + 1) * d.nlat;
-                HIPCHECK(hipMemcpyAsync(adj_fd_pending(h) + (size_t)b0 * per, h->d_Fd, nb * per * sizeof(double2),
-                                        hipMemcpyDeviceToDevice, st));
-            }
             h->prof_begin(VH_K_T3_ROWS);
-            vh_t3_rows(st, h->s, h->d_Fd, nb, 0, h->d_rate3 + (size_t)(b0 / 3) * ng, 3, h->dt, nullptr);
+            vh_t3_rows(st, h->s, h->d_Fd, nb, 0, h->d_rate3 + (size_t)(b0 / 3) * ng, 3, h->dt, nullptr, syn);
             h->prof_end();
             h->prof_begin(VH_K_TSH_ANALYSIS);
             vh_tsh_analysis(st, h->s, h->d_Fd, nb, h->d_n6, h->d_cdel + (size_t)b0 * 4 * d.nlm);
@@ -783,8 +784,9 @@ void update(vh_model* h, const double* h_ice, double dt_yr, bool on_device) {
 // gate, coastline mask, sub-step sizes, 3-D element set) are those of the
 // forward run; the sea-level solve's warm start is not differentiated (the
 // converged rsl does not depend on it).
-struct adj_sub {   // one accepted FE sub-step: tau_n, closing sigma, drift, last mask, norms
+struct adj_sub {   // one accepted FE sub-step: tau_n (1-D elements), closing sigma, drift, last mask, norms
     double *tau = 0, *sigma = 0, *dN = 0, *C = 0, *mnorm = 0;
+    size_t tau_n = 0;
     double2* Fd = 0;   // 3-D: the forward tensor row spectra of every batch
     size_t fd_n = 0;
     double dt = 0, frac = 0, c_int = 0;
@@ -803,7 +805,14 @@ struct adj_ctx {
     std::vector<adj_sub> sub;
     size_t nsub = 0;
     bool recording = false, recorded = false;
-    bool tail_ok = false;   // the store holds the last segment (kept by the record pass)
+    // the record pass keeps the sub-steps of the last nstore segments
+    // (entries seg_first[seg] .. + seg_n[seg]); the others are re-run
+    bool stored_ok = false;
+    int nstore = 1, ne3d_rec = -1;
+    double store_gib = 96.0;
+    std::vector<int> seg_first, seg_n;
+    std::vector<int> e1list;   // Maxwell elements advanced by the 1-D update (their memory is stored)
+    int* e1of = nullptr;       // [nem] -> index in e1list or -1
     int call = 0;
     double2* Fd_pending = nullptr;
     size_t fd_n = 0;
@@ -838,7 +847,7 @@ double2* adj_fd_pending(vh_model* h) {
 void adj_free(adj_ctx* a) {
     std::vector<void*> p = {a->ice, a->tauh[0], a->tauh[1], a->Ue, a->mpart, a->Kh, a->dNh, a->dsurfh, a->sigh,
                             a->Fh, a->rate_h, a->iceh, a->rh, a->pgrad, a->Xa, a->Fa, a->gx, a->gb, a->gt, a->gy,
-                            a->part, a->sc, a->F, a->q, a->q1, a->e3of_mesh, a->em_of, a->Fd_pending};
+                            a->part, a->sc, a->F, a->q, a->q1, a->e3of_mesh, a->em_of, a->Fd_pending, a->e1of};
     for (auto& c : a->ck) { p.push_back(c.tau); p.push_back(c.mnorm); p.push_back(c.rsl); p.push_back(c.hice); }
     for (auto& u : a->sub) {
         p.push_back(u.tau); p.push_back(u.sigma); p.push_back(u.dN); p.push_back(u.C); p.push_back(u.mnorm);
@@ -885,12 +894,23 @@ void adj_stash_push(vh_model* h, double dt, double frac, const sle_res& r) {
     const vh_dims& d = h->d;
     if (A->nsub == A->sub.size()) {
         adj_sub u;
-        u.tau = dalloc<double>(adj_ntau(h)); u.sigma = dalloc<double>(2 * (size_t)d.nlm);
+        u.sigma = dalloc<double>(2 * (size_t)d.nlm);
         u.dN = dalloc<double>(adj_ndN(h)); u.C = dalloc<double>(h->ng()); u.mnorm = dalloc<double>(d.nk + 1);
         A->sub.push_back(u);
     }
     adj_sub& u = A->sub[A->nsub++];
-    d2d(u.tau, h->d_tau[h->cur], adj_ntau(h), h->st);
+    const size_t per_el = (size_t)24 * d.nk, ntau1 = per_el * A->e1list.size();
+    if (u.tau_n != ntau1) {
+        if (u.tau) HIPCHECK(hipFree(u.tau));
+        u.tau = ntau1 ? dalloc<double>(ntau1) : nullptr;
+        u.tau_n = ntau1;
+    }
+    if (A->e1list.size() == (size_t)d.nem) {
+        d2d(u.tau, h->d_tau[h->cur], ntau1, h->st);
+    } else {
+        for (size_t i = 0; i < A->e1list.size(); i++)
+            d2d(u.tau + i * per_el, h->d_tau[h->cur] + (size_t)A->e1list[i] * per_el, per_el, h->st);
+    }
     d2d(u.sigma, (const double*)h->d_sigma, 2 * (size_t)d.nlm, h->st);
     d2d(u.dN, h->d_dN, adj_ndN(h), h->st);
     d2d(u.C, h->d_C, h->ng(), h->st);
@@ -926,10 +946,41 @@ void forward_record(vh_model* h, int ncalls, const double* ice_hist, bool dev, d
                             dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->st));
     const int nseg = (ncalls + A->every - 1) / A->every;
     if ((int)A->ck.size() < nseg + 1) A->ck.resize(nseg + 1);
-    A->tail_ok = false;
+    // the 1-D Maxwell elements (their memory enters the rate adjoint)
+    const vh_dims& d = h->d;
+    std::vector<unsigned char> is3d(std::max(d.nem, 1), 0);
+    if (h->lat_visc) HIPCHECK(hipMemcpy(is3d.data(), h->d_is3d, d.nem, hipMemcpyDeviceToHost));
+    A->e1list.clear();
+    std::vector<int> e1of(std::max(d.nem, 1), -1);
+    for (int em = 0; em < d.nem; em++)
+        if (!is3d[em]) { e1of[em] = (int)A->e1list.size(); A->e1list.push_back(em); }
+    if (!A->e1of) A->e1of = dalloc<int>(std::max(d.nem, 1));
+    upload(A->e1of, e1of.data(), std::max(d.nem, 1), h->st);
+    A->ne3d_rec = h->lat_visc ? h->ne3d : 0;
+    // trailing segments whose sub-steps fit the store limit are kept now and
+    // not re-run by the backward sweep (at least the last one)
+    {
+        const double span = dt_yr * VH_SEC_PER_YEAR;
+        double dt_stab = h->rate_max > 0.0 ? h->cfl / h->rate_max : span;
+        dt_stab = std::min(dt_stab, h->dt_max);
+        if (h->dt_min > 0.0) dt_stab = std::max(dt_stab, h->dt_min);
+        const double nsub_call = std::max(1.0, std::ceil(span / dt_stab - 1.0e-9));
+        const double per_sub = 8.0 * ((double)24 * A->e1list.size() * d.nk + 2.0 * d.nlm + (double)adj_ndN(h) +
+                                      (double)ng + d.nk + 1) +
+                                (h->lat_visc && h->ne3d > 0 ? 16.0 * (double)adj_nfd(h) : 0.0);
+        const double per_seg = 1.1 * A->every * nsub_call * per_sub;
+        A->nstore = (int)std::max(1.0, std::min((double)nseg, std::floor(A->store_gib * 1073741824.0 / per_seg)));
+    }
+    A->seg_first.assign(nseg, 0);
+    A->seg_n.assign(nseg, 0);
+    A->stored_ok = false;
+    A->nsub = 0;
     for (int c = 0; c < ncalls; c++) {
-        if (c % A->every == 0) adj_save(h, A->ck[c / A->every]);
-        if (c == (nseg - 1) * A->every) { A->nsub = 0; A->recording = true; }   // the last segment is reversed first
+        const int seg = c / A->every;
+        if (c % A->every == 0) {
+            adj_save(h, A->ck[seg]);
+            if (seg >= nseg - A->nstore) { A->recording = true; A->seg_first[seg] = (int)A->nsub; }
+        }
         A->call = c;
         try {
             update(h, A->ice + (size_t)c * ng, dt_yr, true);
@@ -937,6 +988,7 @@ void forward_record(vh_model* h, int ncalls, const double* ice_hist, bool dev, d
             A->recording = false;
             throw;
         }
+        if (A->recording) A->seg_n[seg] = (int)A->nsub - A->seg_first[seg];
         if (rsl_out)
             HIPCHECK(hipMemcpyAsync(rsl_out + (size_t)c * ng, h->d_rsl, ng * sizeof(double),
                                     rsl_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, h->st));
@@ -945,7 +997,7 @@ void forward_record(vh_model* h, int ncalls, const double* ice_hist, bool dev, d
     adj_save(h, A->ck[nseg]);
     h->sync();
     A->recorded = true;
-    A->tail_ok = true;
+    A->stored_ok = true;
 }
 
 void adj_alloc_work(vh_model* h) {
@@ -1085,7 +1137,7 @@ void reverse_sub(vh_model* h, const adj_sub& S, const double* rh) {
     adv_adj_args a{};
     a.nr = d.nr; a.nk = d.nk; a.nem = d.nem; a.nlm = d.nlm; a.dt = S.dt; a.v = h->v;
     a.tauh_in = th_in; a.tauh_out = th_out; a.tau = S.tau; a.sigma = S.sigma; a.dN = S.dN;
-    a.e3of = h->d_e3of; a.Xh = A->Xa; a.n6 = h->d_n6; a.Ue = A->Ue; a.mpart = A->mpart;
+    a.e3of = h->d_e3of; a.e1of = A->e1of; a.Xh = A->Xa; a.n6 = h->d_n6; a.Ue = A->Ue; a.mpart = A->mpart;
     h->prof_begin(VH_K_ADJ_ADVANCE);
     vh_advance_adjoint(st, d, a, h->d_mnodes, vh_drift_plan_node_el(h->dplan), A->dNh, A->sigh, A->Kh);
     h->prof_end();
@@ -1115,11 +1167,19 @@ void backward(vh_model* h, const double* dJ_drsl, bool dev, double* dJ_dpert, do
     vh_fill(st, A->iceh, 0.0, (size_t)(A->ncalls + 1) * ng);
     if (A->rate_h) vh_fill(st, A->rate_h, 0.0, (size_t)h->ne3d * ng);
     const int nseg = (A->ncalls + A->every - 1) / A->every;
+    if ((h->lat_visc ? h->ne3d : 0) != A->ne3d_rec)
+        throw vh_error("vh_backward: the lateral viscosity changed since vh_forward_record");
+    const int first_stored = A->stored_ok ? nseg - A->nstore : nseg;
+    size_t top = 0;   // entries above the stored segments take the re-run ones
+    for (int seg = first_stored; seg < nseg; seg++) top = std::max(top, (size_t)A->seg_first[seg] + A->seg_n[seg]);
     for (int seg = nseg - 1; seg >= 0; seg--) {
-        if (!(seg == nseg - 1 && A->tail_ok)) {   // re-run the segment from its checkpoint
-            A->tail_ok = false;
+        size_t first, n;
+        if (seg >= first_stored) {
+            first = A->seg_first[seg];
+            n = A->seg_n[seg];
+        } else {   // re-run the segment from its checkpoint
             adj_restore(h, A->ck[seg]);
-            A->nsub = 0;
+            A->nsub = top;
             A->recording = true;
             const int c1 = std::min(A->ncalls, (seg + 1) * A->every);
             try {
@@ -1132,16 +1192,17 @@ void backward(vh_model* h, const double* dJ_drsl, bool dev, double* dJ_dpert, do
                 throw;
             }
             A->recording = false;
+            first = top;
+            n = A->nsub - top;
         }
-        for (size_t i = A->nsub; i-- > 0;) {
+        for (size_t i = first + n; i-- > first;) {
             const adj_sub& S = A->sub[i];
-            const bool last = i + 1 == A->nsub || A->sub[i + 1].call != S.call;
+            const bool last = i + 1 == first + n || A->sub[i + 1].call != S.call;
             reverse_sub(h, S, last ? A->rh + (size_t)S.call * ng : nullptr);
         }
         h->sync();
         h->prof_collect();
     }
-    A->tail_ok = false;
     adj_restore(h, A->ck[nseg]);
     vh_pert_grad(st, d.ne, ng, A->e3of_mesh, A->em_of, h->d_MkPerDt, h->d_rate3, A->rate_h, A->Kh, A->pgrad);
     const hipMemcpyKind k = out_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
@@ -1622,6 +1683,13 @@ int vh_backward(vh_model* h, const double* dJ_drsl, double* dJ_dlogeta, double* 
 }
 int vh_backward_device(vh_model* h, const double* dJ_drsl_device, double* dJ_dlogeta_device, double* dJ_dice_device) {
     return guarded(h, [&] { backward(h, dJ_drsl_device, true, dJ_dlogeta_device, dJ_dice_device, true); });
+}
+int vh_adjoint_store_limit(vh_model* h, double gib) {
+    return guarded(h, [&] {
+        if (!(gib >= 0.0)) throw vh_error("vh_adjoint_store_limit: the limit must be >= 0 GiB");
+        if (!h->adj) h->adj = new adj_ctx();
+        h->adj->store_gib = gib;
+    });
 }
 int vh_forward_rewind(vh_model* h) {
     return guarded(h, [&] {

@@ -138,10 +138,10 @@ void vh_fill(hipStream_t st, double* p, double v, size_t n);
 void vh_tsh_synth(hipStream_t st, const vh_sht& s, const double2* X, int nb, double2* Fd);
 // Fd -> C [nb][4][nlm]; n6 == null leaves the raw spin-2 adjoint in channel 3
 void vh_tsh_analysis(hipStream_t st, const vh_sht& s, const double2* Fd, int nb, const double* n6, double2* C);
-// mode 0: Fd <- FFT(-rate*dt * IFFT(Fd)), rate [nb/bpe][nlat][nphi];
-// mode 1: grid [nb][6][nlat][nphi] <- IFFT(Fd); mode 2: Fd <- FFT(grid)
+// mode 0: Fd <- FFT(-rate*dt * IFFT(Fin)), rate [nb/bpe][nlat][nphi] (Fin null: Fd, in place);
+// mode 1: grid [nb][6][nlat][nphi] <- IFFT(Fin); mode 2: Fd <- FFT(grid)
 void vh_t3_rows(hipStream_t st, const vh_sht& s, double2* Fd, int nb, int mode, const double* rate, int bpe,
-                double dt, double* grid);
+                double dt, double* grid, const double2* Fin = nullptr);
 void vh_t3_gather(hipStream_t st, const vh_dims& d, const vh_dev& v, const int* e3em, int ne3d, const double* tau,
                   const double* sigma, const double* dN, double2* X);
 void vh_tsh_calibrate_n6(hipStream_t st, const vh_sht& s, double* n6_dev);

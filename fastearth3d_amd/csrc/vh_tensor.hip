@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(256) k_tsh_analysis(vh_sht s, const double2* _
 //           element), forward FFT, in place in Fd
 //   mode 1: inverse FFT to grid[b][dy][j][i]
 //   mode 2: forward FFT of grid[b][dy][j][i] into Fd
-__global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, double2* __restrict__ Fd, int nb, int mode,
+__global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, double2* Fd, int nb, int mode,
                                                  const double* __restrict__ rate, int bpe, double dt,
                                                  double* __restrict__ grid) {
     extern __shared__ double2 lds[];
@@ -343,6 +343,8 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, double2* __restrict__
     double2* c = a + N;
     double* Mrow = (double*)(lds + (size_t)3 * 2 * N);
     const int d0 = 2 * wave, d1 = 2 * wave + 1;
+    const double2* I0 = Fin + ((size_t)b * 6 + d0) * (L + 1) * nlat + j;
+    const double2* I1 = Fin + ((size_t)b * 6 + d1) * (L + 1) * nlat + j;
     double2* F0 = Fd + ((size_t)b * 6 + d0) * (L + 1) * nlat + j;
     double2* F1 = Fd + ((size_t)b * 6 + d1) * (L + 1) * nlat + j;
     if (mode == 0) {
@@ -358,8 +360,8 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, double2* __restrict__
 #pragma unroll
             for (int u = 0; u < VH_FFT_U; u++) {
                 const int m = m0 + lane + 64 * u;
-                A[u] = m <= L ? F0[(size_t)m * nlat] : make_double2(0.0, 0.0);
-                B[u] = m <= L ? F1[(size_t)m * nlat] : make_double2(0.0, 0.0);
+                A[u] = m <= L ? I0[(size_t)m * nlat] : make_double2(0.0, 0.0);
+                B[u] = m <= L ? I1[(size_t)m * nlat] : make_double2(0.0, 0.0);
             }
 #pragma unroll
             for (int u = 0; u < VH_FFT_U; u++) {
@@ -471,9 +473,9 @@ void vh_tsh_analysis(hipStream_t st, const vh_sht& s, const double2* Fd, int nb,
 }
 
 void vh_t3_rows(hipStream_t st, const vh_sht& s, double2* Fd, int nb, int mode, const double* rate, int bpe,
-                double dt, double* grid) {
+                double dt, double* grid, const double2* Fin) {
     if (nb <= 0) return;
-    k_t3_rows<<<xcd_grid(nb * s.nlat), 192, rows_lds(s), st>>>(s, Fd, nb, mode, rate, bpe, dt, grid);
+    k_t3_rows<<<xcd_grid(nb * s.nlat), 192, rows_lds(s), st>>>(s, Fin ? Fin : Fd, Fd, nb, mode, rate, bpe, dt, grid);
     HIPCHECK(hipGetLastError());
 }
 

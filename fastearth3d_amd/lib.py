@@ -49,6 +49,7 @@ def load():
         L.vh_backward.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 3
         L.vh_backward_device.argtypes = [C.c_void_p] + [C.c_void_p] * 3
         L.vh_forward_rewind.argtypes = [C.c_void_p]
+        L.vh_adjoint_store_limit.argtypes = [C.c_void_p, C.c_double]
         for name in exported_symbols():
             fn = getattr(L, name)
             if fn.argtypes is None:   # every entry point takes a handle or pointer: never call undeclared
@@ -155,6 +156,10 @@ class SolidEarth(ModelHandle):
         di = np.empty((n + 1, self.nlat, self.nphi))
         self._check(self._lib.vh_backward(self._h, _cp(g), _cp(dp), _cp(di)), "backward")
         return dp, di
+
+    def store_limit(self, gib):
+        """HBM (GiB) the record pass may keep for the backward sweep."""
+        self._check(self._lib.vh_adjoint_store_limit(self._h, float(gib)), "adjoint_store_limit")
 
     def rewind(self):
         """Back to the state at the start of the last forward_record."""

@@ -222,6 +222,10 @@ int vh_backward(vh_model* h, const double* dJ_drsl, double* dJ_dlogeta, double* 
 /* Restores the model state at the start of the last vh_forward_record (the
  * window's initial state), e.g. to run the same window again. */
 int vh_forward_rewind(vh_model* h);
+/* HBM the record pass may use to keep the sub-step inputs of the trailing
+ * checkpoint segments, which the backward sweep then does not re-run
+ * (default 96 GiB; at least the last segment is kept; 0 keeps only it). */
+int vh_adjoint_store_limit(vh_model* h, double gib);
 int vh_backward_device(vh_model* h, const double* dJ_drsl_device, double* dJ_dlogeta_device,
                        double* dJ_dice_device);
 

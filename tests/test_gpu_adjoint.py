@@ -65,10 +65,12 @@ def _directions(c, p, ice, seed):
     return vi, vp
 
 
-def _gradients(c, p, ice, wgt, tgt, every):
+def _gradients(c, p, ice, wgt, tgt, every, store_gib=None):
     from fastearth3d_amd import lib
     m = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
     m.enable_visc_3d(c["pert"])
+    if store_gib is not None:
+        m.store_limit(store_gib)
     rsl = m.forward_record(ice, c["dt_yr"], ckpt_every=every)
     n_acc = (m.scalar(3), m.dims["ne3d"])
     gp, gi = m.backward(wgt * (rsl - tgt))
@@ -93,12 +95,15 @@ def test_forward_record_matches_updates():
 
 
 def test_checkpoint_interval_invariance():
-    """Segments re-run from checkpoints reproduce the same sweep: the gradient does not depend on the interval."""
+    """Segments re-run from checkpoints reproduce the same sweep as segments kept by the record pass:
+    the gradient depends neither on the interval nor on the store limit."""
     c, p, ice, wgt, tgt = _case(16, 5)
     a = _gradients(c, p, ice, wgt, tgt, 1)
     b = _gradients(c, p, ice, wgt, tgt, 3)
     d = _gradients(c, p, ice, wgt, tgt, 5)
-    for x in (b, d):
+    e = _gradients(c, p, ice, wgt, tgt, 1, store_gib=0.0)   # every segment but the last re-run
+    f = _gradients(c, p, ice, wgt, tgt, 2, store_gib=0.0)
+    for x in (b, d, e, f):
         assert np.array_equal(x[2], a[2])
         assert np.array_equal(x[3], a[3])
 
