@@ -380,6 +380,25 @@ def adjoint_bench(a, L):
 
     kernel_ms = {ADJ_NAMES.get(k, NAMES.get(k)): round(ms[k], 3) for k in range(NCLASS) if cnt[k] > 0}
     ratio = (t_rec + t_bwd) / t_fwd
+    # roofline of the reverse sweep's dominant kernel, k_t3_rows_adj: per
+    # launch it reads the adjoint and forward row spectra of its batches,
+    # writes the adjoint rows back, reads the rate rows and updates the rate
+    # gradient of its elements (all FP64)
+    roof = None
+    if cnt[7] > 0:
+        nbat = 3 * n3
+        nch = (nbat + 47) // 48
+        b = nbat / nch                                    # batches per launch, on average
+        rows = b * 6 * (a.lmax + 1) * nlat * 16.0
+        ab = 3.0 * rows + 3.0 * (b / 3.0) * ng * 8.0
+        avg_s = ms[7] / cnt[7] / 1000.0
+        ach = ab / avg_s / 1e9
+        tf = os.path.join(ROOT, "profiles", "c3_lmax%d_adjoint_traffic.json" % a.lmax)
+        tr = json.load(open(tf))["kernels"].get("k_t3_rows_adj") if os.path.exists(tf) else None
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": tr["hbm_bytes_per_launch"] if tr else None,
+                "kernel": "k_t3_rows_adj", "avg_launch_us": round(avg_s * 1e6, 2), "launches": int(cnt[7]),
+                "alg_bytes_per_launch": int(ab)}
     out = {
         "metric": "adjoint/forward time ratio (lmax 128 3-D, %d steps, 1 GPU)" % ncalls, "value": round(ratio, 4),
         "unit": "x", "n_gpus": 1, "steps": ncalls, "warmup": 1,
@@ -394,7 +413,7 @@ def adjoint_bench(a, L):
         "J": J, "grad_delta_lm_norm": gdelta_norm ** 0.5,
         "grad_check": {"h": h, "branch": "coastline held (sle_fixed_ocean), skip gate off, sle_tol 1e-11", "directions": checks,
                        "max_rel_err": max((x["rel_err"] for x in checks), default=None)},
-        "kernel_ms_reverse_run": kernel_ms, "roofline": None, "cpu_baseline": None,
+        "kernel_ms_reverse_run": kernel_ms, "roofline": roof, "cpu_baseline": None,
     }
     print(json.dumps(out))
 

@@ -32,6 +32,7 @@ void vh_t3_pack_adj(hipStream_t st, const vh_dims& d, const vh_dev& v, const int
 void vh_t3_rows_adj(hipStream_t st, const vh_sht& s, double2* Fa, const double2* Ff, int ne3, const double* rate,
                     double dt, double* rate_h);
 
+// mode 2: y = x - C I/C_int; mode 4: the ice adjoints from t (and I(b))
 struct sle_adj_grid_args {
     int mode, n, nphi;
     double dlon, inv_cint, frac;
@@ -43,6 +44,9 @@ struct sle_adj_grid_args {
     double *x, *b, *y, *ih0, *ih1;
 };
 void vh_sle_adj_grid(hipStream_t st, const sle_adj_grid_args& a);
+// x = rho_w C t (+ rh/W if first), b = x (first) or b += x, part[j] = row sums of x
+void vh_sle_adj_row(hipStream_t st, const vh_sht& s, const double* C, const double* t, const double* rh, int first,
+                    double* x, double* b, double* part);
 void vh_sle_adj_spec(hipStream_t st, int nlm, int nk, int mode, const double2* src, double2* q, const double2* q1,
                      const double* gu, const double* gn, double g, double* dsurfh);
 

@@ -149,13 +149,16 @@ __global__ void __launch_bounds__(256) k_adv_nodes(int nr, int nk, int nmn, vh_d
     sigh[2 * (k + 1) + 1] = si;
 }
 
-// Kh[em] += dt * sum over blocks of mpart (fixed order)
-__global__ void k_mreduce(int nem, int nbx, const double* __restrict__ mpart, double dt, double* Kh) {
-    const int em = blockIdx.x * blockDim.x + threadIdx.x;
-    if (em >= nem) return;
+// Kh[em] += dt * sum over blocks of mpart: one 64-lane block per element,
+// strided partial sums then a fixed butterfly (deterministic)
+__global__ void __launch_bounds__(64) k_mreduce(int nem, int nbx, const double* __restrict__ mpart, double dt,
+                                                double* Kh) {
+    const int em = blockIdx.x, lane = threadIdx.x;
     double s = 0.0;
-    for (int b = 0; b < nbx; b++) s += mpart[(size_t)em * nbx + b];
-    Kh[em] += dt * s;
+    for (int b = lane; b < nbx; b += 64) s += mpart[(size_t)em * nbx + b];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) Kh[em] += dt * s;
 }
 
 void vh_advance_adjoint(hipStream_t st, const vh_dims& d, const adv_adj_args& a0, const int* mnodes,
@@ -165,7 +168,7 @@ void vh_advance_adjoint(hipStream_t st, const vh_dims& d, const adv_adj_args& a0
     dim3 g((d.nk + 63) / 64, (d.nem + 3) / 4);
     k_adv_adj<<<g, 256, 0, st>>>(a);
     HIPCHECK(hipGetLastError());
-    k_mreduce<<<(d.nem + 63) / 64, 64, 0, st>>>(d.nem, (int)g.x, a.mpart, a.dt, Kh);
+    k_mreduce<<<d.nem, 64, 0, st>>>(d.nem, (int)g.x, a.mpart, a.dt, Kh);
     HIPCHECK(hipGetLastError());
     k_adv_nodes<<<(d.nk + 255) / 256, 256, 0, st>>>(d.nr, d.nk, d.nmn, a.v, mnodes, node_el, a.Ue, dNh, sigh);
     HIPCHECK(hipGetLastError());
@@ -337,20 +340,8 @@ __global__ void k_sle_adj_grid(sle_adj_grid_args a) {
     const double W = a.w[j] * a.dlon;
     const double C = a.C[i];
     switch (a.mode) {
-        case 0: {   // x = rh / W + rho_w C t; b = 0
-            const double r = a.rh ? a.rh[i] / W : 0.0;
-            a.x[i] = r + VH_RHO_WATER * C * a.t[i];
-            a.b[i] = 0.0;
-            break;
-        }
-        case 1:     // b += x
-            a.b[i] += a.x[i];
-            break;
         case 2:     // y = x - C I(x) / C_int
             a.y[i] = a.x[i] - C * a.I[0] * a.inv_cint;
-            break;
-        case 3:     // x = rho_w C t
-            a.x[i] = VH_RHO_WATER * C * a.t[i];
             break;
         case 4: {   // dig_h = rho_i W t - (rho_i/rho_w) W I(b)/C_int; ice_h = (1 - C) dig_h
             double g = VH_RHO_ICE * W * a.t[i];
@@ -361,6 +352,40 @@ __global__ void k_sle_adj_grid(sle_adj_grid_args a) {
             break;
         }
     }
+}
+
+// One block per latitude row: x = rho_w C t (+ rh / W on the first
+// iteration), b = x (first) or b += x, and the row sum of x (part[j], for
+// the next iteration's integral of x).
+__global__ void __launch_bounds__(256) k_sle_adj_row(int nphi, double dlon, const double* __restrict__ w,
+                                                     const double* __restrict__ C, const double* __restrict__ t,
+                                                     const double* __restrict__ rh, int first, double* x, double* b,
+                                                     double* part) {
+    __shared__ double red[256];
+    const int j = blockIdx.x;
+    const double W = w[j] * dlon;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nphi; i += blockDim.x) {
+        const size_t id = (size_t)j * nphi + i;
+        double v = VH_RHO_WATER * C[id] * t[id];
+        if (first && rh) v = rh[id] / W + v;
+        x[id] = v;
+        b[id] = first ? v : b[id] + v;
+        acc += v;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[j] = red[0];
+}
+
+void vh_sle_adj_row(hipStream_t st, const vh_sht& s, const double* C, const double* t, const double* rh, int first,
+                    double* x, double* b, double* part) {
+    k_sle_adj_row<<<s.nlat, 256, 0, st>>>(s.nphi, 2.0 * VH_PI / s.nphi, s.w, C, t, rh, first, x, b, part);
+    HIPCHECK(hipGetLastError());
 }
 
 __device__ __forceinline__ int idx_degree(int idx) {

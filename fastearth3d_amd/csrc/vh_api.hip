@@ -1075,21 +1075,17 @@ void sle_adjoint(vh_model* h, const adj_sub& S, const double* rh) {
     if (nl > 0) {
         g.inv_cint = 1.0 / S.c_int;
         sle_adj_grid_args a = g;
-        a.mode = 0; a.rh = rh; a.t = A->gt; a.x = A->gx; a.b = A->gb;
-        vh_sle_adj_grid(st, a);
-        for (int i = 0; i < nl; i++) {
-            a = g; a.mode = 1; a.x = A->gx; a.b = A->gb;
-            vh_sle_adj_grid(st, a);
-            if (i == nl - 1) break;
-            // x <- rho_w C Syn((gn - gu) Anl(x - C I(x)/C_int))
-            vh_surface_integral(st, s, A->gx, A->part, A->sc);
+        // x = rh/W + rho_w C t; b = x; then b = sum_{i<nl} (T~)^i x with
+        // T~ x = rho_w C Syn((gn - gu) Anl(x - C I(x)/C_int))
+        vh_sle_adj_row(st, s, S.C, A->gt, rh, 1, A->gx, A->gb, A->part);
+        for (int i = 1; i < nl; i++) {
+            vh_weighted_total(st, s, A->part, 1, A->sc);
             a = g; a.mode = 2; a.x = A->gx; a.I = A->sc; a.y = A->gy;
             vh_sle_adj_grid(st, a);
             anl(A->gy, A->q);
             vh_sle_adj_spec(st, d.nlm, d.nk, 1, nullptr, A->q, nullptr, h->d_gu, h->d_gn, h->g, nullptr);
             syn(A->q, A->gt);
-            a = g; a.mode = 3; a.t = A->gt; a.x = A->gx;
-            vh_sle_adj_grid(st, a);
+            vh_sle_adj_row(st, s, S.C, A->gt, nullptr, 0, A->gx, A->gb, A->part);
         }
         // y = b - C I(b)/C_int; q = Anl(y) -> dsurf_h; q <- (gn - gu) q + q1
         vh_surface_integral(st, s, A->gb, A->part, A->sc + 1);
