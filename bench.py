@@ -184,7 +184,8 @@ def site_matrix(L, colat, lon):
 
 def adjoint_bench(a, L):
     """Configuration 3: reverse mode over 100 coupling calls of the lmax 128
-    3-D workload (rotation off: the adjoint holds no rotation feedback).
+    3-D workload (sea level with migrating coastlines and rotation, as the
+    forward benchmark).
     Misfit J = sum (rsl - target)^2 at 500 uniform sites x 10 evenly spaced
     output calls, target from a forward run with another seeded delta field;
     gradients w.r.t. the lateral log-viscosity field (and delta's SH
@@ -202,7 +203,6 @@ def adjoint_bench(a, L):
     ncalls, every, nsite, nout = a.adj_calls, 10, 500, 10
     c = configs.config(2, lmax=a.lmax)
     p = c["params"]
-    p.rotation = 0
     nphi, nlat = p.grid()
     ng = nphi * nlat
     colat, lon = configs.gauss_grid(nlat, nphi)
@@ -404,7 +404,7 @@ def adjoint_bench(a, L):
         "unit": "x", "n_gpus": 1, "steps": ncalls, "warmup": 1,
         "ms_per_step": round((t_rec + t_bwd) / ncalls * 1000.0, 3), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "c3 adjoint of the lmax %d 3-D + SLE workload (rotation off), %d calls of %g yr, "
+        "config": {"workload": "c3 adjoint of the lmax %d 3-D + SLE + rotation workload, %d calls of %g yr, "
                                "checkpoint every %d calls, %d sites x %d output calls"
                                % (a.lmax, ncalls, c["dt_yr"], every, nsite, nout),
                    "lmax": a.lmax, "nlat": nlat, "nphi": nphi, "ne3d": int(n3), "sub_steps": int(nsub)},

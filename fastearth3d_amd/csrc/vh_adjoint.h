@@ -32,7 +32,7 @@ void vh_t3_pack_adj(hipStream_t st, const vh_dims& d, const vh_dev& v, const int
 void vh_t3_rows_adj(hipStream_t st, const vh_sht& s, double2* Fa, const double2* Ff, int ne3, const double* rate,
                     double dt, double* rate_h);
 
-// mode 2: y = x - C I/C_int; mode 4: the ice adjoints from t (and I(b))
+// mode 2: y = x - C I/C_int; mode 4: the ice adjoints from t (and I(b)); mode 6: y += x
 struct sle_adj_grid_args {
     int mode, n, nphi;
     double dlon, inv_cint, frac;

@@ -343,6 +343,9 @@ __global__ void k_sle_adj_grid(sle_adj_grid_args a) {
         case 2:     // y = x - C I(x) / C_int
             a.y[i] = a.x[i] - C * a.I[0] * a.inv_cint;
             break;
+        case 6:     // y += x
+            a.y[i] += a.x[i];
+            break;
         case 4: {   // dig_h = rho_i W t - (rho_i/rho_w) W I(b)/C_int; ice_h = (1 - C) dig_h
             double g = VH_RHO_ICE * W * a.t[i];
             if (a.I) g -= (VH_RHO_ICE / VH_RHO_WATER) * W * a.I[0] * a.inv_cint;
@@ -396,7 +399,7 @@ __device__ __forceinline__ int idx_degree(int idx) {
 }
 
 // spectral pieces: mode 0: q1 = sigma_h with m > 0 halved;
-// mode 1: q *= (gn - gu)_l;
+// mode 1: q *= (gn - gu)_l; mode 3: q = c_m src; mode 4: q += src;
 // mode 2: dsurf_h from a = c_m q (U: -a, F: -a/g, V: 0, l >= 1), then
 //         q = (gn - gu)_l q + q1
 __global__ void k_sle_adj_spec(int nlm, int nk, int mode, const double2* __restrict__ src, double2* q,
@@ -409,6 +412,12 @@ __global__ void k_sle_adj_spec(int nlm, int nk, int mode, const double2* __restr
     if (mode == 0) {
         const double2 v = src[idx];
         q[idx] = make_double2(v.x / cm, v.y / cm);
+    } else if (mode == 3) {   // q = c_m src
+        const double2 v = src[idx];
+        q[idx] = make_double2(cm * v.x, cm * v.y);
+    } else if (mode == 4) {   // q += src
+        const double2 v = src[idx], w = q[idx];
+        q[idx] = make_double2(w.x + v.x, w.y + v.y);
     } else if (mode == 1) {
         const double f = gn[l] - gu[l];
         const double2 v = q[idx];

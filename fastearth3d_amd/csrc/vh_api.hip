@@ -794,6 +794,7 @@ struct adj_sub {   // one accepted FE sub-step: tau_n (1-D elements), closing si
 };
 struct adj_ckpt {
     double *tau = 0, *mnorm = 0, *rsl = 0, *hice = 0;
+    double *rmem = 0, *rsc = 0;   // rotation channels: memory and scalars
     double maxn_cur = 0, time_yr = 0, time_resp = 0;
 };
 struct adj_ctx {
@@ -814,6 +815,11 @@ struct adj_ctx {
     std::vector<int> e1list;   // Maxwell elements advanced by the 1-D update (their memory is stored)
     int* e1of = nullptr;       // [nem] -> index in e1list or -1
     int call = 0;
+    // rotation channels (reverse): adjoint state, s_rot adjoint of the call
+    // (density form), the load-inertia adjoint as a spectrum
+    rot_adj ra{};
+    double *srh = 0, *rtot = 0;
+    double2* sig_rot = 0;
     double2* Fd_pending = nullptr;
     size_t fd_n = 0;
     // reverse sweep
@@ -847,8 +853,11 @@ double2* adj_fd_pending(vh_model* h) {
 void adj_free(adj_ctx* a) {
     std::vector<void*> p = {a->ice, a->tauh[0], a->tauh[1], a->Ue, a->mpart, a->Kh, a->dNh, a->dsurfh, a->sigh,
                             a->Fh, a->rate_h, a->iceh, a->rh, a->pgrad, a->Xa, a->Fa, a->gx, a->gb, a->gt, a->gy,
-                            a->part, a->sc, a->F, a->q, a->q1, a->e3of_mesh, a->em_of, a->Fd_pending, a->e1of};
-    for (auto& c : a->ck) { p.push_back(c.tau); p.push_back(c.mnorm); p.push_back(c.rsl); p.push_back(c.hice); }
+                            a->part, a->sc, a->F, a->q, a->q1, a->e3of_mesh, a->em_of, a->Fd_pending, a->e1of, a->ra.mem, a->ra.dN, a->ra.sc, a->srh, a->rtot, a->sig_rot};
+    for (auto& c : a->ck) {
+        p.push_back(c.tau); p.push_back(c.mnorm); p.push_back(c.rsl); p.push_back(c.hice);
+        p.push_back(c.rmem); p.push_back(c.rsc);
+    }
     for (auto& u : a->sub) {
         p.push_back(u.tau); p.push_back(u.sigma); p.push_back(u.dN); p.push_back(u.C); p.push_back(u.mnorm);
         p.push_back(u.Fd);
@@ -876,6 +885,12 @@ void adj_save(vh_model* h, adj_ckpt& c) {
     d2d(c.mnorm + h->d.nk, h->d_maxn + h->cur, 1, h->st);
     d2d(c.rsl, h->d_rsl, ng, h->st);
     d2d(c.hice, h->d_hice, ng, h->st);
+    if (h->rot_on) {
+        const size_t nm = (size_t)2 * (h->d.nr - 1) * 24;
+        if (!c.rmem) { c.rmem = dalloc<double>(nm); c.rsc = dalloc<double>(RS_N); }
+        d2d(c.rmem, h->ra.rs.mem, nm, h->st);
+        d2d(c.rsc, h->ra.rs.sc, RS_N, h->st);
+    }
     c.maxn_cur = h->maxn_cur; c.time_yr = h->time_yr; c.time_resp = h->time_resp;
 }
 
@@ -886,6 +901,10 @@ void adj_restore(vh_model* h, const adj_ckpt& c) {
     d2d(h->d_maxn + h->cur, c.mnorm + h->d.nk, 1, h->st);
     d2d(h->d_rsl, c.rsl, ng, h->st);
     d2d(h->d_hice, c.hice, ng, h->st);
+    if (h->rot_on && c.rmem) {
+        d2d(h->ra.rs.mem, c.rmem, (size_t)2 * (h->d.nr - 1) * 24, h->st);
+        d2d(h->ra.rs.sc, c.rsc, RS_N, h->st);
+    }
     h->maxn_cur = c.maxn_cur; h->time_yr = c.time_yr; h->time_resp = c.time_resp;
 }
 
@@ -925,7 +944,6 @@ void adj_stash_push(vh_model* h, double dt, double frac, const sle_res& r) {
 
 void adj_check(const vh_model* h) {
     if (h->kind != 2) throw vh_error("adjoint: needs earth_response=ve");
-    if (h->rot_on) throw vh_error("adjoint: rotation feedback is not differentiated (rotation must be off)");
 }
 
 void forward_record(vh_model* h, int ncalls, const double* ice_hist, bool dev, double dt_yr, int every,
@@ -1014,12 +1032,20 @@ void adj_alloc_work(vh_model* h) {
         A->sigh = dalloc<double>(2 * (size_t)d.nlm);
         A->Fh = dalloc<double>((size_t)4 * std::max(d.nmn, 1) * d.nk);
         A->gx = dalloc<double>(ng); A->gb = dalloc<double>(ng); A->gt = dalloc<double>(ng); A->gy = dalloc<double>(ng);
-        A->part = dalloc<double>(d.nlat); A->sc = dalloc<double>(4);
+        A->part = dalloc<double>(2 * (size_t)d.nlat); A->sc = dalloc<double>(4);
         A->F = dalloc<double2>((size_t)d.nlat * (d.L + 1));
         A->q = dalloc<double2>(d.nlm); A->q1 = dalloc<double2>(d.nlm);
         A->pgrad = dalloc<double>((size_t)d.ne * ng);
         A->e3of_mesh = dalloc<int>(d.ne); A->em_of = dalloc<int>(d.ne);
         A->work = true;
+    }
+    if (h->rot_on && !A->ra.mem) {
+        A->ra.mem = dalloc<double>((size_t)2 * (d.nr - 1) * 24);
+        A->ra.dN = dalloc<double>((size_t)2 * 4 * d.nr);
+        A->ra.sc = dalloc<double>(RA_N);
+        A->srh = dalloc<double>(ng);
+        A->rtot = dalloc<double>(2);
+        A->sig_rot = dalloc<double2>(d.nlm);
     }
     // 3-D buffers follow the current lateral set
     if (A->rate_h) { HIPCHECK(hipFree(A->rate_h)); A->rate_h = 0; }
@@ -1091,6 +1117,10 @@ void sle_adjoint(vh_model* h, const adj_sub& S, const double* rh) {
         vh_surface_integral(st, s, A->gb, A->part, A->sc + 1);
         a = g; a.mode = 2; a.x = A->gb; a.I = A->sc + 1; a.y = A->gy;
         vh_sle_adj_grid(st, a);
+        if (h->rot_on) {   // s_rot enters Sraw: its adjoint is P^T b_h (density form y)
+            a = g; a.mode = 6; a.x = A->gy; a.y = A->srh;
+            vh_sle_adj_grid(st, a);
+        }
         anl(A->gy, A->q);
         vh_sle_adj_spec(st, d.nlm, d.nk, 2, nullptr, A->q, A->q1, h->d_gu, h->d_gn, h->g, A->dsurfh);
         syn(A->q, A->gt);
@@ -1104,7 +1134,7 @@ void sle_adjoint(vh_model* h, const adj_sub& S, const double* rh) {
     }
 }
 
-void reverse_sub(vh_model* h, const adj_sub& S, const double* rh) {
+void reverse_sub(vh_model* h, const adj_sub& S, const double* rh, const double2* sig_extra) {
     adj_ctx* A = h->adj;
     hipStream_t st = h->st;
     const vh_dims& d = h->d;
@@ -1136,6 +1166,8 @@ void reverse_sub(vh_model* h, const adj_sub& S, const double* rh) {
     a.e3of = h->d_e3of; a.e1of = A->e1of; a.Xh = A->Xa; a.n6 = h->d_n6; a.Ue = A->Ue; a.mpart = A->mpart;
     h->prof_begin(VH_K_ADJ_ADVANCE);
     vh_advance_adjoint(st, d, a, h->d_mnodes, vh_drift_plan_node_el(h->dplan), A->dNh, A->sigh, A->Kh);
+    if (sig_extra)   // the closing sigma also fed the rotation's load inertia
+        vh_sle_adj_spec(st, d.nlm, d.nk, 4, sig_extra, (double2*)A->sigh, nullptr, h->d_gu, h->d_gn, h->g, nullptr);
     h->prof_end();
     h->prof_begin(VH_K_ADJ_SLE);
     sle_adjoint(h, S, rh);
@@ -1145,6 +1177,38 @@ void reverse_sub(vh_model* h, const adj_sub& S, const double* rh) {
                      A->Fh, th_out);
     h->prof_end();
     A->ah ^= 1;
+}
+
+// The rotation part of a call after its stepper (src/vilma_coupling.f90:399-420),
+// transposed: the channel sub-intervals last first, then the load inertia
+// of the closing sigma -> A->sig_rot (added to the last sub-step's sigma adjoint).
+void rot_call_tail_adjoint(vh_model* h) {
+    adj_ctx* A = h->adj;
+    hipStream_t st = h->st;
+    const vh_dims& d = h->d;
+    const double dt = A->dt_yr * VH_SEC_PER_YEAR;
+    const int n_sub = std::max(1, (int)std::ceil(dt / h->dt_fe_max));
+    const double dt_sub = dt / (double)n_sub;
+    for (int k = n_sub - 1; k >= 0; k--) {
+        rot_commit_adj(st, h->ra, A->ra, dt_sub);
+        rot_begin_adj(st, h->ra, A->ra);
+    }
+    // I = int load gamma (cos, sin), load = Syn(sigma): sigma_h = c_m Anl(gamma (I13_h cos + I23_h sin))
+    rot_inertia_adj(st, A->ra, d.nphi, d.nlat, h->d_sc2, h->d_cphi, h->d_sphi, A->gt);
+    vh_fft_forward(st, h->s, A->gt, d.nlat, A->F);
+    vh_legendre_analysis(st, h->s, A->F, 1, A->q);
+    vh_sle_adj_spec(st, d.nlm, d.nk, 3, A->q, A->sig_rot, nullptr, h->d_gu, h->d_gn, h->g, nullptr);
+    vh_fill(st, A->ra.sc + RA_I, 0.0, 2);
+}
+
+// The rotation part of a call before its stepper (rot_begin, s_rot), transposed,
+// from the call's accumulated s_rot adjoint; leaves m_h for the previous call.
+void rot_call_head_adjoint(vh_model* h) {
+    adj_ctx* A = h->adj;
+    hipStream_t st = h->st;
+    rot_srot_adj(st, h->ra, A->ra, h->s, A->srh, h->d_sc2, h->d_cphi, h->d_sphi, A->part, A->rtot);
+    rot_begin_adj(st, h->ra, A->ra);
+    vh_fill(st, A->srh, 0.0, h->ng());
 }
 
 void backward(vh_model* h, const double* dJ_drsl, bool dev, double* dJ_dpert, double* dJ_dice, bool out_dev) {
@@ -1158,6 +1222,12 @@ void backward(vh_model* h, const double* dJ_drsl, bool dev, double* dJ_dpert, do
     HIPCHECK(hipMemcpyAsync(A->rh, dJ_drsl, (size_t)A->ncalls * ng * sizeof(double),
                             dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
     A->ah = 0;
+    if (h->rot_on) {
+        vh_fill(st, A->ra.mem, 0.0, (size_t)2 * (d.nr - 1) * 24);
+        vh_fill(st, A->ra.dN, 0.0, (size_t)2 * 4 * d.nr);
+        vh_fill(st, A->ra.sc, 0.0, RA_N);
+        vh_fill(st, A->srh, 0.0, ng);
+    }
     vh_fill(st, A->tauh[0], 0.0, adj_ntau(h));
     vh_fill(st, A->Kh, 0.0, std::max(d.nem, 1));
     vh_fill(st, A->iceh, 0.0, (size_t)(A->ncalls + 1) * ng);
@@ -1194,7 +1264,10 @@ void backward(vh_model* h, const double* dJ_drsl, bool dev, double* dJ_dpert, do
         for (size_t i = first + n; i-- > first;) {
             const adj_sub& S = A->sub[i];
             const bool last = i + 1 == first + n || A->sub[i + 1].call != S.call;
-            reverse_sub(h, S, last ? A->rh + (size_t)S.call * ng : nullptr);
+            const bool head = i == first || A->sub[i - 1].call != S.call;
+            if (h->rot_on && last) rot_call_tail_adjoint(h);
+            reverse_sub(h, S, last ? A->rh + (size_t)S.call * ng : nullptr, h->rot_on && last ? A->sig_rot : nullptr);
+            if (h->rot_on && head) rot_call_head_adjoint(h);
         }
         h->sync();
         h->prof_collect();

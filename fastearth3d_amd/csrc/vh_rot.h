@@ -43,3 +43,21 @@ void rot_srot(hipStream_t st, const rot_args& a, int nphi, int nlat, const doubl
 void rot_inertia(hipStream_t st, const rot_args& a, const vh_sht& s, const double* load, const double* sc2,
                  const double* cphi, const double* sphi, double* part);
 void rot_solve_commit(hipStream_t st, const rot_args& a, double dt);
+
+// ---- reverse mode: adjoint state of the rotation channels
+enum { RA_M = 0, RA_DF = 2, RA_DU = 6, RA_I = 10, RA_N = 12 };
+struct rot_adj {
+    double* mem;   // [2][ne][24] adjoint of the channel memory
+    double* dN;    // [2][4][nr]  adjoint of the channel drift (cleared by rot_begin_adj)
+    double* sc;    // [RA_N]: m, dF (4), dU (4), I13/I23
+};
+// one sub-interval of solve_m + commit, transposed (m_h in: adjoint of the m it stores)
+void rot_commit_adj(hipStream_t st, const rot_args& a, const rot_adj& h, double dt);
+// rot_begin transposed: mem_h += (drift)^T (dN_h, dU_h, dF_h), which are cleared
+void rot_begin_adj(hipStream_t st, const rot_args& a, const rot_adj& h);
+// s_rot transposed, srot_h in density form (adjoint / quadrature weight): m_h, dF_h, dU_h (tidal)
+void rot_srot_adj(hipStream_t st, const rot_args& a, const rot_adj& h, const vh_sht& s, const double* srot_h,
+                  const double* sc2, const double* cphi, const double* sphi, double* part, double* tot);
+// g = gamma (I13_h cos + I23_h sin): the load inertia's grid adjoint divided by the quadrature weight
+void rot_inertia_adj(hipStream_t st, const rot_adj& h, int nphi, int nlat, const double* sc2, const double* cphi,
+                     const double* sphi, double* g);

@@ -169,3 +169,158 @@ void rot_solve_commit(hipStream_t st, const rot_args& a, double dt) {
     k_rot_solve_commit<<<1, 256, 0, st>>>(a, dt);
     HIPCHECK(hipGetLastError());
 }
+
+// ------------------------------------------------------------ reverse mode
+// Transposes of the channel kernels above (the rotation path is linear in
+// the channel memory, m, the drift scalars and the load inertia). Adjoint
+// state: rot_adj (vh_rot.h). One block each: the channel problem is tiny.
+
+// solve_m + commit transposed, one sub-interval: mem_h <- (1 - M) mem_h,
+// nodal/drift adjoints of the strain term, then the scalar chain
+// m = (I_t/C-A - dF_tidal/ks)/den, I_t = -(Fe c_load + dF_load), c_load = -a^4 I scl.
+__global__ void __launch_bounds__(256) k_rot_commit_adj(rot_args a, rot_adj h, double dt) {
+    __shared__ double red[4][256];
+    const int ne = a.nr - 1, nr = a.nr;
+    double cacc[4] = {0.0, 0.0, 0.0, 0.0};   // (ch 0 re, ch 0 im, ch 1 re, ch 1 im): adjoints of c_load, m
+    for (int par = 0; par < 2; par++) {      // neighbouring elements share a node: one parity per pass
+        for (int t = threadIdx.x; t < 2 * ne; t += blockDim.x) {
+            const int ch = t / ne, e = t % ne;
+            if ((e & 1) != par) continue;
+            const double Me = a.rs.MkPerDt[(size_t)ch * ne + e] * dt;
+            const double om = 1.0 - Me, two_muM = 2.0 * a.mu[e] * Me;
+            double* mh = h.mem + ((size_t)ch * ne + e) * 24;
+            double Eh[2][3][4];
+            for (int sh = 0; sh < 3; sh++)
+                for (int ri = 0; ri < 2; ri++)
+                    for (int lam = 0; lam < 4; lam++) {
+                        const int c = (sh * 2 + ri) * 4 + lam;
+                        const double v = mh[c];
+                        mh[c] = om * v;
+                        Eh[ri][sh][lam] = -two_muM * v;
+                    }
+            const double* xU = a.rs.xU + (size_t)ch * nr;
+            const double* xV = a.rs.xV + (size_t)ch * nr;
+            double* dNh = h.dN + (size_t)ch * 4 * nr;
+            for (int ri = 0; ri < 2; ri++) {
+                double u1 = 0.0, u2 = 0.0, v1 = 0.0, v2 = 0.0;
+                strain_coeffs_adj(Eh[ri][0], Eh[ri][1], Eh[ri][2], 6.0, u1, u2, v1, v2);
+                // U = c xU + dN (re: row 0/2, im: row 1/3)
+                dNh[(0 + ri) * nr + e] += u1; dNh[(0 + ri) * nr + e + 1] += u2;
+                dNh[(2 + ri) * nr + e] += v1; dNh[(2 + ri) * nr + e + 1] += v2;
+                cacc[2 * ch + ri] += xU[e] * u1 + xU[e + 1] * u2 + xV[e] * v1 + xV[e + 1] * v2;
+            }
+        }
+        __syncthreads();
+    }
+    for (int q = 0; q < 4; q++) red[q][threadIdx.x] = cacc[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < 4; q++) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double* S = a.rs.sc;
+        const double aa = S[RS_A], a4 = aa * aa * aa * aa, scl = 5.0 / (4.0 * VH_PI * VH_G * aa);
+        const double Fe = S[RS_FE_LOAD], CA = S[RS_CMA], ks = S[RS_KS], kTe = S[RS_KTE];
+        const double den = 1.0 - kTe / ks;
+        for (int ri = 0; ri < 2; ri++) {
+            const double mh = red[2 + ri][0] + h.sc[RA_M + ri];           // this sub-interval's m
+            const double clh = red[0 + ri][0] - Fe / (CA * den) * mh;     // its c_load
+            h.sc[RA_I + ri] += -a4 * scl * clh;
+            h.sc[RA_DF + 0 + ri] += -mh / (CA * den);
+            h.sc[RA_DF + 2 + ri] += -mh / (ks * den);
+            h.sc[RA_M + ri] = 0.0;                                        // m before it does not enter
+        }
+    }
+}
+
+// rotation begin transposed: drift adjoints (dN_h at the channel nodes,
+// dU_h, dF_h) -> G2^T -> forcing transpose -> mem_h; the drift adjoints are cleared.
+__global__ void __launch_bounds__(256) k_rot_begin_adj(rot_args a, rot_adj h) {
+    extern __shared__ double F[];   // [2 nmn2][4] forcing adjoints, then nothing else
+    const int nr = a.nr, ne = nr - 1, ncol = 2 * a.nmn2, nrow = ncol + 3;
+    const int rhs = threadIdx.x & 3, grp = threadIdx.x >> 2, ngrp = blockDim.x >> 2;
+    const int ch = rhs >> 1, reim = rhs & 1;
+    for (int c = grp; c < ncol; c += ngrp) {
+        double f = 0.0;
+        for (int row = 0; row < nrow - 1; row++) {
+            double y;
+            if (row < ncol) {
+                const int n = a.mnodes2[row >> 1], uv = row & 1;
+                y = h.dN[((size_t)ch * 4 + uv * 2 + reim) * nr + n];
+            } else if (row == ncol) {
+                y = h.sc[RA_DU + 2 * ch + reim];
+            } else {
+                y = h.sc[RA_DF + 2 * ch + reim];
+            }
+            f = fma(a.G2[(size_t)row * ncol + c], y, f);
+        }
+        F[c * 4 + rhs] = f;
+    }
+    __syncthreads();
+    for (int em = grp; em < a.nem2; em += ngrp) {
+        const int e = a.emap2[em], q = a.mnode2[e];
+        const double fl[4] = {F[(2 * q) * 4 + rhs], F[(2 * q + 2) * 4 + rhs], F[(2 * q + 1) * 4 + rhs],
+                              F[(2 * q + 3) * 4 + rhs]};
+        double A[4], B[4], Cc[4];
+        dissipative_element_adj(a.r[e], a.r[e + 1], 6.0, fl, A, B, Cc);
+        double* mh = h.mem + ((size_t)ch * ne + e) * 24;
+        for (int lam = 0; lam < 4; lam++) {
+            mh[(0 * 2 + reim) * 4 + lam] += A[lam];
+            mh[(1 * 2 + reim) * 4 + lam] += B[lam];
+            mh[(2 * 2 + reim) * 4 + lam] += Cc[lam];
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 2 * 4 * nr; t += blockDim.x) h.dN[t] = 0.0;
+    if (threadIdx.x < 4) { h.sc[RA_DU + threadIdx.x] = 0.0; h.sc[RA_DF + threadIdx.x] = 0.0; }
+}
+
+// rotation_s_rot transposed: tot = sum_ij srot_h gamma_j (cos, sin) -> m_h,
+// dF_h (tidal), dU_h (tidal)
+__global__ void k_rot_srot_adj(rot_args a, rot_adj h, const double* __restrict__ tot) {
+    if (threadIdx.x != 0) return;
+    const double* S = a.rs.sc;
+    const double kTe = S[RS_KTE], Ue = S[RS_UE_TIDAL], Om = S[RS_OMEGA], ar = S[RS_A], g = S[RS_G];
+    const double kN = Om * Om * ar * ar / g, ku = Om * Om * ar * ar;
+    for (int ri = 0; ri < 2; ri++) {
+        const double cNh = kN * tot[ri], cUh = -ku * tot[ri];
+        h.sc[RA_M + ri] += (1.0 + kTe) * cNh + Ue * cUh;
+        h.sc[RA_DF + 2 + ri] += -cNh;
+        h.sc[RA_DU + 2 + ri] += cUh;
+    }
+}
+
+// load-inertia transposed: g_ij = gamma_j (I13_h cos phi_i + I23_h sin phi_i)
+__global__ void k_rot_inertia_adj(rot_adj h, int nphi, int nlat, const double* __restrict__ sc2,
+                                  const double* __restrict__ cphi, const double* __restrict__ sphi, double* g) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nphi * nlat) return;
+    const int j = i / nphi, ip = i % nphi;
+    g[i] = sc2[j] * (h.sc[RA_I] * cphi[ip] + h.sc[RA_I + 1] * sphi[ip]);
+}
+
+void rot_commit_adj(hipStream_t st, const rot_args& a, const rot_adj& h, double dt) {
+    k_rot_commit_adj<<<1, 256, 0, st>>>(a, h, dt);
+    HIPCHECK(hipGetLastError());
+}
+void rot_begin_adj(hipStream_t st, const rot_args& a, const rot_adj& h) {
+    const size_t lds = (size_t)4 * std::max(2 * a.nmn2, 1) * sizeof(double);
+    k_rot_begin_adj<<<1, 256, lds, st>>>(a, h);
+    HIPCHECK(hipGetLastError());
+}
+void rot_srot_adj(hipStream_t st, const rot_args& a, const rot_adj& h, const vh_sht& s, const double* srot_h,
+                  const double* sc2, const double* cphi, const double* sphi, double* part, double* tot) {
+    // sum_ij srot_h gamma (cos, sin) = weighted totals of the density-form rows
+    k_rot_inertia_rows<<<s.nlat, 256, 0, st>>>(s.nphi, s.nlat, srot_h, sc2, cphi, sphi, part);
+    vh_weighted_total(st, s, part, 2, tot);
+    k_rot_srot_adj<<<1, 64, 0, st>>>(a, h, tot);
+    HIPCHECK(hipGetLastError());
+}
+void rot_inertia_adj(hipStream_t st, const rot_adj& h, int nphi, int nlat, const double* sc2, const double* cphi,
+                     const double* sphi, double* g) {
+    const int n = nphi * nlat;
+    k_rot_inertia_adj<<<(n + 255) / 256, 256, 0, st>>>(h, nphi, nlat, sc2, cphi, sphi, g);
+    HIPCHECK(hipGetLastError());
+}

@@ -212,7 +212,7 @@ int vh_tensor_analysis(vh_model* h, const double* dyad, double* c);
  * the first call, then each call's ice). The model is left in the state after
  * the recorded calls. The derivative is the one of the recorded branch: skip
  * gate, coastline masks, sub-step sizes and the 3-D element set are held;
- * needs earth_response = "ve" and rotation off. The _device variants take
+ * needs earth_response = "ve" (the rotation feedback is differentiated). The _device variants take
  * device pointers. */
 int vh_forward_record(vh_model* h, int ncalls, const double* h_ice_hist, double dt_yr, int ckpt_every,
                       double* rsl_out);

@@ -8,7 +8,7 @@ restatement of the reference step (src/vilma_coupling.f90:336-437), so the
 gradient is anchored to the reference's forward map and not only to this
 library's. The derivative is that of the recorded branch (skip gate,
 coastline masks, sub-step counts held), so the cases keep that branch fixed:
-skip gate off (skip_tol = 0), rotation off, a tight sea-level tolerance, no
+skip gate off (skip_tol = 0), a tight sea-level tolerance, no
 perturbation of elements whose lateral spread sits near the 3-D threshold,
 and the accepted sub-step count and 3-D element set are checked to be
 unchanged under the perturbation. The scalar-transform transposition identities that the
@@ -22,10 +22,10 @@ from fastearth3d_amd import configs
 pytestmark = pytest.mark.gpu
 
 
-def _case(lmax, ncalls, seed=0):
+def _case(lmax, ncalls, seed=0, rotation=0):
     c = configs.config(2, lmax=lmax)
     p = c["params"]
-    p.rotation = 0
+    p.rotation = rotation
     p.skip_tol = 0.0
     p.sle_tol = 1e-12
     ice = np.stack([c["ice"](k + 60) for k in range(ncalls)])
@@ -94,18 +94,20 @@ def test_forward_record_matches_updates():
     assert np.all(np.isfinite(gp)) and np.all(np.isfinite(gi))
 
 
-def test_checkpoint_interval_invariance():
-    """Segments re-run from checkpoints reproduce the same sweep as segments kept by the record pass:
-    the gradient depends neither on the interval nor on the store limit."""
-    c, p, ice, wgt, tgt = _case(16, 5)
+@pytest.mark.parametrize("rotation", [0, 1])
+def test_checkpoint_interval_invariance(rotation):
+    """Segments re-run from checkpoints reproduce the sweep of segments kept by the record pass: the
+    gradient depends neither on the interval nor on the store limit. (A re-run can differ from the
+    record in the last bit of rsl, and its sea-level solve then by one iteration: 1e-9 relative bound.)"""
+    c, p, ice, wgt, tgt = _case(16, 5, rotation=rotation)
     a = _gradients(c, p, ice, wgt, tgt, 1)
     b = _gradients(c, p, ice, wgt, tgt, 3)
     d = _gradients(c, p, ice, wgt, tgt, 5)
     e = _gradients(c, p, ice, wgt, tgt, 1, store_gib=0.0)   # every segment but the last re-run
     f = _gradients(c, p, ice, wgt, tgt, 2, store_gib=0.0)
     for x in (b, d, e, f):
-        assert np.array_equal(x[2], a[2])
-        assert np.array_equal(x[3], a[3])
+        assert np.abs(x[2] - a[2]).max() <= 1e-9 * np.abs(a[2]).max()
+        assert np.abs(x[3] - a[3]).max() <= 1e-9 * np.abs(a[3]).max()
 
 
 def _fd_check(make, c, p, ice, wgt, tgt, gp, gi, n_acc, vi, vp, h_ice, h_pert, tol):
@@ -125,17 +127,19 @@ def _fd_check(make, c, p, ice, wgt, tgt, gp, gi, n_acc, vi, vp, h_ice, h_pert, t
     assert abs(ad - fd) <= tol * abs(fd), (ad, fd)
 
 
-def test_gradient_vs_gpu_finite_differences():
+@pytest.mark.parametrize("rotation", [0, 1])
+def test_gradient_vs_gpu_finite_differences(rotation):
     from fastearth3d_amd import lib
-    c, p, ice, wgt, tgt = _case(16, 4)
+    c, p, ice, wgt, tgt = _case(16, 4, rotation=rotation)
     rsl, n_acc, gp, gi, _ = _gradients(c, p, ice, wgt, tgt, 2)
     vi, vp = _directions(c, p, ice, 5)
     _fd_check(lib.SolidEarth, c, p, ice, wgt, tgt, gp, gi, n_acc, vi, vp, 0.1, 1e-4, 1e-6)
 
 
-def test_gradient_vs_oracle_finite_differences(oracle):
+@pytest.mark.parametrize("rotation", [0, 1])
+def test_gradient_vs_oracle_finite_differences(oracle, rotation):
     """Directional derivatives of the oracle's forward (the reference step restated) against the GPU adjoint."""
-    c, p, ice, wgt, tgt = _case(12, 3)
+    c, p, ice, wgt, tgt = _case(12, 3, rotation=rotation)
     rsl, n_acc, gp, gi, _ = _gradients(c, p, ice, wgt, tgt, 2)
     ref, n_ref = _run(oracle.OracleModel, c, p, c["pert"], ice)
     assert n_ref == n_acc
@@ -170,15 +174,16 @@ def test_sht_transposes():
     m.finalize()
 
 
-def test_adjoint_rejects_rotation():
+def test_adjoint_errors():
     from fastearth3d_amd import lib
     from fastearth3d_amd.model import VilmaError
     c = configs.config(1, lmax=12)
     p = c["params"]
-    p.rotation = 1
     m = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"])
-    with pytest.raises(VilmaError, match="rotation"):
-        m.forward_record(np.stack([c["ice"](60)]), c["dt_yr"])
     with pytest.raises(VilmaError, match="no forward record"):
         m.backward(np.zeros((0, m.nlat, m.nphi)))
+    with pytest.raises(VilmaError, match="no forward record"):
+        m.rewind()
+    with pytest.raises(VilmaError, match="positive"):
+        m.forward_record(np.stack([c["ice"](60)]), 0.0)
     m.finalize()
