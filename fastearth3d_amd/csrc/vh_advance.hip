@@ -20,8 +20,8 @@ struct advance_args {
     double* tau_out;
     const double* sigma;   // complex degree-major
     const double* dN;      // [mn][4][nk]
-    double* mnorm_out;
-    unsigned long long* maxbits;
+    double* pmax;          // [gridDim.y][nk] per-block per-slot max |tau_{n+1}|
+    double* pblk;          // [gridDim.y * gridDim.x] per-block max
     const int* e3of;       // [nem] -> 3-D element index or -1
     const double2* cdel;   // [i3*3 + shape][lambda][nlm]
     int nlm;
@@ -30,9 +30,8 @@ struct advance_args {
 // Grid: x = tile of 64 slots (lanes, coalesced rows of tau), y = group of
 // four (Maxwell element, shape) pairs (one per wavefront), so every thread
 // moves 8 memory values (re/im x 4 channels) and the launch has thousands of
-// blocks. The per-slot max |tau_{n+1}| is combined across blocks with an
-// integer atomicMax on the bit pattern (|x| >= 0 orders like its bits), into
-// mnorm_out / the global max, both cleared before the launch.
+// blocks. Each block stores its per-slot max |tau_{n+1}| and its overall max
+// (plain stores); k_advance_max combines them (max is exact in any order).
 __global__ void __launch_bounds__(256) k_advance(advance_args a) {
     __shared__ double red[4][64];
     const int lane = threadIdx.x & 63, wg = threadIdx.x >> 6;
@@ -88,33 +87,62 @@ __global__ void __launch_bounds__(256) k_advance(advance_args a) {
     red[wg][lane] = amax;
     __syncthreads();
     if (wg == 0) {
-        const double m = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
-        if (k < nk) atomicMax((unsigned long long*)a.mnorm_out + k, (unsigned long long)__double_as_longlong(m));
-        red[0][lane] = m;
+        double m = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
+        if (k < nk) a.pmax[(size_t)blockIdx.y * nk + k] = m;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+        if (lane == 0) a.pblk[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = m;
     }
-    __syncthreads();
-    if (threadIdx.x < 32) red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + 32]);
-    __syncthreads();
-    if (threadIdx.x == 0) {
+}
+
+// mnorm[k] = max over the advance's y-blocks of pmax[y][k]; the last block
+// also forms the global max of the per-block maxima.
+__global__ void __launch_bounds__(256) k_advance_max(int nk, int ny, int nblk, const double* __restrict__ pmax,
+                                                     const double* __restrict__ pblk, double* mnorm, double* maxn) {
+    if ((int)blockIdx.x == (int)gridDim.x - 1) {
+        __shared__ double red[256];
         double m = 0.0;
-        for (int i = 0; i < 32; i++) m = fmax(m, red[0][i]);
-        atomicMax(a.maxbits, (unsigned long long)__double_as_longlong(m));
+        for (int i = threadIdx.x; i < nblk; i += blockDim.x) m = fmax(m, pblk[i]);
+        red[threadIdx.x] = m;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) maxn[0] = red[0];
+        return;
     }
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    double m = 0.0;
+    for (int y = 0; y < ny; y++) m = fmax(m, pmax[(size_t)y * nk + k]);
+    mnorm[k] = m;
+}
+
+size_t vh_advance_scratch(const vh_dims& d) {
+    const size_t gx = (d.nk + 63) / 64, gy = (3 * (size_t)d.nem + 3) / 4;
+    return gy * d.nk + gx * gy;
 }
 
 void vh_launch_advance(hipStream_t st, const vh_dims& d, const vh_dev& v, double dt, const double* tau_in,
                        double* tau_out, const double* sigma, const double* dN, double* mnorm_out,
-                       double* maxnorm_out, const int* e3of, const double2* cdel) {
+                       double* maxnorm_out, const int* e3of, const double2* cdel, double* scratch) {
+    if (d.nem == 0) {
+        vh_fill(st, mnorm_out, 0.0, d.nk);
+        vh_fill(st, maxnorm_out, 0.0, 1);
+        return;
+    }
     advance_args a;
     a.e3of = e3of; a.cdel = cdel; a.nlm = d.nlm;
     a.nr = d.nr; a.nk = d.nk; a.nem = d.nem; a.dt = dt; a.v = v;
     a.tau_in = tau_in; a.tau_out = tau_out; a.sigma = sigma; a.dN = dN;
-    a.mnorm_out = mnorm_out; a.maxbits = (unsigned long long*)maxnorm_out;
-    vh_fill(st, mnorm_out, 0.0, d.nk);
-    vh_fill(st, maxnorm_out, 0.0, 1);
-    if (d.nem == 0) return;
     dim3 g((d.nk + 63) / 64, (3 * d.nem + 3) / 4);
+    a.pmax = scratch;
+    a.pblk = scratch + (size_t)g.y * d.nk;
     k_advance<<<g, 256, 0, st>>>(a);
+    HIPCHECK(hipGetLastError());
+    k_advance_max<<<(d.nk + 255) / 256 + 1, 256, 0, st>>>(d.nk, (int)g.y, (int)(g.x * g.y), a.pmax, a.pblk, mnorm_out,
+                                                          maxnorm_out);
     HIPCHECK(hipGetLastError());
 }
 

@@ -181,6 +181,7 @@ struct vh_model {
     double *d_xUn = 0, *d_xVn = 0;
     int *d_ipiv = 0, *d_ipiv1 = 0, *d_fail = 0;
     double *d_tau[2] = {0, 0}, *d_mnorm[2] = {0, 0}, *d_maxn = 0;   // d_maxn[2]
+    double* d_advmax = 0;   // the advance's per-block maxima
     int cur = 0;
     double maxn_cur = 0.0;
     double *d_dN = 0, *d_dsurf = 0;
@@ -274,7 +275,7 @@ vh_model::~vh_model() {
     if (dplan) vh_drift_plan_destroy(dplan);
     void* ptrs[] = {d_r, d_mu, d_rho, d_Rk, d_MkPerDt, d_emap, d_mnodes, d_mnode, d_kdeg, d_is3d, d_ab, d_dr, d_dc,
                     d_ab1, d_null1, d_gu, d_gn, d_gv, d_xUn, d_xVn, d_ipiv, d_ipiv1, d_fail, d_tau[0], d_tau[1],
-                    d_mnorm[0], d_mnorm[1], d_maxn, d_dN, d_dsurf, d_plm_lj, d_dplm_lj, d_plm_jl, d_dplm_jl, d_w,
+                    d_mnorm[0], d_mnorm[1], d_maxn, d_advmax, d_dN, d_dsurf, d_plm_lj, d_dplm_lj, d_plm_jl, d_dplm_jl, d_w,
                     d_st, d_ct, d_mstart, d_tw, d_twp, d_ljobs, d_topo0, d_heq, d_C0ref, d_hice, d_ice1, d_rsl, d_rsl_n, d_C,
                     d_Cn, d_dig, d_C0s, d_ice, d_dice, d_wcorr, d_sraw, d_srot, d_zbed, d_gtmp, d_F, d_FC, d_F2,
                     d_sigma, d_lm, d_uN, d_part, d_sc, d_sbar, d_sc2, d_cphi, d_sphi,
@@ -400,6 +401,7 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
         }
         h->d_maxn = dalloc<double>(2);
         vh_fill(st, h->d_maxn, 0.0, 2);
+        h->d_advmax = dalloc<double>(vh_advance_scratch(d));
         h->d_dN = dalloc<double>((size_t)4 * std::max(d.nmn, 1) * d.nk);
         h->d_dsurf = dalloc<double>((size_t)6 * d.nk);
         vh_fill(st, h->d_dN, 0.0, (size_t)4 * std::max(d.nmn, 1) * d.nk);
@@ -616,7 +618,7 @@ void advance_memory(vh_model* h, const double* sigma, int nxt) {
     }
     h->prof_begin(VH_K_ADVANCE);
     vh_launch_advance(st, d, h->v, h->dt, h->d_tau[h->cur], h->d_tau[nxt], sigma, h->d_dN, h->d_mnorm[nxt],
-                      h->d_maxn + nxt, h->d_e3of, h->d_cdel);
+                      h->d_maxn + nxt, h->d_e3of, h->d_cdel, h->d_advmax);
     h->prof_end();
 }
 
