@@ -17,9 +17,9 @@
 // with the centre-of-mass projection folded into G_1. Each step's drift is
 // then a batched product Y_l = G_l F_l, one workgroup per (degree, tile of
 // orders): the tile's forcing F_l is built in LDS straight from the memory
-// (one coalesced pass over tau) and multiplied by G_l (L2-resident) without
-// a sequential triangular sweep. The result equals the band solve up to
-// rounding (the same factor produced G).
+// (one coalesced pass over tau) and multiplied by G_l (L2-resident) on FP64
+// MFMA, without a sequential triangular sweep. The result equals the band
+// solve up to rounding (the same factor produced G).
 #include <vector>
 #include "vh_internal.h"
 #include "vh_radial_dev.h"
@@ -82,7 +82,6 @@ struct drift_args {
     const int* mnode;
     const int* kdeg;
     const int2* node_el;  // [nmn] (Maxwell element below, above the node) or -1
-    double* Fe;           // [em][4][2][nk] element forcing (floc, re/im)
     const double* tau;
     const double* mnorm;
     const double* maxnorm;
@@ -93,129 +92,155 @@ struct drift_args {
 };
 
 #define DRIFT_THREADS 512
+typedef double vh_d4 __attribute__((ext_vector_type(4)));
 
-// Dissipative forcing of every (slot, Maxwell element): Fe[em][c][reim][k] =
-// floc_c (test dofs U_e, U_e+1, V_e, V_e+1), for the slots the skip gate keeps.
-// Grid: x = 64 slots (coalesced), y = 4 elements.
-__global__ void __launch_bounds__(256) k_drift_force(drift_args a) {
-    const int lane = threadIdx.x & 63, em = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k = blockIdx.x * 64 + lane, nk = a.nk;
-    if (k >= nk || em >= a.nem) return;
-    if (!(a.mnorm[k] > a.skip_tol * a.maxnorm[0])) return;
-    const int l = a.kdeg[k], e = a.emap[em];
+// Forcing of the block's right-hand sides (rhs = reim*ts + (m - m0)) at every
+// memory-node dof, built in LDS straight from the memory: per (Maxwell
+// element, rhs) the dissipative forcing -int tau:delta eps on the element's
+// test dofs U_e, U_e+1, V_e, V_e+1, added at its two nodes. Elements of one
+// parity share no node, so a pass per parity has no conflicting updates; a
+// node then sums its element below and above (in either order: two terms).
+template <int RW>
+__device__ __forceinline__ void drift_forcing(const drift_args& a, int l, int kb, int nsl, double thr, double* F) {
+    constexpr int rw = RW, ts = RW / 2;
+    const int ncol = 2 * a.nmn, nk = a.nk;
+    for (int t = threadIdx.x; t < ncol * rw; t += DRIFT_THREADS) F[t] = 0.0;
+    __syncthreads();
+    const int rhs = threadIdx.x % rw, grp = threadIdx.x / rw, ngrp = DRIFT_THREADS / rw;
+    const int ml = rhs % ts, reim = rhs / ts;
+    const bool act = ml < nsl && a.mnorm[kb + ml] > thr;
+    const int k = kb + ml;
     const double Jr = (double)l * (double)(l + 1);
-    const double* t = a.tau + (size_t)em * 24 * nk + k;
-    double* o = a.Fe + (size_t)em * 8 * nk + k;
+    for (int par = 0; par < 2; par++) {
+        if (act)
+            for (int em = par + 2 * grp; em < a.nem; em += 2 * ngrp) {
+                const int e = a.emap[em];
+                const double* t = a.tau + (size_t)em * 24 * nk + k;
+                double A[4], B[4], Cc[4], floc[4];
 #pragma unroll
-    for (int reim = 0; reim < 2; reim++) {
-        double A[4], B[4], Cc[4], floc[4];
-#pragma unroll
-        for (int lam = 0; lam < 4; lam++) {
-            A[lam] = t[(size_t)((0 * 2 + reim) * 4 + lam) * nk];
-            B[lam] = t[(size_t)((1 * 2 + reim) * 4 + lam) * nk];
-            Cc[lam] = t[(size_t)((2 * 2 + reim) * 4 + lam) * nk];
-        }
-        dissipative_element(a.r[e], a.r[e + 1], Jr, A, B, Cc, floc);
-#pragma unroll
-        for (int c = 0; c < 4; c++) o[(size_t)(c * 2 + reim) * nk] = floc[c];
+                for (int lam = 0; lam < 4; lam++) {
+                    A[lam] = t[(size_t)((0 * 2 + reim) * 4 + lam) * nk];
+                    B[lam] = t[(size_t)((1 * 2 + reim) * 4 + lam) * nk];
+                    Cc[lam] = t[(size_t)((2 * 2 + reim) * 4 + lam) * nk];
+                }
+                dissipative_element(a.r[e], a.r[e + 1], Jr, A, B, Cc, floc);
+                const int qlo = a.mnode[e], qup = a.mnode[e + 1];
+                F[(2 * qlo + 0) * rw + rhs] += floc[0];
+                F[(2 * qup + 0) * rw + rhs] += floc[1];
+                F[(2 * qlo + 1) * rw + rhs] += floc[2];
+                F[(2 * qup + 1) * rw + rhs] += floc[3];
+            }
+        __syncthreads();
     }
 }
 
-// Workgroup = (degree l, ts consecutive orders): rhs = reim*ts + (m - m0);
-// RW = 2 ts right-hand sides (64: one wavefront per row group).
+// one solution row of one rhs -> nodal drift dN or surface dU, dF, dV
+__device__ __forceinline__ void drift_store(const drift_args& a, int l, int k, int reim, int row, double y) {
+    const int ncol = 2 * a.nmn, nk = a.nk;
+    if (row < ncol) {
+        const int q = row >> 1, uv = row & 1;
+        a.dN[(size_t)(q * 4 + uv * 2 + reim) * nk + k] = y;
+    } else {
+        const int s = row - ncol;                      // 0: U, 1: F, 2: V
+        const double val = (s == 1 && l == 1 && !a.deg1_cm) ? 0.0 : y;
+        a.dsurf[(size_t)(s * nk + k) * 2 + reim] = val;
+    }
+}
+
+// Workgroup = (degree l, ts consecutive orders): the forcing in LDS, then
+// Y = G_l F. With RW = 64 the product runs on FP64 MFMA (16x16x4 tiles;
+// lane l: A[l%16][l/16], B[l/16][l%16], D[4r + l/16][l%16]): a wavefront per
+// 16-row tile of G_l, all four 16-column tiles of the 64 right-hand sides,
+// the G_l fragment read once per k-step from L2. Otherwise four rows per
+// thread with scalar loads of G.
 template <int RW>
 __global__ void __launch_bounds__(DRIFT_THREADS) k_drift(drift_args a) {
     extern __shared__ double F[];   // [ncol][rw]
     const int2 job = a.jobs[blockIdx.x];
     const int l = job.x, m0 = job.y;
     constexpr int rw = RW, ts = RW / 2;
-    const int ncol = 2 * a.nmn, nrow = ncol + 3, nk = a.nk;
+    const int ncol = 2 * a.nmn, nrow = ncol + 3;
     const int nsl = min(ts, l + 1 - m0);
     const int kb = l * (l + 1) / 2 - 1 + m0;
     const double thr = a.skip_tol * a.maxnorm[0];
-    const int rhs = threadIdx.x % rw, ngrp = DRIFT_THREADS / rw;
-    // with RW = 64 the row group is the wavefront index: uniform, so the G rows
-    // below stream through scalar loads instead of wave-wide vector loads
-    const int grp = RW == 64 ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : threadIdx.x / rw;
-    const int ml = rhs % ts, reim = rhs / ts;
-    const bool valid = ml < nsl;
-    const bool act = valid && a.mnorm[kb + ml] > thr;
-    // forcing at node q, dof uv: the upper-node term of the element below it
-    // plus the lower-node term of the element above it (assembly order of
-    // the reference: the element below comes first, src/vilma_viscoelastic.f90:441-480)
-    for (int c = grp; c < ncol; c += ngrp) {
-        double f = 0.0;
-        if (act) {
-            const int q = c >> 1, uv = c & 1;
-            const int2 ne = a.node_el[q];
-            const size_t kk = (size_t)reim * nk + kb + ml;
-            if (ne.x >= 0) f = f + a.Fe[((size_t)ne.x * 4 + (uv ? 3 : 1)) * 2 * nk + kk];
-            if (ne.y >= 0) f = f + a.Fe[((size_t)ne.y * 4 + (uv ? 2 : 0)) * 2 * nk + kk];
-        }
-        F[c * rw + rhs] = f;
-    }
-    __syncthreads();
-    // Y = G_l F, four rows per thread per pass
+    drift_forcing<RW>(a, l, kb, nsl, thr, F);
     const double* __restrict__ G = a.G + (size_t)(l - 1) * nrow * ncol;
-    const int nq = (nrow + 3) / 4;
-    const int k = kb + ml;
-    // G is read-only for the whole launch: read it through the constant
-    // address space, so the wave-uniform rows become scalar loads
-    typedef const __attribute__((address_space(4))) double cdouble;
-    for (int qi = grp; qi < nq; qi += ngrp) {
-        const int r0 = 4 * qi;
-        cdouble* g0 = (cdouble*)(G + (size_t)min(r0 + 0, nrow - 1) * ncol);
-        cdouble* g1 = (cdouble*)(G + (size_t)min(r0 + 1, nrow - 1) * ncol);
-        cdouble* g2 = (cdouble*)(G + (size_t)min(r0 + 2, nrow - 1) * ncol);
-        cdouble* g3 = (cdouble*)(G + (size_t)min(r0 + 3, nrow - 1) * ncol);
-        double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
-#pragma unroll 4
-        for (int c = 0; c < ncol; c++) {
-            const double f = F[c * rw + rhs];
-            y0 = fma(g0[c], f, y0);
-            y1 = fma(g1[c], f, y1);
-            y2 = fma(g2[c], f, y2);
-            y3 = fma(g3[c], f, y3);
-        }
-        if (!valid) continue;
-        const double y[4] = {y0, y1, y2, y3};
+    if constexpr (RW == 64) {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int nrt = (nrow + 15) / 16;
+        for (int rt = wave; rt < nrt; rt += DRIFT_THREADS / 64) {
+            vh_d4 acc[4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int row = r0 + i;
-            if (row < ncol) {
-                const int q = row >> 1, uv = row & 1;
-                a.dN[(size_t)(q * 4 + uv * 2 + reim) * nk + k] = y[i];
-            } else if (row < nrow) {
-                const int s = row - ncol;                      // 0: U, 1: F, 2: V
-                double val = (s == 1 && l == 1 && !a.deg1_cm) ? 0.0 : y[i];
-                a.dsurf[(size_t)(s * nk + k) * 2 + reim] = val;
+            for (int ct = 0; ct < 4; ct++) acc[ct] = vh_d4{0.0, 0.0, 0.0, 0.0};
+            const int arow = rt * 16 + (lane & 15);
+            const double* Grow = G + (size_t)min(arow, nrow - 1) * ncol;
+            for (int k0 = 0; k0 < ncol; k0 += 4) {
+                const int kk = k0 + (lane >> 4);
+                const bool kok = kk < ncol;
+                const double av = (arow < nrow && kok) ? Grow[kk] : 0.0;
+#pragma unroll
+                for (int ct = 0; ct < 4; ct++) {
+                    const double bv = kok ? F[kk * 64 + ct * 16 + (lane & 15)] : 0.0;
+                    acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[ct], 0, 0, 0);
+                }
             }
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int row = rt * 16 + 4 * r + (lane >> 4), rhs = ct * 16 + (lane & 15);
+                    const int ml = rhs % ts, reim = rhs / ts;
+                    if (row < nrow && ml < nsl) drift_store(a, l, kb + ml, reim, row, acc[ct][r]);
+                }
+        }
+    } else {
+        const int rhs = threadIdx.x % rw, ngrp = DRIFT_THREADS / rw, grp = threadIdx.x / rw;
+        const int ml = rhs % ts, reim = rhs / ts;
+        const bool valid = ml < nsl;
+        const int nq = (nrow + 3) / 4;
+        for (int qi = grp; qi < nq; qi += ngrp) {
+            const int r0 = 4 * qi;
+            const double* g0 = G + (size_t)min(r0 + 0, nrow - 1) * ncol;
+            const double* g1 = G + (size_t)min(r0 + 1, nrow - 1) * ncol;
+            const double* g2 = G + (size_t)min(r0 + 2, nrow - 1) * ncol;
+            const double* g3 = G + (size_t)min(r0 + 3, nrow - 1) * ncol;
+            double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
+            for (int c = 0; c < ncol; c++) {
+                const double f = F[c * rw + rhs];
+                y0 = fma(g0[c], f, y0);
+                y1 = fma(g1[c], f, y1);
+                y2 = fma(g2[c], f, y2);
+                y3 = fma(g3[c], f, y3);
+            }
+            if (!valid) continue;
+            const double y[4] = {y0, y1, y2, y3};
+            for (int i = 0; i < 4; i++)
+                if (r0 + i < nrow) drift_store(a, l, kb + ml, reim, r0 + i, y[i]);
         }
     }
 }
 
 // ------------------------------------------------------------------ adjoint
-// Transpose of the two kernels above for the same skip-gate set (slots the
-// gate drops have a zero drift, so they pass no adjoint back).
-//   k_drift_gT:        Fh = G_l^T [dN_h; dsurf_h] per (slot, re/im),
-//                      Fh[(c*2 + reim)][nk], c = 2 q + uv the node dof;
-//   k_drift_force_adj: tau_h += (forcing)^T of the element's four test dofs.
+// Transpose of k_drift for the same skip-gate set (slots the gate drops have
+// a zero drift, so they pass no adjoint back): Y = [dN_h; dsurf_h] of the
+// block's right-hand sides in LDS, Fh = G_l^T Y (FP64 MFMA for RW = 64), then
+// per (Maxwell element, rhs) the forcing transpose onto the element's memory,
+// tau_h += (forcing)^T of its four test dofs (the block owns its slots).
 template <int RW>
-__global__ void __launch_bounds__(DRIFT_THREADS) k_drift_gT(drift_args a, const double* __restrict__ dNh,
-                                                           const double* __restrict__ dsurfh, double* Fh) {
-    extern __shared__ double Y[];   // [nrow][rw]
+__global__ void __launch_bounds__(DRIFT_THREADS) k_drift_adj(drift_args a, const double* __restrict__ dNh,
+                                                             const double* __restrict__ dsurfh, double* tauh) {
+    extern __shared__ double Y[];   // [nrow][rw], then Fh [ncol][rw]
     const int2 job = a.jobs[blockIdx.x];
     const int l = job.x, m0 = job.y;
     constexpr int rw = RW, ts = RW / 2;
     const int ncol = 2 * a.nmn, nrow = ncol + 3, nk = a.nk;
+    double* Fh = Y + (size_t)nrow * rw;
     const int nsl = min(ts, l + 1 - m0);
     const int kb = l * (l + 1) / 2 - 1 + m0;
     const double thr = a.skip_tol * a.maxnorm[0];
-    const int rhs = threadIdx.x % rw, ngrp = DRIFT_THREADS / rw;
-    const int grp = RW == 64 ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : threadIdx.x / rw;
+    const int rhs = threadIdx.x % rw, grp = threadIdx.x / rw, ngrp = DRIFT_THREADS / rw;
     const int ml = rhs % ts, reim = rhs / ts;
-    const bool valid = ml < nsl;
-    const bool act = valid && a.mnorm[kb + ml] > thr;
+    const bool act = ml < nsl && a.mnorm[kb + ml] > thr;
     const int k = kb + ml;
     for (int row = grp; row < nrow; row += ngrp) {
         double y = 0.0;
@@ -232,47 +257,51 @@ __global__ void __launch_bounds__(DRIFT_THREADS) k_drift_gT(drift_args a, const 
     }
     __syncthreads();
     const double* __restrict__ G = a.G + (size_t)(l - 1) * nrow * ncol;
-    typedef const __attribute__((address_space(4))) double cdouble;
-    const int nq = (ncol + 3) / 4;
-    for (int qi = grp; qi < nq; qi += ngrp) {
-        const int c0 = 4 * qi;
-        const int i0 = min(c0 + 0, ncol - 1), i1 = min(c0 + 1, ncol - 1), i2 = min(c0 + 2, ncol - 1),
-                  i3 = min(c0 + 3, ncol - 1);
-        double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0;
-#pragma unroll 4
-        for (int row = 0; row < nrow; row++) {
-            cdouble* g = (cdouble*)(G + (size_t)row * ncol);
-            const double y = Y[row * rw + rhs];
-            f0 = fma(g[i0], y, f0);
-            f1 = fma(g[i1], y, f1);
-            f2 = fma(g[i2], y, f2);
-            f3 = fma(g[i3], y, f3);
+    if constexpr (RW == 64) {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int nct = (ncol + 15) / 16;
+        for (int rt = wave; rt < nct; rt += DRIFT_THREADS / 64) {
+            vh_d4 acc[4];
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++) acc[ct] = vh_d4{0.0, 0.0, 0.0, 0.0};
+            const int ac = rt * 16 + (lane & 15);   // A row = a column of G_l
+            for (int k0 = 0; k0 < nrow; k0 += 4) {
+                const int kk = k0 + (lane >> 4);    // A column = a row of G_l
+                const bool kok = kk < nrow;
+                const double av = (ac < ncol && kok) ? G[(size_t)kk * ncol + ac] : 0.0;
+#pragma unroll
+                for (int ct = 0; ct < 4; ct++) {
+                    const double bv = kok ? Y[kk * 64 + ct * 16 + (lane & 15)] : 0.0;
+                    acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[ct], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int c = rt * 16 + 4 * r + (lane >> 4);
+                    if (c < ncol) Fh[c * 64 + ct * 16 + (lane & 15)] = acc[ct][r];
+                }
         }
-        if (!valid) continue;
-        const double f[4] = {f0, f1, f2, f3};
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-            if (c0 + i < ncol) Fh[((size_t)(c0 + i) * 2 + reim) * nk + k] = f[i];
+    } else {
+        for (int c = grp; c < ncol; c += ngrp) {
+            double f = 0.0;
+            for (int row = 0; row < nrow; row++) f = fma(G[(size_t)row * ncol + c], Y[row * rw + rhs], f);
+            Fh[c * rw + rhs] = f;
+        }
     }
-}
-
-__global__ void __launch_bounds__(256) k_drift_force_adj(drift_args a, const double* __restrict__ Fh,
-                                                         double* tauh) {
-    const int lane = threadIdx.x & 63, em = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k = blockIdx.x * 64 + lane, nk = a.nk;
-    if (k >= nk || em >= a.nem) return;
-    if (!(a.mnorm[k] > a.skip_tol * a.maxnorm[0])) return;
-    const int l = a.kdeg[k], e = a.emap[em];
+    __syncthreads();
+    if (!act) return;
     const double Jr = (double)l * (double)(l + 1);
-    const int qlo = a.mnode[e], qup = a.mnode[e + 1];
-    double* t = tauh + (size_t)em * 24 * nk + k;
-#pragma unroll
-    for (int reim = 0; reim < 2; reim++) {
+    for (int em = grp; em < a.nem; em += ngrp) {
+        const int e = a.emap[em];
+        const int qlo = a.mnode[e], qup = a.mnode[e + 1];
         // test dofs U_e, U_e+1, V_e, V_e+1 <- (lower, U), (upper, U), (lower, V), (upper, V)
-        const double fl[4] = {Fh[((size_t)(2 * qlo + 0) * 2 + reim) * nk + k], Fh[((size_t)(2 * qup + 0) * 2 + reim) * nk + k],
-                              Fh[((size_t)(2 * qlo + 1) * 2 + reim) * nk + k], Fh[((size_t)(2 * qup + 1) * 2 + reim) * nk + k]};
+        const double fl[4] = {Fh[(2 * qlo + 0) * rw + rhs], Fh[(2 * qup + 0) * rw + rhs],
+                              Fh[(2 * qlo + 1) * rw + rhs], Fh[(2 * qup + 1) * rw + rhs]};
         double A[4], B[4], Cc[4];
         dissipative_element_adj(a.r[e], a.r[e + 1], Jr, fl, A, B, Cc);
+        double* t = tauh + (size_t)em * 24 * nk + k;
 #pragma unroll
         for (int lam = 0; lam < 4; lam++) {
             t[(size_t)((0 * 2 + reim) * 4 + lam) * nk] += A[lam];
@@ -286,7 +315,6 @@ __global__ void __launch_bounds__(256) k_drift_force_adj(drift_args a, const dou
 struct vh_drift_plan {
     int2* jobs = nullptr;
     int2* node_el = nullptr;
-    double* Fe = nullptr;
     double* G = nullptr;
     int njobs = 0, ts = 32, rw = 64;
     size_t lds = 0, lds_t = 0;
@@ -306,7 +334,6 @@ vh_drift_plan* vh_drift_plan_create(hipStream_t st, const vh_dims& d, const vh_d
         }
         HIPCHECK(hipMalloc((void**)&p->node_el, ne.size() * sizeof(int2)));
         HIPCHECK(hipMemcpy(p->node_el, ne.data(), ne.size() * sizeof(int2), hipMemcpyHostToDevice));
-        HIPCHECK(hipMalloc((void**)&p->Fe, (size_t)std::max(d.nem, 1) * 8 * d.nk * sizeof(double)));
     }
     const int ncol = 2 * d.nmn, nrow = ncol + 3;
     // right-hand sides per workgroup: as many as fit a 64 KiB forcing tile
@@ -333,16 +360,16 @@ vh_drift_plan* vh_drift_plan_create(hipStream_t st, const vh_dims& d, const vh_d
         HIPCHECK(hipStreamSynchronize(st));
         HIPCHECK(hipFree(w1));
     }
-    p->lds_t = (size_t)(std::max(ncol, 1) + 3) * rw * 8;
+    p->lds_t = (size_t)(2 * std::max(ncol, 1) + 3) * rw * 8;
     if (p->lds_t > 160 * 1024)
         throw vh_error("drift adjoint: too many memory nodes for the LDS tile (nmn = " + std::to_string(d.nmn) + ")");
     switch (p->rw) {
-        case 64: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_gT<64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
-        case 32: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_gT<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
-        case 16: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_gT<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
-        case 8: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_gT<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
-        case 4: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_gT<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
-        default: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_gT<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
+        case 64: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_adj<64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
+        case 32: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_adj<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
+        case 16: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_adj<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
+        case 8: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_adj<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
+        case 4: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_adj<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
+        default: HIPCHECK(hipFuncSetAttribute((const void*)k_drift_adj<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_t)); break;
     }
     switch (p->rw) {
         case 64: HIPCHECK(hipFuncSetAttribute((const void*)k_drift<64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds)); break;
@@ -360,7 +387,6 @@ void vh_drift_plan_destroy(vh_drift_plan* p) {
     (void)hipFree(p->jobs);
     (void)hipFree(p->G);
     (void)hipFree(p->node_el);
-    (void)hipFree(p->Fe);
     delete p;
 }
 
@@ -370,14 +396,9 @@ void vh_drift_run(hipStream_t st, const vh_drift_plan* p, const vh_dims& d, cons
     drift_args a;
     a.nk = d.nk; a.nem = d.nem; a.nmn = d.nmn; a.ts = p->ts; a.rw = p->rw;
     a.jobs = p->jobs; a.G = p->G; a.r = v.r; a.emap = v.emap; a.mnode = v.mnode; a.kdeg = v.kdeg;
-    a.node_el = p->node_el; a.Fe = p->Fe;
+    a.node_el = p->node_el;
     a.tau = tau; a.mnorm = mnorm; a.maxnorm = maxnorm;
     a.skip_tol = skip_tol; a.deg1_cm = deg1_cm; a.dN = dN; a.dsurf = dsurf;
-    if (d.nem > 0) {
-        dim3 g((d.nk + 63) / 64, (d.nem + 3) / 4);
-        k_drift_force<<<g, 256, 0, st>>>(a);
-        HIPCHECK(hipGetLastError());
-    }
     switch (p->rw) {
         case 64: k_drift<64><<<p->njobs, DRIFT_THREADS, p->lds, st>>>(a); break;
         case 32: k_drift<32><<<p->njobs, DRIFT_THREADS, p->lds, st>>>(a); break;
@@ -392,23 +413,21 @@ void vh_drift_run(hipStream_t st, const vh_drift_plan* p, const vh_dims& d, cons
 void vh_drift_adjoint(hipStream_t st, const vh_drift_plan* p, const vh_dims& d, const vh_dev& v,
                       const double* mnorm, const double* maxnorm, double skip_tol, int deg1_cm,
                       const double* dNh, const double* dsurfh, double* Fh, double* tauh) {
+    (void)Fh;
     if (d.nem == 0) return;
     drift_args a;
     a.nk = d.nk; a.nem = d.nem; a.nmn = d.nmn; a.ts = p->ts; a.rw = p->rw;
     a.jobs = p->jobs; a.G = p->G; a.r = v.r; a.emap = v.emap; a.mnode = v.mnode; a.kdeg = v.kdeg;
-    a.node_el = p->node_el; a.Fe = p->Fe;
+    a.node_el = p->node_el;
     a.tau = nullptr; a.mnorm = mnorm; a.maxnorm = maxnorm;
     a.skip_tol = skip_tol; a.deg1_cm = deg1_cm; a.dN = nullptr; a.dsurf = nullptr;
     switch (p->rw) {
-        case 64: k_drift_gT<64><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, Fh); break;
-        case 32: k_drift_gT<32><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, Fh); break;
-        case 16: k_drift_gT<16><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, Fh); break;
-        case 8: k_drift_gT<8><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, Fh); break;
-        case 4: k_drift_gT<4><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, Fh); break;
-        default: k_drift_gT<2><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, Fh); break;
+        case 64: k_drift_adj<64><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, tauh); break;
+        case 32: k_drift_adj<32><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, tauh); break;
+        case 16: k_drift_adj<16><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, tauh); break;
+        case 8: k_drift_adj<8><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, tauh); break;
+        case 4: k_drift_adj<4><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, tauh); break;
+        default: k_drift_adj<2><<<p->njobs, DRIFT_THREADS, p->lds_t, st>>>(a, dNh, dsurfh, tauh); break;
     }
-    HIPCHECK(hipGetLastError());
-    dim3 g((d.nk + 63) / 64, (d.nem + 3) / 4);
-    k_drift_force_adj<<<g, 256, 0, st>>>(a, Fh, tauh);
     HIPCHECK(hipGetLastError());
 }
