@@ -82,6 +82,7 @@ struct drift_args {
     const int* mnode;
     const int* kdeg;
     const int2* node_el;  // [nmn] (Maxwell element below, above the node) or -1
+    const double* Kt;     // [l-1][em][4][12] forcing map of (element, degree): floc = Kt (A, B, C)
     const double* tau;
     const double* mnorm;
     const double* maxnorm;
@@ -111,24 +112,42 @@ __device__ __forceinline__ void drift_forcing(const drift_args& a, int l, int kb
     const bool act = ml < nsl && a.mnorm[kb + ml] > thr;
     const int k = kb + ml;
     const double Jr = (double)l * (double)(l + 1);
+    constexpr int U = 3;   // elements whose memory a thread has in flight at once
     for (int par = 0; par < 2; par++) {
         if (act)
-            for (int em = par + 2 * grp; em < a.nem; em += 2 * ngrp) {
-                const int e = a.emap[em];
-                const double* t = a.tau + (size_t)em * 24 * nk + k;
-                double A[4], B[4], Cc[4], floc[4];
+            for (int em0 = par + 2 * grp; em0 < a.nem; em0 += 2 * ngrp * U) {
+                double v[U][12];
 #pragma unroll
-                for (int lam = 0; lam < 4; lam++) {
-                    A[lam] = t[(size_t)((0 * 2 + reim) * 4 + lam) * nk];
-                    B[lam] = t[(size_t)((1 * 2 + reim) * 4 + lam) * nk];
-                    Cc[lam] = t[(size_t)((2 * 2 + reim) * 4 + lam) * nk];
+                for (int u = 0; u < U; u++) {
+                    const int em = em0 + 2 * ngrp * u;
+                    const double* t = a.tau + (size_t)min(em, a.nem - 1) * 24 * nk + k;
+#pragma unroll
+                    for (int q = 0; q < 12; q++) {   // (shape, lambda) of this re/im half
+                        const int sh = q >> 2, lam = q & 3;
+                        v[u][q] = em < a.nem ? t[(size_t)((sh * 2 + reim) * 4 + lam) * nk] : 0.0;
+                    }
                 }
-                dissipative_element(a.r[e], a.r[e + 1], Jr, A, B, Cc, floc);
-                const int qlo = a.mnode[e], qup = a.mnode[e + 1];
-                F[(2 * qlo + 0) * rw + rhs] += floc[0];
-                F[(2 * qup + 0) * rw + rhs] += floc[1];
-                F[(2 * qlo + 1) * rw + rhs] += floc[2];
-                F[(2 * qup + 1) * rw + rhs] += floc[3];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int em = em0 + 2 * ngrp * u;
+                    if (em >= a.nem) break;
+                    const int e = a.emap[em];
+                    typedef const __attribute__((address_space(4))) double cdouble;
+                    cdouble* K = (cdouble*)(a.Kt + ((size_t)(l - 1) * a.nem + em) * 48);
+                    double floc[4];
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        double f = 0.0;
+#pragma unroll
+                        for (int j = 0; j < 12; j++) f = fma(K[t * 12 + j], v[u][j], f);
+                        floc[t] = f;
+                    }
+                    const int qlo = a.mnode[e], qup = a.mnode[e + 1];
+                    F[(2 * qlo + 0) * rw + rhs] += floc[0];
+                    F[(2 * qup + 0) * rw + rhs] += floc[1];
+                    F[(2 * qlo + 1) * rw + rhs] += floc[2];
+                    F[(2 * qup + 1) * rw + rhs] += floc[3];
+                }
             }
         __syncthreads();
     }
@@ -174,14 +193,22 @@ __global__ void __launch_bounds__(DRIFT_THREADS) k_drift(drift_args a) {
             for (int ct = 0; ct < 4; ct++) acc[ct] = vh_d4{0.0, 0.0, 0.0, 0.0};
             const int arow = rt * 16 + (lane & 15);
             const double* Grow = G + (size_t)min(arow, nrow - 1) * ncol;
-            for (int k0 = 0; k0 < ncol; k0 += 4) {
-                const int kk = k0 + (lane >> 4);
+            // the wavefront's whole G_l fragment in flight first (ncol <= 128 with RW = 64)
+            double av[32];
+#pragma unroll
+            for (int ks = 0; ks < 32; ks++) {
+                const int kk = 4 * ks + (lane >> 4);
+                av[ks] = (arow < nrow && kk < ncol) ? Grow[kk] : 0.0;
+            }
+#pragma unroll
+            for (int ks = 0; ks < 32; ks++) {
+                if (4 * ks >= ncol) break;
+                const int kk = 4 * ks + (lane >> 4);
                 const bool kok = kk < ncol;
-                const double av = (arow < nrow && kok) ? Grow[kk] : 0.0;
 #pragma unroll
                 for (int ct = 0; ct < 4; ct++) {
                     const double bv = kok ? F[kk * 64 + ct * 16 + (lane & 15)] : 0.0;
-                    acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[ct], 0, 0, 0);
+                    acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], bv, acc[ct], 0, 0, 0);
                 }
             }
 #pragma unroll
@@ -265,14 +292,21 @@ __global__ void __launch_bounds__(DRIFT_THREADS) k_drift_adj(drift_args a, const
 #pragma unroll
             for (int ct = 0; ct < 4; ct++) acc[ct] = vh_d4{0.0, 0.0, 0.0, 0.0};
             const int ac = rt * 16 + (lane & 15);   // A row = a column of G_l
-            for (int k0 = 0; k0 < nrow; k0 += 4) {
-                const int kk = k0 + (lane >> 4);    // A column = a row of G_l
+            double av[33];                          // A column = a row of G_l (nrow <= 131)
+#pragma unroll
+            for (int ks = 0; ks < 33; ks++) {
+                const int kk = 4 * ks + (lane >> 4);
+                av[ks] = (ac < ncol && kk < nrow) ? G[(size_t)kk * ncol + ac] : 0.0;
+            }
+#pragma unroll
+            for (int ks = 0; ks < 33; ks++) {
+                if (4 * ks >= nrow) break;
+                const int kk = 4 * ks + (lane >> 4);
                 const bool kok = kk < nrow;
-                const double av = (ac < ncol && kok) ? G[(size_t)kk * ncol + ac] : 0.0;
 #pragma unroll
                 for (int ct = 0; ct < 4; ct++) {
                     const double bv = kok ? Y[kk * 64 + ct * 16 + (lane & 15)] : 0.0;
-                    acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[ct], 0, 0, 0);
+                    acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], bv, acc[ct], 0, 0, 0);
                 }
             }
 #pragma unroll
@@ -292,29 +326,45 @@ __global__ void __launch_bounds__(DRIFT_THREADS) k_drift_adj(drift_args a, const
     }
     __syncthreads();
     if (!act) return;
-    const double Jr = (double)l * (double)(l + 1);
     for (int em = grp; em < a.nem; em += ngrp) {
         const int e = a.emap[em];
         const int qlo = a.mnode[e], qup = a.mnode[e + 1];
         // test dofs U_e, U_e+1, V_e, V_e+1 <- (lower, U), (upper, U), (lower, V), (upper, V)
         const double fl[4] = {Fh[(2 * qlo + 0) * rw + rhs], Fh[(2 * qup + 0) * rw + rhs],
                               Fh[(2 * qlo + 1) * rw + rhs], Fh[(2 * qup + 1) * rw + rhs]};
-        double A[4], B[4], Cc[4];
-        dissipative_element_adj(a.r[e], a.r[e + 1], Jr, fl, A, B, Cc);
+        typedef const __attribute__((address_space(4))) double cdouble;
+        cdouble* K = (cdouble*)(a.Kt + ((size_t)(l - 1) * a.nem + em) * 48);
         double* t = tauh + (size_t)em * 24 * nk + k;
 #pragma unroll
-        for (int lam = 0; lam < 4; lam++) {
-            t[(size_t)((0 * 2 + reim) * 4 + lam) * nk] += A[lam];
-            t[(size_t)((1 * 2 + reim) * 4 + lam) * nk] += B[lam];
-            t[(size_t)((2 * 2 + reim) * 4 + lam) * nk] += Cc[lam];
+        for (int j = 0; j < 12; j++) {   // (shape, lambda) of this re/im half
+            const double g = fma(K[j], fl[0], fma(K[12 + j], fl[1], fma(K[24 + j], fl[2], K[36 + j] * fl[3])));
+            const int sh = j >> 2, lam = j & 3;
+            t[(size_t)((sh * 2 + reim) * 4 + lam) * nk] += g;
         }
     }
+}
+
+// The dissipative forcing of (element, degree) is linear in the element's
+// 12 memory values of one re/im half: its 4 x 12 matrix, column j the forcing
+// of the unit memory e_j (the same element integrals, evaluated once here).
+__global__ void k_drift_ktab(int L, int nem, const double* __restrict__ r, const int* __restrict__ emap, double* Kt) {
+    const int em = blockIdx.x, j = threadIdx.x % 12, lo = threadIdx.x / 12;
+    const int l = blockIdx.y * (blockDim.x / 12) + lo + 1;
+    if (threadIdx.x >= (blockDim.x / 12) * 12 || l > L || em >= nem) return;
+    const int e = emap[em];
+    const double Jr = (double)l * (double)(l + 1);
+    double A[4] = {0.0, 0.0, 0.0, 0.0}, B[4] = {0.0, 0.0, 0.0, 0.0}, Cc[4] = {0.0, 0.0, 0.0, 0.0}, f[4];
+    if (j < 4) A[j] = 1.0; else if (j < 8) B[j - 4] = 1.0; else Cc[j - 8] = 1.0;
+    dissipative_element(r[e], r[e + 1], Jr, A, B, Cc, f);
+    double* K = Kt + ((size_t)(l - 1) * nem + em) * 48;
+    for (int t = 0; t < 4; t++) K[t * 12 + j] = f[t];
 }
 
 // ------------------------------------------------------------------ plan
 struct vh_drift_plan {
     int2* jobs = nullptr;
     int2* node_el = nullptr;
+    double* Kt = nullptr;
     double* G = nullptr;
     int njobs = 0, ts = 32, rw = 64;
     size_t lds = 0, lds_t = 0;
@@ -335,13 +385,19 @@ vh_drift_plan* vh_drift_plan_create(hipStream_t st, const vh_dims& d, const vh_d
         HIPCHECK(hipMalloc((void**)&p->node_el, ne.size() * sizeof(int2)));
         HIPCHECK(hipMemcpy(p->node_el, ne.data(), ne.size() * sizeof(int2), hipMemcpyHostToDevice));
     }
+    if (d.nem > 0) {
+        HIPCHECK(hipMalloc((void**)&p->Kt, (size_t)d.L * d.nem * 48 * sizeof(double)));
+        dim3 g(d.nem, (d.L + 20) / 21);
+        k_drift_ktab<<<g, 252, 0, st>>>(d.L, d.nem, v.r, v.emap, p->Kt);
+        HIPCHECK(hipGetLastError());
+    }
     const int ncol = 2 * d.nmn, nrow = ncol + 3;
     // right-hand sides per workgroup: as many as fit a 64 KiB forcing tile
     int rw = 64;
     while (rw > 2 && (size_t)std::max(ncol, 1) * rw * 8 > 64 * 1024) rw /= 2;
     if ((size_t)std::max(ncol, 1) * rw * 8 > 160 * 1024)
         throw vh_error("drift: too many memory nodes for the LDS forcing tile (nmn = " + std::to_string(d.nmn) + ")");
-    p->rw = rw; p->ts = rw / 2;
+    p->rw = rw; p->ts = rw / 2;   // rw = 64 implies ncol <= 128 (the MFMA paths hold G fragments for that)
     p->lds = (size_t)std::max(ncol, 1) * rw * 8;
     std::vector<int2> jobs;
     for (int l = 1; l <= d.L; l++)
@@ -387,6 +443,7 @@ void vh_drift_plan_destroy(vh_drift_plan* p) {
     (void)hipFree(p->jobs);
     (void)hipFree(p->G);
     (void)hipFree(p->node_el);
+    (void)hipFree(p->Kt);
     delete p;
 }
 
@@ -396,7 +453,7 @@ void vh_drift_run(hipStream_t st, const vh_drift_plan* p, const vh_dims& d, cons
     drift_args a;
     a.nk = d.nk; a.nem = d.nem; a.nmn = d.nmn; a.ts = p->ts; a.rw = p->rw;
     a.jobs = p->jobs; a.G = p->G; a.r = v.r; a.emap = v.emap; a.mnode = v.mnode; a.kdeg = v.kdeg;
-    a.node_el = p->node_el;
+    a.node_el = p->node_el; a.Kt = p->Kt;
     a.tau = tau; a.mnorm = mnorm; a.maxnorm = maxnorm;
     a.skip_tol = skip_tol; a.deg1_cm = deg1_cm; a.dN = dN; a.dsurf = dsurf;
     switch (p->rw) {
@@ -418,7 +475,7 @@ void vh_drift_adjoint(hipStream_t st, const vh_drift_plan* p, const vh_dims& d, 
     drift_args a;
     a.nk = d.nk; a.nem = d.nem; a.nmn = d.nmn; a.ts = p->ts; a.rw = p->rw;
     a.jobs = p->jobs; a.G = p->G; a.r = v.r; a.emap = v.emap; a.mnode = v.mnode; a.kdeg = v.kdeg;
-    a.node_el = p->node_el;
+    a.node_el = p->node_el; a.Kt = p->Kt;
     a.tau = nullptr; a.mnorm = mnorm; a.maxnorm = maxnorm;
     a.skip_tol = skip_tol; a.deg1_cm = deg1_cm; a.dN = nullptr; a.dsurf = nullptr;
     switch (p->rw) {
