@@ -30,7 +30,7 @@ NAMES = {0: "k_drift", 1: "k_advance", 2: "k_tsh_synth", 3: "k_t3_rows", 4: "k_t
          6: "k_sle_solve"}
 ADJ_NAMES = {2: "k_tsh_synth (forward + adjoint rows)", 4: "k_tsh_analysis", 5: "k_t3_gather + k_t3_pack_adj",
              7: "k_t3_rows_adj", 8: "k_adv_adj + k_adv_nodes", 9: "sea-level transpose (scalar SHT kernels)",
-             10: "k_drift_gT + k_drift_force_adj"}
+             10: "k_drift_adj"}
 NCLASS = 12
 
 
