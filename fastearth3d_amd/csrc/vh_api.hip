@@ -1494,6 +1494,68 @@ int vh_set_memory(vh_model* h, const double* tau) {
     });
 }
 
+// The prognostic state besides the memory (src/vilma_io.f90:425-491): the
+// clock, the rsl that seeds the next sea-level solve, the ice at the call
+// boundary, and the rotation state (m and both channels' memory in the
+// reference packing (NLAM, ne, 6) of rotation_get_memory,
+// src/vilma_rotation.f90:275-307).
+int vh_get_state(vh_model* h, double* time_yr, double* rsl, double* h_ice, double* rot_m, double* rot_load_mem,
+                 double* rot_tidal_mem) {
+    return guarded(h, [&] {
+        const size_t ng = h->ng();
+        const int ne = h->d.ne;
+        if (time_yr) *time_yr = h->time_yr;
+        if (rsl) HIPCHECK(hipMemcpyAsync(rsl, h->d_rsl, ng * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        if (h_ice) HIPCHECK(hipMemcpyAsync(h_ice, h->d_hice, ng * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        std::vector<double> mem((size_t)2 * ne * 24, 0.0), sc(RS_N, 0.0);
+        if (h->rot_on) {
+            HIPCHECK(hipMemcpyAsync(mem.data(), h->ra.rs.mem, mem.size() * sizeof(double), hipMemcpyDeviceToHost, h->st));
+            HIPCHECK(hipMemcpyAsync(sc.data(), h->ra.rs.sc, RS_N * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        }
+        h->sync();
+        if (rot_m) { rot_m[0] = sc[RS_M]; rot_m[1] = sc[RS_M + 1]; }
+        double* outs[2] = {rot_load_mem, rot_tidal_mem};
+        for (int ch = 0; ch < 2; ch++)
+            if (outs[ch])
+                for (int comp = 0; comp < 6; comp++)
+                    for (int e = 0; e < ne; e++)
+                        for (int lam = 0; lam < 4; lam++)
+                            outs[ch][lam + 4 * (e + (size_t)ne * comp)] = mem[((size_t)ch * ne + e) * 24 + comp * 4 + lam];
+    });
+}
+
+int vh_set_state(vh_model* h, double time_yr, const double* rsl, const double* h_ice, const double* rot_m,
+                 const double* rot_load_mem, const double* rot_tidal_mem) {
+    return guarded(h, [&] {
+        const size_t ng = h->ng();
+        const int ne = h->d.ne;
+        h->time_yr = time_yr;
+        h->time_resp = time_yr * VH_SEC_PER_YEAR;
+        if (rsl) {
+            upload(h->d_rsl, rsl, ng, h->st);
+            grid_sub(h->st, (int)ng, h->d_topo0, h->d_rsl, h->d_zbed);
+        }
+        if (h_ice) upload(h->d_hice, h_ice, ng, h->st);
+        if (h->rot_on) {   // ignored with rotation off (a cold start of the rotation, as the reference)
+            if (rot_m) upload(h->ra.rs.sc + RS_M, rot_m, 2, h->st);
+            const double* ins[2] = {rot_load_mem, rot_tidal_mem};
+            std::vector<double> mem((size_t)2 * ne * 24);
+            if (ins[0] || ins[1]) {
+                HIPCHECK(hipMemcpyAsync(mem.data(), h->ra.rs.mem, mem.size() * sizeof(double), hipMemcpyDeviceToHost, h->st));
+                h->sync();
+                for (int ch = 0; ch < 2; ch++)
+                    if (ins[ch])
+                        for (int comp = 0; comp < 6; comp++)
+                            for (int e = 0; e < ne; e++)
+                                for (int lam = 0; lam < 4; lam++)
+                                    mem[((size_t)ch * ne + e) * 24 + comp * 4 + lam] = ins[ch][lam + 4 * (e + (size_t)ne * comp)];
+                upload(h->ra.rs.mem, mem.data(), mem.size(), h->st);
+            }
+        }
+        h->sync();
+    });
+}
+
 int vh_destroy(vh_model* h) {
     delete h;
     return 0;

@@ -23,6 +23,7 @@ module vilma
    public :: wp, MAX_LAYER, vilma_param_class, vilma_par_load
    public :: solid_earth, solid_earth_init, solid_earth_update, solid_earth_finalize
    public :: solid_earth_enable_visc_3d_field, solid_earth_enable_visc_3d_nodes
+   public :: solid_earth_get_restart_state, solid_earth_set_restart_state
 
    integer, parameter :: wp = c_double
    integer, parameter :: MAX_LAYER = 16
@@ -158,6 +159,24 @@ module vilma
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: h, visc
          real(c_double), value :: lid_depth, lid_max, cap
+      end function
+      integer(c_int) function vh_get_memory(h, tau) bind(c, name="vh_get_memory")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h, tau
+      end function
+      integer(c_int) function vh_set_memory(h, tau) bind(c, name="vh_set_memory")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h, tau
+      end function
+      integer(c_int) function vh_get_state(h, t, rsl, hi, m, lmem, tmem) bind(c, name="vh_get_state")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h, rsl, hi, m, lmem, tmem
+         real(c_double), intent(out) :: t
+      end function
+      integer(c_int) function vh_set_state(h, t, rsl, hi, m, lmem, tmem) bind(c, name="vh_set_state")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h, rsl, hi, m, lmem, tmem
+         real(c_double), value :: t
       end function
       integer(c_int) function vh_destroy(h) bind(c, name="vh_destroy")
          import :: c_int, c_ptr
@@ -420,6 +439,39 @@ contains
       call check(self, vh_enable_visc_3d_nodes(self%h, c_loc(visc_node), self%par%visc3d_lid_depth, &
                  self%par%visc3d_lid_log10max, self%par%visc_log10_max), "solid_earth_enable_visc_3d")
    end subroutine solid_earth_enable_visc_3d_nodes
+
+   !! The prognostic state vilma_restart_write / vilma_restart_read carry
+   !! (src/vilma_io.f90:425-491), as arrays in the reference's layouts, for a
+   !! host that keeps its own restart files: time (yr), tau(nlam, ne, nk, 6)
+   !! = tau_a_re, tau_a_im, tau_b_re, tau_b_im, tau_c_re, tau_c_im, the rsl and
+   !! ice of the Gauss grid (nphi, nlat), the polar motion m and both rotation
+   !! channels' memory (nlam, ne, 6) (rotation_get_memory, src/vilma_rotation.f90:275-285).
+   subroutine solid_earth_get_restart_state(self, time, tau, rsl, h_ice, m, load_mem, tidal_mem)
+      type(solid_earth), intent(inout) :: self
+      real(wp), intent(out) :: time
+      real(wp), intent(out), target, contiguous :: tau(:,:,:,:), rsl(:,:), h_ice(:,:), load_mem(:,:,:), tidal_mem(:,:,:)
+      complex(wp), intent(out) :: m
+      real(wp), target :: mm(2)
+      real(c_double) :: t
+      call check(self, vh_get_memory(self%h, c_loc(tau)), "solid_earth_get_restart_state")
+      call check(self, vh_get_state(self%h, t, c_loc(rsl), c_loc(h_ice), c_loc(mm), c_loc(load_mem), &
+                 c_loc(tidal_mem)), "solid_earth_get_restart_state")
+      time = t
+      m = cmplx(mm(1), mm(2), wp)
+   end subroutine solid_earth_get_restart_state
+
+   subroutine solid_earth_set_restart_state(self, time, tau, rsl, h_ice, m, load_mem, tidal_mem)
+      type(solid_earth), intent(inout) :: self
+      real(wp), intent(in) :: time
+      real(wp), intent(in), target, contiguous :: tau(:,:,:,:), rsl(:,:), h_ice(:,:), load_mem(:,:,:), tidal_mem(:,:,:)
+      complex(wp), intent(in) :: m
+      real(wp), target :: mm(2)
+      mm = [real(m, wp), aimag(m)]
+      call check(self, vh_set_memory(self%h, c_loc(tau)), "solid_earth_set_restart_state")
+      call check(self, vh_set_state(self%h, real(time, c_double), c_loc(rsl), c_loc(h_ice), c_loc(mm), &
+                 c_loc(load_mem), c_loc(tidal_mem)), "solid_earth_set_restart_state")
+      call refresh(self)
+   end subroutine solid_earth_set_restart_state
 
    !! solid_earth_finalize (src/vilma_coupling.f90:599-626)
    subroutine solid_earth_finalize(self)

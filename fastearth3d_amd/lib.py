@@ -50,6 +50,8 @@ def load():
         L.vh_backward_device.argtypes = [C.c_void_p] + [C.c_void_p] * 3
         L.vh_forward_rewind.argtypes = [C.c_void_p]
         L.vh_adjoint_store_limit.argtypes = [C.c_void_p, C.c_double]
+        L.vh_get_state.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 6
+        L.vh_set_state.argtypes = [C.c_void_p, C.c_double] + [C.POINTER(C.c_double)] * 5
         for name in exported_symbols():
             fn = getattr(L, name)
             if fn.argtypes is None:   # every entry point takes a handle or pointer: never call undeclared
@@ -131,6 +133,25 @@ class SolidEarth(ModelHandle):
 
     def set_dt(self, dt_s):
         self._check(self._lib.vh_resp_set_dt(self._h, float(dt_s)), "response_set_dt")
+
+    def state(self):
+        """The restart state besides the memory: time (yr), rsl and h_ice
+        (nlat, nphi), rotation m (re, im) and the load / tidal channel memory
+        (6, ne, 4) (C order of the reference's (NLAM, ne, 6))."""
+        ne = self.dims["ne"]
+        t = C.c_double()
+        rsl = np.empty((self.nlat, self.nphi)); hi = np.empty_like(rsl)
+        m = np.empty(2); lm = np.empty((6, ne, 4)); tm = np.empty((6, ne, 4))
+        self._check(self._lib.vh_get_state(self._h, C.byref(t), _cp(rsl), _cp(hi), _cp(m), _cp(lm), _cp(tm)),
+                    "get_state")
+        return dict(time=t.value, rsl=rsl, h_ice=hi, rot_m=m, rot_load_mem=lm, rot_tidal_mem=tm)
+
+    def set_state(self, st):
+        arr = {k: np.ascontiguousarray(st[k], dtype=np.float64) for k in
+               ("rsl", "h_ice", "rot_m", "rot_load_mem", "rot_tidal_mem")}
+        self._check(self._lib.vh_set_state(self._h, float(st["time"]), _cp(arr["rsl"]), _cp(arr["h_ice"]),
+                                           _cp(arr["rot_m"]), _cp(arr["rot_load_mem"]), _cp(arr["rot_tidal_mem"])),
+                    "set_state")
 
     def forward_record(self, ice_hist, dt_yr, ckpt_every=10):
         """Run len(ice_hist) coupling calls keeping checkpoints; returns the

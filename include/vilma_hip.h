@@ -141,6 +141,20 @@ int vh_get_dims(vh_model* h, int which, int* out);
 int vh_get_memory(vh_model* h, double* tau);
 int vh_set_memory(vh_model* h, const double* tau);
 
+/* The rest of the restart state (src/vilma_io.f90:425-491): model time
+ * (years), the rsl that seeds the next sea-level solve and the ice at the
+ * call boundary ((nphi,nlat) each), and with rotation on the polar motion m
+ * (re, im) and both degree-2 channels' memory (load, tidal), each
+ * (NLAM=4, ne, 6) column-major with the last axis [Are, Aim, Bre, Bim, Cre,
+ * Cim] (rotation_get_memory, src/vilma_rotation.f90:275-307). With the
+ * memory (vh_get_memory / vh_set_memory) this continues a run bit for bit.
+ * Any pointer may be NULL; the rotation arguments are zero / ignored with
+ * rotation off. */
+int vh_get_state(vh_model* h, double* time_yr, double* rsl, double* h_ice, double* rot_m,
+                 double* rot_load_mem, double* rot_tidal_mem);
+int vh_set_state(vh_model* h, double time_yr, const double* rsl, const double* h_ice, const double* rot_m,
+                 const double* rot_load_mem, const double* rot_tidal_mem);
+
 /* solid_earth_finalize (src/vilma_coupling.f90:599-626). */
 int vh_destroy(vh_model* h);
 

@@ -214,6 +214,37 @@ def test_restart_roundtrip(oracle):
     np.testing.assert_array_equal(b.memory(), tau)
 
 
+@pytest.mark.parametrize("rotation", [0, 1])
+def test_restart_continuation(rotation):
+    """Memory + state out of one model (reference restart layouts) into a fresh one: the
+    continued run matches the uninterrupted one (src/vilma_io.f90:425-491 restores the same set)."""
+    from fastearth3d_amd import lib
+    c = configs.config(1, lmax=16)
+    p = c["params"]
+    p.rotation = rotation
+    a = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"])
+    for k in range(3):
+        a.update(c["ice"](k + 50), c["dt_yr"])
+    tau, st = a.memory(), a.state()
+    if rotation:
+        assert np.abs(st["rot_load_mem"]).max() > 0.0 and np.abs(st["rot_m"]).max() > 0.0
+    b = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"])
+    b.set_memory(tau)
+    b.set_state(st)
+    sb = b.state()
+    for key in ("rsl", "h_ice", "rot_m", "rot_load_mem", "rot_tidal_mem"):
+        np.testing.assert_array_equal(sb[key], st[key])
+    assert sb["time"] == st["time"]
+    for k in range(3, 5):
+        a.update(c["ice"](k + 50), c["dt_yr"])
+        b.update(c["ice"](k + 50), c["dt_yr"])
+        scale = np.abs(a.rsl).max()
+        assert np.abs(b.rsl - a.rsl).max() <= 1e-13 * scale
+        assert np.array_equal(b.C, a.C)
+    assert rel(b.memory(), a.memory()) <= 1e-13
+    assert b.time == a.time
+
+
 def test_error_paths():
     from fastearth3d_amd import lib
     from fastearth3d_amd.model import VilmaError
