@@ -442,6 +442,7 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
         for (int p = 0, Ns = 1; p < s.nfac; p++) {
             const int R = s.fac[p], tstep = np / (Ns * R);
             s.twoff[p] = (int)twp.size();
+            s.nsm[p] = Ns > 1 ? (unsigned)((0x100000000ull + (unsigned long long)Ns - 1) / (unsigned long long)Ns) : 0u;
             if (Ns > 1)
                 for (int r = 1; r < R; r++)
                     for (int k = 0; k < Ns; k++) twp.push_back(tw[(size_t)(k * r * tstep) % np]);

@@ -110,6 +110,7 @@ struct vh_sht {
     const double2* tw;     // [nphi] e^{-2 pi i t/nphi}
     const double2* twp;    // per-pass FFT twiddles, pass p at twoff[p]: [R-1][Ns] (Ns = product of earlier radices)
     int nfac, fac[16], twoff[16], ntw;
+    unsigned nsm[16];      // per pass: ceil(2^32 / Ns) (Ns > 1), so j / Ns = umulhi(j, nsm) exactly for j, Ns < 2^16
     const int2* ljobs;     // Legendre analysis blocks: (m, first degree offset), 64 degrees each
     int nljobs;
 };

@@ -116,11 +116,13 @@ __device__ __forceinline__ void wave_lds_sync() {
 __device__ __forceinline__ int swz(int i) { return i ^ ((i >> 3) & 7); }
 
 template <int R, bool SwS = false, bool SwD = false>
-__device__ __forceinline__ void fft_pass(const double2* src, double2* dst, int N, int Ns, const double2* tp,
-                                         bool inv, int t, int nthr) {
+__device__ __forceinline__ void fft_pass(const double2* src, double2* dst, int N, int Ns, unsigned nsm,
+                                         const double2* tp, bool inv, int t, int nthr) {
     const int NR = N / R;
     for (int j = t; j < NR; j += nthr) {
-        const int k = j % Ns;
+        // j = q Ns + k by the pass's reciprocal (no integer division)
+        const int q = Ns > 1 ? (int)__umulhi((unsigned)j, nsm) : j;
+        const int k = j - q * Ns;
         double2 v[R], V[R];
 #pragma unroll
         for (int r = 0; r < R; r++) v[r] = src[SwS ? swz(j + r * NR) : j + r * NR];
@@ -129,7 +131,7 @@ __device__ __forceinline__ void fft_pass(const double2* src, double2* dst, int N
             for (int r = 1; r < R; r++) v[r] = cmul(v[r], cconj_if(tp[(r - 1) * Ns + k], inv));
         }
         dft_small<R>(v, V, inv);
-        const int idxD = (j / Ns) * Ns * R + k;
+        const int idxD = q * Ns * R + k;
 #pragma unroll
         for (int r = 0; r < R; r++) dst[SwD ? swz(idxD + r * Ns) : idxD + r * Ns] = V[r];
     }
@@ -151,11 +153,12 @@ __device__ double2* fft_wave(double2* a, double2* b, const vh_sht& s, bool inv, 
         const int R = s.fac[p];
         const double2* tp = (tw ? tw : s.twp) + s.twoff[p];
         const int mode = !sw ? 0 : (p == 0 ? 1 : (p == 1 ? 2 : 0));   // 1: store swizzled, 2: load swizzled
-#define VH_FFT_CASE(RR)                                                                       \
-        case RR:                                                                              \
-            if (mode == 1) fft_pass<RR, false, true>(src, dst, N, Ns, tp, inv, lane, 64);     \
-            else if (mode == 2) fft_pass<RR, true, false>(src, dst, N, Ns, tp, inv, lane, 64); \
-            else fft_pass<RR>(src, dst, N, Ns, tp, inv, lane, 64);                            \
+        const unsigned M = s.nsm[p];
+#define VH_FFT_CASE(RR)                                                                          \
+        case RR:                                                                                 \
+            if (mode == 1) fft_pass<RR, false, true>(src, dst, N, Ns, M, tp, inv, lane, 64);     \
+            else if (mode == 2) fft_pass<RR, true, false>(src, dst, N, Ns, M, tp, inv, lane, 64); \
+            else fft_pass<RR>(src, dst, N, Ns, M, tp, inv, lane, 64);                            \
             break;
         switch (R) {
             VH_FFT_CASE(8)
@@ -165,9 +168,9 @@ __device__ double2* fft_wave(double2* a, double2* b, const vh_sht& s, bool inv, 
             VH_FFT_CASE(3)
             VH_FFT_CASE(5)
             default:   // 7
-                if (mode == 1) fft_pass<7, false, true>(src, dst, N, Ns, tp, inv, lane, 64);
-                else if (mode == 2) fft_pass<7, true, false>(src, dst, N, Ns, tp, inv, lane, 64);
-                else fft_pass<7>(src, dst, N, Ns, tp, inv, lane, 64);
+                if (mode == 1) fft_pass<7, false, true>(src, dst, N, Ns, M, tp, inv, lane, 64);
+                else if (mode == 2) fft_pass<7, true, false>(src, dst, N, Ns, M, tp, inv, lane, 64);
+                else fft_pass<7>(src, dst, N, Ns, M, tp, inv, lane, 64);
                 break;
         }
 #undef VH_FFT_CASE
