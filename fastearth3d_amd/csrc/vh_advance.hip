@@ -98,7 +98,8 @@ __global__ void __launch_bounds__(256) k_advance(advance_args a) {
 // mnorm[k] = max over the advance's y-blocks of pmax[y][k]; the last block
 // also forms the global max of the per-block maxima.
 __global__ void __launch_bounds__(256) k_advance_max(int nk, int ny, int nblk, const double* __restrict__ pmax,
-                                                     const double* __restrict__ pblk, double* mnorm, double* maxn) {
+                                                     const double* __restrict__ pblk, double* mnorm, double* maxn,
+                                                     double* maxn_copy) {
     if ((int)blockIdx.x == (int)gridDim.x - 1) {
         __shared__ double red[256];
         double m = 0.0;
@@ -109,7 +110,10 @@ __global__ void __launch_bounds__(256) k_advance_max(int nk, int ny, int nblk, c
             if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
             __syncthreads();
         }
-        if (threadIdx.x == 0) maxn[0] = red[0];
+        if (threadIdx.x == 0) {
+            maxn[0] = red[0];
+            if (maxn_copy) maxn_copy[0] = red[0];
+        }
         return;
     }
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -126,10 +130,12 @@ size_t vh_advance_scratch(const vh_dims& d) {
 
 void vh_launch_advance(hipStream_t st, const vh_dims& d, const vh_dev& v, double dt, const double* tau_in,
                        double* tau_out, const double* sigma, const double* dN, double* mnorm_out,
-                       double* maxnorm_out, const int* e3of, const double2* cdel, double* scratch) {
+                       double* maxnorm_out, const int* e3of, const double2* cdel, double* scratch,
+                       double* maxn_copy) {
     if (d.nem == 0) {
         vh_fill(st, mnorm_out, 0.0, d.nk);
         vh_fill(st, maxnorm_out, 0.0, 1);
+        if (maxn_copy) vh_fill(st, maxn_copy, 0.0, 1);
         return;
     }
     advance_args a;
@@ -142,7 +148,7 @@ void vh_launch_advance(hipStream_t st, const vh_dims& d, const vh_dev& v, double
     k_advance<<<g, 256, 0, st>>>(a);
     HIPCHECK(hipGetLastError());
     k_advance_max<<<(d.nk + 255) / 256 + 1, 256, 0, st>>>(d.nk, (int)g.y, (int)(g.x * g.y), a.pmax, a.pblk, mnorm_out,
-                                                          maxnorm_out);
+                                                          maxnorm_out, maxn_copy);
     HIPCHECK(hipGetLastError());
 }
 

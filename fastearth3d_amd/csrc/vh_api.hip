@@ -590,7 +590,7 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
 // vilma_advance (src/vilma_response.f90:944-992): the 3-D elements first
 // (advance_memory_3d, :1171-1225, increments into h->d_cdel), then the one
 // streaming pass that applies them and advances every 1-D element.
-void advance_memory(vh_model* h, const double* sigma, int nxt) {
+void advance_memory(vh_model* h, const double* sigma, int nxt, double* maxn_copy = nullptr) {
     hipStream_t st = h->st;
     const vh_dims& d = h->d;
     if (h->lat_visc && h->ne3d > 0) {
@@ -619,7 +619,7 @@ void advance_memory(vh_model* h, const double* sigma, int nxt) {
     }
     h->prof_begin(VH_K_ADVANCE);
     vh_launch_advance(st, d, h->v, h->dt, h->d_tau[h->cur], h->d_tau[nxt], sigma, h->d_dN, h->d_mnorm[nxt],
-                      h->d_maxn + nxt, h->d_e3of, h->d_cdel, h->d_advmax);
+                      h->d_maxn + nxt, h->d_e3of, h->d_cdel, h->d_advmax, maxn_copy);
     h->prof_end();
 }
 
@@ -635,19 +635,17 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
     const vh_sht& s = h->s;
     const int n = (int)h->ng();
     sle_res res;
-    sle_ice(st, n, ice0, ice1, frac, interp, h->d_heq, h->d_topo0, h->d_ice, h->d_dice, h->d_C0s);
+    // sc[16..28): the solve's results (zeroed with wcorr by the ice kernel),
+    // the new memory norm at [11], all read back once at the end of the solve
+    double* r = h->d_sc + 16;
+    sle_ice(st, n, ice0, ice1, frac, interp, h->d_heq, h->d_topo0, h->d_ice, h->d_dice, h->d_C0s, h->d_wcorr, r, 12);
     if (h->kind == 2) {
         h->prof_begin(VH_K_DRIFT);
         vh_drift_run(st, h->dplan, h->d, h->v, h->d_tau[h->cur], h->d_mnorm[h->cur], h->d_maxn + h->cur, h->skip_tol,
                      h->deg1_cm, h->d_dN, h->d_dsurf);
         h->prof_end();
     }
-    vh_fill(st, h->d_wcorr, 0.0, n);
-    // the coastline passes and inner fixed point in one cooperative launch;
-    // sc[16..28): its results (zeroed here), then Cs and the new memory norm,
-    // all read back once at the end of the solve
-    double* r = h->d_sc + 16;
-    vh_fill(st, r, 0.0, 12);
+    // the coastline passes and inner fixed point in one cooperative launch
     sle_solve_args ia{};
     ia.s = s; ia.topo0 = h->d_topo0; ia.ice = h->d_ice; ia.dice = h->d_dice; ia.C0 = h->d_C0s; ia.srot = srot;
     ia.gu = h->d_gu; ia.gn = h->d_gn; ia.dsurf = h->d_dsurf;
@@ -667,12 +665,11 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
     h->prof_end();
     // sigma is the closing load transform; then the memory advance
     int nxt = h->cur ^ 1;
-    if (!report_only && h->kind == 2) advance_memory(h, (const double*)h->d_sigma, nxt);
+    if (!report_only && h->kind == 2) advance_memory(h, (const double*)h->d_sigma, nxt, r + 11);
     if (!report_only) h->time_resp += h->dt;
-    // mass diagnostic (+ the new memory norm, in the same read-back)
-    sle_cs(st, s, h->d_C, h->d_rsl, h->d_part, r + 10);
-    if (h->kind == 2) HIPCHECK(hipMemcpyAsync(r + 11, h->d_maxn + (report_only ? h->cur : nxt), sizeof(double),
-                                              hipMemcpyDeviceToDevice, st));
+    // r[10] = int C rsl (from the solve) for the mass diagnostic, r[11] the new memory norm
+    if (report_only && h->kind == 2)
+        HIPCHECK(hipMemcpyAsync(r + 11, h->d_maxn + h->cur, sizeof(double), hipMemcpyDeviceToDevice, st));
     h->read(r, 12);
     const double* hr = h->h_pin;
     if (hr[2] != 0.0) {

@@ -92,7 +92,8 @@ void vh_launch_setup_deg1_op(hipStream_t st, const vh_dims& d, const int* mnodes
 // scratch: vh_advance_scratch(d) doubles (per-block maxima)
 void vh_launch_advance(hipStream_t st, const vh_dims& d, const vh_dev& v, double dt, const double* tau_in,
                        double* tau_out, const double* sigma, const double* dN, double* mnorm_out,
-                       double* maxnorm_out, const int* e3of, const double2* cdel, double* scratch);
+                       double* maxnorm_out, const int* e3of, const double2* cdel, double* scratch,
+                       double* maxn_copy = nullptr);   // maxn_copy: a second place for the global max
 size_t vh_advance_scratch(const vh_dims& d);
 void vh_launch_memnorm(hipStream_t st, const vh_dims& d, const double* tau, double* mnorm, double* maxnorm);
 

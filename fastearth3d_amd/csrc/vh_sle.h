@@ -2,8 +2,10 @@
 #pragma once
 #include "vh_internal.h"
 
+// also zeroes wcorr (n) and res[0..nres) (the solve's result slots)
 void sle_ice(hipStream_t st, int n, const double* ice0, const double* ice1, double frac, int interp,
-             const double* ice_ref, const double* topo0, double* ice, double* dice, double* C0);
+             const double* ice_ref, const double* topo0, double* ice, double* dice, double* C0, double* wcorr,
+             double* res, int nres);
 // response_apply only: uN = [u; N] (degree-major) from sigma
 void vh_apply_only(hipStream_t st, int nlm, const double2* sigma, const double* gu, const double* gn,
                    const double* dsurf, int nk, double g, int kind, double2* uN);
@@ -21,9 +23,9 @@ void sle_cs(hipStream_t st, const vh_sht& s, const double* C, const double* f, d
 // the final rsl (the solve's closing analysis).
 // res (zeroed by the caller): [0] iterations of the last pass, [1] its last
 // dphi, [2] 1 if a grid barrier wait timed out, [3] C_int of the last pass,
-// [4] iterations over all
-// passes, [5] passes with ocean (C_int > 0), [6], [7] int d_ice_g and
-// int topo0 (C - C0) of the last such pass.
+// [4] iterations over all passes, [5] passes with ocean (C_int > 0), [6], [7]
+// int d_ice_g and int topo0 (C - C0) of the last such pass, [10] int C rsl of
+// the result (int C Sraw + dphi C_int of the last iterate; 0 without ocean).
 // bar: [0] barrier counter (continues across launches), [1] timeout flag,
 // [2] the counter's value at the start of the next launch.
 struct sle_solve_args {

@@ -22,9 +22,11 @@ __device__ __forceinline__ double ocean(double topo, double ice) {
 // (src/vilma_timestep.f90:314-326, src/vilma_sle.f90:216)
 __global__ void k_sle_ice(int n, const double* __restrict__ ice0, const double* __restrict__ ice1, double frac,
                           int interp, const double* __restrict__ ice_ref, const double* __restrict__ topo0,
-                          double* ice, double* dice, double* C0) {
+                          double* ice, double* dice, double* C0, double* wcorr, double* res, int nres) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nres) res[i] = 0.0;
     if (i >= n) return;
+    wcorr[i] = 0.0;
     double v = interp ? ice0[i] + frac * (ice1[i] - ice0[i]) : ice0[i];
     double d = v - ice_ref[i];
     ice[i] = v;
@@ -186,12 +188,13 @@ __device__ __forceinline__ double wave_sum(double v) {
 // dphi = (ice_int - Cs_int + zeta) / C_int with Cs_int from the row sums cs
 // (wavefront 0; result in *sd for the block)
 __device__ __forceinline__ void sle_dphi(const sle_solve_args& a, const double* tot, int nlat, double dlon, int wave,
-                                         int lane, double* sd) {
+                                         int lane, double* sd, double* scs) {
     if (wave == 0) {
         double total = 0.0;
         for (int q = lane; q < nlat; q += 64) total += a.s.w[q] * ld_coh(a.cs + q) * dlon;
         total = wave_sum(total);
         if (lane == 0) {
+            *scs = total;
             const double rho_ratio = VH_RHO_ICE / VH_RHO_WATER;
             const double ice_int = -rho_ratio * tot[1];
             const double zeta = a.subgrid ? tot[2] : 0.0;
@@ -381,7 +384,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     const bool is_row = blk < npairs;
     const int p = blk, m = blk - npairs;
     __shared__ double redmax[4][4];
-    __shared__ double sdphi;
+    __shared__ double sdphi, scs;
     // Inputs written by the kernels before this launch (ice, drift, rsl, ...)
     // were written by other XCDs: an agent-scope acquire invalidates this
     // CU's and XCD's cached copies before anything is read, whatever fences
@@ -390,7 +393,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     unsigned target = a.bar[2];   // the counter keeps counting across launches
     const double scale = 2.0 * VH_PI / N, dlon = 2.0 * VH_PI / N;
     __shared__ double stot[4];    // pass totals: C_int, int d_ice_g, int topo0 (C - C0); flips
-    double n_iters = 0.0, n_passes = 0.0, last_ice = 0.0, last_zeta = 0.0;
+    double n_iters = 0.0, n_passes = 0.0, last_ice = 0.0, last_zeta = 0.0, last_cs = 0.0;
     // row-block LDS
     double* srow = (double*)lds;                          // [2][N]
     double2* TW = (double2*)(srow + 2 * N);               // [s.ntw]
@@ -455,6 +458,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     __syncthreads();
     const double C_int = stot[0];
     if (blk == 0 && tid == 0) a.res[3] = C_int;
+    last_cs = 0.0;   // a pass without ocean leaves int C rsl = 0
     if (C_int > 0.0) {
         // subgrid sloping-coast load (src/vilma_sle.f90:289-291), own rows
         if (is_row)
@@ -584,9 +588,10 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
 
         // ---- dphi (every block); rows: rsl update and row maxima; orders: the next sigma and (u, N)
         if (is_row) {
-            sle_dphi(a, stot, nlat, dlon, wave, lane, &sdphi);
+            sle_dphi(a, stot, nlat, dlon, wave, lane, &sdphi, &scs);
             __syncthreads();
             const double dphi = sdphi;
+            last_cs = scs + dphi * C_int;   // int C rsl of this iterate: int C Sraw + dphi int C
             if (blk == 0 && tid == 0) { a.res[0] = (double)ii; a.res[1] = dphi; }
             const int r0 = 2 * p;
             double dm[2] = {0.0, 0.0}, sm[2] = {0.0, 0.0};
@@ -631,9 +636,10 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
                     cs = ld_coh(a.FC + (size_t)js * (L + 1) + m);
                 }
             }
-            sle_dphi(a, stot, nlat, dlon, wave, lane, &sdphi);
+            sle_dphi(a, stot, nlat, dlon, wave, lane, &sdphi, &scs);
             __syncthreads();
             const double k = VH_RHO_WATER * sdphi;
+            last_cs = scs + sdphi * C_int;
             const double2 fn = make_double2(bn.x + k * cn.x, bn.y + k * cn.y);
             const double2 fs = make_double2(bs.x + k * cs.x, bs.y + k * cs.y);
             sle_order_stage<TabLds>(a, m, T, tstride, U, fn, fs);
@@ -691,6 +697,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
   }
     if (blk == 0 && tid == 0) {
         a.res[4] = n_iters; a.res[5] = n_passes; a.res[6] = last_ice; a.res[7] = last_zeta;
+        a.res[10] = last_cs;
         a.bar[2] = target;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -716,8 +723,10 @@ long sle_solve_max_barriers(const vh_sht& s, int n_outer, int n_inner) {
 static inline unsigned nb256(size_t n) { return (unsigned)((n + 255) / 256); }
 
 void sle_ice(hipStream_t st, int n, const double* ice0, const double* ice1, double frac, int interp,
-             const double* ice_ref, const double* topo0, double* ice, double* dice, double* C0) {
-    k_sle_ice<<<nb256(n), 256, 0, st>>>(n, ice0, ice1, frac, interp, ice_ref, topo0, ice, dice, C0);
+             const double* ice_ref, const double* topo0, double* ice, double* dice, double* C0, double* wcorr,
+             double* res, int nres) {
+    k_sle_ice<<<nb256(std::max(n, nres)), 256, 0, st>>>(n, ice0, ice1, frac, interp, ice_ref, topo0, ice, dice, C0,
+                                                         wcorr, res, nres);
 }
 void vh_apply_only(hipStream_t st, int nlm, const double2* sigma, const double* gu, const double* gn,
                    const double* dsurf, int nk, double g, int kind, double2* uN) {
