@@ -32,6 +32,7 @@ ADJ_NAMES = {2: "k_tsh_synth (forward + adjoint rows)", 4: "k_tsh_analysis", 5: 
              7: "k_t3_rows_adj", 8: "k_adv_adj + k_adv_nodes", 9: "sea-level transpose (scalar SHT kernels)",
              10: "k_drift_adj"}
 NCLASS = 12
+PROF_EVERY = 10   # calls per event-timed call in the forward bench
 
 
 def alg_bytes(dims, cls, chunk_batches=None, sle_iters=None):
@@ -278,7 +279,6 @@ def adjoint_bench(a, L):
     nsub = se.scalar(3) - n0
     check(se, L.vh_forward_rewind(se._h))
     # forward with checkpoints + backward with recompute (misfit gradient timed apart)
-    L.vh_profile(se._h, 1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     record()
@@ -292,6 +292,13 @@ def adjoint_bench(a, L):
     backward()
     torch.cuda.synchronize()
     t_bwd = time.perf_counter() - t0
+    # per-kernel HIP-event timings from one more (untimed) record + backward of
+    # the same window: events around every launch would slow the timed pass
+    check(se, L.vh_forward_rewind(se._h))
+    L.vh_profile(se._h, 1)
+    record()
+    backward()
+    L.vh_profile(se._h, 0)
     ms = (C.c_double * NCLASS)()
     cnt = (C.c_double * NCLASS)()
     L.vh_profile_read(se._h, ms, cnt)
@@ -477,13 +484,21 @@ def main():
 
     for k in range(a.warmup):
         step(k)
-    L.vh_profile(se._h, 1)
+    L.vh_profile(se._h, 1)   # clear the per-kernel totals
+    L.vh_profile(se._h, 0)
     n_acc0 = se.scalar(3)
     n_in0 = se.scalar(6)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(a.warmup, nsteps):
+        # per-kernel HIP events on every PROF_EVERY-th call of the timed region
+        # (an event pair around each launch costs ~8 % of the call when on everywhere)
+        sampled = (k - a.warmup) % PROF_EVERY == PROF_EVERY - 1 or (a.steps < PROF_EVERY and k == nsteps - 1)
+        if sampled:
+            L.vh_profile(se._h, 2)
         step(k)
+        if sampled:
+            L.vh_profile(se._h, 0)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     ms = (C.c_double * NCLASS)()
@@ -506,7 +521,8 @@ def main():
     for k in NAMES:
         if cnt[k] <= 0:
             continue
-        ab = alg_bytes(d, k, nbat / nch if nbat else None, n_inner / cnt[k] if k == 6 else None)
+        # sea-level loop: iterations per launch over the whole timed region (one launch per sub-step)
+        ab = alg_bytes(d, k, nbat / nch if nbat else None, n_inner / max(nsub, 1) if k == 6 else None)
         if not ab:
             continue
         avg_s = ms[k] / cnt[k] / 1000.0

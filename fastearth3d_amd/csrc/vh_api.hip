@@ -1741,7 +1741,8 @@ int vh_profile(vh_model* h, int enable) {
         h->sync();
         h->prof_collect();
         h->prof = enable != 0;
-        for (int c = 0; c < VH_K_NCLASS; c++) { h->prof_ms[c] = 0.0; h->prof_n[c] = 0; }
+        if (enable == 1)
+            for (int c = 0; c < VH_K_NCLASS; c++) { h->prof_ms[c] = 0.0; h->prof_n[c] = 0; }
     });
 }
 

@@ -163,7 +163,8 @@ const char* vh_last_error(const vh_model* h);
 
 /* ---- Measurement hook (no counterpart in the reference): per-kernel-class
  * device time from HIP events recorded on the model's stream around each
- * launch. enable != 0 starts accumulating (and clears the totals). */
+ * launch. enable = 1 starts accumulating and clears the totals, 2 resumes
+ * without clearing, 0 stops (the totals stay readable). */
 enum {
     VH_K_DRIFT = 0,          /* drift: forcing + restricted-operator product   */
     VH_K_ADVANCE = 1,        /* memory advance (1-D update, 3-D increments)    */
