@@ -487,7 +487,7 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
     h->d_uN = dalloc<double2>((size_t)2 * d.nlm);
     h->d_part = dalloc<double>((size_t)7 * nl + 64);
     h->d_sc = dalloc<double>(64); vh_fill(st, h->d_sc, 0.0, 64);
-    h->d_sbar = dalloc<unsigned>(2);
+    h->d_sbar = dalloc<unsigned>(VH_SLE_BAR_WORDS);
     sle_bar_reset(st, h->d_sbar);
 
     // ---- SLE / stepper knobs (build_solver, src/vilma_coupling.f90:291-308)

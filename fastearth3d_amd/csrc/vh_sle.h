@@ -26,8 +26,13 @@ void sle_cs(hipStream_t st, const vh_sht& s, const double* C, const double* f, d
 // [4] iterations over all passes, [5] passes with ocean (C_int > 0), [6], [7]
 // int d_ice_g and int topo0 (C - C0) of the last such pass, [10] int C rsl of
 // the result (int C Sraw + dphi C_int of the last iterate; 0 without ocean).
-// bar: [0] barrier counter (continues across launches), [1] timeout flag,
-// [2] the counter's value at the start of the next launch.
+// bar (VH_SLE_BAR_WORDS words, one 4 KiB page per counter): [1] timeout
+// flag, [2] barriers passed (continues across launches); at
+// VH_SLE_FLAG_STRIDE (1 + k), k < 8, the release words, at
+// VH_SLE_FLAG_STRIDE 9 the top arrival counter, at VH_SLE_FLAG_STRIDE (10 + g)
+// the arrival counter of block group g = blockIdx % 8.
+#define VH_SLE_FLAG_STRIDE 1024
+#define VH_SLE_BAR_WORDS (18 * VH_SLE_FLAG_STRIDE)
 struct sle_solve_args {
     vh_sht s;
     const double *topo0, *ice, *dice, *C0, *srot, *gu, *gn, *dsurf;

@@ -127,11 +127,15 @@ __global__ void k_bsl_integrand(int n, const double* __restrict__ h, const doubl
 // Data handed between blocks (F, FC, F2, cs, part, part3) moves through
 // agent-scope loads and stores (ld_coh / st_coh), which bypass the XCDs'
 // non-coherent caches, so a grid barrier needs no cache write-back or
-// invalidate: each wavefront waits for its stores, then the counter bar[0] is
-// raised and awaited (it runs on across launches; each launch starts from
-// bar[2]). Data from the kernels before the launch is acquired once at entry.
-// The wait is bounded: a block that never arrives raises bar[1] and res[2]
-// instead of a hang.
+// invalidate: each wavefront waits for its stores, then the block arrives in
+// the counter of its group blockIdx % 8; the last block of a group arrives in
+// the top counter, and the block that completes the top count writes the
+// barrier number into eight release words, which the blocks of each group poll
+// (all counters and words on pages of their own, so the arrivals of a hundred
+// blocks that finish together do not queue on one address). The counts run on
+// across launches from the barrier number kept in bar[2]. Data from the
+// kernels before the launch is acquired once at entry. The wait is bounded: a
+// block that never arrives raises bar[1] and res[2] instead of a hang.
 
 #define VH_SLE_SPIN_LIMIT 2000000u
 // P-table values a thread has in flight in the order stage
@@ -140,15 +144,28 @@ __global__ void k_bsl_integrand(int n, const double* __restrict__ h, const doubl
 // grid values per lane a row wavefront loads ahead of its transform
 #define VH_SLE_PU 6
 
-__device__ __forceinline__ void sle_grid_sync(unsigned* bar, unsigned& target, double* res) {
-    target += gridDim.x;
+__device__ __forceinline__ void sle_grid_sync(unsigned* bar, unsigned& nbar, double* res) {
+    nbar += 1u;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_s_waitcnt(0);   // this wavefront's stores are complete
     __syncthreads();
     if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned g = blockIdx.x & 7u, nb = gridDim.x;
+        const unsigned ng = (nb - g + 7u) / 8u, ngroups = nb < 8u ? nb : 8u;
+        unsigned* flag = bar + VH_SLE_FLAG_STRIDE * (1 + g);
+        // arrival in the group counter; the group's last block arrives in the
+        // top counter, and the last group's block raises the release words
+        if (__hip_atomic_fetch_add(bar + VH_SLE_FLAG_STRIDE * (10 + g), 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT) + 1u == ng * nbar &&
+            __hip_atomic_fetch_add(bar + VH_SLE_FLAG_STRIDE * 9, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u ==
+                ngroups * nbar) {
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                __hip_atomic_store(bar + VH_SLE_FLAG_STRIDE * (1 + k), nbar, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
         unsigned spins = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nbar) {
             if (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
             if (++spins > VH_SLE_SPIN_LIMIT) {
                 __hip_atomic_fetch_or(bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -390,7 +407,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     // CU's and XCD's cached copies before anything is read, whatever fences
     // the launch itself carries; a release at the end publishes the outputs.
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    unsigned target = a.bar[2];   // the counter keeps counting across launches
+    unsigned target = a.bar[2];   // barriers passed, counted on across launches
     const double scale = 2.0 * VH_PI / N, dlon = 2.0 * VH_PI / N;
     __shared__ double stot[4];    // pass totals: C_int, int d_ice_g, int topo0 (C - C0); flips
     double n_iters = 0.0, n_passes = 0.0, last_ice = 0.0, last_zeta = 0.0, last_cs = 0.0;
@@ -405,9 +422,19 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     if (is_row) {
         for (int t = tid; t < s.ntw; t += 256) TW[t] = s.twp[t];
     } else if (TabLds) {
-        const int nl = L - m + 1;
+        // 8 table values per thread in flight
+        const int nl = L - m + 1, nt = nh * nl;
         const double* Pa = s.plm_jl + (size_t)s.mstart[m] * nh;
-        for (int t = tid; t < nh * nl; t += 256) T[(t / nl) * tstride + t % nl] = Pa[t];
+        for (int t0 = tid; t0 < nt; t0 += 256 * 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = t0 + 256 * u < nt ? Pa[t0 + 256 * u] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int t = t0 + 256 * u;
+                if (t < nt) T[(t / nl) * tstride + t % nl] = v[u];
+            }
+        }
     }
     __syncthreads();
 
@@ -449,13 +476,22 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     }
     sle_grid_sync(a.bar, target, a.res);
     // every block: the pass totals sum_j w_j dlon part_j (latitude order, as
-    // k_weighted_total), one lane per quantity
-    if (tid < 3) {
-        double total = 0.0;
-        for (int q = 0; q < nlat; q++) total += s.w[q] * ld_coh(a.part3 + (size_t)tid * nlat + q) * dlon;
-        stot[tid] = total;
+    // k_weighted_total): the terms are loaded by all threads into the FFT /
+    // order-stage scratch, then one lane per quantity adds them in order
+    {
+        double* tw3 = is_row ? (double*)RB : (double*)U;   // [3][nlat], free here
+        for (int t = tid; t < 3 * nlat; t += 256) {
+            const int q = t % nlat;
+            tw3[t] = s.w[q] * ld_coh(a.part3 + t) * dlon;
+        }
+        __syncthreads();
+        if (tid < 3) {
+            double total = 0.0;
+            for (int q = 0; q < nlat; q++) total += tw3[tid * nlat + q];
+            stot[tid] = total;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     const double C_int = stot[0];
     if (blk == 0 && tid == 0) a.res[3] = C_int;
     last_cs = 0.0;   // a pass without ocean leaves int C rsl = 0
@@ -705,6 +741,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
 
 __global__ void k_sle_bar_reset(unsigned* bar) {
     if (threadIdx.x < 3) bar[threadIdx.x] = 0u;
+    if (threadIdx.x < 17) bar[VH_SLE_FLAG_STRIDE * (1 + threadIdx.x)] = 0u;
 }
 
 void sle_bar_reset(hipStream_t st, unsigned* bar) {
