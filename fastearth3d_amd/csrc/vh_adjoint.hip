@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(192) k_t3_rows_adj(vh_sht s, double2* __restri
                 }
             }
             wave_lds_sync();
-            z = fft_wave(a, c, s, true, lane);
+            z = fft_wave_t<true>(a, c, s, lane);
             if (pass == 0) {
 #pragma unroll
                 for (int u = 0; u < RA_U; u++) {
@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(192) k_t3_rows_adj(vh_sht s, double2* __restri
         }
         wave_lds_sync();
         double2* other = (z == a) ? c : a;
-        double2* y = fft_wave(z, other, s, false, lane);
+        double2* y = fft_wave_t<false>(z, other, s, lane);
         const double scale = 2.0 * VH_PI / N;
         double2* G0 = Fa + ((size_t)b * 6 + d0) * (L + 1) * nlat + j;
         double2* G1 = Fa + ((size_t)b * 6 + d1) * (L + 1) * nlat + j;

@@ -449,6 +449,11 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
             Ns *= R;
         }
         s.ntw = (int)twp.size();
+        // the fixed-length device transforms derive the same plan at compile time
+        const fft_cplan c = fft_make_cplan(np);
+        bool same = c.nfac == s.nfac;
+        for (int p = 0; same && p < s.nfac; p++) same = c.fac[p] == s.fac[p] && c.twoff[p] == s.twoff[p];
+        if (!same) throw vh_error("sht_grid_init: longitude FFT plan differs from the compile-time rule");
         h->d_twp = dalloc<double2>(std::max<size_t>(twp.size(), 1)); upload(h->d_twp, twp.data(), twp.size(), st);
         h->sync();   // twp goes out of scope
     }

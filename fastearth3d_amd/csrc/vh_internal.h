@@ -98,6 +98,27 @@ size_t vh_advance_scratch(const vh_dims& d);
 void vh_launch_memnorm(hipStream_t st, const vh_dims& d, const double* tau, double* mnorm, double* maxnorm);
 
 // ---------------------------------------------------------------- SHT
+// Factorisation of a longitude FFT of length N (plan of the transform)
+struct fft_cplan { int nfac; int fac[8]; int twoff[8]; int ns[8]; };
+// the run-time plan's rule: factors 8, 6, 4, 2, 3, 5, 7 in that order of
+// preference; twiddles [r-1][k] per pass after the first
+__host__ __device__ constexpr fft_cplan fft_make_cplan(int N) {
+    fft_cplan c{};
+    int n = N, nf = 0;
+    const int pref[7] = {8, 6, 4, 2, 3, 5, 7};
+    for (int q = 0; q < 7 && n > 1; q++)
+        while (n % pref[q] == 0 && nf < 8) { c.fac[nf++] = pref[q]; n /= pref[q]; }
+    c.nfac = n == 1 ? nf : 0;
+    int off = 0, Ns = 1;
+    for (int p = 0; p < nf; p++) {
+        c.twoff[p] = off;
+        c.ns[p] = Ns;
+        if (Ns > 1) off += (c.fac[p] - 1) * Ns;
+        Ns *= c.fac[p];
+    }
+    return c;
+}
+
 struct vh_sht {
     int L, nlat, nphi, nh, nlm;
     const double* plm_lj;  // [m][l][j], per-m offset mstart[m]*nh (synthesis)

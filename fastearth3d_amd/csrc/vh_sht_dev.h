@@ -115,9 +115,12 @@ __device__ __forceinline__ void wave_lds_sync() {
 // same way.
 __device__ __forceinline__ int swz(int i) { return i ^ ((i >> 3) & 7); }
 
-template <int R, bool SwS = false, bool SwD = false>
+// SwS / SwD are run-time flags (one instantiation per radix keeps the
+// instruction footprint of a kernel with several transforms small).
+template <int R>
 __device__ __forceinline__ void fft_pass(const double2* src, double2* dst, int N, int Ns, unsigned nsm,
-                                         const double2* tp, bool inv, int t, int nthr) {
+                                         const double2* tp, bool inv, int t, int nthr, bool SwS = false,
+                                         bool SwD = false) {
     const int NR = N / R;
     for (int j = t; j < NR; j += nthr) {
         // j = q Ns + k by the pass's reciprocal (no integer division)
@@ -154,31 +157,86 @@ __device__ double2* fft_wave(double2* a, double2* b, const vh_sht& s, bool inv, 
         const double2* tp = (tw ? tw : s.twp) + s.twoff[p];
         const int mode = !sw ? 0 : (p == 0 ? 1 : (p == 1 ? 2 : 0));   // 1: store swizzled, 2: load swizzled
         const unsigned M = s.nsm[p];
-#define VH_FFT_CASE(RR)                                                                          \
-        case RR:                                                                                 \
-            if (mode == 1) fft_pass<RR, false, true>(src, dst, N, Ns, M, tp, inv, lane, 64);     \
-            else if (mode == 2) fft_pass<RR, true, false>(src, dst, N, Ns, M, tp, inv, lane, 64); \
-            else fft_pass<RR>(src, dst, N, Ns, M, tp, inv, lane, 64);                            \
-            break;
+        const bool sws = mode == 2, swd = mode == 1;
         switch (R) {
-            VH_FFT_CASE(8)
-            VH_FFT_CASE(6)
-            VH_FFT_CASE(4)
-            VH_FFT_CASE(2)
-            VH_FFT_CASE(3)
-            VH_FFT_CASE(5)
-            default:   // 7
-                if (mode == 1) fft_pass<7, false, true>(src, dst, N, Ns, M, tp, inv, lane, 64);
-                else if (mode == 2) fft_pass<7, true, false>(src, dst, N, Ns, M, tp, inv, lane, 64);
-                else fft_pass<7>(src, dst, N, Ns, M, tp, inv, lane, 64);
-                break;
+            case 8: fft_pass<8>(src, dst, N, Ns, M, tp, inv, lane, 64, sws, swd); break;
+            case 6: fft_pass<6>(src, dst, N, Ns, M, tp, inv, lane, 64, sws, swd); break;
+            case 4: fft_pass<4>(src, dst, N, Ns, M, tp, inv, lane, 64, sws, swd); break;
+            case 2: fft_pass<2>(src, dst, N, Ns, M, tp, inv, lane, 64, sws, swd); break;
+            case 3: fft_pass<3>(src, dst, N, Ns, M, tp, inv, lane, 64, sws, swd); break;
+            case 5: fft_pass<5>(src, dst, N, Ns, M, tp, inv, lane, 64, sws, swd); break;
+            default: fft_pass<7>(src, dst, N, Ns, M, tp, inv, lane, 64, sws, swd); break;
         }
-#undef VH_FFT_CASE
         wave_lds_sync();
         double2* t = src; src = dst; dst = t;
         Ns *= R;
     }
     return src;
+}
+
+// ---- the same transform with the length fixed at compile time
+// For the grid widths of the usual truncations the factorisation, strides and
+// twiddle offsets become constants: index arithmetic folds into the LDS
+// instructions' immediate offsets and shifts, and the direction into the
+// butterflies' signs. Factors (fft_make_cplan, which the run-time plan is
+// checked against), twiddle table and the order of every floating-point
+// operation are those of fft_wave, so both give the same bits.
+template <int N, int P, bool INV>
+__device__ __forceinline__ void fft_pass_n(const double2* __restrict__ src, double2* __restrict__ dst,
+                                           const double2* __restrict__ tw, int lane) {
+    constexpr fft_cplan c = fft_make_cplan(N);
+    constexpr int R = c.fac[P], Ns = c.ns[P], NR = N / R;
+    constexpr bool sw = c.nfac > 1 && (N & 7) == 0, SwS = sw && P == 1, SwD = sw && P == 0;
+    const double2* tp = tw + c.twoff[P];
+#pragma unroll
+    for (int j0 = 0; j0 < NR; j0 += 64) {
+        const int j = j0 + lane;
+        if (NR % 64 == 0 || j < NR) {
+            const int q = j / Ns, k = j - q * Ns;
+            double2 v[R], V[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = src[SwS ? swz(j + r * NR) : j + r * NR];
+            if constexpr (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) v[r] = cmul(v[r], cconj_if(tp[(r - 1) * Ns + k], INV));
+            }
+            dft_small<R>(v, V, INV);
+            const int idxD = q * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) dst[SwD ? swz(idxD + r * Ns) : idxD + r * Ns] = V[r];
+        }
+    }
+}
+
+template <int N, int P, bool INV>
+__device__ __forceinline__ double2* fft_passes_n(double2* src, double2* dst, const double2* tw, int lane) {
+    constexpr fft_cplan c = fft_make_cplan(N);
+    if constexpr (P >= c.nfac) {
+        return src;
+    } else {
+        fft_pass_n<N, P, INV>(src, dst, tw, lane);
+        wave_lds_sync();
+        return fft_passes_n<N, P + 1, INV>(dst, src, tw, lane);
+    }
+}
+
+// grid widths with a compile-time transform: the default 4 lmax of lmax 64
+// and 192, and 384 (lmax 96 by default, lmax 128 on its 192 x 384 grid)
+#define VH_FFT_FIXED_SIZES(X) X(256) X(384) X(768)
+
+// fft_wave with the direction as a template argument: the fixed-size
+// transform where the grid width has one, else the run-time one.
+template <bool INV>
+__device__ __forceinline__ double2* fft_wave_t(double2* a, double2* b, const vh_sht& s, int lane,
+                                               const double2* tw = nullptr) {
+    const double2* t = tw ? tw : s.twp;
+    switch (s.nphi) {
+#define VH_FFT_FIXED_CASE(NN) \
+        case NN: return fft_passes_n<NN, 0, INV>(a, b, t, lane);
+        VH_FFT_FIXED_SIZES(VH_FFT_FIXED_CASE)
+#undef VH_FFT_FIXED_CASE
+        default: return fft_wave(a, b, s, INV, lane, tw);
+    }
 }
 
 // Loads and stores of data that other blocks of a running cooperative launch
@@ -283,7 +341,7 @@ __global__ void __launch_bounds__(256) k_fft_fwd(vh_sht s, In in, int nrows, dou
         }
     }
     wave_lds_sync();
-    double2* z = fft_wave(a, b, s, false, lane);
+    double2* z = fft_wave_t<false>(a, b, s, lane);
     if (has0) lds_to_pair_spectra(z, L, N, scale, F + (size_t)r0 * (L + 1), has1 ? F + (size_t)r1 * (L + 1) : nullptr, lane);
 }
 
@@ -299,7 +357,7 @@ __global__ void __launch_bounds__(256) k_fft_inv(vh_sht s, const double2* __rest
     const bool has0 = r0 < nrows, has1 = r1 < nrows;
     pair_spectra_to_lds(has0 ? F + (size_t)r0 * (L + 1) : nullptr, has1 ? F + (size_t)r1 * (L + 1) : nullptr, L, N,
                         a, lane);
-    double2* z = fft_wave(a, b, s, true, lane);
+    double2* z = fft_wave_t<true>(a, b, s, lane);
     double rs = 0.0;   // consumers may return a per-row term to be summed
     if (has0)
         for (int i0 = 0; i0 < N; i0 += 64 * VH_FFT_U) {

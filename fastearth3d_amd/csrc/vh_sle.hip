@@ -537,7 +537,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
             }
         }
         wave_lds_sync();
-        double2* z = fft_wave(fa, fb, s, false, lane, TW);
+        double2* z = fft_wave_t<false>(fa, fb, s, lane, TW);
         double2* dst = wave == 0 ? a.FC : a.F;
         lds_to_pair_spectra<true>(z, L, N, scale, dst + (size_t)r0 * (L + 1),
                                   has1 ? dst + (size_t)r1 * (L + 1) : nullptr, lane);
@@ -581,7 +581,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
                     }
                     const double2* F2p = a.F2 + (size_t)p * (L + 1) * 4 + 2 * wave;   // rows q = 2j, 2j + 1
                     pair_spectra_to_lds<true>(F2p, F2p + 1, L, N, fa, lane, 4);
-                    double2* z = fft_wave(fa, fb, s, true, lane, TW);
+                    double2* z = fft_wave_t<true>(fa, fb, s, lane, TW);
                     double rs = 0.0;
                     for (int i0 = 0; i0 < N; i0 += 64 * VH_SLE_PU) {
 #pragma unroll
@@ -615,7 +615,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
                 const double* b1 = (const double*)(RB + 2 * (size_t)N + soff);
                 for (int i = lane; i < N; i += 64) P[i] = make_double2(b0[i], has1 ? b1[i] : 0.0);
                 wave_lds_sync();
-                double2* z = fft_wave(P, RB + soff, s, false, lane, TW);
+                double2* z = fft_wave_t<false>(P, RB + soff, s, lane, TW);
                 lds_to_pair_spectra<true>(z, L, N, scale, a.F + (size_t)r0 * (L + 1),
                                           r0 + 1 < nlat ? a.F + (size_t)(r0 + 1) * (L + 1) : nullptr, lane);
             }

@@ -373,7 +373,7 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, d
             }
         }
         wave_lds_sync();
-        z = fft_wave(a, c, s, true, lane);
+        z = fft_wave_t<true>(a, c, s, lane);
         if (mode == 1) {
             double* g0 = grid + (((size_t)b * 6 + d0) * nlat + j) * N;
             double* g1 = grid + (((size_t)b * 6 + d1) * nlat + j) * N;
@@ -394,7 +394,7 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, d
         z = a;
     }
     double2* other = (z == a) ? c : a;
-    double2* y = fft_wave(z, other, s, false, lane);
+    double2* y = fft_wave_t<false>(z, other, s, lane);
     const double scale = 2.0 * VH_PI / N;
     for (int m = lane; m <= L; m += 64) {
         double2 zm = y[m], zc = y[(N - m) % N];

@@ -82,8 +82,10 @@ def test_gains_reference_mesh(oracle):
     assert rel(xg[:, :, nodes], xo[:, :, nodes]) <= 1e-10
 
 
-@pytest.mark.parametrize("L,nlat,nphi", [(32, 0, 0), (128, 192, 384)])
+@pytest.mark.parametrize("L,nlat,nphi", [(32, 0, 0), (64, 0, 0), (128, 192, 384), (192, 0, 0), (40, 0, 0)])
 def test_sht(oracle, L, nlat, nphi):
+    """Scalar transforms against the oracle: run-time FFT plans (nphi 128, 160) and
+    the compile-time ones (256, 384, 768)."""
     c, mo, mg = _pair(oracle, 1, L, nlat=nlat, nphi=nphi)
     rng = np.random.default_rng(11)
     f = rng.standard_normal((mg.nlat, mg.nphi))
