@@ -188,7 +188,7 @@ __device__ __forceinline__ void fft_pass_n(const double2* __restrict__ src, doub
     constexpr int R = c.fac[P], Ns = c.ns[P], NR = N / R;
     constexpr bool sw = c.nfac > 1 && (N & 7) == 0, SwS = sw && P == 1, SwD = sw && P == 0;
     const double2* tp = tw + c.twoff[P];
-#pragma unroll
+#pragma unroll 1
     for (int j0 = 0; j0 < NR; j0 += 64) {
         const int j = j0 + lane;
         if (NR % 64 == 0 || j < NR) {

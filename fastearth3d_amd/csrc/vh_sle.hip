@@ -510,6 +510,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
         last_ice = stot[1];
         last_zeta = stot[2];
     }
+    __syncthreads();   // wcorr of the block's rows, written by all its threads, is read below by wavefronts 0, 1
     // ---- start: FC = FFT(C) (wave 0), F = FFT(load(rsl)) (wave 1)
     if (is_row && wave < 2) {
         const int r0 = 2 * p, r1 = r0 + 1;
