@@ -209,7 +209,11 @@ __device__ __forceinline__ int xcd_block_a(int nblk) {
 // longitude (registers), then Fa <- FFT(-rate dt g_a) in place (the forward
 // row pass, mode 0). At the end rate_h[i3][j][i] += -(2 pi/nphi) w_j dt *
 // the accumulated product summed over the three wavefronts.
-#define RA_U 8   // longitudes per lane (nphi <= 512)
+// RU: longitudes per lane kept in registers (nphi <= 64 RU); spectra
+// loaded RA_LU orders per lane at a time (fewer registers than the row pass)
+#define RA_U 8   // most longitudes per lane (nphi <= 512)
+#define RA_LU 4
+template <int RU>
 __global__ void __launch_bounds__(192) k_t3_rows_adj(vh_sht s, double2* __restrict__ Fa, const double2* __restrict__ Ff,
                                                      int ne3, const double* __restrict__ rate, double dt,
                                                      double* __restrict__ rate_h) {
@@ -224,15 +228,15 @@ __global__ void __launch_bounds__(192) k_t3_rows_adj(vh_sht s, double2* __restri
     double* Mrow = (double*)(lds + (size_t)6 * N);
     const double* rr = rate + ((size_t)i3 * nlat + j) * N;
     for (int i = threadIdx.x; i < N; i += blockDim.x) Mrow[i] = rr[i] * dt;
-    double acc[RA_U];
+    double acc[RU];
 #pragma unroll
-    for (int u = 0; u < RA_U; u++) acc[u] = 0.0;
+    for (int u = 0; u < RU; u++) acc[u] = 0.0;
     const double omA = wave == 2 ? 0.5 : 1.0;   // dyad weights: tp (first of wave 2) counts 1/2, the rest 1
     const int d0 = 2 * wave, d1 = 2 * wave + 1;
     __syncthreads();   // Mrow complete
     for (int sh = 0; sh < 3; sh++) {
         const int b = 3 * i3 + sh;
-        double2 gf[RA_U];
+        double2 gf[RU];
         double2* z = nullptr;
         for (int pass = 0; pass < 2; pass++) {
             const double2* Fs = pass == 0 ? Ff : (const double2*)Fa;
@@ -240,16 +244,16 @@ __global__ void __launch_bounds__(192) k_t3_rows_adj(vh_sht s, double2* __restri
             const double2* F1 = Fs + ((size_t)b * 6 + d1) * (L + 1) * nlat + j;
             for (int i = lane; i < N; i += 64) a[i] = make_double2(0.0, 0.0);
             wave_lds_sync();
-            for (int m0 = 0; m0 <= L; m0 += 64 * VH_FFT_U) {
-                double2 A[VH_FFT_U], B[VH_FFT_U];
+            for (int m0 = 0; m0 <= L; m0 += 64 * RA_LU) {
+                double2 A[RA_LU], B[RA_LU];
 #pragma unroll
-                for (int u = 0; u < VH_FFT_U; u++) {
+                for (int u = 0; u < RA_LU; u++) {
                     const int m = m0 + lane + 64 * u;
                     A[u] = m <= L ? F0[(size_t)m * nlat] : make_double2(0.0, 0.0);
                     B[u] = m <= L ? F1[(size_t)m * nlat] : make_double2(0.0, 0.0);
                 }
 #pragma unroll
-                for (int u = 0; u < VH_FFT_U; u++) {
+                for (int u = 0; u < RA_LU; u++) {
                     const int m = m0 + lane + 64 * u;
                     if (m > L) continue;
                     if (m == 0) { A[u].y = 0.0; B[u].y = 0.0; }
@@ -261,7 +265,7 @@ __global__ void __launch_bounds__(192) k_t3_rows_adj(vh_sht s, double2* __restri
             z = fft_wave_t<true>(a, c, s, lane);
             if (pass == 0) {
 #pragma unroll
-                for (int u = 0; u < RA_U; u++) {
+                for (int u = 0; u < RU; u++) {
                     const int i = lane + 64 * u;
                     gf[u] = i < N ? z[i] : make_double2(0.0, 0.0);
                 }
@@ -269,7 +273,7 @@ __global__ void __launch_bounds__(192) k_t3_rows_adj(vh_sht s, double2* __restri
             }
         }
 #pragma unroll
-        for (int u = 0; u < RA_U; u++) {
+        for (int u = 0; u < RU; u++) {
             const int i = lane + 64 * u;
             if (i < N) {
                 const double2 g = z[i];
@@ -299,7 +303,7 @@ __global__ void __launch_bounds__(192) k_t3_rows_adj(vh_sht s, double2* __restri
     __syncthreads();
     double* red = (double*)a;
 #pragma unroll
-    for (int u = 0; u < RA_U; u++) {
+    for (int u = 0; u < RU; u++) {
         const int i = lane + 64 * u;
         if (i < N) red[i] = acc[u];
     }
@@ -326,7 +330,8 @@ void vh_t3_rows_adj(hipStream_t st, const vh_sht& s, double2* Fa, const double2*
     const size_t lds = (size_t)6 * s.nphi * sizeof(double2) + (size_t)s.nphi * sizeof(double);
     if (s.nphi > 64 * RA_U) throw vh_error("3-D adjoint: nphi above 512 is not supported by the row pass");
     const unsigned nblk = (unsigned)(((ne3 * s.nlat + 7) / 8) * 8);
-    k_t3_rows_adj<<<nblk, 192, lds, st>>>(s, Fa, Ff, ne3, rate, dt, rate_h);
+    if (s.nphi <= 64 * 6) k_t3_rows_adj<6><<<nblk, 192, lds, st>>>(s, Fa, Ff, ne3, rate, dt, rate_h);
+    else k_t3_rows_adj<RA_U><<<nblk, 192, lds, st>>>(s, Fa, Ff, ne3, rate, dt, rate_h);
     HIPCHECK(hipGetLastError());
 }
 
