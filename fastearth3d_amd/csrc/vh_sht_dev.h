@@ -220,6 +220,54 @@ __device__ __forceinline__ double2* fft_passes_n(double2* src, double2* dst, con
     }
 }
 
+// In-place form of a compile-time pass: every lane reads the inputs of its
+// butterflies (NR / 64 rounded up) into registers, the wavefront's LDS reads
+// are ordered before its writes, then the outputs go back into the same
+// buffer. Same arithmetic as fft_pass_n; half the LDS.
+template <int N, int P, bool INV>
+__device__ __forceinline__ void fft_pass_ip(double2* buf, const double2* __restrict__ tw, int lane) {
+    constexpr fft_cplan c = fft_make_cplan(N);
+    constexpr int R = c.fac[P], Ns = c.ns[P], NR = N / R, NB = (NR + 63) / 64;
+    constexpr bool sw = c.nfac > 1 && (N & 7) == 0, SwS = sw && P == 1, SwD = sw && P == 0;
+    const double2* tp = tw + c.twoff[P];
+    double2 v[NB][R];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int j = lane + 64 * b;
+        if (NR % 64 == 0 || j < NR) {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[b][r] = buf[SwS ? swz(j + r * NR) : j + r * NR];
+        }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        const int j = lane + 64 * b;
+        if (NR % 64 == 0 || j < NR) {
+            const int q = j / Ns, k = j - q * Ns;
+            double2 V[R];
+            if constexpr (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) v[b][r] = cmul(v[b][r], cconj_if(tp[(r - 1) * Ns + k], INV));
+            }
+            dft_small<R>(v[b], V, INV);
+            const int idxD = q * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[SwD ? swz(idxD + r * Ns) : idxD + r * Ns] = V[r];
+        }
+    }
+}
+
+template <int N, int P, bool INV>
+__device__ __forceinline__ void fft_passes_ip(double2* buf, const double2* tw, int lane) {
+    constexpr fft_cplan c = fft_make_cplan(N);
+    if constexpr (P < c.nfac) {
+        fft_pass_ip<N, P, INV>(buf, tw, lane);
+        wave_lds_sync();
+        fft_passes_ip<N, P + 1, INV>(buf, tw, lane);
+    }
+}
+
 // grid widths with a compile-time transform: the default 4 lmax of lmax 64
 // and 192, and 384 (lmax 96 by default, lmax 128 on its 192 x 384 grid)
 #define VH_FFT_FIXED_SIZES(X) X(256) X(384) X(768)
@@ -235,6 +283,30 @@ __device__ __forceinline__ double2* fft_wave_t(double2* a, double2* b, const vh_
         case NN: return fft_passes_n<NN, 0, INV>(a, b, t, lane);
         VH_FFT_FIXED_SIZES(VH_FFT_FIXED_CASE)
 #undef VH_FFT_FIXED_CASE
+        default: return fft_wave(a, b, s, INV, lane, tw);
+    }
+}
+
+// True when nphi has a compile-time transform (fft_wave_ip then works in place).
+__host__ __device__ inline bool fft_fixed_size(int n) {
+#define VH_FFT_FIXED_TEST(NN) if (n == NN) return true;
+    VH_FFT_FIXED_SIZES(VH_FFT_FIXED_TEST)
+#undef VH_FFT_FIXED_TEST
+    return false;
+}
+
+// fft_wave_t for callers that budget LDS by fft_fixed_size: in place in a
+// (returned) for the compile-time lengths, else the run-time transform with
+// b as its second buffer.
+template <bool INV>
+__device__ __forceinline__ double2* fft_wave_ip(double2* a, double2* b, const vh_sht& s, int lane,
+                                                const double2* tw = nullptr) {
+    const double2* t = tw ? tw : s.twp;
+    switch (s.nphi) {
+#define VH_FFT_IP_CASE(NN) \
+        case NN: fft_passes_ip<NN, 0, INV>(a, t, lane); return a;
+        VH_FFT_FIXED_SIZES(VH_FFT_IP_CASE)
+#undef VH_FFT_IP_CASE
         default: return fft_wave(a, b, s, INV, lane, tw);
     }
 }

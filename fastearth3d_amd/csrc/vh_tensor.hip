@@ -339,9 +339,11 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, d
     if (lb >= nb * nlat) return;
     const int b = lb / nlat, j = lb % nlat;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double2* a = lds + (size_t)wave * 2 * N;
-    double2* c = a + N;
-    double* Mrow = (double*)(lds + (size_t)3 * 2 * N);
+    // one LDS buffer per wavefront when the transform runs in place, else two
+    const int nbuf = fft_fixed_size(N) ? 1 : 2;
+    double2* a = lds + (size_t)wave * nbuf * N;
+    double2* c = nbuf == 2 ? a + N : nullptr;
+    double* Mrow = (double*)(lds + (size_t)3 * nbuf * N);
     const int d0 = 2 * wave, d1 = 2 * wave + 1;
     const double2* I0 = Fin + ((size_t)b * 6 + d0) * (L + 1) * nlat + j;
     const double2* I1 = Fin + ((size_t)b * 6 + d1) * (L + 1) * nlat + j;
@@ -373,7 +375,7 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, d
             }
         }
         wave_lds_sync();
-        z = fft_wave_t<true>(a, c, s, lane);
+        z = fft_wave_ip<true>(a, c, s, lane);
         if (mode == 1) {
             double* g0 = grid + (((size_t)b * 6 + d0) * nlat + j) * N;
             double* g1 = grid + (((size_t)b * 6 + d1) * nlat + j) * N;
@@ -394,7 +396,7 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, d
         z = a;
     }
     double2* other = (z == a) ? c : a;
-    double2* y = fft_wave_t<false>(z, other, s, lane);
+    double2* y = fft_wave_ip<false>(z, other, s, lane);
     const double scale = 2.0 * VH_PI / N;
     for (int m = lane; m <= L; m += 64) {
         double2 zm = y[m], zc = y[(N - m) % N];
@@ -451,7 +453,9 @@ __global__ void __launch_bounds__(256) k_t3_gather(int nr, int nk, int nlm, vh_d
 #define VH_TSH_TB_SYNTH 3
 #endif
 static size_t synth_lds(const vh_sht& s) { return (size_t)VH_TSH_TB_SYNTH * 5 * (s.L + 1) * sizeof(double2); }
-static size_t rows_lds(const vh_sht& s) { return (size_t)6 * s.nphi * sizeof(double2) + (size_t)s.nphi * sizeof(double); }
+static size_t rows_lds(const vh_sht& s) {
+    return (size_t)3 * (fft_fixed_size(s.nphi) ? 1 : 2) * s.nphi * sizeof(double2) + (size_t)s.nphi * sizeof(double);
+}
 
 void vh_tsh_synth(hipStream_t st, const vh_sht& s, const double2* X, int nb, double2* Fd) {
     if (nb <= 0) return;
