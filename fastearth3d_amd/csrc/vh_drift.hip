@@ -93,6 +93,9 @@ struct drift_args {
 };
 
 #define DRIFT_THREADS 512
+#ifndef VH_DRIFT_RW
+#define VH_DRIFT_RW 32   // right-hand sides per block (orders x re/im), at most
+#endif
 typedef double vh_d4 __attribute__((ext_vector_type(4)));
 
 // Forcing of the block's right-hand sides (rhs = reim*ts + (m - m0)) at every
@@ -167,9 +170,9 @@ __device__ __forceinline__ void drift_store(const drift_args& a, int l, int k, i
 }
 
 // Workgroup = (degree l, ts consecutive orders): the forcing in LDS, then
-// Y = G_l F. With RW = 64 the product runs on FP64 MFMA (16x16x4 tiles;
+// Y = G_l F. With RW >= 16 the product runs on FP64 MFMA (16x16x4 tiles;
 // lane l: A[l%16][l/16], B[l/16][l%16], D[4r + l/16][l%16]): a wavefront per
-// 16-row tile of G_l, all four 16-column tiles of the 64 right-hand sides,
+// 16-row tile of G_l, all RW/16 16-column tiles of the right-hand sides,
 // the G_l fragment read once per k-step from L2. Otherwise four rows per
 // thread with scalar loads of G.
 template <int RW>
@@ -184,13 +187,14 @@ __global__ void __launch_bounds__(DRIFT_THREADS) k_drift(drift_args a) {
     const double thr = a.skip_tol * a.maxnorm[0];
     drift_forcing<RW>(a, l, kb, nsl, thr, F);
     const double* __restrict__ G = a.G + (size_t)(l - 1) * nrow * ncol;
-    if constexpr (RW == 64) {
+    if (RW >= 16 && ncol <= 128) {   // the G fragments of a wavefront fit its registers
+        constexpr int CT = RW >= 16 ? RW / 16 : 1;
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const int nrt = (nrow + 15) / 16;
         for (int rt = wave; rt < nrt; rt += DRIFT_THREADS / 64) {
-            vh_d4 acc[4];
+            vh_d4 acc[CT];
 #pragma unroll
-            for (int ct = 0; ct < 4; ct++) acc[ct] = vh_d4{0.0, 0.0, 0.0, 0.0};
+            for (int ct = 0; ct < CT; ct++) acc[ct] = vh_d4{0.0, 0.0, 0.0, 0.0};
             const int arow = rt * 16 + (lane & 15);
             const double* Grow = G + (size_t)min(arow, nrow - 1) * ncol;
             // the wavefront's whole G_l fragment in flight first (ncol <= 128 with RW = 64)
@@ -206,13 +210,13 @@ __global__ void __launch_bounds__(DRIFT_THREADS) k_drift(drift_args a) {
                 const int kk = 4 * ks + (lane >> 4);
                 const bool kok = kk < ncol;
 #pragma unroll
-                for (int ct = 0; ct < 4; ct++) {
-                    const double bv = kok ? F[kk * 64 + ct * 16 + (lane & 15)] : 0.0;
+                for (int ct = 0; ct < CT; ct++) {
+                    const double bv = kok ? F[kk * RW + ct * 16 + (lane & 15)] : 0.0;
                     acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], bv, acc[ct], 0, 0, 0);
                 }
             }
 #pragma unroll
-            for (int ct = 0; ct < 4; ct++)
+            for (int ct = 0; ct < CT; ct++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int row = rt * 16 + 4 * r + (lane >> 4), rhs = ct * 16 + (lane & 15);
@@ -250,7 +254,7 @@ __global__ void __launch_bounds__(DRIFT_THREADS) k_drift(drift_args a) {
 // ------------------------------------------------------------------ adjoint
 // Transpose of k_drift for the same skip-gate set (slots the gate drops have
 // a zero drift, so they pass no adjoint back): Y = [dN_h; dsurf_h] of the
-// block's right-hand sides in LDS, Fh = G_l^T Y (FP64 MFMA for RW = 64), then
+// block's right-hand sides in LDS, Fh = G_l^T Y (FP64 MFMA for RW >= 16), then
 // per (Maxwell element, rhs) the forcing transpose onto the element's memory,
 // tau_h += (forcing)^T of its four test dofs (the block owns its slots).
 template <int RW>
@@ -284,13 +288,14 @@ __global__ void __launch_bounds__(DRIFT_THREADS) k_drift_adj(drift_args a, const
     }
     __syncthreads();
     const double* __restrict__ G = a.G + (size_t)(l - 1) * nrow * ncol;
-    if constexpr (RW == 64) {
+    if (RW >= 16 && ncol <= 128) {   // the G fragments of a wavefront fit its registers
+        constexpr int CT = RW >= 16 ? RW / 16 : 1;
         const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const int nct = (ncol + 15) / 16;
         for (int rt = wave; rt < nct; rt += DRIFT_THREADS / 64) {
-            vh_d4 acc[4];
+            vh_d4 acc[CT];
 #pragma unroll
-            for (int ct = 0; ct < 4; ct++) acc[ct] = vh_d4{0.0, 0.0, 0.0, 0.0};
+            for (int ct = 0; ct < CT; ct++) acc[ct] = vh_d4{0.0, 0.0, 0.0, 0.0};
             const int ac = rt * 16 + (lane & 15);   // A row = a column of G_l
             double av[33];                          // A column = a row of G_l (nrow <= 131)
 #pragma unroll
@@ -304,17 +309,17 @@ __global__ void __launch_bounds__(DRIFT_THREADS) k_drift_adj(drift_args a, const
                 const int kk = 4 * ks + (lane >> 4);
                 const bool kok = kk < nrow;
 #pragma unroll
-                for (int ct = 0; ct < 4; ct++) {
-                    const double bv = kok ? Y[kk * 64 + ct * 16 + (lane & 15)] : 0.0;
+                for (int ct = 0; ct < CT; ct++) {
+                    const double bv = kok ? Y[kk * RW + ct * 16 + (lane & 15)] : 0.0;
                     acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], bv, acc[ct], 0, 0, 0);
                 }
             }
 #pragma unroll
-            for (int ct = 0; ct < 4; ct++)
+            for (int ct = 0; ct < CT; ct++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int c = rt * 16 + 4 * r + (lane >> 4);
-                    if (c < ncol) Fh[c * 64 + ct * 16 + (lane & 15)] = acc[ct][r];
+                    if (c < ncol) Fh[c * RW + ct * 16 + (lane & 15)] = acc[ct][r];
                 }
         }
     } else {
@@ -393,11 +398,11 @@ vh_drift_plan* vh_drift_plan_create(hipStream_t st, const vh_dims& d, const vh_d
     }
     const int ncol = 2 * d.nmn, nrow = ncol + 3;
     // right-hand sides per workgroup: as many as fit a 64 KiB forcing tile
-    int rw = 64;
+    int rw = VH_DRIFT_RW;
     while (rw > 2 && (size_t)std::max(ncol, 1) * rw * 8 > 64 * 1024) rw /= 2;
     if ((size_t)std::max(ncol, 1) * rw * 8 > 160 * 1024)
         throw vh_error("drift: too many memory nodes for the LDS forcing tile (nmn = " + std::to_string(d.nmn) + ")");
-    p->rw = rw; p->ts = rw / 2;   // rw = 64 implies ncol <= 128 (the MFMA paths hold G fragments for that)
+    p->rw = rw; p->ts = rw / 2;   // the MFMA paths (rw >= 16) also need ncol <= 128
     p->lds = (size_t)std::max(ncol, 1) * rw * 8;
     std::vector<int2> jobs;
     for (int l = 1; l <= d.L; l++)
