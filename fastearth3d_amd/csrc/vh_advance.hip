@@ -97,17 +97,28 @@ __global__ void __launch_bounds__(256) k_advance(advance_args a) {
 
 // mnorm[k] = max over the advance's y-blocks of pmax[y][k]; the last block
 // also forms the global max of the per-block maxima.
+// Per-slot maxima over the advance's block rows, and (last block) the global
+// maximum over the per-block maxima. One 64-thread block per 64 slots so the
+// reduction spreads over the CUs; loads are issued 8 at a time (max is exact
+// in any order).
 __global__ void __launch_bounds__(256) k_advance_max(int nk, int ny, int nblk, const double* __restrict__ pmax,
                                                      const double* __restrict__ pblk, double* mnorm, double* maxn,
                                                      double* maxn_copy) {
     if ((int)blockIdx.x == (int)gridDim.x - 1) {
         __shared__ double red[256];
-        double m = 0.0;
-        for (int i = threadIdx.x; i < nblk; i += blockDim.x) m = fmax(m, pblk[i]);
-        red[threadIdx.x] = m;
+        double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+        int i = threadIdx.x;
+        for (; i + 3 * (int)blockDim.x < nblk; i += 4 * blockDim.x) {
+            m0 = fmax(m0, pblk[i]);
+            m1 = fmax(m1, pblk[i + blockDim.x]);
+            m2 = fmax(m2, pblk[i + 2 * blockDim.x]);
+            m3 = fmax(m3, pblk[i + 3 * blockDim.x]);
+        }
+        for (; i < nblk; i += blockDim.x) m0 = fmax(m0, pblk[i]);
+        red[threadIdx.x] = fmax(fmax(m0, m1), fmax(m2, m3));
         __syncthreads();
         for (int s = 128; s > 0; s >>= 1) {
-            if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+            if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
             __syncthreads();
         }
         if (threadIdx.x == 0) {
@@ -116,11 +127,17 @@ __global__ void __launch_bounds__(256) k_advance_max(int nk, int ny, int nblk, c
         }
         return;
     }
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x >= 64) return;
+    const int k = blockIdx.x * 64 + threadIdx.x;
     if (k >= nk) return;
-    double m = 0.0;
-    for (int y = 0; y < ny; y++) m = fmax(m, pmax[(size_t)y * nk + k]);
-    mnorm[k] = m;
+    double m[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int y = 0;
+    for (; y + 8 <= ny; y += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) m[u] = fmax(m[u], pmax[(size_t)(y + u) * nk + k]);
+    }
+    for (; y < ny; y++) m[0] = fmax(m[0], pmax[(size_t)y * nk + k]);
+    mnorm[k] = fmax(fmax(fmax(m[0], m[1]), fmax(m[2], m[3])), fmax(fmax(m[4], m[5]), fmax(m[6], m[7])));
 }
 
 size_t vh_advance_scratch(const vh_dims& d) {
@@ -147,8 +164,9 @@ void vh_launch_advance(hipStream_t st, const vh_dims& d, const vh_dev& v, double
     a.pblk = scratch + (size_t)g.y * d.nk;
     k_advance<<<g, 256, 0, st>>>(a);
     HIPCHECK(hipGetLastError());
-    k_advance_max<<<(d.nk + 255) / 256 + 1, 256, 0, st>>>(d.nk, (int)g.y, (int)(g.x * g.y), a.pmax, a.pblk, mnorm_out,
-                                                          maxnorm_out, maxn_copy);
+    // blocks 0 .. g.x-1: 64 slots each (threads 64..255 of them idle); the last: the global maximum
+    k_advance_max<<<g.x + 1, 256, 0, st>>>(d.nk, (int)g.y, (int)(g.x * g.y), a.pmax, a.pblk, mnorm_out, maxnorm_out,
+                                           maxn_copy);
     HIPCHECK(hipGetLastError());
 }
 
