@@ -388,6 +388,15 @@ __device__ __forceinline__ void sle_order_stage(const sle_solve_args& a, int m, 
     }
 }
 
+// The kernel's longitude FFT: the compile-time transforms where registers
+// allow (MinW 1); with two wavefronts per SIMD (256 registers) the run-time
+// one, whose smaller register need avoids spills.
+template <bool INV, int MinW>
+__device__ __forceinline__ double2* sle_fft(double2* a, double2* b, const vh_sht& s, int lane, const double2* tw) {
+    if constexpr (MinW == 1) return fft_wave_t<INV>(a, b, s, lane, tw);
+    else return fft_wave(a, b, s, INV, lane, tw);
+}
+
 // MinW: wavefronts per SIMD the build must allow; 2 halves the registers
 // (with spills) so that two blocks share a CU when the grid (latitude pairs +
 // orders, 386 blocks at lmax 192) exceeds the CU count.
@@ -538,7 +547,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
             }
         }
         wave_lds_sync();
-        double2* z = fft_wave_t<false>(fa, fb, s, lane, TW);
+        double2* z = sle_fft<false, MinW>(fa, fb, s, lane, TW);
         double2* dst = wave == 0 ? a.FC : a.F;
         lds_to_pair_spectra<true>(z, L, N, scale, dst + (size_t)r0 * (L + 1),
                                   has1 ? dst + (size_t)r1 * (L + 1) : nullptr, lane);
@@ -582,7 +591,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
                     }
                     const double2* F2p = a.F2 + (size_t)p * (L + 1) * 4 + 2 * wave;   // rows q = 2j, 2j + 1
                     pair_spectra_to_lds<true>(F2p, F2p + 1, L, N, fa, lane, 4);
-                    double2* z = fft_wave_t<true>(fa, fb, s, lane, TW);
+                    double2* z = sle_fft<true, MinW>(fa, fb, s, lane, TW);
                     double rs = 0.0;
                     for (int i0 = 0; i0 < N; i0 += 64 * VH_SLE_PU) {
 #pragma unroll
@@ -616,7 +625,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
                 const double* b1 = (const double*)(RB + 2 * (size_t)N + soff);
                 for (int i = lane; i < N; i += 64) P[i] = make_double2(b0[i], has1 ? b1[i] : 0.0);
                 wave_lds_sync();
-                double2* z = fft_wave_t<false>(P, RB + soff, s, lane, TW);
+                double2* z = sle_fft<false, MinW>(P, RB + soff, s, lane, TW);
                 lds_to_pair_spectra<true>(z, L, N, scale, a.F + (size_t)r0 * (L + 1),
                                           r0 + 1 < nlat ? a.F + (size_t)(r0 + 1) * (L + 1) : nullptr, lane);
             }
