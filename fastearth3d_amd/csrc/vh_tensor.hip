@@ -171,8 +171,14 @@ typedef double vh_d4 __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(256) k_tsh_analysis(vh_sht s, const double2* __restrict__ Fd, int nb,
                                                       const double* __restrict__ n6, double2* __restrict__ C) {
-    __shared__ double Bs[2][TA_JC][65];           // [parity][latitude in chunk][column], rows padded: the fold stores stride over banks
-    __shared__ double Ds[4][16][65];              // per-wavefront staging of one row tile
+    // one LDS array, two views: the folded latitude chunk Bs [parity][latitude
+    // in chunk][column] (rows padded: the fold stores stride over banks) while
+    // the products run, then the per-wavefront staging Ds of one row tile in
+    // the epilogue (the same 4160 doubles)
+    __shared__ double Sh[2 * TA_JC * 65];
+    static_assert(2 * TA_JC * 65 >= 4 * 16 * 65, "epilogue staging must fit the chunk buffer");
+    double (*Bs)[TA_JC][65] = (double (*)[TA_JC][65])Sh;
+    double (*Ds)[16][65] = (double (*)[16][65])Sh;
     const int L = s.L, nh = s.nh, nlat = s.nlat, nlm = s.nlm;
     const int ngrp = (nb + TA_B - 1) / TA_B;
     int m, g;
@@ -280,6 +286,7 @@ __global__ void __launch_bounds__(256) k_tsh_analysis(vh_sht s, const double2* _
         }
     }
     // ---- epilogue: stage each row tile, then lane = (degree row, batch)
+    __syncthreads();   // every wavefront is done with the last chunk before its staging overwrites it
 #pragma unroll
     for (int it = 0; it < TA_ITEMS; it++) {
         const int item = wave + 4 * it;
