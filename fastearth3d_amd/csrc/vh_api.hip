@@ -14,6 +14,7 @@
 #include "vh_rot.h"
 #include "vh_sle.h"
 #include "vh_adjoint.h"
+#include "vh_remap.h"
 #include "../../include/vilma_hip.h"
 
 namespace {
@@ -1502,6 +1503,58 @@ int vh_set_memory(vh_model* h, const double* tau) {
 // boundary, and the rotation state (m and both channels' memory in the
 // reference packing (NLAM, ne, 6) of rotation_get_memory,
 // src/vilma_rotation.f90:275-307).
+// ---- host lon-lat grid <-> Gauss grid (src/vilma_remap.f90)
+struct vh_remap_handle {
+    vh_model* h;
+    vh_remap* r;
+};
+
+int vh_remap_create(vh_model* h, int nlon, int nlat_ll, const double* lon_deg, const double* lat_deg,
+                    vh_remap_handle** out) {
+    return guarded(h, [&] {
+        if (!out || !lon_deg || !lat_deg) throw vh_error("remap_init: null argument");
+        *out = nullptr;
+        std::vector<double> ct, w;
+        gauss_legendre(h->d.nlat, ct, w);   // north-first nodes cos(colat), weights
+        vh_remap* r = vh_remap_build(h->st, h->d.nphi, h->d.nlat, h->colat.data(), w.data(), nlon, nlat_ll,
+                                     lon_deg, lat_deg);
+        *out = new vh_remap_handle{h, r};
+    });
+}
+
+int vh_remap_to_gauss(vh_remap_handle* m, const double* f_ll, double* f_gauss, int conserve_mass) {
+    if (!m) return 2;
+    return guarded(m->h, [&] { vh_remap_to_gauss_host(m->r, f_ll, f_gauss, conserve_mass); });
+}
+
+int vh_remap_to_ll(vh_remap_handle* m, const double* f_gauss, double* f_ll) {
+    if (!m) return 2;
+    return guarded(m->h, [&] { vh_remap_to_ll_host(m->r, f_gauss, f_ll); });
+}
+
+int vh_remap_to_gauss_device(vh_remap_handle* m, const double* f_ll, double* f_gauss, int conserve_mass) {
+    if (!m) return 2;
+    return guarded(m->h, [&] {
+        vh_remap_to_gauss_dev(m->r, f_ll, f_gauss, conserve_mass);
+        HIPCHECK(hipStreamSynchronize(m->h->st));
+    });
+}
+
+int vh_remap_to_ll_device(vh_remap_handle* m, const double* f_gauss, double* f_ll) {
+    if (!m) return 2;
+    return guarded(m->h, [&] {
+        vh_remap_to_ll_dev(m->r, f_gauss, f_ll);
+        HIPCHECK(hipStreamSynchronize(m->h->st));
+    });
+}
+
+int vh_remap_destroy(vh_remap_handle* m) {
+    if (!m) return 0;
+    vh_remap_free(m->r);
+    delete m;
+    return 0;
+}
+
 int vh_get_state(vh_model* h, double* time_yr, double* rsl, double* h_ice, double* rot_m, double* rot_load_mem,
                  double* rot_tidal_mem) {
     return guarded(h, [&] {

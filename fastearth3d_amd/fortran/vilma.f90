@@ -24,6 +24,7 @@ module vilma
    public :: solid_earth, solid_earth_init, solid_earth_update, solid_earth_finalize
    public :: solid_earth_enable_visc_3d_field, solid_earth_enable_visc_3d_nodes
    public :: solid_earth_get_restart_state, solid_earth_set_restart_state
+   public :: remap_ll_gauss, remap_init, remap_to_gauss, remap_to_ll, remap_free
 
    integer, parameter :: wp = c_double
    integer, parameter :: MAX_LAYER = 16
@@ -115,6 +116,15 @@ module vilma
       integer :: nphi = 0, nlat = 0
    end type solid_earth
 
+   !! A map pair between a host lon-lat grid and a model's Gauss grid
+   !! (remap_ll_gauss, src/vilma_remap.f90:44-54): conservative in, bilinear out.
+   type :: remap_ll_gauss
+      type(c_ptr) :: m = c_null_ptr
+      type(c_ptr) :: h = c_null_ptr          !! the model whose grid it maps to
+      integer :: nlon = 0, nlat_ll = 0        !! host dimensions
+      integer :: nphi = 0, nlat = 0           !! Gauss dimensions
+   end type remap_ll_gauss
+
    interface
       integer(c_int) function vh_create(p, nphi, nlat, z, hi, hi0, out) bind(c, name="vh_create")
          import :: c_int, c_ptr, vh_params_c
@@ -177,6 +187,25 @@ module vilma
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: h, rsl, hi, m, lmem, tmem
          real(c_double), value :: t
+      end function
+      integer(c_int) function vh_remap_create(h, nlon, nlat_ll, lon, lat, out) bind(c, name="vh_remap_create")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h, lon, lat
+         integer(c_int), value :: nlon, nlat_ll
+         type(c_ptr), intent(out) :: out
+      end function
+      integer(c_int) function vh_remap_to_gauss(m, f, g, cm) bind(c, name="vh_remap_to_gauss")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: m, f, g
+         integer(c_int), value :: cm
+      end function
+      integer(c_int) function vh_remap_to_ll(m, g, f) bind(c, name="vh_remap_to_ll")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: m, g, f
+      end function
+      integer(c_int) function vh_remap_destroy(m) bind(c, name="vh_remap_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: m
       end function
       integer(c_int) function vh_destroy(h) bind(c, name="vh_destroy")
          import :: c_int, c_ptr
@@ -472,6 +501,61 @@ contains
                  c_loc(load_mem), c_loc(tidal_mem)), "solid_earth_set_restart_state")
       call refresh(self)
    end subroutine solid_earth_set_restart_state
+
+   !! remap_init (src/vilma_remap.f90:56-99) for source cell-centre axes
+   !! lon_src(:), lat_src(:) [degrees, ascending] against the Gauss grid of the
+   !! model se (the reference passes its sht; no weight cache: the weights are
+   !! rebuilt at each init, in milliseconds).
+   subroutine remap_init(self, se, lon_src, lat_src)
+      type(remap_ll_gauss), intent(out) :: self
+      type(solid_earth), intent(in) :: se
+      real(wp), intent(in), target, contiguous :: lon_src(:), lat_src(:)
+      self%nlon = size(lon_src);  self%nlat_ll = size(lat_src)
+      self%nphi = se%nphi;        self%nlat = se%nlat
+      self%h = se%h
+      call check(se, vh_remap_create(se%h, int(self%nlon, c_int), int(self%nlat_ll, c_int), c_loc(lon_src), &
+                 c_loc(lat_src), self%m), "remap_init")
+   end subroutine remap_init
+
+   !! remap_to_gauss (src/vilma_remap.f90:101-138): f_ll(nlon, nlat_ll) -> f_gauss(nphi, nlat)
+   subroutine remap_to_gauss(self, se, f_ll, f_gauss, conserve_mass)
+      type(remap_ll_gauss), intent(in) :: self
+      type(solid_earth), intent(in) :: se
+      real(wp), intent(in), target, contiguous :: f_ll(:,:)
+      real(wp), intent(out), target, contiguous :: f_gauss(:,:)
+      logical, optional, intent(in) :: conserve_mass
+      integer(c_int) :: cm
+      if (size(f_ll,1) /= self%nlon .or. size(f_ll,2) /= self%nlat_ll) &
+         error stop 'vilma_remap: source field shape /= map source grid'
+      if (size(f_gauss,1) /= self%nphi .or. size(f_gauss,2) /= self%nlat) &
+         error stop 'vilma_remap: target field shape /= SHTns grid'
+      cm = 0
+      if (present(conserve_mass)) then
+         if (conserve_mass) cm = 1
+      end if
+      call check(se, vh_remap_to_gauss(self%m, c_loc(f_ll), c_loc(f_gauss), cm), "remap_to_gauss")
+   end subroutine remap_to_gauss
+
+   !! remap_to_ll (src/vilma_remap.f90:140-164): f_gauss(nphi, nlat) -> f_ll(nlon, nlat_ll)
+   subroutine remap_to_ll(self, se, f_gauss, f_ll)
+      type(remap_ll_gauss), intent(in) :: self
+      type(solid_earth), intent(in) :: se
+      real(wp), intent(in), target, contiguous :: f_gauss(:,:)
+      real(wp), intent(out), target, contiguous :: f_ll(:,:)
+      if (size(f_gauss,1) /= self%nphi .or. size(f_gauss,2) /= self%nlat) &
+         error stop 'vilma_remap: source field shape /= SHTns grid'
+      if (size(f_ll,1) /= self%nlon .or. size(f_ll,2) /= self%nlat_ll) &
+         error stop 'vilma_remap: target field shape /= map target grid'
+      call check(se, vh_remap_to_ll(self%m, c_loc(f_gauss), c_loc(f_ll)), "remap_to_ll")
+   end subroutine remap_to_ll
+
+   !! release the map (before its model's solid_earth_finalize)
+   subroutine remap_free(self)
+      type(remap_ll_gauss), intent(inout) :: self
+      integer(c_int) :: rc
+      if (c_associated(self%m)) rc = vh_remap_destroy(self%m)
+      self%m = c_null_ptr
+   end subroutine remap_free
 
    !! solid_earth_finalize (src/vilma_coupling.f90:599-626)
    subroutine solid_earth_finalize(self)

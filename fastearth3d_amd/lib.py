@@ -52,6 +52,13 @@ def load():
         L.vh_adjoint_store_limit.argtypes = [C.c_void_p, C.c_double]
         L.vh_get_state.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 6
         L.vh_set_state.argtypes = [C.c_void_p, C.c_double] + [C.POINTER(C.c_double)] * 5
+        L.vh_remap_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_void_p)]
+        L.vh_remap_to_gauss.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+        L.vh_remap_to_ll.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.vh_remap_to_gauss_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        L.vh_remap_to_ll_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.vh_remap_destroy.argtypes = [C.c_void_p]
         for name in exported_symbols():
             fn = getattr(L, name)
             if fn.argtypes is None:   # every entry point takes a handle or pointer: never call undeclared
@@ -153,6 +160,12 @@ class SolidEarth(ModelHandle):
                                            _cp(arr["rot_m"]), _cp(arr["rot_load_mem"]), _cp(arr["rot_tidal_mem"])),
                     "set_state")
 
+    def remap(self, lon_deg, lat_deg):
+        """Map pair between a host lon-lat grid (cell-centre axes in degrees,
+        ascending) and this model's Gauss grid (remap_init,
+        src/vilma_remap.f90:56-99)."""
+        return Remap(self, lon_deg, lat_deg)
+
     def forward_record(self, ice_hist, dt_yr, ckpt_every=10):
         """Run len(ice_hist) coupling calls keeping checkpoints; returns the
         rsl after each call, (ncalls, nlat, nphi)."""
@@ -197,3 +210,45 @@ def solid_earth_update(se, h_ice, dt_yr):
 
 def solid_earth_finalize(se):
     se.finalize()
+
+
+class Remap:
+    """Host lon-lat grid <-> Gauss grid (src/vilma_remap.f90): conservative
+    to_gauss, bilinear to_ll. Host fields are (nlat_ll, nlon) numpy arrays
+    (longitude fastest, latitude ascending); Gauss fields (nlat, nphi) in the
+    model layout (north-first)."""
+
+    def __init__(self, model, lon_deg, lat_deg):
+        self._m = model
+        self._lib = model._lib
+        lon = np.ascontiguousarray(lon_deg, dtype=np.float64)
+        lat = np.ascontiguousarray(lat_deg, dtype=np.float64)
+        self.nlon, self.nlat_ll = lon.size, lat.size
+        h = C.c_void_p()
+        model._check(self._lib.vh_remap_create(model._h, self.nlon, self.nlat_ll, _cp(lon), _cp(lat), C.byref(h)),
+                     "remap_init")
+        self._r = h
+
+    def to_gauss(self, f_ll, conserve_mass=False):
+        """remap_to_gauss (src/vilma_remap.f90:101-138)."""
+        f = np.ascontiguousarray(f_ll, dtype=np.float64)
+        if f.size != self.nlon * self.nlat_ll:
+            raise VilmaError("remap_to_gauss: source field shape /= map source grid")
+        g = np.empty((self._m.nlat, self._m.nphi))
+        self._m._check(self._lib.vh_remap_to_gauss(self._r, _cp(f), _cp(g), int(bool(conserve_mass))),
+                       "remap_to_gauss")
+        return g
+
+    def to_ll(self, f_gauss):
+        """remap_to_ll (src/vilma_remap.f90:140-164)."""
+        g = np.ascontiguousarray(f_gauss, dtype=np.float64)
+        if g.size != self._m.nlat * self._m.nphi:
+            raise VilmaError("remap_to_ll: source field shape /= Gauss grid")
+        f = np.empty((self.nlat_ll, self.nlon))
+        self._m._check(self._lib.vh_remap_to_ll(self._r, _cp(g), _cp(f)), "remap_to_ll")
+        return f
+
+    def close(self):
+        if self._r:
+            self._lib.vh_remap_destroy(self._r)
+            self._r = None

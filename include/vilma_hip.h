@@ -155,6 +155,31 @@ int vh_get_state(vh_model* h, double* time_yr, double* rsl, double* h_ice, doubl
 int vh_set_state(vh_model* h, double time_yr, const double* rsl, const double* h_ice, const double* rot_m,
                  const double* rot_load_mem, const double* rot_tidal_mem);
 
+/* ---- host lon-lat grid <-> the model's Gauss grid (src/vilma_remap.f90) */
+typedef struct vh_remap_handle vh_remap_handle;
+/* remap_init (src/vilma_remap.f90:56-99) for host cell-centre axes
+ * lon_deg[nlon], lat_deg[nlat_ll] (degrees, strictly ascending, covering the
+ * sphere; longitudes within one turn) against the model's Gauss grid.
+ * Conservative weights host -> Gauss from cell boundaries at the axis
+ * midpoints (latitude ends at the poles, longitude cyclic); bilinear weights
+ * Gauss -> host. The fesm-utils `coords` maps the reference uses are
+ * replaced by this construction for lon-lat host grids. Errors go to
+ * vh_last_error(h); the map is used on h's stream and must be destroyed
+ * before h. */
+int vh_remap_create(vh_model* h, int nlon, int nlat_ll, const double* lon_deg, const double* lat_deg,
+                    vh_remap_handle** out);
+/* remap_to_gauss (src/vilma_remap.f90:101-138): f_ll (nlon, nlat_ll), lon
+ * fastest, latitude ascending -> f_gauss (nphi, nlat) in the model layout
+ * (north-first); conserve_mass != 0 rescales the result so that its Gauss
+ * quadrature integral equals the host area integral (ice thickness). */
+int vh_remap_to_gauss(vh_remap_handle* m, const double* f_ll, double* f_gauss, int conserve_mass);
+/* remap_to_ll (src/vilma_remap.f90:140-164): bilinear, f_gauss -> f_ll. */
+int vh_remap_to_ll(vh_remap_handle* m, const double* f_gauss, double* f_ll);
+/* The same with device (HBM) pointers. */
+int vh_remap_to_gauss_device(vh_remap_handle* m, const double* f_ll, double* f_gauss, int conserve_mass);
+int vh_remap_to_ll_device(vh_remap_handle* m, const double* f_gauss, double* f_ll);
+int vh_remap_destroy(vh_remap_handle* m);
+
 /* solid_earth_finalize (src/vilma_coupling.f90:599-626). */
 int vh_destroy(vh_model* h);
 
