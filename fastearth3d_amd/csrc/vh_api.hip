@@ -1306,6 +1306,18 @@ void from_lm(const vh_model* h, const double* in, double2* dm) {
         }
 }
 
+// the vh_remap_* calls: errors through the model-less channel of vh_last_error(NULL)
+template <class F>
+int remap_guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return 1;
+    }
+}
+
 template <class F>
 int guarded(vh_model* h, F&& f) {
     if (!h) return 2;
@@ -1503,54 +1515,65 @@ int vh_set_memory(vh_model* h, const double* tau) {
 // boundary, and the rotation state (m and both channels' memory in the
 // reference packing (NLAM, ne, 6) of rotation_get_memory,
 // src/vilma_rotation.f90:275-307).
-// ---- host lon-lat grid <-> Gauss grid (src/vilma_remap.f90)
+// ---- host lon-lat grid <-> Gauss grid (src/vilma_remap.f90); a map owns
+// its stream and reports errors through vh_last_error(NULL)
 struct vh_remap_handle {
-    vh_model* h;
-    vh_remap* r;
+    hipStream_t st = nullptr;
+    vh_remap* r = nullptr;
 };
 
-int vh_remap_create(vh_model* h, int nlon, int nlat_ll, const double* lon_deg, const double* lat_deg,
+int vh_remap_create(int nphi, int nlat, int nlon, int nlat_ll, const double* lon_deg, const double* lat_deg,
                     vh_remap_handle** out) {
-    return guarded(h, [&] {
+    return remap_guarded([&] {
         if (!out || !lon_deg || !lat_deg) throw vh_error("remap_init: null argument");
         *out = nullptr;
-        std::vector<double> ct, w;
-        gauss_legendre(h->d.nlat, ct, w);   // north-first nodes cos(colat), weights
-        vh_remap* r = vh_remap_build(h->st, h->d.nphi, h->d.nlat, h->colat.data(), w.data(), nlon, nlat_ll,
-                                     lon_deg, lat_deg);
-        *out = new vh_remap_handle{h, r};
+        if (nlat < 2 || nphi < 2) throw vh_error("remap_init: Gauss grid must be at least 2 x 2");
+        std::vector<double> ct, w, colat(nlat);
+        gauss_legendre(nlat, ct, w);   // north-first nodes cos(colat) and weights, as the model's grid
+        for (int j = 0; j < nlat; j++) colat[j] = std::acos(ct[j]);
+        auto* m = new vh_remap_handle();
+        try {
+            HIPCHECK(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
+            m->r = vh_remap_build(m->st, nphi, nlat, colat.data(), w.data(), nlon, nlat_ll, lon_deg, lat_deg);
+        } catch (...) {
+            if (m->st) (void)hipStreamDestroy(m->st);
+            delete m;
+            throw;
+        }
+        *out = m;
     });
 }
 
 int vh_remap_to_gauss(vh_remap_handle* m, const double* f_ll, double* f_gauss, int conserve_mass) {
     if (!m) return 2;
-    return guarded(m->h, [&] { vh_remap_to_gauss_host(m->r, f_ll, f_gauss, conserve_mass); });
+    return remap_guarded([&] { vh_remap_to_gauss_host(m->r, f_ll, f_gauss, conserve_mass); });
 }
 
 int vh_remap_to_ll(vh_remap_handle* m, const double* f_gauss, double* f_ll) {
     if (!m) return 2;
-    return guarded(m->h, [&] { vh_remap_to_ll_host(m->r, f_gauss, f_ll); });
+    return remap_guarded([&] { vh_remap_to_ll_host(m->r, f_gauss, f_ll); });
 }
 
 int vh_remap_to_gauss_device(vh_remap_handle* m, const double* f_ll, double* f_gauss, int conserve_mass) {
     if (!m) return 2;
-    return guarded(m->h, [&] {
+    return remap_guarded([&] {
         vh_remap_to_gauss_dev(m->r, f_ll, f_gauss, conserve_mass);
-        HIPCHECK(hipStreamSynchronize(m->h->st));
+        HIPCHECK(hipStreamSynchronize(m->st));
     });
 }
 
 int vh_remap_to_ll_device(vh_remap_handle* m, const double* f_gauss, double* f_ll) {
     if (!m) return 2;
-    return guarded(m->h, [&] {
+    return remap_guarded([&] {
         vh_remap_to_ll_dev(m->r, f_gauss, f_ll);
-        HIPCHECK(hipStreamSynchronize(m->h->st));
+        HIPCHECK(hipStreamSynchronize(m->st));
     });
 }
 
 int vh_remap_destroy(vh_remap_handle* m) {
     if (!m) return 0;
     vh_remap_free(m->r);
+    if (m->st) (void)hipStreamDestroy(m->st);
     delete m;
     return 0;
 }

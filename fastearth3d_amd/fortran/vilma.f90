@@ -25,6 +25,7 @@ module vilma
    public :: solid_earth_enable_visc_3d_field, solid_earth_enable_visc_3d_nodes
    public :: solid_earth_get_restart_state, solid_earth_set_restart_state
    public :: remap_ll_gauss, remap_init, remap_to_gauss, remap_to_ll, remap_free
+   public :: grid_class, grid_init
 
    integer, parameter :: wp = c_double
    integer, parameter :: MAX_LAYER = 16
@@ -106,6 +107,26 @@ module vilma
       real(wp), allocatable :: rsl(:,:), z_bed(:,:), C(:,:), C0(:,:), h_ice(:,:), s_rot(:,:)
    end type gauss_grid_fields
 
+   !! A map pair between a host lon-lat grid and a model's Gauss grid
+   !! (remap_ll_gauss, src/vilma_remap.f90:44-54): conservative in, bilinear out.
+   type :: remap_ll_gauss
+      type(c_ptr) :: m = c_null_ptr
+      integer :: nlon = 0, nlat_ll = 0        !! host dimensions
+      integer :: nphi = 0, nlat = 0           !! Gauss dimensions
+   end type remap_ll_gauss
+
+   !! The part of fesm-utils coords::grid_class the coupling reads (grid%G%x,
+   !! grid%G%y: lon-lat cell-centre axes in degrees, ascending), filled by
+   !! grid_init; a host passes it to solid_earth_init as it would the coords one.
+   type :: grid_axes
+      integer :: nx = 0, ny = 0
+      real(wp), allocatable :: x(:), y(:)
+   end type grid_axes
+   type :: grid_class
+      character(len=256) :: name = ""
+      type(grid_axes) :: G
+   end type grid_class
+
    type :: solid_earth
       type(vilma_param_class) :: par
       real(wp), allocatable :: rsl(:,:), z_bed(:,:)
@@ -114,16 +135,13 @@ module vilma
       type(gauss_grid_fields) :: gg
       type(c_ptr) :: h = c_null_ptr
       integer :: nphi = 0, nlat = 0
+      ! host-grid coupling (src/vilma_coupling.f90:115-117): the reference fields
+      ! at host resolution and the map pair, when the host grid is not the Gauss grid
+      logical :: remap = .false.
+      type(remap_ll_gauss) :: map
+      real(wp), allocatable :: z_bed_eq(:,:)
    end type solid_earth
 
-   !! A map pair between a host lon-lat grid and a model's Gauss grid
-   !! (remap_ll_gauss, src/vilma_remap.f90:44-54): conservative in, bilinear out.
-   type :: remap_ll_gauss
-      type(c_ptr) :: m = c_null_ptr
-      type(c_ptr) :: h = c_null_ptr          !! the model whose grid it maps to
-      integer :: nlon = 0, nlat_ll = 0        !! host dimensions
-      integer :: nphi = 0, nlat = 0           !! Gauss dimensions
-   end type remap_ll_gauss
 
    interface
       integer(c_int) function vh_create(p, nphi, nlat, z, hi, hi0, out) bind(c, name="vh_create")
@@ -188,10 +206,11 @@ module vilma
          type(c_ptr), value :: h, rsl, hi, m, lmem, tmem
          real(c_double), value :: t
       end function
-      integer(c_int) function vh_remap_create(h, nlon, nlat_ll, lon, lat, out) bind(c, name="vh_remap_create")
+      integer(c_int) function vh_remap_create(nphi, nlat, nlon, nlat_ll, lon, lat, out) &
+            bind(c, name="vh_remap_create")
          import :: c_int, c_ptr
-         type(c_ptr), value :: h, lon, lat
-         integer(c_int), value :: nlon, nlat_ll
+         integer(c_int), value :: nphi, nlat, nlon, nlat_ll
+         type(c_ptr), value :: lon, lat
          type(c_ptr), intent(out) :: out
       end function
       integer(c_int) function vh_remap_to_gauss(m, f, g, cm) bind(c, name="vh_remap_to_gauss")
@@ -398,38 +417,71 @@ contains
       call check(self, vh_get_field(self%h, 4, c_loc(self%gg%s_rot)), "solid_earth_update")
       call check(self, vh_get_field(self%h, 5, c_loc(self%gg%h_ice)), "solid_earth_update")
       call check(self, vh_get_spectrum(self%h, 0, c_loc(self%sigma_lm)), "solid_earth_update")
-      self%rsl = self%gg%rsl
-      self%z_bed = self%gg%z_bed
+      if (self%remap) then   ! solid_earth_sync_host (src/vilma_coupling.f90:439-449)
+         call remap_to_ll(self%map, self%gg%rsl, self%rsl)
+         self%z_bed = self%z_bed_eq - self%rsl
+      else
+         self%rsl = self%gg%rsl
+         self%z_bed = self%gg%z_bed
+      end if
       call check(self, vh_get_scalar(self%h, 0, v), "solid_earth_update"); self%time = v
       call check(self, vh_get_scalar(self%h, 1, v), "solid_earth_update"); self%bsl = v
       call check(self, vh_get_scalar(self%h, 2, v), "solid_earth_update"); self%worst_mass_resid = v
    end subroutine refresh
 
-   !! solid_earth_init (src/vilma_coupling.f90:144-232), passthrough grid only;
-   !! the caller sets self%par (e.g. by vilma_par_load) first.
+   !! solid_earth_init (src/vilma_coupling.f90:144-232); the caller sets
+   !! self%par (e.g. by vilma_par_load) first. With a grid that is not the
+   !! model's Gauss grid the reference fields come on the host grid and are
+   !! mapped in (conservative; the ice with its mass kept), and rsl / z_bed are
+   !! returned on it (bilinear), as the reference's host-grid coupling does.
    subroutine solid_earth_init(self, z_bed_eq, h_ice_eq, grid, h_ice_init)
       type(solid_earth), intent(inout), target :: self
       real(wp), intent(in), target :: z_bed_eq(:,:), h_ice_eq(:,:)
-      class(*), intent(in), optional :: grid
+      type(grid_class), intent(in), optional :: grid
       real(wp), intent(in), target, optional :: h_ice_init(:,:)
       type(vh_params_c), target :: c
       type(c_ptr) :: hi0
       integer(c_int) :: nlm
+      integer :: gnphi, gnlat
       real(wp), allocatable, target :: z(:,:), hi(:,:), h0(:,:)
-      if (present(grid)) error stop 'solid_earth_init: grid remapping is not supported (fields must be on the Gauss grid)'
-      self%nphi = size(z_bed_eq, 1); self%nlat = size(z_bed_eq, 2)
+      self%remap = .false.
+      if (present(grid)) then
+         call gauss_dims(self%par, gnphi, gnlat)
+         self%remap = .not. grid_is_gauss(grid, gnphi, gnlat)
+      end if
+      if (self%remap) then
+         if (size(z_bed_eq, 1) /= grid%G%nx .or. size(z_bed_eq, 2) /= grid%G%ny) &
+            error stop 'solid_earth_init: z_bed_eq must be on the host grid (nx, ny)'
+         call remap_init_dims(self%map, gnphi, gnlat, grid%G%x, grid%G%y)
+         self%nphi = gnphi;  self%nlat = gnlat
+         self%z_bed_eq = z_bed_eq
+         allocate(z(gnphi, gnlat), hi(gnphi, gnlat))
+         call map_in(self%map, z_bed_eq, z, conserve_mass=.false.)     ! bed: geometry
+         call map_in(self%map, h_ice_eq, hi, conserve_mass=.true.)     ! ice: mass
+      else
+         self%nphi = size(z_bed_eq, 1); self%nlat = size(z_bed_eq, 2)
+         z = z_bed_eq; hi = h_ice_eq
+      end if
       c = to_c(self%par)
-      z = z_bed_eq; hi = h_ice_eq
       hi0 = c_null_ptr
       if (present(h_ice_init)) then
-         h0 = h_ice_init
+         if (self%remap) then
+            allocate(h0(gnphi, gnlat))
+            call map_in(self%map, h_ice_init, h0, conserve_mass=.true.)
+         else
+            h0 = h_ice_init
+         end if
          hi0 = c_loc(h0)
       end if
       self%h = c_null_ptr
       call check(self, vh_create(c, int(self%nphi, c_int), int(self%nlat, c_int), c_loc(z), c_loc(hi), hi0, &
                                  self%h), "solid_earth_init")
       call check(self, vh_get_dims(self%h, 3, nlm), "solid_earth_init")
-      allocate(self%rsl(self%nphi, self%nlat), self%z_bed(self%nphi, self%nlat), self%sigma_lm(nlm))
+      if (self%remap) then
+         allocate(self%rsl(grid%G%nx, grid%G%ny), self%z_bed(grid%G%nx, grid%G%ny), self%sigma_lm(nlm))
+      else
+         allocate(self%rsl(self%nphi, self%nlat), self%z_bed(self%nphi, self%nlat), self%sigma_lm(nlm))
+      end if
       allocate(self%gg%rsl(self%nphi, self%nlat), self%gg%z_bed(self%nphi, self%nlat), &
                self%gg%C(self%nphi, self%nlat), self%gg%C0(self%nphi, self%nlat), &
                self%gg%h_ice(self%nphi, self%nlat), self%gg%s_rot(self%nphi, self%nlat))
@@ -444,9 +496,14 @@ contains
       real(wp), intent(in) :: dt_yr
       real(wp), allocatable, target :: hi(:,:)
       if (.not. c_associated(self%h)) error stop 'solid_earth_update: model not initialised'
-      if (size(h_ice, 1) /= self%nphi .or. size(h_ice, 2) /= self%nlat) &
-         error stop 'solid_earth_update: h_ice must be on the model grid (nphi, nlat)'
-      hi = h_ice
+      if (self%remap) then   ! ice in conservatively, its mass kept (src/vilma_coupling.f90:355-359)
+         allocate(hi(self%nphi, self%nlat))
+         call map_in(self%map, h_ice, hi, conserve_mass=.true.)
+      else
+         if (size(h_ice, 1) /= self%nphi .or. size(h_ice, 2) /= self%nlat) &
+            error stop 'solid_earth_update: h_ice must be on the model grid (nphi, nlat)'
+         hi = h_ice
+      end if
       call check(self, vh_update(self%h, c_loc(hi), real(dt_yr, c_double)), "solid_earth_update")
       call refresh(self)
    end subroutine solid_earth_update
@@ -502,6 +559,36 @@ contains
       call refresh(self)
    end subroutine solid_earth_set_restart_state
 
+   !! nonzero status of a vh_remap_* call -> error stop with its message
+   subroutine check_remap(rc, what)
+      integer(c_int), intent(in) :: rc
+      character(len=*), intent(in) :: what
+      type(c_ptr) :: m
+      character(kind=c_char), pointer :: s(:)
+      character(len=:), allocatable :: msg
+      integer :: i, n
+      if (rc == 0) return
+      m = vh_last_error(c_null_ptr)
+      n = int(c_strlen(m))
+      call c_f_pointer(m, s, [n])
+      allocate(character(len=n) :: msg)
+      do i = 1, n
+         msg(i:i) = s(i)
+      end do
+      write(*, '(a)') trim(what) // ': ' // msg
+      error stop 'vilma: ' // what // ' failed (message above)'
+   end subroutine check_remap
+
+   !! the model Gauss grid of a parameter record (build_sht, src/vilma_coupling.f90:644-662;
+   !! NLAT_MIN = 32, src/vilma_sht.f90:72)
+   subroutine gauss_dims(p, nphi, nlat)
+      type(vilma_param_class), intent(in) :: p
+      integer, intent(out) :: nphi, nlat
+      nlat = p%nlat;  if (nlat <= 0) nlat = 2*p%lmax + 2
+      nphi = p%nphi;  if (nphi <= 0) nphi = 4*p%lmax
+      nlat = max(nlat, 32)
+   end subroutine gauss_dims
+
    !! remap_init (src/vilma_remap.f90:56-99) for source cell-centre axes
    !! lon_src(:), lat_src(:) [degrees, ascending] against the Gauss grid of the
    !! model se (the reference passes its sht; no weight cache: the weights are
@@ -509,21 +596,39 @@ contains
    subroutine remap_init(self, se, lon_src, lat_src)
       type(remap_ll_gauss), intent(out) :: self
       type(solid_earth), intent(in) :: se
-      real(wp), intent(in), target, contiguous :: lon_src(:), lat_src(:)
-      self%nlon = size(lon_src);  self%nlat_ll = size(lat_src)
-      self%nphi = se%nphi;        self%nlat = se%nlat
-      self%h = se%h
-      call check(se, vh_remap_create(se%h, int(self%nlon, c_int), int(self%nlat_ll, c_int), c_loc(lon_src), &
-                 c_loc(lat_src), self%m), "remap_init")
+      real(wp), intent(in) :: lon_src(:), lat_src(:)
+      call remap_init_dims(self, se%nphi, se%nlat, lon_src, lat_src)
    end subroutine remap_init
+
+   subroutine remap_init_dims(self, nphi, nlat, lon_src, lat_src)
+      type(remap_ll_gauss), intent(out) :: self
+      integer, intent(in) :: nphi, nlat
+      real(wp), intent(in) :: lon_src(:), lat_src(:)
+      real(wp), allocatable, target :: lon(:), lat(:)
+      lon = lon_src;  lat = lat_src
+      self%nlon = size(lon);  self%nlat_ll = size(lat)
+      self%nphi = nphi;       self%nlat = nlat
+      call check_remap(vh_remap_create(int(nphi, c_int), int(nlat, c_int), int(self%nlon, c_int), &
+                       int(self%nlat_ll, c_int), c_loc(lon), c_loc(lat), self%m), "remap_init")
+   end subroutine remap_init_dims
 
    !! remap_to_gauss (src/vilma_remap.f90:101-138): f_ll(nlon, nlat_ll) -> f_gauss(nphi, nlat)
    subroutine remap_to_gauss(self, se, f_ll, f_gauss, conserve_mass)
       type(remap_ll_gauss), intent(in) :: self
       type(solid_earth), intent(in) :: se
-      real(wp), intent(in), target, contiguous :: f_ll(:,:)
-      real(wp), intent(out), target, contiguous :: f_gauss(:,:)
+      real(wp), intent(in) :: f_ll(:,:)
+      real(wp), intent(out) :: f_gauss(:,:)
       logical, optional, intent(in) :: conserve_mass
+      if (se%nphi /= self%nphi .or. se%nlat /= self%nlat) error stop 'vilma_remap: map and model grids differ'
+      call map_in(self, f_ll, f_gauss, conserve_mass)
+   end subroutine remap_to_gauss
+
+   subroutine map_in(self, f_ll, f_gauss, conserve_mass)
+      type(remap_ll_gauss), intent(in) :: self
+      real(wp), intent(in) :: f_ll(:,:)
+      real(wp), intent(out) :: f_gauss(:,:)
+      logical, optional, intent(in) :: conserve_mass
+      real(wp), allocatable, target :: fi(:,:), fo(:,:)
       integer(c_int) :: cm
       if (size(f_ll,1) /= self%nlon .or. size(f_ll,2) /= self%nlat_ll) &
          error stop 'vilma_remap: source field shape /= map source grid'
@@ -533,29 +638,84 @@ contains
       if (present(conserve_mass)) then
          if (conserve_mass) cm = 1
       end if
-      call check(se, vh_remap_to_gauss(self%m, c_loc(f_ll), c_loc(f_gauss), cm), "remap_to_gauss")
-   end subroutine remap_to_gauss
+      fi = f_ll
+      allocate(fo(self%nphi, self%nlat))
+      call check_remap(vh_remap_to_gauss(self%m, c_loc(fi), c_loc(fo), cm), "remap_to_gauss")
+      f_gauss = fo
+   end subroutine map_in
 
    !! remap_to_ll (src/vilma_remap.f90:140-164): f_gauss(nphi, nlat) -> f_ll(nlon, nlat_ll)
-   subroutine remap_to_ll(self, se, f_gauss, f_ll)
+   subroutine remap_to_ll(self, f_gauss, f_ll)
       type(remap_ll_gauss), intent(in) :: self
-      type(solid_earth), intent(in) :: se
-      real(wp), intent(in), target, contiguous :: f_gauss(:,:)
-      real(wp), intent(out), target, contiguous :: f_ll(:,:)
+      real(wp), intent(in) :: f_gauss(:,:)
+      real(wp), intent(out) :: f_ll(:,:)
+      real(wp), allocatable, target :: gi(:,:), fo(:,:)
       if (size(f_gauss,1) /= self%nphi .or. size(f_gauss,2) /= self%nlat) &
          error stop 'vilma_remap: source field shape /= SHTns grid'
       if (size(f_ll,1) /= self%nlon .or. size(f_ll,2) /= self%nlat_ll) &
          error stop 'vilma_remap: target field shape /= map target grid'
-      call check(se, vh_remap_to_ll(self%m, c_loc(f_gauss), c_loc(f_ll)), "remap_to_ll")
+      gi = f_gauss
+      allocate(fo(self%nlon, self%nlat_ll))
+      call check_remap(vh_remap_to_ll(self%m, c_loc(gi), c_loc(fo)), "remap_to_ll")
+      f_ll = fo
    end subroutine remap_to_ll
 
-   !! release the map (before its model's solid_earth_finalize)
+   !! release the map
    subroutine remap_free(self)
       type(remap_ll_gauss), intent(inout) :: self
       integer(c_int) :: rc
       if (c_associated(self%m)) rc = vh_remap_destroy(self%m)
       self%m = c_null_ptr
    end subroutine remap_free
+
+   !! coords::grid_init for a lon-lat grid: the cell-centre axes in degrees
+   subroutine grid_init(grid, name, mtype, units, lon180, x, y)
+      type(grid_class), intent(out) :: grid
+      character(len=*), intent(in), optional :: name, mtype, units
+      logical, intent(in), optional :: lon180
+      real(wp), intent(in) :: x(:), y(:)
+      if (present(mtype)) then
+         if (trim(mtype) /= "latlon") error stop 'grid_init: only mtype="latlon" grids are supported'
+      end if
+      if (present(units)) then
+         if (trim(units) /= "degrees") error stop 'grid_init: axes must be in degrees'
+      end if
+      if (present(lon180)) continue
+      if (present(name)) grid%name = name
+      grid%G%nx = size(x);  grid%G%ny = size(y)
+      grid%G%x = x;  grid%G%y = y
+   end subroutine grid_init
+
+   !! grid_is_gauss (src/vilma_coupling.f90:664-690): the host grid IS the
+   !! model's Gauss grid (dimensions, and axes within 1e-6 degrees)
+   logical function grid_is_gauss(grid, nphi, nlat) result(same)
+      type(grid_class), intent(in) :: grid
+      integer, intent(in) :: nphi, nlat
+      real(wp), parameter :: TOL = 1.0e-6_wp, PI = 3.14159265358979323846_wp
+      real(wp) :: x, p0, p1, p2, dp
+      integer :: j, k, it
+      same = .false.
+      if (grid%G%nx /= nphi .or. grid%G%ny /= nlat) return
+      do j = 1, nphi
+         if (abs(grid%G%x(j) - 360.0_wp*real(j - 1, wp)/real(nphi, wp)) > TOL) return
+      end do
+      ! Gauss-Legendre nodes by Newton's method on P_nlat (ascending latitude)
+      do j = 1, nlat
+         x = -cos(PI*(real(j, wp) - 0.25_wp)/(real(nlat, wp) + 0.5_wp))
+         do it = 1, 100
+            p0 = 1.0_wp;  p1 = x
+            do k = 2, nlat
+               p2 = (real(2*k - 1, wp)*x*p1 - real(k - 1, wp)*p0)/real(k, wp)
+               p0 = p1;  p1 = p2
+            end do
+            dp = real(nlat, wp)*(x*p1 - p0)/(x*x - 1.0_wp)
+            if (abs(p1/dp) < 1.0e-15_wp) exit
+            x = x - p1/dp
+         end do
+         if (abs(grid%G%y(j) - asin(x)*180.0_wp/PI) > TOL) return
+      end do
+      same = .true.
+   end function grid_is_gauss
 
    !! solid_earth_finalize (src/vilma_coupling.f90:599-626)
    subroutine solid_earth_finalize(self)
@@ -568,6 +728,9 @@ contains
       if (allocated(self%rsl)) deallocate(self%rsl, self%z_bed, self%sigma_lm)
       if (allocated(self%gg%rsl)) deallocate(self%gg%rsl, self%gg%z_bed, self%gg%C, self%gg%C0, &
                                              self%gg%h_ice, self%gg%s_rot)
+      if (self%remap) call remap_free(self%map)
+      if (allocated(self%z_bed_eq)) deallocate(self%z_bed_eq)
+      self%remap = .false.
    end subroutine solid_earth_finalize
 
 end module vilma

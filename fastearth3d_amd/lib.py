@@ -52,8 +52,8 @@ def load():
         L.vh_adjoint_store_limit.argtypes = [C.c_void_p, C.c_double]
         L.vh_get_state.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 6
         L.vh_set_state.argtypes = [C.c_void_p, C.c_double] + [C.POINTER(C.c_double)] * 5
-        L.vh_remap_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                      C.POINTER(C.c_void_p)]
+        L.vh_remap_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_void_p)]
         L.vh_remap_to_gauss.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
         L.vh_remap_to_ll.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.vh_remap_to_gauss_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
@@ -219,33 +219,37 @@ class Remap:
     model layout (north-first)."""
 
     def __init__(self, model, lon_deg, lat_deg):
-        self._m = model
-        self._lib = model._lib
+        self._lib = load()
+        self.nphi, self.nlat = model.nphi, model.nlat
         lon = np.ascontiguousarray(lon_deg, dtype=np.float64)
         lat = np.ascontiguousarray(lat_deg, dtype=np.float64)
         self.nlon, self.nlat_ll = lon.size, lat.size
         h = C.c_void_p()
-        model._check(self._lib.vh_remap_create(model._h, self.nlon, self.nlat_ll, _cp(lon), _cp(lat), C.byref(h)),
-                     "remap_init")
+        self._check(self._lib.vh_remap_create(self.nphi, self.nlat, self.nlon, self.nlat_ll, _cp(lon), _cp(lat),
+                                              C.byref(h)), "remap_init")
         self._r = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._lib.vh_last_error(None)
+            raise VilmaError("%s: %s" % (what, msg.decode() if msg else "failed"))
 
     def to_gauss(self, f_ll, conserve_mass=False):
         """remap_to_gauss (src/vilma_remap.f90:101-138)."""
         f = np.ascontiguousarray(f_ll, dtype=np.float64)
         if f.size != self.nlon * self.nlat_ll:
             raise VilmaError("remap_to_gauss: source field shape /= map source grid")
-        g = np.empty((self._m.nlat, self._m.nphi))
-        self._m._check(self._lib.vh_remap_to_gauss(self._r, _cp(f), _cp(g), int(bool(conserve_mass))),
-                       "remap_to_gauss")
+        g = np.empty((self.nlat, self.nphi))
+        self._check(self._lib.vh_remap_to_gauss(self._r, _cp(f), _cp(g), int(bool(conserve_mass))), "remap_to_gauss")
         return g
 
     def to_ll(self, f_gauss):
         """remap_to_ll (src/vilma_remap.f90:140-164)."""
         g = np.ascontiguousarray(f_gauss, dtype=np.float64)
-        if g.size != self._m.nlat * self._m.nphi:
+        if g.size != self.nlat * self.nphi:
             raise VilmaError("remap_to_ll: source field shape /= Gauss grid")
         f = np.empty((self.nlat_ll, self.nlon))
-        self._m._check(self._lib.vh_remap_to_ll(self._r, _cp(g), _cp(f)), "remap_to_ll")
+        self._check(self._lib.vh_remap_to_ll(self._r, _cp(g), _cp(f)), "remap_to_ll")
         return f
 
     def close(self):

@@ -159,14 +159,14 @@ int vh_set_state(vh_model* h, double time_yr, const double* rsl, const double* h
 typedef struct vh_remap_handle vh_remap_handle;
 /* remap_init (src/vilma_remap.f90:56-99) for host cell-centre axes
  * lon_deg[nlon], lat_deg[nlat_ll] (degrees, strictly ascending, covering the
- * sphere; longitudes within one turn) against the model's Gauss grid.
+ * sphere; longitudes within one turn) against the Gauss grid of a model with
+ * nphi x nlat points (vh_get_dims; the model itself is not needed).
  * Conservative weights host -> Gauss from cell boundaries at the axis
  * midpoints (latitude ends at the poles, longitude cyclic); bilinear weights
  * Gauss -> host. The fesm-utils `coords` maps the reference uses are
- * replaced by this construction for lon-lat host grids. Errors go to
- * vh_last_error(h); the map is used on h's stream and must be destroyed
- * before h. */
-int vh_remap_create(vh_model* h, int nlon, int nlat_ll, const double* lon_deg, const double* lat_deg,
+ * replaced by this construction for lon-lat host grids. The map owns a
+ * stream; errors of the vh_remap_* calls are in vh_last_error(NULL). */
+int vh_remap_create(int nphi, int nlat, int nlon, int nlat_ll, const double* lon_deg, const double* lat_deg,
                     vh_remap_handle** out);
 /* remap_to_gauss (src/vilma_remap.f90:101-138): f_ll (nlon, nlat_ll), lon
  * fastest, latitude ascending -> f_gauss (nphi, nlat) in the model layout

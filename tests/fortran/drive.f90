@@ -1,20 +1,27 @@
 !! Host-side driver of the Fortran module `vilma` for the tests:
 !!   drive <namelist> --print-params [defaults]  (CPU only: vilma_par_load result)
-!!   drive <namelist> <input.bin> <output.bin>  (GPU: init + update loop)
+!!   drive <namelist> <input.bin> <output.bin> [--host-grid]  (GPU: init + update loop)
 !! input.bin (stream): int32 nphi, nlat, nsteps; real64 dt_yr; z_bed_eq(nphi,nlat);
 !! h_ice(nphi,nlat,nsteps). output.bin: rsl, z_bed, C (nphi,nlat), bsl, time.
+!! With --host-grid the fields are on a lon-lat host grid: nphi, nlat are its
+!! dimensions, and its axes lon(nphi), lat(nlat) [degrees] follow dt_yr; rsl
+!! and z_bed come back on it (C stays on the Gauss grid).
 program drive
    use vilma
    implicit none
    type(solid_earth) :: se
-   character(len=512) :: nml, a2, a3
+   type(grid_class) :: grid
+   character(len=512) :: nml, a2, a3, a4
    integer :: nphi, nlat, nsteps, k, u, i
    real(wp) :: dt
-   real(wp), allocatable :: z(:,:), hice(:,:,:), heq(:,:)
+   real(wp), allocatable :: z(:,:), hice(:,:,:), heq(:,:), lon(:), lat(:)
+   logical :: host
 
    call get_command_argument(1, nml)
    call get_command_argument(2, a2)
    call get_command_argument(3, a3)
+   call get_command_argument(4, a4)
+   host = trim(a4) == "--host-grid"
    if (trim(a2) == "--print-params" .and. len_trim(a3) > 0) then
       call vilma_par_load(se%par, trim(nml), defaults_file=trim(a3))
    else
@@ -36,12 +43,22 @@ program drive
    open(newunit=u, file=trim(a2), access='stream', form='unformatted', status='old', action='read')
    read(u) nphi, nlat, nsteps
    read(u) dt
+   if (host) then
+      allocate(lon(nphi), lat(nlat))
+      read(u) lon
+      read(u) lat
+      call grid_init(grid, name="host", mtype="latlon", units="degrees", x=lon, y=lat)
+   end if
    allocate(z(nphi, nlat), heq(nphi, nlat), hice(nphi, nlat, nsteps))
    read(u) z
    read(u) hice
    close(u)
    heq = 0.0_wp
-   call solid_earth_init(se, z, heq)
+   if (host) then
+      call solid_earth_init(se, z, heq, grid=grid)
+   else
+      call solid_earth_init(se, z, heq)
+   end if
    do k = 1, nsteps
       call solid_earth_update(se, hice(:, :, k), dt)
    end do

@@ -3,15 +3,19 @@
 CPU: vilma_par_load through the intrinsic NAMELIST gives the same record as
 the Python loader (overlay of a run file on a defaults file, years -> s).
 GPU: a Fortran host running solid_earth_init / solid_earth_update through
-the module matches the Python host on the same library bit for bit.
+the module matches the Python host on the same library bit for bit, on the
+Gauss grid and on a lon-lat host grid (through the remap).
 """
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
 
 from fastearth3d_amd import build, configs, namelist
+
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 DEFAULTS = """&vilma
   solver = "v2"
@@ -84,15 +88,7 @@ def test_fortran_rejects_bad_frame(tmp_path):
     assert "deg1_frame" in res.stdout + res.stderr
 
 
-@pytest.mark.gpu
-def test_fortran_host_matches_python_host(tmp_path):
-    """Fortran module -> C-ABI vs Python ctypes -> C-ABI: identical state."""
-    from fastearth3d_amd import lib
-    drv = _driver()
-    c = configs.config(1, lmax=16)
-    p = c["params"]
-    p.rotation = 0
-    nphi, nlat = p.grid()
+def _namelist(tmp_path, p):
     nml = tmp_path / "c.nml"
     lines = ["&vilma", "  lmax = %d" % p.lmax, "  earth = \"custom\"", "  n_layer = %d" % p.n_layer,
              "  r_earth = %r, r_core = %r" % (p.r_earth, p.r_core)]
@@ -102,7 +98,29 @@ def test_fortran_host_matches_python_host(tmp_path):
         lines.append("  %s = %s" % (key, ", ".join(str(getattr(p, key)[i]) for i in range(p.n_layer))))
     lines += ["  rotation = .false.", "  deg1_frame = \"%s\"" % p.deg1_frame.decode(), "/"]
     nml.write_text("\n".join(lines) + "\n")
-    q = namelist.vilma_par_load(str(nml), defaults_file=None)
+    return nml
+
+
+def _run_both(tmp_path, nml, fin, extra=()):
+    """The Fortran driver and the Python host (tests/fortran/py_host.py), each in
+    a process of its own: a process that has run the cooperative sea-level
+    kernel keeps the GPU's cooperative-launch resource, so the test process
+    itself stays off the GPU. Returns both output files' bytes."""
+    ff, fp = tmp_path / "out_f.bin", tmp_path / "out_p.bin"
+    subprocess.run([_driver(), str(nml), str(fin), str(ff)] + list(extra), check=True, timeout=300)
+    subprocess.run([sys.executable, os.path.join(HERE, "fortran", "py_host.py"), str(nml), str(fin), str(fp)]
+                   + list(extra), check=True, timeout=300)
+    return np.fromfile(ff, np.float64), np.fromfile(fp, np.float64)
+
+
+@pytest.mark.gpu
+def test_fortran_host_matches_python_host(tmp_path):
+    """Fortran module -> C-ABI vs Python ctypes -> C-ABI: identical state."""
+    c = configs.config(1, lmax=16)
+    p = c["params"]
+    p.rotation = 0
+    nphi, nlat = p.grid()
+    nml = _namelist(tmp_path, p)
     ice = np.stack([c["ice"](k + 60) for k in range(3)])
     fin = tmp_path / "in.bin"
     with open(fin, "wb") as f:
@@ -110,14 +128,42 @@ def test_fortran_host_matches_python_host(tmp_path):
         np.array([c["dt_yr"]], np.float64).tofile(f)
         c["z_bed_eq"].astype(np.float64).tofile(f)   # (nlat, nphi) C order == (nphi, nlat) Fortran order
         ice.astype(np.float64).tofile(f)
-    fout = tmp_path / "out.bin"
-    subprocess.run([drv, str(nml), str(fin), str(fout)], check=True, timeout=300)
-    o = np.fromfile(fout, np.float64)
-    n = nphi * nlat
-    rsl_f, C_f, bsl_f = o[:n].reshape(nlat, nphi), o[2 * n:3 * n].reshape(nlat, nphi), o[3 * n]
-    se = lib.SolidEarth(q, c["z_bed_eq"], np.zeros_like(c["z_bed_eq"]))
-    for k in range(3):
-        se.update(ice[k], c["dt_yr"])
-    assert np.array_equal(rsl_f, se.rsl)
-    assert np.array_equal(C_f, se.C)
-    assert bsl_f == se.bsl
+    of, op = _run_both(tmp_path, nml, fin)
+    assert of.size == op.size == 3 * nphi * nlat + 2
+    assert np.array_equal(of, op)                   # rsl, z_bed, C, bsl, time
+    assert np.abs(of[:nphi * nlat]).max() > 0.0
+
+
+@pytest.mark.gpu
+def test_fortran_host_grid_coupling(tmp_path):
+    """solid_earth_init with a lon-lat host grid (src/vilma_coupling.f90:172-184,
+    355-359, 439-449): ice and bed mapped in conservatively (the ice with its
+    mass kept), rsl and z_bed returned bilinearly on the host grid. The Fortran
+    host equals a Python host doing the same maps around the same model."""
+    import remap as R
+    c = configs.config(1, lmax=16)
+    p = c["params"]
+    p.rotation = 0
+    nml = _namelist(tmp_path, p)
+    nphi, nlat = p.grid()
+    # a 4 x 3 degree host grid; the host fields sampled (bilinearly, on the CPU)
+    # from the Gauss-grid ones
+    lon = 2.0 + 4.0 * np.arange(90)
+    lat = -88.5 + 3.0 * np.arange(60)
+    z_ll = R.to_ll(c["z_bed_eq"], lon, lat, nlat, nphi)
+    ice_ll = np.stack([np.maximum(R.to_ll(c["ice"](k + 60), lon, lat, nlat, nphi), 0.0) for k in range(3)])
+    fin = tmp_path / "in.bin"
+    with open(fin, "wb") as f:
+        np.array([lon.size, lat.size, 3], np.int32).tofile(f)
+        np.array([c["dt_yr"]], np.float64).tofile(f)
+        lon.astype(np.float64).tofile(f)
+        lat.astype(np.float64).tofile(f)
+        z_ll.astype(np.float64).tofile(f)
+        ice_ll.astype(np.float64).tofile(f)
+    of, op = _run_both(tmp_path, nml, fin, ["--host-grid"])
+    nh = lon.size * lat.size
+    assert of.size == op.size == 2 * nh + nphi * nlat + 2
+    assert np.array_equal(of, op)                   # host rsl, host z_bed, Gauss C, bsl, time
+    rsl = of[:nh].reshape(lat.size, lon.size)
+    assert np.abs(rsl).max() > 0.0
+    assert np.array_equal(of[nh:2 * nh].reshape(lat.size, lon.size), z_ll - rsl)
