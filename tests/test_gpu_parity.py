@@ -1,6 +1,6 @@
 """HIP path vs the oracle, through the C-ABI (marked gpu).
 
-Tolerances follow the north-star contract: 1e-10 relative per step on the
+Tolerances of the parity contract: 1e-10 relative per step on the
 spectral displacement/potential coefficients and the rsl/z_bed fields, 1e-8
 after multi-step runs, and a bit-exact ocean/coastline mask. Sizes are small
 enough for the oracle to finish in seconds, plus full-size (lmax 128)
