@@ -143,6 +143,8 @@ __global__ void k_bsl_integrand(int n, const double* __restrict__ h, const doubl
 #define VH_SLE_LB 4
 // grid values per lane a row wavefront loads ahead of its transform
 #define VH_SLE_PU 6
+// longitudes per thread and row whose inputs the pass head loads together
+#define VH_SLE_PH 2
 
 __device__ __forceinline__ void sle_grid_sync(unsigned* bar, unsigned& nbar, double* res) {
     nbar += 1u;
@@ -455,19 +457,38 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     if (is_row) {
         double* red = (double*)RB;   // [6][256]: (quantity, row of the pair)
         double sp[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        // the inputs of a thread's points of both rows (VH_SLE_PH per row) are
+        // loaded before any is used; sums per thread in the same (row, longitude) order
+        for (int i0 = 0; i0 < N; i0 += 256 * VH_SLE_PH) {
+            double vC0[2][VH_SLE_PH], vice[2][VH_SLE_PH], vdice[2][VH_SLE_PH], vtopo[2][VH_SLE_PH],
+                vx[2][VH_SLE_PH];
 #pragma unroll
-        for (int w = 0; w < 2; w++) {
-            const int j = 2 * p + w;
-            if (j >= nlat) continue;
-            for (int i = tid; i < N; i += 256) {
-                const size_t id = (size_t)j * N + i;
-                const double c = a.fixed_ocean ? a.C0[id]
-                                               : (io == 1 ? ocean(a.topo0[id] - a.rsl[id], a.ice[id]) : a.Cn[id]);
-                const double g = a.ice[id] * (1.0 - c) - (a.ice[id] - a.dice[id]) * (1.0 - a.C0[id]);
-                a.C[id] = c;
-                a.dig[id] = g;
-                sp[3 * w + 0] += c; sp[3 * w + 1] += g; sp[3 * w + 2] += a.topo0[id] * (c - a.C0[id]);
-            }
+            for (int w = 0; w < 2; w++)
+#pragma unroll
+                for (int u = 0; u < VH_SLE_PH; u++) {
+                    const int j = 2 * p + w, i = i0 + tid + 256 * u;
+                    const bool ok = j < nlat && i < N;
+                    const size_t id = ok ? (size_t)j * N + i : 0;
+                    vC0[w][u] = ok ? a.C0[id] : 0.0;
+                    vice[w][u] = ok ? a.ice[id] : 0.0;
+                    vdice[w][u] = ok ? a.dice[id] : 0.0;
+                    vtopo[w][u] = ok ? a.topo0[id] : 0.0;
+                    vx[w][u] = (!ok || a.fixed_ocean) ? 0.0 : (io == 1 ? a.rsl[id] : a.Cn[id]);
+                }
+#pragma unroll
+            for (int w = 0; w < 2; w++)
+#pragma unroll
+                for (int u = 0; u < VH_SLE_PH; u++) {
+                    const int j = 2 * p + w, i = i0 + tid + 256 * u;
+                    if (j >= nlat || i >= N) continue;
+                    const size_t id = (size_t)j * N + i;
+                    const double c = a.fixed_ocean ? vC0[w][u]
+                                                   : (io == 1 ? ocean(vtopo[w][u] - vx[w][u], vice[w][u]) : vx[w][u]);
+                    const double g = vice[w][u] * (1.0 - c) - (vice[w][u] - vdice[w][u]) * (1.0 - vC0[w][u]);
+                    a.C[id] = c;
+                    a.dig[id] = g;
+                    sp[3 * w + 0] += c; sp[3 * w + 1] += g; sp[3 * w + 2] += vtopo[w][u] * (c - vC0[w][u]);
+                }
         }
 #pragma unroll
         for (int q = 0; q < 6; q++) red[q * 256 + tid] = sp[q];
@@ -494,9 +515,17 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
             tw3[t] = s.w[q] * ld_coh(a.part3 + t) * dlon;
         }
         __syncthreads();
-        if (tid < 3) {
+        if (tid < 3) {   // in latitude order, 8 terms read ahead of their sums
             double total = 0.0;
-            for (int q = 0; q < nlat; q++) total += tw3[tid * nlat + q];
+            int q = 0;
+            for (; q + 8 <= nlat; q += 8) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) v[u] = tw3[tid * nlat + q + u];
+#pragma unroll
+                for (int u = 0; u < 8; u++) total += v[u];
+            }
+            for (; q < nlat; q++) total += tw3[tid * nlat + q];
             stot[tid] = total;
         }
         __syncthreads();
