@@ -432,7 +432,10 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     const int tstride = is_row ? 1 : ((L - m + 1) | 1);
     if (is_row) {
         for (int t = tid; t < s.ntw; t += 256) TW[t] = s.twp[t];
-    } else if (TabLds) {
+    }
+    // the order blocks' Legendre table is loaded after the first barrier of the
+    // first pass, while the row blocks run the pass's start transforms
+    auto load_table = [&]() {
         // 8 table values per thread in flight
         const int nl = L - m + 1, nt = nh * nl;
         const double* Pa = s.plm_jl + (size_t)s.mstart[m] * nh;
@@ -446,7 +449,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
                 if (t < nt) T[(t / nl) * tstride + t % nl] = v[u];
             }
         }
-    }
+    };
     __syncthreads();
 
   for (int io = 1; io <= a.n_outer; io++) {
@@ -529,6 +532,9 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
             stot[tid] = total;
         }
         __syncthreads();
+    }
+    if constexpr (TabLds) {
+        if (io == 1 && !is_row) load_table();   // read by the order stage after the next barrier
     }
     const double C_int = stot[0];
     if (blk == 0 && tid == 0) a.res[3] = C_int;
