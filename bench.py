@@ -8,9 +8,10 @@ by default the lmax 128 lateral-viscosity configuration (64 radial elements,
 the metric is quoted on. The ice history is uploaded to HBM before the timed
 region and handed over as device pointers (vh_update_device). W untimed warm-up
 calls, then K timed calls bracketed by device synchronisations. The JSON line
-adds the roofline of the dominant kernel (algorithmic bytes per launch over its
-HIP-event-timed average duration) and the oracle's CPU throughput on a bounded
-sample of the same workload.
+adds the roofline of the dominant kernel (algorithmic bytes — or, for the
+tensor Legendre kernels, FP64 flops — per launch over its HIP-event-timed
+average duration) and the oracle's CPU throughput on a bounded sample of the
+same workload.
 """
 import argparse
 import ctypes as C
@@ -25,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 FP64_PEAK_TFS = 78.6
 NAMES = {0: "k_drift", 1: "k_advance", 2: "k_tsh_synth", 3: "k_t3_rows", 4: "k_tsh_analysis", 5: "k_t3_gather",
          6: "k_sle_solve"}
@@ -77,6 +79,17 @@ def alg_bytes(dims, cls, chunk_batches=None, sle_iters=None):
         # sigma, u, N), plus the loop's start (mask and load transforms)
         per_it = 8 * nlat * nphi * 8 + 3 * nlm * 16
         return sle_iters * per_it + 2 * nlat * nphi * 8
+    return None
+
+
+def alg_flops(dims, cls, chunk_batches=None):
+    """FP64 flops of one launch of a tensor Legendre kernel (synthesis, class 2;
+    analysis, class 4): per element-shape batch 12 nlm nlat — six table
+    products over the parity-folded latitudes, each a complex x real
+    multiply-add (4 flops) per (l, m) and latitude pair. None for the other
+    classes, which are priced by their HBM bytes."""
+    if cls in (2, 4) and chunk_batches:
+        return chunk_batches * 12.0 * dims["nlm"] * dims["nlat"]
     return None
 
 
@@ -526,8 +539,18 @@ def main():
         if not ab:
             continue
         avg_s = ms[k] / cnt[k] / 1000.0
-        ach = ab / avg_s / 1e9
         t = traffic.get(NAMES[k])
+        fl = alg_flops(d, k, nbat / nch if nbat else None)
+        if fl:   # the tensor Legendre kernels: FP64 MFMA-bound
+            ach = fl / avg_s / 1e12
+            per_kernel[NAMES[k]] = {"bound": "mfma", "achieved": round(ach, 2), "peak": FP64_PEAK_TFLOPS,
+                                    "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4),
+                                    "traffic": t["hbm_bytes_per_launch"] if t else None,
+                                    "avg_launch_us": round(avg_s * 1e6, 2), "launches": int(cnt[k]),
+                                    "alg_flops_per_launch": int(fl), "alg_bytes_per_launch": int(ab),
+                                    "total_ms": round(ms[k], 3)}
+            continue
+        ach = ab / avg_s / 1e9
         per_kernel[NAMES[k]] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": round(ach / HBM_PEAK_GBS, 4),
                                 "traffic": t["hbm_bytes_per_launch"] if t else None,
