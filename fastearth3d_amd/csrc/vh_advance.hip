@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(256) k_advance(advance_args a) {
                 const size_t o1 = base + (size_t)((sh * 2 + 1) * 4 + lam) * nk;
                 const double t0 = a.tau_in[o0] + dv.x, t1 = a.tau_in[o1] + dv.y;
                 a.tau_out[o0] = t0; a.tau_out[o1] = t1;
-                amax = fmax(amax, fmax(fabs(t0), fabs(t1)));
+                amax = vh_nanmax(amax, vh_nanmax(fabs(t0), fabs(t1)));
             }
         } else {
             const int l = a.v.kdeg[k];
@@ -80,17 +80,17 @@ __global__ void __launch_bounds__(256) k_advance(advance_args a) {
                     const double sv = sh == 0 ? s[ri][0][lam] : (sh == 1 ? s[ri][1][lam] : s[ri][2][lam]);
                     const double t = om * a.tau_in[off] - two_muM * sv;
                     a.tau_out[off] = t;
-                    amax = fmax(amax, fabs(t));
+                    amax = vh_nanmax(amax, fabs(t));
                 }
         }
     }
     red[wg][lane] = amax;
     __syncthreads();
     if (wg == 0) {
-        double m = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
+        double m = vh_nanmax(vh_nanmax(red[0][lane], red[1][lane]), vh_nanmax(red[2][lane], red[3][lane]));
         if (k < nk) a.pmax[(size_t)blockIdx.y * nk + k] = m;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+        for (int off = 32; off > 0; off >>= 1) m = vh_nanmax(m, __shfl_xor(m, off));
         if (lane == 0) a.pblk[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = m;
     }
 }
@@ -109,16 +109,16 @@ __global__ void __launch_bounds__(256) k_advance_max(int nk, int ny, int nblk, c
         double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
         int i = threadIdx.x;
         for (; i + 3 * (int)blockDim.x < nblk; i += 4 * blockDim.x) {
-            m0 = fmax(m0, pblk[i]);
-            m1 = fmax(m1, pblk[i + blockDim.x]);
-            m2 = fmax(m2, pblk[i + 2 * blockDim.x]);
-            m3 = fmax(m3, pblk[i + 3 * blockDim.x]);
+            m0 = vh_nanmax(m0, pblk[i]);
+            m1 = vh_nanmax(m1, pblk[i + blockDim.x]);
+            m2 = vh_nanmax(m2, pblk[i + 2 * blockDim.x]);
+            m3 = vh_nanmax(m3, pblk[i + 3 * blockDim.x]);
         }
-        for (; i < nblk; i += blockDim.x) m0 = fmax(m0, pblk[i]);
-        red[threadIdx.x] = fmax(fmax(m0, m1), fmax(m2, m3));
+        for (; i < nblk; i += blockDim.x) m0 = vh_nanmax(m0, pblk[i]);
+        red[threadIdx.x] = vh_nanmax(vh_nanmax(m0, m1), vh_nanmax(m2, m3));
         __syncthreads();
         for (int s = 128; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+            if ((int)threadIdx.x < s) red[threadIdx.x] = vh_nanmax(red[threadIdx.x], red[threadIdx.x + s]);
             __syncthreads();
         }
         if (threadIdx.x == 0) {
@@ -134,10 +134,10 @@ __global__ void __launch_bounds__(256) k_advance_max(int nk, int ny, int nblk, c
     int y = 0;
     for (; y + 8 <= ny; y += 8) {
 #pragma unroll
-        for (int u = 0; u < 8; u++) m[u] = fmax(m[u], pmax[(size_t)(y + u) * nk + k]);
+        for (int u = 0; u < 8; u++) m[u] = vh_nanmax(m[u], pmax[(size_t)(y + u) * nk + k]);
     }
-    for (; y < ny; y++) m[0] = fmax(m[0], pmax[(size_t)y * nk + k]);
-    mnorm[k] = fmax(fmax(fmax(m[0], m[1]), fmax(m[2], m[3])), fmax(fmax(m[4], m[5]), fmax(m[6], m[7])));
+    for (; y < ny; y++) m[0] = vh_nanmax(m[0], pmax[(size_t)y * nk + k]);
+    mnorm[k] = vh_nanmax(vh_nanmax(vh_nanmax(m[0], m[1]), vh_nanmax(m[2], m[3])), vh_nanmax(vh_nanmax(m[4], m[5]), vh_nanmax(m[6], m[7])));
 }
 
 size_t vh_advance_scratch(const vh_dims& d) {
@@ -180,18 +180,18 @@ __global__ void __launch_bounds__(256) k_memnorm(int nk, int nem, const double* 
     double amax = 0.0;
     if (k < nk)
         for (int em = eg; em < nem; em += 4)
-            for (int c = 0; c < 24; c++) amax = fmax(amax, fabs(tau[((size_t)em * 24 + c) * nk + k]));
+            for (int c = 0; c < 24; c++) amax = vh_nanmax(amax, fabs(tau[((size_t)em * 24 + c) * nk + k]));
     red[eg][lane] = amax;
     __syncthreads();
     if (eg == 0) {
-        double m = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
+        double m = vh_nanmax(vh_nanmax(red[0][lane], red[1][lane]), vh_nanmax(red[2][lane], red[3][lane]));
         if (k < nk) mnorm[k] = m;
         red[0][lane] = m;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = 0.0;
-        for (int i = 0; i < 64; i++) m = fmax(m, red[0][i]);
+        for (int i = 0; i < 64; i++) m = vh_nanmax(m, red[0][i]);
         atomicMax(maxbits, (unsigned long long)__double_as_longlong(m));
     }
 }

@@ -24,6 +24,11 @@
 
 struct vh_error : std::runtime_error { using std::runtime_error::runtime_error; };
 
+// max that propagates NaN (fmax returns the non-NaN operand): the memory norm
+// feeds the stability guard, whose finite test must see a NaN memory value
+// (src/vilma_timestep.f90:195: "finite = err_inf <= huge", false for NaN)
+__host__ __device__ __forceinline__ double vh_nanmax(double a, double b) { return (b > a || b != b) ? b : a; }
+
 #define VH_KL 6
 #define VH_KV 12
 #define VH_LDAB 19

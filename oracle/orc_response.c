@@ -140,12 +140,18 @@ void orc_resp_free(orc_resp* r) {
     memset(r, 0, sizeof(*r));
 }
 
+/* max that keeps a NaN (fmax drops it): the memory norm feeds the stability
+ * guard, whose finite test (src/vilma_timestep.f90:195, "false for NaN /
+ * +Inf") must see a NaN memory value */
+static inline double nanmax(double a, double b) { return (b > a || b != b) ? b : a; }
+
+/* absmax6 (src/vilma_response.f90:811-826) */
 static double absmax6(size_t n, const double* x1, const double* x2, const double* x3,
                       const double* x4, const double* x5, const double* x6) {
     double a = 0.0;
     for (size_t i = 0; i < n; i++) {
-        a = fmax(a, fabs(x1[i])); a = fmax(a, fabs(x2[i])); a = fmax(a, fabs(x3[i]));
-        a = fmax(a, fabs(x4[i])); a = fmax(a, fabs(x5[i])); a = fmax(a, fabs(x6[i]));
+        a = nanmax(a, fabs(x1[i])); a = nanmax(a, fabs(x2[i])); a = nanmax(a, fabs(x3[i]));
+        a = nanmax(a, fabs(x4[i])); a = nanmax(a, fabs(x5[i])); a = nanmax(a, fabs(x6[i]));
     }
     return a;
 }
@@ -160,7 +166,7 @@ void orc_resp_begin(orc_resp* r) {
         r->mnorm[k] = absmax6(nb, r->Are + k * nb, r->Bre + k * nb, r->Cre + k * nb,
                               r->Aim + k * nb, r->Bim + k * nb, r->Cim + k * nb);
     double mx = 0.0;
-    for (int k = 0; k < r->nk; k++) mx = fmax(mx, r->mnorm[k]);
+    for (int k = 0; k < r->nk; k++) mx = nanmax(mx, r->mnorm[k]);
     double thr = r->skip_tol * mx;
 #pragma omp parallel
     {
@@ -338,11 +344,16 @@ double orc_resp_max_rate(const orc_resp* r) {
 double orc_resp_memory_norm(const orc_resp* r) {
     if (r->kind != RESP_VE || !r->Are) return 0.0;
     size_t nb = (size_t)4 * r->ne;
-    double nrm = 0.0;
-#pragma omp parallel for reduction(max : nrm) schedule(static)
+    /* response_memory_norm (src/vilma_response.f90:1795-1811): per-slot
+     * maxima in parallel, then one ordered NaN-keeping max */
+    double* sk = malloc((size_t)r->nk * sizeof(double));
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < r->nk; k++)
-        nrm = fmax(nrm, absmax6(nb, r->Are + k * nb, r->Aim + k * nb, r->Bre + k * nb,
-                                r->Bim + k * nb, r->Cre + k * nb, r->Cim + k * nb));
+        sk[k] = absmax6(nb, r->Are + k * nb, r->Aim + k * nb, r->Bre + k * nb,
+                        r->Bim + k * nb, r->Cre + k * nb, r->Cim + k * nb);
+    double nrm = 0.0;
+    for (int k = 0; k < r->nk; k++) nrm = nanmax(nrm, sk[k]);
+    free(sk);
     return nrm;
 }
 
