@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <vector>
 
 #include "vh_internal.h"
@@ -206,8 +207,7 @@ struct vh_model {
     double *d_sraw = 0, *d_srot = 0, *d_zbed = 0, *d_gtmp = 0;
     double2 *d_F = 0, *d_FC = 0, *d_F2 = 0, *d_sigma = 0, *d_lm = 0, *d_uN = 0;
     double *d_part = 0, *d_sc = 0;
-    unsigned* d_sbar = 0;   // grid-barrier counter + timeout flag of the fused SLE loop
-    long sbar_budget = 0;   // most the counter can have advanced since its last reset
+    unsigned* d_sbar = 0;   // synchronisation area of the fused sea-level solve
     double* h_pin = nullptr;
     // rotation
     bool rot_on = false;
@@ -218,6 +218,9 @@ struct vh_model {
     int n_outer = 3, n_inner = 20, fixed_ocean = 0, subgrid = 1;
     double tol = 1e-7, cfl = 1, dt_min = 0, dt_max = DBL_MAX, guard = 1e3;
     long n_accept = 0, n_reject = 0, n_solve = 0, n_inner_tot = 0, n_outer_tot = 0;
+    long n_sle_steal = 0;    // sea-level work items run for a block that was not resident
+    int sle_late_blocks = 0;     // diagnostic (VH_SLE_LATE_BLOCKS): solve blocks that start late
+    unsigned sle_late_ticks = 0;
     // kernel timing (HIP events on the model's stream), per kernel class
     bool prof = false;
     struct evpair { hipEvent_t a, b; int cls; };
@@ -493,8 +496,18 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
     h->d_uN = dalloc<double2>((size_t)2 * d.nlm);
     h->d_part = dalloc<double>((size_t)7 * nl + 64);
     h->d_sc = dalloc<double>(64); vh_fill(st, h->d_sc, 0.0, 64);
-    h->d_sbar = dalloc<unsigned>(VH_SLE_BAR_WORDS);
-    sle_bar_reset(st, h->d_sbar);
+    h->d_sbar = dalloc<unsigned>(VH_SLE_BAR_WORDS);   // reset before every solve (sle_ice)
+    if (const char* late = getenv("VH_SLE_LATE_BLOCKS")) {
+        // diagnostic "count:microseconds": that many of the solve's blocks
+        // start that much later than the others (at most 1 s), so the others
+        // run their work items; the results do not change
+        int nb = 0;
+        double us = 0.0;
+        if (sscanf(late, "%d:%lf", &nb, &us) == 2 && nb > 0 && us > 0.0) {
+            h->sle_late_blocks = std::min(nb, sle_solve_grid(h->s));
+            h->sle_late_ticks = (unsigned)(std::min(us, 1.0e6) * 100.0);
+        }
+    }
 
     // ---- SLE / stepper knobs (build_solver, src/vilma_coupling.f90:291-308)
     h->n_outer = p->sle_n_outer; h->n_inner = p->sle_n_inner; h->tol = p->sle_tol;
@@ -644,28 +657,24 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
     // sc[16..28): the solve's results (zeroed with wcorr by the ice kernel),
     // the new memory norm at [11], all read back once at the end of the solve
     double* r = h->d_sc + 16;
-    sle_ice(st, n, ice0, ice1, frac, interp, h->d_heq, h->d_topo0, h->d_ice, h->d_dice, h->d_C0s, h->d_wcorr, r, 12);
+    sle_ice(st, n, ice0, ice1, frac, interp, h->d_heq, h->d_topo0, h->d_ice, h->d_dice, h->d_C0s, h->d_wcorr, r, 12,
+            h->d_sbar, sle_solve_grid(s), h->n_outer);
     if (h->kind == 2) {
         h->prof_begin(VH_K_DRIFT);
         vh_drift_run(st, h->dplan, h->d, h->v, h->d_tau[h->cur], h->d_mnorm[h->cur], h->d_maxn + h->cur, h->skip_tol,
                      h->deg1_cm, h->d_dN, h->d_dsurf);
         h->prof_end();
     }
-    // the coastline passes and inner fixed point in one cooperative launch
+    // the coastline passes and inner fixed point in one launch
     sle_solve_args ia{};
     ia.s = s; ia.topo0 = h->d_topo0; ia.ice = h->d_ice; ia.dice = h->d_dice; ia.C0 = h->d_C0s; ia.srot = srot;
     ia.gu = h->d_gu; ia.gn = h->d_gn; ia.dsurf = h->d_dsurf;
-    ia.C = h->d_C; ia.Cn = h->d_Cn; ia.dig = h->d_dig; ia.wcorr = h->d_wcorr;
+    ia.C = h->d_C; ia.Cn = h->d_Cn; ia.dig = h->d_dig; ia.wcorr = h->d_wcorr; ia.sraw = h->d_sraw;
     ia.nk = h->d.nk; ia.kind = h->kind; ia.subgrid = h->subgrid; ia.n_inner = h->n_inner; ia.n_outer = h->n_outer;
     ia.fixed_ocean = h->fixed_ocean; ia.g = h->g; ia.tol = h->tol; ia.rsl = h->d_rsl;
     ia.part = h->d_part; ia.cs = h->d_part + 3 * (size_t)s.nlat; ia.part3 = h->d_part + 4 * (size_t)s.nlat;
     ia.res = r; ia.F = h->d_F; ia.FC = h->d_FC; ia.F2 = h->d_F2; ia.sigma = h->d_sigma; ia.bar = h->d_sbar;
-    const long step = sle_solve_max_barriers(s, h->n_outer, h->n_inner);
-    if (h->sbar_budget + step >= (1L << 31)) {   // keep the running counter far from wrap-around
-        sle_bar_reset(st, h->d_sbar);
-        h->sbar_budget = 0;
-    }
-    h->sbar_budget += step;
+    ia.late_blocks = h->sle_late_blocks; ia.late_ticks = h->sle_late_ticks;
     h->prof_begin(VH_K_SLE);
     sle_solve_launch(st, ia);
     h->prof_end();
@@ -678,11 +687,8 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
         HIPCHECK(hipMemcpyAsync(r + 11, h->d_maxn + h->cur, sizeof(double), hipMemcpyDeviceToDevice, st));
     h->read(r, 12);
     const double* hr = h->h_pin;
-    if (hr[2] != 0.0) {
-        sle_bar_reset(st, h->d_sbar);
-        h->sbar_budget = 0;
-        throw vh_error("sle_solve: a grid barrier of the fused sea-level loop timed out");
-    }
+    if (hr[2] != 0.0) throw vh_error("sle_solve: a wait of the fused sea-level solve exceeded its bound");
+    h->n_sle_steal += (long)hr[8];
     h->n_inner_tot += (long)hr[4];
     h->n_outer_tot += (long)hr[5];
     const double ice_int = -(VH_RHO_ICE / VH_RHO_WATER) * hr[6];
@@ -1414,6 +1420,7 @@ int vh_get_scalar(vh_model* h, int which, double* out) {
             case VH_S_N_SOLVE: *out = (double)h->n_solve; break;
             case VH_S_N_INNER_TOT: *out = (double)h->n_inner_tot; break;
             case VH_S_N_OUTER_TOT: *out = (double)h->n_outer_tot; break;
+            case VH_S_N_SLE_STEAL: *out = (double)h->n_sle_steal; break;
             case VH_S_POLAR_M_RE:
             case VH_S_POLAR_M_IM:
                 if (!h->rot_on) { *out = 0.0; break; }
@@ -1503,6 +1510,21 @@ int vh_set_memory(vh_model* h, const double* tau) {
                         for (int k = 0; k < d.nk; k++) dst[k] = src[((size_t)k * d.ne + e) * 4 + lam];
                     }
         }
+        // elastic and fluid elements have a zero Maxwell rate, so the update
+        // leaves their memory at its start value, zero (src/vilma_viscoelastic.f90:
+        // 577-584 with Mk = 0); the library keeps memory for Maxwell elements
+        // only, so a state that gives the others memory is refused, not dropped
+        std::vector<char> maxwell(d.ne, 0);
+        for (int em = 0; em < d.nem; em++) maxwell[h->emap[em]] = 1;
+        for (size_t a6 = 0; a6 < 6; a6++)
+            for (int k = 0; k < d.nk; k++)
+                for (int e = 0; e < d.ne; e++) {
+                    if (maxwell[e]) continue;
+                    const double* v = tau + a6 * n1 + ((size_t)k * d.ne + e) * 4;
+                    if (v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0 || v[3] != 0.0)
+                        throw vh_error("vh_set_memory: memory of an elastic or fluid element must be zero (element " +
+                                       std::to_string(e + 1) + ")");
+                }
         upload(h->d_tau[h->cur], dev.data(), dev.size(), h->st);
         vh_launch_memnorm(h->st, d, h->d_tau[h->cur], h->d_mnorm[h->cur], h->d_maxn + h->cur);
         h->read(h->d_maxn + h->cur, 1);

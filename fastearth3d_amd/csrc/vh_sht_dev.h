@@ -145,8 +145,8 @@ __device__ __forceinline__ void fft_pass(const double2* src, double2* dst, int N
 // holding the result. Natural order in and out (Stockham autosort). The
 // caller makes a's contents visible to the whole wavefront first. tw: a copy
 // of the s.ntw per-pass twiddles (e.g. in LDS), or null for s.twp.
-__device__ double2* fft_wave(double2* a, double2* b, const vh_sht& s, bool inv, int lane,
-                             const double2* tw = nullptr) {
+__device__ __forceinline__ double2* fft_wave(double2* a, double2* b, const vh_sht& s, bool inv, int lane,
+                                             const double2* tw = nullptr) {
     const int N = s.nphi;
     int Ns = 1;
     double2 *src = a, *dst = b;

@@ -8,8 +8,8 @@
 // weighted sum) so a rerun gives identical bits.
 #pragma clang fp contract(off)
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
-#include <mutex>
 #include "vh_internal.h"
 #include "vh_sht_dev.h"
 #include "vh_sle.h"
@@ -19,12 +19,25 @@ __device__ __forceinline__ double ocean(double topo, double ice) {
 }
 
 // ice(t) = ice0 + frac*(ice1 - ice0), d_ice = ice - ice_ref, C0 = O(topo0, ice - d_ice)
-// (src/vilma_timestep.f90:314-326, src/vilma_sle.f90:216)
+// (src/vilma_timestep.f90:314-326, src/vilma_sle.f90:216); also the reset of
+// the fused solve's synchronisation area (the solve that follows on the same
+// stream starts from stage 0 with the record of a first pass)
 __global__ void k_sle_ice(int n, const double* __restrict__ ice0, const double* __restrict__ ice1, double frac,
                           int interp, const double* __restrict__ ice_ref, const double* __restrict__ topo0,
-                          double* ice, double* dice, double* C0, double* wcorr, double* res, int nres) {
+                          double* ice, double* dice, double* C0, double* wcorr, double* res, int nres, unsigned* bar,
+                          int nitems, int n_outer) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nres) res[i] = 0.0;
+    if (bar) {
+        if (i < nitems) bar[VH_SLE_CLAIM_OFF + VH_SLE_CLAIM_STRIDE * i] = 0u;
+        if (i < 8) { bar[VH_SLE_FLAG_STRIDE * (1 + i)] = 0u; bar[VH_SLE_FLAG_STRIDE * (10 + i)] = 0u; }
+        if (i < 4) bar[16 * (1 + i)] = i == 0 ? 1u : 0u;   // record slot 0 holds stage 0
+        if (i == 0) { bar[VH_SLE_FLAG_STRIDE * 9] = 0u; bar[1] = 0u; bar[2] = 0u; }
+        if (i < 16) {
+            double* ctl = (double*)(bar + VH_SLE_CTL_OFF);
+            ctl[i] = i == 0 ? (n_outer >= 1 ? 0.0 : 6.0) : (i == 1 ? 1.0 : 0.0);   // kind HEAD (DONE), pass 1
+        }
+    }
     if (i >= n) return;
     wcorr[i] = 0.0;
     double v = interp ? ice0[i] + frac * (ice1[i] - ice0[i]) : ice0[i];
@@ -33,15 +46,6 @@ __global__ void k_sle_ice(int n, const double* __restrict__ ice0, const double* 
     dice[i] = d;
     C0[i] = ocean(topo0[i], v - d);
 }
-
-struct InLoad {
-    const double *dig, *C, *rsl, *wcorr;
-    int nphi;
-    __device__ double operator()(int row, int i) const {
-        size_t id = (size_t)row * nphi + i;
-        return VH_RHO_ICE * dig[id] + VH_RHO_WATER * (C[id] * rsl[id]) + wcorr[id];
-    }
-};
 
 // response_apply (src/vilma_response.f90:842-864) into two spectra [u; N]
 // (degree-major): u = gu*sigma + dUa, N = gn*sigma - dFa/g.
@@ -102,42 +106,57 @@ __global__ void k_bsl_integrand(int n, const double* __restrict__ h, const doubl
     if (i < n) out[i] = h[i] * (1.0 - C[i]) - heq[i] * (1.0 - C0[i]);
 }
 
-// ------------------------------------------------------------ host wrappers
 
-// ---------------------------------------------------------------- fused inner loop
-// The inner fixed point of sle_solve (src/vilma_sle.f90:296-330) in one
-// cooperative launch, so no iteration waits on the host. Row blocks own a
-// latitude pair p (rows 2p, 2p+1), order blocks an order m.
-//   start  rows:   FC = FFT(C), F = FFT(load(rsl))            [barrier]
-//          orders: F -> sigma -> (u, N) -> F2                  [barrier]
-//   per iteration
-//          rows:   inverse FFT of (u_j, N_j) -> Sraw_j (kept in LDS),
-//                  cs_j = sum C Sraw; base = rho_i dI + rho_w C Sraw + wcorr,
-//                  Fb = FFT(base)                              [barrier]
-//          all:    Cs_int = sum_j w_j dlon cs_j, dphi
-//          rows:   rsl = Sraw + dphi, row maxima of |C drsl|, |C rsl|
-//          orders: the next load's spectrum Fb + rho_w dphi FC (the transform
-//                  of rho_i dI + rho_w C rsl + wcorr, by linearity) ->
-//                  sigma -> (u, N) -> F2                        [barrier]
-//          all:    the same exit decision from the row maxima.
-// On exit sigma is the spectrum of the load of the final rsl: the closing
-// analysis of the solve. Each stage's arithmetic is fixed by the sizes alone
-// (partition of the Legendre sums, order of the Cs_int butterfly), so every
-// block reaches the same values and a rerun gives identical bits.
-// Data handed between blocks (F, FC, F2, cs, part, part3) moves through
-// agent-scope loads and stores (ld_coh / st_coh), which bypass the XCDs'
-// non-coherent caches, so a grid barrier needs no cache write-back or
-// invalidate: each wavefront waits for its stores, then the block arrives in
-// the counter of its group blockIdx % 8; the last block of a group arrives in
-// the top counter, and the block that completes the top count writes the
-// barrier number into eight release words, which the blocks of each group poll
-// (all counters and words on pages of their own, so the arrivals of a hundred
-// blocks that finish together do not queue on one address). The counts run on
-// across launches from the barrier number kept in bar[2]. Data from the
-// kernels before the launch is acquired once at entry. The wait is bounded: a
-// block that never arrives raises bar[1] and res[2] instead of a hang.
+// ---------------------------------------------------------------- fused solve
+// The coastline passes and inner fixed point of sle_solve
+// (src/vilma_sle.f90:231-355) in one launch of ordinary blocks, so no
+// iteration waits on the host. The solve is a sequence of stages; each stage
+// has one work item per block of the grid: latitude pair p (rows 2p, 2p+1)
+// for items p < npairs, order m = item - npairs for the others (an item may
+// have nothing to do in a stage):
+//   HEAD   rows: mask C (first pass from topo0 - rsl, later the previous
+//          pass's C_next; fixed ocean: C0), grounded-ice increment dig and
+//          the row sums of C, dig and topo0 (C - C0)
+//   START  rows: subgrid term wcorr, FC = FFT(C), F = FFT(load(rsl));
+//          orders: their Legendre table into LDS
+//   ORD0   orders: F -> sigma -> (u, N) -> F2
+//   ROWS   rows: inverse FFT of (u_j, N_j) -> Sraw_j, cs_j = sum C Sraw,
+//          base = rho_i dig + rho_w C Sraw + wcorr, F = FFT(base)
+//   UPD    rows: rsl = Sraw + dphi, row maxima of |C drsl|, |C rsl|;
+//          orders: the next load's spectrum F + rho_w dphi FC (the transform
+//          of rho_i dig + rho_w C rsl + wcorr, by linearity) -> sigma -> F2
+//   COAST  rows: C_next from topo0 - rsl, flipped cells per row
+// in the order HEAD START ORD0 (ROWS UPD)* COAST per pass, then DONE. On
+// exit sigma is the spectrum of the load of the final rsl: the closing
+// analysis of the solve.
+// After a stage every block in step (one that holds its item of the next
+// stage, so the stage's outputs stay in place) takes the decision the next
+// stage needs (pass totals and C_int; dphi; the inner exit test; the
+// coastline fixed point) from the stage's outputs; the block whose arrival
+// completed the stage also publishes it, with the running counts, in a
+// control record that blocks which fell behind read instead.
+// Each stage's arithmetic is fixed by the sizes alone (partition of the
+// Legendre sums, order of the totals), so a rerun gives identical bits.
+// No block waits for a block that has not started: block b owns item b of
+// every stage and claims the next stage's item before it arrives at the
+// current stage's end; a block that has waited VH_SLE_STEAL_TICKS on a stage
+// claims and runs the stage's unclaimed items (whose owners are not resident:
+// a device shared with other work or another process). A stage therefore
+// completes as long as one block runs, and its result does not depend on
+// which block ran an item. Arrivals go to the counter of the item's group
+// (item % 8); the last item of a group arrives in the top counter (counters
+// on pages of their own, so arrivals that come together do not queue on one
+// address). Grid data written during the launch moves through agent-scope
+// loads and stores (ld_coh / st_coh, coherent across the XCDs' L2s), so no
+// cache write-back or invalidate is needed between stages; inputs written by
+// earlier kernels are acquired once at entry. A block keeps LDS copies (its
+// row pair's Sraw, the FFT twiddles, its order's Legendre table) and uses
+// them only while they are valid. Every wait is bounded: past
+// VH_SLE_WAIT_TICKS the error flag bar[1] and res[2] are raised and every
+// block leaves.
 
-#define VH_SLE_SPIN_LIMIT 2000000u
+#define VH_SLE_STEAL_TICKS 4000ull         // 40 us at the 100 MHz s_memrealtime clock
+#define VH_SLE_WAIT_TICKS 400000000ull     // 4 s
 // P-table values a thread has in flight in the order stage
 #define VH_SLE_JB 8
 #define VH_SLE_LB 4
@@ -145,56 +164,45 @@ __global__ void k_bsl_integrand(int n, const double* __restrict__ h, const doubl
 #define VH_SLE_PU 6
 // longitudes per thread and row whose inputs the pass head loads together
 #define VH_SLE_PH 2
+// longitudes per thread and row loaded together by the later row stages
+#define VH_SLE_UB 4
+// static LDS of the solve kernel (prefetch stage, records, flags), bytes
+#define VH_SLE_STATIC_LDS (4 * 256 * 16 + 1024)
 
-__device__ __forceinline__ void sle_grid_sync(unsigned* bar, unsigned& nbar, double* res) {
-    nbar += 1u;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_s_waitcnt(0);   // this wavefront's stores are complete
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned g = blockIdx.x & 7u, nb = gridDim.x;
-        const unsigned ng = (nb - g + 7u) / 8u, ngroups = nb < 8u ? nb : 8u;
-        unsigned* flag = bar + VH_SLE_FLAG_STRIDE * (1 + g);
-        // arrival in the group counter; the group's last block arrives in the
-        // top counter, and the last group's block raises the release words
-        if (__hip_atomic_fetch_add(bar + VH_SLE_FLAG_STRIDE * (10 + g), 1u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT) + 1u == ng * nbar &&
-            __hip_atomic_fetch_add(bar + VH_SLE_FLAG_STRIDE * 9, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u ==
-                ngroups * nbar) {
-#pragma unroll
-            for (int k = 0; k < 8; k++)
-                __hip_atomic_store(bar + VH_SLE_FLAG_STRIDE * (1 + k), nbar, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        }
-        unsigned spins = 0;
-        while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nbar) {
-            if (__hip_atomic_load(bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
-            if (++spins > VH_SLE_SPIN_LIMIT) {
-                __hip_atomic_fetch_or(bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                res[2] = 1.0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-    __syncthreads();
+enum { SK_HEAD = 0, SK_START, SK_ORD0, SK_ROWS, SK_UPD, SK_COAST, SK_DONE };
+// control record of a stage (doubles; small integers are exact)
+enum { CT_KIND = 0, CT_IO, CT_II, CT_CINT, CT_TOT1, CT_TOT2, CT_DPHI, CT_NITER, CT_NPASS, CT_LICE, CT_LZETA,
+       CT_LCS, CT_ITLAST, CT_DPLAST, CT_STEALS, CT_N = 16 };
+
+__device__ __forceinline__ unsigned* sle_rel(unsigned* bar, int g) { return bar + VH_SLE_FLAG_STRIDE * (1 + g); }
+__device__ __forceinline__ unsigned* sle_top(unsigned* bar) { return bar + VH_SLE_FLAG_STRIDE * 9; }
+__device__ __forceinline__ unsigned* sle_grp(unsigned* bar, int g) { return bar + VH_SLE_FLAG_STRIDE * (10 + g); }
+__device__ __forceinline__ unsigned* sle_claim(unsigned* bar, int i) {
+    return bar + VH_SLE_CLAIM_OFF + VH_SLE_CLAIM_STRIDE * i;
+}
+__host__ __device__ __forceinline__ double* sle_ctl(unsigned* bar, unsigned sn) {
+    return (double*)(bar + VH_SLE_CTL_OFF) + (sn & 3u) * CT_N;
+}
+__device__ __forceinline__ unsigned ld_u(const unsigned* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// LDS of a row block: Sraw [2][nphi] doubles (kept across a barrier), the FFT
-// twiddles (s.ntw complex) and two FFT buffer pairs (2 x 2 nphi complex; the
-// pass head's row sums use the same space).
-// Of an order block: with TabLds its Legendre
-// table [j][l] (row stride (L+1)|1 doubles, loaded once per launch), then the
-// folds, partial sums and (sigma, u, N).
+// LDS of the row work: Sraw [2][nphi] doubles, the FFT twiddles (s.ntw
+// complex) and two FFT buffer pairs (2 x 2 nphi complex; the pass head's row
+// sums and the decisions use the same space). Of the order work: with TabLds
+// the order's Legendre table [j][l] (row stride (L+1)|1 doubles), then the
+// folds, partial sums and (sigma, u, N). The two layouts share the block's LDS.
 __host__ __device__ static size_t sle_tab_doubles(const vh_sht& s) {
     return (((size_t)s.nh * ((s.L + 1) | 1)) + 1) & ~(size_t)1;
 }
+__host__ __device__ static size_t sle_rows_bytes(const vh_sht& s) {
+    return (size_t)2 * s.nphi * sizeof(double) + (size_t)s.ntw * sizeof(double2) +
+           std::max((size_t)4 * s.nphi * sizeof(double2), (size_t)6 * 256 * sizeof(double));
+}
 static size_t sle_solve_lds(const vh_sht& s, bool tab) {
-    const size_t rows = (size_t)2 * s.nphi * sizeof(double) + (size_t)s.ntw * sizeof(double2) +
-                        std::max((size_t)4 * s.nphi * sizeof(double2), (size_t)6 * 256 * sizeof(double));
     const size_t order = (tab ? sle_tab_doubles(s) * sizeof(double) : 0) +
                          ((size_t)2 * s.nh + std::max(256, s.L + 1) + 3 * (s.L + 1) + 16 * (size_t)s.nh) * sizeof(double2);
-    return std::max(rows, order);
+    return std::max(sle_rows_bytes(s), order);
 }
 
 // wavefront sum of v over the block's lanes 0..63 (butterfly: every lane the same bits)
@@ -203,29 +211,11 @@ __device__ __forceinline__ double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
 }
-
-// dphi = (ice_int - Cs_int + zeta) / C_int with Cs_int from the row sums cs
-// (wavefront 0; result in *sd for the block)
-__device__ __forceinline__ void sle_dphi(const sle_solve_args& a, const double* tot, int nlat, double dlon, int wave,
-                                         int lane, double* sd, double* scs) {
-    if (wave == 0) {
-        double total = 0.0;
-        for (int q = lane; q < nlat; q += 64) total += a.s.w[q] * ld_coh(a.cs + q) * dlon;
-        total = wave_sum(total);
-        if (lane == 0) {
-            *scs = total;
-            const double rho_ratio = VH_RHO_ICE / VH_RHO_WATER;
-            const double ice_int = -rho_ratio * tot[1];
-            const double zeta = a.subgrid ? tot[2] : 0.0;
-            *sd = (ice_int - total + zeta) / tot[0];
-        }
-    }
-}
-
-// Order stage of block m: fold of the load spectrum (F, plus kdp * FC when
-// kdp != 0) -> Legendre analysis -> sigma -> response -> Legendre synthesis of
-// (u, N) -> F2. fn/fs: this thread's north / south spectrum values (j = tid < nh)
-// already loaded by the caller.
+// Order work of order m: fold of the load spectrum -> Legendre analysis ->
+// sigma -> response -> Legendre synthesis of (u, N) -> F2. fn/fs: this
+// thread's north / south spectrum values (j = tid < nh) already loaded by the
+// caller. TabLds: the order's table is in T (LDS); otherwise it is read from
+// the plm tables (the same values, the same sums).
 template <bool TabLds>
 __device__ __forceinline__ void sle_order_stage(const sle_solve_args& a, int m, const double* T, int tstride,
                                                 double2* U, double2 fn, double2 fs) {
@@ -280,7 +270,7 @@ __device__ __forceinline__ void sle_order_stage(const sle_solve_args& a, int m, 
             const int l = m + il;
             double2 r = red[il];
             for (int k = 1; k < S; k++) r = make_double2(r.x + red[k * nl + il].x, r.y + red[k * nl + il].y);
-            a.sigma[(size_t)l * (l + 1) / 2 + m] = r;
+            st_coh(a.sigma + (size_t)l * (l + 1) / 2 + m, r);
             Q[il] = r;
         }
         __syncthreads();
@@ -399,177 +389,116 @@ __device__ __forceinline__ double2* sle_fft(double2* a, double2* b, const vh_sht
     else return fft_wave(a, b, s, INV, lane, tw);
 }
 
-// MinW: wavefronts per SIMD the build must allow; 2 halves the registers
-// (with spills) so that two blocks share a CU when the grid (latitude pairs +
-// orders, 386 blocks at lmax 192) exceeds the CU count.
-template <bool TabLds, int MinW>
-__global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
-    extern __shared__ double2 lds[];
-    const vh_sht& s = a.s;
-    const int L = s.L, N = s.nphi, nlat = s.nlat, nh = s.nh;
-    const int npairs = (nlat + 1) / 2, blk = blockIdx.x, tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
-    const bool is_row = blk < npairs;
-    const int p = blk, m = blk - npairs;
-    __shared__ double redmax[4][4];
-    __shared__ double sdphi, scs;
-    // Inputs written by the kernels before this launch (ice, drift, rsl, ...)
-    // were written by other XCDs: an agent-scope acquire invalidates this
-    // CU's and XCD's cached copies before anything is read, whatever fences
-    // the launch itself carries; a release at the end publishes the outputs.
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    unsigned target = a.bar[2];   // barriers passed, counted on across launches
-    const double scale = 2.0 * VH_PI / N, dlon = 2.0 * VH_PI / N;
-    __shared__ double stot[4];    // pass totals: C_int, int d_ice_g, int topo0 (C - C0); flips
-    double n_iters = 0.0, n_passes = 0.0, last_ice = 0.0, last_zeta = 0.0, last_cs = 0.0;
-    // row-block LDS
-    double* srow = (double*)lds;                          // [2][N]
-    double2* TW = (double2*)(srow + 2 * N);               // [s.ntw]
-    double2* RB = TW + s.ntw;                             // [2][2N] FFT buffer pairs
-    // order-block LDS
-    double* T = (double*)lds;                             // [nh][tstride] (TabLds)
-    double2* U = (double2*)(T + (TabLds ? sle_tab_doubles(s) : 0));
-    const int tstride = is_row ? 1 : ((L - m + 1) | 1);
-    if (is_row) {
-        for (int t = tid; t < s.ntw; t += 256) TW[t] = s.twp[t];
-    }
-    // the order blocks' Legendre table is loaded after the first barrier of the
-    // first pass, while the row blocks run the pass's start transforms
-    auto load_table = [&]() {
-        // 8 table values per thread in flight
-        const int nl = L - m + 1, nt = nh * nl;
-        const double* Pa = s.plm_jl + (size_t)s.mstart[m] * nh;
-        for (int t0 = tid; t0 < nt; t0 += 256 * 8) {
-            double v[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = t0 + 256 * u < nt ? Pa[t0 + 256 * u] : 0.0;
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int t = t0 + 256 * u;
-                if (t < nt) T[(t / nl) * tstride + t % nl] = v[u];
-            }
-        }
-    };
-    __syncthreads();
+// ---- row work of latitude pair p (rows 2p, 2p+1)
 
-  for (int io = 1; io <= a.n_outer; io++) {
-    // ---- coastline pass head (src/vilma_sle.f90:231-294), rows: the mask C
-    // (first pass: from topo0 - rsl; later: the previous pass's C_next; fixed
-    // ocean: C0), the grounded-ice increment d_ice_g and, per row, the sums of
-    // C, d_ice_g and topo0 (C - C0) (a 256-thread tree per row)
-    if (is_row) {
-        double* red = (double*)RB;   // [6][256]: (quantity, row of the pair)
-        double sp[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        // the inputs of a thread's points of both rows (VH_SLE_PH per row) are
-        // loaded before any is used; sums per thread in the same (row, longitude) order
-        for (int i0 = 0; i0 < N; i0 += 256 * VH_SLE_PH) {
-            double vC0[2][VH_SLE_PH], vice[2][VH_SLE_PH], vdice[2][VH_SLE_PH], vtopo[2][VH_SLE_PH],
-                vx[2][VH_SLE_PH];
+// HEAD: the coastline pass head (src/vilma_sle.f90:231-294): the mask C (first
+// pass: from topo0 - rsl; later: the previous pass's C_next; fixed ocean: C0),
+// the grounded-ice increment d_ice_g and, per row, the sums of C, d_ice_g and
+// topo0 (C - C0) (a 256-thread tree per row, red [6][256] in LDS)
+__device__ __forceinline__ void sle_row_head(const sle_solve_args& a, int p, int io, double* red) {
+    const int N = a.s.nphi, nlat = a.s.nlat, tid = threadIdx.x;
+    double sp[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    // the inputs of a thread's points of both rows (VH_SLE_PH per row) are
+    // loaded before any is used; sums per thread in the same (row, longitude) order
+    for (int i0 = 0; i0 < N; i0 += 256 * VH_SLE_PH) {
+        double vC0[2][VH_SLE_PH], vice[2][VH_SLE_PH], vdice[2][VH_SLE_PH], vtopo[2][VH_SLE_PH], vx[2][VH_SLE_PH];
+#pragma unroll
+        for (int w = 0; w < 2; w++)
+#pragma unroll
+            for (int u = 0; u < VH_SLE_PH; u++) {
+                const int j = 2 * p + w, i = i0 + tid + 256 * u;
+                const bool ok = j < nlat && i < N;
+                const size_t id = ok ? (size_t)j * N + i : 0;
+                vC0[w][u] = ok ? a.C0[id] : 0.0;
+                vice[w][u] = ok ? a.ice[id] : 0.0;
+                vdice[w][u] = ok ? a.dice[id] : 0.0;
+                vtopo[w][u] = ok ? a.topo0[id] : 0.0;
+                vx[w][u] = (!ok || a.fixed_ocean) ? 0.0 : ld_coh(io == 1 ? a.rsl + id : a.Cn + id);
+            }
+#pragma unroll
+        for (int w = 0; w < 2; w++)
+#pragma unroll
+            for (int u = 0; u < VH_SLE_PH; u++) {
+                const int j = 2 * p + w, i = i0 + tid + 256 * u;
+                if (j >= nlat || i >= N) continue;
+                const size_t id = (size_t)j * N + i;
+                const double c = a.fixed_ocean ? vC0[w][u]
+                                               : (io == 1 ? ocean(vtopo[w][u] - vx[w][u], vice[w][u]) : vx[w][u]);
+                const double g = vice[w][u] * (1.0 - c) - (vice[w][u] - vdice[w][u]) * (1.0 - vC0[w][u]);
+                st_coh(a.C + id, c);
+                st_coh(a.dig + id, g);
+                sp[3 * w + 0] += c; sp[3 * w + 1] += g; sp[3 * w + 2] += vtopo[w][u] * (c - vC0[w][u]);
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) red[q * 256 + tid] = sp[q];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (tid < st)
+#pragma unroll
+            for (int q = 0; q < 6; q++) red[q * 256 + tid] += red[q * 256 + tid + st];
+        __syncthreads();
+    }
+    if (tid < 6) {
+        const int w = tid / 3, q = tid % 3, j = 2 * p + w;
+        if (j < nlat) st_coh(a.part3 + (size_t)q * nlat + j, red[tid * 256]);
+    }
+}
+
+// START: the subgrid sloping-coast load of the own rows (src/vilma_sle.f90:
+// 289-291) when the pass has ocean, then FC = FFT(C) (wave 0) and
+// F = FFT(load(rsl)) (wave 1). The load forms wcorr with the same expression
+// instead of reading the value just stored.
+template <int MinW>
+__device__ __forceinline__ void sle_row_start(const sle_solve_args& a, int p, double C_int, double2* RB,
+                                              const double2* TW) {
+    const vh_sht& s = a.s;
+    const int L = s.L, N = s.nphi, nlat = s.nlat, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const double scale = 2.0 * VH_PI / N;
+    const bool sea = C_int > 0.0;
+    if (sea)
+        for (int i0 = 0; i0 < N; i0 += 256 * VH_SLE_UB) {   // loads batched as in UPD
+            double vc[2][VH_SLE_UB], vt[2][VH_SLE_UB], v0[2][VH_SLE_UB];
 #pragma unroll
             for (int w = 0; w < 2; w++)
 #pragma unroll
-                for (int u = 0; u < VH_SLE_PH; u++) {
+                for (int u = 0; u < VH_SLE_UB; u++) {
                     const int j = 2 * p + w, i = i0 + tid + 256 * u;
                     const bool ok = j < nlat && i < N;
                     const size_t id = ok ? (size_t)j * N + i : 0;
-                    vC0[w][u] = ok ? a.C0[id] : 0.0;
-                    vice[w][u] = ok ? a.ice[id] : 0.0;
-                    vdice[w][u] = ok ? a.dice[id] : 0.0;
-                    vtopo[w][u] = ok ? a.topo0[id] : 0.0;
-                    vx[w][u] = (!ok || a.fixed_ocean) ? 0.0 : (io == 1 ? a.rsl[id] : a.Cn[id]);
+                    vc[w][u] = ok ? ld_coh(a.C + id) : 0.0;
+                    vt[w][u] = ok ? a.topo0[id] : 0.0;
+                    v0[w][u] = ok ? a.C0[id] : 0.0;
                 }
 #pragma unroll
             for (int w = 0; w < 2; w++)
 #pragma unroll
-                for (int u = 0; u < VH_SLE_PH; u++) {
+                for (int u = 0; u < VH_SLE_UB; u++) {
                     const int j = 2 * p + w, i = i0 + tid + 256 * u;
                     if (j >= nlat || i >= N) continue;
-                    const size_t id = (size_t)j * N + i;
-                    const double c = a.fixed_ocean ? vC0[w][u]
-                                                   : (io == 1 ? ocean(vtopo[w][u] - vx[w][u], vice[w][u]) : vx[w][u]);
-                    const double g = vice[w][u] * (1.0 - c) - (vice[w][u] - vdice[w][u]) * (1.0 - vC0[w][u]);
-                    a.C[id] = c;
-                    a.dig[id] = g;
-                    sp[3 * w + 0] += c; sp[3 * w + 1] += g; sp[3 * w + 2] += vtopo[w][u] * (c - vC0[w][u]);
+                    st_coh(a.wcorr + (size_t)j * N + i,
+                           a.subgrid ? -VH_RHO_WATER * vt[w][u] * (vc[w][u] - v0[w][u]) : 0.0);
                 }
         }
-#pragma unroll
-        for (int q = 0; q < 6; q++) red[q * 256 + tid] = sp[q];
-        __syncthreads();
-        for (int st = 128; st > 0; st >>= 1) {
-            if (tid < st)
-#pragma unroll
-                for (int q = 0; q < 6; q++) red[q * 256 + tid] += red[q * 256 + tid + st];
-            __syncthreads();
-        }
-        if (tid < 6) {
-            const int w = tid / 3, q = tid % 3, j = 2 * p + w;
-            if (j < nlat) st_coh(a.part3 + (size_t)q * nlat + j, red[tid * 256]);
-        }
-    }
-    sle_grid_sync(a.bar, target, a.res);
-    // every block: the pass totals sum_j w_j dlon part_j (latitude order, as
-    // k_weighted_total): the terms are loaded by all threads into the FFT /
-    // order-stage scratch, then one lane per quantity adds them in order
-    {
-        double* tw3 = is_row ? (double*)RB : (double*)U;   // [3][nlat], free here
-        for (int t = tid; t < 3 * nlat; t += 256) {
-            const int q = t % nlat;
-            tw3[t] = s.w[q] * ld_coh(a.part3 + t) * dlon;
-        }
-        __syncthreads();
-        if (tid < 3) {   // in latitude order, 8 terms read ahead of their sums
-            double total = 0.0;
-            int q = 0;
-            for (; q + 8 <= nlat; q += 8) {
-                double v[8];
-#pragma unroll
-                for (int u = 0; u < 8; u++) v[u] = tw3[tid * nlat + q + u];
-#pragma unroll
-                for (int u = 0; u < 8; u++) total += v[u];
-            }
-            for (; q < nlat; q++) total += tw3[tid * nlat + q];
-            stot[tid] = total;
-        }
-        __syncthreads();
-    }
-    if constexpr (TabLds) {
-        if (io == 1 && !is_row) load_table();   // read by the order stage after the next barrier
-    }
-    const double C_int = stot[0];
-    if (blk == 0 && tid == 0) a.res[3] = C_int;
-    last_cs = 0.0;   // a pass without ocean leaves int C rsl = 0
-    if (C_int > 0.0) {
-        // subgrid sloping-coast load (src/vilma_sle.f90:289-291), own rows
-        if (is_row)
-            for (int w = 0; w < 2; w++) {
-                const int j = 2 * p + w;
-                if (j >= nlat) continue;
-                for (int i = tid; i < N; i += 256) {
-                    const size_t id = (size_t)j * N + i;
-                    a.wcorr[id] = a.subgrid ? -VH_RHO_WATER * a.topo0[id] * (a.C[id] - a.C0[id]) : 0.0;
-                }
-            }
-        n_passes += 1.0;
-        last_ice = stot[1];
-        last_zeta = stot[2];
-    }
-    __syncthreads();   // wcorr of the block's rows, written by all its threads, is read below by wavefronts 0, 1
-    // ---- start: FC = FFT(C) (wave 0), F = FFT(load(rsl)) (wave 1)
-    if (is_row && wave < 2) {
+    if (wave < 2) {
         const int r0 = 2 * p, r1 = r0 + 1;
         const bool has1 = r1 < nlat;
         double2* fa = RB + (size_t)wave * 2 * N;
         double2* fb = fa + N;
-        const InLoad load{a.dig, a.C, a.rsl, a.wcorr, N};
+        auto load = [&](int row, int i) {
+            const size_t id = (size_t)row * N + i;
+            const double c = ld_coh(a.C + id);
+            const double wc = sea ? (a.subgrid ? -VH_RHO_WATER * a.topo0[id] * (c - a.C0[id]) : 0.0) : ld_coh(a.wcorr + id);
+            return VH_RHO_ICE * ld_coh(a.dig + id) + VH_RHO_WATER * (c * ld_coh(a.rsl + id)) + wc;
+        };
         for (int i0 = 0; i0 < N; i0 += 64 * VH_FFT_U) {
             double v0[VH_FFT_U], v1[VH_FFT_U];
 #pragma unroll
             for (int u = 0; u < VH_FFT_U; u++) {
                 const int i = i0 + lane + 64 * u;
                 if (wave == 0) {
-                    v0[u] = i < N ? a.C[(size_t)r0 * N + i] : 0.0;
-                    v1[u] = (has1 && i < N) ? a.C[(size_t)r1 * N + i] : 0.0;
+                    v0[u] = i < N ? ld_coh(a.C + (size_t)r0 * N + i) : 0.0;
+                    v1[u] = (has1 && i < N) ? ld_coh(a.C + (size_t)r1 * N + i) : 0.0;
                 } else {
                     v0[u] = i < N ? load(r0, i) : 0.0;
                     v1[u] = (has1 && i < N) ? load(r1, i) : 0.0;
@@ -587,147 +516,238 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
         lds_to_pair_spectra<true>(z, L, N, scale, dst + (size_t)r0 * (L + 1),
                                   has1 ? dst + (size_t)r1 * (L + 1) : nullptr, lane);
     }
-    sle_grid_sync(a.bar, target, a.res);
-    if (!is_row) {
-        double2 fn = make_double2(0.0, 0.0), fs = fn;
-        if (tid < nh) {
-            const int js = nlat - 1 - tid;
-            fn = ld_coh(a.F + (size_t)tid * (L + 1) + m);
-            fs = js != tid ? ld_coh(a.F + (size_t)js * (L + 1) + m) : make_double2(0.0, 0.0);
-        }
-        sle_order_stage<TabLds>(a, m, T, tstride, U, fn, fs);
-    }
-    sle_grid_sync(a.bar, target, a.res);
-    if (C_int <= 0.0) break;   // no ocean: load only; sigma above is the closing analysis
+}
 
-    for (int ii = 1; ii <= a.n_inner; ii++) {
-        n_iters += 1.0;
-        // ---- rows: inverse FFT of (u_j, N_j) -> Sraw_j, cs_j, the base load and its FFT
-        if (is_row) {
-            // after a wavefront's transform its result sits in buffer a when the
-            // pass count is even, else in b; the other one takes the row's base load
-            const size_t zoff = (s.nfac & 1) ? (size_t)N : 0, soff = (size_t)N - zoff;
-            if (wave < 2) {
-                const int j = 2 * p + wave;
-                if (j < nlat) {
-                    double2* fa = RB + (size_t)wave * 2 * N;
-                    double2* fb = fa + N;
-                    // the row's mask, rotation term, grounded ice and subgrid term
-                    // in flight over the transform (longitudes below 64 VH_SLE_PU)
-                    double cpre[VH_SLE_PU], spre[VH_SLE_PU], gpre[VH_SLE_PU], wpre[VH_SLE_PU];
+// ROWS: inverse FFT of (u_j, N_j) -> Sraw_j (into srow, and into sraw for a
+// later stage run elsewhere), cs_j = sum C Sraw, the base load
+// rho_i dig + rho_w C Sraw + wcorr and its FFT (the pair's rows packed) -> F
+template <int MinW>
+__device__ __forceinline__ void sle_row_rows(const sle_solve_args& a, int p, double* srow, double2* RB,
+                                             const double2* TW) {
+    const vh_sht& s = a.s;
+    const int L = s.L, N = s.nphi, nlat = s.nlat, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const double scale = 2.0 * VH_PI / N;
+    // after a wavefront's transform its result sits in buffer a when the
+    // pass count is even, else in b; the other one takes the row's base load
+    const size_t zoff = (s.nfac & 1) ? (size_t)N : 0, soff = (size_t)N - zoff;
+    if (wave < 2) {
+        const int j = 2 * p + wave;
+        if (j < nlat) {
+            double2* fa = RB + (size_t)wave * 2 * N;
+            double2* fb = fa + N;
+            // the row's mask, rotation term, grounded ice and subgrid term
+            // in flight over the transform (longitudes below 64 VH_SLE_PU)
+            double cpre[VH_SLE_PU], spre[VH_SLE_PU], gpre[VH_SLE_PU], wpre[VH_SLE_PU];
 #pragma unroll
-                    for (int u = 0; u < VH_SLE_PU; u++) {
-                        const int i = lane + 64 * u;
-                        const size_t id = (size_t)j * N + i;
-                        cpre[u] = i < N ? a.C[id] : 0.0;
-                        spre[u] = (a.srot && i < N) ? a.srot[id] : 0.0;
-                        gpre[u] = i < N ? a.dig[id] : 0.0;
-                        wpre[u] = i < N ? a.wcorr[id] : 0.0;
-                    }
-                    const double2* F2p = a.F2 + (size_t)p * (L + 1) * 4 + 2 * wave;   // rows q = 2j, 2j + 1
-                    pair_spectra_to_lds<true>(F2p, F2p + 1, L, N, fa, lane, 4);
-                    double2* z = sle_fft<true, MinW>(fa, fb, s, lane, TW);
-                    double rs = 0.0;
-                    for (int i0 = 0; i0 < N; i0 += 64 * VH_SLE_PU) {
-#pragma unroll
-                        for (int u = 0; u < VH_SLE_PU; u++) {
-                            const int i = i0 + lane + 64 * u;
-                            if (i >= N) break;
-                            const size_t id = (size_t)j * N + i;
-                            const bool pre = i0 == 0;
-                            double v = z[i].y - z[i].x;
-                            if (a.srot) v = v + (pre ? spre[u] : a.srot[id]);
-                            srow[wave * N + i] = v;
-                            const double c = pre ? cpre[u] : a.C[id];
-                            rs += c * v;
-                            const double base = VH_RHO_ICE * (pre ? gpre[u] : a.dig[id]) + VH_RHO_WATER * (c * v) +
-                                                (pre ? wpre[u] : a.wcorr[id]);
-                            ((double*)(RB + (size_t)wave * 2 * N + soff))[i] = base;
-                        }
-                    }
-                    rs = wave_sum(rs);
-                    if (lane == 0) st_coh(a.cs + j, rs);
-                }
+            for (int u = 0; u < VH_SLE_PU; u++) {
+                const int i = lane + 64 * u;
+                const size_t id = (size_t)j * N + i;
+                cpre[u] = i < N ? ld_coh(a.C + id) : 0.0;
+                spre[u] = (a.srot && i < N) ? a.srot[id] : 0.0;
+                gpre[u] = i < N ? ld_coh(a.dig + id) : 0.0;
+                wpre[u] = i < N ? ld_coh(a.wcorr + id) : 0.0;
             }
-            __syncthreads();
-            if (wave == 0) {
-                // pack the pair's base loads (rows 2p, 2p+1) into wave 0's result
-                // buffer, transform with its other buffer as scratch
-                const int r0 = 2 * p;
-                const bool has1 = r0 + 1 < nlat;
-                double2* P = RB + zoff;
-                const double* b0 = (const double*)(RB + soff);
-                const double* b1 = (const double*)(RB + 2 * (size_t)N + soff);
-                for (int i = lane; i < N; i += 64) P[i] = make_double2(b0[i], has1 ? b1[i] : 0.0);
-                wave_lds_sync();
-                double2* z = sle_fft<false, MinW>(P, RB + soff, s, lane, TW);
-                lds_to_pair_spectra<true>(z, L, N, scale, a.F + (size_t)r0 * (L + 1),
-                                          r0 + 1 < nlat ? a.F + (size_t)(r0 + 1) * (L + 1) : nullptr, lane);
-            }
-        }
-        sle_grid_sync(a.bar, target, a.res);
-
-        // ---- dphi (every block); rows: rsl update and row maxima; orders: the next sigma and (u, N)
-        if (is_row) {
-            sle_dphi(a, stot, nlat, dlon, wave, lane, &sdphi, &scs);
-            __syncthreads();
-            const double dphi = sdphi;
-            last_cs = scs + dphi * C_int;   // int C rsl of this iterate: int C Sraw + dphi int C
-            if (blk == 0 && tid == 0) { a.res[0] = (double)ii; a.res[1] = dphi; }
-            const int r0 = 2 * p;
-            double dm[2] = {0.0, 0.0}, sm[2] = {0.0, 0.0};
+            const double2* F2p = a.F2 + (size_t)p * (L + 1) * 4 + 2 * wave;   // rows q = 2j, 2j + 1
+            pair_spectra_to_lds<true>(F2p, F2p + 1, L, N, fa, lane, 4);
+            double2* z = sle_fft<true, MinW>(fa, fb, s, lane, TW);
+            double rs = 0.0;
+            for (int i0 = 0; i0 < N; i0 += 64 * VH_SLE_PU) {
 #pragma unroll
-            for (int w = 0; w < 2; w++) {
-                const int j = r0 + w;
-                if (j >= nlat) continue;
-                for (int i = tid; i < N; i += 256) {
+                for (int u = 0; u < VH_SLE_PU; u++) {
+                    const int i = i0 + lane + 64 * u;
+                    if (i >= N) break;
                     const size_t id = (size_t)j * N + i;
-                    const double rn = srow[w * N + i] + dphi;
-                    const double c = a.C[id];
-                    dm[w] = fmax(dm[w], fabs(c * (rn - a.rsl[id])));
-                    a.rsl[id] = rn;
-                    sm[w] = fmax(sm[w], fabs(c * rn));
+                    const bool pre = i0 == 0;
+                    double v = z[i].y - z[i].x;
+                    if (a.srot) v = v + (pre ? spre[u] : a.srot[id]);
+                    srow[wave * N + i] = v;
+                    st_coh(a.sraw + id, v);
+                    const double c = pre ? cpre[u] : ld_coh(a.C + id);
+                    rs += c * v;
+                    const double base = VH_RHO_ICE * (pre ? gpre[u] : ld_coh(a.dig + id)) + VH_RHO_WATER * (c * v) +
+                                        (pre ? wpre[u] : ld_coh(a.wcorr + id));
+                    ((double*)(RB + (size_t)wave * 2 * N + soff))[i] = base;
                 }
             }
-#pragma unroll
-            for (int q = 0; q < 2; q++)
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) {
-                    dm[q] = fmax(dm[q], __shfl_xor(dm[q], off));
-                    sm[q] = fmax(sm[q], __shfl_xor(sm[q], off));
-                }
-            if (lane == 0) { redmax[wave][0] = dm[0]; redmax[wave][1] = dm[1]; redmax[wave][2] = sm[0]; redmax[wave][3] = sm[1]; }
-            __syncthreads();
-            if (tid < 4) {
-                const int w = tid & 1;
-                if (r0 + w < nlat) {
-                    const double v = fmax(fmax(redmax[0][tid], redmax[1][tid]), fmax(redmax[2][tid], redmax[3][tid]));
-                    st_coh(a.part + (tid < 2 ? 0 : nlat) + r0 + w, v);
-                }
-            }
-        } else {
-            // this thread's spectrum values in flight while dphi is formed
-            double2 bn = make_double2(0.0, 0.0), bs = bn, cn = bn, cs = bn;
-            const int js = nlat - 1 - tid;
-            if (tid < nh) {
-                bn = ld_coh(a.F + (size_t)tid * (L + 1) + m);
-                cn = ld_coh(a.FC + (size_t)tid * (L + 1) + m);
-                if (js != tid) {
-                    bs = ld_coh(a.F + (size_t)js * (L + 1) + m);
-                    cs = ld_coh(a.FC + (size_t)js * (L + 1) + m);
-                }
-            }
-            sle_dphi(a, stot, nlat, dlon, wave, lane, &sdphi, &scs);
-            __syncthreads();
-            const double k = VH_RHO_WATER * sdphi;
-            last_cs = scs + sdphi * C_int;
-            const double2 fn = make_double2(bn.x + k * cn.x, bn.y + k * cn.y);
-            const double2 fs = make_double2(bs.x + k * cs.x, bs.y + k * cs.y);
-            sle_order_stage<TabLds>(a, m, T, tstride, U, fn, fs);
+            rs = wave_sum(rs);
+            if (lane == 0) st_coh(a.cs + j, rs);
         }
-        sle_grid_sync(a.bar, target, a.res);
+    }
+    __syncthreads();
+    if (wave == 0) {
+        // pack the pair's base loads (rows 2p, 2p+1) into wave 0's result
+        // buffer, transform with its other buffer as scratch
+        const int r0 = 2 * p;
+        const bool has1 = r0 + 1 < nlat;
+        double2* P = RB + zoff;
+        const double* b0 = (const double*)(RB + soff);
+        const double* b1 = (const double*)(RB + 2 * (size_t)N + soff);
+        for (int i = lane; i < N; i += 64) P[i] = make_double2(b0[i], has1 ? b1[i] : 0.0);
+        wave_lds_sync();
+        double2* z = sle_fft<false, MinW>(P, RB + soff, s, lane, TW);
+        lds_to_pair_spectra<true>(z, L, N, scale, a.F + (size_t)r0 * (L + 1),
+                                  has1 ? a.F + (size_t)(r0 + 1) * (L + 1) : nullptr, lane);
+    }
+}
 
-        // ---- exit test, the same in every block (max is exact in any order)
+// UPD: rsl = Sraw + dphi (Sraw from srow when this block made it in the
+// previous stage, else from sraw) and the row maxima of |C drsl|, |C rsl|.
+// A thread's points (VH_SLE_UB per row and chunk) are loaded before any is
+// used: coherent loads and stores keep their program order, so a
+// load-use-store per point would wait one memory round trip per point.
+__device__ __forceinline__ void sle_row_upd(const sle_solve_args& a, int p, double dphi, const double* srow,
+                                            bool sraw_lds, double (*redmax)[4]) {
+    const int N = a.s.nphi, nlat = a.s.nlat, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int r0 = 2 * p;
+    double dm[2] = {0.0, 0.0}, sm[2] = {0.0, 0.0};
+    for (int i0 = 0; i0 < N; i0 += 256 * VH_SLE_UB) {
+        double vc[2][VH_SLE_UB], vr[2][VH_SLE_UB], vs[2][VH_SLE_UB];
+#pragma unroll
+        for (int w = 0; w < 2; w++)
+#pragma unroll
+            for (int u = 0; u < VH_SLE_UB; u++) {
+                const int j = r0 + w, i = i0 + tid + 256 * u;
+                const bool ok = j < nlat && i < N;
+                const size_t id = ok ? (size_t)j * N + i : 0;
+                vc[w][u] = ok ? ld_coh(a.C + id) : 0.0;
+                vr[w][u] = ok ? ld_coh(a.rsl + id) : 0.0;
+                vs[w][u] = (!ok || sraw_lds) ? 0.0 : ld_coh(a.sraw + id);
+            }
+#pragma unroll
+        for (int w = 0; w < 2; w++)
+#pragma unroll
+            for (int u = 0; u < VH_SLE_UB; u++) {
+                const int j = r0 + w, i = i0 + tid + 256 * u;
+                if (j >= nlat || i >= N) continue;
+                const size_t id = (size_t)j * N + i;
+                const double rn = (sraw_lds ? srow[w * N + i] : vs[w][u]) + dphi;
+                const double c = vc[w][u];
+                dm[w] = fmax(dm[w], fabs(c * (rn - vr[w][u])));
+                st_coh(a.rsl + id, rn);
+                sm[w] = fmax(sm[w], fabs(c * rn));
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            dm[q] = fmax(dm[q], __shfl_xor(dm[q], off));
+            sm[q] = fmax(sm[q], __shfl_xor(sm[q], off));
+        }
+    if (lane == 0) { redmax[wave][0] = dm[0]; redmax[wave][1] = dm[1]; redmax[wave][2] = sm[0]; redmax[wave][3] = sm[1]; }
+    __syncthreads();
+    if (tid < 4) {
+        const int w = tid & 1;
+        if (r0 + w < nlat) {
+            const double v = fmax(fmax(redmax[0][tid], redmax[1][tid]), fmax(redmax[2][tid], redmax[3][tid]));
+            st_coh(a.part + (tid < 2 ? 0 : nlat) + r0 + w, v);
+        }
+    }
+}
+
+// COAST: the coastline fixed point (src/vilma_sle.f90:336-344): C_next from
+// topo0 - rsl and the flipped cells per row (loads batched as in UPD)
+__device__ __forceinline__ void sle_row_coast(const sle_solve_args& a, int p, double (*redmax)[4]) {
+    const int N = a.s.nphi, nlat = a.s.nlat, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    double cnt[2] = {0.0, 0.0};
+    for (int i0 = 0; i0 < N; i0 += 256 * VH_SLE_UB) {
+        double vt[2][VH_SLE_UB], vr[2][VH_SLE_UB], vi[2][VH_SLE_UB], vc[2][VH_SLE_UB];
+#pragma unroll
+        for (int w = 0; w < 2; w++)
+#pragma unroll
+            for (int u = 0; u < VH_SLE_UB; u++) {
+                const int j = 2 * p + w, i = i0 + tid + 256 * u;
+                const bool ok = j < nlat && i < N;
+                const size_t id = ok ? (size_t)j * N + i : 0;
+                vt[w][u] = ok ? a.topo0[id] : 0.0;
+                vi[w][u] = ok ? a.ice[id] : 0.0;
+                vr[w][u] = ok ? ld_coh(a.rsl + id) : 0.0;
+                vc[w][u] = ok ? ld_coh(a.C + id) : 0.0;
+            }
+#pragma unroll
+        for (int w = 0; w < 2; w++)
+#pragma unroll
+            for (int u = 0; u < VH_SLE_UB; u++) {
+                const int j = 2 * p + w, i = i0 + tid + 256 * u;
+                if (j >= nlat || i >= N) continue;
+                const double cn = ocean(vt[w][u] - vr[w][u], vi[w][u]);
+                st_coh(a.Cn + (size_t)j * N + i, cn);
+                cnt[w] += (cn != vc[w][u]) ? 1.0 : 0.0;
+            }
+    }
+#pragma unroll
+    for (int w = 0; w < 2; w++) cnt[w] = wave_sum(cnt[w]);
+    if (lane == 0) { redmax[wave][0] = cnt[0]; redmax[wave][1] = cnt[1]; }
+    __syncthreads();
+    if (tid < 2 && 2 * p + tid < nlat)
+        st_coh(a.part + 2 * nlat + 2 * p + tid, redmax[0][tid] + redmax[1][tid] + redmax[2][tid] + redmax[3][tid]);
+}
+
+// The decision after stage `cur` (its control record) from the stage's
+// outputs: the next stage's record into nx (LDS). scr: >= 3 nlat doubles of LDS.
+__device__ __forceinline__ void sle_decide(const sle_solve_args& a, const double* cur, double* nx, double* scr,
+                                           double (*redmax)[4]) {
+    const vh_sht& s = a.s;
+    const int nlat = s.nlat, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const double dlon = 2.0 * VH_PI / s.nphi;
+    if (tid < CT_N) nx[tid] = cur[tid];
+    __syncthreads();
+    const int kind = (int)cur[CT_KIND];
+    if (kind == SK_HEAD) {
+        // the pass totals sum_j w_j dlon part_j (latitude order, as
+        // k_weighted_total): the terms are loaded by all threads, then one lane
+        // per quantity adds them in order, 8 terms read ahead of their sums
+        for (int t = tid; t < 3 * nlat; t += 256) {
+            const int q = t % nlat;
+            scr[t] = s.w[q] * ld_coh(a.part3 + t) * dlon;
+        }
+        __syncthreads();
+        if (tid < 3) {
+            double total = 0.0;
+            int q = 0;
+            for (; q + 8 <= nlat; q += 8) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) v[u] = scr[tid * nlat + q + u];
+#pragma unroll
+                for (int u = 0; u < 8; u++) total += v[u];
+            }
+            for (; q < nlat; q++) total += scr[tid * nlat + q];
+            nx[CT_CINT + tid] = total;   // C_int, int d_ice_g, int topo0 (C - C0)
+        }
+        __syncthreads();
+        if (tid == 0) {
+            nx[CT_KIND] = SK_START;
+            nx[CT_LCS] = 0.0;            // a pass without ocean leaves int C rsl = 0
+            if (nx[CT_CINT] > 0.0) { nx[CT_NPASS] += 1.0; nx[CT_LICE] = nx[CT_TOT1]; nx[CT_LZETA] = nx[CT_TOT2]; }
+        }
+    } else if (kind == SK_START) {
+        if (tid == 0) nx[CT_KIND] = SK_ORD0;
+    } else if (kind == SK_ORD0) {
+        // no ocean: load only; sigma of ORD0 is the closing analysis
+        if (tid == 0) {
+            if (cur[CT_CINT] <= 0.0) nx[CT_KIND] = SK_DONE;
+            else { nx[CT_KIND] = SK_ROWS; nx[CT_II] = 1.0; nx[CT_NITER] += 1.0; }
+        }
+    } else if (kind == SK_ROWS) {
+        // dphi = (ice_int - Cs_int + zeta) / C_int, Cs_int from the row sums cs
+        if (wave == 0) {
+            double total = 0.0;
+            for (int q = lane; q < nlat; q += 64) total += s.w[q] * ld_coh(a.cs + q) * dlon;
+            total = wave_sum(total);
+            if (lane == 0) {
+                const double ice_int = -(VH_RHO_ICE / VH_RHO_WATER) * cur[CT_TOT1];
+                const double zeta = a.subgrid ? cur[CT_TOT2] : 0.0;
+                const double dphi = (ice_int - total + zeta) / cur[CT_CINT];
+                nx[CT_KIND] = SK_UPD;
+                nx[CT_DPHI] = dphi;
+                nx[CT_LCS] = total + dphi * cur[CT_CINT];   // int C rsl of this iterate
+                nx[CT_ITLAST] = cur[CT_II];
+                nx[CT_DPLAST] = dphi;
+            }
+        }
+    } else if (kind == SK_UPD) {
+        // the inner exit test (max is exact in any order)
         double dmx = 0.0, smx = 0.0;
         for (int q = tid; q < nlat; q += 256) {
             dmx = fmax(dmx, ld_coh(a.part + q));
@@ -735,80 +755,373 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) { dmx = fmax(dmx, __shfl_xor(dmx, off)); smx = fmax(smx, __shfl_xor(smx, off)); }
-        __syncthreads();
         if (lane == 0) { redmax[wave][0] = dmx; redmax[wave][1] = smx; }
         __syncthreads();
         dmx = fmax(fmax(redmax[0][0], redmax[1][0]), fmax(redmax[2][0], redmax[3][0]));
         smx = fmax(fmax(redmax[0][1], redmax[1][1]), fmax(redmax[2][1], redmax[3][1]));
-        if (dmx <= a.tol * fmax(smx, DBL_MIN)) break;
-    }
-    if (a.fixed_ocean) break;
-    // ---- coastline fixed point (src/vilma_sle.f90:336-344): C_next from topo0 - rsl,
-    // flipped cells per row; every block stops when none flipped
-    if (is_row) {
-        double cnt[2] = {0.0, 0.0};
-#pragma unroll
-        for (int w = 0; w < 2; w++) {
-            const int j = 2 * p + w;
-            if (j >= nlat) continue;
-            for (int i = tid; i < N; i += 256) {
-                const size_t id = (size_t)j * N + i;
-                const double cn = ocean(a.topo0[id] - a.rsl[id], a.ice[id]);
-                a.Cn[id] = cn;
-                cnt[w] += (cn != a.C[id]) ? 1.0 : 0.0;
+        if (tid == 0) {
+            const bool conv = dmx <= a.tol * fmax(smx, DBL_MIN);
+            if (!conv && cur[CT_II] < (double)a.n_inner) {
+                nx[CT_KIND] = SK_ROWS; nx[CT_II] = cur[CT_II] + 1.0; nx[CT_NITER] += 1.0;
+            } else {
+                nx[CT_KIND] = a.fixed_ocean ? SK_DONE : SK_COAST;
             }
         }
-#pragma unroll
-        for (int w = 0; w < 2; w++) cnt[w] = wave_sum(cnt[w]);
-        if (lane == 0) { redmax[wave][0] = cnt[0]; redmax[wave][1] = cnt[1]; }
+    } else if (kind == SK_COAST) {
+        double nf = 0.0;   // whole numbers: exact in any order
+        for (int q = tid; q < nlat; q += 256) nf += ld_coh(a.part + 2 * nlat + q);
+        nf = wave_sum(nf);
+        if (lane == 0) redmax[wave][0] = nf;
         __syncthreads();
-        if (tid < 2 && 2 * p + tid < nlat)
-            st_coh(a.part + 2 * nlat + 2 * p + tid, redmax[0][tid] + redmax[1][tid] + redmax[2][tid] + redmax[3][tid]);
+        nf = redmax[0][0] + redmax[1][0] + redmax[2][0] + redmax[3][0];
+        if (tid == 0) {
+            if (nf == 0.0 || cur[CT_IO] >= (double)a.n_outer) nx[CT_KIND] = SK_DONE;
+            else { nx[CT_KIND] = SK_HEAD; nx[CT_IO] = cur[CT_IO] + 1.0; nx[CT_II] = 0.0; }
+        }
     }
-    sle_grid_sync(a.bar, target, a.res);
-    double nf = 0.0;   // whole numbers: exact in any order
-    for (int q = tid; q < nlat; q += 256) nf += ld_coh(a.part + 2 * nlat + q);
-    nf = wave_sum(nf);
     __syncthreads();
-    if (lane == 0) redmax[wave][0] = nf;
-    __syncthreads();
-    nf = redmax[0][0] + redmax[1][0] + redmax[2][0] + redmax[3][0];
-    __syncthreads();
-    if (nf == 0.0) break;
-  }
-    if (blk == 0 && tid == 0) {
-        a.res[4] = n_iters; a.res[5] = n_passes; a.res[6] = last_ice; a.res[7] = last_zeta;
-        a.res[10] = last_cs;
-        a.bar[2] = target;
+}
+
+// page-0 words of the synchronisation area: per control-record slot k, the
+// stage number + 1 of the record it holds (written after the record)
+__host__ __device__ __forceinline__ unsigned* sle_slot_seq(unsigned* bar, unsigned sn) { return bar + 16 * (1 + (sn & 3u)); }
+
+// MinW: wavefronts per SIMD the build must allow; 2 halves the registers
+// (with spills) so that two blocks share a CU when the grid (latitude pairs +
+// orders, 386 blocks at lmax 192) exceeds the CU count.
+template <bool TabLds, int MinW>
+__global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
+    extern __shared__ double2 lds[];
+    const vh_sht& s = a.s;
+    const int L = s.L, N = s.nphi, nlat = s.nlat, nh = s.nh;
+    const int npairs = (nlat + 1) / 2, G = npairs + L + 1, own = blockIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    __shared__ double redmax[4][4];
+    __shared__ double sct[CT_N], snx[CT_N];   // this stage's control record, the next one
+    __shared__ unsigned su[2];
+    __shared__ unsigned long long smask;
+    __shared__ double2 spf[4][256];            // an order's prefetched row spectra (nh <= 256)
+    // Inputs written by the kernels before this launch (ice, drift, rsl, ...)
+    // were written by other XCDs: an agent-scope acquire invalidates this
+    // CU's and XCD's cached copies before anything is read, whatever fences
+    // the launch itself carries; a release at the end publishes the outputs.
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (a.late_blocks > 0 && own >= G - a.late_blocks) {   // diagnostic: a block that starts late
+        if (tid == 0) {
+            const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+            while (__builtin_amdgcn_s_memrealtime() - t0 < (unsigned long long)a.late_ticks) __builtin_amdgcn_s_sleep(10);
+        }
+        __syncthreads();
+    }
+    // row layout of the LDS
+    double* srow = (double*)lds;                          // [2][N]
+    double2* TW = (double2*)(srow + 2 * N);               // [s.ntw]
+    double2* RB = TW + s.ntw;                             // [2][2N] FFT buffer pairs
+    // order layout
+    double* T = (double*)lds;                             // [nh][tstride] (TabLds)
+    double2* U = (double2*)(T + (TabLds ? sle_tab_doubles(s) : 0));
+    const bool order_over_rows = !TabLds || sle_tab_doubles(s) * sizeof(double) < sle_rows_bytes(s);
+    // what the LDS holds: the twiddles, the Sraw of a pair (and the stage it
+    // was made in), the Legendre table of an order
+    bool tw_ok = false;
+    int srow_item = -1, tab_m = -1;
+    unsigned srow_stage = 0xffffffffu;
+
+    auto bcast = [&](unsigned v) __attribute__((always_inline)) -> unsigned {   // thread 0's v to every thread
+        if (tid == 0) su[0] = v;
+        __syncthreads();
+        const unsigned r = su[0];
+        __syncthreads();
+        return r;
+    };
+    auto claim = [&](int i, unsigned sn) __attribute__((always_inline)) -> bool {
+        unsigned ok = 0;
+        if (tid == 0)
+            ok = __hip_atomic_fetch_max(sle_claim(a.bar, i), sn + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < sn + 1u;
+        return bcast(ok) != 0u;
+    };
+    auto stage_now = [&]() __attribute__((always_inline)) -> unsigned { return bcast(tid == 0 ? ld_u(sle_rel(a.bar, own & 7)) : 0u); };
+    // the published control record of stage sn into sct (waits for it; false
+    // if the stage has moved on meanwhile or the launch failed)
+    auto read_ctl = [&](unsigned sn) __attribute__((always_inline)) -> bool {
+        unsigned ok = 0;
+        if (tid == 0) {
+            const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+            for (;;) {
+                const unsigned q = ld_u(sle_slot_seq(a.bar, sn));
+                if (q == sn + 1u) { ok = 1; break; }
+                if (q > sn + 1u || ld_u(a.bar + 1) != 0u) break;
+                if (__builtin_amdgcn_s_memrealtime() - t0 > VH_SLE_WAIT_TICKS) {
+                    __hip_atomic_fetch_or(a.bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    st_coh(a.res + 2, 1.0);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        if (!bcast(ok)) return false;
+        if (tid < CT_N) sct[tid] = ld_coh(sle_ctl(a.bar, sn) + tid);
+        __syncthreads();
+        return true;
+    };
+    auto publish = [&](unsigned sn1, const double* rec) __attribute__((always_inline)) {
+        if (tid < CT_N) st_coh(sle_ctl(a.bar, sn1) + tid, rec[tid]);
+        if (tid == 0) {
+            const double steals = (double)ld_u(a.bar + 2);
+            if ((int)rec[CT_KIND] == SK_DONE) {
+                st_coh(a.res + 0, rec[CT_ITLAST]); st_coh(a.res + 1, rec[CT_DPLAST]); st_coh(a.res + 3, rec[CT_CINT]);
+                st_coh(a.res + 4, rec[CT_NITER]); st_coh(a.res + 5, rec[CT_NPASS]); st_coh(a.res + 6, rec[CT_LICE]);
+                st_coh(a.res + 7, rec[CT_LZETA]); st_coh(a.res + 8, steals); st_coh(a.res + 10, rec[CT_LCS]);
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(sle_slot_seq(a.bar, sn1), sn1 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    auto scratch = [&]() __attribute__((always_inline)) -> double* { return (TabLds && tab_m >= 0) ? (double*)U : (double*)RB; };
+    auto load_table = [&](int m) __attribute__((always_inline)) {
+        // 8 table values per thread in flight
+        const int nl = L - m + 1, nt = nh * nl, tstride = nl | 1;
+        const double* Pa = s.plm_jl + (size_t)s.mstart[m] * nh;
+        for (int t0 = tid; t0 < nt; t0 += 256 * 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = t0 + 256 * u < nt ? Pa[t0 + 256 * u] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int t = t0 + 256 * u;
+                if (t < nt) T[(t / nl) * tstride + t % nl] = v[u];
+            }
+        }
+        __syncthreads();
+    };
+    // order m's row spectra at the north / south rows of j = tid < nh: F
+    // (and FC) into spf[0..1][tid] (spf[2..3][tid]), staged in LDS so that no
+    // registers stay live across the row work's transforms
+    bool pf_ok = false;
+    auto fetch_order = [&](int m, bool withc) __attribute__((always_inline)) {
+        const double2 z = make_double2(0.0, 0.0);
+        if (tid < nh) {
+            const int js = nlat - 1 - tid;
+            const double2 f0 = ld_coh(a.F + (size_t)tid * (L + 1) + m);
+            const double2 f1 = js != tid ? ld_coh(a.F + (size_t)js * (L + 1) + m) : z;
+            double2 c0 = z, c1 = z;
+            if (withc) {
+                c0 = ld_coh(a.FC + (size_t)tid * (L + 1) + m);
+                c1 = js != tid ? ld_coh(a.FC + (size_t)js * (L + 1) + m) : z;
+            }
+            spf[0][tid] = f0; spf[1][tid] = f1; spf[2][tid] = c0; spf[3][tid] = c1;
+        }
+    };
+    // the work of item i in stage sn (record sct); stolen: run for an owner
+    // that is not resident
+    auto run_item = [&](int i, unsigned sn, bool stolen) __attribute__((always_inline)) {
+        const int kind = (int)sct[CT_KIND];
+        if (i < npairs) {
+            if (kind == SK_ORD0) return;
+            if (TabLds) tab_m = -1;   // the row layout covers the table
+            if (!tw_ok) {
+                for (int t = tid; t < s.ntw; t += 256) TW[t] = s.twp[t];
+                __syncthreads();
+                tw_ok = true;
+            }
+            switch (kind) {
+                case SK_HEAD: sle_row_head(a, i, (int)sct[CT_IO], (double*)RB); break;
+                case SK_START: sle_row_start<MinW>(a, i, sct[CT_CINT], RB, TW); break;
+                case SK_ROWS:
+                    sle_row_rows<MinW>(a, i, srow, RB, TW);
+                    srow_item = i; srow_stage = sn;
+                    break;
+                case SK_UPD: sle_row_upd(a, i, sct[CT_DPHI], srow, srow_item == i && srow_stage + 1u == sn, redmax); break;
+                case SK_COAST: sle_row_coast(a, i, redmax); break;
+                default: break;
+            }
+            return;
+        }
+        const int m = i - npairs;
+        if (kind != SK_START && kind != SK_ORD0 && kind != SK_UPD) return;
+        const bool tab = TabLds && !stolen;   // a block runs another order's work from the global tables
+        if (tab && tab_m != m) {
+            tw_ok = false; srow_item = -1;    // the table covers the row layout
+            load_table(m);
+            tab_m = m;
+        }
+        if (kind == SK_START) return;          // the owner's table load overlaps the rows' transforms
+        if (order_over_rows) { tw_ok = false; srow_item = -1; }
+        if (!(pf_ok && i == own)) fetch_order(m, kind == SK_UPD);
+        pf_ok = false;
+        double2 fn = make_double2(0.0, 0.0), fs = fn;
+        if (tid < nh) {   // each thread reads back its own entries
+            fn = spf[0][tid]; fs = spf[1][tid];
+            if (kind == SK_UPD) {
+                const double k = VH_RHO_WATER * sct[CT_DPHI];
+                const double2 c0 = spf[2][tid], c1 = spf[3][tid];
+                fn = make_double2(fn.x + k * c0.x, fn.y + k * c0.y);
+                fs = make_double2(fs.x + k * c1.x, fs.y + k * c1.y);
+            }
+        }
+        if constexpr (TabLds) {
+            if (tab) sle_order_stage<true>(a, m, T, (L - m + 1) | 1, U, fn, fs);
+            else sle_order_stage<false>(a, m, T, 1, U, fn, fs);
+        } else {
+            sle_order_stage<false>(a, m, T, 1, U, fn, fs);
+        }
+    };
+    // arrival of item i at the end of stage sn, with (pre) the claim of the
+    // block's own item of stage sn + 1 issued beside it. The block whose
+    // arrival completes the stage releases it: at once when it holds its item
+    // of stage sn + 1 (sync: the stage's outputs stay until that item is
+    // done; it publishes the decision afterwards), else after taking and
+    // publishing the decision. Returns {completed, own item of sn + 1 claimed}.
+    bool pending_publish = false;
+    auto arrive = [&](int i, unsigned sn, bool pre, bool sync) __attribute__((always_inline)) -> bool {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);   // this wavefront's stores are complete
+        __syncthreads();
+        if (tid == 0) {
+            unsigned got = 0, last = 0;
+            unsigned cl = 0xffffffffu;
+            if (pre)
+                cl = __hip_atomic_fetch_max(sle_claim(a.bar, own), sn + 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int g = i & 7;
+            const unsigned ng = (unsigned)(G - g + 7) / 8u;
+            if (__hip_atomic_fetch_add(sle_grp(a.bar, g), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u == ng * (sn + 1u))
+                last = __hip_atomic_fetch_add(sle_top(a.bar), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u ==
+                       8u * (sn + 1u);
+            got = pre ? (cl < sn + 2u) : 0u;
+            su[0] = last; su[1] = got;
+        }
+        __syncthreads();
+        const bool last = su[0] != 0u, got = su[1] != 0u;
+        __syncthreads();
+        if (last) {
+            const bool after = pre ? got : sync;
+            if (!after) {
+                sle_decide(a, sct, snx, scratch(), redmax);
+                publish(sn + 1u, snx);
+            }
+            if (tid == 0)
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    __hip_atomic_store(sle_rel(a.bar, k), sn + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            pending_publish = after;
+        }
+        return pre ? got : last;
+    };
+    // wait until stage sn is released. A block that has waited
+    // VH_SLE_STEAL_TICKS looks for the stage's unclaimed items and claims
+    // one: returns its index (>= 0; the caller runs it), -1 once the stage is
+    // released, -2 on the error bound.
+    auto wait_or_take = [&](unsigned sn, unsigned long long& tsw) __attribute__((always_inline)) -> int {
+        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+        for (;;) {
+            unsigned st = 0;
+            if (tid == 0) {
+                for (;;) {
+                    if (ld_u(sle_rel(a.bar, own & 7)) >= sn + 1u) { st = 0; break; }
+                    if (ld_u(a.bar + 1) != 0u) { st = 2; break; }
+                    const unsigned long long now = __builtin_amdgcn_s_memrealtime();
+                    if (now - t0 > VH_SLE_WAIT_TICKS) {
+                        __hip_atomic_fetch_or(a.bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        st_coh(a.res + 2, 1.0);
+                        st = 2;
+                        break;
+                    }
+                    if (now - tsw > VH_SLE_STEAL_TICKS) { st = 1; break; }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            st = bcast(st);
+            if (st == 0) return -1;
+            if (st == 2) return -2;
+            for (int base = 0; base < G; base += 64) {
+                if (wave == 0) {
+                    const int j = base + lane;
+                    const bool cand = j < G && ld_u(sle_claim(a.bar, j)) < sn + 1u;
+                    const unsigned long long mk = __ballot(cand);
+                    if (lane == 0) smask = mk;
+                }
+                __syncthreads();
+                unsigned long long mk = smask;
+                __syncthreads();
+                while (mk != 0ull) {
+                    const int j = base + __builtin_ctzll(mk);
+                    mk &= mk - 1ull;
+                    if (claim(j, sn)) return j;
+                }
+            }
+            tsw = __builtin_amdgcn_s_memrealtime();
+        }
+    };
+
+    unsigned sn = stage_now();
+    bool have = read_ctl(sn);                  // sct holds the record of stage sn
+    bool claimed = have && claim(own, sn);
+    for (;;) {
+        if (have) {
+            if ((int)sct[CT_KIND] == SK_DONE) break;
+            bool sync = false;                 // this block holds its item of stage sn + 1
+            int item = claimed ? own : -1;
+            bool pre = claimed;                // the own item's arrival claims the next one
+            unsigned long long tsw = 0;
+            int w = 0;
+            for (;;) {
+                if (item >= 0) {
+                    // the one place where work items run (keeps the transforms inlined once)
+                    if (item != own) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    run_item(item, sn, item != own);
+                    if (item != own && tid == 0)
+                        __hip_atomic_fetch_add(a.bar + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (pre) sync = arrive(own, sn, true, false);
+                    else arrive(item, sn, false, sync);
+                    pre = false;
+                    tsw = __builtin_amdgcn_s_memrealtime();
+                }
+                w = wait_or_take(sn, tsw);
+                if (w < 0) break;
+                item = w;
+            }
+            if (w == -2) break;
+            if (sync) {
+                // in step: the decision from the stage's outputs (which stay
+                // until this block's item of stage sn + 1 is done); an order
+                // block's spectra for its next work (after START: F; after
+                // ROWS: F and FC) are loaded first, in flight over the decision
+                const int k0 = (int)sct[CT_KIND];
+                if (own >= npairs && (k0 == SK_START || k0 == SK_ROWS)) {
+                    fetch_order(own - npairs, k0 == SK_ROWS);
+                    pf_ok = true;
+                }
+                sle_decide(a, sct, snx, scratch(), redmax);
+                if (pending_publish) { publish(sn + 1u, snx); pending_publish = false; }
+                if (tid < CT_N) sct[tid] = snx[tid];
+                __syncthreads();
+                sn += 1u;
+                continue;
+            }
+        } else if (bcast(tid == 0 ? ld_u(a.bar + 1) : 0u) != 0u) {
+            break;                             // the launch failed
+        }
+        // behind: the published record of the current stage
+        pending_publish = false;
+        sn = stage_now();
+        have = read_ctl(sn);
+        claimed = have && claim(own, sn);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
 }
 
-__global__ void k_sle_bar_reset(unsigned* bar) {
-    if (threadIdx.x < 3) bar[threadIdx.x] = 0u;
-    if (threadIdx.x < 17) bar[VH_SLE_FLAG_STRIDE * (1 + threadIdx.x)] = 0u;
-}
-
-void sle_bar_reset(hipStream_t st, unsigned* bar) {
-    k_sle_bar_reset<<<1, 64, 0, st>>>(bar);
-    HIPCHECK(hipGetLastError());
-}
+static inline unsigned nb256(size_t n) { return (unsigned)((n + 255) / 256); }
 
 int sle_solve_grid(const vh_sht& s) { return (s.nlat + 1) / 2 + s.L + 1; }
 
-
-
-long sle_solve_max_barriers(const vh_sht& s, int n_outer, int n_inner) {
-    return (long)sle_solve_grid(s) * n_outer * (4 + 2 * (long)n_inner);
-}
-
-static inline unsigned nb256(size_t n) { return (unsigned)((n + 255) / 256); }
-
 void sle_ice(hipStream_t st, int n, const double* ice0, const double* ice1, double frac, int interp,
              const double* ice_ref, const double* topo0, double* ice, double* dice, double* C0, double* wcorr,
-             double* res, int nres) {
-    k_sle_ice<<<nb256(std::max(n, nres)), 256, 0, st>>>(n, ice0, ice1, frac, interp, ice_ref, topo0, ice, dice, C0,
-                                                         wcorr, res, nres);
+             double* res, int nres, unsigned* bar, int nitems, int n_outer) {
+    const size_t nt = std::max<size_t>(std::max(n, nres), std::max(nitems, 64));
+    k_sle_ice<<<nb256(nt), 256, 0, st>>>(n, ice0, ice1, frac, interp, ice_ref, topo0, ice, dice, C0, wcorr, res,
+                                          nres, bar, nitems, n_outer);
+    HIPCHECK(hipGetLastError());
 }
 void vh_apply_only(hipStream_t st, int nlm, const double2* sigma, const double* gu, const double* gn,
                    const double* dsurf, int nk, double g, int kind, double2* uN) {
@@ -829,47 +1142,44 @@ void grid_bsl_integrand(hipStream_t st, int n, const double* h, const double* C,
     k_bsl_integrand<<<nb256(n), 256, 0, st>>>(n, h, C, heq, C0, out);
 }
 
+// Raises the dynamic-LDS limit of one instance of the solve kernel once per
+// process (the attribute belongs to the function, not to a launch).
+template <bool TabLds, int MinW>
+static void sle_lds_limit(size_t lds) {
+    static std::atomic<size_t> set{0};
+    if (lds > 65536 && set.load() < lds) {
+        HIPCHECK(hipFuncSetAttribute((const void*)k_sle_solve<TabLds, MinW>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        set.store(lds);
+    }
+}
+template <bool TabLds, int MinW>
+static void sle_launch_one(hipStream_t st, int grid, size_t lds, const sle_solve_args& a) {
+    sle_lds_limit<TabLds, MinW>(lds);
+    k_sle_solve<TabLds, MinW><<<grid, 256, lds, st>>>(a);
+    HIPCHECK(hipGetLastError());
+}
+
 void sle_solve_launch(hipStream_t st, const sle_solve_args& a) {
     const vh_sht& s = a.s;
     if (s.nh > 256) throw vh_error("sle: the fused sea-level loop supports nlat <= 512");
-    const int grid = sle_solve_grid(s);   // row blocks, then order blocks
-    // the Legendre table of each order stays in LDS when it fits beside the rest
-    const bool tab = sle_solve_lds(s, true) <= (size_t)156 * 1024;
+    const int grid = sle_solve_grid(s);   // latitude-pair items, then order items: one block each
+    if (grid > VH_SLE_MAX_ITEMS) throw vh_error("sle: the fused sea-level loop supports at most 1024 work items");
+    // the Legendre table of each order stays in LDS when it fits beside the
+    // rest (160 KiB per block less the kernel's static LDS)
+    const size_t lds_max = (size_t)160 * 1024 - VH_SLE_STATIC_LDS;
+    const bool tab = sle_solve_lds(s, true) <= lds_max;
     const size_t lds = sle_solve_lds(s, tab);
+    if (lds > lds_max) throw vh_error("sle: the fused sea-level loop needs more than 160 KiB of LDS");
     int dev = 0, ncu = 0;
     HIPCHECK(hipGetDevice(&dev));
     HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     const bool two = grid > ncu;   // one block per CU does not hold the grid
-    const void* fn = tab ? (two ? (const void*)k_sle_solve<true, 2> : (const void*)k_sle_solve<true, 1>)
-                         : (two ? (const void*)k_sle_solve<false, 2> : (const void*)k_sle_solve<false, 1>);
-    // resident-block capacity, checked once per (device, kernel, LDS size)
-    static thread_local int cap_dev = -1;
-    static thread_local const void* cap_fn = nullptr;
-    static thread_local size_t cap_lds = 0;
-    static thread_local long cap = 0;
-    if (cap_dev != dev || cap_fn != fn || cap_lds != lds) {
-        int coop = 0, per_cu = 0;
-        HIPCHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
-        if (!coop) throw vh_error("sle: the device does not support cooperative launches");
-        if (lds > 65536) HIPCHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds));
-        cap_dev = dev; cap_fn = fn; cap_lds = lds; cap = (long)per_cu * ncu;
+    if (tab) {
+        if (two) sle_launch_one<true, 2>(st, grid, lds, a);
+        else sle_launch_one<true, 1>(st, grid, lds, a);
+    } else {
+        if (two) sle_launch_one<false, 2>(st, grid, lds, a);
+        else sle_launch_one<false, 1>(st, grid, lds, a);
     }
-    if (cap < grid)
-        throw vh_error("sle: the fused sea-level loop needs " + std::to_string(grid) +
-                       " resident blocks, the device holds " + std::to_string(cap));
-    sle_solve_args arg = a;
-    void* params[] = {&arg};
-    // Cooperative launches of different models (streams) run one after the
-    // other on the device: two grids that each hold part of the CUs while
-    // waiting for the rest would never finish (the stream waits on the
-    // previous launch's completion event).
-    static std::mutex mu;
-    static hipEvent_t last_ev[64] = {};
-    std::lock_guard<std::mutex> lock(mu);
-    if (dev < 0 || dev >= 64) throw vh_error("sle: device index out of range");
-    if (!last_ev[dev]) HIPCHECK(hipEventCreateWithFlags(&last_ev[dev], hipEventDisableTiming));
-    else HIPCHECK(hipStreamWaitEvent(st, last_ev[dev], 0));
-    HIPCHECK(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(256), params, (unsigned)lds, st));
-    HIPCHECK(hipEventRecord(last_ev[dev], st));
 }

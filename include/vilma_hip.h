@@ -22,11 +22,11 @@
  *     `error stop 'message'` instead (e.g. src/vilma_coupling.f90:248,279);
  *     the Fortran shim turns a nonzero status back into `error stop message`.
  *   - A handle is not thread-safe; distinct handles may be used from different
- *     threads. Their sea-level solves (one cooperative kernel each, holding a
- *     block on most CUs) run one after another on the device; a solve whose
- *     blocks cannot all become resident within its bounded wait (a device
- *     saturated by other handles' work) fails with an error rather than hang,
- *     so throughput runs step the handles of one device from one thread.
+ *     threads and processes at once, each on its own stream. No kernel needs
+ *     its blocks to be resident together: the sea-level solve's blocks take
+ *     over the work of blocks that have not started (VH_S_N_SLE_STEAL counts
+ *     it), so a device shared with other work slows a solve but does not
+ *     stop it.
  */
 #ifndef VILMA_HIP_H
 #define VILMA_HIP_H
@@ -86,7 +86,9 @@ enum {
     VH_S_TIME = 0, VH_S_BSL = 1, VH_S_WORST_MASS_RESID = 2,
     VH_S_N_ACCEPT = 3, VH_S_N_REJECT = 4, VH_S_N_SOLVE = 5,
     VH_S_N_INNER_TOT = 6, VH_S_N_OUTER_TOT = 7,
-    VH_S_POLAR_M_RE = 8, VH_S_POLAR_M_IM = 9
+    VH_S_POLAR_M_RE = 8, VH_S_POLAR_M_IM = 9,
+    VH_S_N_SLE_STEAL = 10   /* sea-level work items a block ran for another that
+                               was not resident (a device shared with other work) */
 };
 enum {
     VH_SP_SIGMA = 0,   /* se%sigma_lm: converged surface load [kg m^-2]      */

@@ -14,16 +14,6 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
 
 
-def pytest_collection_modifyitems(session, config, items):
-    """The tests that run the library in child processes (test_fortran.py) go
-    first: once this process has launched the cooperative sea-level kernel it
-    holds the GPU's cooperative-launch resource, and a child's own launch would
-    fail. The order within each group is kept."""
-    first = [it for it in items if os.path.basename(str(it.fspath)) == "test_fortran.py"]
-    if first:
-        items[:] = first + [it for it in items if os.path.basename(str(it.fspath)) != "test_fortran.py"]
-
-
 def gpu_available():
     try:
         import torch

@@ -102,7 +102,9 @@ def test_guard_rejects_nan_memory(oracle):
     assert rel(mg.rsl, mo.rsl) <= 1e-10
     rsl0 = mg.rsl
     tau = mg.memory()
-    tau[0, 5, 3, 1] = np.nan
+    r, lay = oracle.mesh(c["params"])
+    e = [k for k in range(len(lay)) if c["params"].rheology[lay[k]] == 1][2]   # a Maxwell element
+    tau[0, 5, e, 1] = np.nan
     for m in (mo, mg):
         m.set_memory(tau)
         with pytest.raises(VilmaError, match="explicit guard sub-step collapsed"):

@@ -302,7 +302,9 @@ def test_guard_rejects_nan_memory(oracle):
     m.update(c["ice"](50), c["dt_yr"])
     rsl0 = m.rsl
     tau = m.memory()
-    tau[0, 5, 3, 1] = np.nan
+    r, lay = oracle.mesh(c["params"])
+    e = [k for k in range(len(lay)) if c["params"].rheology[lay[k]] == 1][2]   # a Maxwell element
+    tau[0, 5, e, 1] = np.nan
     m.set_memory(tau)
     n_acc = m.scalar(3)
     with pytest.raises(VilmaError, match="explicit guard sub-step collapsed"):
