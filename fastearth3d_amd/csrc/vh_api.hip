@@ -1557,6 +1557,7 @@ int vh_remap_create(int nphi, int nlat, int nlon, int nlat_ll, const double* lon
         try {
             HIPCHECK(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
             m->r = vh_remap_build(m->st, nphi, nlat, colat.data(), w.data(), nlon, nlat_ll, lon_deg, lat_deg);
+            vh_remap_scratch_setup();   // this device's address of the allocation-free variant's array
         } catch (...) {
             if (m->st) (void)hipStreamDestroy(m->st);
             delete m;
@@ -1569,6 +1570,14 @@ int vh_remap_create(int nphi, int nlat, int nlon, int nlat_ll, const double* lon
 int vh_remap_to_gauss(vh_remap_handle* m, const double* f_ll, double* f_gauss, int conserve_mass) {
     if (!m) return 2;
     return remap_guarded([&] { vh_remap_to_gauss_host(m->r, f_ll, f_gauss, conserve_mass); });
+}
+
+int vh_remap_to_gauss_noalloc(vh_remap_handle* m, const double* f_ll, double* f_gauss, int conserve_mass) {
+    if (!m) return 2;
+    return remap_guarded([&] {
+        if (!f_ll || !f_gauss) throw vh_error("remap_to_gauss_noalloc: null field");
+        vh_remap_to_gauss_noalloc(m->r, f_ll, f_gauss, conserve_mass);
+    });
 }
 
 int vh_remap_to_ll(vh_remap_handle* m, const double* f_gauss, double* f_ll) {

@@ -18,7 +18,9 @@
 // outermost Gauss latitude the nearest row is taken (the reference leaves such
 // choices to `coords`).
 #include <cfloat>
+#include <mutex>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "vh_internal.h"
@@ -270,7 +272,9 @@ vh_remap* vh_remap_build(hipStream_t st, int nphi, int nlat, const double* colat
 
 static inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
 
-void vh_remap_to_gauss_dev(vh_remap* r, const double* f_ll, double* f_g, int conserve_mass) {
+// the conservative map, then (conserve_mass) the two mass totals into tot[0..1]
+// and the rescale; f_ll, f_g and tot in device memory
+static void remap_to_gauss_tot(vh_remap* r, const double* f_ll, double* f_g, int conserve_mass, double* tot) {
     const size_t ng = (size_t)r->nphi * r->nlat, nh = (size_t)r->nlon * r->nlat_ll;
     k_remap_to_gauss<<<nblk(ng), 256, 0, r->st>>>(r->nphi, r->nlat, r->nlon, r->kla, r->klo, r->lat_first,
                                                    r->lat_count, r->lat_w, r->lon_first, r->lon_count, r->lon_w,
@@ -279,11 +283,65 @@ void vh_remap_to_gauss_dev(vh_remap* r, const double* f_ll, double* f_g, int con
     if (conserve_mass) {
         // source integral in steradians (the host cells' areas sum to 4 pi) and
         // the Gauss quadrature integral of the result
-        k_remap_dot<<<1, 256, 0, r->st>>>((int)nh, r->src_wt, f_ll, 0, r->tot);
-        k_remap_dot<<<1, 256, 0, r->st>>>((int)ng, r->gw, f_g, r->nphi, r->tot + 1);
-        k_remap_scale<<<nblk(ng), 256, 0, r->st>>>((int)ng, r->tot, f_g);
+        k_remap_dot<<<1, 256, 0, r->st>>>((int)nh, r->src_wt, f_ll, 0, tot);
+        k_remap_dot<<<1, 256, 0, r->st>>>((int)ng, r->gw, f_g, r->nphi, tot + 1);
+        k_remap_scale<<<nblk(ng), 256, 0, r->st>>>((int)ng, tot, f_g);
         HIPCHECK(hipGetLastError());
     }
+}
+
+void vh_remap_to_gauss_dev(vh_remap* r, const double* f_ll, double* f_g, int conserve_mass) {
+    remap_to_gauss_tot(r, f_ll, f_g, conserve_mass, r->tot);
+}
+
+// ---- the allocation-free remap_to_gauss (vh_remap_to_gauss_noalloc)
+// Its device memory beyond the handle's tables is this file-scope array:
+// the staged host field, the staged Gauss field and the two mass totals
+// (nlon nlat_ll + nphi nlat + 2 doubles at most VH_REMAP_SCRATCH_DOUBLES).
+// Each GPU holds its own copy for the life of the process; its address on a
+// device comes from hipGetSymbolAddress once per device (vh_remap_scratch_setup,
+// called by vh_remap_create). Calls on one GPU share it, so calls that overlap
+// in time on one GPU race; the array keeps the last call's contents.
+#define VH_REMAP_SCRATCH_DOUBLES (1 << 20)
+#define VH_REMAP_MAX_DEVICES 64
+static __device__ __align__(256) double scratch[VH_REMAP_SCRATCH_DOUBLES];
+static double* g_scratch_ptr[VH_REMAP_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+void vh_remap_scratch_setup() {
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= VH_REMAP_MAX_DEVICES) throw vh_error("remap: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return;
+    double* p = nullptr;
+    HIPCHECK(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+}
+
+size_t vh_remap_scratch_doubles() { return VH_REMAP_SCRATCH_DOUBLES; }
+
+void vh_remap_to_gauss_noalloc(vh_remap* r, const double* f_ll, double* f_g, int conserve_mass) {
+    const size_t ng = (size_t)r->nphi * r->nlat, nh = (size_t)r->nlon * r->nlat_ll;
+    if (nh + ng + 2 > (size_t)VH_REMAP_SCRATCH_DOUBLES)
+        throw vh_error("remap_to_gauss_noalloc: host grid (" + std::to_string(nh) + ") plus Gauss grid (" +
+                       std::to_string(ng) + ") points exceed the fixed scratch of " +
+                       std::to_string(VH_REMAP_SCRATCH_DOUBLES - 2) + " values");
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    double* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        base = (dev >= 0 && dev < VH_REMAP_MAX_DEVICES) ? g_scratch_ptr[dev] : nullptr;
+    }
+    if (!base) throw vh_error("remap_to_gauss_noalloc: no remap map was created on this device");
+    double* s_ll = base;
+    double* s_g = base + nh;
+    double* tot = base + nh + ng;
+    HIPCHECK(hipMemcpyAsync(s_ll, f_ll, nh * sizeof(double), hipMemcpyHostToDevice, r->st));
+    remap_to_gauss_tot(r, s_ll, s_g, conserve_mass, tot);
+    HIPCHECK(hipMemcpyAsync(f_g, s_g, ng * sizeof(double), hipMemcpyDeviceToHost, r->st));
+    HIPCHECK(hipStreamSynchronize(r->st));
 }
 
 void vh_remap_to_ll_dev(vh_remap* r, const double* f_g, double* f_ll) {

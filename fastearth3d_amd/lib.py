@@ -56,6 +56,7 @@ def load():
                                       C.POINTER(C.c_double), C.POINTER(C.c_void_p)]
         L.vh_remap_to_gauss.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
         L.vh_remap_to_ll.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.vh_remap_to_gauss_noalloc.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
         L.vh_remap_to_gauss_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         L.vh_remap_to_ll_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.vh_remap_destroy.argtypes = [C.c_void_p]
@@ -241,6 +242,18 @@ class Remap:
             raise VilmaError("remap_to_gauss: source field shape /= map source grid")
         g = np.empty((self.nlat, self.nphi))
         self._check(self._lib.vh_remap_to_gauss(self._r, _cp(f), _cp(g), int(bool(conserve_mass))), "remap_to_gauss")
+        return g
+
+    def to_gauss_noalloc(self, f_ll, conserve_mass=False):
+        """remap_to_gauss through the allocation-free entry point
+        (vh_remap_to_gauss_noalloc): the same result, staged in the library's
+        fixed per-GPU array."""
+        f = np.ascontiguousarray(f_ll, dtype=np.float64)
+        if f.size != self.nlon * self.nlat_ll:
+            raise VilmaError("remap_to_gauss_noalloc: source field shape /= map source grid")
+        g = np.empty((self.nlat, self.nphi))
+        self._check(self._lib.vh_remap_to_gauss_noalloc(self._r, _cp(f), _cp(g), int(bool(conserve_mass))),
+                    "remap_to_gauss_noalloc")
         return g
 
     def to_ll(self, f_gauss):

@@ -175,6 +175,17 @@ int vh_remap_create(int nphi, int nlat, int nlon, int nlat_ll, const double* lon
  * (north-first); conserve_mass != 0 rescales the result so that its Gauss
  * quadrature integral equals the host area integral (ice thickness). */
 int vh_remap_to_gauss(vh_remap_handle* m, const double* f_ll, double* f_gauss, int conserve_mass);
+/* remap_to_gauss as vh_remap_to_gauss (same kernels, same results), for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): no counterpart in
+ * the reference. It calls no hipMalloc, hipFree or allocating library
+ * routine; the host field, the Gauss field and the two mass totals are staged
+ * in one fixed device array per GPU of 1048576 doubles (8 MiB, held until
+ * the process ends), so nlon*nlat_ll + nphi*nlat must not exceed 1048574
+ * (larger grids are refused with an error). Calls on one GPU share that
+ * array: calls that overlap in time on one GPU race. Needs a map created on
+ * the same device (vh_remap_create sets the array up). */
+int vh_remap_to_gauss_noalloc(vh_remap_handle* m, const double* f_ll, double* f_gauss, int conserve_mass);
 /* remap_to_ll (src/vilma_remap.f90:140-164): bilinear, f_gauss -> f_ll. */
 int vh_remap_to_ll(vh_remap_handle* m, const double* f_gauss, double* f_ll);
 /* The same with device (HBM) pointers. */
