@@ -139,7 +139,7 @@ __global__ void k_bsl_integrand(int n, const double* __restrict__ h, const doubl
 // Legendre sums, order of the totals), so a rerun gives identical bits.
 // No block waits for a block that has not started: block b owns item b of
 // every stage and claims the next stage's item before it arrives at the
-// current stage's end; a block that has waited VH_SLE_STEAL_TICKS on a stage
+// current stage's end; a block that has waited VH_SLE_STEAL_POLLS polls on a stage
 // claims and runs the stage's unclaimed items (whose owners are not resident:
 // a device shared with other work or another process). A stage therefore
 // completes as long as one block runs, and its result does not depend on
@@ -152,11 +152,14 @@ __global__ void k_bsl_integrand(int n, const double* __restrict__ h, const doubl
 // earlier kernels are acquired once at entry. A block keeps LDS copies (its
 // row pair's Sraw, the FFT twiddles, its order's Legendre table) and uses
 // them only while they are valid. Every wait is bounded: past
-// VH_SLE_WAIT_TICKS the error flag bar[1] and res[2] are raised and every
+// VH_SLE_WAIT_POLLS the error flag bar[1] and res[2] are raised and every
 // block leaves.
 
-#define VH_SLE_STEAL_TICKS 4000ull         // 40 us at the 100 MHz s_memrealtime clock
-#define VH_SLE_WAIT_TICKS 400000000ull     // 4 s
+// polls of a release word (each an agent-scope load, about a microsecond,
+// plus a short sleep) before a waiting block takes over unclaimed work, and
+// before a wait counts as failed
+#define VH_SLE_STEAL_POLLS 48u
+#define VH_SLE_WAIT_POLLS 4000000u
 // P-table values a thread has in flight in the order stage
 #define VH_SLE_JB 8
 #define VH_SLE_LB 4
@@ -846,12 +849,11 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     auto read_ctl = [&](unsigned sn) __attribute__((always_inline)) -> bool {
         unsigned ok = 0;
         if (tid == 0) {
-            const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-            for (;;) {
+            for (unsigned polls = 0;; polls++) {
                 const unsigned q = ld_u(sle_slot_seq(a.bar, sn));
                 if (q == sn + 1u) { ok = 1; break; }
                 if (q > sn + 1u || ld_u(a.bar + 1) != 0u) break;
-                if (__builtin_amdgcn_s_memrealtime() - t0 > VH_SLE_WAIT_TICKS) {
+                if (polls > VH_SLE_WAIT_POLLS) {
                     __hip_atomic_fetch_or(a.bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     st_coh(a.res + 2, 1.0);
                     break;
@@ -973,7 +975,8 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     // of stage sn + 1 (sync: the stage's outputs stay until that item is
     // done; it publishes the decision afterwards), else after taking and
     // publishing the decision. Returns {completed, own item of sn + 1 claimed}.
-    bool pending_publish = false;
+    bool pending_publish = false;   // this block completed stage pending_sn - 1 and owes record pending_sn
+    unsigned pending_sn = 0;
     auto arrive = [&](int i, unsigned sn, bool pre, bool sync) __attribute__((always_inline)) -> bool {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_s_waitcnt(0);   // this wavefront's stores are complete
@@ -1005,29 +1008,28 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
                 for (int k = 0; k < 8; k++)
                     __hip_atomic_store(sle_rel(a.bar, k), sn + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             pending_publish = after;
+            pending_sn = sn + 1u;
         }
         return pre ? got : last;
     };
     // wait until stage sn is released. A block that has waited
-    // VH_SLE_STEAL_TICKS looks for the stage's unclaimed items and claims
+    // VH_SLE_STEAL_POLLS polls looks for the stage's unclaimed items and claims
     // one: returns its index (>= 0; the caller runs it), -1 once the stage is
     // released, -2 on the error bound.
-    auto wait_or_take = [&](unsigned sn, unsigned long long& tsw) __attribute__((always_inline)) -> int {
-        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+    auto wait_or_take = [&](unsigned sn, unsigned& polls) __attribute__((always_inline)) -> int {
         for (;;) {
             unsigned st = 0;
             if (tid == 0) {
-                for (;;) {
+                for (;; polls++) {
                     if (ld_u(sle_rel(a.bar, own & 7)) >= sn + 1u) { st = 0; break; }
-                    if (ld_u(a.bar + 1) != 0u) { st = 2; break; }
-                    const unsigned long long now = __builtin_amdgcn_s_memrealtime();
-                    if (now - t0 > VH_SLE_WAIT_TICKS) {
+                    if ((polls & 7u) == 7u && ld_u(a.bar + 1) != 0u) { st = 2; break; }
+                    if (polls > VH_SLE_WAIT_POLLS) {
                         __hip_atomic_fetch_or(a.bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         st_coh(a.res + 2, 1.0);
                         st = 2;
                         break;
                     }
-                    if (now - tsw > VH_SLE_STEAL_TICKS) { st = 1; break; }
+                    if (polls % VH_SLE_STEAL_POLLS == VH_SLE_STEAL_POLLS - 1u) { st = 1; polls++; break; }
                     __builtin_amdgcn_s_sleep(1);
                 }
             }
@@ -1050,7 +1052,6 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
                     if (claim(j, sn)) return j;
                 }
             }
-            tsw = __builtin_amdgcn_s_memrealtime();
         }
     };
 
@@ -1059,25 +1060,28 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     bool claimed = have && claim(own, sn);
     for (;;) {
         if (have) {
-            if ((int)sct[CT_KIND] == SK_DONE) break;
+            if ((int)sct[CT_KIND] == SK_DONE) {
+                if (pending_publish && pending_sn == sn) publish(sn, sct);   // the result slots
+                break;
+            }
             bool sync = false;                 // this block holds its item of stage sn + 1
             int item = claimed ? own : -1;
             bool pre = claimed;                // the own item's arrival claims the next one
-            unsigned long long tsw = 0;
+            unsigned polls = 0;                // of this stage's release word (thread 0)
             int w = 0;
             for (;;) {
                 if (item >= 0) {
                     // the one place where work items run (keeps the transforms inlined once)
                     if (item != own) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                     run_item(item, sn, item != own);
+                    if (pending_publish && pending_sn == sn) { publish(sn, sct); pending_publish = false; }
                     if (item != own && tid == 0)
                         __hip_atomic_fetch_add(a.bar + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (pre) sync = arrive(own, sn, true, false);
                     else arrive(item, sn, false, sync);
                     pre = false;
-                    tsw = __builtin_amdgcn_s_memrealtime();
                 }
-                w = wait_or_take(sn, tsw);
+                w = wait_or_take(sn, polls);
                 if (w < 0) break;
                 item = w;
             }
@@ -1093,10 +1097,11 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
                     pf_ok = true;
                 }
                 sle_decide(a, sct, snx, scratch(), redmax);
-                if (pending_publish) { publish(sn + 1u, snx); pending_publish = false; }
                 if (tid < CT_N) sct[tid] = snx[tid];
                 __syncthreads();
                 sn += 1u;
+                // the record of stage sn is published (pending_publish) after this
+                // block's own work of the stage, off the stage's critical path
                 continue;
             }
         } else if (bcast(tid == 0 ? ld_u(a.bar + 1) : 0u) != 0u) {
