@@ -31,6 +31,7 @@
 // row kernel's column access is shared through L2 by an XCD-aware block
 // order); batch b = 3*i3 + s.
 #include <vector>
+#include <algorithm>
 #include "vh_internal.h"
 #include "vh_radial_dev.h"
 #include "vh_sht_dev.h"
@@ -62,7 +63,7 @@ static inline unsigned xcd_order_grid(int ngrp, int L) { return (unsigned)(8 * n
 // of the order are loaded once per block for all TB batches; the pair swaps
 // its partial sums and the even lane writes the north row, the odd lane the
 // south row.
-template <int TB>
+template <int TB, bool JCHUNK>
 __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __restrict__ X, int nb,
                                                    double2* __restrict__ Fd) {
     extern __shared__ double2 cs[];   // [TB][5][nl]: c1, c2, -J c5, c6, -J c6
@@ -85,11 +86,14 @@ __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __re
         if (lam == 3) cb[4 * nl + il] = make_double2(-J * v.x, -J * v.y);
     }
     __syncthreads();
-    const int j = threadIdx.x >> 1, par = threadIdx.x & 1;
-    const bool live = j < nh;   // both lanes of a pair agree
-    const int jj = live ? j : 0;
     const double* P = s.plm_lj + (size_t)s.mstart[m] * nh;
     const double* D = s.dplm_lj + (size_t)s.mstart[m] * nh;
+    // lane pairs take north latitudes j0 + threadIdx.x / 2, 128 at a time
+    // (JCHUNK: more than 128 north latitudes)
+    for (int j0 = 0; j0 < (JCHUNK ? nh : 1); j0 += 128) {
+    const int j = j0 + (threadIdx.x >> 1), par = threadIdx.x & 1;
+    const bool live = j < nh;   // both lanes of a pair agree
+    const int jj = live ? j : 0;
     // [batch][sum]: A1, A2, D2, A5, A6, L6, D6 over this lane's parity
     double2 acc[TB][7];
 #pragma unroll
@@ -148,6 +152,7 @@ __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __re
         for (int dy = 0; dy < 6; dy++)
             Fd[(((size_t)(b0 + bb) * 6 + dy) * (L + 1) + m) * nlat + row] = out[dy];
     }
+    }
 }
 
 // ----------------------------------------------------------- analysis
@@ -167,8 +172,10 @@ typedef double vh_d4 __attribute__((ext_vector_type(4)));
 #ifndef TA_JC
 #define TA_JC 32        // latitudes per LDS chunk
 #endif
-#define TA_ITEMS 3      // row tiles per wavefront (4 wavefronts, <= 10 tiles)
-
+// TA_ITEMS: row tiles per wavefront (4 wavefronts): 3 holds the 16-degree
+// tiles of both parities up to lmax 159 (ceil(ne/16) + ceil(no/16) <= 12),
+// 4 up to lmax 255
+template <int TA_ITEMS>
 __global__ void __launch_bounds__(256) k_tsh_analysis(vh_sht s, const double2* __restrict__ Fd, int nb,
                                                       const double* __restrict__ n6, double2* __restrict__ C) {
     // one LDS array, two views: the folded latitude chunk Bs [parity][latitude
@@ -466,20 +473,23 @@ static size_t rows_lds(const vh_sht& s) {
 
 void vh_tsh_synth(hipStream_t st, const vh_sht& s, const double2* X, int nb, double2* Fd) {
     if (nb <= 0) return;
-    if (s.nh > 128) throw vh_error("tensor transform: nlat above 256 is not supported");
     const int ngrp = (nb + VH_TSH_TB_SYNTH - 1) / VH_TSH_TB_SYNTH;
-    const int nthr = ((2 * s.nh + 63) / 64) * 64;
-    k_tsh_synth<VH_TSH_TB_SYNTH><<<xcd_grid(ngrp * (s.L + 1)), nthr, synth_lds(s), st>>>(s, X, nb, Fd);
+    const int nthr = std::min(256, ((2 * s.nh + 63) / 64) * 64);
+    if (s.nh > 128)
+        k_tsh_synth<VH_TSH_TB_SYNTH, true><<<xcd_grid(ngrp * (s.L + 1)), nthr, synth_lds(s), st>>>(s, X, nb, Fd);
+    else
+        k_tsh_synth<VH_TSH_TB_SYNTH, false><<<xcd_grid(ngrp * (s.L + 1)), nthr, synth_lds(s), st>>>(s, X, nb, Fd);
     HIPCHECK(hipGetLastError());
 }
 
 void vh_tsh_analysis(hipStream_t st, const vh_sht& s, const double2* Fd, int nb, const double* n6, double2* C) {
     if (nb <= 0) return;
     // row tiles of one order: ceil(ne/16) + ceil(no/16) <= 4 wavefronts x TA_ITEMS
-    if ((s.L + 2) / 2 > 16 * 5 || (s.L + 1) / 2 > 16 * 5)
-        throw vh_error("tensor analysis: lmax above 159 is not supported by the MFMA tiling");
+    const int tiles = ((s.L + 2) / 2 + 15) / 16 + ((s.L + 1) / 2 + 15) / 16;
+    if (tiles > 16) throw vh_error("tensor analysis: lmax above 255 is not supported by the MFMA tiling");
     const int ngrp = (nb + TA_B - 1) / TA_B;
-    k_tsh_analysis<<<xcd_order_grid(ngrp, s.L), 256, 0, st>>>(s, Fd, nb, n6, C);
+    if (tiles <= 12) k_tsh_analysis<3><<<xcd_order_grid(ngrp, s.L), 256, 0, st>>>(s, Fd, nb, n6, C);
+    else k_tsh_analysis<4><<<xcd_order_grid(ngrp, s.L), 256, 0, st>>>(s, Fd, nb, n6, C);
     HIPCHECK(hipGetLastError());
 }
 

@@ -48,10 +48,12 @@ def _pair(oracle, idx, lmax, **over):
     return c, mo, mg
 
 
-@pytest.mark.parametrize("L", [16, 48, 64])
+@pytest.mark.parametrize("L", [16, 48, 64, 96, 192])
 def test_tensor_transforms(oracle, L):
     """Tensor synthesis / analysis against the oracle; nphi 64 and 192 take the run-time
-    longitude FFT, 256 (lmax 64) the compile-time in-place one."""
+    longitude FFT, 256 (lmax 64) the compile-time in-place one. lmax 192 (386 latitudes)
+    runs the synthesis over two chunks of north latitudes and the analysis with four
+    row tiles per wave."""
     c, mo, mg = _pair(oracle, 1, L)
     nlm = mo.dims["nlm"]
     x = _random_tensor(L, nlm, 3)
@@ -69,10 +71,11 @@ def test_tensor_transforms(oracle, L):
     assert rel(mg.tensor_analysis(dg), x) <= 1e-10
 
 
-@pytest.mark.parametrize("L", [16, 64])
+@pytest.mark.parametrize("L", [16, 64, 192])
 def test_advance_3d(oracle, L):
     """One 3-D advance from a random memory state (config-2 shaped pert); at lmax 64 the row
-    pass runs the compile-time in-place FFT (nphi 256)."""
+    pass runs the compile-time in-place FFT (nphi 256); lmax 192 is the largest 3-D member
+    of the ensemble workload."""
     c, mo, mg = _pair(oracle, 2, L)
     for m in (mo, mg):
         m.enable_visc_3d(c["pert"])
