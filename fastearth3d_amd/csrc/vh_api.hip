@@ -255,6 +255,13 @@ struct vh_model {
     }
     bool lat_visc = false;
     int ne3d = 0;
+    // the lateral field of the last enable through the API, for the spin-up's
+    // 1-D phase: 1 = per-element perturbation (lat_field, ne x grid; lat_mk the
+    // 1-D rates it was applied to), 2 = nodal absolute log10 viscosity
+    // (lat_field, nr x grid; lat_lid: lid depth, lid log10 max, cap)
+    int lat_src = 0;
+    std::vector<double> lat_field, lat_mk;
+    double lat_lid[3] = {0.0, 0.0, 0.0};
     std::vector<int> e3mesh;   // mesh element of each 3-D element
     adj_ctx* adj = nullptr;    // reverse-mode record (vh_forward_record / vh_backward)
 
@@ -1742,32 +1749,153 @@ void enable_lateral(vh_model* h, const double* pert) {
     h->sync();
 }
 
-int vh_enable_visc_3d(vh_model* h, const double* pert_elem) {
-    return guarded(h, [&] { enable_lateral(h, pert_elem); });
-}
-
 // response_enable_lateral_visc_from_nodes (src/vilma_response.f90:1120-1169):
 // element value = mean of its two nodes' log10 viscosity, capped in the lid.
+static void enable_lateral_nodes(vh_model* h, const double* visc_node, double lid_depth, double lid_log10max,
+                          double log10_cap) {
+    if (h->kind != 2) throw vh_error("solid_earth_enable_visc_3d: lateral viscosity needs earth_response=ve");
+    const int ne = h->d.ne, nr = h->d.nr;
+    const size_t ng = (size_t)h->d.nphi * h->d.nlat;
+    double r_lid = DBL_MAX;
+    if (lid_depth > 0.0) r_lid = h->r[nr - 1] - lid_depth;
+    std::vector<double> pert(ng * ne, 0.0);
+    for (int e = 0; e < ne; e++) {
+        if (h->MkPerDt[e] == 0.0) continue;
+        double logeta_ref = std::log10(h->mu[e] / h->MkPerDt[e]);
+        bool in_lid = h->r[e] >= r_lid - 1.0e-6 * h->r[nr - 1];
+        for (size_t sp = 0; sp < ng; sp++) {
+            double ea = 0.5 * (visc_node[sp + (size_t)e * ng] + visc_node[sp + (size_t)(e + 1) * ng]);
+            if (in_lid && ea > lid_log10max) ea = log10_cap;
+            pert[(size_t)e * ng + sp] = ea - logeta_ref;
+        }
+    }
+    enable_lateral(h, pert.data());
+}
+
+int vh_enable_visc_3d(vh_model* h, const double* pert_elem) {
+    return guarded(h, [&] {
+        if (!pert_elem) throw vh_error("enable_lateral_visc: null field");
+        const size_t n = (size_t)h->d.ne * h->d.nphi * h->d.nlat;
+        std::vector<double> mk = h->MkPerDt;
+        enable_lateral(h, pert_elem);
+        h->lat_src = 1;
+        h->lat_field.assign(pert_elem, pert_elem + n);
+        h->lat_mk = mk;
+    });
+}
+
 int vh_enable_visc_3d_nodes(vh_model* h, const double* visc_node, double lid_depth, double lid_log10max,
                             double log10_cap) {
     return guarded(h, [&] {
-        if (h->kind != 2) throw vh_error("solid_earth_enable_visc_3d: lateral viscosity needs earth_response=ve");
-        const int ne = h->d.ne, nr = h->d.nr;
-        const size_t ng = (size_t)h->d.nphi * h->d.nlat;
-        double r_lid = DBL_MAX;
-        if (lid_depth > 0.0) r_lid = h->r[nr - 1] - lid_depth;
-        std::vector<double> pert(ng * ne, 0.0);
-        for (int e = 0; e < ne; e++) {
-            if (h->MkPerDt[e] == 0.0) continue;
-            double logeta_ref = std::log10(h->mu[e] / h->MkPerDt[e]);
-            bool in_lid = h->r[e] >= r_lid - 1.0e-6 * h->r[nr - 1];
-            for (size_t sp = 0; sp < ng; sp++) {
-                double ea = 0.5 * (visc_node[sp + (size_t)e * ng] + visc_node[sp + (size_t)(e + 1) * ng]);
-                if (in_lid && ea > lid_log10max) ea = log10_cap;
-                pert[(size_t)e * ng + sp] = ea - logeta_ref;
-            }
+        if (!visc_node) throw vh_error("enable_lateral_visc_from_nodes: null field");
+        const size_t n = (size_t)h->d.nr * h->d.nphi * h->d.nlat;
+        enable_lateral_nodes(h, visc_node, lid_depth, lid_log10max, log10_cap);
+        h->lat_src = 2;
+        h->lat_field.assign(visc_node, visc_node + n);
+        h->lat_lid[0] = lid_depth; h->lat_lid[1] = lid_log10max; h->lat_lid[2] = log10_cap;
+    });
+}
+
+// relax_hold (src/vilma_coupling.f90:519-571): hold ice and relax to
+// bed-stationary convergence in passes of growing length (SPINUP_DT0 = 200 yr,
+// x SPINUP_GROW = 1.6, at most SPINUP_MAXP = 60 passes, :65-67), each a
+// coupling call; exit when the mean bed velocity |dz_bed| / pass < tol, or at
+// t_cap years (t_cap < 0: no cap) with a warning. One line per pass, as the
+// reference logs.
+static void relax_hold(vh_model* h, const double* ice, double t_cap, const char* label, double tol) {
+    const double dt0 = 200.0, grow = 1.6;
+    const int maxp = 60;
+    const size_t ng = h->ng();
+    hipStream_t st = h->st;
+    double* z_prev = dalloc<double>(ng);
+    std::vector<double> diff(ng);
+    double pass_dt = dt0, t_done = 0.0, rrate = DBL_MAX;
+    bool converged = false;
+    try {
+        HIPCHECK(hipMemcpyAsync(z_prev, h->d_zbed, ng * sizeof(double), hipMemcpyDeviceToDevice, st));
+        if (t_cap > 0.0) printf(" spin-up [%s]: relax LGM memory vs reference, cap %.0f yr\n", label, t_cap);
+        else printf(" spin-up [%s]: relax LGM memory vs reference (1-D pre-spin)\n", label);
+        for (int it = 1; it <= maxp; it++) {
+            if (t_cap > 0.0) pass_dt = std::min(pass_dt, t_cap - t_done);
+            if (pass_dt <= 0.0) break;
+            const long nsub0 = h->n_accept;
+            update(h, ice, pass_dt, false);
+            t_done += pass_dt;
+            const long nsub = std::max(1L, h->n_accept - nsub0);
+            const double mean_dt = pass_dt / (double)nsub;
+            grid_absdiff(st, (int)ng, h->d_zbed, z_prev, h->d_gtmp);
+            vh_surface_integral(st, h->s, h->d_gtmp, h->d_part, h->d_sc + 14);
+            HIPCHECK(hipMemcpyAsync(diff.data(), h->d_gtmp, ng * sizeof(double), hipMemcpyDeviceToHost, st));
+            h->read(h->d_sc + 14, 1);
+            const double rmean = h->h_pin[0] / (4.0 * VH_PI);
+            double rmax = 0.0;
+            for (double v : diff) rmax = std::max(rmax, v);
+            rrate = rmean / pass_dt;
+            printf("   pass %3d:  <dt>=%9.1f yr (%ld sub)  d|z_bed|=%9.4f m  max=%9.2f m  v=%9.2e m/yr  v/tol=%8.1e\n",
+                   it, mean_dt, nsub, rmean, rmax, rrate, rrate / tol);
+            if (rrate < tol) { converged = true; break; }
+            HIPCHECK(hipMemcpyAsync(z_prev, h->d_zbed, ng * sizeof(double), hipMemcpyDeviceToDevice, st));
+            pass_dt *= grow;
         }
-        enable_lateral(h, pert.data());
+        if (t_cap > 0.0 && !converged)
+            printf(" WARNING: spin-up [%s] hit equil_time_max=%.1f yr before convergence (v=%9.2e m/yr, %8.1ex tol)\n",
+                   label, t_done, rrate, rrate / tol);
+        fflush(stdout);
+        h->sync();
+    } catch (...) {
+        (void)hipFree(z_prev);
+        throw;
+    }
+    (void)hipFree(z_prev);
+}
+
+// solid_earth_spinup (src/vilma_coupling.f90:453-517): seed the entering ice
+// with a zero-length call, then (pre_spinup_1d) relax the model with its
+// lateral viscosity replaced by the lateral mean (a 1-D model on the mean
+// profile; the memory carries over) and restore the real field, then relax
+// the full model up to equil_time_max years.
+static void spinup(vh_model* h, const double* ice, double t_max, double tol, bool pre_1d) {
+    if (!pre_1d && t_max <= 0.0) return;
+    if (tol <= 0.0) throw vh_error("solid_earth_spinup: equil_rate_tol must be positive");
+    update(h, ice, 0.0, false);
+    if (pre_1d) {
+        const size_t ng = h->ng();
+        if (h->lat_src == 2) {   // lateral mean of log10(eta) per node
+            const int nr = h->d.nr;
+            std::vector<double> unif((size_t)nr * ng);
+            for (int r = 0; r < nr; r++) {
+                double sum = 0.0;
+                for (size_t i = 0; i < ng; i++) sum += h->lat_field[(size_t)r * ng + i];
+                std::fill(unif.begin() + (size_t)r * ng, unif.begin() + (size_t)(r + 1) * ng, sum / (double)ng);
+            }
+            enable_lateral_nodes(h, unif.data(), h->lat_lid[0], h->lat_lid[1], h->lat_lid[2]);
+        } else if (h->lat_src == 1) {   // lateral mean of the perturbation per element
+            const int ne = h->d.ne;
+            std::vector<double> unif((size_t)ne * ng);
+            for (int e = 0; e < ne; e++) {
+                double sum = 0.0;
+                for (size_t i = 0; i < ng; i++) sum += h->lat_field[(size_t)e * ng + i];
+                std::fill(unif.begin() + (size_t)e * ng, unif.begin() + (size_t)(e + 1) * ng, sum / (double)ng);
+            }
+            h->MkPerDt = h->lat_mk;
+            enable_lateral(h, unif.data());
+        }
+        relax_hold(h, ice, -1.0, "1-D ", tol);
+        if (h->lat_src == 2) {
+            enable_lateral_nodes(h, h->lat_field.data(), h->lat_lid[0], h->lat_lid[1], h->lat_lid[2]);
+        } else if (h->lat_src == 1) {
+            h->MkPerDt = h->lat_mk;
+            enable_lateral(h, h->lat_field.data());
+        }
+    }
+    if (t_max > 0.0) relax_hold(h, ice, t_max, "full", tol);
+}
+
+int vh_spinup(vh_model* h, const double* h_ice_lgm, double equil_time_max_yr, double equil_rate_tol,
+              int pre_spinup_1d) {
+    return guarded(h, [&] {
+        if (!h_ice_lgm) throw vh_error("solid_earth_spinup: null ice field");
+        spinup(h, h_ice_lgm, equil_time_max_yr, equil_rate_tol, pre_spinup_1d != 0);
     });
 }
 

@@ -64,6 +64,7 @@ struct sle_solve_args {
 void sle_solve_launch(hipStream_t st, const sle_solve_args& a);
 int sle_solve_grid(const vh_sht& s);   // work items (= blocks) of the fused solve
 void grid_sub(hipStream_t st, int n, const double* a, const double* b, double* out);
+void grid_absdiff(hipStream_t st, int n, const double* a, const double* b, double* out);   // |a - b|
 void grid_ocean(hipStream_t st, int n, const double* topo, const double* ice, double* C);
 void grid_bsl_integrand(hipStream_t st, int n, const double* h, const double* C, const double* heq,
                         const double* C0, double* out);

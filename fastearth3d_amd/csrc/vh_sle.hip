@@ -95,6 +95,10 @@ __global__ void k_sub(int n, const double* __restrict__ a, const double* __restr
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = a[i] - b[i];
 }
+__global__ void k_absdiff(int n, const double* __restrict__ a, const double* __restrict__ b, double* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = fabs(a[i] - b[i]);
+}
 __global__ void k_ocean(int n, const double* __restrict__ topo, const double* __restrict__ ice, double* C) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) C[i] = ocean(topo[i], ice[i]);
@@ -1138,6 +1142,9 @@ void sle_cs(hipStream_t st, const vh_sht& s, const double* C, const double* f, d
 }
 void grid_sub(hipStream_t st, int n, const double* a, const double* b, double* out) {
     k_sub<<<nb256(n), 256, 0, st>>>(n, a, b, out);
+}
+void grid_absdiff(hipStream_t st, int n, const double* a, const double* b, double* out) {
+    k_absdiff<<<nb256(n), 256, 0, st>>>(n, a, b, out);
 }
 void grid_ocean(hipStream_t st, int n, const double* topo, const double* ice, double* C) {
     k_ocean<<<nb256(n), 256, 0, st>>>(n, topo, ice, C);

@@ -9,20 +9,23 @@
 !! library becomes `error stop <message>` as in the reference.
 !!
 !! Differences at this boundary: the optional `grid` argument of
-!! solid_earth_init (coords::grid_class remapping) is not supported, only the
-!! passthrough case (fields already on the model's Gauss grid); the lateral
-!! viscosity comes in as an array (solid_earth_enable_visc_3d_field /
-!! _nodes) instead of being read from netCDF; vilma_par_load reads the group
-!! "vilma" with the intrinsic NAMELIST facility (overlay of `filename` on
-!! `defaults_file`, dt_* in years converted to seconds).
+!! solid_earth_init takes this module's lon-lat look-alike of
+!! coords::grid_class (grid_init); the lateral viscosity of
+!! solid_earth_enable_visc_3d comes in as an array (a generic interface over
+!! the per-element perturbation (nphi, nlat, ne) and the nodal absolute
+!! log10 viscosity (nphi*nlat, nr)) instead of being read from netCDF;
+!! vilma_par_load reads the group "vilma" with the intrinsic NAMELIST facility
+!! (overlay of `filename` on `defaults_file`, dt_* and equil_time_max in
+!! years converted to seconds).
 module vilma
    use, intrinsic :: iso_c_binding
    implicit none
    private
 
-   public :: wp, MAX_LAYER, vilma_param_class, vilma_par_load
+   public :: wp, MAX_LAYER, vilma_param_class, vilma_par_load, vilma_par_print
    public :: solid_earth, solid_earth_init, solid_earth_update, solid_earth_finalize
-   public :: solid_earth_enable_visc_3d_field, solid_earth_enable_visc_3d_nodes
+   public :: solid_earth_enable_visc_3d, solid_earth_enable_visc_3d_field, solid_earth_enable_visc_3d_nodes
+   public :: solid_earth_spinup, solid_earth_sync_host
    public :: solid_earth_get_restart_state, solid_earth_set_restart_state
    public :: remap_ll_gauss, remap_init, remap_to_gauss, remap_to_ll, remap_free
    public :: grid_class, grid_init
@@ -139,8 +142,15 @@ module vilma
       ! at host resolution and the map pair, when the host grid is not the Gauss grid
       logical :: remap = .false.
       type(remap_ll_gauss) :: map
-      real(wp), allocatable :: z_bed_eq(:,:)
+      ! the reference state and the current ice at host resolution (src/vilma_coupling.f90:131-133)
+      real(wp), allocatable :: z_bed_eq(:,:), h_ice_eq(:,:), h_ice(:,:)
    end type solid_earth
+
+   !! solid_earth_enable_visc_3d (src/vilma_coupling.f90:320-334) with the
+   !! lateral field given in memory, per element or per node
+   interface solid_earth_enable_visc_3d
+      module procedure solid_earth_enable_visc_3d_field, solid_earth_enable_visc_3d_nodes
+   end interface solid_earth_enable_visc_3d
 
 
    interface
@@ -225,6 +235,12 @@ module vilma
       integer(c_int) function vh_remap_destroy(m) bind(c, name="vh_remap_destroy")
          import :: c_int, c_ptr
          type(c_ptr), value :: m
+      end function
+      integer(c_int) function vh_spinup(h, ice, tmax, tol, pre1d) bind(c, name="vh_spinup")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h, ice
+         real(c_double), value :: tmax, tol
+         integer(c_int), value :: pre1d
       end function
       integer(c_int) function vh_destroy(h) bind(c, name="vh_destroy")
          import :: c_int, c_ptr
@@ -417,13 +433,7 @@ contains
       call check(self, vh_get_field(self%h, 4, c_loc(self%gg%s_rot)), "solid_earth_update")
       call check(self, vh_get_field(self%h, 5, c_loc(self%gg%h_ice)), "solid_earth_update")
       call check(self, vh_get_spectrum(self%h, 0, c_loc(self%sigma_lm)), "solid_earth_update")
-      if (self%remap) then   ! solid_earth_sync_host (src/vilma_coupling.f90:439-449)
-         call remap_to_ll(self%map, self%gg%rsl, self%rsl)
-         self%z_bed = self%z_bed_eq - self%rsl
-      else
-         self%rsl = self%gg%rsl
-         self%z_bed = self%gg%z_bed
-      end if
+      call sync_host_fields(self)   ! solid_earth_sync_host (src/vilma_coupling.f90:439-451)
       call check(self, vh_get_scalar(self%h, 0, v), "solid_earth_update"); self%time = v
       call check(self, vh_get_scalar(self%h, 1, v), "solid_earth_update"); self%bsl = v
       call check(self, vh_get_scalar(self%h, 2, v), "solid_earth_update"); self%worst_mass_resid = v
@@ -486,6 +496,13 @@ contains
                self%gg%C(self%nphi, self%nlat), self%gg%C0(self%nphi, self%nlat), &
                self%gg%h_ice(self%nphi, self%nlat), self%gg%s_rot(self%nphi, self%nlat))
       call check(self, vh_get_field(self%h, 3, c_loc(self%gg%C0)), "solid_earth_init")
+      self%h_ice_eq = h_ice_eq
+      if (present(h_ice_init)) then
+         self%h_ice = h_ice_init
+      else
+         self%h_ice = h_ice_eq
+      end if
+      if (.not. self%remap) self%z_bed_eq = z_bed_eq
       call refresh(self)
    end subroutine solid_earth_init
 
@@ -505,8 +522,56 @@ contains
          hi = h_ice
       end if
       call check(self, vh_update(self%h, c_loc(hi), real(dt_yr, c_double)), "solid_earth_update")
+      self%h_ice = h_ice
       call refresh(self)
    end subroutine solid_earth_update
+
+   !! solid_earth_spinup (src/vilma_coupling.f90:453-517): relax to isostatic
+   !! equilibrium under the start-slice ice h_ice_lgm (host grid), holding the
+   !! reference state as the datum, as se%par asks (pre_spinup_1d,
+   !! equil_time_max, equil_rate_tol); a no-op when both are off.
+   subroutine solid_earth_spinup(self, h_ice_lgm)
+      type(solid_earth), intent(inout), target :: self
+      real(wp), intent(in) :: h_ice_lgm(:,:)
+      real(wp), allocatable, target :: hi(:,:)
+      integer(c_int) :: pre1d
+      if (.not. c_associated(self%h)) error stop 'solid_earth_spinup: model not initialised'
+      if (.not. self%par%pre_spinup_1d .and. self%par%equil_time_max <= 0.0_wp) return
+      if (self%remap) then
+         allocate(hi(self%nphi, self%nlat))
+         call map_in(self%map, h_ice_lgm, hi, conserve_mass=.true.)
+      else
+         if (size(h_ice_lgm, 1) /= self%nphi .or. size(h_ice_lgm, 2) /= self%nlat) &
+            error stop 'solid_earth_spinup: h_ice_lgm must be on the model grid (nphi, nlat)'
+         hi = h_ice_lgm
+      end if
+      pre1d = 0
+      if (self%par%pre_spinup_1d) pre1d = 1
+      call check(self, vh_spinup(self%h, c_loc(hi), real(self%par%equil_time_max / sec_per_year, c_double), &
+                 real(self%par%equil_rate_tol, c_double), pre1d), "solid_earth_spinup")
+      self%h_ice = h_ice_lgm
+      call refresh(self)
+   end subroutine solid_earth_spinup
+
+   !! solid_earth_sync_host (src/vilma_coupling.f90:439-451): the host-grid
+   !! rsl (bilinear from the Gauss grid, or a copy) and z_bed = z_bed_eq - rsl
+   !! from the model's current state
+   subroutine solid_earth_sync_host(self)
+      type(solid_earth), intent(inout), target :: self
+      if (.not. c_associated(self%h)) error stop 'solid_earth_sync_host: model not initialised'
+      call check(self, vh_get_field(self%h, 0, c_loc(self%gg%rsl)), "solid_earth_sync_host")
+      call sync_host_fields(self)
+   end subroutine solid_earth_sync_host
+
+   subroutine sync_host_fields(self)
+      type(solid_earth), intent(inout) :: self
+      if (self%remap) then
+         call remap_to_ll(self%map, self%gg%rsl, self%rsl)
+      else
+         self%rsl = self%gg%rsl
+      end if
+      self%z_bed = self%z_bed_eq - self%rsl
+   end subroutine sync_host_fields
 
    !! solid_earth_enable_visc_3d (src/vilma_coupling.f90:320-334) with the
    !! lateral field given in memory: log10-viscosity perturbation per element,
@@ -514,6 +579,9 @@ contains
    subroutine solid_earth_enable_visc_3d_field(self, pert_elem)
       type(solid_earth), intent(inout) :: self
       real(wp), intent(in), target, contiguous :: pert_elem(:,:,:)
+      if (size(pert_elem, 1) /= self%nphi .or. size(pert_elem, 2) /= self%nlat .or. &
+          size(pert_elem, 3) /= dim_of(self, 5)) &
+         error stop 'enable_lateral_visc: pert_elem must be (nphi,nlat,ne)'
       call check(self, vh_enable_visc_3d(self%h, c_loc(pert_elem)), "solid_earth_enable_visc_3d")
    end subroutine solid_earth_enable_visc_3d_field
 
@@ -522,6 +590,8 @@ contains
    subroutine solid_earth_enable_visc_3d_nodes(self, visc_node)
       type(solid_earth), intent(inout) :: self
       real(wp), intent(in), target, contiguous :: visc_node(:,:)
+      if (size(visc_node, 1) /= self%nphi * self%nlat .or. size(visc_node, 2) /= dim_of(self, 4)) &
+         error stop 'enable_lateral_visc_from_nodes: visc_node must be (nphi*nlat, nr)'
       call check(self, vh_enable_visc_3d_nodes(self%h, c_loc(visc_node), self%par%visc3d_lid_depth, &
                  self%par%visc3d_lid_log10max, self%par%visc_log10_max), "solid_earth_enable_visc_3d")
    end subroutine solid_earth_enable_visc_3d_nodes
@@ -539,6 +609,7 @@ contains
       complex(wp), intent(out) :: m
       real(wp), target :: mm(2)
       real(c_double) :: t
+      call check_restart_shapes(self, tau, rsl, h_ice, load_mem, tidal_mem)
       call check(self, vh_get_memory(self%h, c_loc(tau)), "solid_earth_get_restart_state")
       call check(self, vh_get_state(self%h, t, c_loc(rsl), c_loc(h_ice), c_loc(mm), c_loc(load_mem), &
                  c_loc(tidal_mem)), "solid_earth_get_restart_state")
@@ -553,11 +624,71 @@ contains
       complex(wp), intent(in) :: m
       real(wp), target :: mm(2)
       mm = [real(m, wp), aimag(m)]
+      call check_restart_shapes(self, tau, rsl, h_ice, load_mem, tidal_mem)
       call check(self, vh_set_memory(self%h, c_loc(tau)), "solid_earth_set_restart_state")
       call check(self, vh_set_state(self%h, real(time, c_double), c_loc(rsl), c_loc(h_ice), c_loc(mm), &
                  c_loc(load_mem), c_loc(tidal_mem)), "solid_earth_set_restart_state")
       call refresh(self)
    end subroutine solid_earth_set_restart_state
+
+   !! a dimension of the model (vh_get_dims selector: 4 nr, 5 ne, 6 nk, 8 nlam)
+   integer function dim_of(self, which) result(n)
+      type(solid_earth), intent(inout) :: self
+      integer, intent(in) :: which
+      integer(c_int) :: v
+      call check(self, vh_get_dims(self%h, int(which, c_int), v), "vilma")
+      n = v
+   end function dim_of
+
+   !! the restart arrays must have the reference's shapes: the library reads
+   !! and writes that many values through their addresses
+   subroutine check_restart_shapes(self, tau, rsl, h_ice, load_mem, tidal_mem)
+      type(solid_earth), intent(inout) :: self
+      real(wp), intent(in) :: tau(:,:,:,:), rsl(:,:), h_ice(:,:), load_mem(:,:,:), tidal_mem(:,:,:)
+      integer :: nlam, ne, nk
+      nlam = dim_of(self, 8);  ne = dim_of(self, 5);  nk = dim_of(self, 6)
+      if (size(tau, 1) /= nlam .or. size(tau, 2) /= ne .or. size(tau, 3) /= nk .or. size(tau, 4) /= 6) &
+         error stop 'restart state: tau must be (nlam, ne, nk, 6)'
+      if (size(rsl, 1) /= self%nphi .or. size(rsl, 2) /= self%nlat .or. &
+          size(h_ice, 1) /= self%nphi .or. size(h_ice, 2) /= self%nlat) &
+         error stop 'restart state: rsl and h_ice must be (nphi, nlat)'
+      if (size(load_mem) /= 4 * ne * 6 .or. size(tidal_mem) /= 4 * ne * 6) &
+         error stop 'restart state: rotation memory must be (4, ne, 6)'
+   end subroutine check_restart_shapes
+
+   !! vilma_par_print (src/vilma_params.f90:361): echo the record's active
+   !! settings to stdout, or to `unit`
+   subroutine vilma_par_print(p, unit)
+      type(vilma_param_class), intent(in) :: p
+      integer, intent(in), optional :: unit
+      integer :: u, k
+      u = 6
+      if (present(unit)) u = unit
+      write(u, '(a)') ' [vilma] configuration (MI355X library)'
+      write(u, '(2a)') '   solver:   ', trim(p%solver)
+      write(u, '(a,3(1x,i0))') '   lmax, nlat, nphi:', p%lmax, p%nlat, p%nphi
+      write(u, '(2a)') '   earth:    ', trim(p%earth)
+      if (trim(p%earth) == "custom") then
+         do k = 1, p%n_layer
+            write(u, '(a,i0,a,2es11.3,a,f8.1,a,2es11.3,a,i0,a,i0)') '     layer ', k, ' r ', p%r_bot(k), &
+               p%r_top(k), ' rho ', p%rho(k), ' mu, eta ', p%mu(k), p%eta(k), ' rheology ', p%rheology(k), &
+               ' elements ', p%n_elem_layer(k)
+         end do
+      end if
+      write(u, '(4a,i0)') '   response: ', trim(p%earth_response), '   scheme: ', trim(p%scheme), p%max_couple_iter
+      write(u, '(a,2(1x,i0),es10.2,2(1x,l1))') '   sle (n_outer, n_inner, tol, fixed_ocean, subgrid):', &
+         p%sle_n_outer, p%sle_n_inner, p%sle_tol, p%sle_fixed_ocean, p%sle_subgrid
+      write(u, '(a,4es11.3)') '   dt_init [s], rtol, atol, cfl:', p%dt_init, p%rtol, p%atol, p%cfl
+      write(u, '(a,l1,2a)') '   rotation: ', p%rotation, '   deg1_frame: ', trim(p%deg1_frame)
+      write(u, '(a,2es11.3,1x,l1)') '   spin-up (equil_time_max [s], equil_rate_tol, pre_spinup_1d):', &
+         p%equil_time_max, p%equil_rate_tol, p%pre_spinup_1d
+      if (p%l_visc_3d) then
+         write(u, '(a,es10.2,a,2f7.2,a,l1)') '   lateral viscosity: visc3d_tol ', p%visc3d_tol, &
+            '  clamp ', p%visc_log10_min, p%visc_log10_max, '  l_toroidal ', p%l_toroidal
+         if (p%visc3d_lid_depth > 0.0_wp) write(u, '(a,f7.1,a,f6.2)') '     lid rule: above ', &
+            p%visc3d_lid_depth * 1.0e-3_wp, ' km capped where log10(eta) > ', p%visc3d_lid_log10max
+      end if
+   end subroutine vilma_par_print
 
    !! nonzero status of a vh_remap_* call -> error stop with its message
    subroutine check_remap(rc, what)

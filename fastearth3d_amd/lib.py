@@ -60,6 +60,7 @@ def load():
         L.vh_remap_to_gauss_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         L.vh_remap_to_ll_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.vh_remap_destroy.argtypes = [C.c_void_p]
+        L.vh_spinup.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int]
         for name in exported_symbols():
             fn = getattr(L, name)
             if fn.argtypes is None:   # every entry point takes a handle or pointer: never call undeclared
@@ -155,11 +156,21 @@ class SolidEarth(ModelHandle):
         return dict(time=t.value, rsl=rsl, h_ice=hi, rot_m=m, rot_load_mem=lm, rot_tidal_mem=tm)
 
     def set_state(self, st):
-        arr = {k: np.ascontiguousarray(st[k], dtype=np.float64) for k in
-               ("rsl", "h_ice", "rot_m", "rot_load_mem", "rot_tidal_mem")}
+        ng, ne = self.nphi * self.nlat, self.dims["ne"]
+        n = dict(rsl=ng, h_ice=ng, rot_m=2, rot_load_mem=24 * ne, rot_tidal_mem=24 * ne)
+        arr = {k: self._sized(st[k], n[k], "set_state: %s" % k) for k in n}
         self._check(self._lib.vh_set_state(self._h, float(st["time"]), _cp(arr["rsl"]), _cp(arr["h_ice"]),
                                            _cp(arr["rot_m"]), _cp(arr["rot_load_mem"]), _cp(arr["rot_tidal_mem"])),
                     "set_state")
+
+    def spinup(self, h_ice_lgm, equil_time_max_yr, equil_rate_tol=1.0e-3, pre_spinup_1d=False):
+        """solid_earth_spinup (src/vilma_coupling.f90:453-517): relax under the
+        held start-slice ice h_ice_lgm (model grid), optionally first with the
+        lateral viscosity replaced by its lateral mean, then for at most
+        equil_time_max_yr years, until the bed velocity < equil_rate_tol m/yr."""
+        h = self._grid_in(h_ice_lgm)
+        self._check(self._lib.vh_spinup(self._h, dptr(h), float(equil_time_max_yr), float(equil_rate_tol),
+                                        int(bool(pre_spinup_1d))), "solid_earth_spinup")
 
     def remap(self, lon_deg, lat_deg):
         """Map pair between a host lon-lat grid (cell-centre axes in degrees,

@@ -64,13 +64,23 @@ class ModelHandle:
         h = self._grid_in(h_ice)
         self._check(self._fn("update")(self._h, dptr(h), float(dt_yr)), "solid_earth_update")
 
+    def _sized(self, a, n, what):
+        """a as a C-contiguous float64 array of exactly n values (the library
+        reads n values through the pointer)."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.size != n:
+            raise VilmaError("%s: array has %d values, expected %d" % (what, a.size, n))
+        return a
+
     def enable_visc_3d(self, pert_elem):
-        p = np.ascontiguousarray(pert_elem, dtype=np.float64)
+        p = self._sized(pert_elem, self.dims["ne"] * self.nphi * self.nlat,
+                        "enable_lateral_visc: pert_elem must be (ne, nlat, nphi)")
         self._check(self._fn("enable_visc_3d")(self._h, dptr(p)), "enable_lateral_visc")
         self.dims["ne3d"] = self._dim(_abi.D_NE3D)
 
     def enable_visc_3d_nodes(self, visc_node, lid_depth=0.0, lid_log10max=22.0, log10_cap=30.0):
-        v = np.ascontiguousarray(visc_node, dtype=np.float64)
+        v = self._sized(visc_node, self.dims["nr"] * self.nphi * self.nlat,
+                        "enable_lateral_visc_from_nodes: visc_node must be (nr, nlat, nphi)")
         self._check(self._fn("enable_visc_3d_nodes")(self._h, dptr(v), lid_depth, lid_log10max, log10_cap),
                     "enable_lateral_visc_from_nodes")
         self.dims["ne3d"] = self._dim(_abi.D_NE3D)
@@ -97,7 +107,8 @@ class ModelHandle:
         return out
 
     def set_memory(self, tau):
-        t = np.ascontiguousarray(tau, dtype=np.float64)
+        d = self.dims
+        t = self._sized(tau, 6 * d["nk"] * d["ne"] * d["nlam"], "set_memory: tau must be (6, nk, ne, nlam)")
         self._check(self._fn("set_memory")(self._h, dptr(t)), "set_memory")
 
     # ------------------------------------------------------- response probes

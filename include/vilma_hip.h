@@ -131,6 +131,21 @@ int vh_update(vh_model* h, const double* h_ice, double dt_yr);
  * counterpart in the reference, whose fields always live on the host. */
 int vh_update_device(vh_model* h, const double* h_ice_device, double dt_yr);
 
+/* solid_earth_spinup (src/vilma_coupling.f90:453-517, relax_hold :519-571):
+ * bring the model to isostatic equilibrium under h_ice_lgm ((nphi, nlat), the
+ * Gauss grid) holding the reference state as the datum: a zero-length call
+ * seeds the entering ice; with pre_spinup_1d the model first relaxes with
+ * its lateral viscosity (the last vh_enable_visc_3d* field) replaced by the
+ * lateral mean, then the real field is restored; then the full model relaxes
+ * for at most equil_time_max_yr years (<= 0: skipped). Each relaxation runs
+ * coupling calls of growing length (200 yr, x 1.6, at most 60) until the mean
+ * bed velocity falls below equil_rate_tol [m/yr], logging one line per call
+ * to stdout as the reference does. The reference reads equil_time_max,
+ * equil_rate_tol and pre_spinup_1d from the parameter record; here they are
+ * arguments (the Fortran shim passes se%par's). */
+int vh_spinup(vh_model* h, const double* h_ice_lgm, double equil_time_max_yr, double equil_rate_tol,
+              int pre_spinup_1d);
+
 /* Outputs the host reads from the solid_earth components (:71-140). */
 int vh_get_field(vh_model* h, int field, double* out);           /* (nphi,nlat) */
 int vh_get_scalar(vh_model* h, int which, double* out);

@@ -1,27 +1,31 @@
 !! Host-side driver of the Fortran module `vilma` for the tests:
 !!   drive <namelist> --print-params [defaults]  (CPU only: vilma_par_load result)
-!!   drive <namelist> <input.bin> <output.bin> [--host-grid]  (GPU: init + update loop)
+!!   drive <namelist> <input.bin> <output.bin> [--host-grid] [--spinup]  (GPU: init + update loop)
 !! input.bin (stream): int32 nphi, nlat, nsteps; real64 dt_yr; z_bed_eq(nphi,nlat);
 !! h_ice(nphi,nlat,nsteps). output.bin: rsl, z_bed, C (nphi,nlat), bsl, time.
 !! With --host-grid the fields are on a lon-lat host grid: nphi, nlat are its
 !! dimensions, and its axes lon(nphi), lat(nlat) [degrees] follow dt_yr; rsl
-!! and z_bed come back on it (C stays on the Gauss grid).
+!! and z_bed come back on it (C stays on the Gauss grid). With --spinup,
+!! solid_earth_spinup under the first ice field precedes the loop (the
+!! namelist's equil_time_max, equil_rate_tol, pre_spinup_1d).
 program drive
    use vilma
    implicit none
    type(solid_earth) :: se
    type(grid_class) :: grid
-   character(len=512) :: nml, a2, a3, a4
+   character(len=512) :: nml, a2, a3, a4, a5
    integer :: nphi, nlat, nsteps, k, u, i
    real(wp) :: dt
    real(wp), allocatable :: z(:,:), hice(:,:,:), heq(:,:), lon(:), lat(:)
-   logical :: host
+   logical :: host, spin
 
    call get_command_argument(1, nml)
    call get_command_argument(2, a2)
    call get_command_argument(3, a3)
    call get_command_argument(4, a4)
-   host = trim(a4) == "--host-grid"
+   call get_command_argument(5, a5)
+   host = trim(a4) == "--host-grid" .or. trim(a5) == "--host-grid"
+   spin = trim(a4) == "--spinup" .or. trim(a5) == "--spinup"
    if (trim(a2) == "--print-params" .and. len_trim(a3) > 0) then
       call vilma_par_load(se%par, trim(nml), defaults_file=trim(a3))
    else
@@ -59,6 +63,7 @@ program drive
    else
       call solid_earth_init(se, z, heq)
    end if
+   if (spin) call solid_earth_spinup(se, hice(:, :, 1))
    do k = 1, nsteps
       call solid_earth_update(se, hice(:, :, k), dt)
    end do

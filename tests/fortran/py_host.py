@@ -1,7 +1,8 @@
 """The Python host of the Fortran driver's workload, run as its own process
 (tests/test_fortran.py): same namelist, same input.bin / output.bin formats as
 tests/fortran/drive.f90, through the ctypes front end of the same library.
-Usage: python py_host.py <namelist> <input.bin> <output.bin> [--host-grid]"""
+Usage: python py_host.py <namelist> <input.bin> <output.bin> [--host-grid]
+       [--spinup equil_time_max_yr equil_rate_tol pre_spinup_1d]"""
 import os
 import sys
 
@@ -13,7 +14,11 @@ from fastearth3d_amd import lib, namelist  # noqa: E402
 
 def main():
     nml, fin, fout = sys.argv[1:4]
-    host = len(sys.argv) > 4 and sys.argv[4] == "--host-grid"
+    host = "--host-grid" in sys.argv[4:]
+    spin = None
+    if "--spinup" in sys.argv[4:]:
+        k = sys.argv.index("--spinup")
+        spin = (float(sys.argv[k + 1]), float(sys.argv[k + 2]), int(sys.argv[k + 3]))
     q = namelist.vilma_par_load(nml, defaults_file=None)
     with open(fin, "rb") as f:
         nx, ny, nsteps = np.fromfile(f, np.int32, 3)
@@ -32,6 +37,8 @@ def main():
         g.nphi, g.nlat = nphi, nlat
         rm = lib.Remap(g, lon, lat)
         se = lib.SolidEarth(q, rm.to_gauss(z), rm.to_gauss(np.zeros_like(z), conserve_mass=True))
+        if spin:
+            se.spinup(rm.to_gauss(ice[0], conserve_mass=True), *spin)
         for k in range(nsteps):
             se.update(rm.to_gauss(ice[k], conserve_mass=True), dt)
         rsl = rm.to_ll(se.rsl)
@@ -39,6 +46,8 @@ def main():
         rm.close()
     else:
         se = lib.SolidEarth(q, z, np.zeros_like(z))
+        if spin:
+            se.spinup(ice[0], *spin)
         for k in range(nsteps):
             se.update(ice[k], dt)
         rsl, zb = se.rsl, se.z_bed

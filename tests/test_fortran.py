@@ -88,7 +88,7 @@ def test_fortran_rejects_bad_frame(tmp_path):
     assert "deg1_frame" in res.stdout + res.stderr
 
 
-def _namelist(tmp_path, p):
+def _namelist(tmp_path, p, extra=()):
     nml = tmp_path / "c.nml"
     lines = ["&vilma", "  lmax = %d" % p.lmax, "  earth = \"custom\"", "  n_layer = %d" % p.n_layer,
              "  r_earth = %r, r_core = %r" % (p.r_earth, p.r_core)]
@@ -96,20 +96,18 @@ def _namelist(tmp_path, p):
         lines.append("  %s = %s" % (key, ", ".join(repr(getattr(p, key)[i]) for i in range(p.n_layer))))
     for key in ("rheology", "n_elem_layer"):
         lines.append("  %s = %s" % (key, ", ".join(str(getattr(p, key)[i]) for i in range(p.n_layer))))
-    lines += ["  rotation = .false.", "  deg1_frame = \"%s\"" % p.deg1_frame.decode(), "/"]
+    lines += ["  rotation = .false.", "  deg1_frame = \"%s\"" % p.deg1_frame.decode()] + list(extra) + ["/"]
     nml.write_text("\n".join(lines) + "\n")
     return nml
 
 
-def _run_both(tmp_path, nml, fin, extra=()):
-    """The Fortran driver and the Python host (tests/fortran/py_host.py), each in
-    a process of its own: a process that has run the cooperative sea-level
-    kernel keeps the GPU's cooperative-launch resource, so the test process
-    itself stays off the GPU. Returns both output files' bytes."""
+def _run_both(tmp_path, nml, fin, extra=(), py_extra=()):
+    """The Fortran driver and the Python host (tests/fortran/py_host.py), each
+    in a process of its own. Returns both output files' bytes."""
     ff, fp = tmp_path / "out_f.bin", tmp_path / "out_p.bin"
     subprocess.run([_driver(), str(nml), str(fin), str(ff)] + list(extra), check=True, timeout=300)
     subprocess.run([sys.executable, os.path.join(HERE, "fortran", "py_host.py"), str(nml), str(fin), str(fp)]
-                   + list(extra), check=True, timeout=300)
+                   + list(extra) + list(py_extra), check=True, timeout=300)
     return np.fromfile(ff, np.float64), np.fromfile(fp, np.float64)
 
 
@@ -167,3 +165,26 @@ def test_fortran_host_grid_coupling(tmp_path):
     rsl = of[:nh].reshape(lat.size, lon.size)
     assert np.abs(rsl).max() > 0.0
     assert np.array_equal(of[nh:2 * nh].reshape(lat.size, lon.size), z_ll - rsl)
+
+
+@pytest.mark.gpu
+def test_fortran_spinup_matches_python_host(tmp_path):
+    """solid_earth_spinup through the Fortran module (the namelist's
+    equil_time_max = 3000 yr, equil_rate_tol) then three coupling calls equal
+    the Python host calling vh_spinup with the same values, bit for bit."""
+    c = configs.config(1, lmax=16)
+    p = c["params"]
+    p.rotation = 0
+    nphi, nlat = p.grid()
+    nml = _namelist(tmp_path, p, ["  equil_time_max = 3000.0, equil_rate_tol = 1.0e-4, pre_spinup_1d = .false."])
+    ice = np.stack([c["ice"](k + 60) for k in range(3)])
+    fin = tmp_path / "in.bin"
+    with open(fin, "wb") as f:
+        np.array([nphi, nlat, 3], np.int32).tofile(f)
+        np.array([c["dt_yr"]], np.float64).tofile(f)
+        c["z_bed_eq"].astype(np.float64).tofile(f)
+        ice.astype(np.float64).tofile(f)
+    of, op = _run_both(tmp_path, nml, fin, ["--spinup"], ["3000.0", "1.0e-4", "0"])
+    assert of.size == op.size == 3 * nphi * nlat + 2
+    assert np.array_equal(of, op)
+    assert of[-1] > 3000.0          # the spin-up advanced the clock before the three calls

@@ -109,3 +109,22 @@ def test_workload_generators():
     assert np.all(pert[depth < 80e3] == 0.0)
     top = np.argmin(np.where(depth > 80e3, depth, np.inf))
     assert pert[top].std() == pytest.approx(0.5, rel=1e-6)
+
+
+def test_wrong_array_shapes_are_refused(oracle):
+    """Arrays handed to entry points that read a fixed number of values through
+    a pointer are size-checked before the call (the lateral field per element
+    and per node, the restart memory and state)."""
+    from fastearth3d_amd import configs
+    from fastearth3d_amd.model import VilmaError
+    c = configs.config(2, lmax=8)
+    m = oracle.OracleModel(c["params"], c["z_bed_eq"], c["h_ice_eq"])
+    ne, nr = m.dims["ne"], m.dims["nr"]
+    with pytest.raises(VilmaError, match="pert_elem"):
+        m.enable_visc_3d(np.zeros((ne - 1, m.nlat, m.nphi)))
+    with pytest.raises(VilmaError, match="visc_node"):
+        m.enable_visc_3d_nodes(np.zeros((ne, m.nlat, m.nphi)))
+    with pytest.raises(VilmaError, match="tau"):
+        m.set_memory(np.zeros(m.memory().size - 4))
+    with pytest.raises(VilmaError, match="grid field"):
+        m.update(np.zeros(m.nlat * m.nphi + 1), 10.0)
