@@ -10,10 +10,14 @@ region and handed over as device pointers (vh_update_device). W untimed warm-up
 calls, then K timed calls bracketed by device synchronisations. The JSON line
 adds the roofline of the dominant kernel (algorithmic bytes — or, for the
 tensor Legendre kernels, FP64 flops — per launch over its HIP-event-timed
-average duration) and the oracle's CPU throughput on a bounded sample of the
-same workload.
+average duration), the oracle's CPU throughput on a bounded sample of the
+same workload, and the metric's two other parts: ens/s of the 64-member
+uneven ensemble (with its ratio to stepping the members one after another)
+and the adjoint / forward time ratio (--no-extras leaves them out;
+--config 4 and --config 3 measure them alone, in full).
 """
 import argparse
+import copy
 import ctypes as C
 import json
 import os
@@ -114,58 +118,121 @@ def cpu_baseline(cfg, lmax, calls):
     return calls * c["dt_yr"] / 1000.0 / el, el, nsub
 
 
-def ensemble_bench(a, L):
-    """Configuration 4: an uneven ensemble (lmax 64/96/128/192, 32-64
-    elements, 16 of 64 members 3-D, sea level + rotation, dt 100 yr). Each
-    member is its own model, stepped one after another on one host thread
-    (the fused sea-level kernel of a member takes the whole device), every
-    member's ice history resident in HBM. value = timed member calls per
-    second / 200 (completed 200-call members per second)."""
+def ensemble_models(n, calls, mem=None):
+    """The uneven ensemble's members (configs.ensemble_members): per member
+    its workload and its ice history for `calls` coupling calls, resident in
+    HBM ([calls][nlat][nphi], the domes times the history factor of each
+    call, formed on the device: the same products as the host's)."""
     import torch
-    from fastearth3d_amd import configs, lib
-    n = a.members
-    members = []
+    from fastearth3d_amd import configs
+    mem = mem or configs.ensemble_members(n)
+    out = []
     for i in range(n):
-        c = configs.ensemble_member(i, n)
-        se = lib.SolidEarth(c["params"], c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
+        c = configs.ensemble_member(i, n, members=mem)
+        fac = [float(configs.ice_history_factor((k + 1) * c["dt_yr"] / 1000.0)) for k in range(calls)]
+        d = torch.from_numpy(c["domes"]).to("cuda:0")
+        hist = d.unsqueeze(0) * torch.tensor(fac, dtype=torch.float64, device="cuda:0").view(-1, 1, 1)
+        out.append((c, hist.contiguous()))
+    torch.cuda.synchronize()
+    return mem, out
+
+
+def _ens_models(lib, ws, sle_blocks=0):
+    from fastearth3d_amd import _abi
+    ms = []
+    for c, _ in ws:
+        m = lib.SolidEarth(c["params"], c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
         if c["pert"] is not None:
-            se.enable_visc_3d(c["pert"])
-        ice = torch.from_numpy(np.stack([c["ice"](k) for k in range(a.warmup + a.steps)])).to("cuda:0")
-        members.append((se, ice, c))
-    torch.cuda.synchronize()
+            m.enable_visc_3d(c["pert"])
+        m.set_tuning(_abi.T_SLE_BLOCKS, sle_blocks)
+        ms.append(m)
+    return ms
 
-    def run(i, k0, k1):
-        se, ice, c = members[i]
-        for k in range(k0, k1):
-            if L.vh_update_device(se._h, C.c_void_p(ice[k].data_ptr()), c["dt_yr"]):
-                raise RuntimeError(se._err())
-        return se.scalar(3)
 
-    for i in range(n):
-        run(i, 0, a.warmup)
+def ensemble_measure(L, ws, warmup, calls, workers, tune_calls=2):
+    """The same calls from the same state two ways: every member stepped
+    alone, one after another (vh_update_device, one block per work item in
+    the sea-level solve), and all members through vh_ensemble_run_device
+    after the ensemble's autotuner (vh_ensemble_tune, 3 x tune_calls calls)
+    has picked the solve's launch shape. Warm-up and tuning calls are run
+    untimed by both. Returns (sequential s, ensemble s, sub-steps timed,
+    per-member timed seconds of the sequential pass, same results, chosen
+    VH_T_SLE_BLOCKS)."""
+    import torch
+    from fastearth3d_amd import lib
+    pre = warmup + 3 * tune_calls
+    seq = _ens_models(lib, ws)
+    for m, (c, h) in zip(seq, ws):
+        for k in range(pre):
+            m.update_device(h[k].data_ptr(), c["dt_yr"])
     torch.cuda.synchronize()
-    sub0 = sum(m[0].scalar(3) for m in members)
+    n0 = sum(m.scalar(3) for m in seq)
+    per = []
     t0 = time.perf_counter()
-    for i in range(n):
-        run(i, a.warmup, a.warmup + a.steps)
+    for m, (c, h) in zip(seq, ws):
+        t1 = time.perf_counter()
+        for k in range(pre, pre + calls):
+            m.update_device(h[k].data_ptr(), c["dt_yr"])
+        per.append(time.perf_counter() - t1)
     torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    nsub = sum(m[0].scalar(3) for m in members) - sub0
+    t_seq = time.perf_counter() - t0
+    nsub = sum(m.scalar(3) for m in seq) - n0
+    ref = [(m.rsl, m.scalar(3)) for m in seq]
+    for m in seq:
+        m.finalize()
+    par = _ens_models(lib, ws)
+    ens = lib.Ensemble(par, n_workers=workers)
+    dts = [c["dt_yr"] for c, _ in ws]
+    if warmup:
+        ens.run_device(warmup, [h.data_ptr() for _, h in ws], dts)
+    chosen = ens.tune(tune_calls, [h[warmup].data_ptr() for _, h in ws], dts) if tune_calls > 0 else 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.run_device(calls, [h[pre].data_ptr() for _, h in ws], dts)
+    torch.cuda.synchronize()
+    t_ens = time.perf_counter() - t0
+    same = all(np.array_equal(m.rsl, r) and m.scalar(3) == s for m, (r, s) in zip(par, ref))
+    ens.close()
+    for m in par:
+        m.finalize()
+    return t_seq, t_ens, int(nsub), per, same, chosen
+
+
+def ensemble_bench(a, L, emit=True):
+    """The uneven ensemble: 64 members of their own earths (lithosphere
+    60-120 km, upper / lower mantle 1e20-1e22 Pa s, 32-64 elements, lmax
+    64/96/128/192, 16 with lateral viscosity), sea level + rotation, dt
+    100 yr, each with its own topography and ice history (resident in HBM).
+    value = completed 200-call members per second through
+    vh_ensemble_run_device over --steps calls of every member (200: the whole
+    workload); the same calls of the members stepped one after another give
+    the sequential rate beside it."""
+    n = a.members
+    mem, ws = ensemble_models(n, a.warmup + 3 * a.tune_calls + a.steps)
+    t_seq, t_ens, nsub, per, same, chosen = ensemble_measure(L, ws, a.warmup, a.steps, a.ens_workers, a.tune_calls)
     calls = n * a.steps
-    kyr = sum(m[2]["dt_yr"] for m in members) * a.steps / 1000.0
-    mem = configs.ensemble_members(n)
+    kyr = sum(c["dt_yr"] for c, _ in ws) * a.steps / 1000.0
+    crit = max(per)
     out = {
-        "metric": "ens/s (64-member uneven ensemble, 1 GPU)", "value": round(calls / 200.0 / el, 4),
+        "metric": "ens/s (64-member uneven ensemble, 1 GPU)", "value": round(calls / 200.0 / t_ens, 4),
         "unit": "ens/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": round(el / a.steps * 1000.0, 3), "higher_is_better": True, "scaling": "weak",
+        "ms_per_step": round(t_ens / a.steps * 1000.0, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "c4 ensemble: %d members, lmax 64/96/128/192, 32-64 elements, %d 3-D, "
-                               "SLE + rotation, dt 100 yr" % (n, sum(x[2] for x in mem)),
-                   "members": n, "calls_per_member_timed": a.steps,
-                   "sub_steps_timed": int(nsub)},
-        "member_kyr_per_s": round(kyr / el, 4), "member_calls_per_s": round(calls / el, 2),
+        "config": {"workload": "c4 uneven ensemble: %d members, lmax 64/96/128/192, 32-64 elements, %d 3-D, "
+                               "own earths, topography and ice, SLE + rotation, dt 100 yr" % (n, sum(x[2] for x in mem)),
+                   "members": n, "calls_per_member_timed": a.steps, "sub_steps_timed": nsub,
+                   "workers": a.ens_workers, "tuned_sle_blocks": chosen,
+                   "tuning_calls_untimed": 3 * a.tune_calls},
+        "sequential_ens_per_s": round(calls / 200.0 / t_seq, 4),
+        "speedup_vs_sequential": round(t_seq / t_ens, 3),
+        # no schedule of independent members can beat the slowest member run alone
+        "bound_by_slowest_member": round(t_seq / crit, 3),
+        "ensemble_equals_sequential": same,
+        "member_kyr_per_s": round(kyr / t_ens, 4), "member_calls_per_s": round(calls / t_ens, 2),
     }
-    print(json.dumps(out))
+    if emit:
+        print(json.dumps(out))
+    return out
 
 
 def site_matrix(L, colat, lon):
@@ -196,7 +263,7 @@ def site_matrix(L, colat, lon):
     return np.stack(cols, axis=1)
 
 
-def adjoint_bench(a, L):
+def adjoint_bench(a, L, emit=True):
     """Configuration 3: reverse mode over 100 coupling calls of the lmax 128
     3-D workload (sea level with migrating coastlines and rotation, as the
     forward benchmark).
@@ -204,8 +271,10 @@ def adjoint_bench(a, L):
     output calls, target from a forward run with another seeded delta field;
     gradients w.r.t. the lateral log-viscosity field (and delta's SH
     coefficients l <= 32 per element, by the transposed synthesis) and the
-    ice history. value = wall(forward_record + backward) / wall(plain
-    forward) over the same calls from the same state (vh_forward_rewind);
+    ice history. value = wall(forward_record + misfit gradient + backward)
+    / wall(plain forward) over the same calls from the same state
+    (vh_forward_rewind); the misfit and dJ/drsl are two FP64 GEMMs on the
+    device against the (500 x grid) site-evaluation matrix;
     memory checkpointed every 10 calls. Then central differences (h = 1e-4)
     along --fd-dirs seeded delta directions (l <= 32, elements away from the
     3-D threshold) and one ice direction, on fresh models."""
@@ -238,6 +307,17 @@ def adjoint_bench(a, L):
     def site_values(se, rsl_grid):
         return (E @ se.sht_analysis(rsl_grid)).real
 
+    # the site evaluation as one real (nsite, ng) matrix S on the device:
+    # v = Re(E analysis(rsl)) = S rsl, row s = the transpose of the evaluation
+    # at site s applied to e_s, W synthesis(conj(E_s) / c_m)
+    ev = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
+    S_np = np.stack([W * ev.sht_synthesis(np.conj(E[k]) / cm) for k in range(nsite)]).reshape(nsite, ng)
+    probe = configs.random_sh_field(colat, lon, 12, -2.0, 5)
+    assert np.abs(S_np @ probe.ravel() - site_values(ev, probe)).max() <= 1e-10 * np.abs(probe).max()
+    ev.finalize()
+    S_d = torch.from_numpy(S_np).to("cuda:0")
+    del S_np
+
     # target: another seeded delta field, same ice
     tgt_pert = configs.delta_visc(p, colat, lon, 1037)
     st = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
@@ -248,6 +328,7 @@ def adjoint_bench(a, L):
         if k in outs:
             target[k] = site_values(st, st.rsl)
     st.finalize()
+    T_d = torch.from_numpy(np.stack([target[k] for k in outs])).to("cuda:0")
 
     se = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
     se.enable_visc_3d(c["pert"])
@@ -257,16 +338,11 @@ def adjoint_bench(a, L):
     gi_d = torch.empty((ncalls + 1, nlat, nphi), dtype=torch.float64, device="cuda:0")
 
     def misfit_grad(rsl_out):
-        J = 0.0
+        """J and dJ/drsl (into dJ_d) on the device: two FP64 GEMMs with S."""
         dJ_d.zero_()
-        host = rsl_out[outs].cpu().numpy()
-        for i, k in enumerate(outs):
-            r = site_values(se, host[i]) - target[k]
-            J += float(np.sum(r * r))
-            ah = (2.0 * r) @ np.conj(E)                 # dJ/da (stored re/im parts)
-            g = W * se.sht_synthesis(ah / cm)          # Anl^T: dJ/drsl on the grid
-            dJ_d[k] = torch.from_numpy(g).to("cuda:0")
-        return J
+        R = rsl_out[outs].reshape(nout, ng) @ S_d.T - T_d   # site residuals [nout, nsite]
+        dJ_d[outs] = ((2.0 * R) @ S_d).reshape(nout, nlat, nphi)
+        return float((R * R).sum())
 
     def record():
         check(se, L.vh_forward_record_device(se._h, ncalls, C.c_void_p(ice.data_ptr()), c["dt_yr"], every,
@@ -399,7 +475,7 @@ def adjoint_bench(a, L):
     helper.finalize()
 
     kernel_ms = {ADJ_NAMES.get(k, NAMES.get(k)): round(ms[k], 3) for k in range(NCLASS) if cnt[k] > 0}
-    ratio = (t_rec + t_bwd) / t_fwd
+    ratio = (t_rec + t_mis + t_bwd) / t_fwd
     # roofline of the reverse sweep's dominant kernel, k_t3_rows_adj: per
     # launch it reads the adjoint and forward row spectra of its batches,
     # writes the adjoint rows back, reads the rate rows and updates the rate
@@ -429,13 +505,15 @@ def adjoint_bench(a, L):
                                % (a.lmax, ncalls, c["dt_yr"], every, nsite, nout),
                    "lmax": a.lmax, "nlat": nlat, "nphi": nphi, "ne3d": int(n3), "sub_steps": int(nsub)},
         "forward_s": round(t_fwd, 4), "record_s": round(t_rec, 4), "backward_s": round(t_bwd, 4),
-        "misfit_gradient_s": round(t_mis, 4), "ratio_incl_misfit": round((t_rec + t_bwd + t_mis) / t_fwd, 4),
+        "misfit_gradient_s": round(t_mis, 4), "ratio_excl_misfit": round((t_rec + t_bwd) / t_fwd, 4),
         "J": J, "grad_delta_lm_norm": gdelta_norm ** 0.5,
         "grad_check": {"h": h, "branch": "coastline held (sle_fixed_ocean), skip gate off, sle_tol 1e-11", "directions": checks,
                        "max_rel_err": max((x["rel_err"] for x in checks), default=None)},
         "kernel_ms_reverse_run": kernel_ms, "roofline": roof, "cpu_baseline": None,
     }
-    print(json.dumps(out))
+    if emit:
+        print(json.dumps(out))
+    return out
 
 
 def main():
@@ -448,8 +526,14 @@ def main():
     ap.add_argument("--cpu-calls", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--members", type=int, default=64, help="ensemble size (--config 4)")
+    ap.add_argument("--ens-workers", type=int, default=16, help="host threads of the ensemble (--config 4)")
+    ap.add_argument("--tune-calls", type=int, default=2, help="calls per autotuner candidate (--config 4)")
+    ap.add_argument("--ens-calls", type=int, default=10,
+                    help="timed calls per member of the ensemble part of the default line")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default line without the ensemble and adjoint parts")
     ap.add_argument("--adj-calls", type=int, default=100, help="coupling calls of the adjoint window (--config 3)")
-    ap.add_argument("--fd-dirs", type=int, default=2, help="finite-difference check directions (--config 3)")
+    ap.add_argument("--fd-dirs", type=int, default=20, help="finite-difference check directions (--config 3)")
     ap.add_argument("--fd-seed", type=int, default=5)
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 times): the reference "
@@ -583,8 +667,33 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 2)
-    print(json.dumps(out))
     se.finalize()
+    if not a.no_extras:
+        # the metric's other two parts, measured in the same run: the uneven
+        # ensemble (--ens-calls calls of each of the 64 members, after one
+        # warm-up call and the autotuner's calls) and the adjoint / forward
+        # time ratio over 100 calls (gradient check left to --config 3)
+        ea = copy.copy(a)
+        ea.steps, ea.warmup = a.ens_calls, 1
+        e = ensemble_bench(ea, L, emit=False)
+        out["ens_per_s"] = e["value"]
+        out["ens_sequential_per_s"] = e["sequential_ens_per_s"]
+        out["ens_speedup_vs_sequential"] = e["speedup_vs_sequential"]
+        out["ens_detail"] = {"members": e["config"]["members"], "calls_per_member_timed": a.ens_calls,
+                             "sub_steps_timed": e["config"]["sub_steps_timed"],
+                             "tuned_sle_blocks": e["config"]["tuned_sle_blocks"],
+                             "bound_by_slowest_member": e["bound_by_slowest_member"],
+                             "ensemble_equals_sequential": e["ensemble_equals_sequential"],
+                             "basis": "ens/s = members x timed calls / 200 / wall s (completed 200-call members per "
+                                      "second at the measured rate)"}
+        aa = copy.copy(a)
+        aa.fd_dirs = 0
+        r = adjoint_bench(aa, L, emit=False)
+        out["adjoint_forward_ratio"] = r["value"]
+        out["adjoint_detail"] = {"calls": r["steps"], "forward_s": r["forward_s"], "record_s": r["record_s"],
+                                 "misfit_gradient_s": r["misfit_gradient_s"], "backward_s": r["backward_s"],
+                                 "ratio_excl_misfit": r["ratio_excl_misfit"]}
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

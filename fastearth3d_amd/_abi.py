@@ -53,6 +53,7 @@ F_RSL, F_ZBED, F_C, F_C0, F_SROT, F_HICE = range(6)
  S_N_INNER_TOT, S_N_OUTER_TOT, S_POLAR_M_RE, S_POLAR_M_IM, S_N_SLE_STEAL) = range(11)
 SP_SIGMA, SP_U, SP_N = range(3)
 D_LMAX, D_NLAT, D_NPHI, D_NLM, D_NR, D_NE, D_NK, D_NE3D, D_NLAM, D_NEM, D_NMN = range(11)
+T_SLE_BLOCKS = 0
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

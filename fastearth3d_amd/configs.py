@@ -105,10 +105,43 @@ def random_sh_field(colat, lon, lmax, slope, seed):
     return out
 
 
-def ocean_topography(colat, lon, lmax, seed, ocean_frac=0.70, amp=5000.0):
+def random_sh_field_fast(colat, lon, lmax, slope, seed):
+    """The field of random_sh_field (same coefficients, same draw order),
+    evaluated per order m as a Legendre column sum over the latitudes and
+    then one matrix product over the longitudes; O(lmax^2 nlat + lmax nlat
+    nphi) instead of O(lmax^2 nlat nphi)."""
+    rng = np.random.default_rng(seed)
+    x = np.cos(colat)
+    st = np.sin(colat)
+    ac = np.zeros((colat.size, lmax + 1))
+    asn = np.zeros((colat.size, lmax + 1))
+    for m in range(0, lmax + 1):
+        pmm = np.full_like(x, 1.0 / np.sqrt(4 * np.pi))
+        for k in range(1, m + 1):
+            pmm = pmm * np.sqrt((2 * k + 1) / (2 * k)) * st
+        p_prev, p = None, pmm
+        for l in range(m, lmax + 1):
+            if l == m + 1:
+                p_prev, p = p, np.sqrt(2 * m + 3) * x * pmm
+            elif l > m + 1:
+                a = np.sqrt((4 * l * l - 1) / (l * l - m * m))
+                b = np.sqrt(((l - 1) ** 2 - m * m) / (4 * (l - 1) ** 2 - 1))
+                p_prev, p = p, a * (x * p - b * p_prev)
+            if l == 0:
+                continue
+            amp = np.sqrt(float(l) ** slope / (2 * l + 1))
+            ca, sa = rng.standard_normal(2) * amp
+            w = 1.0 if m == 0 else np.sqrt(2.0)
+            ac[:, m] += w * ca * p
+            asn[:, m] += w * sa * p
+    mm = np.arange(lmax + 1)[:, None] * lon[None, :]
+    return ac @ np.cos(mm) + asn @ np.sin(mm)
+
+
+def ocean_topography(colat, lon, lmax, seed, ocean_frac=0.70, amp=5000.0, fast=False):
     """Random topography (power ~ l^-2 up to lmax) offset so `ocean_frac` of
     the grid is below sea level and scaled to +-amp metres."""
-    f = random_sh_field(colat, lon, lmax, -2.0, seed)
+    f = (random_sh_field_fast if fast else random_sh_field)(colat, lon, lmax, -2.0, seed)
     f = f - np.quantile(f, ocean_frac)
     return f * (amp / np.abs(f).max())
 
@@ -147,12 +180,12 @@ def element_midpoint_depths(params):
     return np.array(depths)
 
 
-def delta_visc(params, colat, lon, seed, lmax_delta=32, std=0.5):
+def delta_visc(params, colat, lon, seed, lmax_delta=32, std=0.5, fast=False):
     """log10 viscosity perturbation per element (ne, nlat, nphi): a random SH
     field (l <= lmax_delta) of standard deviation `std`, full strength from
     the lithosphere base to 100 km depth, decaying as exp(-(z-100 km)/300 km)
     below, zero in the lithosphere."""
-    f = random_sh_field(colat, lon, lmax_delta, -2.0, seed)
+    f = (random_sh_field_fast if fast else random_sh_field)(colat, lon, lmax_delta, -2.0, seed)
     f = (f - f.mean()) * (std / f.std())
     z = element_midpoint_depths(params)
     litho = params.r_earth - params.r_bot[0]
@@ -161,45 +194,63 @@ def delta_visc(params, colat, lon, seed, lmax_delta=32, std=0.5):
 
 
 ENSEMBLE_LMAX = (64, 96, 128, 192)
-_topo_cache = {}
 
 
-def ensemble_members(n=64):
-    """The members of the ensemble workload: (lmax, ne, is_3d) per member.
-    lmax cycles through 64, 96, 128, 192; ne spreads over [32, 64]; 16 of the
-    members are laterally heterogeneous (3-D), taken among those with
-    lmax <= 128 (the tensor transforms are built for nlat <= 256)."""
-    cand = [i for i in range(n) if ENSEMBLE_LMAX[i % 4] != 192]
-    n3 = 16 * n // 64
-    three = set(cand[3 * k + k % 3] for k in range(n3) if 3 * k + k % 3 < len(cand))
-    return [(ENSEMBLE_LMAX[i % 4], 32 + (7 * i) % 33, i in three) for i in range(n)]
+def member_params(lmax, ne, litho_km, eta_um, eta_lm, nlat=0, nphi=0):
+    """The five-layer earth with a lithosphere of litho_km, upper mantle
+    (lithosphere base to 670 km, split at 420 km) of viscosity eta_um and
+    lower mantle (670 km to the core) of eta_lm; densities and rigidities of
+    the five-layer earth; ne elements split over the layers as there."""
+    p = five_layer_params(lmax, ne, nlat, nphi)
+    p.r_bot[0] = p.r_top[1] = p.r_earth - litho_km * KM
+    p.eta[1] = p.eta[2] = eta_um
+    p.eta[3] = eta_lm
+    return p
 
 
-def ensemble_member(i, n=64, seed=0):
-    """Workload of ensemble member i: the sea-level + rotation setting of
-    configuration 1 (migrating coastlines, subgrid term, dt 100 yr, 200
-    calls) at the member's lmax and element count, its own three ice domes,
-    and for 3-D members a lateral viscosity field of its own. The topography
-    is shared by the members of one lmax (its random field is generated up to
-    degree min(lmax, 64))."""
-    L, ne, is3d = ensemble_members(n)[i]
+def ensemble_members(n=64, seed=0):
+    """The members of the uneven ensemble: per member (lmax, ne, is_3d,
+    litho_km, eta_um, eta_lm) from its own seeded draws: lmax from
+    {64, 96, 128, 192}, ne uniform in 32..64, lithosphere uniform in
+    60-120 km, upper- and lower-mantle viscosity log-uniform in
+    1e20-1e22 Pa s. A quarter of the members (16 of 64), picked by a seeded
+    permutation, are laterally heterogeneous (3-D)."""
+    three = set(np.random.default_rng(seed + 4242).permutation(n)[: n // 4].tolist())
+    out = []
+    for i in range(n):
+        r = np.random.default_rng([seed, 7, i])
+        L = int(r.choice(ENSEMBLE_LMAX))
+        ne = int(r.integers(32, 65))
+        litho = float(r.uniform(60.0, 120.0))
+        eu, el = (float(10.0 ** r.uniform(20.0, 22.0)) for _ in range(2))
+        out.append((L, ne, i in three, litho, eu, el))
+    return out
+
+
+def ensemble_member(i, n=64, seed=0, members=None):
+    """Workload of ensemble member i: its own earth (ensemble_members), the
+    sea-level + rotation setting of the 1-D sea-level workload (migrating
+    coastlines, subgrid term, dt 100 yr, 200 calls) at the member's lmax
+    (Gauss grid 192 x 384 at lmax 128, the default grid otherwise) with its
+    own topography (power ~ l^-2 up to lmax, 70 % ocean, +-5 km) and three
+    ice domes, and for 3-D members a lateral viscosity field of its own
+    (std 0.5 up to degree 32). `members` overrides the drawn member list."""
+    L, ne, is3d, litho, eu, el = (members or ensemble_members(n, seed))[i]
     nlat, nphi = (192, 384) if L == 128 else (0, 0)
-    p = five_layer_params(L, ne, nlat, nphi)
+    p = member_params(L, ne, litho, eu, el, nlat, nphi)
     p.rotation = 1
     nphi, nlat = p.grid()
     colat, lon = gauss_grid(nlat, nphi)
-    if L not in _topo_cache:
-        _topo_cache[L] = ocean_topography(colat, lon, min(L, 64), seed + 11)
-    z = _topo_cache[L]
+    z = ocean_topography(colat, lon, L, seed + 1000 + i, fast=True)
     domes = three_domes(colat, lon, seed + 101 + i)
     pert = None
     if is3d:
         p.l_visc_3d = 1
-        pert = delta_visc(p, colat, lon, seed + 211 + i)
+        pert = delta_visc(p, colat, lon, seed + 211 + i, fast=True)
     dt = 100.0
     return dict(name="c4m%d" % i, params=p, z_bed_eq=z, h_ice_eq=np.zeros_like(z), h_ice_init=None,
                 ice=lambda k, dt=dt: domes * float(ice_history_factor((k + 1) * dt / 1000.0)),
-                dt_yr=dt, steps=200, pert=pert)
+                domes=domes, dt_yr=dt, steps=200, pert=pert)
 
 
 def config(idx, lmax=None, ne=None, seed=0):

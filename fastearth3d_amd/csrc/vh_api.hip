@@ -220,6 +220,7 @@ struct vh_model {
     long n_accept = 0, n_reject = 0, n_solve = 0, n_inner_tot = 0, n_outer_tot = 0;
     long n_sle_steal = 0;    // sea-level work items run for a block that was not resident
     int sle_late_blocks = 0;     // diagnostic (VH_SLE_LATE_BLOCKS): solve blocks that start late
+    int sle_blocks = 0;          // blocks of the fused sea-level solve (0: one per work item)
     unsigned sle_late_ticks = 0;
     // kernel timing (HIP events on the model's stream), per kernel class
     bool prof = false;
@@ -681,7 +682,7 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
     ia.fixed_ocean = h->fixed_ocean; ia.g = h->g; ia.tol = h->tol; ia.rsl = h->d_rsl;
     ia.part = h->d_part; ia.cs = h->d_part + 3 * (size_t)s.nlat; ia.part3 = h->d_part + 4 * (size_t)s.nlat;
     ia.res = r; ia.F = h->d_F; ia.FC = h->d_FC; ia.F2 = h->d_F2; ia.sigma = h->d_sigma; ia.bar = h->d_sbar;
-    ia.late_blocks = h->sle_late_blocks; ia.late_ticks = h->sle_late_ticks;
+    ia.late_blocks = h->sle_late_blocks; ia.late_ticks = h->sle_late_ticks; ia.blocks = h->sle_blocks;
     h->prof_begin(VH_K_SLE);
     sle_solve_launch(st, ia);
     h->prof_end();
@@ -1973,6 +1974,27 @@ int vh_resp_advance(vh_model* h, const double* sigma_lm) {
         h->maxn_cur = h->h_pin[0];
         h->cur = nxt;
         h->time_resp += h->dt;
+    });
+}
+
+int vh_set_tuning(vh_model* h, int which, int value) {
+    return guarded(h, [&] {
+        switch (which) {
+            case VH_T_SLE_BLOCKS:
+                if (value < 0) throw vh_error("vh_set_tuning: VH_T_SLE_BLOCKS must be >= 0");
+                h->sle_blocks = value;
+                break;
+            default: throw vh_error("vh_set_tuning: unknown knob");
+        }
+    });
+}
+
+int vh_get_tuning(vh_model* h, int which, int* value) {
+    return guarded(h, [&] {
+        switch (which) {
+            case VH_T_SLE_BLOCKS: *value = h->sle_blocks; break;
+            default: throw vh_error("vh_get_tuning: unknown knob");
+        }
     });
 }
 

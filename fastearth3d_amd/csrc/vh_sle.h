@@ -60,9 +60,13 @@ struct sle_solve_args {
     // diagnostic: the last late_blocks blocks start late_ticks (100 MHz) late
     int late_blocks;
     unsigned late_ticks;
+    // blocks of the launch (<= 0: one per work item); block b owns items b,
+    // b + blocks, ... (fewer blocks leave the device to other work, e.g. the
+    // other members of an ensemble)
+    int blocks;
 };
 void sle_solve_launch(hipStream_t st, const sle_solve_args& a);
-int sle_solve_grid(const vh_sht& s);   // work items (= blocks) of the fused solve
+int sle_solve_grid(const vh_sht& s);   // work items of the fused solve
 void grid_sub(hipStream_t st, int n, const double* a, const double* b, double* out);
 void grid_absdiff(hipStream_t st, int n, const double* a, const double* b, double* out);   // |a - b|
 void grid_ocean(hipStream_t st, int n, const double* topo, const double* ice, double* C);

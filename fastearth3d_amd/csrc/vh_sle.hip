@@ -141,9 +141,11 @@ __global__ void k_bsl_integrand(int n, const double* __restrict__ h, const doubl
 // control record that blocks which fell behind read instead.
 // Each stage's arithmetic is fixed by the sizes alone (partition of the
 // Legendre sums, order of the totals), so a rerun gives identical bits.
-// No block waits for a block that has not started: block b owns item b of
-// every stage and claims the next stage's item before it arrives at the
-// current stage's end; a block that has waited VH_SLE_STEAL_POLLS polls on a stage
+// No block waits for a block that has not started: of a launch of B blocks,
+// block b owns items b, b + B, ... of every stage (B = the item count unless
+// the caller asks for fewer blocks), claims each as it reaches it, and claims
+// its first item of the next stage before it arrives at the current stage's
+// end; a block that has waited VH_SLE_STEAL_POLLS polls on a stage
 // claims and runs the stage's unclaimed items (whose owners are not resident:
 // a device shared with other work or another process). A stage therefore
 // completes as long as one block runs, and its result does not depend on
@@ -796,16 +798,19 @@ __host__ __device__ __forceinline__ unsigned* sle_slot_seq(unsigned* bar, unsign
 // MinW: wavefronts per SIMD the build must allow; 2 halves the registers
 // (with spills) so that two blocks share a CU when the grid (latitude pairs +
 // orders, 386 blocks at lmax 192) exceeds the CU count.
-template <bool TabLds, int MinW>
+// Multi: the launch has fewer blocks than items (a block owns several).
+template <bool TabLds, int MinW, bool Multi>
 __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     extern __shared__ double2 lds[];
     const vh_sht& s = a.s;
     const int L = s.L, N = s.nphi, nlat = s.nlat, nh = s.nh;
-    const int npairs = (nlat + 1) / 2, G = npairs + L + 1, own = blockIdx.x;
+    // block own owns items own, own + B, own + 2B, ... of every stage
+    const int npairs = (nlat + 1) / 2, G = npairs + L + 1, own = blockIdx.x, B = gridDim.x;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     __shared__ double redmax[4][4];
     __shared__ double sct[CT_N], snx[CT_N];   // this stage's control record, the next one
     __shared__ unsigned su[2];
+    __shared__ int slater;                     // the block's next owned item of the stage (item count B..)
     __shared__ unsigned long long smask;
     __shared__ double2 spf[4][256];            // an order's prefetched row spectra (nh <= 256)
     // Inputs written by the kernels before this launch (ice, drift, rsl, ...)
@@ -813,7 +818,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     // CU's and XCD's cached copies before anything is read, whatever fences
     // the launch itself carries; a release at the end publishes the outputs.
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (a.late_blocks > 0 && own >= G - a.late_blocks) {   // diagnostic: a block that starts late
+    if (a.late_blocks > 0 && own >= B - a.late_blocks) {   // diagnostic: a block that starts late
         if (tid == 0) {
             const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
             while (__builtin_amdgcn_s_memrealtime() - t0 < (unsigned long long)a.late_ticks) __builtin_amdgcn_s_sleep(10);
@@ -1073,17 +1078,28 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
             bool pre = claimed;                // the own item's arrival claims the next one
             unsigned polls = 0;                // of this stage's release word (thread 0)
             int w = 0;
+            if (Multi && tid == 0) slater = own + B;   // read after the next barrier
             for (;;) {
                 if (item >= 0) {
                     // the one place where work items run (keeps the transforms inlined once)
                     if (item != own) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                     run_item(item, sn, item != own);
                     if (pending_publish && pending_sn == sn) { publish(sn, sct); pending_publish = false; }
-                    if (item != own && tid == 0)
+                    if ((Multi ? item % B : item) != own && tid == 0)   // an item of another block
                         __hip_atomic_fetch_add(a.bar + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (pre) sync = arrive(own, sn, true, false);
                     else arrive(item, sn, false, sync);
                     pre = false;
+                }
+                // the block's further owned items, each claimed as the block reaches it
+                item = -1;
+                if constexpr (Multi) {
+                    __syncthreads();
+                    for (int j = slater; j < G; j += B) {
+                        if (claim(j, sn)) { item = j; break; }   // claim ends with barriers
+                    }
+                    if (tid == 0) slater = item >= 0 ? item + B : G;
+                    if (item >= 0) continue;
                 }
                 w = wait_or_take(sn, polls);
                 if (w < 0) break;
@@ -1156,27 +1172,29 @@ void grid_bsl_integrand(hipStream_t st, int n, const double* h, const double* C,
 
 // Raises the dynamic-LDS limit of one instance of the solve kernel once per
 // process (the attribute belongs to the function, not to a launch).
-template <bool TabLds, int MinW>
+template <bool TabLds, int MinW, bool Multi>
 static void sle_lds_limit(size_t lds) {
     static std::atomic<size_t> set{0};
     if (lds > 65536 && set.load() < lds) {
-        HIPCHECK(hipFuncSetAttribute((const void*)k_sle_solve<TabLds, MinW>,
+        HIPCHECK(hipFuncSetAttribute((const void*)k_sle_solve<TabLds, MinW, Multi>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         set.store(lds);
     }
 }
-template <bool TabLds, int MinW>
+template <bool TabLds, int MinW, bool Multi = false>
 static void sle_launch_one(hipStream_t st, int grid, size_t lds, const sle_solve_args& a) {
-    sle_lds_limit<TabLds, MinW>(lds);
-    k_sle_solve<TabLds, MinW><<<grid, 256, lds, st>>>(a);
+    sle_lds_limit<TabLds, MinW, Multi>(lds);
+    k_sle_solve<TabLds, MinW, Multi><<<grid, 256, lds, st>>>(a);
     HIPCHECK(hipGetLastError());
 }
 
 void sle_solve_launch(hipStream_t st, const sle_solve_args& a) {
     const vh_sht& s = a.s;
     if (s.nh > 256) throw vh_error("sle: the fused sea-level loop supports nlat <= 512");
-    const int grid = sle_solve_grid(s);   // latitude-pair items, then order items: one block each
-    if (grid > VH_SLE_MAX_ITEMS) throw vh_error("sle: the fused sea-level loop supports at most 1024 work items");
+    const int items = sle_solve_grid(s);   // latitude-pair items, then order items
+    if (items > VH_SLE_MAX_ITEMS) throw vh_error("sle: the fused sea-level loop supports at most 1024 work items");
+    // one block per item, or a.blocks blocks owning items round-robin
+    const int grid = a.blocks > 0 ? std::min(items, a.blocks) : items;
     // the Legendre table of each order stays in LDS when it fits beside the
     // rest (160 KiB per block less the kernel's static LDS)
     const size_t lds_max = (size_t)160 * 1024 - VH_SLE_STATIC_LDS;
@@ -1187,6 +1205,12 @@ void sle_solve_launch(hipStream_t st, const sle_solve_args& a) {
     HIPCHECK(hipGetDevice(&dev));
     HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     const bool two = grid > ncu;   // one block per CU does not hold the grid
+    if (grid < items) {            // blocks owning several items: at most one per CU asked for
+        if (two) throw vh_error("sle: VH_T_SLE_BLOCKS above the CU count");
+        if (tab) sle_launch_one<true, 1, true>(st, grid, lds, a);
+        else sle_launch_one<false, 1, true>(st, grid, lds, a);
+        return;
+    }
     if (tab) {
         if (two) sle_launch_one<true, 2>(st, grid, lds, a);
         else sle_launch_one<true, 1>(st, grid, lds, a);

@@ -61,11 +61,23 @@ def load():
         L.vh_remap_to_ll_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.vh_remap_destroy.argtypes = [C.c_void_p]
         L.vh_spinup.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int]
+        L.vh_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.vh_get_tuning.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        L.vh_ensemble_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.vh_ensemble_update.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_double)]
+        L.vh_ensemble_update_device.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_double)]
+        L.vh_ensemble_run_device.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_double)]
+        L.vh_ensemble_tune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_int)]
+        L.vh_ensemble_failed_member.argtypes = [C.c_void_p]
+        L.vh_ensemble_last_error.argtypes = [C.c_void_p]
+        L.vh_ensemble_last_error.restype = C.c_char_p
+        L.vh_ensemble_destroy.argtypes = [C.c_void_p]
         for name in exported_symbols():
             fn = getattr(L, name)
             if fn.argtypes is None:   # every entry point takes a handle or pointer: never call undeclared
                 raise VilmaError("libvilma_hip.so: %s has no ctypes signature" % name)
-            if fn.restype is C.c_int or name == "vh_last_error":
+            if fn.restype is C.c_int or name in ("vh_last_error", "vh_ensemble_last_error"):
                 continue
             fn.restype = C.c_int
         _lib = L
@@ -222,6 +234,80 @@ def solid_earth_update(se, h_ice, dt_yr):
 
 def solid_earth_finalize(se):
     se.finalize()
+
+
+class Ensemble:
+    """An uneven ensemble of SolidEarth members (vh_ensemble_*): a pool of
+    n_workers host threads steps the members, longest first, each on its own
+    stream; results equal stepping them one after another. The members stay
+    owned by the caller (finalize them after close())."""
+
+    def __init__(self, members, n_workers=0):
+        self._lib = load()
+        self.members = list(members)
+        n = len(self.members)
+        hs = (C.c_void_p * n)(*[m._h for m in self.members])
+        out = C.c_void_p()
+        if self._lib.vh_ensemble_create(hs, n, int(n_workers), C.byref(out)) != 0:
+            raise VilmaError("vh_ensemble_create: invalid member list")
+        self._e = out.value
+        self._dt = C.c_double * n
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._lib.vh_ensemble_last_error(self._e)
+            raise VilmaError("%s: %s" % (what, msg.decode() if msg else "failed"))
+
+    def failed_member(self):
+        return self._lib.vh_ensemble_failed_member(self._e)
+
+    def update(self, h_ice, dt_yr):
+        """One solid_earth_update of every member: h_ice[i] the (nlat, nphi)
+        grid of member i, dt_yr a number or one per member."""
+        n = len(self.members)
+        arrs = [m._grid_in(a) for m, a in zip(self.members, h_ice)]
+        if len(arrs) != n:
+            raise VilmaError("ensemble update: %d ice fields for %d members" % (len(arrs), n))
+        dts = self._dt(*([float(dt_yr)] * n if np.isscalar(dt_yr) else [float(x) for x in dt_yr]))
+        ptrs = (C.c_void_p * n)(*[a.ctypes.data for a in arrs])
+        self._check(self._lib.vh_ensemble_update(self._e, ptrs, dts), "ensemble update")
+
+    def update_device(self, ptrs, dt_yr):
+        """The same with device pointers (one per member)."""
+        n = len(self.members)
+        dts = self._dt(*([float(dt_yr)] * n if np.isscalar(dt_yr) else [float(x) for x in dt_yr]))
+        self._check(self._lib.vh_ensemble_update_device(self._e, (C.c_void_p * n)(*ptrs), dts),
+                    "ensemble update")
+
+    def run_device(self, ncalls, hist_ptrs, dt_yr):
+        """ncalls updates of every member; hist_ptrs[i] points to member i's
+        [ncalls][nlat][nphi] ice history in device memory."""
+        n = len(self.members)
+        dts = self._dt(*([float(dt_yr)] * n if np.isscalar(dt_yr) else [float(x) for x in dt_yr]))
+        self._check(self._lib.vh_ensemble_run_device(self._e, int(ncalls), (C.c_void_p * n)(*hist_ptrs), dts),
+                    "ensemble run")
+
+    def tune(self, ncalls, hist_ptrs, dt_yr):
+        """The autotuner (vh_ensemble_tune): 3 x ncalls real updates of every
+        member from hist_ptrs (as run_device); returns the chosen
+        VH_T_SLE_BLOCKS, now set on every member."""
+        n = len(self.members)
+        dts = self._dt(*([float(dt_yr)] * n if np.isscalar(dt_yr) else [float(x) for x in dt_yr]))
+        out = C.c_int(0)
+        self._check(self._lib.vh_ensemble_tune(self._e, int(ncalls), (C.c_void_p * n)(*hist_ptrs), dts,
+                                               C.byref(out)), "ensemble tune")
+        return out.value
+
+    def close(self):
+        if self._e:
+            self._lib.vh_ensemble_destroy(self._e)
+            self._e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Remap:

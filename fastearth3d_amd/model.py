@@ -64,6 +64,15 @@ class ModelHandle:
         h = self._grid_in(h_ice)
         self._check(self._fn("update")(self._h, dptr(h), float(dt_yr)), "solid_earth_update")
 
+    def set_tuning(self, which, value):
+        """A launch-shape knob (vh_set_tuning; results do not depend on it)."""
+        self._check(self._fn("set_tuning")(self._h, int(which), int(value)), "set_tuning")
+
+    def update_device(self, ptr, dt_yr):
+        """solid_earth_update with the ice grid at device address ptr (the
+        library's vh_update_device; not offered by the oracle)."""
+        self._check(self._fn("update_device")(self._h, C.c_void_p(int(ptr)), float(dt_yr)), "solid_earth_update")
+
     def _sized(self, a, n, what):
         """a as a C-contiguous float64 array of exactly n values (the library
         reads n values through the pointer)."""

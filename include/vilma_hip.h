@@ -214,6 +214,54 @@ int vh_destroy(vh_model* h);
 /* Message of the last failure on this handle (or of the last vh_create). */
 const char* vh_last_error(const vh_model* h);
 
+/* ---- Launch-shape knobs (no counterpart in the reference; the results do
+ * not depend on them, bit for bit). VH_T_SLE_BLOCKS: blocks of the fused
+ * sea-level solve, 0 (the default) one per work item (latitude pairs +
+ * orders, lmax + 1 + nlat / 2), else that many blocks (at most the device's
+ * CU count; a solve with more items than that is refused) owning the items
+ * round-robin: fewer blocks solve one model more slowly but leave the device
+ * to other work, e.g. the other members of an ensemble. */
+enum { VH_T_SLE_BLOCKS = 0 };
+int vh_set_tuning(vh_model* h, int which, int value);
+int vh_get_tuning(vh_model* h, int which, int* value);
+
+/* ---- Uneven ensemble (the vh_ensemble_* set). The reference steps an
+ * ensemble as separate solid_earth instances (src/vilma_coupling.f90:127-176,
+ * :336-437 per instance), one after another on the host. Here the members
+ * are models from vh_create (any lmax, grid, element count, earth model,
+ * lateral viscosity), still owned by the caller, advanced by a pool of
+ * n_workers host threads (<= 0: 16; at most one per member) that take members
+ * longest first and step each on its own stream, so that several members'
+ * kernels share the GPU. Results equal those of stepping the members one
+ * after another, bit for bit. The handle is not thread-safe, and a member
+ * must not be used otherwise while an ensemble call runs. On failure the
+ * call stops taking further members and returns nonzero;
+ * vh_ensemble_failed_member and vh_ensemble_last_error name the first member
+ * that failed and its message; members of a vh_ensemble_run_device call may
+ * then stand at different times (vh_get_scalar VH_S_TIME). */
+typedef struct vh_ensemble vh_ensemble;
+int vh_ensemble_create(vh_model* const* members, int n_members, int n_workers, vh_ensemble** out);
+/* one solid_earth_update of every member: h_ice[i] the host grid of member i
+ * ((nphi, nlat) of that member), dt_yr[i] its interval */
+int vh_ensemble_update(vh_ensemble* e, const double* const* h_ice, const double* dt_yr);
+/* the same with each member's ice in device memory */
+int vh_ensemble_update_device(vh_ensemble* e, const double* const* h_ice_device, const double* dt_yr);
+/* ncalls updates of every member without a barrier between calls: member i's
+ * call k takes the grid at ice_hist_device[i] + k * nphi_i * nlat_i (device
+ * memory, [ncalls][nlat_i][nphi_i]) */
+int vh_ensemble_run_device(vh_ensemble* e, int ncalls, const double* const* ice_hist_device, const double* dt_yr);
+/* The ensemble's autotuner: runs 3 x ncalls real updates of every member
+ * (candidate k on calls k*ncalls .. (k+1)*ncalls - 1 of the histories, laid
+ * out as for vh_ensemble_run_device) with the candidate launch shapes of the
+ * sea-level solve (VH_T_SLE_BLOCKS 0, 64, 32) and leaves every member on the
+ * fastest, returned in *chosen_sle_blocks. The members advance by 3 * ncalls
+ * calls, with the same results as any other setting would give. */
+int vh_ensemble_tune(vh_ensemble* e, int ncalls, const double* const* ice_hist_device, const double* dt_yr,
+                     int* chosen_sle_blocks);
+int vh_ensemble_failed_member(const vh_ensemble* e);   /* -1: none */
+const char* vh_ensemble_last_error(const vh_ensemble* e);
+int vh_ensemble_destroy(vh_ensemble* e);            /* the members stay valid */
+
 /* ---- Measurement hook (no counterpart in the reference): per-kernel-class
  * device time from HIP events recorded on the model's stream around each
  * launch. enable = 1 starts accumulating and clears the totals, 2 resumes
