@@ -97,6 +97,23 @@ def alg_flops(dims, cls, chunk_batches=None):
     return None
 
 
+def host_cpu():
+    """The host's CPU model, NUMA nodes and the cores this process may use."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        numa = len([d for d in os.listdir("/sys/devices/system/node") if d.startswith("node") and d[4:].isdigit()])
+    except OSError:
+        numa = None
+    return {"cpu_model": model, "numa_nodes": numa, "cores_available": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline(cfg, lmax, calls):
     """The oracle (OpenMP C restatement of the same step) timed on the host
     cores over `calls` coupling calls of the same workload, from the same
@@ -522,6 +539,8 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--offset", type=int, default=200,
+                    help="untimed calls before the warm-up (the timed window samples mid-history)")
     ap.add_argument("--lmax", type=int, default=128)
     ap.add_argument("--cpu-calls", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
@@ -570,7 +589,11 @@ def main():
     se = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
     if c["pert"] is not None:
         se.enable_visc_3d(c["pert"])
-    nsteps = a.warmup + a.steps
+    # the timed calls start --offset calls into the history (run untimed
+    # first), where the ice load and the sea-level iteration counts are those
+    # of the bulk of the run rather than of its first, nearly ice-free calls
+    off = max(0, min(a.offset, c["steps"] - a.warmup - a.steps))
+    nsteps = off + a.warmup + a.steps
     ice = torch.from_numpy(np.stack([c["ice"](k) for k in range(nsteps)])).to("cuda:0")
     torch.cuda.synchronize()
 
@@ -579,7 +602,7 @@ def main():
         if rc:
             raise RuntimeError(se._err())
 
-    for k in range(a.warmup):
+    for k in range(off + a.warmup):
         step(k)
     L.vh_profile(se._h, 1)   # clear the per-kernel totals
     L.vh_profile(se._h, 0)
@@ -587,10 +610,10 @@ def main():
     n_in0 = se.scalar(6)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(a.warmup, nsteps):
+    for k in range(off + a.warmup, nsteps):
         # per-kernel HIP events on every PROF_EVERY-th call of the timed region
         # (an event pair around each launch costs ~8 % of the call when on everywhere)
-        sampled = (k - a.warmup) % PROF_EVERY == PROF_EVERY - 1 or (a.steps < PROF_EVERY and k == nsteps - 1)
+        sampled = (k - off - a.warmup) % PROF_EVERY == PROF_EVERY - 1 or (a.steps < PROF_EVERY and k == nsteps - 1)
         if sampled:
             L.vh_profile(se._h, 2)
         step(k)
@@ -649,7 +672,7 @@ def main():
     if not a.no_cpu:
         v, t, ns = cpu_baseline(a.config, a.lmax, a.cpu_calls)
         cpu = {"value": round(v, 6), "unit": "model-kyr/s",
-               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())), "kind": "port",
+               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())), "kind": "port", **host_cpu(),
                "sample": "first %d coupling call(s) of %g yr (%d FE sub-steps) of the same workload, %.1f s" % (
                    a.cpu_calls, c["dt_yr"], int(ns), t)}
 
@@ -662,7 +685,8 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": names[a.config], "lmax": a.lmax, "nlat": se.nlat, "nphi": se.nphi,
                    "radial_elements": d["ne"], "maxwell_elements": d["nem"], "dt_yr": c["dt_yr"],
-                   "sub_steps_timed": int(nsub), "ne3d": int(se.dims.get("ne3d", 0))},
+                   "sub_steps_timed": int(nsub), "ne3d": int(se.dims.get("ne3d", 0)),
+                   "timed_calls": [off + a.warmup, nsteps - 1], "sle_inner_per_sub_step": round(n_inner / max(nsub, 1), 2)},
         "roofline": roof, "kernel_roofline": per_kernel, "kernel_ms": kernel_ms, "cpu_baseline": cpu,
     }
     if cpu:

@@ -183,3 +183,21 @@ def test_adjoint_errors():
     with pytest.raises(VilmaError, match="positive"):
         m.forward_record(np.stack([c["ice"](60)]), 0.0)
     m.finalize()
+
+
+def test_autograd_function_harness():
+    """The forward / backward pair wrapped in a torch.autograd.Function (test
+    harness, tests/child_autograd.py, its own process so that torch's HIP
+    runtime starts before the library loads): the autograd gradient equals
+    the C-ABI's bit for bit and matches central differences within 1e-6."""
+    import json
+    import os
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    r = subprocess.run([sys.executable, os.path.join(here, "child_autograd.py")], capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["autograd_equals_capi"] and out["ok"]
