@@ -87,8 +87,10 @@ enum {
     VH_S_N_ACCEPT = 3, VH_S_N_REJECT = 4, VH_S_N_SOLVE = 5,
     VH_S_N_INNER_TOT = 6, VH_S_N_OUTER_TOT = 7,
     VH_S_POLAR_M_RE = 8, VH_S_POLAR_M_IM = 9,
-    VH_S_N_SLE_STEAL = 10   /* sea-level work items a block ran for another that
+    VH_S_N_SLE_STEAL = 10,  /* sea-level work items a block ran for another that
                                was not resident (a device shared with other work) */
+    VH_S_N_FLOOR = 11,      /* trap: steps accepted at dt_min over tolerance     */
+    VH_S_N_COUPLE_TOT = 12  /* sea-level <-> memory passes over all solves       */
 };
 enum {
     VH_SP_SIGMA = 0,   /* se%sigma_lm: converged surface load [kg m^-2]      */

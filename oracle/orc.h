@@ -73,6 +73,9 @@ void orc_strain_constants(double Jr, double norm[4], double sa[4][4], double sb[
 void orc_dissipative_rhs(int ne, const double* r, double sa[4][4], double sb[4][4], double sc[4][4],
                          const double norm[4], const double* A, const double* B, const double* C,
                          double* f);
+void orc_advance_memory_trap(int ne, const double* mu, const double* Mk, const double* Un, const double* Vn,
+                             const double* Up, const double* Vp, double Jr, double* A, double* B, double* C,
+                             const unsigned char* active);
 void orc_advance_memory_fe(int ne, const double* mu, const double* Mk, const double* Un,
                            const double* Vn, double Jr, double* A, double* B, double* C,
                            const unsigned char* active);
@@ -141,9 +144,18 @@ typedef struct {
     double *Mk3, *MkPerDt3;  /* [e][nlat][nphi] */
     int* e3d;
     orc_tsh tsh;
-    /* controller snapshot */
+    /* controller snapshot (buffer A) */
     double *Are_s, *Aim_s, *Bre_s, *Bim_s, *Cre_s, *Cim_s;
     double time_s;
+    /* implicit (trapezoidal) memory, src/vilma_response.f90:1514-1663 */
+    int implicit;
+    double couple_tol;
+    int couple_pass, couple_done;
+    cplx *sigma_n, *sigma_next, *sigma_n_s, *dUa_prev;
+    int sigma_primed, sigma_primed_s, have_next;
+    double *Are0, *Aim0, *Bre0, *Bim0, *Cre0, *Cim0;   /* tau_n of the step */
+    double *edUn_re, *edUn_im, *edVn_re, *edVn_im;     /* endpoint drift [k][node] */
+    double *Are_c, *Aim_c, *Bre_c, *Bim_c, *Cre_c, *Cim_c;   /* coarse stash (buffer B) */
 } orc_resp;
 
 int  orc_resp_init_ve(orc_resp* r, const orc_earth* e, const orc_mesh* m, const orc_sht* s, double dt, char* err);
@@ -155,6 +167,13 @@ void orc_resp_advance(orc_resp* r, const orc_sht* s, const cplx* sigma);
 void orc_resp_set_dt(orc_resp* r, double dt);
 double orc_resp_max_rate(const orc_resp* r);
 double orc_resp_memory_norm(const orc_resp* r);
+void orc_resp_set_implicit(orc_resp* r, int nlm);
+void orc_resp_prepare_endpoint(orc_resp* r);
+void orc_resp_advance_endpoint(orc_resp* r, const orc_sht* s, const cplx* sigma);
+void orc_resp_finalize_step(orc_resp* r);
+void orc_resp_prime_sigma(orc_resp* r, const cplx* sigma);
+void orc_resp_stash_coarse(orc_resp* r);
+void orc_resp_coarse_fine_error(const orc_resp* r, double* err_inf, double* tau_inf);
 void orc_resp_save(orc_resp* r);
 void orc_resp_restore(orc_resp* r);
 int  orc_resp_enable_lateral(orc_resp* r, const orc_sht* s, const double* pert, char* err);
@@ -162,6 +181,7 @@ int  orc_resp_enable_lateral(orc_resp* r, const orc_sht* s, const double* pert, 
 /* ------------------------------------------------------------------ SLE */
 typedef struct {
     int n_outer, n_inner, max_mem_iter, fixed_ocean, subgrid, warm_start;
+    long n_couple_tot;
     double tol;
     long n_solve, n_outer_tot, n_inner_tot;
 } orc_sle;
@@ -217,7 +237,8 @@ typedef struct orc_model {
     orc_rot rot;
     /* stepper (src/vilma_timestep.f90:38-75) */
     double cfl, dt_min, dt_max, guard_growth;
-    long n_accept, n_reject, n_solve;
+    double rtol, atol, safety, grow_max, shrink_min, dt_try;
+    long n_accept, n_reject, n_solve, n_floor;
     double worst_mass_resid;
     /* gauss-grid state */
     int np, nl;

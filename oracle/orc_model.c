@@ -53,7 +53,9 @@ int orc_create(const vh_params* p, int nphi, int nlat, const double* z_bed_eq, c
     orc_model* m = calloc(1, sizeof(orc_model));
     *out = m;
     m->par = *p;
-    if (strcmp(p->scheme, "fe") != 0) { snprintf(err, 256, "scheme \"%s\" not supported (fe only)", p->scheme); return 1; }
+    if (strcmp(p->scheme, "fe") != 0 && strcmp(p->scheme, "trap") != 0) {
+        snprintf(err, 256, "scheme \"%s\" not supported (fe|trap)", p->scheme); return 1;
+    }
     if (p->l_toroidal) { snprintf(err, 256, "l_toroidal = .true. is not supported"); return 1; }
     if ((p->mmax >= 0 && p->mmax != p->lmax) || p->mres != 1) { snprintf(err, 256, "mmax/mres other than lmax/1 not supported"); return 1; }
     int nl = p->nlat > 0 ? p->nlat : 2 * p->lmax + 2;
@@ -78,10 +80,14 @@ int orc_create(const vh_params* p, int nphi, int nlat, const double* z_bed_eq, c
         snprintf(err, 256, "solid_earth_init: unknown earth_response (use ve|elastic|null)"); return 1;
     }
     m->resp.visc3d_tol = p->visc3d_tol;
+    /* the scheme reaches the response only for ve (src/vilma_coupling.f90:269-281) */
+    if (m->resp.kind == RESP_VE && strcmp(p->scheme, "trap") == 0) orc_resp_set_implicit(&m->resp, m->sht.nlm);
     m->sle.n_outer = p->sle_n_outer; m->sle.n_inner = p->sle_n_inner; m->sle.tol = p->sle_tol;
     m->sle.max_mem_iter = p->sle_max_mem_iter; m->sle.fixed_ocean = p->sle_fixed_ocean;
     m->sle.subgrid = p->sle_subgrid; m->sle.warm_start = 1;
     m->cfl = p->cfl; m->dt_min = p->dt_min; m->dt_max = p->dt_max; m->guard_growth = 1.0e3;
+    m->rtol = p->rtol; m->atol = p->atol; m->safety = p->safety; m->grow_max = p->grow_max;
+    m->shrink_min = p->shrink_min; m->dt_try = p->dt_init;   /* src/vilma_coupling.f90:300-308 */
     size_t ng = (size_t)np * nl;
     m->s_rot = calloc(ng, 8);
     if (p->rotation) {
@@ -140,6 +146,85 @@ int orc_enable_visc_3d_nodes(orc_model* m, const double* visc_node, double lid_d
     return rc;
 }
 
+/* one SLE solve with the ice interpolated at t_eval (solve_at,
+ * src/vilma_timestep.f90:312-326) */
+static void solve_at(orc_model* m, const double* ice0, const double* ice1, double t0, double span, double t_eval,
+                     const double* s_rot, double* ice_now, double* dice_now) {
+    size_t ng = (size_t)m->np * m->nl;
+    orc_sle_result res;
+    double frac = (t_eval - t0) / span;
+    for (size_t i = 0; i < ng; i++) {
+        ice_now[i] = ice0[i] + frac * (ice1[i] - ice0[i]);
+        dice_now[i] = ice_now[i] - m->h_ice_eq[i];
+    }
+    m->n_solve++;
+    orc_sle_solve(&m->sle, &m->sht, &m->resp, dice_now, ice_now, m->z_bed_eq, m->rsl, m->C, &res, 0, m->sigma_lm, s_rot);
+    m->worst_mass_resid = fmax(m->worst_mass_resid, res.mass_resid);
+}
+
+/* stepper_advance, implicit (trapezoidal) branch with the step-doubling
+ * controller (src/vilma_timestep.f90:229-307) */
+static int stepper_trap(orc_model* m, const double* ice0, const double* ice1, double t0, double t1,
+                        const double* s_rot, double* ice_now, double* dice_now, double* rsl_n) {
+    size_t ng = (size_t)m->np * m->nl;
+    double span = t1 - t0;
+    orc_resp* r = &m->resp;
+    if (!r->sigma_primed) {   /* the start-of-step load sigma_0 from a report-only solve */
+        orc_sle_result res;
+        cplx* sig0 = calloc(m->sht.nlm, sizeof(cplx));
+        memcpy(ice_now, ice0, ng * 8);
+        for (size_t i = 0; i < ng; i++) dice_now[i] = ice0[i] - m->h_ice_eq[i];
+        orc_sle_solve(&m->sle, &m->sht, r, dice_now, ice_now, m->z_bed_eq, m->rsl, m->C, &res, 1, sig0, s_rot);
+        orc_resp_prime_sigma(r, sig0);
+        free(sig0);
+    }
+    const int p = 2;                                   /* scheme_order(trap) */
+    const double ricfac = (double)((1 << p) - 1), expo = 1.0 / (double)(p + 1);
+    double t = t0;
+    if (m->dt_try <= 0.0) m->dt_try = span;
+    for (;;) {
+        if (t >= t1 - 1.0e-9 * span) break;
+        double dt = fmin(m->dt_try, t1 - t);
+        int at_floor = m->dt_min > 0.0 && dt <= m->dt_min * (1.0 + 1.0e-9);
+        memcpy(rsl_n, m->rsl, ng * 8);
+        orc_resp_save(r);
+        orc_resp_set_dt(r, dt);
+        solve_at(m, ice0, ice1, t0, span, t + dt, s_rot, ice_now, dice_now);            /* coarse */
+        orc_resp_stash_coarse(r);
+        orc_resp_restore(r);
+        memcpy(m->rsl, rsl_n, ng * 8);
+        orc_resp_set_dt(r, 0.5 * dt);
+        solve_at(m, ice0, ice1, t0, span, t + 0.5 * dt, s_rot, ice_now, dice_now);      /* fine 1 */
+        solve_at(m, ice0, ice1, t0, span, t + dt, s_rot, ice_now, dice_now);            /* fine 2 */
+        double err_inf, tau_inf;
+        orc_resp_coarse_fine_error(r, &err_inf, &tau_inf);
+        orc_resp_set_dt(r, dt);
+        double errsc = (err_inf / ricfac) / (m->atol + m->rtol * tau_inf);
+        if (!(errsc <= DBL_MAX)) {
+            snprintf(m->err, 256, "vilma_timestep: implicit local-error estimate went non-finite "
+                                  "(viscosity too stiff, or the state has already diverged)");
+            return 1;
+        }
+        int accept = errsc <= 1.0 || at_floor;
+        if (accept) {
+            t = t + dt; m->n_accept++;
+            if (at_floor && errsc > 1.0) m->n_floor++;
+        } else {
+            orc_resp_restore(r);
+            memcpy(m->rsl, rsl_n, ng * 8);
+            m->n_reject++;
+        }
+        double fac = errsc <= 0.0 ? m->grow_max : m->safety * pow(errsc, -expo);
+        fac = fmin(m->grow_max, fmax(m->shrink_min, fac));
+        m->dt_try = fmin(m->dt_max, fmax(m->dt_min, dt * fac));
+        if (m->dt_try <= 1.0e-12 * span) {
+            snprintf(m->err, 256, "vilma_timestep: implicit adaptive step collapsed (tolerance unreachable)");
+            return 1;
+        }
+    }
+    return 0;
+}
+
 /* stepper_advance, forward-Euler branch (src/vilma_timestep.f90:79-227) */
 static int stepper_advance(orc_model* m, const double* ice0, const double* ice1, double t0, double t1,
                            const double* s_rot) {
@@ -155,6 +240,11 @@ static int stepper_advance(orc_model* m, const double* ice0, const double* ice1,
         m->worst_mass_resid = res.mass_resid;
         free(ice_now); free(dice_now); free(rsl_n);
         return 0;
+    }
+    if (m->resp.implicit) {
+        int rc = stepper_trap(m, ice0, ice1, t0, t1, s_rot, ice_now, dice_now, rsl_n);
+        free(ice_now); free(dice_now); free(rsl_n);
+        return rc;
     }
     double rate = orc_resp_max_rate(&m->resp);
     double dt_stab = rate > 0.0 ? m->cfl / rate : span;
@@ -279,6 +369,9 @@ int orc_get_scalar(orc_model* m, int which, double* out) {
         case VH_S_N_OUTER_TOT: *out = (double)m->sle.n_outer_tot; break;
         case VH_S_POLAR_M_RE: *out = creal(m->rot.m); break;
         case VH_S_POLAR_M_IM: *out = cimag(m->rot.m); break;
+        case VH_S_N_SLE_STEAL: *out = 0.0; break;
+        case VH_S_N_FLOOR: *out = (double)m->n_floor; break;
+        case VH_S_N_COUPLE_TOT: *out = (double)m->sle.n_couple_tot; break;
         default: return 1;
     }
     return 0;

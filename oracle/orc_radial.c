@@ -623,3 +623,25 @@ void orc_advance_memory_fe(int ne, const double* mu, const double* Mk, const dou
         }
     }
 }
+
+/* advance_memory, trapezoidal rule (src/vilma_viscoelastic.f90:551-565):
+ * tau+ = [(1 - M/2) tau_n - mu M (eps_n + eps_{n+1})] / (1 + M/2); Un/Vn the
+ * endpoint nodal fields, Up/Vp the start-of-step ones. A/B/C hold tau_n on
+ * entry. */
+void orc_advance_memory_trap(int ne, const double* mu, const double* Mk, const double* Un, const double* Vn,
+                             const double* Up, const double* Vp, double Jr, double* A, double* B, double* C,
+                             const unsigned char* active) {
+    for (int e = 0; e < ne; e++) {
+        if (active && !active[e]) continue;
+        double a[4], b[4], c[4], ap[4], bp[4], cp[4];
+        orc_strain_coeffs(Un[e], Un[e + 1], Vn[e], Vn[e + 1], Jr, a, b, c);
+        orc_strain_coeffs(Up[e], Up[e + 1], Vp[e], Vp[e + 1], Jr, ap, bp, cp);
+        double den = 1.0 + 0.5 * Mk[e];
+        double c_old = (1.0 - 0.5 * Mk[e]) / den, w_eps = mu[e] * Mk[e] / den;
+        for (int m = 0; m < 4; m++) {
+            A[e * 4 + m] = c_old * A[e * 4 + m] - w_eps * (a[m] + ap[m]);
+            B[e * 4 + m] = c_old * B[e * 4 + m] - w_eps * (b[m] + bp[m]);
+            C[e * 4 + m] = c_old * C[e * 4 + m] - w_eps * (c[m] + cp[m]);
+        }
+    }
+}

@@ -10,6 +10,7 @@
  *   :1665-1811 (set_dt, save/restore, max_rate, memory_norm).
  * Memory arrays keep the reference layout (nlam, ne, nk) column-major.
  */
+#include <float.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -137,6 +138,10 @@ void orc_resp_free(orc_resp* r) {
     free(r->Mk3); free(r->MkPerDt3); free(r->e3d);
     if (r->tsh_ready) orc_tsh_free(&r->tsh);
     free(r->Are_s); free(r->Aim_s); free(r->Bre_s); free(r->Bim_s); free(r->Cre_s); free(r->Cim_s);
+    free(r->sigma_n); free(r->sigma_next); free(r->sigma_n_s); free(r->dUa_prev);
+    free(r->Are0); free(r->Aim0); free(r->Bre0); free(r->Bim0); free(r->Cre0); free(r->Cim0);
+    free(r->edUn_re); free(r->edUn_im); free(r->edVn_re); free(r->edVn_im);
+    free(r->Are_c); free(r->Aim_c); free(r->Bre_c); free(r->Bim_c); free(r->Cre_c); free(r->Cim_c);
     memset(r, 0, sizeof(*r));
 }
 
@@ -156,9 +161,9 @@ static double absmax6(size_t n, const double* x1, const double* x2, const double
     return a;
 }
 
-/* solve_drift (src/vilma_response.f90:694-809) */
-void orc_resp_begin(orc_resp* r) {
-    if (r->kind != RESP_VE) return;
+/* solve_drift (src/vilma_response.f90:694-809) into the nodal targets
+ * (U, V re/im per slot and node) and the report drift dUa / dFa / dVa */
+static void resp_drift(orc_resp* r, double* tUr, double* tUi, double* tVr, double* tVi) {
     int ne = r->ne, nr = r->nr, nd = r->ndof, L = r->lmax;
     size_t nb = (size_t)4 * ne;
 #pragma omp parallel for schedule(static)
@@ -174,8 +179,8 @@ void orc_resp_begin(orc_resp* r) {
 #pragma omp for schedule(dynamic)
         for (int l = 1; l <= L; l++) {
             for (int k = r->kbeg[l]; k < r->kbeg[l + 1]; k++) {
-                double* Ur = r->dUn_re + (size_t)k * nr; double* Ui = r->dUn_im + (size_t)k * nr;
-                double* Vr = r->dVn_re + (size_t)k * nr; double* Vi = r->dVn_im + (size_t)k * nr;
+                double* Ur = tUr + (size_t)k * nr; double* Ui = tUi + (size_t)k * nr;
+                double* Vr = tVr + (size_t)k * nr; double* Vi = tVi + (size_t)k * nr;
                 if (r->mnorm[k] <= thr) {
                     r->dUa[k] = 0.0; r->dFa[k] = 0.0; r->dVa[k] = 0.0;
                     memset(Ur, 0, nr * 8); memset(Ui, 0, nr * 8); memset(Vr, 0, nr * 8); memset(Vi, 0, nr * 8);
@@ -201,6 +206,12 @@ void orc_resp_begin(orc_resp* r) {
         }
         free(fre); free(fim); free(xre); free(xim);
     }
+}
+
+/* response_begin_step: the drift of the entering memory into dUn / dVn */
+void orc_resp_begin(orc_resp* r) {
+    if (r->kind != RESP_VE) return;
+    resp_drift(r, r->dUn_re, r->dUn_im, r->dVn_re, r->dVn_im);
 }
 
 /* response_apply for every kind (src/vilma_response.f90:241-252, :376-384, :491-511, :842-864) */
@@ -367,6 +378,9 @@ void orc_resp_save(orc_resp* r) {
     memcpy(r->Are_s, r->Are, n * 8); memcpy(r->Aim_s, r->Aim, n * 8); memcpy(r->Bre_s, r->Bre, n * 8);
     memcpy(r->Bim_s, r->Bim, n * 8); memcpy(r->Cre_s, r->Cre, n * 8); memcpy(r->Cim_s, r->Cim, n * 8);
     r->time_s = r->time;
+    /* response_save_state also keeps sigma_n (src/vilma_response.f90:1695-1723) */
+    if (r->sigma_n) memcpy(r->sigma_n_s, r->sigma_n, (size_t)r->nlm * sizeof(cplx));
+    r->sigma_primed_s = r->sigma_primed;
 }
 
 void orc_resp_restore(orc_resp* r) {
@@ -375,6 +389,8 @@ void orc_resp_restore(orc_resp* r) {
     size_t n = (size_t)4 * r->ne * r->nk;
     memcpy(r->Are, r->Are_s, n * 8); memcpy(r->Aim, r->Aim_s, n * 8); memcpy(r->Bre, r->Bre_s, n * 8);
     memcpy(r->Bim, r->Bim_s, n * 8); memcpy(r->Cre, r->Cre_s, n * 8); memcpy(r->Cim, r->Cim_s, n * 8);
+    if (r->sigma_n) memcpy(r->sigma_n, r->sigma_n_s, (size_t)r->nlm * sizeof(cplx));
+    r->sigma_primed = r->sigma_primed_s;
 }
 
 /* response_enable_lateral_visc (src/vilma_response.f90:994-1053); toroidal
@@ -407,4 +423,223 @@ int orc_resp_enable_lateral(orc_resp* r, const orc_sht* s, const double* pert, c
     for (int e = 0; e < r->ne; e++) if (!r->active1d[e]) r->e3d[q++] = e;
     r->lat_visc = 1;
     return 0;
+}
+
+/* ------------------------------------------------------------------ TRAP
+ * The implicit (trapezoidal, Crank-Nicolson) memory rule with the
+ * sea-level <-> memory co-convergence (src/vilma_response.f90:1514-1663) and
+ * the step-doubling controller's buffers (:1682-1788). */
+
+static double* dupz(size_t n) { return calloc(n ? n : 1, sizeof(double)); }
+
+void orc_resp_set_implicit(orc_resp* r, int nlm) {
+    r->implicit = 1;
+    r->couple_tol = 1.0e-6;   /* src/vilma_response.f90:177 */
+    r->nlm = nlm;
+    if (r->kind != RESP_VE || r->Are0) return;
+    size_t n = (size_t)4 * r->ne * r->nk, nn = (size_t)r->nk * r->nr;
+    r->Are0 = dupz(n); r->Aim0 = dupz(n); r->Bre0 = dupz(n); r->Bim0 = dupz(n); r->Cre0 = dupz(n); r->Cim0 = dupz(n);
+    r->Are_c = dupz(n); r->Aim_c = dupz(n); r->Bre_c = dupz(n); r->Bim_c = dupz(n); r->Cre_c = dupz(n); r->Cim_c = dupz(n);
+    if (!r->Are_s) {
+        r->Are_s = dupz(n); r->Aim_s = dupz(n); r->Bre_s = dupz(n); r->Bim_s = dupz(n); r->Cre_s = dupz(n); r->Cim_s = dupz(n);
+    }
+    r->edUn_re = dupz(nn); r->edUn_im = dupz(nn); r->edVn_re = dupz(nn); r->edVn_im = dupz(nn);
+    r->sigma_n = calloc(nlm, sizeof(cplx)); r->sigma_next = calloc(nlm, sizeof(cplx));
+    r->sigma_n_s = calloc(nlm, sizeof(cplx)); r->dUa_prev = calloc(r->nk ? r->nk : 1, sizeof(cplx));
+}
+
+/* ve_response_prepare_endpoint (:1598-1616): snapshot tau_n, seed the
+ * endpoint drift with the entering drift, keep the report drift */
+void orc_resp_prepare_endpoint(orc_resp* r) {
+    r->couple_pass = 0;
+    r->couple_done = 0;
+    if (!r->implicit || r->kind != RESP_VE) return;
+    size_t n = (size_t)4 * r->ne * r->nk, nn = (size_t)r->nk * r->nr;
+    memcpy(r->Are0, r->Are, n * 8); memcpy(r->Aim0, r->Aim, n * 8); memcpy(r->Bre0, r->Bre, n * 8);
+    memcpy(r->Bim0, r->Bim, n * 8); memcpy(r->Cre0, r->Cre, n * 8); memcpy(r->Cim0, r->Cim, n * 8);
+    memcpy(r->edUn_re, r->dUn_re, nn * 8); memcpy(r->edUn_im, r->dUn_im, nn * 8);
+    memcpy(r->edVn_re, r->dVn_re, nn * 8); memcpy(r->edVn_im, r->dVn_im, nn * 8);
+    memcpy(r->dUa_prev, r->dUa, (size_t)r->nk * sizeof(cplx));
+}
+
+/* advance_shape_tensor_trap for the three shapes of element e
+ * (src/vilma_response.f90:1371-1478): tau_n from the snapshot, eps_n from
+ * (sigma_n, dUn), eps_{n+1} from (sigma_{n+1}, edUn); pointwise
+ * tau+ - tau_n = -M/(1 + M/2) tau_n - mu M/(1 + M/2) (eps_n + eps_{n+1}) */
+static void advance_element_3d_trap(orc_resp* r, const orc_sht* s, const cplx* sigma, const cplx* sig_n, int e,
+                                    cplx* cm, cplx* cn, cplx* c1, cplx* cdel, double* dt0, double* den, double* de1) {
+    int nlm = s->nlm, nr = r->nr, ne = r->ne;
+    size_t nb = (size_t)4 * ne, ng = (size_t)s->nphi * s->nlat;
+    memset(cm, 0, 12 * (size_t)nlm * sizeof(cplx)); memset(cn, 0, 12 * (size_t)nlm * sizeof(cplx));
+    memset(c1, 0, 12 * (size_t)nlm * sizeof(cplx));
+    for (int k = 0; k < r->nk; k++) {
+        int l = r->kdeg[k], lm = r->k2lm[k];
+        double sre = creal(sigma[lm]), sim = cimag(sigma[lm]);
+        double srn = creal(sig_n[lm]), sin_ = cimag(sig_n[lm]);
+        const double* xU = r->xUn + (size_t)(l - 1) * nr; const double* xV = r->xVn + (size_t)(l - 1) * nr;
+        size_t o = (size_t)k * nr;
+        double a[4][4], b[4][4], c[4][4];   /* [eps_n re, eps_n im, eps_1 re, eps_1 im][lam] */
+        orc_strain_coeffs(srn * xU[e] + r->dUn_re[o + e], srn * xU[e + 1] + r->dUn_re[o + e + 1],
+                          srn * xV[e] + r->dVn_re[o + e], srn * xV[e + 1] + r->dVn_re[o + e + 1], r->Jr[l], a[0], b[0], c[0]);
+        orc_strain_coeffs(sin_ * xU[e] + r->dUn_im[o + e], sin_ * xU[e + 1] + r->dUn_im[o + e + 1],
+                          sin_ * xV[e] + r->dVn_im[o + e], sin_ * xV[e + 1] + r->dVn_im[o + e + 1], r->Jr[l], a[1], b[1], c[1]);
+        orc_strain_coeffs(sre * xU[e] + r->edUn_re[o + e], sre * xU[e + 1] + r->edUn_re[o + e + 1],
+                          sre * xV[e] + r->edVn_re[o + e], sre * xV[e + 1] + r->edVn_re[o + e + 1], r->Jr[l], a[2], b[2], c[2]);
+        orc_strain_coeffs(sim * xU[e] + r->edUn_im[o + e], sim * xU[e + 1] + r->edUn_im[o + e + 1],
+                          sim * xV[e] + r->edVn_im[o + e], sim * xV[e + 1] + r->edVn_im[o + e + 1], r->Jr[l], a[3], b[3], c[3]);
+        size_t mo = k * nb + (size_t)e * 4;
+        for (int lam = 0; lam < 4; lam++) {
+            cn[(0 * 4 + lam) * nlm + lm] = a[0][lam] + I * a[1][lam];
+            cn[(1 * 4 + lam) * nlm + lm] = b[0][lam] + I * b[1][lam];
+            cn[(2 * 4 + lam) * nlm + lm] = c[0][lam] + I * c[1][lam];
+            c1[(0 * 4 + lam) * nlm + lm] = a[2][lam] + I * a[3][lam];
+            c1[(1 * 4 + lam) * nlm + lm] = b[2][lam] + I * b[3][lam];
+            c1[(2 * 4 + lam) * nlm + lm] = c[2][lam] + I * c[3][lam];
+            cm[(0 * 4 + lam) * nlm + lm] = r->Are0[mo + lam] + I * r->Aim0[mo + lam];
+            cm[(1 * 4 + lam) * nlm + lm] = r->Bre0[mo + lam] + I * r->Bim0[mo + lam];
+            cm[(2 * 4 + lam) * nlm + lm] = r->Cre0[mo + lam] + I * r->Cim0[mo + lam];
+        }
+    }
+    const double* M3 = r->Mk3 + (size_t)e * ng;
+    for (int sh = 0; sh < 3; sh++) {
+        cplx* cc = cm + (size_t)sh * 4 * nlm;
+        orc_tsh_synth(&r->tsh, s, cc, dt0);
+        orc_tsh_synth(&r->tsh, s, cn + (size_t)sh * 4 * nlm, den);
+        orc_tsh_synth(&r->tsh, s, c1 + (size_t)sh * 4 * nlm, de1);
+        for (size_t id = 0; id < ng; id++) {
+            const double wt = M3[id] / (1.0 + 0.5 * M3[id]), we = r->mu[e] * M3[id] / (1.0 + 0.5 * M3[id]);
+            for (int p = 0; p < 6; p++)
+                dt0[p * ng + id] = -wt * dt0[p * ng + id] - we * (den[p * ng + id] + de1[p * ng + id]);
+        }
+        orc_tsh_analysis(&r->tsh, s, dt0, cdel);
+        for (int q = 0; q < 4 * nlm; q++) cc[q] = cc[q] + cdel[q];
+    }
+    for (int k = 0; k < r->nk; k++) {
+        int lm = r->k2lm[k];
+        size_t mo = k * nb + (size_t)e * 4;
+        for (int lam = 0; lam < 4; lam++) {
+            r->Are[mo + lam] = creal(cm[(0 * 4 + lam) * nlm + lm]); r->Aim[mo + lam] = cimag(cm[(0 * 4 + lam) * nlm + lm]);
+            r->Bre[mo + lam] = creal(cm[(1 * 4 + lam) * nlm + lm]); r->Bim[mo + lam] = cimag(cm[(1 * 4 + lam) * nlm + lm]);
+            r->Cre[mo + lam] = creal(cm[(2 * 4 + lam) * nlm + lm]); r->Cim[mo + lam] = cimag(cm[(2 * 4 + lam) * nlm + lm]);
+        }
+    }
+}
+
+/* trapezoid_advance_all (:1524-1596): every slot reset to tau_n and the
+ * 1-D-effective elements advanced spectrally, then the 3-D elements from
+ * the snapshot */
+static void trapezoid_advance_all(orc_resp* r, const orc_sht* s, const cplx* sigma) {
+    int nr = r->nr, ne = r->ne;
+    size_t nb = (size_t)4 * ne;
+    const cplx* sig_n = r->sigma_primed ? r->sigma_n : sigma;   /* first step: eps_n from sigma_{n+1} */
+#pragma omp parallel
+    {
+        double *Ur = malloc(nr * 8), *Ui = malloc(nr * 8), *Vr = malloc(nr * 8), *Vi = malloc(nr * 8);
+        double *Urn = malloc(nr * 8), *Uin = malloc(nr * 8), *Vrn = malloc(nr * 8), *Vin = malloc(nr * 8);
+#pragma omp for schedule(static)
+        for (int k = 0; k < r->nk; k++) {
+            int l = r->kdeg[k], lm = r->k2lm[k];
+            double sre = creal(sigma[lm]), sim = cimag(sigma[lm]);
+            double srn = creal(sig_n[lm]), sin_ = cimag(sig_n[lm]);
+            const double* xU = r->xUn + (size_t)(l - 1) * nr; const double* xV = r->xVn + (size_t)(l - 1) * nr;
+            size_t o = (size_t)k * nr;
+            for (int n = 0; n < nr; n++) {
+                Urn[n] = srn * xU[n] + r->dUn_re[o + n]; Vrn[n] = srn * xV[n] + r->dVn_re[o + n];
+                Uin[n] = sin_ * xU[n] + r->dUn_im[o + n]; Vin[n] = sin_ * xV[n] + r->dVn_im[o + n];
+                Ur[n] = sre * xU[n] + r->edUn_re[o + n]; Vr[n] = sre * xV[n] + r->edVn_re[o + n];
+                Ui[n] = sim * xU[n] + r->edUn_im[o + n]; Vi[n] = sim * xV[n] + r->edVn_im[o + n];
+            }
+            memcpy(r->Are + k * nb, r->Are0 + k * nb, nb * 8); memcpy(r->Bre + k * nb, r->Bre0 + k * nb, nb * 8);
+            memcpy(r->Cre + k * nb, r->Cre0 + k * nb, nb * 8);
+            orc_advance_memory_trap(ne, r->mu, r->Mk, Ur, Vr, Urn, Vrn, r->Jr[l], r->Are + k * nb, r->Bre + k * nb,
+                                    r->Cre + k * nb, r->active1d);
+            memcpy(r->Aim + k * nb, r->Aim0 + k * nb, nb * 8); memcpy(r->Bim + k * nb, r->Bim0 + k * nb, nb * 8);
+            memcpy(r->Cim + k * nb, r->Cim0 + k * nb, nb * 8);
+            orc_advance_memory_trap(ne, r->mu, r->Mk, Ui, Vi, Uin, Vin, r->Jr[l], r->Aim + k * nb, r->Bim + k * nb,
+                                    r->Cim + k * nb, r->active1d);
+        }
+        free(Ur); free(Ui); free(Vr); free(Vi); free(Urn); free(Uin); free(Vrn); free(Vin);
+    }
+    if (r->lat_visc && r->ne3d > 0) {
+        size_t ng = (size_t)s->nphi * s->nlat;
+        int nlm = s->nlm;
+#pragma omp parallel
+        {
+            cplx* cm = malloc((size_t)12 * nlm * sizeof(cplx));
+            cplx* cn = malloc((size_t)12 * nlm * sizeof(cplx));
+            cplx* c1 = malloc((size_t)12 * nlm * sizeof(cplx));
+            cplx* cdel = malloc((size_t)4 * nlm * sizeof(cplx));
+            double *dt0 = malloc(6 * ng * 8), *den = malloc(6 * ng * 8), *de1 = malloc(6 * ng * 8);
+#pragma omp for schedule(dynamic)
+            for (int ei = 0; ei < r->ne3d; ei++)
+                advance_element_3d_trap(r, s, sigma, sig_n, r->e3d[ei], cm, cn, c1, cdel, dt0, den, de1);
+            free(cm); free(cn); free(c1); free(cdel); free(dt0); free(den); free(de1);
+        }
+    }
+}
+
+/* ve_response_advance_endpoint (:1618-1646): FE one Maxwell update; TRAP one
+ * trapezoid pass from tau_n, the endpoint and report drift re-solved from the
+ * new tau_{n+1}, converged once the surface drift settles (pass >= 2) */
+void orc_resp_advance_endpoint(orc_resp* r, const orc_sht* s, const cplx* sigma) {
+    if (!r->implicit || r->kind != RESP_VE) {
+        orc_resp_advance(r, s, sigma);
+        r->couple_pass = 1;
+        r->couple_done = 1;
+        return;
+    }
+    trapezoid_advance_all(r, s, sigma);
+    memcpy(r->sigma_next, sigma, (size_t)r->nlm * sizeof(cplx));
+    r->have_next = 1;
+    resp_drift(r, r->edUn_re, r->edUn_im, r->edVn_re, r->edVn_im);
+    r->couple_pass++;
+    /* drift_change (:1499-1512), complex moduli */
+    double cnorm = 0.0, snorm = 0.0;
+    for (int k = 0; k < r->nk; k++) {
+        cnorm = fmax(cnorm, cabs(r->dUa[k] - r->dUa_prev[k]));
+        snorm = fmax(snorm, cabs(r->dUa[k]));
+    }
+    memcpy(r->dUa_prev, r->dUa, (size_t)r->nk * sizeof(cplx));
+    r->couple_done = r->couple_pass >= 2 && cnorm <= r->couple_tol * fmax(snorm, DBL_MIN);
+}
+
+/* ve_response_finalize_step (:1653-1663) */
+void orc_resp_finalize_step(orc_resp* r) {
+    if (r->implicit && r->have_next) {
+        memcpy(r->sigma_n, r->sigma_next, (size_t)r->nlm * sizeof(cplx));
+        r->sigma_primed = 1;
+    }
+    r->time += r->dt;
+}
+
+/* response_prime_sigma (:1838-1847) */
+void orc_resp_prime_sigma(orc_resp* r, const cplx* sigma) {
+    if (!r->sigma_n) return;
+    memcpy(r->sigma_n, sigma, (size_t)r->nlm * sizeof(cplx));
+    r->sigma_primed = 1;
+}
+
+/* response_stash_coarse (:1742-1757) */
+void orc_resp_stash_coarse(orc_resp* r) {
+    if (r->kind != RESP_VE) return;
+    size_t n = (size_t)4 * r->ne * r->nk;
+    memcpy(r->Are_c, r->Are, n * 8); memcpy(r->Aim_c, r->Aim, n * 8); memcpy(r->Bre_c, r->Bre, n * 8);
+    memcpy(r->Bim_c, r->Bim, n * 8); memcpy(r->Cre_c, r->Cre, n * 8); memcpy(r->Cim_c, r->Cim, n * 8);
+}
+
+/* response_coarse_fine_error (:1759-1788): max |tau_fine - tau_coarse| and
+ * max |tau_fine| over every slot, element and coefficient (NaN kept) */
+void orc_resp_coarse_fine_error(const orc_resp* r, double* err_inf, double* tau_inf) {
+    *err_inf = 0.0; *tau_inf = 0.0;
+    if (r->kind != RESP_VE) return;
+    size_t n = (size_t)4 * r->ne * r->nk;
+    const double* f[6] = {r->Are, r->Aim, r->Bre, r->Bim, r->Cre, r->Cim};
+    const double* c[6] = {r->Are_c, r->Aim_c, r->Bre_c, r->Bim_c, r->Cre_c, r->Cim_c};
+    double e = 0.0, t = 0.0;
+    for (int q = 0; q < 6; q++)
+        for (size_t i = 0; i < n; i++) {
+            e = nanmax(e, fabs(f[q][i] - c[q][i]));
+            t = nanmax(t, fabs(f[q][i]));
+        }
+    *err_inf = e; *tau_inf = t;
 }

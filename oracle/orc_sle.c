@@ -3,7 +3,8 @@
  * with migrating coastlines and grounded-ice flotation.
  *
  * TEST INFRASTRUCTURE ONLY (see orc.h). Restates src/vilma_sle.f90:124-405
- * (sle_solve, forward-Euler response branch) and :407-425 (ocean_function).
+ * (sle_solve, with the memory co-convergence passes of the implicit scheme)
+ * and :407-425 (ocean_function).
  */
 #include <float.h>
 #include <math.h>
@@ -33,6 +34,10 @@ void orc_sle_solve(orc_sle* sle, const orc_sht* s, orc_resp* resp, const double*
     for (int i = 0; i < ng; i++) ref[i] = ice[i] - d_ice[i];
     orc_ocean_function(ng, topo0, ref, C0);
     orc_resp_begin(resp);
+    /* the sea-level <-> memory co-convergence (src/vilma_sle.f90:218-366):
+     * FE converges in one pass; TRAP re-converges the load against each new
+     * tau_{n+1} estimate until the report drift settles */
+    if (!report_only) orc_resp_prepare_endpoint(resp);
     int n_mem = report_only ? 1 : sle->max_mem_iter;
     for (int im = 1; im <= n_mem; im++) {
         for (int io = 1; io <= sle->n_outer; io++) {
@@ -87,11 +92,11 @@ void orc_sle_solve(orc_sle* sle, const orc_sht* s, orc_resp* resp, const double*
         for (int i = 0; i < ng; i++) load[i] = ORC_RHO_ICE * d_ice_g[i] + ORC_RHO_WATER * (C[i] * rsl[i]) + wcorr[i];
         orc_sht_analysis(s, load, load_lm);
         if (report_only) break;
-        orc_resp_advance(resp, s, load_lm);   /* forward-Euler: converged after one pass */
+        orc_resp_advance_endpoint(resp, s, load_lm);
         res->n_couple_done = im;
-        break;
+        if (resp->couple_done) break;
     }
-    if (!report_only) resp->time += resp->dt;
+    if (!report_only) { orc_resp_finalize_step(resp); sle->n_couple_tot += res->n_couple_done; }
     if (sigma_out) memcpy(sigma_out, load_lm, nlm * sizeof(cplx));
     for (int i = 0; i < ng; i++) tmp[i] = C[i] * rsl[i];
     double Cs_int = orc_surface_integral(s, tmp) - zeta_int;
