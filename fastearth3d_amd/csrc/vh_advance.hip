@@ -25,6 +25,10 @@ struct advance_args {
     const int* e3of;       // [nem] -> 3-D element index or -1
     const double2* cdel;   // [i3*3 + shape][lambda][nlm]
     int nlm;
+    // trapezoidal rule: dN is the endpoint drift (eps_{n+1} with sigma), dN0
+    // the start-of-step drift (eps_n with sigma_n)
+    const double* sigma_n;
+    const double* dN0;
 };
 
 // Grid: x = tile of 64 slots (lanes, coalesced rows of tau), y = group of
@@ -32,6 +36,9 @@ struct advance_args {
 // moves 8 memory values (re/im x 4 channels) and the launch has thousands of
 // blocks. Each block stores its per-slot max |tau_{n+1}| and its overall max
 // (plain stores); k_advance_max combines them (max is exact in any order).
+// Trap: tau_{n+1} = [(1 - M/2) tau_n - mu M (eps_n + eps_{n+1})] / (1 + M/2)
+// (src/vilma_viscoelastic.f90:551-565) in place of forward Euler.
+template <bool Trap>
 __global__ void __launch_bounds__(256) k_advance(advance_args a) {
     __shared__ double red[4][64];
     const int lane = threadIdx.x & 63, wg = threadIdx.x >> 6;
@@ -71,7 +78,27 @@ __global__ void __launch_bounds__(256) k_advance(advance_args a) {
             strain_coeffs(U0r, U1r, V0r, V1r, Jr, s[0][0], s[0][1], s[0][2]);
             strain_coeffs(U0i, U1i, V0i, V1i, Jr, s[1][0], s[1][1], s[1][2]);
             const double Me = a.v.MkPerDt[e] * a.dt;   // response_set_dt: Mk = MkPerDt*dt
-            const double om = 1.0 - Me, two_muM = 2.0 * a.v.mu[e] * Me;
+            double om = 1.0 - Me, two_muM = 2.0 * a.v.mu[e] * Me;
+            if constexpr (Trap) {   // eps_n from (sigma_n, dN0), summed into s; weights of the rule
+                const double srn = a.sigma_n[2 * (k + 1)], sin_ = a.sigma_n[2 * (k + 1) + 1];
+                const double* p0 = a.dN0 + (size_t)n0 * 4 * nk + k;
+                const double* p1 = a.dN0 + (size_t)n1 * 4 * nk + k;
+                double t[2][3][4];
+                strain_coeffs(srn * xU[e] + p0[0], srn * xU[e + 1] + p1[0], srn * xV[e] + p0[(size_t)2 * nk],
+                              srn * xV[e + 1] + p1[(size_t)2 * nk], Jr, t[0][0], t[0][1], t[0][2]);
+                strain_coeffs(sin_ * xU[e] + p0[(size_t)nk], sin_ * xU[e + 1] + p1[(size_t)nk],
+                              sin_ * xV[e] + p0[(size_t)3 * nk], sin_ * xV[e + 1] + p1[(size_t)3 * nk], Jr,
+                              t[1][0], t[1][1], t[1][2]);
+#pragma unroll
+                for (int ri = 0; ri < 2; ri++)
+#pragma unroll
+                    for (int q = 0; q < 3; q++)
+#pragma unroll
+                        for (int lam = 0; lam < 4; lam++) s[ri][q][lam] += t[ri][q][lam];
+                const double den = 1.0 + 0.5 * Me;
+                om = (1.0 - 0.5 * Me) / den;
+                two_muM = a.v.mu[e] * Me / den;   // weight of eps_n + eps_{n+1}
+            }
 #pragma unroll
             for (int ri = 0; ri < 2; ri++)
 #pragma unroll
@@ -148,7 +175,7 @@ size_t vh_advance_scratch(const vh_dims& d) {
 void vh_launch_advance(hipStream_t st, const vh_dims& d, const vh_dev& v, double dt, const double* tau_in,
                        double* tau_out, const double* sigma, const double* dN, double* mnorm_out,
                        double* maxnorm_out, const int* e3of, const double2* cdel, double* scratch,
-                       double* maxn_copy) {
+                       double* maxn_copy, const double* sigma_n, const double* dN0) {
     if (d.nem == 0) {
         vh_fill(st, mnorm_out, 0.0, d.nk);
         vh_fill(st, maxnorm_out, 0.0, 1);
@@ -159,10 +186,12 @@ void vh_launch_advance(hipStream_t st, const vh_dims& d, const vh_dev& v, double
     a.e3of = e3of; a.cdel = cdel; a.nlm = d.nlm;
     a.nr = d.nr; a.nk = d.nk; a.nem = d.nem; a.dt = dt; a.v = v;
     a.tau_in = tau_in; a.tau_out = tau_out; a.sigma = sigma; a.dN = dN;
+    a.sigma_n = sigma_n; a.dN0 = dN0;
     dim3 g((d.nk + 63) / 64, (3 * d.nem + 3) / 4);
     a.pmax = scratch;
     a.pblk = scratch + (size_t)g.y * d.nk;
-    k_advance<<<g, 256, 0, st>>>(a);
+    if (sigma_n) k_advance<true><<<g, 256, 0, st>>>(a);
+    else k_advance<false><<<g, 256, 0, st>>>(a);
     HIPCHECK(hipGetLastError());
     // blocks 0 .. g.x-1: 64 slots each (threads 64..255 of them idle); the last: the global maximum
     k_advance_max<<<g.x + 1, 256, 0, st>>>(d.nk, (int)g.y, (int)(g.x * g.y), a.pmax, a.pblk, mnorm_out, maxnorm_out,
@@ -199,5 +228,75 @@ __global__ void __launch_bounds__(256) k_memnorm(int nk, int nem, const double* 
 void vh_launch_memnorm(hipStream_t st, const vh_dims& d, const double* tau, double* mnorm, double* maxnorm) {
     vh_fill(st, maxnorm, 0.0, 1);
     k_memnorm<<<(d.nk + 63) / 64, 256, 0, st>>>(d.nk, d.nem, tau, mnorm, (unsigned long long*)maxnorm);
+    HIPCHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------ TRAP norms
+// Maxima of non-negative values through atomicMax on their bit patterns
+// (the order of non-negative doubles is that of their bits; a NaN from fabs
+// sorts above +inf, so it is kept). out must start at zero.
+__device__ __forceinline__ void max_bits(unsigned long long* out, double v) {
+    atomicMax(out, (unsigned long long)__double_as_longlong(v));
+}
+
+__device__ __forceinline__ double block_max(double v, double* red) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = vh_nanmax(v, __shfl_xor(v, off));
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double m = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = vh_nanmax(m, red[i]);
+    return m;
+}
+
+// response_coarse_fine_error (src/vilma_response.f90:1759-1788): out[0] =
+// max |fine - coarse|, out[1] = max |fine| over n values
+__global__ void __launch_bounds__(256) k_coarse_fine(size_t n, const double* __restrict__ fine,
+                                                     const double* __restrict__ coarse, unsigned long long* out) {
+    __shared__ double red[8];
+    double e = 0.0, t = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double f = fine[i];
+        e = vh_nanmax(e, fabs(f - coarse[i]));
+        t = vh_nanmax(t, fabs(f));
+    }
+    e = block_max(e, red);
+    __syncthreads();
+    t = block_max(t, red + 4);
+    if (threadIdx.x == 0) { max_bits(out, e); max_bits(out + 1, t); }
+}
+
+// drift_change (src/vilma_response.f90:1499-1512) of the report drift dUa
+// (the first nk complex values of dsurf): out[0] = max |dUa - prev|, out[1]
+// = max |dUa|; then prev <- dUa (keep_drift)
+__global__ void __launch_bounds__(256) k_drift_change(int nk, const double2* __restrict__ dsurf, double2* prev,
+                                                      unsigned long long* out) {
+    __shared__ double red[8];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double c = 0.0, s = 0.0;
+    if (k < nk) {
+        const double2 u = dsurf[k], p = prev[k];
+        c = hypot(u.x - p.x, u.y - p.y);
+        s = hypot(u.x, u.y);
+        prev[k] = u;
+    }
+    c = block_max(c, red);
+    __syncthreads();
+    s = block_max(s, red + 4);
+    if (threadIdx.x == 0) { max_bits(out, c); max_bits(out + 1, s); }
+}
+
+void vh_coarse_fine(hipStream_t st, size_t n, const double* fine, const double* coarse, double* out2) {
+    vh_fill(st, out2, 0.0, 2);
+    const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 2048);
+    k_coarse_fine<<<std::max(nb, 1u), 256, 0, st>>>(n, fine, coarse, (unsigned long long*)out2);
+    HIPCHECK(hipGetLastError());
+}
+
+void vh_drift_change(hipStream_t st, int nk, const double* dsurf, double* prev, double* out2) {
+    vh_fill(st, out2, 0.0, 2);
+    k_drift_change<<<(nk + 255) / 256, 256, 0, st>>>(nk, (const double2*)dsurf, (double2*)prev,
+                                                     (unsigned long long*)out2);
     HIPCHECK(hipGetLastError());
 }

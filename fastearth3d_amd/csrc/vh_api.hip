@@ -222,6 +222,17 @@ struct vh_model {
     int sle_late_blocks = 0;     // diagnostic (VH_SLE_LATE_BLOCKS): solve blocks that start late
     int sle_blocks = 0;          // blocks of the fused sea-level solve (0: one per work item)
     unsigned sle_late_ticks = 0;
+    // trapezoidal memory scheme (scheme "trap" with earth_response "ve"):
+    // controller knobs and counters, the start-of-step load sigma_n, the
+    // endpoint drift, the previous report drift, and the controller's saved
+    // (A) and coarse (B) memory
+    bool trap = false;
+    int max_mem_iter = 20;
+    double rtol = 1e-4, atol = 1e-3, safety = 0.9, grow_max = 5.0, shrink_min = 0.2, dt_try = 0.0;
+    long n_floor = 0, n_couple_tot = 0;
+    bool sig_primed = false, sig_primed_s = false;
+    double time_resp_s = 0.0;
+    double *d_sig_n = 0, *d_sig_n_s = 0, *d_edN = 0, *d_dua_prev = 0, *d_tau_s = 0, *d_tau_c = 0, *d_mnorm_s = 0;
     // kernel timing (HIP events on the model's stream), per kernel class
     bool prof = false;
     struct evpair { hipEvent_t a, b; int cls; };
@@ -292,7 +303,8 @@ vh_model::~vh_model() {
                     d_Cn, d_dig, d_C0s, d_ice, d_dice, d_wcorr, d_sraw, d_srot, d_zbed, d_gtmp, d_F, d_FC, d_F2,
                     d_sigma, d_lm, d_uN, d_part, d_sc, d_sbar, d_sc2, d_cphi, d_sphi,
                     ra.rs.mem, ra.rs.MkPerDt, ra.rs.xU, ra.rs.xV, ra.rs.dN, ra.rs.sc, d_emap2, d_mnodes2,
-                    d_mnode2, d_G2, d_e3em, d_e3of, d_rate3, d_n6, d_X, d_Fd, d_cdel};
+                    d_mnode2, d_G2, d_e3em, d_e3of, d_rate3, d_n6, d_X, d_Fd, d_cdel,
+                    d_sig_n, d_sig_n_s, d_edN, d_dua_prev, d_tau_s, d_tau_c, d_mnorm_s};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : evpool) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
@@ -305,7 +317,8 @@ namespace {
 void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_bed_eq, const double* h_ice_eq,
            const double* h_ice_init) {
     h->par = *p;
-    if (strcmp(p->scheme, "fe")) throw vh_error(std::string("scheme \"") + p->scheme + "\" not supported (fe only)");
+    if (strcmp(p->scheme, "fe") && strcmp(p->scheme, "trap"))
+        throw vh_error(std::string("scheme \"") + p->scheme + "\" not supported (fe|trap)");
     if (p->l_toroidal) throw vh_error("l_toroidal = .true. is not supported");
     if ((p->mmax >= 0 && p->mmax != p->lmax) || p->mres != 1) throw vh_error("mmax/mres other than lmax/1 not supported");
     if (p->lmax < 2) throw vh_error("lmax must be >= 2");
@@ -521,6 +534,11 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
     h->n_outer = p->sle_n_outer; h->n_inner = p->sle_n_inner; h->tol = p->sle_tol;
     h->fixed_ocean = p->sle_fixed_ocean; h->subgrid = p->sle_subgrid;
     h->cfl = p->cfl; h->dt_min = p->dt_min; h->dt_max = p->dt_max;
+    // the scheme reaches the response only for ve (src/vilma_coupling.f90:269-281)
+    h->trap = h->kind == 2 && !strcmp(p->scheme, "trap");
+    h->max_mem_iter = std::max(1, p->sle_max_mem_iter);
+    h->rtol = p->rtol; h->atol = p->atol; h->safety = p->safety; h->grow_max = p->grow_max;
+    h->shrink_min = p->shrink_min; h->dt_try = p->dt_init;
 
     // ---- rotation (rotation_init, src/vilma_rotation.f90:114-152)
     if (p->rotation && h->kind >= 1) {
@@ -689,7 +707,7 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
     // sigma is the closing load transform; then the memory advance
     int nxt = h->cur ^ 1;
     if (!report_only && h->kind == 2) advance_memory(h, (const double*)h->d_sigma, nxt, r + 11);
-    if (!report_only) h->time_resp += h->dt;
+    if (!report_only) { h->time_resp += h->dt; h->n_couple_tot++; }   // forward Euler: one pass
     // r[10] = int C rsl (from the solve) for the mass diagnostic, r[11] the new memory norm
     if (report_only && h->kind == 2)
         HIPCHECK(hipMemcpyAsync(r + 11, h->d_maxn + h->cur, sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -709,6 +727,221 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
     return res;
 }
 
+// ------------------------------------------------------------------ TRAP
+// The trapezoidal (Crank-Nicolson) memory scheme: the sea-level <->
+// memory co-convergence of one solve (src/vilma_sle.f90:218-366,
+// ve_response_prepare/advance_endpoint src/vilma_response.f90:1598-1646)
+// and the step-doubling controller (src/vilma_timestep.f90:229-307).
+// tau_n stays in d_tau[cur] for the whole solve; every pass writes its
+// tau_{n+1} estimate into d_tau[cur ^ 1] and the endpoint drift of that
+// estimate into d_edN / d_dsurf; the solve commits the last estimate.
+void trap_alloc(vh_model* h) {
+    if (h->d_tau_s) return;
+    const vh_dims& d = h->d;
+    hipStream_t st = h->st;
+    const size_t nm = (size_t)24 * std::max(d.nem, 1) * d.nk, ndn = (size_t)4 * std::max(d.nmn, 1) * d.nk;
+    h->d_tau_s = dalloc<double>(nm); h->d_tau_c = dalloc<double>(nm);
+    h->d_mnorm_s = dalloc<double>((size_t)d.nk + 1);
+    h->d_edN = dalloc<double>(ndn);
+    h->d_dua_prev = dalloc<double>((size_t)2 * d.nk);
+    h->d_sig_n = dalloc<double>((size_t)2 * d.nlm); h->d_sig_n_s = dalloc<double>((size_t)2 * d.nlm);
+    vh_fill(st, h->d_sig_n, 0.0, (size_t)2 * d.nlm);
+}
+
+void d2d(double* dst, const double* src, size_t n, hipStream_t st);
+
+// trapezoid_advance_all (src/vilma_response.f90:1524-1596): 3-D elements by
+// the tensor transforms of tau_n + mu (eps_n + eps_{n+1}) weighted by
+// -M/(1 + M/2), then every Maxwell element's tau_{n+1} in one pass
+void trap_advance(vh_model* h, const double* sigma_n) {
+    hipStream_t st = h->st;
+    const vh_dims& d = h->d;
+    const int cur = h->cur, nxt = cur ^ 1;
+    const double* sig1 = (const double*)h->d_sigma;
+    if (h->lat_visc && h->ne3d > 0) {
+        const size_t ng = (size_t)d.nphi * d.nlat;
+        h->prof_begin(VH_K_T3_GATHER);
+        vh_t3_gather_trap(st, d, h->v, h->d_e3em, h->ne3d, h->d_tau[cur], sigma_n, h->d_dN, sig1, h->d_edN, h->d_X);
+        h->prof_end();
+        for (int b0 = 0; b0 < h->nbat; b0 += h->tchunk) {
+            const int nb = std::min(h->tchunk, h->nbat - b0);
+            h->prof_begin(VH_K_TSH_SYNTH);
+            vh_tsh_synth(st, h->s, h->d_X + (size_t)b0 * 4 * d.nlm, nb, h->d_Fd);
+            h->prof_end();
+            h->prof_begin(VH_K_T3_ROWS);
+            vh_t3_rows(st, h->s, h->d_Fd, nb, 3, h->d_rate3 + (size_t)(b0 / 3) * ng, 3, h->dt, nullptr);
+            h->prof_end();
+            h->prof_begin(VH_K_TSH_ANALYSIS);
+            vh_tsh_analysis(st, h->s, h->d_Fd, nb, h->d_n6, h->d_cdel + (size_t)b0 * 4 * d.nlm);
+            h->prof_end();
+        }
+    }
+    h->prof_begin(VH_K_ADVANCE);
+    vh_launch_advance(st, d, h->v, h->dt, h->d_tau[cur], h->d_tau[nxt], sig1, h->d_edN, h->d_mnorm[nxt],
+                      h->d_maxn + nxt, h->d_e3of, h->d_cdel, h->d_advmax, nullptr, sigma_n, h->d_dN);
+    h->prof_end();
+}
+
+// sle_solve with the co-convergence passes: each pass one fused sea-level
+// solve against the current report drift, the trapezoid advance with its
+// closing load, the endpoint (and report) drift of the new estimate and the
+// drift change; done once the drift settles (pass >= 2) or after
+// max_mem_iter passes. Commits tau_{n+1}, sigma_n and the response time.
+sle_res sle_solve_trap(vh_model* h, const double* ice0, const double* ice1, double frac, const double* srot) {
+    hipStream_t st = h->st;
+    const vh_sht& s = h->s;
+    const vh_dims& d = h->d;
+    const int n = (int)h->ng();
+    double* r = h->d_sc + 16;
+    const int cur = h->cur, nxt = cur ^ 1;
+    sle_res res;
+    sle_ice(st, n, ice0, ice1, frac, 1, h->d_heq, h->d_topo0, h->d_ice, h->d_dice, h->d_C0s, h->d_wcorr, r, 12,
+            h->d_sbar, sle_solve_grid(s), h->n_outer);
+    // response_begin_step: the drift of tau_n; prepare_endpoint: the endpoint
+    // drift starts as it, the report drift is kept for the change test
+    h->prof_begin(VH_K_DRIFT);
+    vh_drift_run(st, h->dplan, d, h->v, h->d_tau[cur], h->d_mnorm[cur], h->d_maxn + cur, h->skip_tol, h->deg1_cm,
+                 h->d_dN, h->d_dsurf);
+    h->prof_end();
+    d2d(h->d_edN, h->d_dN, (size_t)4 * std::max(d.nmn, 1) * d.nk, st);
+    d2d(h->d_dua_prev, h->d_dsurf, (size_t)2 * d.nk, st);
+    // sigma_n: the committed start-of-step load, or (not yet primed) this
+    // pass's own load (src/vilma_response.f90:1553-1558)
+    const double* sigma_n = h->sig_primed ? h->d_sig_n : (const double*)h->d_sigma;
+    sle_solve_args ia{};
+    ia.s = s; ia.topo0 = h->d_topo0; ia.ice = h->d_ice; ia.dice = h->d_dice; ia.C0 = h->d_C0s; ia.srot = srot;
+    ia.gu = h->d_gu; ia.gn = h->d_gn; ia.dsurf = h->d_dsurf;
+    ia.C = h->d_C; ia.Cn = h->d_Cn; ia.dig = h->d_dig; ia.wcorr = h->d_wcorr; ia.sraw = h->d_sraw;
+    ia.nk = d.nk; ia.kind = h->kind; ia.subgrid = h->subgrid; ia.n_inner = h->n_inner; ia.n_outer = h->n_outer;
+    ia.fixed_ocean = h->fixed_ocean; ia.g = h->g; ia.tol = h->tol; ia.rsl = h->d_rsl;
+    ia.part = h->d_part; ia.cs = h->d_part + 3 * (size_t)s.nlat; ia.part3 = h->d_part + 4 * (size_t)s.nlat;
+    ia.res = r; ia.F = h->d_F; ia.FC = h->d_FC; ia.F2 = h->d_F2; ia.sigma = h->d_sigma; ia.bar = h->d_sbar;
+    ia.late_blocks = h->sle_late_blocks; ia.late_ticks = h->sle_late_ticks; ia.blocks = h->sle_blocks;
+    int pass = 0;
+    const double* hr = h->h_pin;
+    for (int im = 1; im <= h->max_mem_iter; im++) {
+        // the same ice fields again and a reset of the result slots and the
+        // synchronisation area for the next launch
+        if (im > 1)
+            sle_ice(st, n, ice0, ice1, frac, 1, h->d_heq, h->d_topo0, h->d_ice, h->d_dice, h->d_C0s, h->d_wcorr, r,
+                    12, h->d_sbar, sle_solve_grid(s), h->n_outer);
+        h->prof_begin(VH_K_SLE);
+        sle_solve_launch(st, ia);
+        h->prof_end();
+        trap_advance(h, sigma_n);
+        h->prof_begin(VH_K_DRIFT);
+        vh_drift_run(st, h->dplan, d, h->v, h->d_tau[nxt], h->d_mnorm[nxt], h->d_maxn + nxt, h->skip_tol,
+                     h->deg1_cm, h->d_edN, h->d_dsurf);
+        h->prof_end();
+        vh_drift_change(st, d.nk, h->d_dsurf, h->d_dua_prev, r + 12);
+        h->read(r, 14);
+        if (hr[2] != 0.0) throw vh_error("sle_solve: a wait of the fused sea-level solve exceeded its bound");
+        h->n_sle_steal += (long)hr[8];
+        h->n_inner_tot += (long)hr[4];
+        h->n_outer_tot += (long)hr[5];
+        pass++;
+        // couple_tol = 1e-6 (src/vilma_response.f90:177)
+        if (pass >= 2 && hr[12] <= 1.0e-6 * std::max(hr[13], DBL_MIN)) break;
+    }
+    h->n_couple_tot += pass;
+    // ve_response_finalize_step: sigma_{n+1} becomes sigma_n; time advances
+    d2d(h->d_sig_n, (const double*)h->d_sigma, (size_t)2 * d.nlm, st);
+    h->sig_primed = true;
+    h->time_resp += h->dt;
+    h->cur = nxt;
+    const double ice_int = -(VH_RHO_ICE / VH_RHO_WATER) * hr[6];
+    const double zeta_int = h->subgrid ? hr[7] : 0.0;
+    const double Cs = hr[10] - zeta_int;
+    res.mass_resid = std::fabs(ice_int) > 0.0 ? std::fabs(Cs - ice_int) / std::fabs(ice_int) : std::fabs(Cs);
+    res.n_last = hr[0];
+    res.c_int = hr[3];
+    return res;
+}
+
+// response_save_state / response_restore_state (buffer A: tau_n, its slot
+// maxima, sigma_n, the response time)
+void trap_save(vh_model* h) {
+    const vh_dims& d = h->d;
+    const size_t nm = (size_t)24 * std::max(d.nem, 1) * d.nk;
+    d2d(h->d_tau_s, h->d_tau[h->cur], nm, h->st);
+    d2d(h->d_mnorm_s, h->d_mnorm[h->cur], d.nk, h->st);
+    d2d(h->d_mnorm_s + d.nk, h->d_maxn + h->cur, 1, h->st);
+    d2d(h->d_sig_n_s, h->d_sig_n, (size_t)2 * d.nlm, h->st);
+    h->sig_primed_s = h->sig_primed;
+    h->time_resp_s = h->time_resp;
+}
+void trap_restore(vh_model* h) {
+    const vh_dims& d = h->d;
+    const size_t nm = (size_t)24 * std::max(d.nem, 1) * d.nk;
+    d2d(h->d_tau[h->cur], h->d_tau_s, nm, h->st);
+    d2d(h->d_mnorm[h->cur], h->d_mnorm_s, d.nk, h->st);
+    d2d(h->d_maxn + h->cur, h->d_mnorm_s + d.nk, 1, h->st);
+    d2d(h->d_sig_n, h->d_sig_n_s, (size_t)2 * d.nlm, h->st);
+    h->sig_primed = h->sig_primed_s;
+    h->time_resp = h->time_resp_s;
+}
+
+// stepper_advance, implicit branch (src/vilma_timestep.f90:229-307)
+void stepper_trap(vh_model* h, double t0, double t1, const double* srot) {
+    const double span = t1 - t0;
+    const int n = (int)h->ng();
+    const vh_dims& d = h->d;
+    hipStream_t st = h->st;
+    trap_alloc(h);
+    if (!h->sig_primed) {   // sigma_0 from a report-only solve with the entering memory
+        sle_solve(h, h->d_hice, h->d_hice, 0.0, 0, true, srot);
+        d2d(h->d_sig_n, (const double*)h->d_sigma, (size_t)2 * d.nlm, st);
+        h->sig_primed = true;
+    }
+    const int p = 2;   // scheme_order(trap)
+    const double ricfac = (double)((1 << p) - 1), expo = 1.0 / (double)(p + 1);
+    const size_t nm = (size_t)24 * std::max(d.nem, 1) * d.nk;
+    double t = t0;
+    if (h->dt_try <= 0.0) h->dt_try = span;
+    auto solve_at = [&](double t_eval) {
+        h->n_solve++;
+        sle_res r = sle_solve_trap(h, h->d_hice, h->d_ice1, (t_eval - t0) / span, srot);
+        h->worst_mass_resid = std::max(h->worst_mass_resid, r.mass_resid);
+    };
+    for (;;) {
+        if (t >= t1 - 1.0e-9 * span) break;
+        const double dt = std::min(h->dt_try, t1 - t);
+        const bool at_floor = h->dt_min > 0.0 && dt <= h->dt_min * (1.0 + 1.0e-9);
+        HIPCHECK(hipMemcpyAsync(h->d_rsl_n, h->d_rsl, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        trap_save(h);
+        h->dt = dt;
+        solve_at(t + dt);                                        // coarse: one dt
+        d2d(h->d_tau_c, h->d_tau[h->cur], nm, st);               // buffer B
+        trap_restore(h);
+        HIPCHECK(hipMemcpyAsync(h->d_rsl, h->d_rsl_n, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        h->dt = 0.5 * dt;
+        solve_at(t + 0.5 * dt);                                  // fine sub-step 1
+        solve_at(t + dt);                                        // fine sub-step 2
+        vh_coarse_fine(st, nm, h->d_tau[h->cur], h->d_tau_c, h->d_sc + 30);
+        h->read(h->d_sc + 30, 2);
+        const double err_inf = h->h_pin[0], tau_inf = h->h_pin[1];
+        h->dt = dt;
+        const double errsc = (err_inf / ricfac) / (h->atol + h->rtol * tau_inf);
+        if (!(errsc <= DBL_MAX))
+            throw vh_error("vilma_timestep: implicit local-error estimate went non-finite "
+                           "(viscosity too stiff, or the state has already diverged)");
+        const bool accept = errsc <= 1.0 || at_floor;
+        if (accept) {
+            t = t + dt;
+            h->n_accept++;
+            if (at_floor && errsc > 1.0) h->n_floor++;
+        } else {
+            trap_restore(h);
+            HIPCHECK(hipMemcpyAsync(h->d_rsl, h->d_rsl_n, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+            h->n_reject++;
+        }
+        double fac = errsc <= 0.0 ? h->grow_max : h->safety * std::pow(errsc, -expo);
+        fac = std::min(h->grow_max, std::max(h->shrink_min, fac));
+        h->dt_try = std::min(h->dt_max, std::max(h->dt_min, dt * fac));
+        if (h->dt_try <= 1.0e-12 * span) throw vh_error("vilma_timestep: implicit adaptive step collapsed (tolerance unreachable)");
+    }
+}
+
 // stepper_advance, forward-Euler branch (src/vilma_timestep.f90:79-227)
 void stepper_advance(vh_model* h, double t0, double t1, const double* srot) {
     const double span = t1 - t0;
@@ -717,6 +950,10 @@ void stepper_advance(vh_model* h, double t0, double t1, const double* srot) {
     if (span <= 0.0) {
         sle_res r = sle_solve(h, h->d_hice, h->d_hice, 0.0, 0, true, srot);
         h->worst_mass_resid = r.mass_resid;
+        return;
+    }
+    if (h->trap) {
+        stepper_trap(h, t0, t1, srot);
         return;
     }
     double rate = h->kind == 2 ? h->rate_max : 0.0;
@@ -969,6 +1206,7 @@ void adj_check(const vh_model* h) {
 void forward_record(vh_model* h, int ncalls, const double* ice_hist, bool dev, double dt_yr, int every,
                     double* rsl_out, bool rsl_dev) {
     adj_check(h);
+    if (h->trap) throw vh_error("vh_forward_record: the reverse sweep differentiates scheme fe only");
     if (ncalls <= 0) throw vh_error("vh_forward_record: ncalls must be positive");
     if (!(dt_yr > 0.0)) throw vh_error("vh_forward_record: dt_yr must be positive");
     if (!h->adj) h->adj = new adj_ctx();
@@ -1429,6 +1667,8 @@ int vh_get_scalar(vh_model* h, int which, double* out) {
             case VH_S_N_INNER_TOT: *out = (double)h->n_inner_tot; break;
             case VH_S_N_OUTER_TOT: *out = (double)h->n_outer_tot; break;
             case VH_S_N_SLE_STEAL: *out = (double)h->n_sle_steal; break;
+            case VH_S_N_FLOOR: *out = (double)h->n_floor; break;
+            case VH_S_N_COUPLE_TOT: *out = (double)h->n_couple_tot; break;
             case VH_S_POLAR_M_RE:
             case VH_S_POLAR_M_IM:
                 if (!h->rot_on) { *out = 0.0; break; }

@@ -98,8 +98,16 @@ void vh_launch_setup_deg1_op(hipStream_t st, const vh_dims& d, const int* mnodes
 void vh_launch_advance(hipStream_t st, const vh_dims& d, const vh_dev& v, double dt, const double* tau_in,
                        double* tau_out, const double* sigma, const double* dN, double* mnorm_out,
                        double* maxnorm_out, const int* e3of, const double2* cdel, double* scratch,
-                       double* maxn_copy = nullptr);   // maxn_copy: a second place for the global max
+                       double* maxn_copy = nullptr,   // maxn_copy: a second place for the global max
+                       // trapezoidal rule (sigma_n non-null): dN is the endpoint drift and
+                       // (sigma_n, dN0) give the start-of-step strain
+                       const double* sigma_n = nullptr, const double* dN0 = nullptr);
 size_t vh_advance_scratch(const vh_dims& d);
+// out2[0] = max |fine - coarse|, out2[1] = max |fine| over n values (NaN kept)
+void vh_coarse_fine(hipStream_t st, size_t n, const double* fine, const double* coarse, double* out2);
+// out2[0] = max |dUa - prev|, out2[1] = max |dUa| over the nk complex dUa at
+// the head of dsurf; then prev <- dUa
+void vh_drift_change(hipStream_t st, int nk, const double* dsurf, double* prev, double* out2);
 void vh_launch_memnorm(hipStream_t st, const vh_dims& d, const double* tau, double* mnorm, double* maxnorm);
 
 // ---------------------------------------------------------------- SHT
@@ -169,11 +177,17 @@ void vh_tsh_synth(hipStream_t st, const vh_sht& s, const double2* X, int nb, dou
 // Fd -> C [nb][4][nlm]; n6 == null leaves the raw spin-2 adjoint in channel 3
 void vh_tsh_analysis(hipStream_t st, const vh_sht& s, const double2* Fd, int nb, const double* n6, double2* C);
 // mode 0: Fd <- FFT(-rate*dt * IFFT(Fin)), rate [nb/bpe][nlat][nphi] (Fin null: Fd, in place);
-// mode 1: grid [nb][6][nlat][nphi] <- IFFT(Fin); mode 2: Fd <- FFT(grid)
+// mode 1: grid [nb][6][nlat][nphi] <- IFFT(Fin); mode 2: Fd <- FFT(grid);
+// mode 3: as mode 0 with the trapezoidal weight M / (1 + M/2), M = rate*dt
 void vh_t3_rows(hipStream_t st, const vh_sht& s, double2* Fd, int nb, int mode, const double* rate, int bpe,
                 double dt, double* grid, const double2* Fin = nullptr);
 void vh_t3_gather(hipStream_t st, const vh_dims& d, const vh_dev& v, const int* e3em, int ne3d, const double* tau,
                   const double* sigma, const double* dN, double2* X);
+// trapezoidal rule: X = tau_n + mu (eps_n + eps_{n+1}), eps_n from
+// (sigma_n, dN0), eps_{n+1} from (sigma, dN1)
+void vh_t3_gather_trap(hipStream_t st, const vh_dims& d, const vh_dev& v, const int* e3em, int ne3d,
+                       const double* tau, const double* sigma_n, const double* dN0, const double* sigma,
+                       const double* dN1, double2* X);
 void vh_tsh_calibrate_n6(hipStream_t st, const vh_sht& s, double* n6_dev);
 
 // ---------------------------------------------------------------- drift plan

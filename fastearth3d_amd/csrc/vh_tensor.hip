@@ -344,6 +344,8 @@ __global__ void __launch_bounds__(256) k_tsh_analysis(vh_sht s, const double2* _
 //           element), forward FFT, in place in Fd
 //   mode 1: inverse FFT to grid[b][dy][j][i]
 //   mode 2: forward FFT of grid[b][dy][j][i] into Fd
+//   mode 3: as mode 0 with -M/(1 + M/2), M = rate*dt (trapezoidal rule,
+//           src/vilma_response.f90:1447-1478, in increment form)
 __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, double2* Fd, int nb, int mode,
                                                  const double* __restrict__ rate, int bpe, double dt,
                                                  double* __restrict__ grid) {
@@ -363,9 +365,12 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, d
     const double2* I1 = Fin + ((size_t)b * 6 + d1) * (L + 1) * nlat + j;
     double2* F0 = Fd + ((size_t)b * 6 + d0) * (L + 1) * nlat + j;
     double2* F1 = Fd + ((size_t)b * 6 + d1) * (L + 1) * nlat + j;
-    if (mode == 0) {
+    if (mode == 0 || mode == 3) {
         const double* rr = rate + ((size_t)(b / bpe) * nlat + j) * N;
-        for (int i = threadIdx.x; i < N; i += blockDim.x) Mrow[i] = rr[i] * dt;
+        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+            const double M = rr[i] * dt;
+            Mrow[i] = mode == 3 ? M / (1.0 + 0.5 * M) : M;   // trapezoidal weight
+        }
     }
     double2* z;
     if (mode != 2) {
@@ -461,6 +466,62 @@ __global__ void __launch_bounds__(256) k_t3_gather(int nr, int nk, int nlm, vh_d
         }
 }
 
+// X[(i3*3 + s)*4 + lam][k+1] = tau_s + mu (eps_n,s + eps_{n+1},s) for the
+// trapezoidal rule (gather_tensor_coeffs_trap, src/vilma_response.f90:1371-1445;
+// the rule's increment -M/(1+M/2) [tau_n + mu (eps_n + eps_{n+1})] is
+// linear in this one tensor, so the three syntheses of the reference become one)
+__global__ void __launch_bounds__(256) k_t3_gather_trap(int nr, int nk, int nlm, vh_dev v, const int* __restrict__ e3em,
+        const double* __restrict__ tau, const double* __restrict__ sigma_n, const double* __restrict__ dN0,
+        const double* __restrict__ sigma, const double* __restrict__ dN1, double2* __restrict__ X) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, i3 = blockIdx.y;
+    double2* Xb = X + (size_t)i3 * 12 * nlm;
+    if (k == 0)
+        for (int q = 0; q < 12; q++) Xb[(size_t)q * nlm] = make_double2(0.0, 0.0);
+    if (k >= nk) return;
+    const int em = e3em[i3], e = v.emap[em];
+    const int l = v.kdeg[k];
+    const double Jr = (double)l * (double)(l + 1);
+    const double* xU = v.xUn + (size_t)(l - 1) * nr;
+    const double* xV = v.xVn + (size_t)(l - 1) * nr;
+    const int n0 = v.mnode[e], n1 = v.mnode[e + 1];
+    double sc[2][3][4];
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+        for (int sh = 0; sh < 3; sh++)
+#pragma unroll
+            for (int lam = 0; lam < 4; lam++) sc[q][sh][lam] = 0.0;
+#pragma unroll
+    for (int w = 0; w < 2; w++) {   // eps_n, then eps_{n+1}
+        const double* sg = w == 0 ? sigma_n : sigma;
+        const double* dN = w == 0 ? dN0 : dN1;
+        const double sre = sg[2 * (k + 1)], sim = sg[2 * (k + 1) + 1];
+        const double* q0 = dN + (size_t)n0 * 4 * nk + k;
+        const double* q1 = dN + (size_t)n1 * 4 * nk + k;
+        double t[2][3][4];
+        strain_coeffs(sre * xU[e] + q0[0], sre * xU[e + 1] + q1[0], sre * xV[e] + q0[(size_t)2 * nk],
+                      sre * xV[e + 1] + q1[(size_t)2 * nk], Jr, t[0][0], t[0][1], t[0][2]);
+        strain_coeffs(sim * xU[e] + q0[(size_t)nk], sim * xU[e + 1] + q1[(size_t)nk], sim * xV[e] + q0[(size_t)3 * nk],
+                      sim * xV[e + 1] + q1[(size_t)3 * nk], Jr, t[1][0], t[1][1], t[1][2]);
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int sh = 0; sh < 3; sh++)
+#pragma unroll
+                for (int lam = 0; lam < 4; lam++) sc[q][sh][lam] += t[q][sh][lam];
+    }
+    const double mu = v.mu[e];
+    const double* t = tau + (size_t)em * 24 * nk + k;
+#pragma unroll
+    for (int sh = 0; sh < 3; sh++)
+#pragma unroll
+        for (int lam = 0; lam < 4; lam++) {
+            double re = t[(size_t)((sh * 2 + 0) * 4 + lam) * nk] + mu * sc[0][sh][lam];
+            double im = t[(size_t)((sh * 2 + 1) * 4 + lam) * nk] + mu * sc[1][sh][lam];
+            Xb[(size_t)(sh * 4 + lam) * nlm + k + 1] = make_double2(re, im);
+        }
+}
+
 // ------------------------------------------------------------- host side
 // batches per block of the tensor Legendre kernels (table reuse vs registers)
 #ifndef VH_TSH_TB_SYNTH
@@ -497,6 +558,15 @@ void vh_t3_rows(hipStream_t st, const vh_sht& s, double2* Fd, int nb, int mode, 
                 double dt, double* grid, const double2* Fin) {
     if (nb <= 0) return;
     k_t3_rows<<<xcd_grid(nb * s.nlat), 192, rows_lds(s), st>>>(s, Fin ? Fin : Fd, Fd, nb, mode, rate, bpe, dt, grid);
+    HIPCHECK(hipGetLastError());
+}
+
+void vh_t3_gather_trap(hipStream_t st, const vh_dims& d, const vh_dev& v, const int* e3em, int ne3d,
+                       const double* tau, const double* sigma_n, const double* dN0, const double* sigma,
+                       const double* dN1, double2* X) {
+    if (ne3d <= 0) return;
+    dim3 g((d.nk + 255) / 256, ne3d);
+    k_t3_gather_trap<<<g, 256, 0, st>>>(d.nr, d.nk, d.nlm, v, e3em, tau, sigma_n, dN0, sigma, dN1, X);
     HIPCHECK(hipGetLastError());
 }
 
