@@ -312,6 +312,10 @@ vh_model::~vh_model() {
     if (st) (void)hipStreamDestroy(st);
 }
 
+#ifndef VH_TCHUNK
+#define VH_TCHUNK 48
+#endif
+
 namespace {
 
 void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_bed_eq, const double* h_ice_eq,
@@ -1974,7 +1978,7 @@ void enable_lateral(vh_model* h, const double* pert) {
     h->ne3d = (int)e3.size();
     h->e3mesh = e3;
     h->nbat = 3 * h->ne3d;
-    h->tchunk = std::min(h->nbat, 48);
+    h->tchunk = std::min(h->nbat, VH_TCHUNK);
     h->d_e3em = dalloc<int>(e3em.size()); upload(h->d_e3em, e3em.data(), e3em.size(), st);
     h->d_e3of = dalloc<int>(e3of.size()); upload(h->d_e3of, e3of.data(), e3of.size(), st);
     upload(h->d_is3d, is3d.data(), (size_t)d.nem, st);

@@ -63,6 +63,10 @@ static inline unsigned xcd_order_grid(int ngrp, int L) { return (unsigned)(8 * n
 // of the order are loaded once per block for all TB batches; the pair swaps
 // its partial sums and the even lane writes the north row, the odd lane the
 // south row.
+// degrees per parity whose table values the synthesis loads together
+#ifndef VH_TSH_SU
+#define VH_TSH_SU 4
+#endif
 template <int TB, bool JCHUNK>
 __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __restrict__ X, int nb,
                                                    double2* __restrict__ Fd) {
@@ -100,8 +104,7 @@ __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __re
     for (int bb = 0; bb < TB; bb++)
 #pragma unroll
         for (int q = 0; q < 7; q++) acc[bb][q] = make_double2(0.0, 0.0);
-    for (int il = par; il < nl; il += 2) {
-        const double pv = P[(size_t)il * nh + jj], dv = D[(size_t)il * nh + jj];
+    auto term = [&](int il, double pv, double dv) __attribute__((always_inline)) {
 #pragma unroll
         for (int bb = 0; bb < TB; bb++) {
             const double2* cb = cs + (size_t)bb * 5 * nl;
@@ -116,7 +119,21 @@ __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __re
             a[5].x = fma(l6.x, pv, a[5].x); a[5].y = fma(l6.y, pv, a[5].y);
             a[6].x = fma(c6.x, dv, a[6].x); a[6].y = fma(c6.y, dv, a[6].y);
         }
+    };
+    // VH_TSH_SU degrees of this parity per step, their table values loaded
+    // together (latency of the table rows overlapped)
+    int il = par;
+    for (; il + 2 * (VH_TSH_SU - 1) < nl; il += 2 * VH_TSH_SU) {
+        double pv[VH_TSH_SU], dv[VH_TSH_SU];
+#pragma unroll
+        for (int u = 0; u < VH_TSH_SU; u++) {
+            pv[u] = P[(size_t)(il + 2 * u) * nh + jj];
+            dv[u] = D[(size_t)(il + 2 * u) * nh + jj];
+        }
+#pragma unroll
+        for (int u = 0; u < VH_TSH_SU; u++) term(il + 2 * u, pv[u], dv[u]);
     }
+    for (; il < nl; il += 2) term(il, P[(size_t)il * nh + jj], D[(size_t)il * nh + jj]);
     // even lane: north row (E + O; dP-type E + O), odd lane: south row
     // (P-type E - O; dP-type O - E); own = this lane's parity sum
     const int row = par == 0 ? jj : nlat - 1 - jj;
