@@ -41,6 +41,19 @@ NCLASS = 12
 PROF_EVERY = 10   # calls per event-timed call in the forward bench
 
 
+def _tuned(name):
+    """A build-time knob of the library (fastearth3d_amd/csrc/vh_tuned.h)."""
+    import re
+    for line in open(os.path.join(ROOT, "fastearth3d_amd", "csrc", "vh_tuned.h")):
+        m = re.match(r"#define %s (\d+)" % name, line)
+        if m:
+            return int(m.group(1))
+    raise KeyError(name)
+
+
+TCHUNK = _tuned("VH_TCHUNK")   # tensor batches per launch
+
+
 def alg_bytes(dims, cls, chunk_batches=None, sle_iters=None):
     """Algorithmic HBM bytes of one launch of kernel class cls.
 
@@ -500,7 +513,7 @@ def adjoint_bench(a, L, emit=True):
     roof = None
     if cnt[7] > 0:
         nbat = 3 * n3
-        nch = (nbat + 47) // 48
+        nch = (nbat + TCHUNK - 1) // TCHUNK
         b = nbat / nch                                    # batches per launch, on average
         rows = b * 6 * (a.lmax + 1) * nlat * 16.0
         ab = 3.0 * rows + 3.0 * (b / 3.0) * ng * 8.0
@@ -632,9 +645,9 @@ def main():
     d = dict(se.dims)
     d["nlat"], d["nphi"] = se.nlat, se.nphi
     nbat = 3 * d.get("ne3d", 0)
-    # the tensor kernels run in chunks of up to 48 batches; the average launch
+    # the tensor kernels run in chunks of up to TCHUNK batches; the average launch
     # covers nbat / (number of chunks) batches
-    nch = (nbat + 47) // 48 if nbat else 1
+    nch = (nbat + TCHUNK - 1) // TCHUNK if nbat else 1
     tf = os.path.join(ROOT, "profiles", "c%d_lmax%d_traffic.json" % (a.config, a.lmax))
     traffic = json.load(open(tf))["kernels"] if os.path.exists(tf) else {}
     per_kernel = {}

@@ -93,3 +93,8 @@ def dptr(a):
 
 def iptr(a):
     return a.ctypes.data_as(_ip)
+
+# kernel classes of vh_profile / vh_profile_read (VH_K_* of include/vilma_hip.h)
+KERNEL_CLASSES = {0: "k_drift", 1: "k_advance", 2: "k_tsh_synth", 3: "k_t3_rows", 4: "k_tsh_analysis",
+                  5: "k_t3_gather", 6: "k_sle_solve", 7: "k_t3_rows_adj", 8: "k_adv_adj", 9: "sle_adjoint",
+                  10: "k_drift_adj", 11: "other"}

@@ -312,10 +312,6 @@ vh_model::~vh_model() {
     if (st) (void)hipStreamDestroy(st);
 }
 
-#ifndef VH_TCHUNK
-#define VH_TCHUNK 48
-#endif
-
 namespace {
 
 void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_bed_eq, const double* h_ice_eq,

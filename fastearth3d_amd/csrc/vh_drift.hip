@@ -93,9 +93,6 @@ struct drift_args {
 };
 
 #define DRIFT_THREADS 512
-#ifndef VH_DRIFT_RW
-#define VH_DRIFT_RW 32   // right-hand sides per block (orders x re/im), at most
-#endif
 typedef double vh_d4 __attribute__((ext_vector_type(4)));
 
 // Forcing of the block's right-hand sides (rhs = reim*ts + (m - m0)) at every

@@ -18,6 +18,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "vh_tuned.h"
 #include <stdexcept>
 #include <string>
 #include <vector>

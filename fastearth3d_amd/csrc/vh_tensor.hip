@@ -63,10 +63,6 @@ static inline unsigned xcd_order_grid(int ngrp, int L) { return (unsigned)(8 * n
 // of the order are loaded once per block for all TB batches; the pair swaps
 // its partial sums and the even lane writes the north row, the odd lane the
 // south row.
-// degrees per parity whose table values the synthesis loads together
-#ifndef VH_TSH_SU
-#define VH_TSH_SU 4
-#endif
 template <int TB, bool JCHUNK>
 __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __restrict__ X, int nb,
                                                    double2* __restrict__ Fd) {
@@ -540,10 +536,7 @@ __global__ void __launch_bounds__(256) k_t3_gather_trap(int nr, int nk, int nlm,
 }
 
 // ------------------------------------------------------------- host side
-// batches per block of the tensor Legendre kernels (table reuse vs registers)
-#ifndef VH_TSH_TB_SYNTH
-#define VH_TSH_TB_SYNTH 3
-#endif
+// LDS of the synthesis (VH_TSH_TB_SYNTH batches per block) and of the row pass
 static size_t synth_lds(const vh_sht& s) { return (size_t)VH_TSH_TB_SYNTH * 5 * (s.L + 1) * sizeof(double2); }
 static size_t rows_lds(const vh_sht& s) {
     return (size_t)3 * (fft_fixed_size(s.nphi) ? 1 : 2) * s.nphi * sizeof(double2) + (size_t)s.nphi * sizeof(double);

@@ -1,0 +1,17 @@
+// vh_tuned.h -- launch and tile shapes of the hot kernels, as chosen by the
+// project's autotuner (python -m fastearth3d_amd.autotune, which measures
+// every candidate on the GPU and rewrites this file). A -D on the compiler
+// line still overrides each value.
+#pragma once
+#ifndef VH_TSH_TB_SYNTH
+#define VH_TSH_TB_SYNTH 3   // batches per block of the tensor synthesis
+#endif
+#ifndef VH_TSH_SU
+#define VH_TSH_SU 4         // degrees per parity whose table values the synthesis loads together
+#endif
+#ifndef VH_DRIFT_RW
+#define VH_DRIFT_RW 32      // right-hand sides (orders x re/im) per block of the drift solve
+#endif
+#ifndef VH_TCHUNK
+#define VH_TCHUNK 72        // tensor batches per synthesis / row / analysis launch
+#endif
