@@ -1780,6 +1780,66 @@ int vh_set_memory(vh_model* h, const double* tau) {
     });
 }
 
+// read_ve_state's cross-resolution branch (src/vilma_io.f90:515-552): the
+// memory of a run at a lower truncation (nk_src slots, degree-grouped, so the
+// shared low-degree block is a prefix of every slot index) copied into the
+// low degrees, the higher ones zero; sigma_n is not carried over (the next
+// trapezoidal step re-primes it)
+int vh_set_memory_lowres(vh_model* h, const double* tau, int nk_src) {
+    return guarded(h, [&] {
+        if (h->kind != 2) throw vh_error("vh_set_memory_lowres: no viscoelastic memory");
+        const vh_dims& d = h->d;
+        if (nk_src > d.nk)
+            throw vh_error("vilma_restart_read: cross-resolution downsampling (file lmax > model) not supported");
+        bool ok = false;
+        for (int L = 1; L <= d.L; L++)
+            if (L * (L + 3) / 2 == nk_src) ok = true;   // slots of degrees 1..L
+        if (!ok) throw vh_error("vilma_restart_read: cross-resolution restart incompatible (mmax/mres mismatch)");
+        const size_t n_src = (size_t)4 * d.ne * nk_src, n_dst = (size_t)4 * d.ne * d.nk;
+        std::vector<double> full(6 * n_dst, 0.0);
+        for (int a6 = 0; a6 < 6; a6++)   // (nlam, ne, nk) column-major: k slowest, so a prefix
+            std::copy(tau + a6 * n_src, tau + (a6 + 1) * n_src, full.begin() + a6 * n_dst);
+        if (vh_set_memory(h, full.data())) throw vh_error(h->err);
+        h->sig_primed = false;
+    });
+}
+
+// The step state of the trapezoidal scheme that a restart carries
+// (src/vilma_io.f90:453-457, :548-551): the controller's next step size
+// (years) and the start-of-step load sigma_n (interleaved complex, SHTns LM
+// order, nlm values) with its primed flag.
+int vh_get_step_state(vh_model* h, double* dt_try_yr, int* sigma_primed, double* sigma_n) {
+    return guarded(h, [&] {
+        if (dt_try_yr) *dt_try_yr = h->dt_try / VH_SEC_PER_YEAR;
+        if (sigma_primed) *sigma_primed = h->sig_primed ? 1 : 0;
+        if (sigma_n) {
+            if (!h->d_sig_n) {
+                std::fill(sigma_n, sigma_n + 2 * (size_t)h->d.nlm, 0.0);
+                return;
+            }
+            std::vector<double2> t(h->d.nlm);
+            HIPCHECK(hipMemcpyAsync(t.data(), h->d_sig_n, t.size() * sizeof(double2), hipMemcpyDeviceToHost, h->st));
+            h->sync();
+            to_lm(h, (const double2*)t.data(), sigma_n);
+        }
+    });
+}
+
+int vh_set_step_state(vh_model* h, double dt_try_yr, int sigma_primed, const double* sigma_n) {
+    return guarded(h, [&] {
+        h->dt_try = dt_try_yr * VH_SEC_PER_YEAR;
+        if (h->kind != 2 || !h->trap) { h->sig_primed = sigma_primed != 0; return; }
+        trap_alloc(h);
+        if (sigma_n) {
+            std::vector<double2> t(h->d.nlm);
+            from_lm(h, sigma_n, t.data());
+            upload((double2*)h->d_sig_n, t.data(), t.size(), h->st);
+            h->sync();
+        }
+        h->sig_primed = sigma_primed != 0 && sigma_n != nullptr;
+    });
+}
+
 // The prognostic state besides the memory (src/vilma_io.f90:425-491): the
 // clock, the rsl that seeds the next sea-level solve, the ice at the call
 // boundary, and the rotation state (m and both channels' memory in the

@@ -26,7 +26,7 @@ module vilma
    public :: solid_earth, solid_earth_init, solid_earth_update, solid_earth_finalize
    public :: solid_earth_enable_visc_3d, solid_earth_enable_visc_3d_field, solid_earth_enable_visc_3d_nodes
    public :: solid_earth_spinup, solid_earth_sync_host
-   public :: solid_earth_get_restart_state, solid_earth_set_restart_state
+   public :: solid_earth_get_restart_state, solid_earth_set_restart_state, solid_earth_set_restart_memory
    public :: remap_ll_gauss, remap_init, remap_to_gauss, remap_to_ll, remap_free
    public :: grid_class, grid_init
 
@@ -205,6 +205,23 @@ module vilma
       integer(c_int) function vh_set_memory(h, tau) bind(c, name="vh_set_memory")
          import :: c_int, c_ptr
          type(c_ptr), value :: h, tau
+      end function
+      integer(c_int) function vh_set_memory_lowres(h, tau, nk_src) bind(c, name="vh_set_memory_lowres")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: h, tau
+         integer(c_int), value :: nk_src
+      end function
+      integer(c_int) function vh_get_step_state(h, dt_try, primed, sig) bind(c, name="vh_get_step_state")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h, sig
+         real(c_double) :: dt_try
+         integer(c_int) :: primed
+      end function
+      integer(c_int) function vh_set_step_state(h, dt_try, primed, sig) bind(c, name="vh_set_step_state")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: h, sig
+         real(c_double), value :: dt_try
+         integer(c_int), value :: primed
       end function
       integer(c_int) function vh_get_state(h, t, rsl, hi, m, lmem, tmem) bind(c, name="vh_get_state")
          import :: c_int, c_ptr, c_double
@@ -602,34 +619,89 @@ contains
    !! = tau_a_re, tau_a_im, tau_b_re, tau_b_im, tau_c_re, tau_c_im, the rsl and
    !! ice of the Gauss grid (nphi, nlat), the polar motion m and both rotation
    !! channels' memory (nlam, ne, 6) (rotation_get_memory, src/vilma_rotation.f90:275-285).
-   subroutine solid_earth_get_restart_state(self, time, tau, rsl, h_ice, m, load_mem, tidal_mem)
+   !! optional: the adaptive controller's next step dt_try [yr] and, for
+   !! scheme "trap", the start-of-step load sigma_n(nlm) (zero and primed =
+   !! .false. before the first trapezoidal step), src/vilma_io.f90:453-457,548-551
+   subroutine solid_earth_get_restart_state(self, time, tau, rsl, h_ice, m, load_mem, tidal_mem, dt_try, &
+                                            sigma_n, sigma_primed)
       type(solid_earth), intent(inout) :: self
       real(wp), intent(out) :: time
       real(wp), intent(out), target, contiguous :: tau(:,:,:,:), rsl(:,:), h_ice(:,:), load_mem(:,:,:), tidal_mem(:,:,:)
       complex(wp), intent(out) :: m
+      real(wp), intent(out), optional :: dt_try
+      complex(wp), intent(out), optional, target, contiguous :: sigma_n(:)
+      logical, intent(out), optional :: sigma_primed
       real(wp), target :: mm(2)
-      real(c_double) :: t
+      real(c_double) :: t, dtt
+      integer(c_int) :: pr
+      type(c_ptr) :: ps
       call check_restart_shapes(self, tau, rsl, h_ice, load_mem, tidal_mem)
       call check(self, vh_get_memory(self%h, c_loc(tau)), "solid_earth_get_restart_state")
       call check(self, vh_get_state(self%h, t, c_loc(rsl), c_loc(h_ice), c_loc(mm), c_loc(load_mem), &
                  c_loc(tidal_mem)), "solid_earth_get_restart_state")
       time = t
       m = cmplx(mm(1), mm(2), wp)
+      ps = c_null_ptr
+      if (present(sigma_n)) then
+         if (size(sigma_n) /= size(self%sigma_lm)) error stop 'restart state: sigma_n must have nlm values'
+         ps = c_loc(sigma_n)
+      end if
+      call check(self, vh_get_step_state(self%h, dtt, pr, ps), "solid_earth_get_restart_state")
+      if (present(dt_try)) dt_try = dtt
+      if (present(sigma_primed)) sigma_primed = pr /= 0
    end subroutine solid_earth_get_restart_state
 
-   subroutine solid_earth_set_restart_state(self, time, tau, rsl, h_ice, m, load_mem, tidal_mem)
+   subroutine solid_earth_set_restart_state(self, time, tau, rsl, h_ice, m, load_mem, tidal_mem, dt_try, &
+                                            sigma_n)
       type(solid_earth), intent(inout) :: self
       real(wp), intent(in) :: time
       real(wp), intent(in), target, contiguous :: tau(:,:,:,:), rsl(:,:), h_ice(:,:), load_mem(:,:,:), tidal_mem(:,:,:)
       complex(wp), intent(in) :: m
+      real(wp), intent(in), optional :: dt_try
+      complex(wp), intent(in), optional, target, contiguous :: sigma_n(:)
       real(wp), target :: mm(2)
+      type(c_ptr) :: ps
+      integer(c_int) :: pr
+      real(c_double) :: dtt
       mm = [real(m, wp), aimag(m)]
       call check_restart_shapes(self, tau, rsl, h_ice, load_mem, tidal_mem)
       call check(self, vh_set_memory(self%h, c_loc(tau)), "solid_earth_set_restart_state")
       call check(self, vh_set_state(self%h, real(time, c_double), c_loc(rsl), c_loc(h_ice), c_loc(mm), &
                  c_loc(load_mem), c_loc(tidal_mem)), "solid_earth_set_restart_state")
+      if (present(dt_try) .or. present(sigma_n)) then
+         ! what is not given keeps its current value
+         call check(self, vh_get_step_state(self%h, dtt, pr, c_null_ptr), "solid_earth_set_restart_state")
+         if (present(dt_try)) dtt = dt_try
+         ps = c_null_ptr
+         if (present(sigma_n)) then
+            if (size(sigma_n) /= size(self%sigma_lm)) error stop 'restart state: sigma_n must have nlm values'
+            ps = c_loc(sigma_n)
+            pr = 1
+         else
+            pr = 0
+         end if
+         call check(self, vh_set_step_state(self%h, dtt, pr, ps), "solid_earth_set_restart_state")
+      end if
       call refresh(self)
    end subroutine solid_earth_set_restart_state
+
+   !! read_ve_state's memory (src/vilma_io.f90:492-552): tau(nlam, ne, nk_file, 6)
+   !! of a run at the same truncation, or at a lower one (cross-resolution:
+   !! the shared low-degree block, higher degrees zero; the spatial fields and
+   !! sigma_n are not carried over, so the host seeds with a zero-length update)
+   subroutine solid_earth_set_restart_memory(self, tau)
+      type(solid_earth), intent(inout) :: self
+      real(wp), intent(in), target, contiguous :: tau(:,:,:,:)
+      integer :: nlam, ne, nk
+      nlam = dim_of(self, 8);  ne = dim_of(self, 5);  nk = dim_of(self, 6)
+      if (size(tau, 1) /= nlam .or. size(tau, 2) /= ne .or. size(tau, 4) /= 6) &
+         error stop 'restart memory: tau must be (nlam, ne, nk_file, 6)'
+      if (size(tau, 3) == nk) then
+         call check(self, vh_set_memory(self%h, c_loc(tau)), "vilma_restart_read")
+      else
+         call check(self, vh_set_memory_lowres(self%h, c_loc(tau), int(size(tau, 3), c_int)), "vilma_restart_read")
+      end if
+   end subroutine solid_earth_set_restart_memory
 
    !! a dimension of the model (vh_get_dims selector: 4 nr, 5 ne, 6 nk, 8 nlam)
    integer function dim_of(self, which) result(n)

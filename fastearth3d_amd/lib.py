@@ -61,6 +61,9 @@ def load():
         L.vh_remap_to_ll_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.vh_remap_destroy.argtypes = [C.c_void_p]
         L.vh_spinup.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int]
+        L.vh_set_memory_lowres.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int]
+        L.vh_get_step_state.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.vh_set_step_state.argtypes = [C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_double)]
         L.vh_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.vh_get_tuning.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
         L.vh_ensemble_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(C.c_void_p)]
@@ -174,6 +177,31 @@ class SolidEarth(ModelHandle):
         self._check(self._lib.vh_set_state(self._h, float(st["time"]), _cp(arr["rsl"]), _cp(arr["h_ice"]),
                                            _cp(arr["rot_m"]), _cp(arr["rot_load_mem"]), _cp(arr["rot_tidal_mem"])),
                     "set_state")
+
+    def set_memory_lowres(self, tau):
+        """The memory of a run at a lower truncation, (6, nk_src, ne, nlam) as
+        memory() returns it, into the low degrees (the rest zero; the
+        cross-resolution restart of src/vilma_io.f90:515-552)."""
+        t = np.ascontiguousarray(tau, dtype=np.float64)
+        d = self.dims
+        if t.ndim != 4 or t.shape[0] != 6 or t.shape[2] != d["ne"] or t.shape[3] != d["nlam"]:
+            raise VilmaError("set_memory_lowres: tau must be (6, nk_src, ne, nlam)")
+        self._check(self._lib.vh_set_memory_lowres(self._h, _cp(t), int(t.shape[1])), "set_memory_lowres")
+
+    def step_state(self):
+        """dt_try (yr), sigma_primed and sigma_n (LM order) of the adaptive
+        controller / trapezoidal scheme."""
+        dt, pr = C.c_double(), C.c_int()
+        sig = np.empty(self.dims["nlm"], dtype=np.complex128)
+        self._check(self._lib.vh_get_step_state(self._h, C.byref(dt), C.byref(pr),
+                                                sig.ctypes.data_as(C.POINTER(C.c_double))), "get_step_state")
+        return dict(dt_try=dt.value, sigma_primed=bool(pr.value), sigma_n=sig)
+
+    def set_step_state(self, st):
+        sig = None if st.get("sigma_n") is None else np.ascontiguousarray(st["sigma_n"], dtype=np.complex128)
+        self._check(self._lib.vh_set_step_state(self._h, float(st["dt_try"]), int(bool(st["sigma_primed"])),
+                                                None if sig is None else sig.ctypes.data_as(C.POINTER(C.c_double))),
+                    "set_step_state")
 
     def spinup(self, h_ice_lgm, equil_time_max_yr, equil_rate_tol=1.0e-3, pre_spinup_1d=False):
         """solid_earth_spinup (src/vilma_coupling.f90:453-517): relax under the

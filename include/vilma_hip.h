@@ -52,7 +52,7 @@ typedef struct vh_params {
     /* additive: explicit radial element count per layer (surface-first);
      * 0 keeps the reference depth-target rule of src/vilma_radial_fe.f90:140-224 */
     int    n_elem_layer[VH_MAX_LAYER];
-    char   scheme[8];                       /* "fe" (forward Euler memory)         */
+    char   scheme[8];                       /* "fe" forward Euler | "trap" trapezoid */
     int    max_couple_iter;
     char   earth_response[8];               /* "ve" | "elastic" | "null"           */
     int    sle_n_outer, sle_n_inner;
@@ -173,6 +173,21 @@ int vh_get_state(vh_model* h, double* time_yr, double* rsl, double* h_ice, doubl
                  double* rot_load_mem, double* rot_tidal_mem);
 int vh_set_state(vh_model* h, double time_yr, const double* rsl, const double* h_ice, const double* rot_m,
                  const double* rot_load_mem, const double* rot_tidal_mem);
+/* read_ve_state's cross-resolution restart (src/vilma_io.f90:515-552): tau
+ * in the vh_set_memory layout of a run at a lower truncation L' (nk_src =
+ * L'(L'+3)/2 slots, degree-grouped: its slots are the model's first nk_src)
+ * into the low degrees of the memory, the higher degrees zero; sigma_n of the
+ * trapezoidal scheme is re-primed by the next step. nk_src above the model's
+ * nk (downsampling) or not a whole number of degrees is refused with the
+ * reference's messages. */
+int vh_set_memory_lowres(vh_model* h, const double* tau, int nk_src);
+/* The step state a restart carries besides the fields (src/vilma_io.f90:
+ * 453-457, :548-551): the adaptive controller's next step size dt_try
+ * (years) and, for scheme "trap", the start-of-step load sigma_n (2*nlm,
+ * SHTns LM order) with its primed flag. Any pointer may be NULL; a NULL
+ * sigma_n on set leaves sigma_n to be re-primed. */
+int vh_get_step_state(vh_model* h, double* dt_try_yr, int* sigma_primed, double* sigma_n);
+int vh_set_step_state(vh_model* h, double dt_try_yr, int sigma_primed, const double* sigma_n);
 
 /* ---- host lon-lat grid <-> the model's Gauss grid (src/vilma_remap.f90) */
 typedef struct vh_remap_handle vh_remap_handle;

@@ -252,7 +252,7 @@ def test_error_paths():
     from fastearth3d_amd.model import VilmaError
     c = configs.config(0, lmax=8)
     p = c["params"]
-    p.scheme = b"trap"
+    p.scheme = b"etd1"   # not built (trap is)
     with pytest.raises(VilmaError, match="scheme"):
         lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"])
     p = configs.config(0, lmax=8)["params"]
