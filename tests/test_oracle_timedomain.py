@@ -1,7 +1,7 @@
 """The oracle's time stepping pinned against data and closed forms the
 reference itself holds, CPU only:
 
-* time-dependent loading Love numbers h(t), k(t) of every degree 2..32 under
+* time-dependent loading Love numbers h(t), k(t) of every degree 2..48 under
   a Heaviside unit load, stepped through the oracle's full per-slot driver
   (drift of src/vilma_response.f90:694-809, response_apply :842-864, the
   forward-Euler advance :944-992) for every order m of every degree, against
@@ -55,7 +55,7 @@ def heaviside_run(oracle):
     """Unit load on every (l, m) slot (real 1 at m = 0, 1 + 1i at m > 0) held
     from t = 0, stepped with dt = 25 yr through begin_step / apply / advance.
     Returns (L, g, a, {t_kyr: (u_lm, N_lm)})."""
-    L, dtyr = 32, 25.0
+    L, dtyr = 48, 25.0
     p = _m3_params(L)
     nphi, nlat = p.grid()
     m = oracle.OracleModel(p, np.full((nlat, nphi), 1000.0), np.zeros((nlat, nphi)))
@@ -101,7 +101,7 @@ def test_normal_modes_sum_to_fluid_limit():
 
 
 def test_love_time_dependent_all_degrees(heaviside_run):
-    """h(t), k(t) of the stepped per-slot response at every degree 2..32 and
+    """h(t), k(t) of the stepped per-slot response at every degree 2..48 and
     t = 0, 1, 2, 5, 10 kyr within 1 % of the normal-mode solution (measured
     <= 0.42 %), and every order m of a degree responds exactly as m = 0."""
     L, g, a, out = heaviside_run
@@ -177,9 +177,9 @@ def test_spada_disc_elastic_profile(oracle):
 def test_spada_disc_ve_centre(heaviside_run):
     """Viscoelastic centre uplift u(0, t) = sum_n sig_n U_n(t) at t = 1, 2, 5,
     10 kyr (tests/test_benchmark_disc.f90 check_ve_centre: degrees 2..128,
-    degree 1 skipped, 3 %). U_n(t) for n = 2..32 is the oracle's stepped
-    per-slot response; the tail n = 33..128, which the oracle-sized run does
-    not reach (3-4 % of u(0) at 1 kyr), is taken from the normal-mode table
+    degree 1 skipped, 3 %). U_n(t) for n = 2..48 is the oracle's stepped
+    per-slot response; the tail n = 49..128, which the oracle-sized run does
+    not reach (a few per cent of u(0) at 1 kyr), is taken from the normal-mode table
     (h_n(t) phi_n / g), so the comparison measures the oracle on the degrees
     it ran."""
     ref = json.load(open(os.path.join(GOLDEN, "spada2011_disc.json")))
