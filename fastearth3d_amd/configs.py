@@ -286,3 +286,61 @@ def config(idx, lmax=None, ne=None, seed=0):
                     ice=lambda k, dt=dt: domes * float(ice_history_factor((k + 1) * dt / 1000.0)),
                     dt_yr=dt, steps=steps, pert=pert)
     raise ValueError("config %d" % idx)
+
+
+# Martinec et al. (2018) sea-level benchmark cases, data-file letters
+# (tests/test_benchmark_sle.f90:210-247): cap height [m], basin centre
+# (colat, lon) [deg], basin bmax, b0 [m], fixed ocean (SLE1), Heaviside (T0),
+# paleotopography spin-up.
+MARTINEC_CASES = {
+    "C2": (1500.0, (100.0, 320.0), (760.0, 1200.0), True, True, False),
+    "D3": (500.0, (35.0, 25.0), (3800.0, 6000.0), True, False, False),
+    "E2": (500.0, (35.0, 25.0), (3800.0, 6000.0), False, False, False),
+    "F1": (500.0, (35.0, 25.0), (3800.0, 6000.0), False, False, True),
+}
+
+
+def martinec_cap(colat, lon, colat0_deg, lon0_deg, alpha_deg, h0):
+    """h0 sqrt[(cos d - cos a)/(1 - cos a)] within angular radius a of the
+    centre (src/vilma_field.f90:27-50)."""
+    d = _angdist(colat, lon, np.radians(colat0_deg), np.radians(lon0_deg))
+    ca = np.cos(np.radians(alpha_deg))
+    return np.where(d <= np.radians(alpha_deg), h0 * np.sqrt(np.clip((np.cos(d) - ca) / (1.0 - ca), 0.0, None)), 0.0)
+
+
+def martinec_basin(colat, lon, colat0_deg, lon0_deg, bmax, b0, sigma_deg=26.0):
+    """bmax - b0 exp(-d^2 / 2 sigma^2) (src/vilma_field.f90:52-70)."""
+    d = _angdist(colat, lon, np.radians(colat0_deg), np.radians(lon0_deg))
+    s = np.radians(sigma_deg)
+    return bmax - b0 * np.exp(-d * d / (2.0 * s * s))
+
+
+def martinec2018(case, lmax=128):
+    """Workload of a Martinec (2018) sea-level benchmark case
+    (tests/test_benchmark_sle.f90:100-135, :255-290): the M3-L70-V01 earth on
+    the (2 lmax, 4 lmax) Gauss grid, a sqrt cap of radius 10 deg centred at
+    colat 25 / lon 75 on an exponential basin, dt 20 yr from a relaxed,
+    ice-free state. T0: full ice for 500 calls; T1: the cap (radius and
+    height) grows linearly over 500 calls, then is held 250. No rotational
+    feedback; 20 inner sea-level iterations. `spinup` asks the caller for the
+    paleotopography iteration of case F1."""
+    h0, (bc, bl), (bmax, b0), fixed, heaviside, spinup = MARTINEC_CASES[case]
+    p = params_defaults()
+    p.earth = b"M3-L70-V01"
+    p.lmax, p.nlat, p.nphi = lmax, 2 * lmax, 4 * lmax
+    p.rotation = 0
+    p.sle_n_inner = 20
+    p.sle_fixed_ocean = 1 if fixed else 0
+    p.sle_subgrid = 1
+    nphi, nlat = p.grid()
+    colat, lon = gauss_grid(nlat, nphi)
+    basin = martinec_basin(colat, lon, bc, bl, bmax, b0)
+    steps = 500 if heaviside else 750
+
+    def ice(k):
+        frac = 1.0 if heaviside else min(1.0, (k + 1) / 500.0)
+        return martinec_cap(colat, lon, 25.0, 75.0, 10.0 * frac, h0 * frac)
+
+    return dict(name="martinec_" + case, params=p, z_bed_eq=basin, h_ice_eq=np.zeros_like(basin),
+                h_ice_init=None, ice=ice, dt_yr=20.0, steps=steps, pert=None, basin=basin, spinup=spinup,
+                colat=colat, lon=lon)
