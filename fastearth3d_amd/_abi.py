@@ -51,7 +51,7 @@ class VhParams(C.Structure):
 F_RSL, F_ZBED, F_C, F_C0, F_SROT, F_HICE = range(6)
 (S_TIME, S_BSL, S_WORST_MASS_RESID, S_N_ACCEPT, S_N_REJECT, S_N_SOLVE,
  S_N_INNER_TOT, S_N_OUTER_TOT, S_POLAR_M_RE, S_POLAR_M_IM, S_N_SLE_STEAL, S_N_FLOOR, S_N_COUPLE_TOT) = range(13)
-SP_SIGMA, SP_U, SP_N = range(3)
+SP_SIGMA, SP_U, SP_N, SP_TOR = range(4)
 D_LMAX, D_NLAT, D_NPHI, D_NLM, D_NR, D_NE, D_NK, D_NE3D, D_NLAM, D_NEM, D_NMN = range(11)
 T_SLE_BLOCKS = 0
 

@@ -86,6 +86,7 @@ class ModelHandle:
                         "enable_lateral_visc: pert_elem must be (ne, nlat, nphi)")
         self._check(self._fn("enable_visc_3d")(self._h, dptr(p)), "enable_lateral_visc")
         self.dims["ne3d"] = self._dim(_abi.D_NE3D)
+        self.dims["nlam"] = self._dim(_abi.D_NLAM)   # 6 once the toroidal channels are carried
 
     def enable_visc_3d_nodes(self, visc_node, lid_depth=0.0, lid_log10max=22.0, log10_cap=30.0):
         v = self._sized(visc_node, self.dims["nr"] * self.nphi * self.nlat,
@@ -93,6 +94,7 @@ class ModelHandle:
         self._check(self._fn("enable_visc_3d_nodes")(self._h, dptr(v), lid_depth, lid_log10max, log10_cap),
                     "enable_lateral_visc_from_nodes")
         self.dims["ne3d"] = self._dim(_abi.D_NE3D)
+        self.dims["nlam"] = self._dim(_abi.D_NLAM)
 
     def field(self, which):
         out = np.empty((self.nlat, self.nphi))

@@ -95,7 +95,11 @@ enum {
 enum {
     VH_SP_SIGMA = 0,   /* se%sigma_lm: converged surface load [kg m^-2]      */
     VH_SP_U = 1,       /* u_lm = gu*sigma + dUa at the last report           */
-    VH_SP_N = 2        /* N_lm = gn*sigma - dFa/g                            */
+    VH_SP_N = 2,       /* N_lm = gn*sigma - dFa/g                            */
+    VH_SP_TOR = 3      /* t_lm: surface toroidal displacement e_r x grad_1
+                          sum t Y (response_horizontal_toroidal,
+                          src/vilma_response.f90:884-900); zero unless the
+                          toroidal channels are carried                      */
 };
 enum {
     VH_D_LMAX = 0, VH_D_NLAT = 1, VH_D_NPHI = 2, VH_D_NLM = 3, VH_D_NR = 4,

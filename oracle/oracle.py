@@ -36,10 +36,20 @@ def lib():
                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_void_p), C.c_char_p]
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_params_defaults.argtypes = [C.POINTER(VhParams)]
-        for n in ("orc_sht_analysis_c", "orc_sht_synthesis_c", "orc_tensor_synth_c", "orc_tensor_analysis_c"):
+        for n in ("orc_sht_analysis_c", "orc_sht_synthesis_c"):
             getattr(L, n).argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
-        for n in ("orc_tensor_n6_c", "orc_rot_info_c"):
+        for n in ("orc_tensor_synth_c", "orc_tensor_analysis_c"):
+            getattr(L, n).argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double)]
+        for n in ("orc_tensor_n6_c", "orc_tensor_n4_c", "orc_rot_info_c"):
             getattr(L, n).argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.orc_tor_synthesis_c.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 3
+        L.orc_tor_analysis_c.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 3
+        L.orc_mesh_props_c.argtypes = [C.POINTER(VhParams), C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int,
+                                       C.c_char_p]
+        L.orc_tor_forcing_c.argtypes = [C.POINTER(VhParams), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                        C.c_char_p]
+        L.orc_tor_operator_c.argtypes = [C.POINTER(VhParams), C.c_int, C.POINTER(C.c_double), C.c_long,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_char_p]
         L.orc_sph_synthesis_c.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 3
         L.orc_sph_analysis_c.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 3
         L.orc_surface_integral_c.restype = C.c_double
@@ -115,19 +125,37 @@ class OracleModel(ModelHandle):
         return colat, lon, w
 
     def tensor_synth(self, c):
+        """c (4 or 6, nlm): channels lambda 1, 2, 5, 6 (and 3, 4) -> dyads (6, nlat, nphi)."""
+        c = np.ascontiguousarray(c, dtype=np.complex128)
         d = np.empty((6, self.nlat, self.nphi))
-        self._lib.orc_tensor_synth_c(self._h, _cp(np.ascontiguousarray(c, dtype=np.complex128)), _cp(d))
+        self._lib.orc_tensor_synth_c(self._h, _cp(c), c.shape[0], _cp(d))
         return d
 
-    def tensor_analysis(self, d):
-        c = np.empty((4, self.dims["nlm"]), dtype=np.complex128)
-        self._lib.orc_tensor_analysis_c(self._h, _cp(np.ascontiguousarray(d, dtype=np.float64)), _cp(c))
+    def tensor_analysis(self, d, nch=4):
+        c = np.empty((nch, self.dims["nlm"]), dtype=np.complex128)
+        self._lib.orc_tensor_analysis_c(self._h, _cp(np.ascontiguousarray(d, dtype=np.float64)), nch, _cp(c))
         return c
 
     def tensor_n6(self):
         n6 = np.empty(self.dims["lmax"] + 1)
         self._lib.orc_tensor_n6_c(self._h, _cp(n6))
         return n6
+
+    def tensor_n4(self):
+        n4 = np.empty(self.dims["lmax"] + 1)
+        self._lib.orc_tensor_n4_c(self._h, _cp(n4))
+        return n4
+
+    def tor_synthesis(self, q):
+        vt = np.empty((self.nlat, self.nphi)); vp = np.empty((self.nlat, self.nphi))
+        self._lib.orc_tor_synthesis_c(self._h, _cp(np.ascontiguousarray(q, dtype=np.complex128)), _cp(vt), _cp(vp))
+        return vt, vp
+
+    def tor_analysis(self, vt, vp):
+        q = np.empty(self.dims["nlm"], dtype=np.complex128)
+        self._lib.orc_tor_analysis_c(self._h, _cp(np.ascontiguousarray(vt, dtype=np.float64)),
+                                     _cp(np.ascontiguousarray(vp, dtype=np.float64)), _cp(q))
+        return q
 
     def rot_info(self):
         o = np.empty(7)
@@ -192,6 +220,45 @@ def dense_operator(params, j):
     A = np.zeros((nd, nd), order="F")
     lib().orc_dense_operator(C.byref(params), j, _cp(A), nd * nd, err)
     return A
+
+
+def tor_operator(params, j, b=None):
+    """The degree-j toroidal W stiffness (nr x nr, before pinning and
+    bordering) and, with b, the assembled solve A^-1 b."""
+    err = C.create_string_buffer(512)
+    nr = lib().orc_tor_operator_c(C.byref(params), j, None, 0, None, None, err)
+    if nr < 0:
+        raise VilmaError(err.value.decode())
+    A = np.zeros((nr, nr), order="F")
+    x = None
+    if b is None:
+        lib().orc_tor_operator_c(C.byref(params), j, _cp(A), nr * nr, None, None, err)
+        return A
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.empty(nr)
+    if lib().orc_tor_operator_c(C.byref(params), j, _cp(A), nr * nr, _cp(bb), _cp(x), err) < 0:
+        raise VilmaError(err.value.decode())
+    return A, x
+
+
+def mesh_props(params):
+    """Node radii, per-element shear modulus and rheology (0 elastic,
+    1 Maxwell, 2 fluid) of the params' mesh."""
+    r, _ = mesh(params)
+    err = C.create_string_buffer(512)
+    mu = np.empty(r.size - 1); rh = np.empty(r.size - 1, dtype=np.int32)
+    lib().orc_mesh_props_c(C.byref(params), _cp(mu), iptr(rh), r.size - 1, err)
+    return r, mu, rh
+
+
+def tor_forcing_of_w(params, j, W):
+    """The toroidal dissipative forcing of the memory 2 mu eps(W)."""
+    err = C.create_string_buffer(512)
+    Wc = np.ascontiguousarray(W, dtype=np.float64)
+    f = np.empty(Wc.size)
+    if lib().orc_tor_forcing_c(C.byref(params), j, _cp(Wc), _cp(f), err) < 0:
+        raise VilmaError(err.value.decode())
+    return f
 
 
 def mesh(params):

@@ -70,15 +70,30 @@ static inline int idx_p(int e) { return 4 * e + 3; }  /* element e       */
 void orc_strain_coeffs(double u1, double u2, double v1, double v2, double Jr,
                        double a[4], double b[4], double c[4]);
 void orc_strain_constants(double Jr, double norm[4], double sa[4][4], double sb[4][4], double sc[4][4]);
+/* A/B/C per slot are [e][ld]: ld = 4, or 6 with the toroidal channels */
 void orc_dissipative_rhs(int ne, const double* r, double sa[4][4], double sb[4][4], double sc[4][4],
                          const double norm[4], const double* A, const double* B, const double* C,
-                         double* f);
+                         int ld, double* f);
 void orc_advance_memory_trap(int ne, const double* mu, const double* Mk, const double* Un, const double* Vn,
                              const double* Up, const double* Vp, double Jr, double* A, double* B, double* C,
-                             const unsigned char* active);
+                             int ld, const unsigned char* active);
 void orc_advance_memory_fe(int ne, const double* mu, const double* Mk, const double* Un,
-                           const double* Vn, double Jr, double* A, double* B, double* C,
+                           const double* Vn, double Jr, double* A, double* B, double* C, int ld,
                            const unsigned char* active);
+
+/* toroidal degree of freedom (src/vilma_radial_fe.f90:699-817,
+ * src/vilma_viscoelastic.f90:373-439, :591-629) */
+typedef struct { int j, nr; orc_bband sys; unsigned char* pinned; } orc_torop;
+double* orc_build_tor_operator(const double* r, const double* mu, int ne, int j);
+int  orc_torop_assemble(orc_torop* op, const double* r, const double* mu, int ne, int j, char* err);
+void orc_torop_solve(const orc_torop* op, const double* b, double* x);
+void orc_torop_free(orc_torop* op);
+void orc_strain_coeffs_tor(double w1, double w2, double Jr, double a[2], double b[2], double c[2]);
+void orc_strain_constants_tor(double Jr, double norm[2], double sa[2][2], double sb[2][2], double sc[2][2]);
+void orc_dissipative_rhs_tor(int ne, const double* r, double sa[2][2], double sb[2][2], double sc[2][2],
+                             const double norm[2], const double* A, const double* B, const double* C, double* f);
+void orc_advance_memory_tor(int ne, const double* mu, const double* Mk, const double* Wn, const double* Wp,
+                            double Jr, double* A, double* B, double* C, const unsigned char* active);
 
 /* ------------------------------------------------------------------ SHT */
 typedef struct {
@@ -99,6 +114,9 @@ void orc_sht_analysis(const orc_sht* s, const double* f, cplx* q);
 void orc_sht_synthesis(const orc_sht* s, const cplx* q, double* f);
 void orc_sph_synthesis(const orc_sht* s, const cplx* q, double* vt, double* vp);
 void orc_sph_analysis(const orc_sht* s, const double* vt, const double* vp, cplx* q);
+/* toroidal pair (vt, vp) = e_r x grad_1 (sum q Y) = (-F, E) and its analysis */
+void orc_tor_synthesis(const orc_sht* s, const cplx* q, double* vt, double* vp);
+void orc_tor_analysis(const orc_sht* s, const double* vt, const double* vp, cplx* q);
 void orc_fft_rows_forward(const orc_sht* s, const double* f, int nrows, cplx* F);  /* m=0..lmax, scaled 2pi/nphi */
 void orc_fft_rows_inverse(const orc_sht* s, const cplx* F, int nrows, double* f);
 double orc_eval_point(const orc_sht* s, const cplx* q, double colat, double lon);
@@ -107,12 +125,14 @@ double orc_eval_point(const orc_sht* s, const cplx* q, double colat, double lon)
 typedef struct {
     int lmax, nlm, nphi, nlat;
     int *ldeg, *mord;
-    double *llp1, *cott, *invsin, *n6;
+    double *llp1, *cott, *invsin, *n6, *n4;
 } orc_tsh;
 int  orc_tsh_init(orc_tsh* t, const orc_sht* s);
 void orc_tsh_free(orc_tsh* t);
-void orc_tsh_synth(const orc_tsh* t, const orc_sht* s, const cplx* c /* 4*nlm, [lam][lm] */, double* dyad /* 6 planes */);
-void orc_tsh_analysis(const orc_tsh* t, const orc_sht* s, const double* dyad, cplx* c);
+/* nch = 4 (lambda 1,2,5,6) or 6 (+ lambda 3,4 as channels 4, 5) */
+void orc_tsh_synth(const orc_tsh* t, const orc_sht* s, const cplx* c /* nch*nlm, [lam][lm] */, int nch,
+                   double* dyad /* 6 planes */);
+void orc_tsh_analysis(const orc_tsh* t, const orc_sht* s, const double* dyad, int nch, cplx* c);
 
 /* ------------------------------------------------------------- response */
 typedef struct {
@@ -156,6 +176,14 @@ typedef struct {
     double *Are0, *Aim0, *Bre0, *Bim0, *Cre0, *Cim0;   /* tau_n of the step */
     double *edUn_re, *edUn_im, *edVn_re, *edVn_im;     /* endpoint drift [k][node] */
     double *Are_c, *Aim_c, *Bre_c, *Bim_c, *Cre_c, *Cim_c;   /* coarse stash (buffer B) */
+    /* toroidal channels (src/vilma_response.f90:126-146, :1055-1109): once
+     * carried, every memory array is [k][e][6] with lambda 3, 4 at 4, 5 */
+    int nlam, toroidal;
+    orc_torop* tops;                                   /* [lmax], tops[l-1] */
+    double (*nrmt)[2];
+    double (*sat)[2][2], (*sbt)[2][2], (*sct)[2][2];
+    cplx *dWa, *dWa_prev;                              /* [nk] surface W */
+    double *dWn_re, *dWn_im, *edWn_re, *edWn_im;       /* [k][node] */
 } orc_resp;
 
 int  orc_resp_init_ve(orc_resp* r, const orc_earth* e, const orc_mesh* m, const orc_sht* s, double dt, char* err);

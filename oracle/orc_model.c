@@ -56,7 +56,6 @@ int orc_create(const vh_params* p, int nphi, int nlat, const double* z_bed_eq, c
     if (strcmp(p->scheme, "fe") != 0 && strcmp(p->scheme, "trap") != 0) {
         snprintf(err, 256, "scheme \"%s\" not supported (fe|trap)", p->scheme); return 1;
     }
-    if (p->l_toroidal) { snprintf(err, 256, "l_toroidal = .true. is not supported"); return 1; }
     if ((p->mmax >= 0 && p->mmax != p->lmax) || p->mres != 1) { snprintf(err, 256, "mmax/mres other than lmax/1 not supported"); return 1; }
     int nl = p->nlat > 0 ? p->nlat : 2 * p->lmax + 2;
     int np = p->nphi > 0 ? p->nphi : 4 * p->lmax;
@@ -70,6 +69,7 @@ int orc_create(const vh_params* p, int nphi, int nlat, const double* z_bed_eq, c
     m->resp.deg1_cm = strcmp(p->deg1_frame, "cm") == 0;
     m->resp.skip_tol = p->skip_tol;
     m->resp.visc3d_tol = p->visc3d_tol;
+    m->resp.toroidal = p->l_toroidal != 0;   /* src/vilma_coupling.f90:282 */
     if (strcmp(p->earth_response, "ve") == 0) {
         if (orc_resp_init_ve(&m->resp, &m->earth, &m->mesh, &m->sht, dt0, err)) return 1;
     } else if (strcmp(p->earth_response, "elastic") == 0) {
@@ -383,6 +383,12 @@ int orc_get_spectrum(orc_model* m, int which, double* out) {
     orc_resp_apply(&m->resp, &m->sht, m->sigma_lm, m->u_lm, m->n_lm);
     if (which == VH_SP_U) { memcpy(out, m->u_lm, nlm * sizeof(cplx)); return 0; }
     if (which == VH_SP_N) { memcpy(out, m->n_lm, nlm * sizeof(cplx)); return 0; }
+    if (which == VH_SP_TOR) {   /* response_horizontal_toroidal (src/vilma_response.f90:884-900) */
+        memset(out, 0, nlm * sizeof(cplx));
+        if (m->resp.kind == RESP_VE && m->resp.nlam > 4)
+            for (int k = 0; k < m->resp.nk; k++) ((cplx*)out)[m->resp.k2lm[k]] = m->resp.dWa[k];
+        return 0;
+    }
     return 1;
 }
 
@@ -398,7 +404,7 @@ int orc_get_dims(orc_model* m, int which, int* out) {
         case VH_D_NE: *out = m->mesh.ne; break;
         case VH_D_NK: *out = m->resp.nk; break;
         case VH_D_NE3D: *out = m->resp.ne3d; break;
-        case VH_D_NLAM: *out = 4; break;
+        case VH_D_NLAM: *out = m->resp.kind == RESP_VE ? m->resp.nlam : 4; break;
         case VH_D_NEM: *out = nem; break;
         case VH_D_NMN: {
             int nmn = 0;
@@ -419,7 +425,7 @@ int orc_get_dims(orc_model* m, int which, int* out) {
 
 int orc_get_memory(orc_model* m, double* tau) {
     if (m->resp.kind != RESP_VE) return 1;
-    size_t n = (size_t)4 * m->resp.ne * m->resp.nk;
+    size_t n = (size_t)m->resp.nlam * m->resp.ne * m->resp.nk;
     double* arr[6] = {m->resp.Are, m->resp.Aim, m->resp.Bre, m->resp.Bim, m->resp.Cre, m->resp.Cim};
     for (int q = 0; q < 6; q++) memcpy(tau + q * n, arr[q], n * 8);
     return 0;
@@ -427,7 +433,7 @@ int orc_get_memory(orc_model* m, double* tau) {
 
 int orc_set_memory(orc_model* m, const double* tau) {
     if (m->resp.kind != RESP_VE) return 1;
-    size_t n = (size_t)4 * m->resp.ne * m->resp.nk;
+    size_t n = (size_t)m->resp.nlam * m->resp.ne * m->resp.nk;
     double* arr[6] = {m->resp.Are, m->resp.Aim, m->resp.Bre, m->resp.Bim, m->resp.Cre, m->resp.Cim};
     for (int q = 0; q < 6; q++) memcpy(arr[q], tau + q * n, n * 8);
     return 0;
@@ -477,18 +483,24 @@ double orc_surface_integral_c(orc_model* m, const double* f) { return orc_surfac
 int orc_grid_c(orc_model* m, double* colat, double* w) {
     memcpy(colat, m->sht.colat, m->nl * 8); memcpy(w, m->sht.w, m->nl * 8); return 0;
 }
-int orc_tensor_synth_c(orc_model* m, const double* c, double* dyad) {
+int orc_tensor_synth_c(orc_model* m, const double* c, int nch, double* dyad) {
     if (!m->resp.tsh_ready) { orc_tsh_init(&m->resp.tsh, &m->sht); m->resp.tsh_ready = 1; }
-    orc_tsh_synth(&m->resp.tsh, &m->sht, (const cplx*)c, dyad); return 0;
+    orc_tsh_synth(&m->resp.tsh, &m->sht, (const cplx*)c, nch, dyad); return 0;
 }
-int orc_tensor_analysis_c(orc_model* m, const double* dyad, double* c) {
+int orc_tensor_analysis_c(orc_model* m, const double* dyad, int nch, double* c) {
     if (!m->resp.tsh_ready) { orc_tsh_init(&m->resp.tsh, &m->sht); m->resp.tsh_ready = 1; }
-    orc_tsh_analysis(&m->resp.tsh, &m->sht, dyad, (cplx*)c); return 0;
+    orc_tsh_analysis(&m->resp.tsh, &m->sht, dyad, nch, (cplx*)c); return 0;
 }
 int orc_tensor_n6_c(orc_model* m, double* n6) {
     if (!m->resp.tsh_ready) { orc_tsh_init(&m->resp.tsh, &m->sht); m->resp.tsh_ready = 1; }
     memcpy(n6, m->resp.tsh.n6, (m->sht.lmax + 1) * 8); return 0;
 }
+int orc_tensor_n4_c(orc_model* m, double* n4) {
+    if (!m->resp.tsh_ready) { orc_tsh_init(&m->resp.tsh, &m->sht); m->resp.tsh_ready = 1; }
+    memcpy(n4, m->resp.tsh.n4, (m->sht.lmax + 1) * 8); return 0;
+}
+int orc_tor_synthesis_c(orc_model* m, const double* q, double* vt, double* vp) { orc_tor_synthesis(&m->sht, (const cplx*)q, vt, vp); return 0; }
+int orc_tor_analysis_c(orc_model* m, const double* vt, const double* vp, double* q) { orc_tor_analysis(&m->sht, vt, vp, (cplx*)q); return 0; }
 int orc_rot_info_c(orc_model* m, double* out) {
     out[0] = m->rot.kTe; out[1] = m->rot.hTe; out[2] = m->rot.k_s_fluid; out[3] = m->rot.k_s_flat;
     out[4] = m->rot.dt_fe_max; out[5] = creal(m->rot.m); out[6] = cimag(m->rot.m);
@@ -550,6 +562,70 @@ int orc_dense_operator(const vh_params* p, int j, double* A_out, int cap, char* 
     }
     orc_mesh_free(&m);
     return nd;
+}
+
+/* Toroidal W operator of degree j on the mesh of p, dense nr x nr column-
+ * major, before pinning and bordering (build_toroidal_operator,
+ * src/vilma_radial_fe.f90:699-726); returns nr. With x and b given, also the
+ * assembled (pinned, bordered) solve x = A^-1 b. */
+int orc_tor_operator_c(const vh_params* p, int j, double* A_out, int cap, const double* b, double* x, char* err) {
+    orc_earth e; orc_mesh m;
+    if (orc_build_earth(p, &e, err)) return -1;
+    if (orc_mesh_build(&e, p->n_elem_layer, &m, err)) return -1;
+    int nr = m.nr;
+    double* mu = malloc(m.ne * 8);
+    for (int k = 0; k < m.ne; k++) mu[k] = e.L[m.lay[k]].mu;   /* as the response's element shear */
+    if (A_out && (long)nr * nr <= cap) {
+        double* A = orc_build_tor_operator(m.r, mu, m.ne, j);
+        memcpy(A_out, A, (size_t)nr * nr * 8);
+        free(A);
+    }
+    if (b && x) {
+        orc_torop op;
+        if (orc_torop_assemble(&op, m.r, mu, m.ne, j, err)) { free(mu); orc_mesh_free(&m); return -1; }
+        orc_torop_solve(&op, b, x);
+        orc_torop_free(&op);
+    }
+    free(mu); orc_mesh_free(&m);
+    return nr;
+}
+
+/* per-element shear modulus and rheology of p's mesh (as the response sees
+ * them); returns ne */
+int orc_mesh_props_c(const vh_params* p, double* mu, int* rheo, int cap, char* err) {
+    orc_earth e; orc_mesh m;
+    if (orc_build_earth(p, &e, err)) return -1;
+    if (orc_mesh_build(&e, p->n_elem_layer, &m, err)) return -1;
+    int ne = m.ne;
+    if (mu && ne <= cap)
+        for (int k = 0; k < ne; k++) { mu[k] = e.L[m.lay[k]].mu; rheo[k] = e.L[m.lay[k]].rheology; }
+    orc_mesh_free(&m);
+    return ne;
+}
+
+/* The toroidal forcing of the memory tau = 2 mu eps(W) of a nodal W at degree
+ * j, through strain_coeffs_tor and dissipative_rhs_tor (channels 4, 5 of a
+ * 6-channel memory); f has nr entries. */
+int orc_tor_forcing_c(const vh_params* p, int j, const double* W, double* f, char* err) {
+    orc_earth e; orc_mesh m;
+    if (orc_build_earth(p, &e, err)) return -1;
+    if (orc_mesh_build(&e, p->n_elem_layer, &m, err)) return -1;
+    int ne = m.ne;
+    double Jr = (double)j * (j + 1.0);
+    double *A = calloc((size_t)6 * ne, 8), *B = calloc((size_t)6 * ne, 8), *C = calloc((size_t)6 * ne, 8);
+    for (int k = 0; k < ne; k++) {
+        double a[2], b[2], c[2], mu = e.L[m.lay[k]].mu;
+        orc_strain_coeffs_tor(W[k], W[k + 1], Jr, a, b, c);
+        for (int q = 0; q < 2; q++) {
+            A[k * 6 + 4 + q] = 2.0 * mu * a[q]; B[k * 6 + 4 + q] = 2.0 * mu * b[q]; C[k * 6 + 4 + q] = 2.0 * mu * c[q];
+        }
+    }
+    double norm[2], sa[2][2], sb[2][2], sc[2][2];
+    orc_strain_constants_tor(Jr, norm, sa, sb, sc);
+    memset(f, 0, (size_t)m.nr * 8);
+    orc_dissipative_rhs_tor(ne, m.r, sa, sb, sc, norm, A, B, C, f);
+    free(A); free(B); free(C); orc_mesh_free(&m);
+    return m.nr;
 }
 
 int orc_mesh_c(const vh_params* p, double* r, int* lay, int cap, char* err) {

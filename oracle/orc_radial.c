@@ -576,10 +576,10 @@ void orc_strain_constants(double Jr, double norm[4], double sa[4][4], double sb[
 }
 
 /* -int tau:delta eps, 2-point radial Gauss (src/vilma_viscoelastic.f90:441-480).
- * A/B/C are [e][lam]. */
+ * A/B/C are [e][ld] (ld = 4, or 6 with the toroidal channels appended). */
 void orc_dissipative_rhs(int ne, const double* r, double sa[4][4], double sb[4][4], double sc[4][4],
                          const double norm[4], const double* A, const double* B, const double* C,
-                         double* f) {
+                         int ld, double* f) {
     const double xg = 0.5773502691896257;
     const double gp[2] = {-xg, xg};
     for (int e = 0; e < ne; e++) {
@@ -590,7 +590,7 @@ void orc_dissipative_rhs(int ne, const double* r, double sa[4][4], double sb[4][
             double psik = (rk1 - ra) / h, psik1 = (ra - rk) / h;
             double tauV[4];
             for (int m = 0; m < 4; m++)
-                tauV[m] = A[e * 4 + m] / h + B[e * 4 + m] * psik / ra + C[e * 4 + m] * psik1 / ra;
+                tauV[m] = A[e * ld + m] / h + B[e * ld + m] * psik / ra + C[e * ld + m] * psik1 / ra;
             for (int t = 0; t < 4; t++) {
                 double D = 0.0;
                 for (int m = 0; m < 4; m++) {
@@ -609,7 +609,7 @@ void orc_dissipative_rhs(int ne, const double* r, double sa[4][4], double sb[4][
 
 /* forward-Euler memory update (src/vilma_viscoelastic.f90:482-589, FE at :577-584) */
 void orc_advance_memory_fe(int ne, const double* mu, const double* Mk, const double* Un,
-                           const double* Vn, double Jr, double* A, double* B, double* C,
+                           const double* Vn, double Jr, double* A, double* B, double* C, int ld,
                            const unsigned char* active) {
     for (int e = 0; e < ne; e++) {
         if (active && !active[e]) continue;
@@ -617,9 +617,9 @@ void orc_advance_memory_fe(int ne, const double* mu, const double* Mk, const dou
         orc_strain_coeffs(Un[e], Un[e + 1], Vn[e], Vn[e + 1], Jr, a, b, c);
         double om = 1.0 - Mk[e], two_muM = 2.0 * mu[e] * Mk[e];
         for (int m = 0; m < 4; m++) {
-            A[e * 4 + m] = om * A[e * 4 + m] - two_muM * a[m];
-            B[e * 4 + m] = om * B[e * 4 + m] - two_muM * b[m];
-            C[e * 4 + m] = om * C[e * 4 + m] - two_muM * c[m];
+            A[e * ld + m] = om * A[e * ld + m] - two_muM * a[m];
+            B[e * ld + m] = om * B[e * ld + m] - two_muM * b[m];
+            C[e * ld + m] = om * C[e * ld + m] - two_muM * c[m];
         }
     }
 }
@@ -630,7 +630,7 @@ void orc_advance_memory_fe(int ne, const double* mu, const double* Mk, const dou
  * entry. */
 void orc_advance_memory_trap(int ne, const double* mu, const double* Mk, const double* Un, const double* Vn,
                              const double* Up, const double* Vp, double Jr, double* A, double* B, double* C,
-                             const unsigned char* active) {
+                             int ld, const unsigned char* active) {
     for (int e = 0; e < ne; e++) {
         if (active && !active[e]) continue;
         double a[4], b[4], c[4], ap[4], bp[4], cp[4];
@@ -639,9 +639,166 @@ void orc_advance_memory_trap(int ne, const double* mu, const double* Mk, const d
         double den = 1.0 + 0.5 * Mk[e];
         double c_old = (1.0 - 0.5 * Mk[e]) / den, w_eps = mu[e] * Mk[e] / den;
         for (int m = 0; m < 4; m++) {
-            A[e * 4 + m] = c_old * A[e * 4 + m] - w_eps * (a[m] + ap[m]);
-            B[e * 4 + m] = c_old * B[e * 4 + m] - w_eps * (b[m] + bp[m]);
-            C[e * 4 + m] = c_old * C[e * 4 + m] - w_eps * (c[m] + cp[m]);
+            A[e * ld + m] = c_old * A[e * ld + m] - w_eps * (a[m] + ap[m]);
+            B[e * ld + m] = c_old * B[e * ld + m] - w_eps * (b[m] + bp[m]);
+            C[e * ld + m] = c_old * C[e * ld + m] - w_eps * (c[m] + cp[m]);
+        }
+    }
+}
+
+/* ------------------------------------------------ toroidal channels */
+
+/* int psi_a r^3 dr (src/vilma_radial_integrals.f90:117-126) */
+static void el_k4(double rk, double rk1, double v[2]) {
+    double h = rk1 - rk;
+    v[0] = h / 20.0 * (rk1 * rk1 * rk1 + 2.0 * rk1 * rk1 * rk + 3.0 * rk1 * rk * rk + 4.0 * rk * rk * rk);
+    v[1] = h / 20.0 * (4.0 * rk1 * rk1 * rk1 + 3.0 * rk1 * rk1 * rk + 2.0 * rk1 * rk * rk + rk * rk * rk);
+}
+
+/* build_toroidal_operator (src/vilma_radial_fe.f90:699-726): the eq-80 W
+ * block of degree j, dense column-major nr x nr,
+ *   mu_k { J [I1 - I3(b,a) - I3(a,b) + I6] + J (J - 2) I6 } W^a dW^b */
+double* orc_build_tor_operator(const double* r, const double* mu, int ne, int j) {
+    int nr = ne + 1;
+    double Jr = (double)j * (double)(j + 1);
+    double* A = calloc((size_t)nr * nr, sizeof(double));
+    for (int e = 0; e < ne; e++) {
+        if (mu[e] == 0.0) continue;
+        double i1[2][2], i3[2][2], i6[2][2];
+        el_i1(r[e], r[e + 1], i1); el_i3(r[e], r[e + 1], i3); el_i6(r[e], r[e + 1], i6);
+        int gmap[2] = {e, e + 1};
+        for (int ia = 0; ia < 2; ia++)
+            for (int ib = 0; ib < 2; ib++)
+                A[gmap[ib] + (size_t)gmap[ia] * nr] += mu[e] * (Jr * (i1[ia][ib] - i3[ib][ia] - i3[ia][ib] + i6[ia][ib])
+                                                                + Jr * (Jr - 2.0) * i6[ia][ib]);
+    }
+    return A;
+}
+
+/* toroidal_operator_assemble (src/vilma_radial_fe.f90:773-798): pin the nodes
+ * all of whose elements are fluid (:728-740), border degree 1 by one
+ * net-rotation constraint per solid shell (:742-771), factor once. */
+int orc_torop_assemble(orc_torop* op, const double* r, const double* mu, int ne, int j, char* err) {
+    memset(op, 0, sizeof(*op));
+    int nr = ne + 1;
+    op->j = j; op->nr = nr;
+    double* A = orc_build_tor_operator(r, mu, ne, j);
+    op->pinned = malloc(nr);
+    memset(op->pinned, 1, nr);
+    for (int e = 0; e < ne; e++)
+        if (mu[e] > 0.0) { op->pinned[e] = 0; op->pinned[e + 1] = 0; }
+    for (int k = 0; k < nr; k++)
+        if (op->pinned[k]) A[k + (size_t)k * nr] = 1.0;
+    int rc;
+    if (j == 1) {
+        int* shell = calloc(ne, sizeof(int));
+        int ns = 0, prev = 0;
+        for (int e = 0; e < ne; e++) {
+            int solid = mu[e] > 0.0;
+            if (solid && !prev) ns++;
+            if (solid) shell[e] = ns;
+            prev = solid;
+        }
+        if (ns > 4) { free(shell); free(A); snprintf(err, 256, "toroidal_operator_assemble: more than 4 solid shells"); return 1; }
+        double* w = calloc((size_t)nr * (ns > 0 ? ns : 1), sizeof(double));
+        for (int e = 0; e < ne; e++) {
+            if (!shell[e]) continue;
+            double k4[2];
+            el_k4(r[e], r[e + 1], k4);
+            w[e + (size_t)(shell[e] - 1) * nr] += k4[0];
+            w[e + 1 + (size_t)(shell[e] - 1) * nr] += k4[1];
+        }
+        rc = bband_factor(&op->sys, A, nr, w, ns);
+        free(w); free(shell);
+    } else {
+        rc = bband_factor(&op->sys, A, nr, NULL, 0);
+    }
+    free(A);
+    if (rc) { snprintf(err, 256, "bordered_band_factor: band LU factorization failed"); return 1; }
+    return 0;
+}
+
+/* toroidal_operator_solve_vec (src/vilma_radial_fe.f90:800-810): pinned dofs
+ * take zero whatever b holds there */
+void orc_torop_solve(const orc_torop* op, const double* b, double* x) {
+    double* bd = malloc(op->nr * sizeof(double));
+    for (int k = 0; k < op->nr; k++) bd[k] = op->pinned[k] ? 0.0 : b[k];
+    bband_solve(&op->sys, bd, x, NULL);
+    free(bd);
+}
+
+void orc_torop_free(orc_torop* op) { bband_free(&op->sys); free(op->pinned); memset(op, 0, sizeof(*op)); }
+
+/* strain_coeffs_tor (src/vilma_viscoelastic.f90:373-393): lambda = 3, 4 */
+void orc_strain_coeffs_tor(double w1, double w2, double Jr, double a[2], double b[2], double c[2]) {
+    a[0] = -w1 + w2; b[0] = -w1; c[0] = -w2;
+    a[1] = 0.0; b[1] = w1; c[1] = w2;
+    if (Jr <= 2.0) { b[1] = 0.0; c[1] = 0.0; }
+}
+
+/* ve_strain_constants_tor (src/vilma_viscoelastic.f90:395-405) */
+void orc_strain_constants_tor(double Jr, double norm[2], double sa[2][2], double sb[2][2], double sc[2][2]) {
+    norm[0] = 0.5 * Jr; norm[1] = 0.5 * Jr * (Jr - 2.0);
+    orc_strain_coeffs_tor(1.0, 0.0, Jr, sa[0], sb[0], sc[0]);
+    orc_strain_coeffs_tor(0.0, 1.0, Jr, sa[1], sb[1], sc[1]);
+}
+
+/* dissipative_rhs_tor (src/vilma_viscoelastic.f90:407-439): the forcing on
+ * the nodal W test dofs from channels 4, 5 of the 6-channel memory
+ * A/B/C [e][6]; f has nr entries */
+void orc_dissipative_rhs_tor(int ne, const double* r, double sa[2][2], double sb[2][2], double sc[2][2],
+                             const double norm[2], const double* A, const double* B, const double* C, double* f) {
+    const double xg = 0.5773502691896257;
+    const double gp[2] = {-xg, xg};
+    for (int e = 0; e < ne; e++) {
+        double rk = r[e], rk1 = r[e + 1], h = rk1 - rk;
+        double floc[2] = {0, 0};
+        for (int ig = 0; ig < 2; ig++) {
+            double ra = 0.5 * (h * gp[ig] + rk + rk1);
+            double psik = (rk1 - ra) / h, psik1 = (ra - rk) / h;
+            double tauV[2];
+            for (int m = 0; m < 2; m++)
+                tauV[m] = A[e * 6 + 4 + m] / h + B[e * 6 + 4 + m] * psik / ra + C[e * 6 + 4 + m] * psik1 / ra;
+            for (int t = 0; t < 2; t++) {
+                double D = 0.0;
+                for (int m = 0; m < 2; m++) {
+                    double deps = sa[t][m] / h + sb[t][m] * psik / ra + sc[t][m] * psik1 / ra;
+                    D = D + norm[m] * tauV[m] * deps;
+                }
+                floc[t] = floc[t] - D * ra * ra * h * 0.5;
+            }
+        }
+        f[e] += floc[0];
+        f[e + 1] += floc[1];
+    }
+}
+
+/* advance_memory_tor (src/vilma_viscoelastic.f90:591-629) for the forward-
+ * Euler (Wp == NULL) and trapezoidal (Wp = start-of-step W) rules, channels
+ * 4, 5 of A/B/C [e][6] */
+void orc_advance_memory_tor(int ne, const double* mu, const double* Mk, const double* Wn, const double* Wp,
+                            double Jr, double* A, double* B, double* C, const unsigned char* active) {
+    for (int e = 0; e < ne; e++) {
+        if (active && !active[e]) continue;
+        double a[2], b[2], c[2], ap[2], bp[2], cp[2];
+        orc_strain_coeffs_tor(Wn[e], Wn[e + 1], Jr, a, b, c);
+        double* At = A + e * 6 + 4; double* Bt = B + e * 6 + 4; double* Ct = C + e * 6 + 4;
+        if (Wp) {
+            orc_strain_coeffs_tor(Wp[e], Wp[e + 1], Jr, ap, bp, cp);
+            double den = 1.0 + 0.5 * Mk[e];
+            double c_old = (1.0 - 0.5 * Mk[e]) / den, w_eps = mu[e] * Mk[e] / den;
+            for (int m = 0; m < 2; m++) {
+                At[m] = c_old * At[m] - w_eps * (a[m] + ap[m]);
+                Bt[m] = c_old * Bt[m] - w_eps * (b[m] + bp[m]);
+                Ct[m] = c_old * Ct[m] - w_eps * (c[m] + cp[m]);
+            }
+        } else {
+            double om = 1.0 - Mk[e], two_muM = 2.0 * mu[e] * Mk[e];
+            for (int m = 0; m < 2; m++) {
+                At[m] = om * At[m] - two_muM * a[m];
+                Bt[m] = om * Bt[m] - two_muM * b[m];
+                Ct[m] = om * Ct[m] - two_muM * c[m];
+            }
         }
     }
 }

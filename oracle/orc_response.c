@@ -7,8 +7,11 @@
  *   :386-420 (deg1_to_cm), :422-489 (elastic), :542-677 (init_ve),
  *   :694-826 (solve_drift + skip gate), :842-864 (apply), :944-992 (FE
  *   advance), :994-1053 (enable lateral visc), :1171-1319 (3-D advance),
- *   :1665-1811 (set_dt, save/restore, max_rate, memory_norm).
- * Memory arrays keep the reference layout (nlam, ne, nk) column-major.
+ *   :1665-1811 (set_dt, save/restore, max_rate, memory_norm), and the
+ *   toroidal degree of freedom (:694-809 tor branch, :966-991, :1055-1109,
+ *   :1227-1319, :1371-1445, :1524-1596).
+ * Memory arrays keep the reference layout (nlam, ne, nk) column-major, nlam =
+ * 4, or 6 once the toroidal channels are carried.
  */
 #include <float.h>
 #include <math.h>
@@ -53,11 +56,12 @@ int orc_resp_init_elastic(orc_resp* r, const orc_earth* e, const orc_mesh* m, in
 }
 
 int orc_resp_init_ve(orc_resp* r, const orc_earth* e, const orc_mesh* m, const orc_sht* s, double dt, char* err) {
-    int deg1_cm = r->deg1_cm;
+    int deg1_cm = r->deg1_cm, toroidal = r->toroidal;
     double skip_tol = r->skip_tol, visc3d_tol = r->visc3d_tol;
     memset(r, 0, sizeof(*r));
-    r->deg1_cm = deg1_cm; r->skip_tol = skip_tol; r->visc3d_tol = visc3d_tol;
+    r->deg1_cm = deg1_cm; r->skip_tol = skip_tol; r->visc3d_tol = visc3d_tol; r->toroidal = toroidal;
     r->kind = RESP_VE;
+    r->nlam = 4;
     int L = s->lmax;
     r->lmax = L; r->nlm = s->nlm; r->nr = m->nr; r->ne = m->ne; r->ndof = 4 * m->nr - 1;
     r->dt = dt; r->time = 0.0;
@@ -142,6 +146,9 @@ void orc_resp_free(orc_resp* r) {
     free(r->Are0); free(r->Aim0); free(r->Bre0); free(r->Bim0); free(r->Cre0); free(r->Cim0);
     free(r->edUn_re); free(r->edUn_im); free(r->edVn_re); free(r->edVn_im);
     free(r->Are_c); free(r->Aim_c); free(r->Bre_c); free(r->Bim_c); free(r->Cre_c); free(r->Cim_c);
+    if (r->tops) for (int l = 1; l <= r->lmax; l++) orc_torop_free(&r->tops[l - 1]);
+    free(r->tops); free(r->nrmt); free(r->sat); free(r->sbt); free(r->sct);
+    free(r->dWa); free(r->dWa_prev); free(r->dWn_re); free(r->dWn_im); free(r->edWn_re); free(r->edWn_im);
     memset(r, 0, sizeof(*r));
 }
 
@@ -162,10 +169,12 @@ static double absmax6(size_t n, const double* x1, const double* x2, const double
 }
 
 /* solve_drift (src/vilma_response.f90:694-809) into the nodal targets
- * (U, V re/im per slot and node) and the report drift dUa / dFa / dVa */
-static void resp_drift(orc_resp* r, double* tUr, double* tUi, double* tVr, double* tVi) {
+ * (U, V re/im per slot and node; W too once the toroidal channels are
+ * carried) and the report drift dUa / dFa / dVa (and dWa) */
+static void resp_drift(orc_resp* r, double* tUr, double* tUi, double* tVr, double* tVi, double* tWr, double* tWi) {
     int ne = r->ne, nr = r->nr, nd = r->ndof, L = r->lmax;
-    size_t nb = (size_t)4 * ne;
+    const int ld = r->nlam, tor = r->nlam > 4;
+    size_t nb = (size_t)ld * ne;
 #pragma omp parallel for schedule(static)
     for (int k = 0; k < r->nk; k++)
         r->mnorm[k] = absmax6(nb, r->Are + k * nb, r->Bre + k * nb, r->Cre + k * nb,
@@ -176,6 +185,7 @@ static void resp_drift(orc_resp* r, double* tUr, double* tUi, double* tVr, doubl
 #pragma omp parallel
     {
         double *fre = malloc(nd * 8), *fim = malloc(nd * 8), *xre = malloc(nd * 8), *xim = malloc(nd * 8);
+        double *gre = malloc(nr * 8), *gim = malloc(nr * 8);
 #pragma omp for schedule(dynamic)
         for (int l = 1; l <= L; l++) {
             for (int k = r->kbeg[l]; k < r->kbeg[l + 1]; k++) {
@@ -184,13 +194,17 @@ static void resp_drift(orc_resp* r, double* tUr, double* tUi, double* tVr, doubl
                 if (r->mnorm[k] <= thr) {
                     r->dUa[k] = 0.0; r->dFa[k] = 0.0; r->dVa[k] = 0.0;
                     memset(Ur, 0, nr * 8); memset(Ui, 0, nr * 8); memset(Vr, 0, nr * 8); memset(Vi, 0, nr * 8);
+                    if (tor) {
+                        r->dWa[k] = 0.0;
+                        memset(tWr + (size_t)k * nr, 0, nr * 8); memset(tWi + (size_t)k * nr, 0, nr * 8);
+                    }
                     continue;
                 }
                 memset(fre, 0, nd * 8); memset(fim, 0, nd * 8);
                 orc_dissipative_rhs(ne, r->r, r->sa[l], r->sb[l], r->sc[l], r->nrmc[l],
-                                    r->Are + k * nb, r->Bre + k * nb, r->Cre + k * nb, fre);
+                                    r->Are + k * nb, r->Bre + k * nb, r->Cre + k * nb, ld, fre);
                 orc_dissipative_rhs(ne, r->r, r->sa[l], r->sb[l], r->sc[l], r->nrmc[l],
-                                    r->Aim + k * nb, r->Bim + k * nb, r->Cim + k * nb, fim);
+                                    r->Aim + k * nb, r->Bim + k * nb, r->Cim + k * nb, ld, fim);
                 orc_radop_solve(&r->ops[l - 1], fre, xre, NULL);
                 orc_radop_solve(&r->ops[l - 1], fim, xim, NULL);
                 if (l == 1 && r->deg1_cm) { deg1_to_cm(&r->ops[0], xre); deg1_to_cm(&r->ops[0], xim); }
@@ -202,16 +216,29 @@ static void resp_drift(orc_resp* r, double* tUr, double* tUi, double* tVr, doubl
                     Ur[n] = xre[idx_u(n)]; Ui[n] = xim[idx_u(n)];
                     Vr[n] = xre[idx_v(n)]; Vi[n] = xim[idx_v(n)];
                 }
+                if (tor) {
+                    /* the toroidal drift: its own forcing and operator, no
+                     * frame projection (the l = 1 border fixes it) */
+                    double* Wr = tWr + (size_t)k * nr; double* Wi = tWi + (size_t)k * nr;
+                    memset(gre, 0, nr * 8); memset(gim, 0, nr * 8);
+                    orc_dissipative_rhs_tor(ne, r->r, r->sat[l], r->sbt[l], r->sct[l], r->nrmt[l],
+                                            r->Are + k * nb, r->Bre + k * nb, r->Cre + k * nb, gre);
+                    orc_dissipative_rhs_tor(ne, r->r, r->sat[l], r->sbt[l], r->sct[l], r->nrmt[l],
+                                            r->Aim + k * nb, r->Bim + k * nb, r->Cim + k * nb, gim);
+                    orc_torop_solve(&r->tops[l - 1], gre, Wr);
+                    orc_torop_solve(&r->tops[l - 1], gim, Wi);
+                    r->dWa[k] = Wr[nr - 1] + I * Wi[nr - 1];
+                }
             }
         }
-        free(fre); free(fim); free(xre); free(xim);
+        free(fre); free(fim); free(xre); free(xim); free(gre); free(gim);
     }
 }
 
 /* response_begin_step: the drift of the entering memory into dUn / dVn */
 void orc_resp_begin(orc_resp* r) {
     if (r->kind != RESP_VE) return;
-    resp_drift(r, r->dUn_re, r->dUn_im, r->dVn_re, r->dVn_im);
+    resp_drift(r, r->dUn_re, r->dUn_im, r->dVn_re, r->dVn_im, r->dWn_re, r->dWn_im);
 }
 
 /* response_apply for every kind (src/vilma_response.f90:241-252, :376-384, :491-511, :842-864) */
@@ -241,10 +268,11 @@ void orc_resp_apply(const orc_resp* r, const orc_sht* s, const cplx* sigma, cplx
 static void advance_element_3d(orc_resp* r, const orc_sht* s, const cplx* sigma, int e,
                                cplx* cm, cplx* ce, cplx* cdel, double* dtau, double* deps) {
     int nlm = s->nlm, nr = r->nr, ne = r->ne;
-    size_t nb = (size_t)4 * ne;
+    const int nc = r->nlam;
+    size_t nb = (size_t)nc * ne;
     size_t ng = (size_t)s->nphi * s->nlat;
-    /* cm/ce: 3 shapes x [lam][lm] */
-    memset(cm, 0, 3 * 4 * nlm * sizeof(cplx)); memset(ce, 0, 3 * 4 * nlm * sizeof(cplx));
+    /* cm/ce: 3 shapes x [lam][lm], nc channels */
+    memset(cm, 0, 3 * (size_t)nc * nlm * sizeof(cplx)); memset(ce, 0, 3 * (size_t)nc * nlm * sizeof(cplx));
     for (int k = 0; k < r->nk; k++) {
         int l = r->kdeg[k], lm = r->k2lm[k];
         double sre = creal(sigma[lm]), sim = cimag(sigma[lm]);
@@ -257,35 +285,48 @@ static void advance_element_3d(orc_resp* r, const orc_sht* s, const cplx* sigma,
         double ar[4], br[4], cr[4], ai[4], bi[4], ci[4];
         orc_strain_coeffs(Ur, Ur1, Vr, Vr1, r->Jr[l], ar, br, cr);
         orc_strain_coeffs(Ui, Ui1, Vi, Vi1, r->Jr[l], ai, bi, ci);
-        size_t mo = k * nb + (size_t)e * 4;
+        size_t mo = k * nb + (size_t)e * nc;
         for (int lam = 0; lam < 4; lam++) {
-            ce[(0 * 4 + lam) * nlm + lm] = ar[lam] + I * ai[lam];
-            ce[(1 * 4 + lam) * nlm + lm] = br[lam] + I * bi[lam];
-            ce[(2 * 4 + lam) * nlm + lm] = cr[lam] + I * ci[lam];
-            cm[(0 * 4 + lam) * nlm + lm] = r->Are[mo + lam] + I * r->Aim[mo + lam];
-            cm[(1 * 4 + lam) * nlm + lm] = r->Bre[mo + lam] + I * r->Bim[mo + lam];
-            cm[(2 * 4 + lam) * nlm + lm] = r->Cre[mo + lam] + I * r->Cim[mo + lam];
+            ce[(0 * nc + lam) * nlm + lm] = ar[lam] + I * ai[lam];
+            ce[(1 * nc + lam) * nlm + lm] = br[lam] + I * bi[lam];
+            ce[(2 * nc + lam) * nlm + lm] = cr[lam] + I * ci[lam];
+        }
+        if (nc == 6) {
+            /* channels 4, 5: the strain of the nodal W drift (no load term) */
+            double at[2], bt[2], ct[2], ati[2], bti[2], cti[2];
+            orc_strain_coeffs_tor(r->dWn_re[o + e], r->dWn_re[o + e + 1], r->Jr[l], at, bt, ct);
+            orc_strain_coeffs_tor(r->dWn_im[o + e], r->dWn_im[o + e + 1], r->Jr[l], ati, bti, cti);
+            for (int q = 0; q < 2; q++) {
+                ce[(0 * nc + 4 + q) * nlm + lm] = at[q] + I * ati[q];
+                ce[(1 * nc + 4 + q) * nlm + lm] = bt[q] + I * bti[q];
+                ce[(2 * nc + 4 + q) * nlm + lm] = ct[q] + I * cti[q];
+            }
+        }
+        for (int lam = 0; lam < nc; lam++) {
+            cm[(0 * nc + lam) * nlm + lm] = r->Are[mo + lam] + I * r->Aim[mo + lam];
+            cm[(1 * nc + lam) * nlm + lm] = r->Bre[mo + lam] + I * r->Bim[mo + lam];
+            cm[(2 * nc + lam) * nlm + lm] = r->Cre[mo + lam] + I * r->Cim[mo + lam];
         }
     }
     double twoMu = 2.0 * r->mu[e];
     const double* M3 = r->Mk3 + (size_t)e * ng;
     for (int sh = 0; sh < 3; sh++) {
-        cplx* c = cm + (size_t)sh * 4 * nlm;
-        orc_tsh_synth(&r->tsh, s, c, dtau);
-        orc_tsh_synth(&r->tsh, s, ce + (size_t)sh * 4 * nlm, deps);
+        cplx* c = cm + (size_t)sh * nc * nlm;
+        orc_tsh_synth(&r->tsh, s, c, nc, dtau);
+        orc_tsh_synth(&r->tsh, s, ce + (size_t)sh * nc * nlm, nc, deps);
         for (int p = 0; p < 6; p++)
             for (size_t id = 0; id < ng; id++)
                 dtau[p * ng + id] = -M3[id] * (dtau[p * ng + id] + twoMu * deps[p * ng + id]);
-        orc_tsh_analysis(&r->tsh, s, dtau, cdel);
-        for (int q = 0; q < 4 * nlm; q++) c[q] = c[q] + cdel[q];
+        orc_tsh_analysis(&r->tsh, s, dtau, nc, cdel);
+        for (int q = 0; q < nc * nlm; q++) c[q] = c[q] + cdel[q];
     }
     for (int k = 0; k < r->nk; k++) {
         int lm = r->k2lm[k];
-        size_t mo = k * nb + (size_t)e * 4;
-        for (int lam = 0; lam < 4; lam++) {
-            r->Are[mo + lam] = creal(cm[(0 * 4 + lam) * nlm + lm]); r->Aim[mo + lam] = cimag(cm[(0 * 4 + lam) * nlm + lm]);
-            r->Bre[mo + lam] = creal(cm[(1 * 4 + lam) * nlm + lm]); r->Bim[mo + lam] = cimag(cm[(1 * 4 + lam) * nlm + lm]);
-            r->Cre[mo + lam] = creal(cm[(2 * 4 + lam) * nlm + lm]); r->Cim[mo + lam] = cimag(cm[(2 * 4 + lam) * nlm + lm]);
+        size_t mo = k * nb + (size_t)e * nc;
+        for (int lam = 0; lam < nc; lam++) {
+            r->Are[mo + lam] = creal(cm[(0 * nc + lam) * nlm + lm]); r->Aim[mo + lam] = cimag(cm[(0 * nc + lam) * nlm + lm]);
+            r->Bre[mo + lam] = creal(cm[(1 * nc + lam) * nlm + lm]); r->Bim[mo + lam] = cimag(cm[(1 * nc + lam) * nlm + lm]);
+            r->Cre[mo + lam] = creal(cm[(2 * nc + lam) * nlm + lm]); r->Cim[mo + lam] = cimag(cm[(2 * nc + lam) * nlm + lm]);
         }
     }
 }
@@ -294,15 +335,16 @@ static void advance_element_3d(orc_resp* r, const orc_sht* s, const cplx* sigma,
 void orc_resp_advance(orc_resp* r, const orc_sht* s, const cplx* sigma) {
     if (r->kind != RESP_VE) return;
     int nr = r->nr, ne = r->ne;
-    size_t nb = (size_t)4 * ne;
+    const int ld = r->nlam;
+    size_t nb = (size_t)ld * ne;
     if (r->lat_visc && r->ne3d > 0) {
         size_t ng = (size_t)s->nphi * s->nlat;
         int nlm = s->nlm;
 #pragma omp parallel
         {
-            cplx* cm = malloc((size_t)12 * nlm * sizeof(cplx));
-            cplx* ce = malloc((size_t)12 * nlm * sizeof(cplx));
-            cplx* cdel = malloc((size_t)4 * nlm * sizeof(cplx));
+            cplx* cm = malloc((size_t)18 * nlm * sizeof(cplx));
+            cplx* ce = malloc((size_t)18 * nlm * sizeof(cplx));
+            cplx* cdel = malloc((size_t)6 * nlm * sizeof(cplx));
             double* dtau = malloc(6 * ng * 8); double* deps = malloc(6 * ng * 8);
 #pragma omp for schedule(dynamic)
             for (int ei = 0; ei < r->ne3d; ei++) advance_element_3d(r, s, sigma, r->e3d[ei], cm, ce, cdel, dtau, deps);
@@ -324,8 +366,14 @@ void orc_resp_advance(orc_resp* r, const orc_sht* s, const cplx* sigma) {
                 Vre[n] = sre * xV[n] + r->dVn_re[o + n];
                 Vim[n] = sim * xV[n] + r->dVn_im[o + n];
             }
-            orc_advance_memory_fe(ne, r->mu, r->Mk, Ure, Vre, r->Jr[l], r->Are + k * nb, r->Bre + k * nb, r->Cre + k * nb, r->active1d);
-            orc_advance_memory_fe(ne, r->mu, r->Mk, Uim, Vim, r->Jr[l], r->Aim + k * nb, r->Bim + k * nb, r->Cim + k * nb, r->active1d);
+            orc_advance_memory_fe(ne, r->mu, r->Mk, Ure, Vre, r->Jr[l], r->Are + k * nb, r->Bre + k * nb, r->Cre + k * nb, ld, r->active1d);
+            orc_advance_memory_fe(ne, r->mu, r->Mk, Uim, Vim, r->Jr[l], r->Aim + k * nb, r->Bim + k * nb, r->Cim + k * nb, ld, r->active1d);
+            if (ld > 4) {   /* W is pure drift: no sigma term */
+                orc_advance_memory_tor(ne, r->mu, r->Mk, r->dWn_re + o, NULL, r->Jr[l], r->Are + k * nb, r->Bre + k * nb,
+                                       r->Cre + k * nb, r->active1d);
+                orc_advance_memory_tor(ne, r->mu, r->Mk, r->dWn_im + o, NULL, r->Jr[l], r->Aim + k * nb, r->Bim + k * nb,
+                                       r->Cim + k * nb, r->active1d);
+            }
         }
         free(Ure); free(Uim); free(Vre); free(Vim);
     }
@@ -354,7 +402,7 @@ double orc_resp_max_rate(const orc_resp* r) {
 
 double orc_resp_memory_norm(const orc_resp* r) {
     if (r->kind != RESP_VE || !r->Are) return 0.0;
-    size_t nb = (size_t)4 * r->ne;
+    size_t nb = (size_t)r->nlam * r->ne;
     /* response_memory_norm (src/vilma_response.f90:1795-1811): per-slot
      * maxima in parallel, then one ordered NaN-keeping max */
     double* sk = malloc((size_t)r->nk * sizeof(double));
@@ -370,7 +418,7 @@ double orc_resp_memory_norm(const orc_resp* r) {
 
 void orc_resp_save(orc_resp* r) {
     if (r->kind != RESP_VE) { r->time_s = r->time; return; }
-    size_t n = (size_t)4 * r->ne * r->nk;
+    size_t n = (size_t)r->nlam * r->ne * r->nk;
     if (!r->Are_s) {
         r->Are_s = malloc(n * 8); r->Aim_s = malloc(n * 8); r->Bre_s = malloc(n * 8);
         r->Bim_s = malloc(n * 8); r->Cre_s = malloc(n * 8); r->Cim_s = malloc(n * 8);
@@ -386,15 +434,58 @@ void orc_resp_save(orc_resp* r) {
 void orc_resp_restore(orc_resp* r) {
     r->time = r->time_s;
     if (r->kind != RESP_VE) return;
-    size_t n = (size_t)4 * r->ne * r->nk;
+    size_t n = (size_t)r->nlam * r->ne * r->nk;
     memcpy(r->Are, r->Are_s, n * 8); memcpy(r->Aim, r->Aim_s, n * 8); memcpy(r->Bre, r->Bre_s, n * 8);
     memcpy(r->Bim, r->Bim_s, n * 8); memcpy(r->Cre, r->Cre_s, n * 8); memcpy(r->Cim, r->Cim_s, n * 8);
     if (r->sigma_n) memcpy(r->sigma_n, r->sigma_n_s, (size_t)r->nlm * sizeof(cplx));
     r->sigma_primed = r->sigma_primed_s;
 }
 
-/* response_enable_lateral_visc (src/vilma_response.f90:994-1053); toroidal
- * channels are not carried (l_toroidal = .false.). pert is [e][nlat][nphi]. */
+/* (nlam, ne, nk) -> (6, ne, nk), spheroidal channels kept, the new ones zero
+ * (enable_toroidal's widen, src/vilma_response.f90:1092-1108) */
+static void widen(double** x, int ne, int nk) {
+    if (!*x) return;
+    double* y = calloc((size_t)6 * ne * nk, sizeof(double));
+    for (size_t ke = 0; ke < (size_t)ne * nk; ke++) memcpy(y + ke * 6, *x + ke * 4, 4 * sizeof(double));
+    free(*x);
+    *x = y;
+}
+
+/* enable_toroidal (src/vilma_response.f90:1055-1109): per-degree W operators
+ * and constants, the W drift arrays, and every memory array and state buffer
+ * widened to six channels */
+static int enable_toroidal(orc_resp* r, char* err) {
+    int L = r->lmax, fail = 0;
+    r->tops = calloc(L, sizeof(orc_torop));
+    r->nrmt = calloc(L + 1, sizeof(*r->nrmt));
+    r->sat = calloc(L + 1, sizeof(*r->sat)); r->sbt = calloc(L + 1, sizeof(*r->sbt)); r->sct = calloc(L + 1, sizeof(*r->sct));
+#pragma omp parallel for schedule(dynamic)
+    for (int l = 1; l <= L; l++) {
+        char e2[256];
+        if (orc_torop_assemble(&r->tops[l - 1], r->r, r->mu, r->ne, l, e2)) fail = 1;
+        orc_strain_constants_tor(r->Jr[l], r->nrmt[l], r->sat[l], r->sbt[l], r->sct[l]);
+    }
+    if (fail) { snprintf(err, 256, "toroidal_operator_assemble: band LU factorization failed"); return 1; }
+    size_t nn = (size_t)r->nr * r->nk;
+    r->dWa = calloc(r->nk ? r->nk : 1, sizeof(cplx));
+    r->dWn_re = calloc(nn, 8); r->dWn_im = calloc(nn, 8);
+    r->nlam = 6;
+    double** arrs[18] = {&r->Are, &r->Aim, &r->Bre, &r->Bim, &r->Cre, &r->Cim,
+                         &r->Are0, &r->Aim0, &r->Bre0, &r->Bim0, &r->Cre0, &r->Cim0,
+                         &r->Are_s, &r->Aim_s, &r->Bre_s, &r->Bim_s, &r->Cre_s, &r->Cim_s};
+    for (int q = 0; q < 18; q++) widen(arrs[q], r->ne, r->nk);
+    double** arrc[6] = {&r->Are_c, &r->Aim_c, &r->Bre_c, &r->Bim_c, &r->Cre_c, &r->Cim_c};
+    for (int q = 0; q < 6; q++) widen(arrc[q], r->ne, r->nk);
+    if (r->Are0) {   /* ensure_commit_scratch_tor (:1111-1118) */
+        r->edWn_re = calloc(nn, 8); r->edWn_im = calloc(nn, 8);
+        r->dWa_prev = calloc(r->nk ? r->nk : 1, sizeof(cplx));
+    }
+    return 0;
+}
+
+/* response_enable_lateral_visc (src/vilma_response.f90:994-1053); with the
+ * toroidal flag, the first genuinely 3-D element turns the toroidal channels
+ * on for good. pert is [e][nlat][nphi]. */
 int orc_resp_enable_lateral(orc_resp* r, const orc_sht* s, const double* pert, char* err) {
     if (r->kind != RESP_VE) { snprintf(err, 256, "solid_earth_enable_visc_3d: lateral viscosity needs earth_response=ve"); return 1; }
     size_t ng = (size_t)s->nphi * s->nlat;
@@ -422,6 +513,7 @@ int orc_resp_enable_lateral(orc_resp* r, const orc_sht* s, const double* pert, c
     int q = 0;
     for (int e = 0; e < r->ne; e++) if (!r->active1d[e]) r->e3d[q++] = e;
     r->lat_visc = 1;
+    if (r->toroidal && r->ne3d > 0 && r->nlam == 4) return enable_toroidal(r, err);
     return 0;
 }
 
@@ -437,7 +529,7 @@ void orc_resp_set_implicit(orc_resp* r, int nlm) {
     r->couple_tol = 1.0e-6;   /* src/vilma_response.f90:177 */
     r->nlm = nlm;
     if (r->kind != RESP_VE || r->Are0) return;
-    size_t n = (size_t)4 * r->ne * r->nk, nn = (size_t)r->nk * r->nr;
+    size_t n = (size_t)r->nlam * r->ne * r->nk, nn = (size_t)r->nk * r->nr;
     r->Are0 = dupz(n); r->Aim0 = dupz(n); r->Bre0 = dupz(n); r->Bim0 = dupz(n); r->Cre0 = dupz(n); r->Cim0 = dupz(n);
     r->Are_c = dupz(n); r->Aim_c = dupz(n); r->Bre_c = dupz(n); r->Bim_c = dupz(n); r->Cre_c = dupz(n); r->Cim_c = dupz(n);
     if (!r->Are_s) {
@@ -446,6 +538,10 @@ void orc_resp_set_implicit(orc_resp* r, int nlm) {
     r->edUn_re = dupz(nn); r->edUn_im = dupz(nn); r->edVn_re = dupz(nn); r->edVn_im = dupz(nn);
     r->sigma_n = calloc(nlm, sizeof(cplx)); r->sigma_next = calloc(nlm, sizeof(cplx));
     r->sigma_n_s = calloc(nlm, sizeof(cplx)); r->dUa_prev = calloc(r->nk ? r->nk : 1, sizeof(cplx));
+    if (r->nlam > 4) {
+        r->edWn_re = dupz(nn); r->edWn_im = dupz(nn);
+        r->dWa_prev = calloc(r->nk ? r->nk : 1, sizeof(cplx));
+    }
 }
 
 /* ve_response_prepare_endpoint (:1598-1616): snapshot tau_n, seed the
@@ -454,12 +550,16 @@ void orc_resp_prepare_endpoint(orc_resp* r) {
     r->couple_pass = 0;
     r->couple_done = 0;
     if (!r->implicit || r->kind != RESP_VE) return;
-    size_t n = (size_t)4 * r->ne * r->nk, nn = (size_t)r->nk * r->nr;
+    size_t n = (size_t)r->nlam * r->ne * r->nk, nn = (size_t)r->nk * r->nr;
     memcpy(r->Are0, r->Are, n * 8); memcpy(r->Aim0, r->Aim, n * 8); memcpy(r->Bre0, r->Bre, n * 8);
     memcpy(r->Bim0, r->Bim, n * 8); memcpy(r->Cre0, r->Cre, n * 8); memcpy(r->Cim0, r->Cim, n * 8);
     memcpy(r->edUn_re, r->dUn_re, nn * 8); memcpy(r->edUn_im, r->dUn_im, nn * 8);
     memcpy(r->edVn_re, r->dVn_re, nn * 8); memcpy(r->edVn_im, r->dVn_im, nn * 8);
     memcpy(r->dUa_prev, r->dUa, (size_t)r->nk * sizeof(cplx));
+    if (r->nlam > 4) {
+        memcpy(r->edWn_re, r->dWn_re, nn * 8); memcpy(r->edWn_im, r->dWn_im, nn * 8);
+        memcpy(r->dWa_prev, r->dWa, (size_t)r->nk * sizeof(cplx));
+    }
 }
 
 /* advance_shape_tensor_trap for the three shapes of element e
@@ -469,9 +569,10 @@ void orc_resp_prepare_endpoint(orc_resp* r) {
 static void advance_element_3d_trap(orc_resp* r, const orc_sht* s, const cplx* sigma, const cplx* sig_n, int e,
                                     cplx* cm, cplx* cn, cplx* c1, cplx* cdel, double* dt0, double* den, double* de1) {
     int nlm = s->nlm, nr = r->nr, ne = r->ne;
-    size_t nb = (size_t)4 * ne, ng = (size_t)s->nphi * s->nlat;
-    memset(cm, 0, 12 * (size_t)nlm * sizeof(cplx)); memset(cn, 0, 12 * (size_t)nlm * sizeof(cplx));
-    memset(c1, 0, 12 * (size_t)nlm * sizeof(cplx));
+    const int nc = r->nlam;
+    size_t nb = (size_t)nc * ne, ng = (size_t)s->nphi * s->nlat;
+    memset(cm, 0, 3 * (size_t)nc * nlm * sizeof(cplx)); memset(cn, 0, 3 * (size_t)nc * nlm * sizeof(cplx));
+    memset(c1, 0, 3 * (size_t)nc * nlm * sizeof(cplx));
     for (int k = 0; k < r->nk; k++) {
         int l = r->kdeg[k], lm = r->k2lm[k];
         double sre = creal(sigma[lm]), sim = cimag(sigma[lm]);
@@ -487,40 +588,58 @@ static void advance_element_3d_trap(orc_resp* r, const orc_sht* s, const cplx* s
                           sre * xV[e] + r->edVn_re[o + e], sre * xV[e + 1] + r->edVn_re[o + e + 1], r->Jr[l], a[2], b[2], c[2]);
         orc_strain_coeffs(sim * xU[e] + r->edUn_im[o + e], sim * xU[e + 1] + r->edUn_im[o + e + 1],
                           sim * xV[e] + r->edVn_im[o + e], sim * xV[e + 1] + r->edVn_im[o + e + 1], r->Jr[l], a[3], b[3], c[3]);
-        size_t mo = k * nb + (size_t)e * 4;
+        size_t mo = k * nb + (size_t)e * nc;
         for (int lam = 0; lam < 4; lam++) {
-            cn[(0 * 4 + lam) * nlm + lm] = a[0][lam] + I * a[1][lam];
-            cn[(1 * 4 + lam) * nlm + lm] = b[0][lam] + I * b[1][lam];
-            cn[(2 * 4 + lam) * nlm + lm] = c[0][lam] + I * c[1][lam];
-            c1[(0 * 4 + lam) * nlm + lm] = a[2][lam] + I * a[3][lam];
-            c1[(1 * 4 + lam) * nlm + lm] = b[2][lam] + I * b[3][lam];
-            c1[(2 * 4 + lam) * nlm + lm] = c[2][lam] + I * c[3][lam];
-            cm[(0 * 4 + lam) * nlm + lm] = r->Are0[mo + lam] + I * r->Aim0[mo + lam];
-            cm[(1 * 4 + lam) * nlm + lm] = r->Bre0[mo + lam] + I * r->Bim0[mo + lam];
-            cm[(2 * 4 + lam) * nlm + lm] = r->Cre0[mo + lam] + I * r->Cim0[mo + lam];
+            cn[(0 * nc + lam) * nlm + lm] = a[0][lam] + I * a[1][lam];
+            cn[(1 * nc + lam) * nlm + lm] = b[0][lam] + I * b[1][lam];
+            cn[(2 * nc + lam) * nlm + lm] = c[0][lam] + I * c[1][lam];
+            c1[(0 * nc + lam) * nlm + lm] = a[2][lam] + I * a[3][lam];
+            c1[(1 * nc + lam) * nlm + lm] = b[2][lam] + I * b[3][lam];
+            c1[(2 * nc + lam) * nlm + lm] = c[2][lam] + I * c[3][lam];
+        }
+        if (nc == 6) {
+            /* W: eps_n from the tau_n drift dWn, eps_{n+1} from the endpoint's edWn */
+            double ta[4][2], tb[4][2], tc[4][2];
+            orc_strain_coeffs_tor(r->dWn_re[o + e], r->dWn_re[o + e + 1], r->Jr[l], ta[0], tb[0], tc[0]);
+            orc_strain_coeffs_tor(r->dWn_im[o + e], r->dWn_im[o + e + 1], r->Jr[l], ta[1], tb[1], tc[1]);
+            orc_strain_coeffs_tor(r->edWn_re[o + e], r->edWn_re[o + e + 1], r->Jr[l], ta[2], tb[2], tc[2]);
+            orc_strain_coeffs_tor(r->edWn_im[o + e], r->edWn_im[o + e + 1], r->Jr[l], ta[3], tb[3], tc[3]);
+            for (int q = 0; q < 2; q++) {
+                cn[(0 * nc + 4 + q) * nlm + lm] = ta[0][q] + I * ta[1][q];
+                cn[(1 * nc + 4 + q) * nlm + lm] = tb[0][q] + I * tb[1][q];
+                cn[(2 * nc + 4 + q) * nlm + lm] = tc[0][q] + I * tc[1][q];
+                c1[(0 * nc + 4 + q) * nlm + lm] = ta[2][q] + I * ta[3][q];
+                c1[(1 * nc + 4 + q) * nlm + lm] = tb[2][q] + I * tb[3][q];
+                c1[(2 * nc + 4 + q) * nlm + lm] = tc[2][q] + I * tc[3][q];
+            }
+        }
+        for (int lam = 0; lam < nc; lam++) {
+            cm[(0 * nc + lam) * nlm + lm] = r->Are0[mo + lam] + I * r->Aim0[mo + lam];
+            cm[(1 * nc + lam) * nlm + lm] = r->Bre0[mo + lam] + I * r->Bim0[mo + lam];
+            cm[(2 * nc + lam) * nlm + lm] = r->Cre0[mo + lam] + I * r->Cim0[mo + lam];
         }
     }
     const double* M3 = r->Mk3 + (size_t)e * ng;
     for (int sh = 0; sh < 3; sh++) {
-        cplx* cc = cm + (size_t)sh * 4 * nlm;
-        orc_tsh_synth(&r->tsh, s, cc, dt0);
-        orc_tsh_synth(&r->tsh, s, cn + (size_t)sh * 4 * nlm, den);
-        orc_tsh_synth(&r->tsh, s, c1 + (size_t)sh * 4 * nlm, de1);
+        cplx* cc = cm + (size_t)sh * nc * nlm;
+        orc_tsh_synth(&r->tsh, s, cc, nc, dt0);
+        orc_tsh_synth(&r->tsh, s, cn + (size_t)sh * nc * nlm, nc, den);
+        orc_tsh_synth(&r->tsh, s, c1 + (size_t)sh * nc * nlm, nc, de1);
         for (size_t id = 0; id < ng; id++) {
             const double wt = M3[id] / (1.0 + 0.5 * M3[id]), we = r->mu[e] * M3[id] / (1.0 + 0.5 * M3[id]);
             for (int p = 0; p < 6; p++)
                 dt0[p * ng + id] = -wt * dt0[p * ng + id] - we * (den[p * ng + id] + de1[p * ng + id]);
         }
-        orc_tsh_analysis(&r->tsh, s, dt0, cdel);
-        for (int q = 0; q < 4 * nlm; q++) cc[q] = cc[q] + cdel[q];
+        orc_tsh_analysis(&r->tsh, s, dt0, nc, cdel);
+        for (int q = 0; q < nc * nlm; q++) cc[q] = cc[q] + cdel[q];
     }
     for (int k = 0; k < r->nk; k++) {
         int lm = r->k2lm[k];
-        size_t mo = k * nb + (size_t)e * 4;
-        for (int lam = 0; lam < 4; lam++) {
-            r->Are[mo + lam] = creal(cm[(0 * 4 + lam) * nlm + lm]); r->Aim[mo + lam] = cimag(cm[(0 * 4 + lam) * nlm + lm]);
-            r->Bre[mo + lam] = creal(cm[(1 * 4 + lam) * nlm + lm]); r->Bim[mo + lam] = cimag(cm[(1 * 4 + lam) * nlm + lm]);
-            r->Cre[mo + lam] = creal(cm[(2 * 4 + lam) * nlm + lm]); r->Cim[mo + lam] = cimag(cm[(2 * 4 + lam) * nlm + lm]);
+        size_t mo = k * nb + (size_t)e * nc;
+        for (int lam = 0; lam < nc; lam++) {
+            r->Are[mo + lam] = creal(cm[(0 * nc + lam) * nlm + lm]); r->Aim[mo + lam] = cimag(cm[(0 * nc + lam) * nlm + lm]);
+            r->Bre[mo + lam] = creal(cm[(1 * nc + lam) * nlm + lm]); r->Bim[mo + lam] = cimag(cm[(1 * nc + lam) * nlm + lm]);
+            r->Cre[mo + lam] = creal(cm[(2 * nc + lam) * nlm + lm]); r->Cim[mo + lam] = cimag(cm[(2 * nc + lam) * nlm + lm]);
         }
     }
 }
@@ -530,7 +649,8 @@ static void advance_element_3d_trap(orc_resp* r, const orc_sht* s, const cplx* s
  * the snapshot */
 static void trapezoid_advance_all(orc_resp* r, const orc_sht* s, const cplx* sigma) {
     int nr = r->nr, ne = r->ne;
-    size_t nb = (size_t)4 * ne;
+    const int ld = r->nlam;
+    size_t nb = (size_t)ld * ne;
     const cplx* sig_n = r->sigma_primed ? r->sigma_n : sigma;   /* first step: eps_n from sigma_{n+1} */
 #pragma omp parallel
     {
@@ -552,11 +672,17 @@ static void trapezoid_advance_all(orc_resp* r, const orc_sht* s, const cplx* sig
             memcpy(r->Are + k * nb, r->Are0 + k * nb, nb * 8); memcpy(r->Bre + k * nb, r->Bre0 + k * nb, nb * 8);
             memcpy(r->Cre + k * nb, r->Cre0 + k * nb, nb * 8);
             orc_advance_memory_trap(ne, r->mu, r->Mk, Ur, Vr, Urn, Vrn, r->Jr[l], r->Are + k * nb, r->Bre + k * nb,
-                                    r->Cre + k * nb, r->active1d);
+                                    r->Cre + k * nb, ld, r->active1d);
             memcpy(r->Aim + k * nb, r->Aim0 + k * nb, nb * 8); memcpy(r->Bim + k * nb, r->Bim0 + k * nb, nb * 8);
             memcpy(r->Cim + k * nb, r->Cim0 + k * nb, nb * 8);
             orc_advance_memory_trap(ne, r->mu, r->Mk, Ui, Vi, Uin, Vin, r->Jr[l], r->Aim + k * nb, r->Bim + k * nb,
-                                    r->Cim + k * nb, r->active1d);
+                                    r->Cim + k * nb, ld, r->active1d);
+            if (ld > 4) {   /* W: eps_n from tau_n's drift, eps_{n+1} from the endpoint's */
+                orc_advance_memory_tor(ne, r->mu, r->Mk, r->edWn_re + o, r->dWn_re + o, r->Jr[l], r->Are + k * nb,
+                                       r->Bre + k * nb, r->Cre + k * nb, r->active1d);
+                orc_advance_memory_tor(ne, r->mu, r->Mk, r->edWn_im + o, r->dWn_im + o, r->Jr[l], r->Aim + k * nb,
+                                       r->Bim + k * nb, r->Cim + k * nb, r->active1d);
+            }
         }
         free(Ur); free(Ui); free(Vr); free(Vi); free(Urn); free(Uin); free(Vrn); free(Vin);
     }
@@ -565,10 +691,10 @@ static void trapezoid_advance_all(orc_resp* r, const orc_sht* s, const cplx* sig
         int nlm = s->nlm;
 #pragma omp parallel
         {
-            cplx* cm = malloc((size_t)12 * nlm * sizeof(cplx));
-            cplx* cn = malloc((size_t)12 * nlm * sizeof(cplx));
-            cplx* c1 = malloc((size_t)12 * nlm * sizeof(cplx));
-            cplx* cdel = malloc((size_t)4 * nlm * sizeof(cplx));
+            cplx* cm = malloc((size_t)18 * nlm * sizeof(cplx));
+            cplx* cn = malloc((size_t)18 * nlm * sizeof(cplx));
+            cplx* c1 = malloc((size_t)18 * nlm * sizeof(cplx));
+            cplx* cdel = malloc((size_t)6 * nlm * sizeof(cplx));
             double *dt0 = malloc(6 * ng * 8), *den = malloc(6 * ng * 8), *de1 = malloc(6 * ng * 8);
 #pragma omp for schedule(dynamic)
             for (int ei = 0; ei < r->ne3d; ei++)
@@ -591,15 +717,21 @@ void orc_resp_advance_endpoint(orc_resp* r, const orc_sht* s, const cplx* sigma)
     trapezoid_advance_all(r, s, sigma);
     memcpy(r->sigma_next, sigma, (size_t)r->nlm * sizeof(cplx));
     r->have_next = 1;
-    resp_drift(r, r->edUn_re, r->edUn_im, r->edVn_re, r->edVn_im);
+    resp_drift(r, r->edUn_re, r->edUn_im, r->edVn_re, r->edVn_im, r->edWn_re, r->edWn_im);
     r->couple_pass++;
-    /* drift_change (:1499-1512), complex moduli */
+    /* drift_change (:1499-1512), complex moduli; W(a) joins once carried */
     double cnorm = 0.0, snorm = 0.0;
     for (int k = 0; k < r->nk; k++) {
         cnorm = fmax(cnorm, cabs(r->dUa[k] - r->dUa_prev[k]));
         snorm = fmax(snorm, cabs(r->dUa[k]));
     }
+    if (r->nlam > 4)
+        for (int k = 0; k < r->nk; k++) {
+            cnorm = fmax(cnorm, cabs(r->dWa[k] - r->dWa_prev[k]));
+            snorm = fmax(snorm, cabs(r->dWa[k]));
+        }
     memcpy(r->dUa_prev, r->dUa, (size_t)r->nk * sizeof(cplx));
+    if (r->nlam > 4) memcpy(r->dWa_prev, r->dWa, (size_t)r->nk * sizeof(cplx));
     r->couple_done = r->couple_pass >= 2 && cnorm <= r->couple_tol * fmax(snorm, DBL_MIN);
 }
 
@@ -622,7 +754,7 @@ void orc_resp_prime_sigma(orc_resp* r, const cplx* sigma) {
 /* response_stash_coarse (:1742-1757) */
 void orc_resp_stash_coarse(orc_resp* r) {
     if (r->kind != RESP_VE) return;
-    size_t n = (size_t)4 * r->ne * r->nk;
+    size_t n = (size_t)r->nlam * r->ne * r->nk;
     memcpy(r->Are_c, r->Are, n * 8); memcpy(r->Aim_c, r->Aim, n * 8); memcpy(r->Bre_c, r->Bre, n * 8);
     memcpy(r->Bim_c, r->Bim, n * 8); memcpy(r->Cre_c, r->Cre, n * 8); memcpy(r->Cim_c, r->Cim, n * 8);
 }
@@ -632,7 +764,7 @@ void orc_resp_stash_coarse(orc_resp* r) {
 void orc_resp_coarse_fine_error(const orc_resp* r, double* err_inf, double* tau_inf) {
     *err_inf = 0.0; *tau_inf = 0.0;
     if (r->kind != RESP_VE) return;
-    size_t n = (size_t)4 * r->ne * r->nk;
+    size_t n = (size_t)r->nlam * r->ne * r->nk;
     const double* f[6] = {r->Are, r->Aim, r->Bre, r->Bim, r->Cre, r->Cim};
     const double* c[6] = {r->Are_c, r->Aim_c, r->Bre_c, r->Bim_c, r->Cre_c, r->Cim_c};
     double e = 0.0, t = 0.0;

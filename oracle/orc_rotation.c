@@ -52,8 +52,8 @@ static void ch_free(orc_ch2* c) {
 static void ch_begin(orc_ch2* c) {
     int nd = c->ndof, nr = c->nr;
     double *fre = calloc(nd, 8), *fim = calloc(nd, 8), *xre = malloc(nd * 8), *xim = malloc(nd * 8);
-    orc_dissipative_rhs(c->ne, c->r, c->sa, c->sb, c->sc, c->norm, c->Are, c->Bre, c->Cre, fre);
-    orc_dissipative_rhs(c->ne, c->r, c->sa, c->sb, c->sc, c->norm, c->Aim, c->Bim, c->Cim, fim);
+    orc_dissipative_rhs(c->ne, c->r, c->sa, c->sb, c->sc, c->norm, c->Are, c->Bre, c->Cre, 4, fre);
+    orc_dissipative_rhs(c->ne, c->r, c->sa, c->sb, c->sc, c->norm, c->Aim, c->Bim, c->Cim, 4, fim);
     orc_radop_solve(&c->op, fre, xre, NULL);
     orc_radop_solve(&c->op, fim, xim, NULL);
     c->dF = xre[idx_f(nr - 1)] + I * xim[idx_f(nr - 1)];
@@ -73,8 +73,8 @@ static void ch_commit(orc_ch2* c, cplx coeff) {
         Ure[n] = cr * c->xUn[n] + c->dUn_re[n]; Uim[n] = ci * c->xUn[n] + c->dUn_im[n];
         Vre[n] = cr * c->xVn[n] + c->dVn_re[n]; Vim[n] = ci * c->xVn[n] + c->dVn_im[n];
     }
-    orc_advance_memory_fe(c->ne, c->mu, c->Mk, Ure, Vre, c->Jr, c->Are, c->Bre, c->Cre, NULL);
-    orc_advance_memory_fe(c->ne, c->mu, c->Mk, Uim, Vim, c->Jr, c->Aim, c->Bim, c->Cim, NULL);
+    orc_advance_memory_fe(c->ne, c->mu, c->Mk, Ure, Vre, c->Jr, c->Are, c->Bre, c->Cre, 4, NULL);
+    orc_advance_memory_fe(c->ne, c->mu, c->Mk, Uim, Vim, c->Jr, c->Aim, c->Bim, c->Cim, 4, NULL);
     free(Ure); free(Uim); free(Vre); free(Vim);
 }
 

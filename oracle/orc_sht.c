@@ -346,6 +346,76 @@ void orc_sph_analysis(const orc_sht* s, const double* vt, const double* vp, cplx
     free(FT); free(FP);
 }
 
+/* Toroidal synthesis (vt, vp) = e_r x grad_1 (sum q Y) = (-F, E), the
+ * module's orientation (src/vilma_sht.f90:258-277: the negated SHTns field). */
+void orc_tor_synthesis(const orc_sht* s, const cplx* q, double* vt, double* vp) {
+    int L = s->lmax, nh = s->nh, nl = s->nlat;
+    cplx* FT = malloc((size_t)nl * (L + 1) * sizeof(cplx));
+    cplx* FP = malloc((size_t)nl * (L + 1) * sizeof(cplx));
+#pragma omp parallel for schedule(dynamic)
+    for (int m = 0; m <= L; m++) {
+        for (int j = 0; j < nh; j++) {
+            cplx e = 0.0, o = 0.0, de = 0.0, dO = 0.0;
+            for (int l = m; l <= L; l++) {
+                size_t id = (size_t)(s->mstart[m] + l - m);
+                cplx c = q[id];
+                double P = s->plm[id * nh + j], D = s->dplm[id * nh + j];
+                if ((l + m) % 2 == 0) { e += c * P; de += c * D; }
+                else { o += c * P; dO += c * D; }
+            }
+            cplx mim_s = -I * (double)m / s->st[j];
+            int js = south(s, j);
+            FT[(size_t)j * (L + 1) + m] = mim_s * (e + o);
+            FP[(size_t)j * (L + 1) + m] = de + dO;
+            if (js >= 0) {
+                FT[(size_t)js * (L + 1) + m] = mim_s * (e - o);
+                FP[(size_t)js * (L + 1) + m] = dO - de;
+            }
+        }
+    }
+    orc_fft_rows_inverse(s, FT, nl, vt);
+    orc_fft_rows_inverse(s, FP, nl, vp);
+    free(FT); free(FP);
+}
+
+/* Toroidal part of the vector analysis in the same orientation
+ * (src/vilma_sht.f90:293-308): q_lm = (1/J) int v . conj(e_r x grad_1 Y_lm). */
+void orc_tor_analysis(const orc_sht* s, const double* vt, const double* vp, cplx* q) {
+    int L = s->lmax, nh = s->nh, nl = s->nlat;
+    cplx* FT = malloc((size_t)nl * (L + 1) * sizeof(cplx));
+    cplx* FP = malloc((size_t)nl * (L + 1) * sizeof(cplx));
+    orc_fft_rows_forward(s, vt, nl, FT);
+    orc_fft_rows_forward(s, vp, nl, FP);
+#pragma omp parallel for schedule(dynamic)
+    for (int m = 0; m <= L; m++) {
+        cplx* Te = malloc(nh * sizeof(cplx)); cplx* To = malloc(nh * sizeof(cplx));
+        cplx* Pe = malloc(nh * sizeof(cplx)); cplx* Po = malloc(nh * sizeof(cplx));
+        for (int j = 0; j < nh; j++) {
+            int js = south(s, j);
+            cplx tn = FT[(size_t)j * (L + 1) + m], ts = js >= 0 ? FT[(size_t)js * (L + 1) + m] : 0.0;
+            cplx pn = FP[(size_t)j * (L + 1) + m], ps = js >= 0 ? FP[(size_t)js * (L + 1) + m] : 0.0;
+            cplx im_s = I * (double)m / s->st[j];
+            /* vt pairs with (i m/s) P, vp with dP; even l+m: P symmetric, dP antisymmetric */
+            Te[j] = s->w[j] * im_s * (tn + ts); To[j] = s->w[j] * im_s * (tn - ts);
+            Pe[j] = s->w[j] * (pn - ps); Po[j] = s->w[j] * (pn + ps);
+        }
+        for (int l = m; l <= L; l++) {
+            size_t id = (size_t)(s->mstart[m] + l - m);
+            if (l == 0) { q[id] = 0.0; continue; }
+            const double* P = s->plm + id * nh;
+            const double* D = s->dplm + id * nh;
+            int ev = (l + m) % 2 == 0;
+            const cplx* T = ev ? Te : To;
+            const cplx* PP = ev ? Pe : Po;
+            cplx acc = 0.0;
+            for (int j = 0; j < nh; j++) acc += P[j] * T[j] + D[j] * PP[j];
+            q[id] = acc / ((double)l * (l + 1.0));
+        }
+        free(Te); free(To); free(Pe); free(Po);
+    }
+    free(FT); free(FP);
+}
+
 /* Point evaluation of a scalar field (sht_grid_eval_point, src/vilma_sht.f90:310-325). */
 double orc_eval_point(const orc_sht* s, const cplx* q, double colat, double lon) {
     int L = s->lmax;
@@ -417,6 +487,33 @@ static void spin2_adjoint(const orc_tsh* t, const orc_sht* s, const double* dtt,
     free(D); free(vt); free(vp); free(q); free(sq);
 }
 
+/* unnormalised adjoint of the toroidal spin-2 synthesis (src/vilma_tensor_sh.f90:
+ * 360-387): 1/2 Sg*(tp) - Sh*(tt - pp) */
+static void spin2_adjoint_tor(const orc_tsh* t, const orc_sht* s, const double* dtt, const double* dpp,
+                              const double* dtp, cplx* craw) {
+    size_t ng = (size_t)s->nphi * s->nlat;
+    int nlm = s->nlm;
+    double *D = malloc(ng * 8), *vt = malloc(ng * 8), *vp = malloc(ng * 8);
+    cplx* q = malloc(nlm * sizeof(cplx));
+    cplx* sq = malloc(nlm * sizeof(cplx));
+    orc_sht_analysis(s, dtp, q);
+    for (int lm = 0; lm < nlm; lm++) craw[lm] = -0.5 * t->llp1[lm] * q[lm];
+    byprof(s, dtp, t->cott, vt); memset(vp, 0, ng * 8);
+    orc_sph_analysis(s, vt, vp, sq);
+    for (int lm = 0; lm < nlm; lm++) craw[lm] = craw[lm] - t->llp1[lm] * sq[lm];
+    memset(vt, 0, ng * 8); byprof(s, dtp, t->invsin, vp);
+    orc_sph_analysis(s, vt, vp, sq);
+    for (int lm = 0; lm < nlm; lm++) craw[lm] = craw[lm] + (I * (double)t->mord[lm]) * t->llp1[lm] * sq[lm];
+    for (size_t id = 0; id < ng; id++) D[id] = dtt[id] - dpp[id];
+    byprof(s, D, t->invsin, vt); memset(vp, 0, ng * 8);
+    orc_sph_analysis(s, vt, vp, sq);
+    for (int lm = 0; lm < nlm; lm++) craw[lm] = craw[lm] + (I * (double)t->mord[lm]) * t->llp1[lm] * sq[lm];
+    memset(vt, 0, ng * 8); byprof(s, D, t->cott, vp);
+    orc_sph_analysis(s, vt, vp, sq);
+    for (int lm = 0; lm < nlm; lm++) craw[lm] = craw[lm] + t->llp1[lm] * sq[lm];
+    free(D); free(vt); free(vp); free(q); free(sq);
+}
+
 /* src/vilma_tensor_sh.f90:76-133 */
 int orc_tsh_init(orc_tsh* t, const orc_sht* s) {
     memset(t, 0, sizeof(*t));
@@ -434,6 +531,7 @@ int orc_tsh_init(orc_tsh* t, const orc_sht* s) {
         t->invsin[j] = 1.0 / sin(s->colat[j]);
     }
     t->n6 = calloc(s->lmax + 1, sizeof(double));
+    t->n4 = calloc(s->lmax + 1, sizeof(double));
     size_t ng = (size_t)s->nphi * s->nlat;
     cplx* c6 = calloc(s->nlm, sizeof(cplx));
     cplx* craw = malloc(s->nlm * sizeof(cplx));
@@ -447,19 +545,24 @@ int orc_tsh_init(orc_tsh* t, const orc_sht* s) {
         for (size_t id = 0; id < ng; id++) { tt[id] = sg[id]; pp[id] = -sg[id]; tp[id] = 4.0 * sh[id]; }
         spin2_adjoint(t, s, tt, pp, tp, craw);
         t->n6[l] = creal(craw[lm]);
+        /* the same coefficient as Z4: tt = -Sh, pp = +Sh, tp = Sg */
+        for (size_t id = 0; id < ng; id++) { tt[id] = -sh[id]; pp[id] = sh[id]; tp[id] = sg[id]; }
+        spin2_adjoint_tor(t, s, tt, pp, tp, craw);
+        t->n4[l] = creal(craw[lm]);
     }
     free(c6); free(craw); free(sg); free(sh); free(tt); free(pp); free(tp);
     return 0;
 }
 
 void orc_tsh_free(orc_tsh* t) {
-    free(t->ldeg); free(t->mord); free(t->llp1); free(t->cott); free(t->invsin); free(t->n6);
+    free(t->ldeg); free(t->mord); free(t->llp1); free(t->cott); free(t->invsin); free(t->n6); free(t->n4);
     memset(t, 0, sizeof(*t));
 }
 
-/* Tensor coefficients c[lam*nlm + lm] (lam = 1,2,5,6 -> 0..3) -> six dyadic
- * planes rr, rt, rp, tt, tp, pp (src/vilma_tensor_sh.f90:168-203). */
-void orc_tsh_synth(const orc_tsh* t, const orc_sht* s, const cplx* c, double* dyad) {
+/* Tensor coefficients c[lam*nlm + lm] (lam = 1,2,5,6 -> 0..3; with nch = 6
+ * also lam = 3,4 -> 4,5) -> six dyadic planes rr, rt, rp, tt, tp, pp
+ * (src/vilma_tensor_sh.f90:168-242). */
+void orc_tsh_synth(const orc_tsh* t, const orc_sht* s, const cplx* c, int nch, double* dyad) {
     size_t ng = (size_t)s->nphi * s->nlat;
     int nlm = s->nlm;
     double *tr = malloc(ng * 8), *sg = malloc(ng * 8), *sh = malloc(ng * 8);
@@ -474,11 +577,26 @@ void orc_tsh_synth(const orc_tsh* t, const orc_sht* s, const cplx* c, double* dy
         dyad[5 * ng + id] = tr[id] - sg[id];          /* pp */
         dyad[4 * ng + id] = 4.0 * sh[id];             /* tp */
     }
+    if (nch == 6) {
+        /* Z3: (rt, rp) += e_r x grad_1; Z4: (tt, pp, tp) += (-Sh, +Sh, Sg) */
+        double *vt = malloc(ng * 8), *vp = malloc(ng * 8);
+        orc_tor_synthesis(s, c + 4 * nlm, vt, vp);
+        spin2_fields(t, s, c + 5 * nlm, sg, sh);
+        for (size_t id = 0; id < ng; id++) {
+            dyad[1 * ng + id] += vt[id];
+            dyad[2 * ng + id] += vp[id];
+            dyad[3 * ng + id] += -sh[id];
+            dyad[5 * ng + id] += sh[id];
+            dyad[4 * ng + id] += sg[id];
+        }
+        free(vt); free(vp);
+    }
     free(tr); free(sg); free(sh); free(sc);
 }
 
-/* Six dyadic planes -> tensor coefficients (src/vilma_tensor_sh.f90:270-322). */
-void orc_tsh_analysis(const orc_tsh* t, const orc_sht* s, const double* dyad, cplx* c) {
+/* Six dyadic planes -> tensor coefficients, nch = 4 or 6 channels
+ * (src/vilma_tensor_sh.f90:270-322). */
+void orc_tsh_analysis(const orc_tsh* t, const orc_sht* s, const double* dyad, int nch, cplx* c) {
     size_t ng = (size_t)s->nphi * s->nlat;
     int nlm = s->nlm;
     double* tmp = malloc(ng * 8);
@@ -493,6 +611,14 @@ void orc_tsh_analysis(const orc_tsh* t, const orc_sht* s, const double* dyad, cp
     for (int lm = 0; lm < nlm; lm++) {
         double n6 = t->n6[t->ldeg[lm]];
         c[3 * nlm + lm] = n6 != 0.0 ? craw[lm] / n6 : 0.0;
+    }
+    if (nch == 6) {
+        orc_tor_analysis(s, dyad + 1 * ng, dyad + 2 * ng, c + 4 * nlm);
+        spin2_adjoint_tor(t, s, dyad + 3 * ng, dyad + 5 * ng, dyad + 4 * ng, craw);
+        for (int lm = 0; lm < nlm; lm++) {
+            double n4 = t->n4[t->ldeg[lm]];
+            c[5 * nlm + lm] = n4 != 0.0 ? craw[lm] / n4 : 0.0;
+        }
     }
     free(tmp); free(craw);
 }
