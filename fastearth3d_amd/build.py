@@ -15,7 +15,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libvilma_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = ["vh_setup.hip", "vh_drift.hip", "vh_advance.hip", "vh_sht.hip", "vh_sle.hip", "vh_rot.hip",
-           "vh_tensor.hip", "vh_adjoint.hip", "vh_remap.hip", "vh_api.hip", "vh_ensemble.hip"]
+           "vh_tensor.hip", "vh_toroidal.hip", "vh_adjoint.hip", "vh_remap.hip", "vh_api.hip", "vh_ensemble.hip"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
          "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I" + os.path.join(HERE, "..", "include")]

@@ -23,8 +23,8 @@ struct advance_args {
     double* pmax;          // [gridDim.y][nk] per-block per-slot max |tau_{n+1}|
     double* pblk;          // [gridDim.y * gridDim.x] per-block max
     const int* e3of;       // [nem] -> 3-D element index or -1
-    const double2* cdel;   // [i3*3 + shape][lambda][nlm]
-    int nlm;
+    const double2* cdel;   // [i3*3 + shape][lambda][nlm], nch channels
+    int nlm, nch;
     // trapezoidal rule: dN is the endpoint drift (eps_{n+1} with sigma), dN0
     // the start-of-step drift (eps_n with sigma_n)
     const double* sigma_n;
@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(256) k_advance(advance_args a) {
         const int em = pair / 3, sh = pair % 3;
         const size_t base = (size_t)em * 24 * nk + k;
         if (a.v.is3d[em]) {   // tau_{n+1} = tau_n + 3-D increment (increment form)
-            const double2* cd = a.cdel + ((size_t)a.e3of[em] * 3 + sh) * 4 * a.nlm + k + 1;
+            const double2* cd = a.cdel + ((size_t)a.e3of[em] * 3 + sh) * a.nch * a.nlm + k + 1;
 #pragma unroll
             for (int lam = 0; lam < 4; lam++) {
                 const double2 dv = cd[(size_t)lam * a.nlm];
@@ -167,15 +167,16 @@ __global__ void __launch_bounds__(256) k_advance_max(int nk, int ny, int nblk, c
     mnorm[k] = vh_nanmax(vh_nanmax(vh_nanmax(m[0], m[1]), vh_nanmax(m[2], m[3])), vh_nanmax(vh_nanmax(m[4], m[5]), vh_nanmax(m[6], m[7])));
 }
 
-size_t vh_advance_scratch(const vh_dims& d) {
-    const size_t gx = (d.nk + 63) / 64, gy = (3 * (size_t)d.nem + 3) / 4;
+size_t vh_advance_scratch(const vh_dims& d, bool tor) {
+    const size_t gx = (d.nk + 63) / 64, gy = (3 * (size_t)d.nem + 3) / 4 * (tor ? 2 : 1);
     return gy * d.nk + gx * gy;
 }
 
 void vh_launch_advance(hipStream_t st, const vh_dims& d, const vh_dev& v, double dt, const double* tau_in,
                        double* tau_out, const double* sigma, const double* dN, double* mnorm_out,
                        double* maxnorm_out, const int* e3of, const double2* cdel, double* scratch,
-                       double* maxn_copy, const double* sigma_n, const double* dN0) {
+                       double* maxn_copy, const double* sigma_n, const double* dN0, int nch,
+                       const vh_adv_tor& tor) {
     if (d.nem == 0) {
         vh_fill(st, mnorm_out, 0.0, d.nk);
         vh_fill(st, maxnorm_out, 0.0, 1);
@@ -183,33 +184,48 @@ void vh_launch_advance(hipStream_t st, const vh_dims& d, const vh_dev& v, double
         return;
     }
     advance_args a;
-    a.e3of = e3of; a.cdel = cdel; a.nlm = d.nlm;
+    a.e3of = e3of; a.cdel = cdel; a.nlm = d.nlm; a.nch = nch;
     a.nr = d.nr; a.nk = d.nk; a.nem = d.nem; a.dt = dt; a.v = v;
     a.tau_in = tau_in; a.tau_out = tau_out; a.sigma = sigma; a.dN = dN;
     a.sigma_n = sigma_n; a.dN0 = dN0;
     dim3 g((d.nk + 63) / 64, (3 * d.nem + 3) / 4);
+    const bool with_tor = tor.taut_in != nullptr;
+    const unsigned ny = g.y * (with_tor ? 2 : 1);   // reduction rows: spheroidal, then toroidal
     a.pmax = scratch;
-    a.pblk = scratch + (size_t)g.y * d.nk;
+    a.pblk = scratch + (size_t)ny * d.nk;
     if (sigma_n) k_advance<true><<<g, 256, 0, st>>>(a);
     else k_advance<false><<<g, 256, 0, st>>>(a);
     HIPCHECK(hipGetLastError());
+    if (with_tor) {
+        tor_adv_args t;
+        t.nk = d.nk; t.nem = d.nem; t.nlm = d.nlm; t.yoff = (int)g.y; t.dt = dt;
+        t.kdeg = v.kdeg; t.emap = v.emap; t.mu = v.mu; t.MkPerDt = v.MkPerDt; t.is3d = v.is3d;
+        t.e3of = e3of; t.enode = vh_tor_plan_enode(tor.plan); t.cdel = cdel;
+        t.tau_in = tor.taut_in; t.tau_out = tor.taut_out; t.dW = tor.dW; t.dW0 = tor.dW0;
+        t.pmax = a.pmax; t.pblk = a.pblk;
+        vh_launch_advance_tor(st, t, sigma_n != nullptr);
+    }
     // blocks 0 .. g.x-1: 64 slots each (threads 64..255 of them idle); the last: the global maximum
-    k_advance_max<<<g.x + 1, 256, 0, st>>>(d.nk, (int)g.y, (int)(g.x * g.y), a.pmax, a.pblk, mnorm_out, maxnorm_out,
+    k_advance_max<<<g.x + 1, 256, 0, st>>>(d.nk, (int)ny, (int)(g.x * ny), a.pmax, a.pblk, mnorm_out, maxnorm_out,
                                            maxn_copy);
     HIPCHECK(hipGetLastError());
 }
 
 // Per-slot max |tau| and the global max of a memory state that did not come
 // out of an advance (a restart or a host-set memory).
-__global__ void __launch_bounds__(256) k_memnorm(int nk, int nem, const double* __restrict__ tau, double* mnorm,
+__global__ void __launch_bounds__(256) k_memnorm(int nk, int nem, const double* __restrict__ tau,
+                                                 const double* __restrict__ taut, double* mnorm,
                                                  unsigned long long* maxbits) {
     __shared__ double red[4][64];
     const int lane = threadIdx.x & 63, eg = threadIdx.x >> 6;
     const int k = blockIdx.x * 64 + lane;
     double amax = 0.0;
     if (k < nk)
-        for (int em = eg; em < nem; em += 4)
+        for (int em = eg; em < nem; em += 4) {
             for (int c = 0; c < 24; c++) amax = vh_nanmax(amax, fabs(tau[((size_t)em * 24 + c) * nk + k]));
+            if (taut)
+                for (int c = 0; c < 12; c++) amax = vh_nanmax(amax, fabs(taut[((size_t)em * 12 + c) * nk + k]));
+        }
     red[eg][lane] = amax;
     __syncthreads();
     if (eg == 0) {
@@ -225,9 +241,10 @@ __global__ void __launch_bounds__(256) k_memnorm(int nk, int nem, const double* 
     }
 }
 
-void vh_launch_memnorm(hipStream_t st, const vh_dims& d, const double* tau, double* mnorm, double* maxnorm) {
+void vh_launch_memnorm(hipStream_t st, const vh_dims& d, const double* tau, double* mnorm, double* maxnorm,
+                       const double* taut) {
     vh_fill(st, maxnorm, 0.0, 1);
-    k_memnorm<<<(d.nk + 63) / 64, 256, 0, st>>>(d.nk, d.nem, tau, mnorm, (unsigned long long*)maxnorm);
+    k_memnorm<<<(d.nk + 63) / 64, 256, 0, st>>>(d.nk, d.nem, tau, taut, mnorm, (unsigned long long*)maxnorm);
     HIPCHECK(hipGetLastError());
 }
 
@@ -269,8 +286,10 @@ __global__ void __launch_bounds__(256) k_coarse_fine(size_t n, const double* __r
 
 // drift_change (src/vilma_response.f90:1499-1512) of the report drift dUa
 // (the first nk complex values of dsurf): out[0] = max |dUa - prev|, out[1]
-// = max |dUa|; then prev <- dUa (keep_drift)
+// = max |dUa|; then prev <- dUa (keep_drift). With the toroidal channels the
+// surface W drift dWa joins both maxima (and prev_t <- dWa).
 __global__ void __launch_bounds__(256) k_drift_change(int nk, const double2* __restrict__ dsurf, double2* prev,
+                                                      const double2* __restrict__ dwa, double2* prev_t,
                                                       unsigned long long* out) {
     __shared__ double red[8];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -280,6 +299,12 @@ __global__ void __launch_bounds__(256) k_drift_change(int nk, const double2* __r
         c = hypot(u.x - p.x, u.y - p.y);
         s = hypot(u.x, u.y);
         prev[k] = u;
+        if (dwa) {
+            const double2 w = dwa[k], q = prev_t[k];
+            c = vh_nanmax(c, hypot(w.x - q.x, w.y - q.y));
+            s = vh_nanmax(s, hypot(w.x, w.y));
+            prev_t[k] = w;
+        }
     }
     c = block_max(c, red);
     __syncthreads();
@@ -287,16 +312,19 @@ __global__ void __launch_bounds__(256) k_drift_change(int nk, const double2* __r
     if (threadIdx.x == 0) { max_bits(out, c); max_bits(out + 1, s); }
 }
 
-void vh_coarse_fine(hipStream_t st, size_t n, const double* fine, const double* coarse, double* out2) {
-    vh_fill(st, out2, 0.0, 2);
+void vh_coarse_fine(hipStream_t st, size_t n, const double* fine, const double* coarse, double* out2,
+                    bool accumulate) {
+    if (!accumulate) vh_fill(st, out2, 0.0, 2);
     const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 2048);
     k_coarse_fine<<<std::max(nb, 1u), 256, 0, st>>>(n, fine, coarse, (unsigned long long*)out2);
     HIPCHECK(hipGetLastError());
 }
 
-void vh_drift_change(hipStream_t st, int nk, const double* dsurf, double* prev, double* out2) {
+void vh_drift_change(hipStream_t st, int nk, const double* dsurf, double* prev, double* out2,
+                     const double* dWa, double* prev_t) {
     vh_fill(st, out2, 0.0, 2);
     k_drift_change<<<(nk + 255) / 256, 256, 0, st>>>(nk, (const double2*)dsurf, (double2*)prev,
+                                                     (const double2*)dWa, (double2*)prev_t,
                                                      (unsigned long long*)out2);
     HIPCHECK(hipGetLastError());
 }

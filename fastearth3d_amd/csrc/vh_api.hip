@@ -267,6 +267,17 @@ struct vh_model {
     }
     bool lat_visc = false;
     int ne3d = 0;
+    // toroidal degree of freedom (l_toroidal): requested at create, carried
+    // from the first genuinely 3-D element on (six memory channels; the
+    // tensor batches then have nch = 6 channels). Toroidal memory taut
+    // [em][12][k] (ping-pong like tau), nodal W drift dWt [nmn][2][k] and
+    // surface dWa [nk] complex; the trapezoidal scheme adds the endpoint
+    // drift, the previous surface drift, the saved and the coarse memory.
+    bool tor_req = false, tor_on = false;
+    int nch = 4;
+    vh_tor_plan* tplan = nullptr;
+    double *d_taut[2] = {0, 0}, *d_dWt = 0, *d_dWa = 0, *d_n4 = 0;
+    double *d_edWt = 0, *d_dwa_prev = 0, *d_taut_s = 0, *d_taut_c = 0;
     // the lateral field of the last enable through the API, for the spin-up's
     // 1-D phase: 1 = per-element perturbation (lat_field, ne x grid; lat_mk the
     // 1-D rates it was applied to), 2 = nodal absolute log10 viscosity
@@ -296,6 +307,10 @@ struct vh_model {
 vh_model::~vh_model() {
     if (adj) adj_free(adj);
     if (dplan) vh_drift_plan_destroy(dplan);
+    if (tplan) vh_tor_plan_destroy(tplan);
+    for (void* p : {(void*)d_taut[0], (void*)d_taut[1], (void*)d_dWt, (void*)d_dWa, (void*)d_n4, (void*)d_edWt,
+                    (void*)d_dwa_prev, (void*)d_taut_s, (void*)d_taut_c})
+        if (p) (void)hipFree(p);
     void* ptrs[] = {d_r, d_mu, d_rho, d_Rk, d_MkPerDt, d_emap, d_mnodes, d_mnode, d_kdeg, d_is3d, d_ab, d_dr, d_dc,
                     d_ab1, d_null1, d_gu, d_gn, d_gv, d_xUn, d_xVn, d_ipiv, d_ipiv1, d_fail, d_tau[0], d_tau[1],
                     d_mnorm[0], d_mnorm[1], d_maxn, d_advmax, d_dN, d_dsurf, d_plm_lj, d_dplm_lj, d_plm_jl, d_dplm_jl, d_w,
@@ -319,7 +334,6 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
     h->par = *p;
     if (strcmp(p->scheme, "fe") && strcmp(p->scheme, "trap"))
         throw vh_error(std::string("scheme \"") + p->scheme + "\" not supported (fe|trap)");
-    if (p->l_toroidal) throw vh_error("l_toroidal = .true. is not supported");
     if ((p->mmax >= 0 && p->mmax != p->lmax) || p->mres != 1) throw vh_error("mmax/mres other than lmax/1 not supported");
     if (p->lmax < 2) throw vh_error("lmax must be >= 2");
     int nl = p->nlat > 0 ? p->nlat : 2 * p->lmax + 2;
@@ -335,6 +349,7 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
     h->deg1_cm = !strcmp(p->deg1_frame, "cm");
     h->skip_tol = p->skip_tol;
     h->visc3d_tol = p->visc3d_tol;
+    h->tor_req = p->l_toroidal != 0;   // src/vilma_coupling.f90:282
     HIPCHECK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
     hipStream_t st = h->st;
     HIPCHECK(hipHostMalloc((void**)&h->h_pin, (2 * (size_t)std::max(nl, 32) + 64) * sizeof(double),
@@ -632,16 +647,34 @@ void setup(vh_model* h, const vh_params* p, int nphi, int nlat, const double* z_
     h->sync();
 }
 
+// The toroidal channels of one advance (none unless they are carried): memory
+// cur -> nxt, nodal W drift dW (and, trapezoidal rule, the start drift dW0)
+vh_adv_tor adv_tor(const vh_model* h, int cur, int nxt, const double* dW, const double* dW0) {
+    vh_adv_tor t;
+    if (!h->tor_on) return t;
+    t.plan = h->tplan; t.taut_in = h->d_taut[cur]; t.taut_out = h->d_taut[nxt]; t.dW = dW; t.dW0 = dW0;
+    return t;
+}
+
+// response_begin_step's toroidal branch (src/vilma_response.f90:694-809): the
+// W drift of the memory in buffer b into dW (nodal) and d_dWa (surface)
+void tor_drift(vh_model* h, int b, double* dW) {
+    if (!h->tor_on) return;
+    vh_tor_drift(h->st, h->tplan, h->d, h->v, h->d_taut[b], h->d_mnorm[b], h->d_maxn + b, h->skip_tol, dW, h->d_dWa);
+}
+
 // vilma_advance (src/vilma_response.f90:944-992): the 3-D elements first
 // (advance_memory_3d, :1171-1225, increments into h->d_cdel), then the one
 // streaming pass that applies them and advances every 1-D element.
 void advance_memory(vh_model* h, const double* sigma, int nxt, double* maxn_copy = nullptr) {
     hipStream_t st = h->st;
     const vh_dims& d = h->d;
+    const int nch = h->nch;
     if (h->lat_visc && h->ne3d > 0) {
         const size_t ng = (size_t)d.nphi * d.nlat;
         h->prof_begin(VH_K_T3_GATHER);
-        vh_t3_gather(st, d, h->v, h->d_e3em, h->ne3d, h->d_tau[h->cur], sigma, h->d_dN, h->d_X);
+        vh_t3_gather(st, d, h->v, h->d_e3em, h->ne3d, h->d_tau[h->cur], sigma, h->d_dN, h->d_X, nch,
+                     h->d_taut[h->cur], h->d_dWt);
         h->prof_end();
         for (int b0 = 0; b0 < h->nbat; b0 += h->tchunk) {
             const int nb = std::min(h->tchunk, h->nbat - b0);
@@ -652,19 +685,20 @@ void advance_memory(vh_model* h, const double* sigma, int nxt, double* maxn_copy
             if (h->adj && adj_recording(h->adj))
                 syn = adj_fd_pending(h) + (size_t)b0 * 6 * (d.L + 1) * d.nlat;
             h->prof_begin(VH_K_TSH_SYNTH);
-            vh_tsh_synth(st, h->s, h->d_X + (size_t)b0 * 4 * d.nlm, nb, syn);
+            vh_tsh_synth(st, h->s, h->d_X + (size_t)b0 * nch * d.nlm, nb, syn, nch);
             h->prof_end();
             h->prof_begin(VH_K_T3_ROWS);
             vh_t3_rows(st, h->s, h->d_Fd, nb, 0, h->d_rate3 + (size_t)(b0 / 3) * ng, 3, h->dt, nullptr, syn);
             h->prof_end();
             h->prof_begin(VH_K_TSH_ANALYSIS);
-            vh_tsh_analysis(st, h->s, h->d_Fd, nb, h->d_n6, h->d_cdel + (size_t)b0 * 4 * d.nlm);
+            vh_tsh_analysis(st, h->s, h->d_Fd, nb, h->d_n6, h->d_cdel + (size_t)b0 * nch * d.nlm, nch, h->d_n4);
             h->prof_end();
         }
     }
     h->prof_begin(VH_K_ADVANCE);
     vh_launch_advance(st, d, h->v, h->dt, h->d_tau[h->cur], h->d_tau[nxt], sigma, h->d_dN, h->d_mnorm[nxt],
-                      h->d_maxn + nxt, h->d_e3of, h->d_cdel, h->d_advmax, maxn_copy);
+                      h->d_maxn + nxt, h->d_e3of, h->d_cdel, h->d_advmax, maxn_copy, nullptr, nullptr, nch,
+                      adv_tor(h, h->cur, nxt, h->d_dWt, nullptr));
     h->prof_end();
 }
 
@@ -689,6 +723,7 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
         h->prof_begin(VH_K_DRIFT);
         vh_drift_run(st, h->dplan, h->d, h->v, h->d_tau[h->cur], h->d_mnorm[h->cur], h->d_maxn + h->cur, h->skip_tol,
                      h->deg1_cm, h->d_dN, h->d_dsurf);
+        tor_drift(h, h->cur, h->d_dWt);
         h->prof_end();
     }
     // the coastline passes and inner fixed point in one launch
@@ -735,7 +770,21 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
 // tau_n stays in d_tau[cur] for the whole solve; every pass writes its
 // tau_{n+1} estimate into d_tau[cur ^ 1] and the endpoint drift of that
 // estimate into d_edN / d_dsurf; the solve commits the last estimate.
+// the toroidal part of the trapezoidal buffers (ensure_commit_scratch_tor,
+// src/vilma_response.f90:1111-1118, and the widened save / coarse buffers)
+void trap_alloc_tor(vh_model* h) {
+    if (!h->tor_on || h->d_taut_s) return;
+    const vh_dims& d = h->d;
+    const size_t nt = (size_t)12 * std::max(d.nem, 1) * d.nk, ndw = (size_t)2 * std::max(d.nmn, 1) * d.nk;
+    h->d_taut_s = dalloc<double>(nt); h->d_taut_c = dalloc<double>(nt);
+    h->d_edWt = dalloc<double>(ndw);
+    h->d_dwa_prev = dalloc<double>((size_t)2 * d.nk);
+    vh_fill(h->st, h->d_taut_s, 0.0, nt); vh_fill(h->st, h->d_taut_c, 0.0, nt);
+    vh_fill(h->st, h->d_edWt, 0.0, ndw); vh_fill(h->st, h->d_dwa_prev, 0.0, (size_t)2 * d.nk);
+}
+
 void trap_alloc(vh_model* h) {
+    trap_alloc_tor(h);
     if (h->d_tau_s) return;
     const vh_dims& d = h->d;
     hipStream_t st = h->st;
@@ -756,29 +805,31 @@ void d2d(double* dst, const double* src, size_t n, hipStream_t st);
 void trap_advance(vh_model* h, const double* sigma_n) {
     hipStream_t st = h->st;
     const vh_dims& d = h->d;
-    const int cur = h->cur, nxt = cur ^ 1;
+    const int cur = h->cur, nxt = cur ^ 1, nch = h->nch;
     const double* sig1 = (const double*)h->d_sigma;
     if (h->lat_visc && h->ne3d > 0) {
         const size_t ng = (size_t)d.nphi * d.nlat;
         h->prof_begin(VH_K_T3_GATHER);
-        vh_t3_gather_trap(st, d, h->v, h->d_e3em, h->ne3d, h->d_tau[cur], sigma_n, h->d_dN, sig1, h->d_edN, h->d_X);
+        vh_t3_gather_trap(st, d, h->v, h->d_e3em, h->ne3d, h->d_tau[cur], sigma_n, h->d_dN, sig1, h->d_edN, h->d_X,
+                          nch, h->d_taut[cur], h->d_dWt, h->d_edWt);
         h->prof_end();
         for (int b0 = 0; b0 < h->nbat; b0 += h->tchunk) {
             const int nb = std::min(h->tchunk, h->nbat - b0);
             h->prof_begin(VH_K_TSH_SYNTH);
-            vh_tsh_synth(st, h->s, h->d_X + (size_t)b0 * 4 * d.nlm, nb, h->d_Fd);
+            vh_tsh_synth(st, h->s, h->d_X + (size_t)b0 * nch * d.nlm, nb, h->d_Fd, nch);
             h->prof_end();
             h->prof_begin(VH_K_T3_ROWS);
             vh_t3_rows(st, h->s, h->d_Fd, nb, 3, h->d_rate3 + (size_t)(b0 / 3) * ng, 3, h->dt, nullptr);
             h->prof_end();
             h->prof_begin(VH_K_TSH_ANALYSIS);
-            vh_tsh_analysis(st, h->s, h->d_Fd, nb, h->d_n6, h->d_cdel + (size_t)b0 * 4 * d.nlm);
+            vh_tsh_analysis(st, h->s, h->d_Fd, nb, h->d_n6, h->d_cdel + (size_t)b0 * nch * d.nlm, nch, h->d_n4);
             h->prof_end();
         }
     }
     h->prof_begin(VH_K_ADVANCE);
     vh_launch_advance(st, d, h->v, h->dt, h->d_tau[cur], h->d_tau[nxt], sig1, h->d_edN, h->d_mnorm[nxt],
-                      h->d_maxn + nxt, h->d_e3of, h->d_cdel, h->d_advmax, nullptr, sigma_n, h->d_dN);
+                      h->d_maxn + nxt, h->d_e3of, h->d_cdel, h->d_advmax, nullptr, sigma_n, h->d_dN, nch,
+                      adv_tor(h, cur, nxt, h->d_edWt, h->d_dWt));
     h->prof_end();
 }
 
@@ -802,9 +853,14 @@ sle_res sle_solve_trap(vh_model* h, const double* ice0, const double* ice1, doub
     h->prof_begin(VH_K_DRIFT);
     vh_drift_run(st, h->dplan, d, h->v, h->d_tau[cur], h->d_mnorm[cur], h->d_maxn + cur, h->skip_tol, h->deg1_cm,
                  h->d_dN, h->d_dsurf);
+    tor_drift(h, cur, h->d_dWt);
     h->prof_end();
     d2d(h->d_edN, h->d_dN, (size_t)4 * std::max(d.nmn, 1) * d.nk, st);
     d2d(h->d_dua_prev, h->d_dsurf, (size_t)2 * d.nk, st);
+    if (h->tor_on) {
+        d2d(h->d_edWt, h->d_dWt, (size_t)2 * std::max(d.nmn, 1) * d.nk, st);
+        d2d(h->d_dwa_prev, h->d_dWa, (size_t)2 * d.nk, st);
+    }
     // sigma_n: the committed start-of-step load, or (not yet primed) this
     // pass's own load (src/vilma_response.f90:1553-1558)
     const double* sigma_n = h->sig_primed ? h->d_sig_n : (const double*)h->d_sigma;
@@ -832,8 +888,10 @@ sle_res sle_solve_trap(vh_model* h, const double* ice0, const double* ice1, doub
         h->prof_begin(VH_K_DRIFT);
         vh_drift_run(st, h->dplan, d, h->v, h->d_tau[nxt], h->d_mnorm[nxt], h->d_maxn + nxt, h->skip_tol,
                      h->deg1_cm, h->d_edN, h->d_dsurf);
+        tor_drift(h, nxt, h->d_edWt);
         h->prof_end();
-        vh_drift_change(st, d.nk, h->d_dsurf, h->d_dua_prev, r + 12);
+        vh_drift_change(st, d.nk, h->d_dsurf, h->d_dua_prev, r + 12, h->tor_on ? h->d_dWa : nullptr,
+                        h->tor_on ? h->d_dwa_prev : nullptr);
         h->read(r, 14);
         if (hr[2] != 0.0) throw vh_error("sle_solve: a wait of the fused sea-level solve exceeded its bound");
         h->n_sle_steal += (long)hr[8];
@@ -864,6 +922,7 @@ void trap_save(vh_model* h) {
     const vh_dims& d = h->d;
     const size_t nm = (size_t)24 * std::max(d.nem, 1) * d.nk;
     d2d(h->d_tau_s, h->d_tau[h->cur], nm, h->st);
+    if (h->tor_on) d2d(h->d_taut_s, h->d_taut[h->cur], nm / 2, h->st);
     d2d(h->d_mnorm_s, h->d_mnorm[h->cur], d.nk, h->st);
     d2d(h->d_mnorm_s + d.nk, h->d_maxn + h->cur, 1, h->st);
     d2d(h->d_sig_n_s, h->d_sig_n, (size_t)2 * d.nlm, h->st);
@@ -874,6 +933,7 @@ void trap_restore(vh_model* h) {
     const vh_dims& d = h->d;
     const size_t nm = (size_t)24 * std::max(d.nem, 1) * d.nk;
     d2d(h->d_tau[h->cur], h->d_tau_s, nm, h->st);
+    if (h->tor_on) d2d(h->d_taut[h->cur], h->d_taut_s, nm / 2, h->st);
     d2d(h->d_mnorm[h->cur], h->d_mnorm_s, d.nk, h->st);
     d2d(h->d_maxn + h->cur, h->d_mnorm_s + d.nk, 1, h->st);
     d2d(h->d_sig_n, h->d_sig_n_s, (size_t)2 * d.nlm, h->st);
@@ -912,12 +972,14 @@ void stepper_trap(vh_model* h, double t0, double t1, const double* srot) {
         h->dt = dt;
         solve_at(t + dt);                                        // coarse: one dt
         d2d(h->d_tau_c, h->d_tau[h->cur], nm, st);               // buffer B
+        if (h->tor_on) d2d(h->d_taut_c, h->d_taut[h->cur], nm / 2, st);
         trap_restore(h);
         HIPCHECK(hipMemcpyAsync(h->d_rsl, h->d_rsl_n, n * sizeof(double), hipMemcpyDeviceToDevice, st));
         h->dt = 0.5 * dt;
         solve_at(t + 0.5 * dt);                                  // fine sub-step 1
         solve_at(t + dt);                                        // fine sub-step 2
         vh_coarse_fine(st, nm, h->d_tau[h->cur], h->d_tau_c, h->d_sc + 30);
+        if (h->tor_on) vh_coarse_fine(st, nm / 2, h->d_taut[h->cur], h->d_taut_c, h->d_sc + 30, true);
         h->read(h->d_sc + 30, 2);
         const double err_inf = h->h_pin[0], tau_inf = h->h_pin[1];
         h->dt = dt;
@@ -1207,6 +1269,7 @@ void forward_record(vh_model* h, int ncalls, const double* ice_hist, bool dev, d
                     double* rsl_out, bool rsl_dev) {
     adj_check(h);
     if (h->trap) throw vh_error("vh_forward_record: the reverse sweep differentiates scheme fe only");
+    if (h->tor_on) throw vh_error("vh_forward_record: the reverse sweep does not carry the toroidal channels");
     if (ncalls <= 0) throw vh_error("vh_forward_record: ncalls must be positive");
     if (!(dt_yr > 0.0)) throw vh_error("vh_forward_record: dt_yr must be positive");
     if (!h->adj) h->adj = new adj_ctx();
@@ -1689,6 +1752,16 @@ int vh_get_spectrum(vh_model* h, int which, double* out) {
             to_lm(h, tmp.data(), out);
             return;
         }
+        if (which == VH_SP_TOR) {   // response_horizontal_toroidal (src/vilma_response.f90:884-900)
+            std::fill(tmp.begin(), tmp.end(), make_double2(0.0, 0.0));
+            if (h->tor_on) {
+                HIPCHECK(hipMemcpyAsync(tmp.data() + 1, h->d_dWa, (size_t)h->d.nk * sizeof(double2),
+                                        hipMemcpyDeviceToHost, h->st));
+                h->sync();
+            }
+            to_lm(h, tmp.data(), out);
+            return;
+        }
         if (which != VH_SP_U && which != VH_SP_N) throw vh_error("vh_get_spectrum: unknown selector");
         // response_apply at the current load and frozen drift
         std::vector<double2> uN(2 * (size_t)h->d.nlm);
@@ -1710,7 +1783,7 @@ int vh_get_dims(vh_model* h, int which, int* out) {
             case VH_D_NE: *out = h->d.ne; break;
             case VH_D_NK: *out = h->d.nk; break;
             case VH_D_NE3D: *out = h->ne3d; break;
-            case VH_D_NLAM: *out = 4; break;
+            case VH_D_NLAM: *out = h->tor_on ? 6 : 4; break;
             case VH_D_NEM: *out = h->d.nem; break;
             case VH_D_NMN: *out = h->d.nmn; break;
             default: throw vh_error("vh_get_dims: unknown selector");
@@ -1718,26 +1791,37 @@ int vh_get_dims(vh_model* h, int which, int* out) {
     });
 }
 
-// reference restart layout: 6 arrays (nlam, ne, nk) column-major
+// reference restart layout: 6 arrays (nlam, ne, nk) column-major, nlam = 4,
+// or 6 once the toroidal channels are carried (lambda 3, 4 at 5, 6)
 int vh_get_memory(vh_model* h, double* tau) {
     return guarded(h, [&] {
         if (h->kind != 2) throw vh_error("vh_get_memory: no viscoelastic memory");
         const vh_dims& d = h->d;
-        std::vector<double> dev((size_t)24 * d.nem * d.nk);
+        const int nl = h->tor_on ? 6 : 4;
+        std::vector<double> dev((size_t)24 * d.nem * d.nk), devt(h->tor_on ? (size_t)12 * d.nem * d.nk : 0);
         HIPCHECK(hipMemcpyAsync(dev.data(), h->d_tau[h->cur], dev.size() * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        if (h->tor_on)
+            HIPCHECK(hipMemcpyAsync(devt.data(), h->d_taut[h->cur], devt.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                    h->st));
         h->sync();
-        size_t n1 = (size_t)4 * d.ne * d.nk;
+        size_t n1 = (size_t)nl * d.ne * d.nk;
         memset(tau, 0, 6 * n1 * sizeof(double));
         // array order Are, Aim, Bre, Bim, Cre, Cim = (shape, reim)
         for (int em = 0; em < d.nem; em++) {
             int e = h->emap[em];
             for (int sh = 0; sh < 3; sh++)
-                for (int ri = 0; ri < 2; ri++)
+                for (int ri = 0; ri < 2; ri++) {
+                    double* dst = tau + (size_t)(sh * 2 + ri) * n1;
                     for (int lam = 0; lam < 4; lam++) {
                         const double* src = dev.data() + ((size_t)em * 24 + (sh * 2 + ri) * 4 + lam) * d.nk;
-                        double* dst = tau + (size_t)(sh * 2 + ri) * n1;
-                        for (int k = 0; k < d.nk; k++) dst[((size_t)k * d.ne + e) * 4 + lam] = src[k];
+                        for (int k = 0; k < d.nk; k++) dst[((size_t)k * d.ne + e) * nl + lam] = src[k];
                     }
+                    if (h->tor_on)
+                        for (int q = 0; q < 2; q++) {
+                            const double* src = devt.data() + ((size_t)em * 12 + (sh * 2 + ri) * 2 + q) * d.nk;
+                            for (int k = 0; k < d.nk; k++) dst[((size_t)k * d.ne + e) * nl + 4 + q] = src[k];
+                        }
+                }
         }
     });
 }
@@ -1746,17 +1830,24 @@ int vh_set_memory(vh_model* h, const double* tau) {
     return guarded(h, [&] {
         if (h->kind != 2) throw vh_error("vh_set_memory: no viscoelastic memory");
         const vh_dims& d = h->d;
-        std::vector<double> dev((size_t)24 * d.nem * d.nk);
-        size_t n1 = (size_t)4 * d.ne * d.nk;
+        const int nl = h->tor_on ? 6 : 4;
+        std::vector<double> dev((size_t)24 * d.nem * d.nk), devt(h->tor_on ? (size_t)12 * d.nem * d.nk : 0);
+        size_t n1 = (size_t)nl * d.ne * d.nk;
         for (int em = 0; em < d.nem; em++) {
             int e = h->emap[em];
             for (int sh = 0; sh < 3; sh++)
-                for (int ri = 0; ri < 2; ri++)
+                for (int ri = 0; ri < 2; ri++) {
+                    const double* src = tau + (size_t)(sh * 2 + ri) * n1;
                     for (int lam = 0; lam < 4; lam++) {
                         double* dst = dev.data() + ((size_t)em * 24 + (sh * 2 + ri) * 4 + lam) * d.nk;
-                        const double* src = tau + (size_t)(sh * 2 + ri) * n1;
-                        for (int k = 0; k < d.nk; k++) dst[k] = src[((size_t)k * d.ne + e) * 4 + lam];
+                        for (int k = 0; k < d.nk; k++) dst[k] = src[((size_t)k * d.ne + e) * nl + lam];
                     }
+                    if (h->tor_on)
+                        for (int q = 0; q < 2; q++) {
+                            double* dst = devt.data() + ((size_t)em * 12 + (sh * 2 + ri) * 2 + q) * d.nk;
+                            for (int k = 0; k < d.nk; k++) dst[k] = src[((size_t)k * d.ne + e) * nl + 4 + q];
+                        }
+                }
         }
         // elastic and fluid elements have a zero Maxwell rate, so the update
         // leaves their memory at its start value, zero (src/vilma_viscoelastic.f90:
@@ -1768,13 +1859,16 @@ int vh_set_memory(vh_model* h, const double* tau) {
             for (int k = 0; k < d.nk; k++)
                 for (int e = 0; e < d.ne; e++) {
                     if (maxwell[e]) continue;
-                    const double* v = tau + a6 * n1 + ((size_t)k * d.ne + e) * 4;
-                    if (v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0 || v[3] != 0.0)
-                        throw vh_error("vh_set_memory: memory of an elastic or fluid element must be zero (element " +
-                                       std::to_string(e + 1) + ")");
+                    const double* v = tau + a6 * n1 + ((size_t)k * d.ne + e) * nl;
+                    for (int lam = 0; lam < nl; lam++)
+                        if (v[lam] != 0.0)
+                            throw vh_error("vh_set_memory: memory of an elastic or fluid element must be zero (element " +
+                                           std::to_string(e + 1) + ")");
                 }
         upload(h->d_tau[h->cur], dev.data(), dev.size(), h->st);
-        vh_launch_memnorm(h->st, d, h->d_tau[h->cur], h->d_mnorm[h->cur], h->d_maxn + h->cur);
+        if (h->tor_on) upload(h->d_taut[h->cur], devt.data(), devt.size(), h->st);
+        vh_launch_memnorm(h->st, d, h->d_tau[h->cur], h->d_mnorm[h->cur], h->d_maxn + h->cur,
+                          h->tor_on ? h->d_taut[h->cur] : nullptr);
         h->read(h->d_maxn + h->cur, 1);
         h->maxn_cur = h->h_pin[0];
     });
@@ -1795,7 +1889,8 @@ int vh_set_memory_lowres(vh_model* h, const double* tau, int nk_src) {
         for (int L = 1; L <= d.L; L++)
             if (L * (L + 3) / 2 == nk_src) ok = true;   // slots of degrees 1..L
         if (!ok) throw vh_error("vilma_restart_read: cross-resolution restart incompatible (mmax/mres mismatch)");
-        const size_t n_src = (size_t)4 * d.ne * nk_src, n_dst = (size_t)4 * d.ne * d.nk;
+        const int nl = h->tor_on ? 6 : 4;   // the source carries the model's channels
+        const size_t n_src = (size_t)nl * d.ne * nk_src, n_dst = (size_t)nl * d.ne * d.nk;
         std::vector<double> full(6 * n_dst, 0.0);
         for (int a6 = 0; a6 < 6; a6++)   // (nlam, ne, nk) column-major: k slowest, so a prefix
             std::copy(tau + a6 * n_src, tau + (a6 + 1) * n_src, full.begin() + a6 * n_dst);
@@ -1991,8 +2086,34 @@ void tensor_init(vh_model* h) {
     h->tsh_ready = true;
 }
 
+// enable_toroidal (src/vilma_response.f90:1055-1109): the W operators and
+// forcing tables, the toroidal memory (zero: the exact toroidal state of a run
+// that has only been spheroidal), the W drift arrays, the Z4 normalisation,
+// and the trapezoidal buffers if that scheme has allocated its own. The
+// advance's reductions gain the toroidal rows.
+void enable_toroidal(vh_model* h) {
+    const vh_dims& d = h->d;
+    hipStream_t st = h->st;
+    h->sync();   // the reduction scratch is replaced below
+    h->tplan = vh_tor_plan_create(st, d, h->r, h->mu, h->emap, h->mnodes, h->mnode_of);
+    const size_t nt = (size_t)12 * std::max(d.nem, 1) * d.nk, ndw = (size_t)2 * std::max(d.nmn, 1) * d.nk;
+    for (int b = 0; b < 2; b++) { h->d_taut[b] = dalloc<double>(nt); vh_fill(st, h->d_taut[b], 0.0, nt); }
+    h->d_dWt = dalloc<double>(ndw); vh_fill(st, h->d_dWt, 0.0, ndw);
+    h->d_dWa = dalloc<double>((size_t)2 * d.nk); vh_fill(st, h->d_dWa, 0.0, (size_t)2 * d.nk);
+    h->d_n4 = dalloc<double>(d.L + 1);
+    vh_tsh_calibrate_n4(st, h->s, h->d_n4);
+    HIPCHECK(hipFree(h->d_advmax));
+    h->d_advmax = nullptr;
+    h->d_advmax = dalloc<double>(vh_advance_scratch(d, true));
+    h->tor_on = true;
+    h->nch = 6;
+    if (h->d_tau_s) trap_alloc_tor(h);
+    h->sync();
+}
+
 // response_enable_lateral_visc (src/vilma_response.f90:994-1053); pert is
-// [e][nlat][nphi] (phi fastest), toroidal channels not carried.
+// [e][nlat][nphi] (phi fastest). With l_toroidal, the first genuinely 3-D
+// element turns the toroidal channels on for good (:1048-1052).
 void enable_lateral(vh_model* h, const double* pert) {
     if (h->kind != 2) throw vh_error("solid_earth_enable_visc_3d: lateral viscosity needs earth_response=ve");
     const vh_dims& d = h->d;
@@ -2040,8 +2161,9 @@ void enable_lateral(vh_model* h, const double* pert) {
     upload(h->d_is3d, is3d.data(), (size_t)d.nem, st);
     h->d_rate3 = dalloc<double>(ng * h->ne3d);
     for (int i = 0; i < h->ne3d; i++) upload(h->d_rate3 + (size_t)i * ng, rate3.data() + (size_t)e3[i] * ng, ng, st);
-    h->d_X = dalloc<double2>((size_t)h->nbat * 4 * d.nlm);
-    h->d_cdel = dalloc<double2>((size_t)h->nbat * 4 * d.nlm);
+    if (h->tor_req && h->ne3d > 0 && !h->tor_on) enable_toroidal(h);
+    h->d_X = dalloc<double2>((size_t)h->nbat * h->nch * d.nlm);
+    h->d_cdel = dalloc<double2>((size_t)h->nbat * h->nch * d.nlm);
     h->d_Fd = dalloc<double2>((size_t)std::max(h->tchunk, 1) * 6 * (d.L + 1) * d.nlat);
     upload(h->d_MkPerDt, h->MkPerDt.data(), (size_t)ne, st);
     h->rate_max = r3max;
@@ -2203,17 +2325,26 @@ int vh_spinup(vh_model* h, const double* h_ice_lgm, double equil_time_max_yr, do
 // tensor_sh_synth / tensor_sh_analysis (src/vilma_tensor_sh.f90:168-322):
 // c is [4][nlm] complex in SHTns LM order (lambda = 1, 2, 5, 6), dyad is
 // [6][nlat][nphi] (rr, rt, rp, tt, tp, pp).
-int vh_tensor_synthesis(vh_model* h, const double* c, double* dyad) {
+// n4 on first use by the six-channel transforms (the model calibrates it
+// itself when the toroidal channels are carried)
+static void tensor_init_n4(vh_model* h) {
+    if (h->d_n4) return;
+    h->d_n4 = dalloc<double>(h->d.L + 1);
+    vh_tsh_calibrate_n4(h->st, h->s, h->d_n4);
+}
+
+int vh_tensor_synthesis_ch(vh_model* h, int nch, const double* c, double* dyad) {
     return guarded(h, [&] {
+        if (nch != 4 && nch != 6) throw vh_error("tensor_sh_synth: nch must be 4 or 6");
         tensor_init(h);
         const vh_dims& d = h->d;
         const size_t ng = (size_t)d.nphi * d.nlat;
-        std::vector<double2> X((size_t)4 * d.nlm);
-        for (int lam = 0; lam < 4; lam++) from_lm(h, c + (size_t)2 * lam * d.nlm, X.data() + (size_t)lam * d.nlm);
+        std::vector<double2> X((size_t)nch * d.nlm);
+        for (int lam = 0; lam < nch; lam++) from_lm(h, c + (size_t)2 * lam * d.nlm, X.data() + (size_t)lam * d.nlm);
         double2 *dX = dalloc<double2>(X.size()), *dF = dalloc<double2>((size_t)6 * (d.L + 1) * d.nlat);
         double* dg = dalloc<double>(6 * ng);
         upload(dX, X.data(), X.size(), h->st);
-        vh_tsh_synth(h->st, h->s, dX, 1, dF);
+        vh_tsh_synth(h->st, h->s, dX, 1, dF, nch);
         vh_t3_rows(h->st, h->s, dF, 1, 1, nullptr, 1, 0.0, dg);
         HIPCHECK(hipMemcpyAsync(dyad, dg, 6 * ng * sizeof(double), hipMemcpyDeviceToHost, h->st));
         h->sync();
@@ -2221,21 +2352,57 @@ int vh_tensor_synthesis(vh_model* h, const double* c, double* dyad) {
     });
 }
 
-int vh_tensor_analysis(vh_model* h, const double* dyad, double* c) {
+int vh_tensor_analysis_ch(vh_model* h, int nch, const double* dyad, double* c) {
     return guarded(h, [&] {
+        if (nch != 4 && nch != 6) throw vh_error("tensor_sh_analysis: nch must be 4 or 6");
         tensor_init(h);
+        if (nch == 6) tensor_init_n4(h);
         const vh_dims& d = h->d;
         const size_t ng = (size_t)d.nphi * d.nlat;
-        double2 *dC = dalloc<double2>((size_t)4 * d.nlm), *dF = dalloc<double2>((size_t)6 * (d.L + 1) * d.nlat);
+        double2 *dC = dalloc<double2>((size_t)nch * d.nlm), *dF = dalloc<double2>((size_t)6 * (d.L + 1) * d.nlat);
         double* dg = dalloc<double>(6 * ng);
         upload(dg, dyad, 6 * ng, h->st);
         vh_t3_rows(h->st, h->s, dF, 1, 2, nullptr, 1, 0.0, dg);
-        vh_tsh_analysis(h->st, h->s, dF, 1, h->d_n6, dC);
-        std::vector<double2> C((size_t)4 * d.nlm);
+        vh_tsh_analysis(h->st, h->s, dF, 1, h->d_n6, dC, nch, h->d_n4);
+        std::vector<double2> C((size_t)nch * d.nlm);
         HIPCHECK(hipMemcpyAsync(C.data(), dC, C.size() * sizeof(double2), hipMemcpyDeviceToHost, h->st));
         h->sync();
-        for (int lam = 0; lam < 4; lam++) to_lm(h, C.data() + (size_t)lam * d.nlm, c + (size_t)2 * lam * d.nlm);
+        for (int lam = 0; lam < nch; lam++) to_lm(h, C.data() + (size_t)lam * d.nlm, c + (size_t)2 * lam * d.nlm);
         (void)hipFree(dC); (void)hipFree(dF); (void)hipFree(dg);
+    });
+}
+
+int vh_tensor_synthesis(vh_model* h, const double* c, double* dyad) { return vh_tensor_synthesis_ch(h, 4, c, dyad); }
+int vh_tensor_analysis(vh_model* h, const double* dyad, double* c) { return vh_tensor_analysis_ch(h, 4, dyad, c); }
+
+// n6(l) and n4(l), l = 0..L (the spin-2 normalisations of the tensor analysis)
+int vh_tensor_norms(vh_model* h, double* n6, double* n4) {
+    return guarded(h, [&] {
+        tensor_init(h);
+        tensor_init_n4(h);
+        if (n6) HIPCHECK(hipMemcpyAsync(n6, h->d_n6, (h->d.L + 1) * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        if (n4) HIPCHECK(hipMemcpyAsync(n4, h->d_n4, (h->d.L + 1) * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        h->sync();
+    });
+}
+
+// The toroidal W drift of the last begin_step at every mesh node
+// (dWn_re / dWn_im of the reference, zero at nodes that carry no memory):
+// [2][nk][nr]; zero unless the toroidal channels are carried.
+int vh_get_drift_tor(vh_model* h, double* dwn) {
+    return guarded(h, [&] {
+        const vh_dims& d = h->d;
+        const size_t nn = (size_t)d.nr * d.nk;
+        memset(dwn, 0, 2 * nn * sizeof(double));
+        if (!h->tor_on) return;
+        std::vector<double> dw((size_t)2 * d.nmn * d.nk);
+        HIPCHECK(hipMemcpyAsync(dw.data(), h->d_dWt, dw.size() * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        h->sync();
+        for (int q = 0; q < d.nmn; q++) {
+            const int node = h->mnodes[q];
+            for (int ri = 0; ri < 2; ri++)
+                for (int k = 0; k < d.nk; k++) dwn[ri * nn + (size_t)k * d.nr + node] = dw[((size_t)q * 2 + ri) * d.nk + k];
+        }
     });
 }
 
@@ -2245,6 +2412,7 @@ int vh_resp_begin_step(vh_model* h) {
         if (h->kind != 2) return;
         vh_drift_run(h->st, h->dplan, h->d, h->v, h->d_tau[h->cur], h->d_mnorm[h->cur], h->d_maxn + h->cur,
                      h->skip_tol, h->deg1_cm, h->d_dN, h->d_dsurf);
+        tor_drift(h, h->cur, h->d_dWt);
         h->sync();
     });
 }

@@ -97,20 +97,40 @@ void vh_launch_setup_deg1_op(hipStream_t st, const vh_dims& d, const int* mnodes
 // 3-D elements (v.is3d) take tau_in + their increment from cdel
 // ([i3*3 + shape][lambda][nlm] complex, e3of[em] = i3) instead of the 1-D update.
 // scratch: vh_advance_scratch(d) doubles (per-block maxima)
+struct vh_tor_plan;
+// The toroidal channels of one advance (all null when they are not carried):
+// taut_in/out [em][12][k], the nodal W drift dW [nmn][2][k] (the endpoint
+// drift under the trapezoidal rule, dW0 then the start-of-step one).
+struct vh_adv_tor {
+    const vh_tor_plan* plan = nullptr;
+    const double* taut_in = nullptr;
+    double* taut_out = nullptr;
+    const double* dW = nullptr;
+    const double* dW0 = nullptr;
+};
 void vh_launch_advance(hipStream_t st, const vh_dims& d, const vh_dev& v, double dt, const double* tau_in,
                        double* tau_out, const double* sigma, const double* dN, double* mnorm_out,
                        double* maxnorm_out, const int* e3of, const double2* cdel, double* scratch,
                        double* maxn_copy = nullptr,   // maxn_copy: a second place for the global max
                        // trapezoidal rule (sigma_n non-null): dN is the endpoint drift and
                        // (sigma_n, dN0) give the start-of-step strain
-                       const double* sigma_n = nullptr, const double* dN0 = nullptr);
-size_t vh_advance_scratch(const vh_dims& d);
+                       const double* sigma_n = nullptr, const double* dN0 = nullptr,
+                       // channels per tensor batch of cdel (4, or 6 with the toroidal ones)
+                       int nch = 4, const vh_adv_tor& tor = vh_adv_tor());
+// scratch doubles of the advance's reductions (tor: with the toroidal rows)
+size_t vh_advance_scratch(const vh_dims& d, bool tor = false);
 // out2[0] = max |fine - coarse|, out2[1] = max |fine| over n values (NaN kept)
-void vh_coarse_fine(hipStream_t st, size_t n, const double* fine, const double* coarse, double* out2);
+// accumulate: combine with what out2 already holds instead of starting at zero
+void vh_coarse_fine(hipStream_t st, size_t n, const double* fine, const double* coarse, double* out2,
+                    bool accumulate = false);
 // out2[0] = max |dUa - prev|, out2[1] = max |dUa| over the nk complex dUa at
 // the head of dsurf; then prev <- dUa
-void vh_drift_change(hipStream_t st, int nk, const double* dsurf, double* prev, double* out2);
-void vh_launch_memnorm(hipStream_t st, const vh_dims& d, const double* tau, double* mnorm, double* maxnorm);
+// (dWa, prev_t non-null: the toroidal surface drift joins both maxima and is kept likewise)
+void vh_drift_change(hipStream_t st, int nk, const double* dsurf, double* prev, double* out2,
+                     const double* dWa = nullptr, double* prev_t = nullptr);
+// taut (toroidal memory, [em][12][k]) joins the norm when non-null
+void vh_launch_memnorm(hipStream_t st, const vh_dims& d, const double* tau, double* mnorm, double* maxnorm,
+                       const double* taut = nullptr);
 
 // ---------------------------------------------------------------- SHT
 // Factorisation of a longitude FFT of length N (plan of the transform)
@@ -174,23 +194,66 @@ void vh_weighted_total(hipStream_t st, const vh_sht& s, const double* part, int 
 void vh_fill(hipStream_t st, double* p, double v, size_t n);
 
 // ---------------------------------------------------------------- tensor (3-D)
-// X [nb][4][nlm] tensor coefficients -> Fd [nb][6][L+1][nlat] dyad row spectra
-void vh_tsh_synth(hipStream_t st, const vh_sht& s, const double2* X, int nb, double2* Fd);
-// Fd -> C [nb][4][nlm]; n6 == null leaves the raw spin-2 adjoint in channel 3
-void vh_tsh_analysis(hipStream_t st, const vh_sht& s, const double2* Fd, int nb, const double* n6, double2* C);
+// X [nb][nch][nlm] tensor coefficients (nch = 4: lambda 1, 2, 5, 6; nch = 6:
+// also lambda 3, 4 as channels 4, 5) -> Fd [nb][6][L+1][nlat] dyad row spectra
+void vh_tsh_synth(hipStream_t st, const vh_sht& s, const double2* X, int nb, double2* Fd, int nch = 4);
+// Fd -> C [nb][nch][nlm]; n6 (n4) == null leaves the raw spin-2 adjoint in
+// channel 3 (5)
+void vh_tsh_analysis(hipStream_t st, const vh_sht& s, const double2* Fd, int nb, const double* n6, double2* C,
+                     int nch = 4, const double* n4 = nullptr);
 // mode 0: Fd <- FFT(-rate*dt * IFFT(Fin)), rate [nb/bpe][nlat][nphi] (Fin null: Fd, in place);
 // mode 1: grid [nb][6][nlat][nphi] <- IFFT(Fin); mode 2: Fd <- FFT(grid);
 // mode 3: as mode 0 with the trapezoidal weight M / (1 + M/2), M = rate*dt
 void vh_t3_rows(hipStream_t st, const vh_sht& s, double2* Fd, int nb, int mode, const double* rate, int bpe,
                 double dt, double* grid, const double2* Fin = nullptr);
+// X [3 ne3d][nch][nlm]; with nch = 6 the toroidal channels come from taut
+// and the nodal W drift dWt ([nmn][2][nk])
 void vh_t3_gather(hipStream_t st, const vh_dims& d, const vh_dev& v, const int* e3em, int ne3d, const double* tau,
-                  const double* sigma, const double* dN, double2* X);
+                  const double* sigma, const double* dN, double2* X, int nch = 4, const double* taut = nullptr,
+                  const double* dWt = nullptr);
 // trapezoidal rule: X = tau_n + mu (eps_n + eps_{n+1}), eps_n from
-// (sigma_n, dN0), eps_{n+1} from (sigma, dN1)
+// (sigma_n, dN0 / dWt0), eps_{n+1} from (sigma, dN1 / dWt1)
 void vh_t3_gather_trap(hipStream_t st, const vh_dims& d, const vh_dev& v, const int* e3em, int ne3d,
                        const double* tau, const double* sigma_n, const double* dN0, const double* sigma,
-                       const double* dN1, double2* X);
+                       const double* dN1, double2* X, int nch = 4, const double* taut = nullptr,
+                       const double* dWt0 = nullptr, const double* dWt1 = nullptr);
 void vh_tsh_calibrate_n6(hipStream_t st, const vh_sht& s, double* n6_dev);
+// n4(l): the raw toroidal spin-2 adjoint of the unit sectoral (l, m = l)
+// toroidal spin-2 synthesis (channel 5), the Z4 normalisation
+void vh_tsh_calibrate_n4(hipStream_t st, const vh_sht& s, double* n4_dev);
+
+// ---------------------------------------------------------------- toroidal
+// Restricted inverses of the per-degree W operators and the forcing tables
+// (host set-up, extended precision), built when the toroidal channels are
+// first carried.
+vh_tor_plan* vh_tor_plan_create(hipStream_t st, const vh_dims& d, const std::vector<double>& r,
+                                const std::vector<double>& mu, const std::vector<int>& emap,
+                                const std::vector<int>& mnodes, const std::vector<int>& mnode_of);
+void vh_tor_plan_destroy(vh_tor_plan* p);
+// W drift of the toroidal memory taut: nodal dWt [nmn][2][nk], surface dWa
+// [nk] complex; zero where the skip gate is closed
+void vh_tor_drift(hipStream_t st, const vh_tor_plan* p, const vh_dims& d, const vh_dev& v, const double* taut,
+                  const double* mnorm, const double* maxnorm, double skip_tol, double* dWt, double* dWa);
+struct tor_adv_args {
+    int nk, nem, nlm, yoff;
+    double dt;
+    const int* kdeg;
+    const int* emap;
+    const double* mu;
+    const double* MkPerDt;
+    const unsigned char* is3d;
+    const int* e3of;
+    const int2* enode;
+    const double2* cdel;   // [i3*3 + shape][6][nlm]
+    const double* tau_in;
+    double* tau_out;
+    const double* dW;
+    const double* dW0;     // trapezoidal rule only
+    double* pmax;
+    double* pblk;
+};
+void vh_launch_advance_tor(hipStream_t st, const tor_adv_args& a, bool trap);
+const int2* vh_tor_plan_enode(const vh_tor_plan* p);
 
 // ---------------------------------------------------------------- drift plan
 struct vh_drift_plan;

@@ -13,6 +13,14 @@ __device__ __forceinline__ void strain_coeffs(double u1, double u2, double v1, d
     if (Jr <= 2.0) { b[3] = 0.0; c[3] = 0.0; }
 }
 
+// toroidal strain coefficients (lambda = 3, 4) of nodal W; no lambda-4 part at
+// degree 1 (src/vilma_viscoelastic.f90:373-393)
+__device__ __forceinline__ void tor_strain(double w1, double w2, double Jr, double a[2], double b[2], double c[2]) {
+    a[0] = -w1 + w2; b[0] = -w1; c[0] = -w2;
+    a[1] = 0.0; b[1] = w1; c[1] = w2;
+    if (Jr <= 2.0) { b[1] = 0.0; c[1] = 0.0; }
+}
+
 // Dissipative forcing of one element for one re/im memory block
 // (A,B,C)[lambda] -> floc[4] on the test dofs U_e, U_e+1, V_e, V_e+1.
 __device__ __forceinline__ void dissipative_element(double rk, double rk1, double Jr,

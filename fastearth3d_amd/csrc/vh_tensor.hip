@@ -4,8 +4,10 @@
 //     c_s <- c_s + analysis( -Mk3(:,:,e) * synth(c_s + 2 mu_e eps_s) )
 //
 // (increment form, src/vilma_response.f90:1171-1225, :1227-1319), where the
-// tensor transforms map the four spheroidal channels lambda = 1, 2, 5, 6 to
-// the six dyads rr, rt, rp, tt, tp, pp (src/vilma_tensor_sh.f90:168-322).
+// tensor transforms map the four spheroidal channels lambda = 1, 2, 5, 6 --
+// and, when the toroidal degree of freedom is carried, lambda = 3, 4 as
+// channels 4, 5 -- to the six dyads rr, rt, rp, tt, tp, pp
+// (src/vilma_tensor_sh.f90:168-387).
 //
 // The reference composes each tensor transform from 3 + 3 (synthesis) and
 // 3 + 5 (analysis) scalar/vector SHTs plus grid-space products with cot and
@@ -18,14 +20,19 @@
 //   Sg = sum c6 [-J P - 2 cot dP + 2 m^2/s^2 P]      Sh = (i m/s) sum c6 (dP - cot P)
 //   tt = tr + Sg,  pp = tr - Sg,  tp = 4 Sh
 //
+// and with the toroidal channels (Z3 = e_r x grad_1, Z4 its spin-2 partner)
+//
+//   rt += -(i m/s) sum c3 P     rp += sum c3 dP
+//   tt += -Sh4,  pp += Sh4,  tp += Sg4   (Sg4, Sh4: the spin-2 sums of c4)
+//
 // and the analysis is its quadrature adjoint with the reference's
-// normalisations (c2 / J, c5 / (-2J), c6 / n6(l), n6 calibrated at m = l as
-// in src/vilma_tensor_sh.f90:99-133). The pointwise product with the grid
+// normalisations (c2 / J, c5 / (-2J), c6 / n6(l), c3 / J, c4 / n4(l); n6 and
+// n4 calibrated at m = l as in src/vilma_tensor_sh.f90:99-133). The pointwise product with the grid
 // field -Mk3 needs real space: one row kernel per (batch, latitude) does
 // inverse FFT -> multiply -> forward FFT for the six dyads in LDS, so the
 // grid never reaches HBM.
 //
-// Layouts: coefficients X[b][lam][idx] (complex, degree-major idx =
+// Layouts: coefficients X[b][lam][idx] (nch = 4 or 6 channels; complex, degree-major idx =
 // l(l+1)/2 + m); dyad row spectra Fd[b][dy][m][j] (complex, latitude
 // contiguous, so the Legendre kernels read and write coalesced rows and the
 // row kernel's column access is shared through L2 by an XCD-aware block
@@ -63,10 +70,12 @@ static inline unsigned xcd_order_grid(int ngrp, int L) { return (unsigned)(8 * n
 // of the order are loaded once per block for all TB batches; the pair swaps
 // its partial sums and the even lane writes the north row, the odd lane the
 // south row.
-template <int TB, bool JCHUNK>
+template <int TB, bool JCHUNK, int NCH>
 __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __restrict__ X, int nb,
                                                    double2* __restrict__ Fd) {
-    extern __shared__ double2 cs[];   // [TB][5][nl]: c1, c2, -J c5, c6, -J c6
+    // coefficient slots per batch and Legendre sums per batch
+    constexpr int NS = NCH == 4 ? 5 : 8, NQ = NCH == 4 ? 7 : 12;
+    extern __shared__ double2 cs[];   // [TB][NS][nl]: c1, c2, -J c5, c6, -J c6 (, c3, c4, -J c4)
     const int L = s.L, nh = s.nh, nlat = s.nlat, nlm = s.nlm;
     const int ngrp = (nb + TB - 1) / TB;
     // orders fastest within a batch group: neighbouring orders share the
@@ -75,15 +84,19 @@ __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __re
     if (lb >= ngrp * (L + 1)) return;
     const int b0 = (lb / (L + 1)) * TB, m = lb % (L + 1), nl = L - m + 1;
     const int nbb = min(TB, nb - b0);
-    for (int t = threadIdx.x; t < TB * 4 * nl; t += blockDim.x) {
-        const int bb = t / (4 * nl), r = t % (4 * nl), lam = r / nl, il = r % nl, l = m + il;
+    for (int t = threadIdx.x; t < TB * NCH * nl; t += blockDim.x) {
+        const int bb = t / (NCH * nl), r = t % (NCH * nl), lam = r / nl, il = r % nl, l = m + il;
         double2 v = make_double2(0.0, 0.0);
-        if (bb < nbb) v = X[((size_t)(b0 + bb) * 4 + lam) * nlm + l * (l + 1) / 2 + m];
+        if (bb < nbb) v = X[((size_t)(b0 + bb) * NCH + lam) * nlm + l * (l + 1) / 2 + m];
         const double J = (double)l * (double)(l + 1);
-        if (lam == 2) v = make_double2(-J * v.x, -J * v.y);
-        double2* cb = cs + (size_t)bb * 5 * nl;
-        cb[lam * nl + il] = v;
-        if (lam == 3) cb[4 * nl + il] = make_double2(-J * v.x, -J * v.y);
+        double2* cb = cs + (size_t)bb * NS * nl;
+        if (lam < 4) {
+            cb[lam * nl + il] = lam == 2 ? make_double2(-J * v.x, -J * v.y) : v;
+            if (lam == 3) cb[4 * nl + il] = make_double2(-J * v.x, -J * v.y);
+        } else {   // c3 -> slot 5, c4 -> slot 6 and -J c4 -> slot 7
+            cb[(lam + 1) * nl + il] = v;
+            if (lam == 5) cb[7 * nl + il] = make_double2(-J * v.x, -J * v.y);
+        }
     }
     __syncthreads();
     const double* P = s.plm_lj + (size_t)s.mstart[m] * nh;
@@ -94,16 +107,17 @@ __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __re
     const int j = j0 + (threadIdx.x >> 1), par = threadIdx.x & 1;
     const bool live = j < nh;   // both lanes of a pair agree
     const int jj = live ? j : 0;
-    // [batch][sum]: A1, A2, D2, A5, A6, L6, D6 over this lane's parity
-    double2 acc[TB][7];
+    // [batch][sum]: A1, A2, D2, A5, A6, L6, D6 (, A3, D3, A4, L4, D4) over
+    // this lane's parity
+    double2 acc[TB][NQ];
 #pragma unroll
     for (int bb = 0; bb < TB; bb++)
 #pragma unroll
-        for (int q = 0; q < 7; q++) acc[bb][q] = make_double2(0.0, 0.0);
+        for (int q = 0; q < NQ; q++) acc[bb][q] = make_double2(0.0, 0.0);
     auto term = [&](int il, double pv, double dv) __attribute__((always_inline)) {
 #pragma unroll
         for (int bb = 0; bb < TB; bb++) {
-            const double2* cb = cs + (size_t)bb * 5 * nl;
+            const double2* cb = cs + (size_t)bb * NS * nl;
             const double2 c1 = cb[il], c2 = cb[nl + il], c5 = cb[2 * nl + il], c6 = cb[3 * nl + il],
                           l6 = cb[4 * nl + il];
             double2* a = acc[bb];
@@ -114,6 +128,14 @@ __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __re
             a[4].x = fma(c6.x, pv, a[4].x); a[4].y = fma(c6.y, pv, a[4].y);
             a[5].x = fma(l6.x, pv, a[5].x); a[5].y = fma(l6.y, pv, a[5].y);
             a[6].x = fma(c6.x, dv, a[6].x); a[6].y = fma(c6.y, dv, a[6].y);
+            if constexpr (NCH == 6) {
+                const double2 c3 = cb[5 * nl + il], c4 = cb[6 * nl + il], l4 = cb[7 * nl + il];
+                a[7].x = fma(c3.x, pv, a[7].x); a[7].y = fma(c3.y, pv, a[7].y);
+                a[8].x = fma(c3.x, dv, a[8].x); a[8].y = fma(c3.y, dv, a[8].y);
+                a[9].x = fma(c4.x, pv, a[9].x); a[9].y = fma(c4.y, pv, a[9].y);
+                a[10].x = fma(l4.x, pv, a[10].x); a[10].y = fma(l4.y, pv, a[10].y);
+                a[11].x = fma(c4.x, dv, a[11].x); a[11].y = fma(c4.y, dv, a[11].y);
+            }
         }
     };
     // VH_TSH_SU degrees of this parity per step, their table values loaded
@@ -138,13 +160,13 @@ __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __re
     const double mis = (double)m * is, g2 = 2.0 * mis * mis;
 #pragma unroll
     for (int bb = 0; bb < TB; bb++) {
-        double2 S[7];
+        double2 S[NQ];
 #pragma unroll
-        for (int q = 0; q < 7; q++) {
+        for (int q = 0; q < NQ; q++) {
             const double2 own = acc[bb][q];
             const double2 oth = make_double2(__shfl_xor(own.x, 1), __shfl_xor(own.y, 1));
             const double2 e = par == 0 ? own : oth, o = par == 0 ? oth : own;
-            const bool dtype = (q == 2 || q == 6);
+            const bool dtype = (q == 2 || q == 6 || q == 8 || q == 11);
             if (par == 0) S[q] = make_double2(e.x + o.x, e.y + o.y);
             else S[q] = dtype ? make_double2(o.x - e.x, o.y - e.y) : make_double2(e.x - o.x, e.y - o.y);
         }
@@ -161,6 +183,17 @@ __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __re
         out[3] = make_double2(tr.x + Sg.x, tr.y + Sg.y);           // tt
         out[4] = make_double2(4.0 * Sh.x, 4.0 * Sh.y);             // tp
         out[5] = make_double2(tr.x - Sg.x, tr.y - Sg.y);           // pp
+        if constexpr (NCH == 6) {
+            out[1].x += mis * S[7].y; out[1].y -= mis * S[7].x;       // rt += -(i m/s) A3
+            out[2].x += S[8].x; out[2].y += S[8].y;                   // rp += D3
+            const double2 Sg4 = make_double2(S[10].x - 2.0 * cot * S[11].x + g2 * S[9].x,
+                                             S[10].y - 2.0 * cot * S[11].y + g2 * S[9].y);
+            const double2 h4 = make_double2(S[11].x - cot * S[9].x, S[11].y - cot * S[9].y);
+            const double2 Sh4 = make_double2(-mis * h4.y, mis * h4.x);   // i (m/s) (D4 - cot A4)
+            out[3].x -= Sh4.x; out[3].y -= Sh4.y;                     // tt -= Sh4
+            out[5].x += Sh4.x; out[5].y += Sh4.y;                     // pp += Sh4
+            out[4].x += Sg4.x; out[4].y += Sg4.y;                     // tp += Sg4
+        }
 #pragma unroll
         for (int dy = 0; dy < 6; dy++)
             Fd[(((size_t)(b0 + bb) * 6 + dy) * (L + 1) + m) * nlat + row] = out[dy];
@@ -171,26 +204,41 @@ __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __re
 // ----------------------------------------------------------- analysis
 // FP64 MFMA form. For one order m and degree parity p the analysis is the
 // product  S[l][col] = sum_j T[l][j] B[j][col]  with T = P or dP over the north
-// latitudes and B the latitude pair's folded, weighted fields: per batch five
-// P-type complex fields (f1, f2P, f5, fD, fb: 10 real columns) and two dP-type
-// (f2D, fd: 4 columns). Block = (order m, 4 batches): P-type columns 0..39
-// (three 16-wide tiles, 40..47 zero), dP-type 48..63 (one tile). Latitudes
-// are taken 16 at a time: the block folds them into LDS (B), then every
-// wavefront runs its (parity, 16-degree) row tiles through
-// v_mfma_f64_16x16x4 (lane l: A[l%16][l/16], B[l/16][l%16], D[4r+l/16][l%16])
-// with the P/dP fragments read straight from the [m][j][l] tables. The
-// per-(degree, batch) normalisations follow from the accumulated tiles.
+// latitudes and B the latitude pair's folded, weighted fields. Four channels:
+// per batch five P-type complex fields (f1, f2P, f5, fD, fb: 10 real
+// columns) and two dP-type (f2D, fd: 4 columns); block = (order m, 4
+// batches): P-type columns 0..39 (three 16-wide tiles, 40..47 zero), dP-type
+// 48..63 (one tile). Six channels add per batch the P-type f3P, fD4, fb4 and
+// the dP-type f3D, fd4 (16 + 8 real columns); block = (order m, 2 batches):
+// P-type 0..31, dP-type 32..47 (three tiles). Latitudes are taken TA_JC at a
+// time: the block folds them into LDS (B), then every wavefront runs its
+// (parity, 16-degree) row tiles through v_mfma_f64_16x16x4 (lane l:
+// A[l%16][l/16], B[l/16][l%16], D[4r+l/16][l%16]) with the P/dP fragments
+// read straight from the [m][j][l] tables. The per-(degree, batch)
+// normalisations follow from the accumulated tiles.
 typedef double vh_d4 __attribute__((ext_vector_type(4)));
-#define TA_B 4          // batches per block
 #ifndef TA_JC
 #define TA_JC 32        // latitudes per LDS chunk
 #endif
+// column layout of one channel count
+template <int NCH> struct ta_cols {
+    static constexpr int TB = NCH == 4 ? 4 : 2;        // batches per block
+    static constexpr int NQP = NCH == 4 ? 5 : 8;       // P-type complex fields per batch
+    static constexpr int NQD = NCH == 4 ? 2 : 4;       // dP-type complex fields per batch
+    static constexpr int CP = 2 * NQP, CD = 2 * NQD;   // real columns per batch
+    static constexpr int PC = NCH == 4 ? 48 : 32;      // first dP-type column
+    static constexpr int NT = NCH == 4 ? 4 : 3;        // 16-column tiles
+    static_assert(TB * CP <= PC && PC + TB * CD == 16 * NT && 16 * NT <= 64, "analysis column layout");
+};
 // TA_ITEMS: row tiles per wavefront (4 wavefronts): 3 holds the 16-degree
 // tiles of both parities up to lmax 159 (ceil(ne/16) + ceil(no/16) <= 12),
 // 4 up to lmax 255
-template <int TA_ITEMS>
+template <int TA_ITEMS, int NCH>
 __global__ void __launch_bounds__(256) k_tsh_analysis(vh_sht s, const double2* __restrict__ Fd, int nb,
-                                                      const double* __restrict__ n6, double2* __restrict__ C) {
+                                                      const double* __restrict__ n6, const double* __restrict__ n4,
+                                                      double2* __restrict__ C) {
+    using K = ta_cols<NCH>;
+    constexpr int TB = K::TB, NQP = K::NQP, NQD = K::NQD, CP = K::CP, CD = K::CD, PC = K::PC, NT = K::NT;
     // one LDS array, two views: the folded latitude chunk Bs [parity][latitude
     // in chunk][column] (rows padded: the fold stores stride over banks) while
     // the products run, then the per-wavefront staging Ds of one row tile in
@@ -200,31 +248,31 @@ __global__ void __launch_bounds__(256) k_tsh_analysis(vh_sht s, const double2* _
     double (*Bs)[TA_JC][65] = (double (*)[TA_JC][65])Sh;
     double (*Ds)[16][65] = (double (*)[16][65])Sh;
     const int L = s.L, nh = s.nh, nlat = s.nlat, nlm = s.nlm;
-    const int ngrp = (nb + TA_B - 1) / TA_B;
+    const int ngrp = (nb + TB - 1) / TB;
     int m, g;
     if (!xcd_order_block(ngrp, L, m, g)) return;
-    const int b0 = g * TA_B, nl = L - m + 1, nbb = min(TA_B, nb - b0);
+    const int b0 = g * TB, nl = L - m + 1, nbb = min(TB, nb - b0);
     const int ne = (nl + 1) / 2, no = nl / 2;
     const int nte = (ne + 15) / 16, nto = (no + 15) / 16, nitems = nte + nto;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const double ms = (double)m;
     const double* P = s.plm_jl + (size_t)s.mstart[m] * nh;
     const double* D = s.dplm_jl + (size_t)s.mstart[m] * nh;
-    vh_d4 acc[TA_ITEMS][4];
+    vh_d4 acc[TA_ITEMS][NT];
 #pragma unroll
     for (int it = 0; it < TA_ITEMS; it++)
 #pragma unroll
-        for (int t = 0; t < 4; t++) acc[it][t] = vh_d4{0.0, 0.0, 0.0, 0.0};
+        for (int t = 0; t < NT; t++) acc[it][t] = vh_d4{0.0, 0.0, 0.0, 0.0};
     for (int j0 = 0; j0 < nh; j0 += TA_JC) {
         __syncthreads();   // previous chunk's fragments consumed
-        // ---- fold the latitude pairs (j, nlat-1-j), j = j0 + jl, of 4 batches
-        for (int t = threadIdx.x; t < TA_B * TA_JC; t += blockDim.x) {
+        // ---- fold the latitude pairs (j, nlat-1-j), j = j0 + jl, of the block's batches
+        for (int t = threadIdx.x; t < TB * TA_JC; t += blockDim.x) {
             const int bb = t / TA_JC, jl = t % TA_JC, j = j0 + jl;
-            double2 f[2][7];
+            double2 f[2][NQP + NQD];
 #pragma unroll
             for (int h = 0; h < 2; h++)
 #pragma unroll
-                for (int q = 0; q < 7; q++) f[h][q] = make_double2(0.0, 0.0);
+                for (int q = 0; q < NQP + NQD; q++) f[h][q] = make_double2(0.0, 0.0);
             if (bb < nbb && j < nh) {
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
@@ -237,29 +285,39 @@ __global__ void __launch_bounds__(256) k_tsh_analysis(vh_sht s, const double2* _
                     const double mis = ms * is;
                     const double2 Dv = make_double2(d[3].x - d[5].x, d[3].y - d[5].y);     // tt - pp
                     const double2 tp = d[4];
+                    const double g2 = 2.0 * mis * mis, hc = 2.0 * mis * cot;
                     f[h][0] = make_double2(w * d[0].x, w * d[0].y);                      // rr
                     f[h][1] = make_double2(w * (mis * d[2].y), w * (-mis * d[2].x));      // (-i m/s) rp
                     f[h][2] = make_double2(w * (d[3].x + d[5].x), w * (d[3].y + d[5].y)); // tt + pp
                     f[h][3] = make_double2(w * Dv.x, w * Dv.y);
-                    const double g2 = 2.0 * mis * mis, hc = 2.0 * mis * cot;               // 2 m^2/s^2 D + 2 i m cot/s tp
+                    // 2 m^2/s^2 D + 2 i m cot/s tp
                     f[h][4] = make_double2(w * (g2 * Dv.x - hc * tp.y), w * (g2 * Dv.y + hc * tp.x));
-                    f[h][5] = make_double2(w * d[1].x, w * d[1].y);                       // rt
-                    f[h][6] = make_double2(w * (-2.0 * cot * Dv.x + 2.0 * mis * tp.y),    // -2 cot D - 2 i m/s tp
-                                           w * (-2.0 * cot * Dv.y - 2.0 * mis * tp.x));
+                    f[h][NQP] = make_double2(w * d[1].x, w * d[1].y);                     // rt
+                    f[h][NQP + 1] = make_double2(w * (-2.0 * cot * Dv.x + 2.0 * mis * tp.y),   // -2 cot D - 2 i m/s tp
+                                                 w * (-2.0 * cot * Dv.y - 2.0 * mis * tp.x));
+                    if constexpr (NCH == 6) {
+                        f[h][5] = make_double2(w * (-mis * d[1].y), w * (mis * d[1].x));    // (i m/s) rt
+                        f[h][6] = make_double2(w * tp.x, w * tp.y);                          // tp
+                        // 2 m^2/s^2 tp - 2 i m cot/s D
+                        f[h][7] = make_double2(w * (g2 * tp.x + hc * Dv.y), w * (g2 * tp.y - hc * Dv.x));
+                        f[h][NQP + 2] = make_double2(w * d[2].x, w * d[2].y);                 // rp
+                        f[h][NQP + 3] = make_double2(w * (-2.0 * cot * tp.x - 2.0 * mis * Dv.y),   // -2 cot tp + 2 i m/s D
+                                                     w * (-2.0 * cot * tp.y + 2.0 * mis * Dv.x));
+                    }
                 }
             }
 #pragma unroll
-            for (int q = 0; q < 7; q++) {
-                const bool dtype = q >= 5;
+            for (int q = 0; q < NQP + NQD; q++) {
+                const bool dtype = q >= NQP;
                 const double2 a = f[0][q], c = f[1][q];
                 const double2 plus = make_double2(a.x + c.x, a.y + c.y), minus = make_double2(a.x - c.x, a.y - c.y);
                 const double2 ev = dtype ? minus : plus, od = dtype ? plus : minus;   // even / odd l+m
-                const int col = dtype ? 48 + bb * 4 + 2 * (q - 5) : bb * 10 + 2 * q;
+                const int col = dtype ? PC + bb * CD + 2 * (q - NQP) : bb * CP + 2 * q;
                 Bs[0][jl][col] = ev.x; Bs[0][jl][col + 1] = ev.y;
                 Bs[1][jl][col] = od.x; Bs[1][jl][col + 1] = od.y;
             }
             if (bb == 0)
-                for (int c = 40; c < 48; c++) { Bs[0][jl][c] = 0.0; Bs[1][jl][c] = 0.0; }
+                for (int c = TB * CP; c < PC; c++) { Bs[0][jl][c] = 0.0; Bs[1][jl][c] = 0.0; }
         }
         __syncthreads();
         // ---- MFMA over this chunk's latitudes for the wavefront's row tiles;
@@ -295,10 +353,10 @@ __global__ void __launch_bounds__(256) k_tsh_analysis(vh_sht s, const double2* _
                 for (int it = 0; it < TA_ITEMS; it++) {
                     if (wave + 4 * it >= nitems) break;
                     const double* Bp = &Bs[par_[it]][jl][0];
-                    acc[it][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(aP[it], Bp[(lane & 15)], acc[it][0], 0, 0, 0);
-                    acc[it][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(aP[it], Bp[16 + (lane & 15)], acc[it][1], 0, 0, 0);
-                    acc[it][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(aP[it], Bp[32 + (lane & 15)], acc[it][2], 0, 0, 0);
-                    acc[it][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(aD[it], Bp[48 + (lane & 15)], acc[it][3], 0, 0, 0);
+#pragma unroll
+                    for (int t = 0; t < NT; t++)
+                        acc[it][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(16 * t < PC ? aP[it] : aD[it],
+                                                                          Bp[16 * t + (lane & 15)], acc[it][t], 0, 0, 0);
                 }
 #pragma unroll
                 for (int it = 0; it < TA_ITEMS; it++) { aP[it] = nP[it]; aD[it] = nD[it]; }
@@ -314,7 +372,7 @@ __global__ void __launch_bounds__(256) k_tsh_analysis(vh_sht s, const double2* _
         const int p = item < nte ? 0 : 1, rt = item < nte ? item : item - nte;
         const int np_ = p ? no : ne;
 #pragma unroll
-        for (int t = 0; t < 4; t++)
+        for (int t = 0; t < NT; t++)
 #pragma unroll
             for (int r = 0; r < 4; r++) Ds[wave][4 * r + (lane >> 4)][t * 16 + (lane & 15)] = acc[it][t][r];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -324,15 +382,17 @@ __global__ void __launch_bounds__(256) k_tsh_analysis(vh_sht s, const double2* _
         if (i < np_ && bb < nbb) {
             const int l = m + 2 * i + p;
             const double* row = &Ds[wave][rr][0];
-            const double2 s1 = make_double2(row[bb * 10 + 0], row[bb * 10 + 1]);
-            const double2 s2 = make_double2(row[bb * 10 + 2] + row[48 + bb * 4 + 0], row[bb * 10 + 3] + row[48 + bb * 4 + 1]);
-            const double2 s5 = make_double2(row[bb * 10 + 4], row[bb * 10 + 5]);
-            const double2 sD = make_double2(row[bb * 10 + 6], row[bb * 10 + 7]);
-            const double2 sb = make_double2(row[bb * 10 + 8], row[bb * 10 + 9]);
-            const double2 sd = make_double2(row[48 + bb * 4 + 2], row[48 + bb * 4 + 3]);
+            const double* pr = row + bb * CP;        // P-type columns of this batch
+            const double* dr = row + PC + bb * CD;   // dP-type columns
+            const double2 s1 = make_double2(pr[0], pr[1]);
+            const double2 s2 = make_double2(pr[2] + dr[0], pr[3] + dr[1]);
+            const double2 s5 = make_double2(pr[4], pr[5]);
+            const double2 sD = make_double2(pr[6], pr[7]);
+            const double2 sb = make_double2(pr[8], pr[9]);
+            const double2 sd = make_double2(dr[2], dr[3]);
             const double J = (double)l * (double)(l + 1);
             const size_t idx = (size_t)l * (l + 1) / 2 + m;
-            double2* Cb = C + (size_t)(b0 + bb) * 4 * nlm;
+            double2* Cb = C + (size_t)(b0 + bb) * NCH * nlm;
             Cb[idx] = s1;
             Cb[nlm + idx] = J > 0.0 ? make_double2(s2.x / J, s2.y / J) : make_double2(0.0, 0.0);
             Cb[2 * nlm + idx] = J > 0.0 ? make_double2(s5.x / (-2.0 * J), s5.y / (-2.0 * J)) : make_double2(0.0, 0.0);
@@ -343,6 +403,22 @@ __global__ void __launch_bounds__(256) k_tsh_analysis(vh_sht s, const double2* _
                 c6 = n != 0.0 ? make_double2(raw.x / n, raw.y / n) : make_double2(0.0, 0.0);
             }
             Cb[3 * nlm + idx] = c6;
+            if constexpr (NCH == 6) {
+                // c3 = (1/J) int v . conj(e_r x grad_1 Y); c4: the toroidal
+                // spin-2 adjoint 1/2 Sg*(tp) - Sh*(tt - pp) over n4
+                const double2 s3 = make_double2(pr[10] + dr[4], pr[11] + dr[5]);
+                const double2 sD4 = make_double2(pr[12], pr[13]);
+                const double2 sb4 = make_double2(pr[14], pr[15]);
+                const double2 sd4 = make_double2(dr[6], dr[7]);
+                Cb[4 * nlm + idx] = J > 0.0 ? make_double2(s3.x / J, s3.y / J) : make_double2(0.0, 0.0);
+                const double2 raw4 = make_double2(0.5 * (-J * sD4.x + sb4.x + sd4.x), 0.5 * (-J * sD4.y + sb4.y + sd4.y));
+                double2 c4 = raw4;
+                if (n4) {
+                    const double n = n4[l];
+                    c4 = n != 0.0 ? make_double2(raw4.x / n, raw4.y / n) : make_double2(0.0, 0.0);
+                }
+                Cb[5 * nlm + idx] = c4;
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -442,14 +518,42 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, d
 }
 
 // ------------------------------------------------------------- gather
-// X[(i3*3 + s)*4 + lam][k+1] = tau_s + 2 mu eps_s of slot k, element e3em[i3].
+// Toroidal channels 4, 5 of X: taut_s + wgt * (eps_s(W1) [+ eps_s(W0)]) of
+// slot k, element em (memory nodes n0, n1), written after the spheroidal four.
+__device__ __forceinline__ void gather_tor(int nk, int nlm, int k, int em, int n0, int n1, double Jr, double wgt,
+                                           const double* __restrict__ taut, const double* __restrict__ W1,
+                                           const double* __restrict__ W0, double2* Xb) {
+    double st[2][3][2];
+#pragma unroll
+    for (int ri = 0; ri < 2; ri++) {
+        tor_strain(W1[((size_t)n0 * 2 + ri) * nk + k], W1[((size_t)n1 * 2 + ri) * nk + k], Jr, st[ri][0], st[ri][1],
+                   st[ri][2]);
+        if (W0) {
+            double s0[3][2];
+            tor_strain(W0[((size_t)n0 * 2 + ri) * nk + k], W0[((size_t)n1 * 2 + ri) * nk + k], Jr, s0[0], s0[1], s0[2]);
+#pragma unroll
+            for (int c = 0; c < 3; c++) { st[ri][c][0] += s0[c][0]; st[ri][c][1] += s0[c][1]; }
+        }
+    }
+    const double* t = taut + (size_t)em * 12 * nk + k;
+#pragma unroll
+    for (int sh = 0; sh < 3; sh++)
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const double re = t[(size_t)((sh * 2 + 0) * 2 + q) * nk] + wgt * st[0][sh][q];
+            const double im = t[(size_t)((sh * 2 + 1) * 2 + q) * nk] + wgt * st[1][sh][q];
+            Xb[(size_t)(sh * 6 + 4 + q) * nlm + k + 1] = make_double2(re, im);
+        }
+}
+
+// X[(i3*3 + s)*nch + lam][k+1] = tau_s + 2 mu eps_s of slot k, element e3em[i3].
 __global__ void __launch_bounds__(256) k_t3_gather(int nr, int nk, int nlm, vh_dev v, const int* __restrict__ e3em,
         const double* __restrict__ tau, const double* __restrict__ sigma, const double* __restrict__ dN,
-        double2* __restrict__ X) {
+        double2* __restrict__ X, int nch, const double* __restrict__ taut, const double* __restrict__ dWt) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x, i3 = blockIdx.y;
-    double2* Xb = X + (size_t)i3 * 12 * nlm;
+    double2* Xb = X + (size_t)i3 * 3 * nch * nlm;
     if (k == 0)
-        for (int q = 0; q < 12; q++) Xb[(size_t)q * nlm] = make_double2(0.0, 0.0);
+        for (int q = 0; q < 3 * nch; q++) Xb[(size_t)q * nlm] = make_double2(0.0, 0.0);
     if (k >= nk) return;
     const int em = e3em[i3], e = v.emap[em];
     const int l = v.kdeg[k];
@@ -475,8 +579,9 @@ __global__ void __launch_bounds__(256) k_t3_gather(int nr, int nk, int nlm, vh_d
         for (int lam = 0; lam < 4; lam++) {
             double re = t[(size_t)((sh * 2 + 0) * 4 + lam) * nk] + twoMu * sc[0][sh][lam];
             double im = t[(size_t)((sh * 2 + 1) * 4 + lam) * nk] + twoMu * sc[1][sh][lam];
-            Xb[(size_t)(sh * 4 + lam) * nlm + k + 1] = make_double2(re, im);
+            Xb[(size_t)(sh * nch + lam) * nlm + k + 1] = make_double2(re, im);
         }
+    if (nch == 6) gather_tor(nk, nlm, k, em, n0, n1, Jr, twoMu, taut, dWt, nullptr, Xb);
 }
 
 // X[(i3*3 + s)*4 + lam][k+1] = tau_s + mu (eps_n,s + eps_{n+1},s) for the
@@ -485,11 +590,12 @@ __global__ void __launch_bounds__(256) k_t3_gather(int nr, int nk, int nlm, vh_d
 // linear in this one tensor, so the three syntheses of the reference become one)
 __global__ void __launch_bounds__(256) k_t3_gather_trap(int nr, int nk, int nlm, vh_dev v, const int* __restrict__ e3em,
         const double* __restrict__ tau, const double* __restrict__ sigma_n, const double* __restrict__ dN0,
-        const double* __restrict__ sigma, const double* __restrict__ dN1, double2* __restrict__ X) {
+        const double* __restrict__ sigma, const double* __restrict__ dN1, double2* __restrict__ X, int nch,
+        const double* __restrict__ taut, const double* __restrict__ dWt0, const double* __restrict__ dWt1) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x, i3 = blockIdx.y;
-    double2* Xb = X + (size_t)i3 * 12 * nlm;
+    double2* Xb = X + (size_t)i3 * 3 * nch * nlm;
     if (k == 0)
-        for (int q = 0; q < 12; q++) Xb[(size_t)q * nlm] = make_double2(0.0, 0.0);
+        for (int q = 0; q < 3 * nch; q++) Xb[(size_t)q * nlm] = make_double2(0.0, 0.0);
     if (k >= nk) return;
     const int em = e3em[i3], e = v.emap[em];
     const int l = v.kdeg[k];
@@ -531,36 +637,58 @@ __global__ void __launch_bounds__(256) k_t3_gather_trap(int nr, int nk, int nlm,
         for (int lam = 0; lam < 4; lam++) {
             double re = t[(size_t)((sh * 2 + 0) * 4 + lam) * nk] + mu * sc[0][sh][lam];
             double im = t[(size_t)((sh * 2 + 1) * 4 + lam) * nk] + mu * sc[1][sh][lam];
-            Xb[(size_t)(sh * 4 + lam) * nlm + k + 1] = make_double2(re, im);
+            Xb[(size_t)(sh * nch + lam) * nlm + k + 1] = make_double2(re, im);
         }
+    if (nch == 6) gather_tor(nk, nlm, k, em, n0, n1, Jr, mu, taut, dWt1, dWt0, Xb);
 }
 
 // ------------------------------------------------------------- host side
-// LDS of the synthesis (VH_TSH_TB_SYNTH batches per block) and of the row pass
-static size_t synth_lds(const vh_sht& s) { return (size_t)VH_TSH_TB_SYNTH * 5 * (s.L + 1) * sizeof(double2); }
+// LDS of the synthesis (VH_TSH_TB_SYNTH batches per block with four
+// channels, two batches with six) and of the row pass
+#define TSH_TB6 2
+static size_t synth_lds(const vh_sht& s, int nch) {
+    return nch == 4 ? (size_t)VH_TSH_TB_SYNTH * 5 * (s.L + 1) * sizeof(double2)
+                    : (size_t)TSH_TB6 * 8 * (s.L + 1) * sizeof(double2);
+}
 static size_t rows_lds(const vh_sht& s) {
     return (size_t)3 * (fft_fixed_size(s.nphi) ? 1 : 2) * s.nphi * sizeof(double2) + (size_t)s.nphi * sizeof(double);
 }
 
-void vh_tsh_synth(hipStream_t st, const vh_sht& s, const double2* X, int nb, double2* Fd) {
+void vh_tsh_synth(hipStream_t st, const vh_sht& s, const double2* X, int nb, double2* Fd, int nch) {
     if (nb <= 0) return;
-    const int ngrp = (nb + VH_TSH_TB_SYNTH - 1) / VH_TSH_TB_SYNTH;
+    if (nch != 4 && nch != 6) throw vh_error("tensor synthesis: 4 or 6 channels");
+    const int tb = nch == 4 ? VH_TSH_TB_SYNTH : TSH_TB6;
+    const int ngrp = (nb + tb - 1) / tb;
     const int nthr = std::min(256, ((2 * s.nh + 63) / 64) * 64);
-    if (s.nh > 128)
-        k_tsh_synth<VH_TSH_TB_SYNTH, true><<<xcd_grid(ngrp * (s.L + 1)), nthr, synth_lds(s), st>>>(s, X, nb, Fd);
-    else
-        k_tsh_synth<VH_TSH_TB_SYNTH, false><<<xcd_grid(ngrp * (s.L + 1)), nthr, synth_lds(s), st>>>(s, X, nb, Fd);
+    const dim3 g(xcd_grid(ngrp * (s.L + 1)));
+    const size_t lds = synth_lds(s, nch);
+    if (nch == 4) {
+        if (s.nh > 128) k_tsh_synth<VH_TSH_TB_SYNTH, true, 4><<<g, nthr, lds, st>>>(s, X, nb, Fd);
+        else k_tsh_synth<VH_TSH_TB_SYNTH, false, 4><<<g, nthr, lds, st>>>(s, X, nb, Fd);
+    } else {
+        if (s.nh > 128) k_tsh_synth<TSH_TB6, true, 6><<<g, nthr, lds, st>>>(s, X, nb, Fd);
+        else k_tsh_synth<TSH_TB6, false, 6><<<g, nthr, lds, st>>>(s, X, nb, Fd);
+    }
     HIPCHECK(hipGetLastError());
 }
 
-void vh_tsh_analysis(hipStream_t st, const vh_sht& s, const double2* Fd, int nb, const double* n6, double2* C) {
+void vh_tsh_analysis(hipStream_t st, const vh_sht& s, const double2* Fd, int nb, const double* n6, double2* C,
+                     int nch, const double* n4) {
     if (nb <= 0) return;
+    if (nch != 4 && nch != 6) throw vh_error("tensor analysis: 4 or 6 channels");
     // row tiles of one order: ceil(ne/16) + ceil(no/16) <= 4 wavefronts x TA_ITEMS
     const int tiles = ((s.L + 2) / 2 + 15) / 16 + ((s.L + 1) / 2 + 15) / 16;
     if (tiles > 16) throw vh_error("tensor analysis: lmax above 255 is not supported by the MFMA tiling");
-    const int ngrp = (nb + TA_B - 1) / TA_B;
-    if (tiles <= 12) k_tsh_analysis<3><<<xcd_order_grid(ngrp, s.L), 256, 0, st>>>(s, Fd, nb, n6, C);
-    else k_tsh_analysis<4><<<xcd_order_grid(ngrp, s.L), 256, 0, st>>>(s, Fd, nb, n6, C);
+    const int tb = nch == 4 ? ta_cols<4>::TB : ta_cols<6>::TB;
+    const int ngrp = (nb + tb - 1) / tb;
+    const dim3 g(xcd_order_grid(ngrp, s.L));
+    if (nch == 4) {
+        if (tiles <= 12) k_tsh_analysis<3, 4><<<g, 256, 0, st>>>(s, Fd, nb, n6, nullptr, C);
+        else k_tsh_analysis<4, 4><<<g, 256, 0, st>>>(s, Fd, nb, n6, nullptr, C);
+    } else {
+        if (tiles <= 12) k_tsh_analysis<3, 6><<<g, 256, 0, st>>>(s, Fd, nb, n6, n4, C);
+        else k_tsh_analysis<4, 6><<<g, 256, 0, st>>>(s, Fd, nb, n6, n4, C);
+    }
     HIPCHECK(hipGetLastError());
 }
 
@@ -573,18 +701,22 @@ void vh_t3_rows(hipStream_t st, const vh_sht& s, double2* Fd, int nb, int mode, 
 
 void vh_t3_gather_trap(hipStream_t st, const vh_dims& d, const vh_dev& v, const int* e3em, int ne3d,
                        const double* tau, const double* sigma_n, const double* dN0, const double* sigma,
-                       const double* dN1, double2* X) {
+                       const double* dN1, double2* X, int nch, const double* taut, const double* dWt0,
+                       const double* dWt1) {
     if (ne3d <= 0) return;
+    if (nch == 6 && !(taut && dWt0 && dWt1)) throw vh_error("tensor gather: toroidal channels without their memory");
     dim3 g((d.nk + 255) / 256, ne3d);
-    k_t3_gather_trap<<<g, 256, 0, st>>>(d.nr, d.nk, d.nlm, v, e3em, tau, sigma_n, dN0, sigma, dN1, X);
+    k_t3_gather_trap<<<g, 256, 0, st>>>(d.nr, d.nk, d.nlm, v, e3em, tau, sigma_n, dN0, sigma, dN1, X, nch, taut,
+                                        dWt0, dWt1);
     HIPCHECK(hipGetLastError());
 }
 
 void vh_t3_gather(hipStream_t st, const vh_dims& d, const vh_dev& v, const int* e3em, int ne3d, const double* tau,
-                  const double* sigma, const double* dN, double2* X) {
+                  const double* sigma, const double* dN, double2* X, int nch, const double* taut, const double* dWt) {
     if (ne3d <= 0) return;
+    if (nch == 6 && !(taut && dWt)) throw vh_error("tensor gather: toroidal channels without their memory");
     dim3 g((d.nk + 255) / 256, ne3d);
-    k_t3_gather<<<g, 256, 0, st>>>(d.nr, d.nk, d.nlm, v, e3em, tau, sigma, dN, X);
+    k_t3_gather<<<g, 256, 0, st>>>(d.nr, d.nk, d.nlm, v, e3em, tau, sigma, dN, X, nch, taut, dWt);
     HIPCHECK(hipGetLastError());
 }
 
@@ -614,5 +746,32 @@ void vh_tsh_calibrate_n6(hipStream_t st, const vh_sht& s, double* n6_dev) {
     std::vector<double> n6(L + 1, 0.0);
     for (int l = 2; l <= L; l++) n6[l] = C[(size_t)3 * nlm + l * (l + 1) / 2 + l].x;
     HIPCHECK(hipMemcpyAsync(n6_dev, n6.data(), (L + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipStreamSynchronize(st));
+}
+
+// n4(l), l = 2..L: as n6 for the toroidal spin-2 channel (unit sectoral c4
+// synthesises tt = -Sh4, pp = Sh4, tp = Sg4; its raw adjoint at (l, l)).
+void vh_tsh_calibrate_n4(hipStream_t st, const vh_sht& s, double* n4_dev) {
+    const int L = s.L, nlm = s.nlm;
+    std::vector<double2> X((size_t)6 * nlm, make_double2(0.0, 0.0));
+    for (int l = 2; l <= L; l++) X[(size_t)5 * nlm + l * (l + 1) / 2 + l] = make_double2(1.0, 0.0);
+    double2 *dX = nullptr, *dF = nullptr, *dC = nullptr;
+    double* dg = nullptr;
+    HIPCHECK(hipMalloc((void**)&dX, X.size() * sizeof(double2)));
+    HIPCHECK(hipMalloc((void**)&dF, (size_t)6 * (L + 1) * s.nlat * sizeof(double2)));
+    HIPCHECK(hipMalloc((void**)&dC, X.size() * sizeof(double2)));
+    HIPCHECK(hipMalloc((void**)&dg, (size_t)6 * s.nlat * s.nphi * sizeof(double)));
+    HIPCHECK(hipMemcpyAsync(dX, X.data(), X.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+    vh_tsh_synth(st, s, dX, 1, dF, 6);
+    vh_t3_rows(st, s, dF, 1, 1, nullptr, 1, 0.0, dg);
+    vh_t3_rows(st, s, dF, 1, 2, nullptr, 1, 0.0, dg);
+    vh_tsh_analysis(st, s, dF, 1, nullptr, dC, 6, nullptr);
+    std::vector<double2> C(X.size());
+    HIPCHECK(hipMemcpyAsync(C.data(), dC, C.size() * sizeof(double2), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    (void)hipFree(dX); (void)hipFree(dF); (void)hipFree(dC); (void)hipFree(dg);
+    std::vector<double> n4(L + 1, 0.0);
+    for (int l = 2; l <= L; l++) n4[l] = C[(size_t)5 * nlm + l * (l + 1) / 2 + l].x;
+    HIPCHECK(hipMemcpyAsync(n4_dev, n4.data(), (L + 1) * sizeof(double), hipMemcpyHostToDevice, st));
     HIPCHECK(hipStreamSynchronize(st));
 }

@@ -40,6 +40,10 @@ def load():
         L.vh_sht_synthesis.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.vh_tensor_synthesis.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.vh_tensor_analysis.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.vh_tensor_synthesis_ch.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.vh_tensor_analysis_ch.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.vh_tensor_norms.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.vh_get_drift_tor.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.vh_update_device.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
         L.vh_profile.argtypes = [C.c_void_p, C.c_int]
         L.vh_profile_read.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -131,18 +135,34 @@ class SolidEarth(ModelHandle):
         return f
 
     def tensor_synth(self, c):
-        """tensor_sh_synth: c (4, nlm) complex LM order -> dyads (6, nlat, nphi)."""
+        """tensor_sh_synth: c (4 or 6, nlm) complex LM order (lambda 1, 2, 5, 6
+        and, with six rows, 3, 4) -> dyads (6, nlat, nphi)."""
+        c = np.ascontiguousarray(c, dtype=np.complex128)
+        if c.ndim != 2 or c.shape[0] not in (4, 6) or c.shape[1] != self.dims["nlm"]:
+            raise VilmaError("tensor_sh_synth: c must be (4 or 6, nlm)")
         d = np.empty((6, self.nlat, self.nphi))
-        self._check(self._lib.vh_tensor_synthesis(self._h, _cp(np.ascontiguousarray(c, dtype=np.complex128)),
-                                                  _cp(d)), "tensor_sh_synth")
+        self._check(self._lib.vh_tensor_synthesis_ch(self._h, c.shape[0], _cp(c), _cp(d)), "tensor_sh_synth")
         return d
 
-    def tensor_analysis(self, d):
-        """tensor_sh_analysis: dyads (6, nlat, nphi) -> c (4, nlm) complex."""
-        c = np.empty((4, self.dims["nlm"]), dtype=np.complex128)
-        self._check(self._lib.vh_tensor_analysis(self._h, _cp(np.ascontiguousarray(d, dtype=np.float64)),
-                                                 _cp(c)), "tensor_sh_analysis")
+    def tensor_analysis(self, d, nch=4):
+        """tensor_sh_analysis: dyads (6, nlat, nphi) -> c (nch, nlm) complex."""
+        d = self._sized(d, 6 * self.nlat * self.nphi, "tensor_sh_analysis: dyads must be (6, nlat, nphi)")
+        c = np.empty((nch, self.dims["nlm"]), dtype=np.complex128)
+        self._check(self._lib.vh_tensor_analysis_ch(self._h, int(nch), _cp(d), _cp(c)), "tensor_sh_analysis")
         return c
+
+    def tensor_norms(self):
+        """(n6, n4) per degree 0..lmax: the spin-2 normalisations."""
+        n6 = np.empty(self.dims["lmax"] + 1); n4 = np.empty_like(n6)
+        self._check(self._lib.vh_tensor_norms(self._h, _cp(n6), _cp(n4)), "tensor_sh_init")
+        return n6, n4
+
+    def drift_tor(self):
+        """Nodal toroidal W drift (2, nk, nr): re, im (zero unless carried)."""
+        d = self.dims
+        out = np.empty((2, d["nk"], d["nr"]))
+        self._check(self._lib.vh_get_drift_tor(self._h, _cp(out)), "get_drift_tor")
+        return out
 
     def apply(self, sigma):
         u = np.empty(self.dims["nlm"], dtype=np.complex128)

@@ -320,6 +320,11 @@ int vh_resp_set_dt(vh_model* h, double dt_s);
  * slot order k2lm of :629-648) and nodal dUn_re,dUn_im,dVn_re,dVn_im as
  * (nr, nk) column-major. Either pointer group may be NULL. */
 int vh_get_drift(vh_model* h, double* dsurf /* 3*2*nk */, double* dnodal /* 4*nr*nk */);
+/* The toroidal W drift of the last begin_step at every mesh node, dWn_re and
+ * dWn_im as (nr, nk) column-major (the tor branch of solve_drift,
+ * src/vilma_response.f90:694-809; zero at nodes without memory, and
+ * everywhere unless the toroidal channels are carried). */
+int vh_get_drift_tor(vh_model* h, double* dwn /* 2*nr*nk */);
 /* Per-degree elastic gains gu, gn, gv (0..lmax) and unit-load nodal fields
  * xUn, xVn (nr, lmax) of response_init_ve (:599-627). */
 int vh_get_gains(vh_model* h, double* g3 /* 3*(lmax+1) */, double* xn /* 2*nr*lmax */);
@@ -333,6 +338,14 @@ int vh_sht_synthesis(vh_model* h, const double* lm, double* grid);
  * The first call calibrates n6 (tensor_sh_init, :99-133). */
 int vh_tensor_synthesis(vh_model* h, const double* c, double* dyad);
 int vh_tensor_analysis(vh_model* h, const double* dyad, double* c);
+/* The same with nch = 4 or 6 channels: with 6, c is [6][nlm] and channels 4, 5
+ * are the toroidal lambda = 3, 4 (Z3 = e_r x grad_1, Z4 its spin-2 partner,
+ * src/vilma_tensor_sh.f90:168-242, :360-387), normalised by J and n4. */
+int vh_tensor_synthesis_ch(vh_model* h, int nch, const double* c, double* dyad);
+int vh_tensor_analysis_ch(vh_model* h, int nch, const double* dyad, double* c);
+/* The spin-2 normalisations n6(l), n4(l), l = 0..lmax, of the tensor analysis
+ * (tensor_sh_init's calibration at m = l, src/vilma_tensor_sh.f90:99-133). */
+int vh_tensor_norms(vh_model* h, double* n6, double* n4);
 
 /* ---- Reverse mode (no counterpart in the reference, which has no adjoint).
  * Gradients of a misfit J(rsl_1..rsl_n) of the rsl after each of n coupling
