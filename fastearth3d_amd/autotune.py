@@ -139,6 +139,7 @@ def main():
     ap.add_argument("--calls", type=int, default=8)
     ap.add_argument("--write", action="store_true")
     ap.add_argument("--report")
+    ap.add_argument("--knobs", help="comma-separated subset of the knobs to search (default: all)")
     ap.add_argument("--time", help=argparse.SUPPRESS)
     a = ap.parse_args()
     if a.time:
@@ -148,7 +149,10 @@ def main():
     rows = []
     work = tempfile.mkdtemp(prefix="vh_autotune_")
     try:
+        only = set(a.knobs.split(",")) if a.knobs else None
         for knob, (unit, cands, kernels) in KNOBS.items():
+            if only is not None and knob not in only:
+                continue
             scores = {}
             for v in cands:
                 vals = dict(best, **{knob: v})
