@@ -18,6 +18,8 @@ The knobs (defaults in csrc/vh_tuned.h):
   VH_TSH_SU        degrees per parity whose table values the synthesis loads together
   VH_DRIFT_RW      right-hand sides per block of the drift solve
   VH_TCHUNK        tensor batches per synthesis / row / analysis launch
+  VH_TA_PF         k-steps of table fragments the tensor analysis keeps in flight
+  VH_TA_JC         latitudes per LDS chunk of the tensor analysis
 The sea-level solve's launch shape is a run-time knob (VH_T_SLE_BLOCKS),
 tuned per ensemble by vh_ensemble_tune.
 """
@@ -40,6 +42,8 @@ KNOBS = {
     "VH_TSH_SU": ("vh_tensor.hip", [1, 2, 4, 8], ["k_tsh_synth"]),
     "VH_DRIFT_RW": ("vh_drift.hip", [16, 32, 64], ["k_drift"]),
     "VH_TCHUNK": ("vh_api.hip", [24, 36, 48, 72], ["k_tsh_synth", "k_t3_rows", "k_tsh_analysis"]),
+    "VH_TA_PF": ("vh_tensor.hip", [1, 2, 3, 4], ["k_tsh_analysis"]),
+    "VH_TA_JC": ("vh_tensor.hip", [32, 48, 64, 96], ["k_tsh_analysis"]),
 }
 
 

@@ -217,9 +217,7 @@ __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __re
 // read straight from the [m][j][l] tables. The per-(degree, batch)
 // normalisations follow from the accumulated tiles.
 typedef double vh_d4 __attribute__((ext_vector_type(4)));
-#ifndef TA_JC
-#define TA_JC 32        // latitudes per LDS chunk
-#endif
+#define TA_JC VH_TA_JC   // latitudes per LDS chunk
 // column layout of one channel count
 template <int NCH> struct ta_cols {
     static constexpr int TB = NCH == 4 ? 4 : 2;        // batches per block
@@ -263,7 +261,36 @@ __global__ void __launch_bounds__(256) k_tsh_analysis(vh_sht s, const double2* _
     for (int it = 0; it < TA_ITEMS; it++)
 #pragma unroll
         for (int t = 0; t < NT; t++) acc[it][t] = vh_d4{0.0, 0.0, 0.0, 0.0};
+    // the wavefront's row tiles: parity, degree offset, live rows
+    int il_[TA_ITEMS], par_[TA_ITEMS];
+    bool row_[TA_ITEMS];
+#pragma unroll
+    for (int it = 0; it < TA_ITEMS; it++) {
+        const int item = wave + 4 * it;
+        const int p = item < nte ? 0 : 1, rt = item < nte ? item : item - nte;
+        const int i = rt * 16 + (lane & 15);
+        par_[it] = p;
+        il_[it] = 2 * i + p;
+        row_[it] = item < nitems && i < (p ? no : ne);
+    }
+    // P/dP fragments of k-step ks of the chunk at j0 (the A operand: rows of
+    // the [m][j][l] tables, read straight from global memory)
+    auto frag = [&](int j0c, int ks, double (&fP)[TA_ITEMS], double (&fD)[TA_ITEMS]) {
+        const int j = j0c + ks * 4 + (lane >> 4);
+#pragma unroll
+        for (int it = 0; it < TA_ITEMS; it++) {
+            const bool ok = row_[it] && j < nh;
+            fP[it] = ok ? P[(size_t)j * nl + il_[it]] : 0.0;
+            fD[it] = ok ? D[(size_t)j * nl + il_[it]] : 0.0;
+        }
+    };
+    constexpr int PF = VH_TA_PF, KS = TA_JC / 4;   // fragments in flight ahead of the products
+    static_assert(PF >= 1 && PF < KS, "prefetch distance");
     for (int j0 = 0; j0 < nh; j0 += TA_JC) {
+        // the chunk's first fragments are in flight while the block folds
+        double aP[PF + 1][TA_ITEMS], aD[PF + 1][TA_ITEMS];
+#pragma unroll
+        for (int q = 0; q < PF; q++) frag(j0, q, aP[q], aD[q]);
         __syncthreads();   // previous chunk's fragments consumed
         // ---- fold the latitude pairs (j, nlat-1-j), j = j0 + jl, of the block's batches
         for (int t = threadIdx.x; t < TB * TA_JC; t += blockDim.x) {
@@ -321,45 +348,21 @@ __global__ void __launch_bounds__(256) k_tsh_analysis(vh_sht s, const double2* _
         }
         __syncthreads();
         // ---- MFMA over this chunk's latitudes for the wavefront's row tiles;
-        // the next k-step's P/dP fragments are in flight while this one runs
-        {
-            int il_[TA_ITEMS], par_[TA_ITEMS];
-            bool row_[TA_ITEMS];
+        // the fragments of the next PF k-steps are in flight while one runs
+        // (a ring of PF + 1 register sets, indices fixed by the full unroll)
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            if (ks + PF < KS) frag(j0, ks + PF, aP[(ks + PF) % (PF + 1)], aD[(ks + PF) % (PF + 1)]);
+            const int cur = ks % (PF + 1);
+            const int jl = ks * 4 + (lane >> 4);
 #pragma unroll
             for (int it = 0; it < TA_ITEMS; it++) {
-                const int item = wave + 4 * it;
-                const int p = item < nte ? 0 : 1, rt = item < nte ? item : item - nte;
-                const int i = rt * 16 + (lane & 15);
-                par_[it] = p;
-                il_[it] = 2 * i + p;
-                row_[it] = item < nitems && i < (p ? no : ne);
-            }
-            auto frag = [&](int ks, double (&aP)[TA_ITEMS], double (&aD)[TA_ITEMS]) {
-                const int j = j0 + ks * 4 + (lane >> 4);
+                if (wave + 4 * it >= nitems) break;
+                const double* Bp = &Bs[par_[it]][jl][0];
 #pragma unroll
-                for (int it = 0; it < TA_ITEMS; it++) {
-                    const bool ok = row_[it] && j < nh;
-                    aP[it] = ok ? P[(size_t)j * nl + il_[it]] : 0.0;
-                    aD[it] = ok ? D[(size_t)j * nl + il_[it]] : 0.0;
-                }
-            };
-            double aP[TA_ITEMS], aD[TA_ITEMS], nP[TA_ITEMS], nD[TA_ITEMS];
-            frag(0, aP, aD);
-#pragma unroll 1
-            for (int ks = 0; ks < TA_JC / 4; ks++) {
-                if (ks + 1 < TA_JC / 4) frag(ks + 1, nP, nD);
-                const int jl = ks * 4 + (lane >> 4);
-#pragma unroll
-                for (int it = 0; it < TA_ITEMS; it++) {
-                    if (wave + 4 * it >= nitems) break;
-                    const double* Bp = &Bs[par_[it]][jl][0];
-#pragma unroll
-                    for (int t = 0; t < NT; t++)
-                        acc[it][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(16 * t < PC ? aP[it] : aD[it],
-                                                                          Bp[16 * t + (lane & 15)], acc[it][t], 0, 0, 0);
-                }
-#pragma unroll
-                for (int it = 0; it < TA_ITEMS; it++) { aP[it] = nP[it]; aD[it] = nD[it]; }
+                for (int t = 0; t < NT; t++)
+                    acc[it][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(16 * t < PC ? aP[cur][it] : aD[cur][it],
+                                                                      Bp[16 * t + (lane & 15)], acc[it][t], 0, 0, 0);
             }
         }
     }

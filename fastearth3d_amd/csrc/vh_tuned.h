@@ -15,3 +15,9 @@
 #ifndef VH_TCHUNK
 #define VH_TCHUNK 72        // tensor batches per synthesis / row / analysis launch
 #endif
+#ifndef VH_TA_PF
+#define VH_TA_PF 1          // k-steps of table fragments the tensor analysis keeps in flight
+#endif
+#ifndef VH_TA_JC
+#define VH_TA_JC 48         // latitudes per LDS chunk of the tensor analysis
+#endif
