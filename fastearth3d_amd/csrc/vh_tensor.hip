@@ -84,18 +84,31 @@ __global__ void __launch_bounds__(256) k_tsh_synth(vh_sht s, const double2* __re
     if (lb >= ngrp * (L + 1)) return;
     const int b0 = (lb / (L + 1)) * TB, m = lb % (L + 1), nl = L - m + 1;
     const int nbb = min(TB, nb - b0);
-    for (int t = threadIdx.x; t < TB * NCH * nl; t += blockDim.x) {
-        const int bb = t / (NCH * nl), r = t % (NCH * nl), lam = r / nl, il = r % nl, l = m + il;
-        double2 v = make_double2(0.0, 0.0);
-        if (bb < nbb) v = X[((size_t)(b0 + bb) * NCH + lam) * nlm + l * (l + 1) / 2 + m];
-        const double J = (double)l * (double)(l + 1);
-        double2* cb = cs + (size_t)bb * NS * nl;
-        if (lam < 4) {
-            cb[lam * nl + il] = lam == 2 ? make_double2(-J * v.x, -J * v.y) : v;
-            if (lam == 3) cb[4 * nl + il] = make_double2(-J * v.x, -J * v.y);
-        } else {   // c3 -> slot 5, c4 -> slot 6 and -J c4 -> slot 7
-            cb[(lam + 1) * nl + il] = v;
-            if (lam == 5) cb[7 * nl + il] = make_double2(-J * v.x, -J * v.y);
+    // coefficients into LDS, SQ loads per thread issued before their stores
+    constexpr int SQ = 4;
+    for (int t0 = threadIdx.x; t0 < TB * NCH * nl; t0 += SQ * blockDim.x) {
+        double2 v[SQ];
+#pragma unroll
+        for (int q = 0; q < SQ; q++) {
+            const int t = t0 + q * blockDim.x;
+            const int bb = t / (NCH * nl), r = t % (NCH * nl), lam = r / nl, il = r % nl, l = m + il;
+            v[q] = make_double2(0.0, 0.0);
+            if (t < TB * NCH * nl && bb < nbb) v[q] = X[((size_t)(b0 + bb) * NCH + lam) * nlm + l * (l + 1) / 2 + m];
+        }
+#pragma unroll
+        for (int q = 0; q < SQ; q++) {
+            const int t = t0 + q * blockDim.x;
+            if (t >= TB * NCH * nl) break;
+            const int bb = t / (NCH * nl), r = t % (NCH * nl), lam = r / nl, il = r % nl, l = m + il;
+            const double J = (double)l * (double)(l + 1);
+            double2* cb = cs + (size_t)bb * NS * nl;
+            if (lam < 4) {
+                cb[lam * nl + il] = lam == 2 ? make_double2(-J * v[q].x, -J * v[q].y) : v[q];
+                if (lam == 3) cb[4 * nl + il] = make_double2(-J * v[q].x, -J * v[q].y);
+            } else {   // c3 -> slot 5, c4 -> slot 6 and -J c4 -> slot 7
+                cb[(lam + 1) * nl + il] = v[q];
+                if (lam == 5) cb[7 * nl + il] = make_double2(-J * v[q].x, -J * v[q].y);
+            }
         }
     }
     __syncthreads();
@@ -457,11 +470,23 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, d
     const double2* I1 = Fin + ((size_t)b * 6 + d1) * (L + 1) * nlat + j;
     double2* F0 = Fd + ((size_t)b * 6 + d0) * (L + 1) * nlat + j;
     double2* F1 = Fd + ((size_t)b * 6 + d1) * (L + 1) * nlat + j;
-    if (mode == 0 || mode == 3) {
+    // the rate row's loads are issued first and land in LDS once the
+    // spectra's loads are in flight, so the two latencies overlap
+    // (two per thread: rows up to 384 points; the rest of a longer row
+    // goes straight to LDS)
+    constexpr int RQ = 2;
+    double rv[RQ];
+    const bool prod = mode == 0 || mode == 3;
+    if (prod) {
         const double* rr = rate + ((size_t)(b / bpe) * nlat + j) * N;
-        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < RQ; q++) {
+            const int i = threadIdx.x + q * blockDim.x;
+            rv[q] = i < N ? rr[i] : 0.0;
+        }
+        for (int i = threadIdx.x + RQ * blockDim.x; i < N; i += blockDim.x) {
             const double M = rr[i] * dt;
-            Mrow[i] = mode == 3 ? M / (1.0 + 0.5 * M) : M;   // trapezoidal weight
+            Mrow[i] = mode == 3 ? M / (1.0 + 0.5 * M) : M;
         }
     }
     double2* z;
@@ -475,6 +500,16 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, d
                 const int m = m0 + lane + 64 * u;
                 A[u] = m <= L ? I0[(size_t)m * nlat] : make_double2(0.0, 0.0);
                 B[u] = m <= L ? I1[(size_t)m * nlat] : make_double2(0.0, 0.0);
+            }
+            if (prod && m0 == 0) {   // the rate values arrived while the spectra are in flight
+#pragma unroll
+                for (int q = 0; q < RQ; q++) {
+                    const int i = threadIdx.x + q * blockDim.x;
+                    if (i < N) {
+                        const double M = rv[q] * dt;
+                        Mrow[i] = mode == 3 ? M / (1.0 + 0.5 * M) : M;   // trapezoidal weight
+                    }
+                }
             }
 #pragma unroll
             for (int u = 0; u < VH_FFT_U; u++) {
