@@ -20,6 +20,8 @@ The knobs (defaults in csrc/vh_tuned.h):
   VH_TCHUNK        tensor batches per synthesis / row / analysis launch
   VH_TA_PF         k-steps of table fragments the tensor analysis keeps in flight
   VH_TA_JC         latitudes per LDS chunk of the tensor analysis
+  VH_DRIFT_MINW    wavefronts per SIMD the drift's register budget must allow
+  VH_DRIFT_AK      k-steps of the G_l fragment the drift's wavefront loads at a time
 The sea-level solve's launch shape is a run-time knob (VH_T_SLE_BLOCKS),
 tuned per ensemble by vh_ensemble_tune.
 """
@@ -44,6 +46,8 @@ KNOBS = {
     "VH_TCHUNK": ("vh_api.hip", [24, 36, 48, 72], ["k_tsh_synth", "k_t3_rows", "k_tsh_analysis"]),
     "VH_TA_PF": ("vh_tensor.hip", [1, 2, 3, 4], ["k_tsh_analysis"]),
     "VH_TA_JC": ("vh_tensor.hip", [32, 48, 64, 96], ["k_tsh_analysis"]),
+    "VH_DRIFT_MINW": ("vh_drift.hip", [1, 4], ["k_drift"]),
+    "VH_DRIFT_AK": ("vh_drift.hip", [8, 16, 32], ["k_drift"]),
 }
 
 

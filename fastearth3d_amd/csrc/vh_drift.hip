@@ -173,7 +173,7 @@ __device__ __forceinline__ void drift_store(const drift_args& a, int l, int k, i
 // the G_l fragment read once per k-step from L2. Otherwise four rows per
 // thread with scalar loads of G.
 template <int RW>
-__global__ void __launch_bounds__(DRIFT_THREADS) k_drift(drift_args a) {
+__global__ void __launch_bounds__(DRIFT_THREADS, VH_DRIFT_MINW) k_drift(drift_args a) {
     extern __shared__ double F[];   // [ncol][rw]
     const int2 job = a.jobs[blockIdx.x];
     const int l = job.x, m0 = job.y;
@@ -194,22 +194,29 @@ __global__ void __launch_bounds__(DRIFT_THREADS) k_drift(drift_args a) {
             for (int ct = 0; ct < CT; ct++) acc[ct] = vh_d4{0.0, 0.0, 0.0, 0.0};
             const int arow = rt * 16 + (lane & 15);
             const double* Grow = G + (size_t)min(arow, nrow - 1) * ncol;
-            // the wavefront's whole G_l fragment in flight first (ncol <= 128 with RW = 64)
-            double av[32];
+            // the wavefront's G_l fragment in flight VH_DRIFT_AK k-steps at a
+            // time (ncol <= 128 with RW = 64: 32 k-steps in all)
+            constexpr int AK = VH_DRIFT_AK;
 #pragma unroll
-            for (int ks = 0; ks < 32; ks++) {
-                const int kk = 4 * ks + (lane >> 4);
-                av[ks] = (arow < nrow && kk < ncol) ? Grow[kk] : 0.0;
-            }
+            for (int k0 = 0; k0 < 32; k0 += AK) {
+                if (4 * k0 >= ncol) break;
+                double av[AK];
 #pragma unroll
-            for (int ks = 0; ks < 32; ks++) {
-                if (4 * ks >= ncol) break;
-                const int kk = 4 * ks + (lane >> 4);
-                const bool kok = kk < ncol;
+                for (int q = 0; q < AK; q++) {
+                    const int kk = 4 * (k0 + q) + (lane >> 4);
+                    av[q] = (arow < nrow && kk < ncol) ? Grow[kk] : 0.0;
+                }
 #pragma unroll
-                for (int ct = 0; ct < CT; ct++) {
-                    const double bv = kok ? F[kk * RW + ct * 16 + (lane & 15)] : 0.0;
-                    acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], bv, acc[ct], 0, 0, 0);
+                for (int q = 0; q < AK; q++) {
+                    const int ks = k0 + q;
+                    if (4 * ks >= ncol) break;
+                    const int kk = 4 * ks + (lane >> 4);
+                    const bool kok = kk < ncol;
+#pragma unroll
+                    for (int ct = 0; ct < CT; ct++) {
+                        const double bv = kok ? F[kk * RW + ct * 16 + (lane & 15)] : 0.0;
+                        acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv, acc[ct], 0, 0, 0);
+                    }
                 }
             }
 #pragma unroll

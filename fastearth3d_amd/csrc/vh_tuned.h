@@ -21,3 +21,9 @@
 #ifndef VH_TA_JC
 #define VH_TA_JC 48         // latitudes per LDS chunk of the tensor analysis
 #endif
+#ifndef VH_DRIFT_AK
+#define VH_DRIFT_AK 8       // k-steps of the G_l fragment the drift's wavefront loads at a time
+#endif
+#ifndef VH_DRIFT_MINW
+#define VH_DRIFT_MINW 1     // wavefronts per SIMD the drift's register budget must allow (4: two blocks per CU)
+#endif

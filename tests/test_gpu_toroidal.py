@@ -15,7 +15,8 @@ src/vilma_response.f90:694-809 tor branch, :1055-1109 enabling):
 * whole coupling calls of the lateral-viscosity workload at lmax 16 with the
   channels carried, forward Euler and the trapezoidal scheme: ocean function
   bit for bit, rsl, sigma, the memory and t_lm within 1e-10, equal step
-  counts;
+  counts; twenty calls at lmax 12 within 1e-8 with equal sub-step and
+  sea-level iteration counts;
 * a restart of the six-channel memory continuing bit for bit, and the
   reverse sweep refusing the toroidal channels.
 """
@@ -210,4 +211,22 @@ def test_toroidal_restart_continuation():
     with pytest.raises(lib.VilmaError, match="toroidal"):
         b.forward_record([c["ice"](63)], c["dt_yr"])
     for m in (a, b):
+        m.finalize()
+
+
+def test_toroidal_long_run_matches_oracle(oracle):
+    """Twenty coupling calls with the channels carried (lmax 12): rsl, sigma,
+    t_lm and the memory within 1e-8, the mask bit for bit and the same
+    sub-step counts after every call."""
+    c, mo, mg = _run_pair(oracle, b"fe", lmax=12)
+    for k in range(20):
+        for m in (mo, mg):
+            m.update(c["ice"](k + 60), c["dt_yr"])
+        assert np.array_equal(mg.C, mo.C), k
+        assert rel(mg.rsl, mo.rsl) <= 1e-8, k
+        assert rel(mg.sigma_lm, mo.sigma_lm) <= 1e-8, k
+        assert rel(mg.spectrum(_abi.SP_TOR), mo.spectrum(_abi.SP_TOR)) <= 1e-8, k
+        assert tuple(mg.scalar(i) for i in (3, 4, 5, 6)) == tuple(mo.scalar(i) for i in (3, 4, 5, 6)), k
+    assert rel(mg.memory(), mo.memory()) <= 1e-8
+    for m in (mo, mg):
         m.finalize()
