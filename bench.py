@@ -167,7 +167,7 @@ def ensemble_models(n, calls, mem=None):
     return mem, out
 
 
-def _ens_models(lib, ws, sle_blocks=0):
+def _ens_models(lib, ws, sle_blocks=0, graph=0):
     from fastearth3d_amd import _abi
     ms = []
     for c, _ in ws:
@@ -175,11 +175,12 @@ def _ens_models(lib, ws, sle_blocks=0):
         if c["pert"] is not None:
             m.enable_visc_3d(c["pert"])
         m.set_tuning(_abi.T_SLE_BLOCKS, sle_blocks)
+        m.set_tuning(_abi.T_GRAPH, graph)
         ms.append(m)
     return ms
 
 
-def ensemble_measure(L, ws, warmup, calls, workers, tune_calls=2):
+def ensemble_measure(L, ws, warmup, calls, workers, tune_calls=2, graph=0):
     """The same calls from the same state two ways: every member stepped
     alone, one after another (vh_update_device, one block per work item in
     the sea-level solve), and all members through vh_ensemble_run_device
@@ -187,11 +188,11 @@ def ensemble_measure(L, ws, warmup, calls, workers, tune_calls=2):
     has picked the solve's launch shape. Warm-up and tuning calls are run
     untimed by both. Returns (sequential s, ensemble s, sub-steps timed,
     per-member timed seconds of the sequential pass, same results, chosen
-    VH_T_SLE_BLOCKS)."""
+    VH_T_SLE_BLOCKS). graph: both ways with the sub-steps replayed as hipGraphs."""
     import torch
     from fastearth3d_amd import lib
     pre = warmup + 3 * tune_calls
-    seq = _ens_models(lib, ws)
+    seq = _ens_models(lib, ws, graph=graph)
     for m, (c, h) in zip(seq, ws):
         for k in range(pre):
             m.update_device(h[k].data_ptr(), c["dt_yr"])
@@ -210,7 +211,7 @@ def ensemble_measure(L, ws, warmup, calls, workers, tune_calls=2):
     ref = [(m.rsl, m.scalar(3)) for m in seq]
     for m in seq:
         m.finalize()
-    par = _ens_models(lib, ws)
+    par = _ens_models(lib, ws, graph=graph)
     ens = lib.Ensemble(par, n_workers=workers)
     dts = [c["dt_yr"] for c, _ in ws]
     if warmup:
@@ -239,7 +240,8 @@ def ensemble_bench(a, L, emit=True):
     the sequential rate beside it."""
     n = a.members
     mem, ws = ensemble_models(n, a.warmup + 3 * a.tune_calls + a.steps)
-    t_seq, t_ens, nsub, per, same, chosen = ensemble_measure(L, ws, a.warmup, a.steps, a.ens_workers, a.tune_calls)
+    t_seq, t_ens, nsub, per, same, chosen = ensemble_measure(L, ws, a.warmup, a.steps, a.ens_workers, a.tune_calls,
+                                                             a.graph)
     calls = n * a.steps
     kyr = sum(c["dt_yr"] for c, _ in ws) * a.steps / 1000.0
     crit = max(per)
@@ -252,7 +254,7 @@ def ensemble_bench(a, L, emit=True):
                                "own earths, topography and ice, SLE + rotation, dt 100 yr" % (n, sum(x[2] for x in mem)),
                    "members": n, "calls_per_member_timed": a.steps, "sub_steps_timed": nsub,
                    "workers": a.ens_workers, "tuned_sle_blocks": chosen,
-                   "tuning_calls_untimed": 3 * a.tune_calls},
+                   "tuning_calls_untimed": 3 * a.tune_calls, "graph_replay": bool(a.graph)},
         "sequential_ens_per_s": round(calls / 200.0 / t_seq, 4),
         "speedup_vs_sequential": round(t_seq / t_ens, 3),
         # no schedule of independent members can beat the slowest member run alone
@@ -557,6 +559,8 @@ def main():
     ap.add_argument("--lmax", type=int, default=128)
     ap.add_argument("--cpu-calls", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", type=int, default=0, choices=(0, 1),
+                    help="1: replay each forward-Euler sub-step as a captured hipGraph (VH_T_GRAPH)")
     ap.add_argument("--members", type=int, default=64, help="ensemble size (--config 4)")
     ap.add_argument("--ens-workers", type=int, default=16, help="host threads of the ensemble (--config 4)")
     ap.add_argument("--tune-calls", type=int, default=2, help="calls per autotuner candidate (--config 4)")
@@ -585,7 +589,7 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calib_bytes_read": n * 8, "copy_calib_bytes_written": n * 8, "launches": 5}))
         return
-    from fastearth3d_amd import configs, lib
+    from fastearth3d_amd import _abi, configs, lib
     L = lib.load()
     L.vh_update_device.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
     L.vh_profile.argtypes = [C.c_void_p, C.c_int]
@@ -602,6 +606,7 @@ def main():
     se = lib.SolidEarth(p, c["z_bed_eq"], c["h_ice_eq"], c["h_ice_init"])
     if c["pert"] is not None:
         se.enable_visc_3d(c["pert"])
+    se.set_tuning(_abi.T_GRAPH, a.graph)
     # the timed calls start --offset calls into the history (run untimed
     # first), where the ice load and the sea-level iteration counts are those
     # of the bulk of the run rather than of its first, nearly ice-free calls
@@ -699,7 +704,8 @@ def main():
         "config": {"workload": names[a.config], "lmax": a.lmax, "nlat": se.nlat, "nphi": se.nphi,
                    "radial_elements": d["ne"], "maxwell_elements": d["nem"], "dt_yr": c["dt_yr"],
                    "sub_steps_timed": int(nsub), "ne3d": int(se.dims.get("ne3d", 0)),
-                   "timed_calls": [off + a.warmup, nsteps - 1], "sle_inner_per_sub_step": round(n_inner / max(nsub, 1), 2)},
+                   "timed_calls": [off + a.warmup, nsteps - 1], "sle_inner_per_sub_step": round(n_inner / max(nsub, 1), 2),
+                   "graph_replay": bool(a.graph)},
         "roofline": roof, "kernel_roofline": per_kernel, "kernel_ms": kernel_ms, "cpu_baseline": cpu,
     }
     if cpu:

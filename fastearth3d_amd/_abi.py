@@ -50,10 +50,11 @@ class VhParams(C.Structure):
 # selector constants (enums of include/vilma_hip.h)
 F_RSL, F_ZBED, F_C, F_C0, F_SROT, F_HICE = range(6)
 (S_TIME, S_BSL, S_WORST_MASS_RESID, S_N_ACCEPT, S_N_REJECT, S_N_SOLVE,
- S_N_INNER_TOT, S_N_OUTER_TOT, S_POLAR_M_RE, S_POLAR_M_IM, S_N_SLE_STEAL, S_N_FLOOR, S_N_COUPLE_TOT) = range(13)
+ S_N_INNER_TOT, S_N_OUTER_TOT, S_POLAR_M_RE, S_POLAR_M_IM, S_N_SLE_STEAL, S_N_FLOOR, S_N_COUPLE_TOT,
+ S_N_GRAPH_LAUNCH, S_N_GRAPH_CAPTURE) = range(15)
 SP_SIGMA, SP_U, SP_N, SP_TOR = range(4)
 D_LMAX, D_NLAT, D_NPHI, D_NLM, D_NR, D_NE, D_NK, D_NE3D, D_NLAM, D_NEM, D_NMN = range(11)
-T_SLE_BLOCKS = 0
+T_SLE_BLOCKS, T_GRAPH = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

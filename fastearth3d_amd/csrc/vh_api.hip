@@ -221,6 +221,18 @@ struct vh_model {
     long n_sle_steal = 0;    // sea-level work items run for a block that was not resident
     int sle_late_blocks = 0;     // diagnostic (VH_SLE_LATE_BLOCKS): solve blocks that start late
     int sle_blocks = 0;          // blocks of the fused sea-level solve (0: one per work item)
+    // hipGraph replay of the forward-Euler sub-step (VH_T_GRAPH): one
+    // executable graph per (memory buffer, interpolation fraction, step,
+    // rotation field), captured on first use; dropped whenever a buffer or
+    // launch shape it was captured with changes
+    bool use_graph = false;
+    struct sub_graph { int cur; double frac, dt; const double* srot; hipGraphExec_t exec; };
+    std::vector<sub_graph> graphs;
+    long n_graph_launch = 0, n_graph_capture = 0;
+    void drop_graphs() {
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
+    }
     unsigned sle_late_ticks = 0;
     // trapezoidal memory scheme (scheme "trap" with earth_response "ve"):
     // controller knobs and counters, the start-of-step load sigma_n, the
@@ -305,6 +317,7 @@ struct vh_model {
 };
 
 vh_model::~vh_model() {
+    drop_graphs();
     if (adj) adj_free(adj);
     if (dplan) vh_drift_plan_destroy(dplan);
     if (tplan) vh_tor_plan_destroy(tplan);
@@ -708,18 +721,21 @@ void adj_stash_push(vh_model* h, double dt, double frac, const sle_res& r);
 
 // sle_solve (src/vilma_sle.f90:124-405), forward-Euler response branch.
 // The advance writes tau_{n+1} into the spare buffer; the caller commits it.
-sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double frac, int interp, bool report_only,
-                  const double* srot) {
+// The stream work of one solve: ice, drift, the fused solve and (unless
+// report_only) the memory advance. prepare: set the kernels' launch
+// attributes only, enqueue nothing.
+void sle_enqueue(vh_model* h, const double* ice0, const double* ice1, double frac, int interp, bool report_only,
+                 const double* srot, bool prepare = false) {
     hipStream_t st = h->st;
     const vh_sht& s = h->s;
     const int n = (int)h->ng();
-    sle_res res;
     // sc[16..28): the solve's results (zeroed with wcorr by the ice kernel),
     // the new memory norm at [11], all read back once at the end of the solve
     double* r = h->d_sc + 16;
-    sle_ice(st, n, ice0, ice1, frac, interp, h->d_heq, h->d_topo0, h->d_ice, h->d_dice, h->d_C0s, h->d_wcorr, r, 12,
-            h->d_sbar, sle_solve_grid(s), h->n_outer);
-    if (h->kind == 2) {
+    if (!prepare)
+        sle_ice(st, n, ice0, ice1, frac, interp, h->d_heq, h->d_topo0, h->d_ice, h->d_dice, h->d_C0s, h->d_wcorr, r,
+                12, h->d_sbar, sle_solve_grid(s), h->n_outer);
+    if (h->kind == 2 && !prepare) {
         h->prof_begin(VH_K_DRIFT);
         vh_drift_run(st, h->dplan, h->d, h->v, h->d_tau[h->cur], h->d_mnorm[h->cur], h->d_maxn + h->cur, h->skip_tol,
                      h->deg1_cm, h->d_dN, h->d_dsurf);
@@ -736,12 +752,58 @@ sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double fr
     ia.part = h->d_part; ia.cs = h->d_part + 3 * (size_t)s.nlat; ia.part3 = h->d_part + 4 * (size_t)s.nlat;
     ia.res = r; ia.F = h->d_F; ia.FC = h->d_FC; ia.F2 = h->d_F2; ia.sigma = h->d_sigma; ia.bar = h->d_sbar;
     ia.late_blocks = h->sle_late_blocks; ia.late_ticks = h->sle_late_ticks; ia.blocks = h->sle_blocks;
+    if (prepare) {
+        sle_solve_launch(st, ia, false);
+        return;
+    }
     h->prof_begin(VH_K_SLE);
     sle_solve_launch(st, ia);
     h->prof_end();
     // sigma is the closing load transform; then the memory advance
     int nxt = h->cur ^ 1;
     if (!report_only && h->kind == 2) advance_memory(h, (const double*)h->d_sigma, nxt, r + 11);
+}
+
+// sle_enqueue as one graph launch: the executable graph of this (buffer,
+// fraction, step, rotation field), captured from sle_enqueue on first use
+void sle_enqueue_graph(vh_model* h, double frac, const double* srot) {
+    hipGraphExec_t ex = nullptr;
+    for (const auto& g : h->graphs)
+        if (g.cur == h->cur && g.frac == frac && g.dt == h->dt && g.srot == srot) { ex = g.exec; break; }
+    if (!ex) {
+        if (h->graphs.size() >= 256) h->drop_graphs();
+        sle_enqueue(h, h->d_hice, h->d_ice1, frac, 1, false, srot, true);
+        HIPCHECK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+        hipGraph_t g = nullptr;
+        try {
+            sle_enqueue(h, h->d_hice, h->d_ice1, frac, 1, false, srot);
+        } catch (...) {
+            (void)hipStreamEndCapture(h->st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCHECK(hipStreamEndCapture(h->st, &g));
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHECK(e);
+        h->graphs.push_back({h->cur, frac, h->dt, srot, ex});
+        h->n_graph_capture++;
+    }
+    HIPCHECK(hipGraphLaunch(ex, h->st));
+    h->n_graph_launch++;
+}
+
+sle_res sle_solve(vh_model* h, const double* ice0, const double* ice1, double frac, int interp, bool report_only,
+                  const double* srot) {
+    sle_res res;
+    double* r = h->d_sc + 16;
+    // graph replay for the stepper's sub-steps (not while timing kernels with
+    // events or recording for the reverse sweep, whose store moves per sub-step)
+    const bool graph = h->use_graph && !report_only && interp == 1 && ice0 == h->d_hice && ice1 == h->d_ice1 &&
+                       !h->prof && !(h->adj && adj_recording(h->adj));
+    if (graph) sle_enqueue_graph(h, frac, srot);
+    else sle_enqueue(h, ice0, ice1, frac, interp, report_only, srot);
+    hipStream_t st = h->st;
     if (!report_only) { h->time_resp += h->dt; h->n_couple_tot++; }   // forward Euler: one pass
     // r[10] = int C rsl (from the solve) for the mass diagnostic, r[11] the new memory norm
     if (report_only && h->kind == 2)
@@ -1732,6 +1794,8 @@ int vh_get_scalar(vh_model* h, int which, double* out) {
             case VH_S_N_SLE_STEAL: *out = (double)h->n_sle_steal; break;
             case VH_S_N_FLOOR: *out = (double)h->n_floor; break;
             case VH_S_N_COUPLE_TOT: *out = (double)h->n_couple_tot; break;
+            case VH_S_N_GRAPH_LAUNCH: *out = (double)h->n_graph_launch; break;
+            case VH_S_N_GRAPH_CAPTURE: *out = (double)h->n_graph_capture; break;
             case VH_S_POLAR_M_RE:
             case VH_S_POLAR_M_IM:
                 if (!h->rot_on) { *out = 0.0; break; }
@@ -2095,6 +2159,7 @@ void enable_toroidal(vh_model* h) {
     const vh_dims& d = h->d;
     hipStream_t st = h->st;
     h->sync();   // the reduction scratch is replaced below
+    h->drop_graphs();
     h->tplan = vh_tor_plan_create(st, d, h->r, h->mu, h->emap, h->mnodes, h->mnode_of);
     const size_t nt = (size_t)12 * std::max(d.nem, 1) * d.nk, ndw = (size_t)2 * std::max(d.nmn, 1) * d.nk;
     for (int b = 0; b < 2; b++) { h->d_taut[b] = dalloc<double>(nt); vh_fill(st, h->d_taut[b], 0.0, nt); }
@@ -2149,6 +2214,8 @@ void enable_lateral(vh_model* h, const double* pert) {
         e3em[i] = em; e3of[em] = (int)i; is3d[em] = 1;
     }
     hipStream_t st = h->st;
+    h->sync();
+    h->drop_graphs();   // the tensor buffers are replaced below
     for (void* p : {(void*)h->d_e3em, (void*)h->d_e3of, (void*)h->d_rate3, (void*)h->d_X, (void*)h->d_Fd,
                     (void*)h->d_cdel})
         if (p) HIPCHECK(hipFree(p));
@@ -2451,6 +2518,12 @@ int vh_set_tuning(vh_model* h, int which, int value) {
             case VH_T_SLE_BLOCKS:
                 if (value < 0) throw vh_error("vh_set_tuning: VH_T_SLE_BLOCKS must be >= 0");
                 h->sle_blocks = value;
+                h->drop_graphs();
+                break;
+            case VH_T_GRAPH:
+                if (value != 0 && value != 1) throw vh_error("vh_set_tuning: VH_T_GRAPH must be 0 or 1");
+                h->use_graph = value == 1;
+                if (!h->use_graph) h->drop_graphs();
                 break;
             default: throw vh_error("vh_set_tuning: unknown knob");
         }
@@ -2461,6 +2534,7 @@ int vh_get_tuning(vh_model* h, int which, int* value) {
     return guarded(h, [&] {
         switch (which) {
             case VH_T_SLE_BLOCKS: *value = h->sle_blocks; break;
+            case VH_T_GRAPH: *value = h->use_graph ? 1 : 0; break;
             default: throw vh_error("vh_get_tuning: unknown knob");
         }
     });

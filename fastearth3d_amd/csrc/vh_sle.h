@@ -65,7 +65,9 @@ struct sle_solve_args {
     // other members of an ensemble)
     int blocks;
 };
-void sle_solve_launch(hipStream_t st, const sle_solve_args& a);
+// launch = false: only the kernel's LDS attribute is set (before a stream
+// capture, where a function attribute cannot be changed)
+void sle_solve_launch(hipStream_t st, const sle_solve_args& a, bool launch = true);
 int sle_solve_grid(const vh_sht& s);   // work items of the fused solve
 void grid_sub(hipStream_t st, int n, const double* a, const double* b, double* out);
 void grid_absdiff(hipStream_t st, int n, const double* a, const double* b, double* out);   // |a - b|

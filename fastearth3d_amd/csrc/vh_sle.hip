@@ -1182,13 +1182,14 @@ static void sle_lds_limit(size_t lds) {
     }
 }
 template <bool TabLds, int MinW, bool Multi = false>
-static void sle_launch_one(hipStream_t st, int grid, size_t lds, const sle_solve_args& a) {
+static void sle_launch_one(hipStream_t st, int grid, size_t lds, const sle_solve_args& a, bool launch) {
     sle_lds_limit<TabLds, MinW, Multi>(lds);
+    if (!launch) return;
     k_sle_solve<TabLds, MinW, Multi><<<grid, 256, lds, st>>>(a);
     HIPCHECK(hipGetLastError());
 }
 
-void sle_solve_launch(hipStream_t st, const sle_solve_args& a) {
+void sle_solve_launch(hipStream_t st, const sle_solve_args& a, bool launch) {
     const vh_sht& s = a.s;
     if (s.nh > 256) throw vh_error("sle: the fused sea-level loop supports nlat <= 512");
     const int items = sle_solve_grid(s);   // latitude-pair items, then order items
@@ -1207,15 +1208,15 @@ void sle_solve_launch(hipStream_t st, const sle_solve_args& a) {
     const bool two = grid > ncu;   // one block per CU does not hold the grid
     if (grid < items) {            // blocks owning several items: at most one per CU asked for
         if (two) throw vh_error("sle: VH_T_SLE_BLOCKS above the CU count");
-        if (tab) sle_launch_one<true, 1, true>(st, grid, lds, a);
-        else sle_launch_one<false, 1, true>(st, grid, lds, a);
+        if (tab) sle_launch_one<true, 1, true>(st, grid, lds, a, launch);
+        else sle_launch_one<false, 1, true>(st, grid, lds, a, launch);
         return;
     }
     if (tab) {
-        if (two) sle_launch_one<true, 2>(st, grid, lds, a);
-        else sle_launch_one<true, 1>(st, grid, lds, a);
+        if (two) sle_launch_one<true, 2>(st, grid, lds, a, launch);
+        else sle_launch_one<true, 1>(st, grid, lds, a, launch);
     } else {
-        if (two) sle_launch_one<false, 2>(st, grid, lds, a);
-        else sle_launch_one<false, 1>(st, grid, lds, a);
+        if (two) sle_launch_one<false, 2>(st, grid, lds, a, launch);
+        else sle_launch_one<false, 1>(st, grid, lds, a, launch);
     }
 }

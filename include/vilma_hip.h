@@ -90,7 +90,9 @@ enum {
     VH_S_N_SLE_STEAL = 10,  /* sea-level work items a block ran for another that
                                was not resident (a device shared with other work) */
     VH_S_N_FLOOR = 11,      /* trap: steps accepted at dt_min over tolerance     */
-    VH_S_N_COUPLE_TOT = 12  /* sea-level <-> memory passes over all solves       */
+    VH_S_N_COUPLE_TOT = 12, /* sea-level <-> memory passes over all solves       */
+    VH_S_N_GRAPH_LAUNCH = 13, /* sub-steps replayed as a hipGraph (VH_T_GRAPH)   */
+    VH_S_N_GRAPH_CAPTURE = 14 /* sub-step graphs captured                        */
 };
 enum {
     VH_SP_SIGMA = 0,   /* se%sigma_lm: converged surface load [kg m^-2]      */
@@ -241,8 +243,12 @@ const char* vh_last_error(const vh_model* h);
  * orders, lmax + 1 + nlat / 2), else that many blocks (at most the device's
  * CU count; a solve with more items than that is refused) owning the items
  * round-robin: fewer blocks solve one model more slowly but leave the device
- * to other work, e.g. the other members of an ensemble. */
-enum { VH_T_SLE_BLOCKS = 0 };
+ * to other work, e.g. the other members of an ensemble. VH_T_GRAPH: 1 replays
+ * each forward-Euler sub-step (ice interpolation, drift, sea-level solve,
+ * memory advance) as one captured hipGraph, one graph per (memory buffer,
+ * interpolation fraction, step), captured on first use; 0 (the default)
+ * launches the kernels one by one. */
+enum { VH_T_SLE_BLOCKS = 0, VH_T_GRAPH = 1 };
 int vh_set_tuning(vh_model* h, int which, int value);
 int vh_get_tuning(vh_model* h, int which, int* value);
 
