@@ -259,6 +259,9 @@ def ensemble_bench(a, L, emit=True):
         "speedup_vs_sequential": round(t_seq / t_ens, 3),
         # no schedule of independent members can beat the slowest member run alone
         "bound_by_slowest_member": round(t_seq / crit, 3),
+        # share of the sequential time spent in the laterally heterogeneous
+        # members, whose tensor kernels fill the device on their own
+        "sequential_share_3d": round(sum(t for t, (c, _) in zip(per, ws) if c["pert"] is not None) / sum(per), 4),
         "ensemble_equals_sequential": same,
         "member_kyr_per_s": round(kyr / t_ens, 4), "member_calls_per_s": round(calls / t_ens, 2),
     }
@@ -726,6 +729,7 @@ def main():
                              "sub_steps_timed": e["config"]["sub_steps_timed"],
                              "tuned_sle_blocks": e["config"]["tuned_sle_blocks"],
                              "bound_by_slowest_member": e["bound_by_slowest_member"],
+                             "sequential_share_3d": e["sequential_share_3d"],
                              "ensemble_equals_sequential": e["ensemble_equals_sequential"],
                              "basis": "ens/s = members x timed calls / 200 / wall s (completed 200-call members per "
                                       "second at the measured rate)"}

@@ -22,6 +22,7 @@ The knobs (defaults in csrc/vh_tuned.h):
   VH_TA_JC         latitudes per LDS chunk of the tensor analysis
   VH_DRIFT_MINW    wavefronts per SIMD the drift's register budget must allow
   VH_DRIFT_AK      k-steps of the G_l fragment the drift's wavefront loads at a time
+  VH_DRIFT_ONEPASS 1: the drift builds its forcing in one pass with LDS atomics
 The sea-level solve's launch shape is a run-time knob (VH_T_SLE_BLOCKS),
 tuned per ensemble by vh_ensemble_tune.
 """
@@ -48,6 +49,7 @@ KNOBS = {
     "VH_TA_JC": ("vh_tensor.hip", [32, 48, 64, 96], ["k_tsh_analysis"]),
     "VH_DRIFT_MINW": ("vh_drift.hip", [1, 4], ["k_drift"]),
     "VH_DRIFT_AK": ("vh_drift.hip", [8, 16, 32], ["k_drift"]),
+    "VH_DRIFT_ONEPASS": ("vh_drift.hip", [0, 1], ["k_drift"]),
 }
 
 

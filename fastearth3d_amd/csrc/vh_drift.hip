@@ -113,13 +113,18 @@ __device__ __forceinline__ void drift_forcing(const drift_args& a, int l, int kb
     const int k = kb + ml;
     const double Jr = (double)l * (double)(l + 1);
     constexpr int U = 3;   // elements whose memory a thread has in flight at once
-    for (int par = 0; par < 2; par++) {
+    // VH_DRIFT_ONEPASS: every element in one pass, the two contributions a
+    // node receives (element below, element above) added with LDS atomics,
+    // which commute exactly for two terms on a zero start; else one pass per
+    // element parity (elements of one parity share no node)
+    constexpr int NPAR = VH_DRIFT_ONEPASS ? 1 : 2, EST = VH_DRIFT_ONEPASS ? 1 : 2;
+    for (int par = 0; par < NPAR; par++) {
         if (act)
-            for (int em0 = par + 2 * grp; em0 < a.nem; em0 += 2 * ngrp * U) {
+            for (int em0 = par + EST * grp; em0 < a.nem; em0 += EST * ngrp * U) {
                 double v[U][12];
 #pragma unroll
                 for (int u = 0; u < U; u++) {
-                    const int em = em0 + 2 * ngrp * u;
+                    const int em = em0 + EST * ngrp * u;
                     const double* t = a.tau + (size_t)min(em, a.nem - 1) * 24 * nk + k;
 #pragma unroll
                     for (int q = 0; q < 12; q++) {   // (shape, lambda) of this re/im half
@@ -129,7 +134,7 @@ __device__ __forceinline__ void drift_forcing(const drift_args& a, int l, int kb
                 }
 #pragma unroll
                 for (int u = 0; u < U; u++) {
-                    const int em = em0 + 2 * ngrp * u;
+                    const int em = em0 + EST * ngrp * u;
                     if (em >= a.nem) break;
                     const int e = a.emap[em];
                     typedef const __attribute__((address_space(4))) double cdouble;
@@ -143,10 +148,17 @@ __device__ __forceinline__ void drift_forcing(const drift_args& a, int l, int kb
                         floc[t] = f;
                     }
                     const int qlo = a.mnode[e], qup = a.mnode[e + 1];
-                    F[(2 * qlo + 0) * rw + rhs] += floc[0];
-                    F[(2 * qup + 0) * rw + rhs] += floc[1];
-                    F[(2 * qlo + 1) * rw + rhs] += floc[2];
-                    F[(2 * qup + 1) * rw + rhs] += floc[3];
+                    if constexpr (VH_DRIFT_ONEPASS) {
+                        atomicAdd(&F[(2 * qlo + 0) * rw + rhs], floc[0]);
+                        atomicAdd(&F[(2 * qup + 0) * rw + rhs], floc[1]);
+                        atomicAdd(&F[(2 * qlo + 1) * rw + rhs], floc[2]);
+                        atomicAdd(&F[(2 * qup + 1) * rw + rhs], floc[3]);
+                    } else {
+                        F[(2 * qlo + 0) * rw + rhs] += floc[0];
+                        F[(2 * qup + 0) * rw + rhs] += floc[1];
+                        F[(2 * qlo + 1) * rw + rhs] += floc[2];
+                        F[(2 * qup + 1) * rw + rhs] += floc[3];
+                    }
                 }
             }
         __syncthreads();

@@ -27,3 +27,6 @@
 #ifndef VH_DRIFT_MINW
 #define VH_DRIFT_MINW 1     // wavefronts per SIMD the drift's register budget must allow (4: two blocks per CU)
 #endif
+#ifndef VH_DRIFT_ONEPASS
+#define VH_DRIFT_ONEPASS 1  // 1: the drift builds its forcing in one pass with LDS atomics
+#endif
