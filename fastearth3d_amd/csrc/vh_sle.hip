@@ -206,7 +206,8 @@ __host__ __device__ static size_t sle_tab_doubles(const vh_sht& s) {
 }
 __host__ __device__ static size_t sle_rows_bytes(const vh_sht& s) {
     return (size_t)2 * s.nphi * sizeof(double) + (size_t)s.ntw * sizeof(double2) +
-           std::max((size_t)4 * s.nphi * sizeof(double2), (size_t)6 * 256 * sizeof(double));
+           std::max((size_t)4 * s.nphi * sizeof(double2), (size_t)6 * 256 * sizeof(double)) +
+           (s.L < 64 * 3 ? (size_t)2 * 2 * 3 * 64 * sizeof(double2) : 0);   // prefetched pair spectra (L < 192)
 }
 static size_t sle_solve_lds(const vh_sht& s, bool tab) {
     const size_t order = (tab ? sle_tab_doubles(s) * sizeof(double) : 0) +
@@ -527,12 +528,52 @@ __device__ __forceinline__ void sle_row_start(const sle_solve_args& a, int p, do
     }
 }
 
+// 16 bytes per lane from global memory (coherent, as ld_coh) straight into
+// LDS: lane k's bytes land at dst + k, dst the same for the whole wavefront.
+// No registers hold them; they are in LDS once the vector memory counter of
+// the wavefront has drained.
+__device__ __forceinline__ void lds_dma16(const double2* src, double2* dst) {
+    __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)dst, 16, 0,
+                                     16 /* sc1: coherent at agent scope */);
+}
+// A row block's pair spectra for a ROWS item, loaded over a stage decision:
+// orders m = lane + 64 u of wavefront w's row (F2 entries 2w, 2w + 1 of the
+// pair) into stg[2u][lane], stg[2u + 1][lane] (stg: 2 SLE_RPF x 64 complex
+// per wavefront), then packed as pair_spectra_to_lds packs them.
+constexpr int SLE_RPF = 3;   // lmax < 64 SLE_RPF
+__device__ __forceinline__ void row_spectra_issue(const double2* F2p, int L, int lane, double2* stg) {
+#pragma unroll
+    for (int u = 0; u < SLE_RPF; u++) {
+        if (lane + 64 * u <= L) {
+            const double2* f = F2p + (size_t)(lane + 64 * u) * 4;
+            lds_dma16(f, stg + 2 * u * 64);
+            lds_dma16(f + 1, stg + (2 * u + 1) * 64);
+        }
+    }
+}
+__device__ __forceinline__ void row_spectra_pack(const double2* stg, int L, int N, double2* a, int lane) {
+    for (int i = lane; i < N; i += 64) a[i] = make_double2(0.0, 0.0);
+    wave_lds_sync();
+#pragma unroll 1   // (unrolled, the kernel spills)
+    for (int u = 0; u < SLE_RPF; u++) {
+        const int m = lane + 64 * u;
+        if (m > L) continue;
+        double2 x = stg[2 * u * 64 + lane], y = stg[(2 * u + 1) * 64 + lane];
+        if (m == 0) { x.y = 0.0; y.y = 0.0; }
+        a[m] = make_double2(x.x - y.y, x.y + y.x);
+        if (m > 0) a[N - m] = make_double2(x.x + y.y, -x.y + y.x);
+    }
+    wave_lds_sync();
+}
+
 // ROWS: inverse FFT of (u_j, N_j) -> Sraw_j (into srow, and into sraw for a
 // later stage run elsewhere), cs_j = sum C Sraw, the base load
 // rho_i dig + rho_w C Sraw + wcorr and its FFT (the pair's rows packed) -> F
+// stg: null, or the pair's spectra already in LDS (row_spectra_issue over the
+// previous stage's decision, complete)
 template <int MinW>
 __device__ __forceinline__ void sle_row_rows(const sle_solve_args& a, int p, double* srow, double2* RB,
-                                             const double2* TW) {
+                                             const double2* TW, const double2* stg) {
     const vh_sht& s = a.s;
     const int L = s.L, N = s.nphi, nlat = s.nlat, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const double scale = 2.0 * VH_PI / N;
@@ -557,7 +598,8 @@ __device__ __forceinline__ void sle_row_rows(const sle_solve_args& a, int p, dou
                 wpre[u] = i < N ? ld_coh(a.wcorr + id) : 0.0;
             }
             const double2* F2p = a.F2 + (size_t)p * (L + 1) * 4 + 2 * wave;   // rows q = 2j, 2j + 1
-            pair_spectra_to_lds<true>(F2p, F2p + 1, L, N, fa, lane, 4);
+            if (stg) row_spectra_pack(stg + (size_t)wave * 2 * SLE_RPF * 64, L, N, fa, lane);
+            else pair_spectra_to_lds<true>(F2p, F2p + 1, L, N, fa, lane, 4);
             double2* z = sle_fft<true, MinW>(fa, fb, s, lane, TW);
             double rs = 0.0;
             for (int i0 = 0; i0 < N; i0 += 64 * VH_SLE_PU) {
@@ -835,7 +877,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     const bool order_over_rows = !TabLds || sle_tab_doubles(s) * sizeof(double) < sle_rows_bytes(s);
     // what the LDS holds: the twiddles, the Sraw of a pair (and the stage it
     // was made in), the Legendre table of an order
-    bool tw_ok = false;
+    bool tw_ok = false, rows_pf = false;   // rows_pf: the own pair's spectra are packed for its next ROWS item
     int srow_item = -1, tab_m = -1;
     unsigned srow_stage = 0xffffffffu;
 
@@ -924,6 +966,21 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
             spf[0][tid] = f0; spf[1][tid] = f1; spf[2][tid] = c0; spf[3][tid] = c1;
         }
     };
+    // the same entries copied straight into spf (complete once the vector
+    // memory counter has drained), in flight over a stage decision
+    auto fetch_order_dma = [&](int m, bool withc) __attribute__((always_inline)) {
+        if (tid < nh) {
+            const int js = nlat - 1 - tid, w0 = 64 * wave;
+            lds_dma16(a.F + (size_t)tid * (L + 1) + m, &spf[0][w0]);
+            if (withc) lds_dma16(a.FC + (size_t)tid * (L + 1) + m, &spf[2][w0]);
+            if (js != tid) {
+                lds_dma16(a.F + (size_t)js * (L + 1) + m, &spf[1][w0]);
+                if (withc) lds_dma16(a.FC + (size_t)js * (L + 1) + m, &spf[3][w0]);
+            } else {
+                spf[1][tid] = make_double2(0.0, 0.0); spf[3][tid] = make_double2(0.0, 0.0);
+            }
+        }
+    };
     // the work of item i in stage sn (record sct); stolen: run for an owner
     // that is not resident
     auto run_item = [&](int i, unsigned sn, bool stolen) __attribute__((always_inline)) {
@@ -940,7 +997,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
                 case SK_HEAD: sle_row_head(a, i, (int)sct[CT_IO], (double*)RB); break;
                 case SK_START: sle_row_start<MinW>(a, i, sct[CT_CINT], RB, TW); break;
                 case SK_ROWS:
-                    sle_row_rows<MinW>(a, i, srow, RB, TW);
+                    sle_row_rows<MinW>(a, i, srow, RB, TW, rows_pf && i == own ? RB + (size_t)4 * N : nullptr);
                     srow_item = i; srow_stage = sn;
                     break;
                 case SK_UPD: sle_row_upd(a, i, sct[CT_DPHI], srow, srow_item == i && srow_stage + 1u == sn, redmax); break;
@@ -1084,6 +1141,7 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
                     // the one place where work items run (keeps the transforms inlined once)
                     if (item != own) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                     run_item(item, sn, item != own);
+                    rows_pf = false;
                     if (pending_publish && pending_sn == sn) { publish(sn, sct); pending_publish = false; }
                     if ((Multi ? item % B : item) != own && tid == 0)   // an item of another block
                         __hip_atomic_fetch_add(a.bar + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1108,17 +1166,28 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
             if (w == -2) break;
             if (sync) {
                 // in step: the decision from the stage's outputs (which stay
-                // until this block's item of stage sn + 1 is done); an order
-                // block's spectra for its next work (after START: F; after
-                // ROWS: F and FC) are loaded first, in flight over the decision
+                // until this block's item of stage sn + 1 is done). The
+                // spectra of the block's next work are loaded first, in flight
+                // over the decision, and staged in LDS after it: an order
+                // block's after START (F) and ROWS (F and FC), a row block's
+                // pair after ORD0 and UPD (F2, used when ROWS follows)
                 const int k0 = (int)sct[CT_KIND];
-                if (own >= npairs && (k0 == SK_START || k0 == SK_ROWS)) {
-                    fetch_order(own - npairs, k0 == SK_ROWS);
-                    pf_ok = true;
-                }
+                const bool opf = own >= npairs && (k0 == SK_START || k0 == SK_ROWS);
+                const bool rpf = own < npairs && (k0 == SK_ORD0 || k0 == SK_UPD) && L < 64 * SLE_RPF;
+                const bool rw = rpf && wave < 2 && 2 * own + wave < nlat;   // this wavefront's row
+                if (opf) fetch_order_dma(own - npairs, k0 == SK_ROWS);
+                if (rw)   // staged beyond the FFT buffers, unused by the decision
+                    row_spectra_issue(a.F2 + (size_t)own * (L + 1) * 4 + 2 * wave, L, lane,
+                                      RB + (size_t)4 * N + (size_t)wave * 2 * SLE_RPF * 64);
                 sle_decide(a, sct, snx, scratch(), redmax);
                 if (tid < CT_N) sct[tid] = snx[tid];
                 __syncthreads();
+                // the copies are complete before anything reads or reuses their LDS
+                // (each thread of an order block reads back its own entries,
+                // each row wavefront its own row's)
+                if (opf || rw) __builtin_amdgcn_s_waitcnt(0);
+                pf_ok = opf;
+                rows_pf = rpf && (int)sct[CT_KIND] == SK_ROWS;
                 sn += 1u;
                 // the record of stage sn is published (pending_publish) after this
                 // block's own work of the stage, off the stage's critical path
