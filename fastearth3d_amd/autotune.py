@@ -23,6 +23,7 @@ The knobs (defaults in csrc/vh_tuned.h):
   VH_DRIFT_MINW    wavefronts per SIMD the drift's register budget must allow
   VH_DRIFT_AK      k-steps of the G_l fragment the drift's wavefront loads at a time
   VH_DRIFT_ONEPASS 1: the drift builds its forcing in one pass with LDS atomics
+  VH_ROWS_RJ       latitudes per block of the grouped row pass
 The sea-level solve's launch shape is a run-time knob (VH_T_SLE_BLOCKS),
 tuned per ensemble by vh_ensemble_tune.
 """
@@ -50,6 +51,7 @@ KNOBS = {
     "VH_DRIFT_MINW": ("vh_drift.hip", [1, 4], ["k_drift"]),
     "VH_DRIFT_AK": ("vh_drift.hip", [8, 16, 32], ["k_drift"]),
     "VH_DRIFT_ONEPASS": ("vh_drift.hip", [0, 1], ["k_drift"]),
+    "VH_ROWS_RJ": ("vh_tensor.hip", [2, 4, 8], ["k_t3_rows"]),
 }
 
 

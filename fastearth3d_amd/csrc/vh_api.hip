@@ -2236,6 +2236,7 @@ void enable_lateral(vh_model* h, const double* pert) {
     h->rate_max = r3max;
     for (int e = 0; e < ne; e++) h->rate_max = std::max(h->rate_max, h->MkPerDt[e]);
     h->lat_visc = true;
+    vh_t3_rows_prepare(h->s);
     h->sync();
 }
 

@@ -206,6 +206,8 @@ void vh_tsh_analysis(hipStream_t st, const vh_sht& s, const double2* Fd, int nb,
 // mode 3: as mode 0 with the trapezoidal weight M / (1 + M/2), M = rate*dt
 void vh_t3_rows(hipStream_t st, const vh_sht& s, double2* Fd, int nb, int mode, const double* rate, int bpe,
                 double dt, double* grid, const double2* Fin = nullptr);
+// the row pass's launch attributes for this grid, set outside any stream capture
+void vh_t3_rows_prepare(const vh_sht& s);
 // X [3 ne3d][nch][nlm]; with nch = 6 the toroidal channels come from taut
 // and the nodal W drift dWt ([nmn][2][nk])
 void vh_t3_gather(hipStream_t st, const vh_dims& d, const vh_dev& v, const int* e3em, int ne3d, const double* tau,

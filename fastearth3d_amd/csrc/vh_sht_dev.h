@@ -327,6 +327,16 @@ __device__ __forceinline__ void st_coh(double2* p, double2 v) {
     st_coh((double*)p, v.x);
     st_coh((double*)p + 1, v.y);
 }
+// 16 bytes per lane from global memory straight into LDS: lane k's bytes land
+// at dst + k, dst the same for the whole wavefront. No registers hold them;
+// they are in LDS once the wavefront's vector memory counter has drained.
+// Coh: coherent at agent scope, as ld_coh (data other running blocks wrote).
+template <bool Coh = false>
+__device__ __forceinline__ void lds_dma16(const double2* src, double2* dst) {
+    __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)dst, 16, 0,
+                                     Coh ? 16 /* sc1 */ : 0);
+}
+
 template <bool Coh>
 __device__ __forceinline__ double2 ld_g(const double2* p) {
     if constexpr (Coh) return ld_coh(p); else return *p;

@@ -528,14 +528,6 @@ __device__ __forceinline__ void sle_row_start(const sle_solve_args& a, int p, do
     }
 }
 
-// 16 bytes per lane from global memory (coherent, as ld_coh) straight into
-// LDS: lane k's bytes land at dst + k, dst the same for the whole wavefront.
-// No registers hold them; they are in LDS once the vector memory counter of
-// the wavefront has drained.
-__device__ __forceinline__ void lds_dma16(const double2* src, double2* dst) {
-    __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)dst, 16, 0,
-                                     16 /* sc1: coherent at agent scope */);
-}
 // A row block's pair spectra for a ROWS item, loaded over a stage decision:
 // orders m = lane + 64 u of wavefront w's row (F2 entries 2w, 2w + 1 of the
 // pair) into stg[2u][lane], stg[2u + 1][lane] (stg: 2 SLE_RPF x 64 complex
@@ -546,8 +538,8 @@ __device__ __forceinline__ void row_spectra_issue(const double2* F2p, int L, int
     for (int u = 0; u < SLE_RPF; u++) {
         if (lane + 64 * u <= L) {
             const double2* f = F2p + (size_t)(lane + 64 * u) * 4;
-            lds_dma16(f, stg + 2 * u * 64);
-            lds_dma16(f + 1, stg + (2 * u + 1) * 64);
+            lds_dma16<true>(f, stg + 2 * u * 64);
+            lds_dma16<true>(f + 1, stg + (2 * u + 1) * 64);
         }
     }
 }
@@ -971,11 +963,11 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
     auto fetch_order_dma = [&](int m, bool withc) __attribute__((always_inline)) {
         if (tid < nh) {
             const int js = nlat - 1 - tid, w0 = 64 * wave;
-            lds_dma16(a.F + (size_t)tid * (L + 1) + m, &spf[0][w0]);
-            if (withc) lds_dma16(a.FC + (size_t)tid * (L + 1) + m, &spf[2][w0]);
+            lds_dma16<true>(a.F + (size_t)tid * (L + 1) + m, &spf[0][w0]);
+            if (withc) lds_dma16<true>(a.FC + (size_t)tid * (L + 1) + m, &spf[2][w0]);
             if (js != tid) {
-                lds_dma16(a.F + (size_t)js * (L + 1) + m, &spf[1][w0]);
-                if (withc) lds_dma16(a.FC + (size_t)js * (L + 1) + m, &spf[3][w0]);
+                lds_dma16<true>(a.F + (size_t)js * (L + 1) + m, &spf[1][w0]);
+                if (withc) lds_dma16<true>(a.FC + (size_t)js * (L + 1) + m, &spf[3][w0]);
             } else {
                 spf[1][tid] = make_double2(0.0, 0.0); spf[3][tid] = make_double2(0.0, 0.0);
             }

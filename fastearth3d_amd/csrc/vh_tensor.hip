@@ -37,6 +37,7 @@
 // contiguous, so the Legendre kernels read and write coalesced rows and the
 // row kernel's column access is shared through L2 by an XCD-aware block
 // order); batch b = 3*i3 + s.
+#include <atomic>
 #include <vector>
 #include <algorithm>
 #include "vh_internal.h"
@@ -451,9 +452,10 @@ __global__ void __launch_bounds__(256) k_tsh_analysis(vh_sht s, const double2* _
 //   mode 2: forward FFT of grid[b][dy][j][i] into Fd
 //   mode 3: as mode 0 with -M/(1 + M/2), M = rate*dt (trapezoidal rule,
 //           src/vilma_response.f90:1447-1478, in increment form)
+// twl: the twiddles are copied into the block's LDS (else read from s.twp).
 __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, double2* Fd, int nb, int mode,
                                                  const double* __restrict__ rate, int bpe, double dt,
-                                                 double* __restrict__ grid) {
+                                                 double* __restrict__ grid, int twl) {
     extern __shared__ double2 lds[];
     const int L = s.L, N = s.nphi, nlat = s.nlat;
     const int lb = xcd_block(nb * nlat);
@@ -464,29 +466,28 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, d
     const int nbuf = fft_fixed_size(N) ? 1 : 2;
     double2* a = lds + (size_t)wave * nbuf * N;
     double2* c = nbuf == 2 ? a + N : nullptr;
-    double* Mrow = (double*)(lds + (size_t)3 * nbuf * N);
+    double2* TW = lds + (size_t)3 * nbuf * N;                    // [s.ntw] when twl
+    double* Rrow = (double*)(TW + (twl ? s.ntw : 0));            // the batch element's rate row [N]
+    const double2* tw = twl ? TW : nullptr;
     const int d0 = 2 * wave, d1 = 2 * wave + 1;
     const double2* I0 = Fin + ((size_t)b * 6 + d0) * (L + 1) * nlat + j;
     const double2* I1 = Fin + ((size_t)b * 6 + d1) * (L + 1) * nlat + j;
     double2* F0 = Fd + ((size_t)b * 6 + d0) * (L + 1) * nlat + j;
     double2* F1 = Fd + ((size_t)b * 6 + d1) * (L + 1) * nlat + j;
-    // the rate row's loads are issued first and land in LDS once the
-    // spectra's loads are in flight, so the two latencies overlap
-    // (two per thread: rows up to 384 points; the rest of a longer row
-    // goes straight to LDS)
-    constexpr int RQ = 2;
-    double rv[RQ];
+    // the twiddles and the rate row go into LDS first, copied by the memory
+    // unit (no registers held) while the spectra's loads are in flight; the
+    // block waits for them once, before its first transform
+    if (twl)
+        for (int t0 = wave * 64; t0 < s.ntw; t0 += 192)
+            if (t0 + lane < s.ntw) lds_dma16(s.twp + t0 + lane, TW + t0);
     const bool prod = mode == 0 || mode == 3;
     if (prod) {
         const double* rr = rate + ((size_t)(b / bpe) * nlat + j) * N;
-#pragma unroll
-        for (int q = 0; q < RQ; q++) {
-            const int i = threadIdx.x + q * blockDim.x;
-            rv[q] = i < N ? rr[i] : 0.0;
-        }
-        for (int i = threadIdx.x + RQ * blockDim.x; i < N; i += blockDim.x) {
-            const double M = rr[i] * dt;
-            Mrow[i] = mode == 3 ? M / (1.0 + 0.5 * M) : M;
+        if ((N & 1) == 0) {   // 16-byte aligned pairs of the row
+            for (int c0 = wave * 64; c0 < N / 2; c0 += 192)
+                if (c0 + lane < N / 2) lds_dma16((const double2*)rr + c0 + lane, (double2*)Rrow + c0);
+        } else {
+            for (int i = threadIdx.x; i < N; i += blockDim.x) Rrow[i] = rr[i];
         }
     }
     double2* z;
@@ -501,16 +502,6 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, d
                 A[u] = m <= L ? I0[(size_t)m * nlat] : make_double2(0.0, 0.0);
                 B[u] = m <= L ? I1[(size_t)m * nlat] : make_double2(0.0, 0.0);
             }
-            if (prod && m0 == 0) {   // the rate values arrived while the spectra are in flight
-#pragma unroll
-                for (int q = 0; q < RQ; q++) {
-                    const int i = threadIdx.x + q * blockDim.x;
-                    if (i < N) {
-                        const double M = rv[q] * dt;
-                        Mrow[i] = mode == 3 ? M / (1.0 + 0.5 * M) : M;   // trapezoidal weight
-                    }
-                }
-            }
 #pragma unroll
             for (int u = 0; u < VH_FFT_U; u++) {
                 const int m = m0 + lane + 64 * u;
@@ -520,17 +511,18 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, d
                 if (m > 0) a[N - m] = make_double2(A[u].x + B[u].y, -A[u].y + B[u].x);
             }
         }
-        wave_lds_sync();
-        z = fft_wave_ip<true>(a, c, s, lane);
+        __builtin_amdgcn_s_waitcnt(0);   // the copies into LDS are complete
+        __syncthreads();                 // and visible to the whole block
+        z = fft_wave_ip<true>(a, c, s, lane, tw);
         if (mode == 1) {
             double* g0 = grid + (((size_t)b * 6 + d0) * nlat + j) * N;
             double* g1 = grid + (((size_t)b * 6 + d1) * nlat + j) * N;
             for (int i = lane; i < N; i += 64) { g0[i] = z[i].x; g1[i] = z[i].y; }
             return;
         }
-        __syncthreads();   // the rate row is written by the whole block
         for (int i = lane; i < N; i += 64) {
-            const double M = Mrow[i];
+            const double M0 = Rrow[i] * dt;
+            const double M = mode == 3 ? M0 / (1.0 + 0.5 * M0) : M0;   // trapezoidal weight
             z[i] = make_double2(-M * z[i].x, -M * z[i].y);
         }
         wave_lds_sync();
@@ -538,11 +530,12 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, d
         const double* g0 = grid + (((size_t)b * 6 + d0) * nlat + j) * N;
         const double* g1 = grid + (((size_t)b * 6 + d1) * nlat + j) * N;
         for (int i = lane; i < N; i += 64) a[i] = make_double2(g0[i], g1[i]);
-        wave_lds_sync();
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
         z = a;
     }
     double2* other = (z == a) ? c : a;
-    double2* y = fft_wave_ip<false>(z, other, s, lane);
+    double2* y = fft_wave_ip<false>(z, other, s, lane, tw);
     const double scale = 2.0 * VH_PI / N;
     for (int m = lane; m <= L; m += 64) {
         double2 zm = y[m], zc = y[(N - m) % N];
@@ -552,6 +545,98 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, d
         const double2 B = make_double2(0.5 * Dd.y, -0.5 * Dd.x);
         F0[(size_t)m * nlat] = make_double2(scale * A.x, scale * A.y);
         F1[(size_t)m * nlat] = make_double2(scale * B.x, scale * B.y);
+    }
+}
+
+// Row pass of modes 0 and 3 over RJ consecutive latitudes and one dyad pair
+// per block, for the grid widths with a compile-time transform: wavefront w
+// transforms latitude j0 + w; the spectra are read and written as whole
+// lines (the RJ latitudes of one order and dyad are RJ x 16 contiguous
+// bytes), so the vector memory unit handles RJ times fewer lines than with
+// one latitude per block. Same arithmetic as k_t3_rows, so the same bits.
+// LDS: RJ transform buffers of N points (stride N + 1), the twiddles, RJ rate rows.
+template <int N, int RJ>
+__global__ void __launch_bounds__(64 * RJ) k_t3_rows_g(vh_sht s, const double2* Fin, double2* Fd, int nb, int mode,
+                                                       const double* __restrict__ rate, int bpe, double dt) {
+    static_assert(64 % RJ == 0, "RJ divides the wavefront");
+    constexpr int MS = 64;                     // orders per load round (64 RJ threads over RJ latitudes)
+    constexpr int MU = (N / 2 + MS - 1) / MS;  // load rounds (2L + 1 <= N)
+    constexpr int NP = N + 1;                  // buffer stride: the RJ buffers' entry m in distinct LDS banks
+    extern __shared__ double2 lds[];
+    const int L = s.L, nlat = s.nlat, ng = (nlat + RJ - 1) / RJ;
+    const int lb = xcd_block(nb * ng * 3);
+    if (lb >= nb * ng * 3) return;
+    // the three pairs of one (batch, latitude group) are neighbours on one XCD (shared rate rows)
+    const int p = lb % 3, g = (lb / 3) % ng, b = lb / (3 * ng);
+    const int j0 = g * RJ, nj = min(RJ, nlat - j0);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    double2* TW = lds + (size_t)RJ * NP;             // [s.ntw]
+    double* Rr = (double*)(TW + s.ntw);              // [RJ][N]
+    // twiddles and this wavefront's rate row copied into LDS by the memory unit
+    for (int t0 = wave * 64; t0 < s.ntw; t0 += 64 * RJ)
+        if (t0 + lane < s.ntw) lds_dma16(s.twp + t0 + lane, TW + t0);
+    if (wave < nj) {
+        const double2* rr = (const double2*)(rate + ((size_t)(b / bpe) * nlat + j0 + wave) * N);
+        double2* rd = (double2*)(Rr + (size_t)wave * N);
+#pragma unroll
+        for (int c0 = 0; c0 < N / 2; c0 += 64)
+            if (N / 2 % 64 == 0 || c0 + lane < N / 2) lds_dma16(rr + c0 + lane, rd + c0);
+    }
+    // spectra: thread (order mm + 64u, latitude j0 + jj)
+    const int jj = tid % RJ, mm = tid / RJ;
+    const bool jok = jj < nj;
+    const size_t dstr = (size_t)(L + 1) * nlat;      // between dyads
+    const size_t off = ((size_t)b * 6 + 2 * p) * dstr + j0 + jj;
+    double2 A[MU], B[MU];
+#pragma unroll
+    for (int u = 0; u < MU; u++) {
+        const int m = mm + MS * u;
+        const bool ok = jok && m <= L;
+        A[u] = ok ? Fin[off + (size_t)m * nlat] : make_double2(0.0, 0.0);
+        B[u] = ok ? Fin[off + dstr + (size_t)m * nlat] : make_double2(0.0, 0.0);
+    }
+    {
+        double2* a = lds + (size_t)wave * NP;          // the orders without spectrum
+        for (int i = L + 1 + lane; i < N - L; i += 64) a[i] = make_double2(0.0, 0.0);
+        double2* aj = lds + (size_t)jj * NP;
+#pragma unroll
+        for (int u = 0; u < MU; u++) {
+            const int m = mm + MS * u;
+            if (!jok || m > L) continue;
+            double2 x = A[u], y = B[u];
+            if (m == 0) { x.y = 0.0; y.y = 0.0; }
+            aj[m] = make_double2(x.x - y.y, x.y + y.x);
+            if (m > 0) aj[N - m] = make_double2(x.x + y.y, -x.y + y.x);
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);   // the copies into LDS are complete
+    __syncthreads();
+    if (wave < nj) {
+        double2* z = lds + (size_t)wave * NP;
+        fft_passes_ip<N, 0, true>(z, TW, lane);
+        const double* R = Rr + (size_t)wave * N;
+        for (int i = lane; i < N; i += 64) {
+            const double M0 = R[i] * dt;
+            const double M = mode == 3 ? M0 / (1.0 + 0.5 * M0) : M0;   // trapezoidal weight
+            z[i] = make_double2(-M * z[i].x, -M * z[i].y);
+        }
+        wave_lds_sync();
+        fft_passes_ip<N, 0, false>(z, TW, lane);
+    }
+    __syncthreads();
+    const double scale = 2.0 * VH_PI / N;
+    const double2* y = lds + (size_t)jj * NP;
+#pragma unroll
+    for (int u = 0; u < MU; u++) {
+        const int m = mm + MS * u;
+        if (!jok || m > L) continue;
+        double2 zm = y[m], zc = y[(N - m) % N];
+        zc.y = -zc.y;
+        const double2 Aq = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+        const double2 Dd = csub(zm, zc);
+        const double2 Bq = make_double2(0.5 * Dd.y, -0.5 * Dd.x);
+        Fd[off + (size_t)m * nlat] = make_double2(scale * Aq.x, scale * Aq.y);
+        Fd[off + dstr + (size_t)m * nlat] = make_double2(scale * Bq.x, scale * Bq.y);
     }
 }
 
@@ -688,8 +773,10 @@ static size_t synth_lds(const vh_sht& s, int nch) {
     return nch == 4 ? (size_t)VH_TSH_TB_SYNTH * 5 * (s.L + 1) * sizeof(double2)
                     : (size_t)TSH_TB6 * 8 * (s.L + 1) * sizeof(double2);
 }
-static size_t rows_lds(const vh_sht& s) {
-    return (size_t)3 * (fft_fixed_size(s.nphi) ? 1 : 2) * s.nphi * sizeof(double2) + (size_t)s.nphi * sizeof(double);
+// LDS of the row pass: the wavefronts' FFT buffers, the twiddles (twl), the rate row
+static size_t rows_lds(const vh_sht& s, bool twl) {
+    return (size_t)3 * (fft_fixed_size(s.nphi) ? 1 : 2) * s.nphi * sizeof(double2) +
+           (twl ? (size_t)s.ntw * sizeof(double2) : 0) + (size_t)s.nphi * sizeof(double);
 }
 
 void vh_tsh_synth(hipStream_t st, const vh_sht& s, const double2* X, int nb, double2* Fd, int nch) {
@@ -730,10 +817,50 @@ void vh_tsh_analysis(hipStream_t st, const vh_sht& s, const double2* Fd, int nb,
     HIPCHECK(hipGetLastError());
 }
 
+// The grouped row pass (k_t3_rows_g) of one grid width: its LDS bytes, and
+// the dynamic-LDS limit raised once per process (a function attribute; set
+// before any graph capture through vh_t3_rows_prepare).
+template <int N>
+static size_t rows_g_lds(const vh_sht& s) {
+    return ((size_t)VH_ROWS_RJ * (N + 1) + s.ntw) * sizeof(double2) + (size_t)VH_ROWS_RJ * N * sizeof(double);
+}
+template <int N>
+static void rows_g_limit(size_t lds) {
+    static std::atomic<size_t> set{0};
+    if (lds > 65536 && set.load() < lds) {
+        HIPCHECK(hipFuncSetAttribute((const void*)k_t3_rows_g<N, VH_ROWS_RJ>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        set.store(lds);
+    }
+}
+// true (and the attribute set) when the grouped pass serves this grid
+static bool rows_g_ready(const vh_sht& s) {
+    if (2 * s.L + 1 > s.nphi || (s.nphi & 1)) return false;
+    switch (s.nphi) {
+        case 256: { const size_t l = rows_g_lds<256>(s); if (l > 160 * 1024) return false; rows_g_limit<256>(l); return true; }
+        case 384: { const size_t l = rows_g_lds<384>(s); if (l > 160 * 1024) return false; rows_g_limit<384>(l); return true; }
+        default: return false;
+    }
+}
+void vh_t3_rows_prepare(const vh_sht& s) { (void)rows_g_ready(s); }
+
 void vh_t3_rows(hipStream_t st, const vh_sht& s, double2* Fd, int nb, int mode, const double* rate, int bpe,
                 double dt, double* grid, const double2* Fin) {
     if (nb <= 0) return;
-    k_t3_rows<<<xcd_grid(nb * s.nlat), 192, rows_lds(s), st>>>(s, Fin ? Fin : Fd, Fd, nb, mode, rate, bpe, dt, grid);
+    if ((mode == 0 || mode == 3) && rows_g_ready(s)) {
+        const int ng = (s.nlat + VH_ROWS_RJ - 1) / VH_ROWS_RJ;
+        const dim3 g(xcd_grid(nb * ng * 3));
+        const double2* in = Fin ? Fin : Fd;
+        if (s.nphi == 256)
+            k_t3_rows_g<256, VH_ROWS_RJ><<<g, 64 * VH_ROWS_RJ, rows_g_lds<256>(s), st>>>(s, in, Fd, nb, mode, rate, bpe, dt);
+        else
+            k_t3_rows_g<384, VH_ROWS_RJ><<<g, 64 * VH_ROWS_RJ, rows_g_lds<384>(s), st>>>(s, in, Fd, nb, mode, rate, bpe, dt);
+        HIPCHECK(hipGetLastError());
+        return;
+    }
+    const bool twl = rows_lds(s, true) <= (size_t)64 * 1024;   // within the default dynamic LDS limit
+    k_t3_rows<<<xcd_grid(nb * s.nlat), 192, rows_lds(s, twl), st>>>(s, Fin ? Fin : Fd, Fd, nb, mode, rate, bpe, dt,
+                                                                   grid, twl ? 1 : 0);
     HIPCHECK(hipGetLastError());
 }
 

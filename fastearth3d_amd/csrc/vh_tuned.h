@@ -30,3 +30,6 @@
 #ifndef VH_DRIFT_ONEPASS
 #define VH_DRIFT_ONEPASS 1  // 1: the drift builds its forcing in one pass with LDS atomics
 #endif
+#ifndef VH_ROWS_RJ
+#define VH_ROWS_RJ 8        // latitudes per block of the grouped row pass (one wavefront each)
+#endif
