@@ -17,7 +17,9 @@
 //             Gauss weights: so Delta^T = K^{-1} Delta K and the forward
 //             kernels serve the adjoint between two diagonal scalings;
 //   rates     dJ/dM(grid) = -(2 pi/nphi) w_j sum_dyads Omega (S K c_h)(S c).
+#include <atomic>
 #include "vh_internal.h"
+#include "vh_tuned.h"
 #include "vh_radial_dev.h"
 #include "vh_sht_dev.h"
 #include "vh_adjoint.h"
@@ -316,6 +318,143 @@ __global__ void __launch_bounds__(192) k_t3_rows_adj(vh_sht s, double2* __restri
     for (int i = threadIdx.x; i < N; i += blockDim.x) out[i] += f * (r0[i] + r1[i] + r2[i]);
 }
 
+// The same reverse row pass over RJ consecutive latitudes per block (one
+// wavefront each; every wavefront runs its latitude's three dyad pairs and
+// three shapes in the order above, the pairs' products in separate
+// accumulators summed at the end as above), for the grid widths with a
+// compile-time transform: the spectra are read and written as whole lines
+// of RJ latitudes, so the vector memory unit handles RJ times fewer lines.
+// The operations of k_t3_rows_adj in the same order with the in-place
+// transform (results agree to rounding). LDS: RJ transform buffers (stride
+// N + 1 against bank conflicts), the twiddles, RJ rate rows.
+template <int N, int RJ>
+__global__ void __launch_bounds__(64 * RJ, VH_ROWS_ADJ_MINW) k_t3_rows_adj_g(vh_sht s, double2* __restrict__ Fa,
+                                                           const double2* __restrict__ Ff, int ne3,
+                                                           const double* __restrict__ rate, double dt,
+                                                           double* __restrict__ rate_h) {
+    static_assert(N % 64 == 0 && 64 % RJ == 0, "grouped reverse row pass shape");
+    constexpr int RU = N / 64, MS = 64, MU = (N / 2 + MS - 1) / MS, NP = N + 1;
+    extern __shared__ double2 lds[];
+    const int L = s.L, nlat = s.nlat, ng = (nlat + RJ - 1) / RJ;
+    const int lb = xcd_block_a(ne3 * ng);
+    if (lb >= ne3 * ng) return;
+    const int i3 = lb / ng, j0 = (lb % ng) * RJ, nj = min(RJ, nlat - j0);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    double2* TW = lds + (size_t)RJ * NP;
+    double* Rr = (double*)(TW + s.ntw);
+    for (int t0 = wave * 64; t0 < s.ntw; t0 += 64 * RJ)
+        if (t0 + lane < s.ntw) lds_dma16(s.twp + t0 + lane, TW + t0);
+    if (wave < nj) {
+        const double2* rr = (const double2*)(rate + ((size_t)i3 * nlat + j0 + wave) * N);
+        double2* rd = (double2*)(Rr + (size_t)wave * N);
+#pragma unroll
+        for (int c0 = 0; c0 < N / 2; c0 += 64)
+            if (N / 2 % 64 == 0 || c0 + lane < N / 2) lds_dma16(rr + c0 + lane, rd + c0);
+    }
+    const int jj = tid % RJ, mm = tid / RJ;
+    const bool jok = jj < nj;
+    const size_t dstr = (size_t)(L + 1) * nlat;
+    double2* z = lds + (size_t)wave * NP;         // this wavefront's latitude
+    double2* aj = lds + (size_t)jj * NP;          // the latitude this thread loads and stores
+    // rows (2p, 2p + 1) of batch b from F into the buffers (after a barrier)
+    auto fill = [&](const double2* F, size_t off) __attribute__((always_inline)) {
+        double2 A[MU], B[MU];
+#pragma unroll
+        for (int u = 0; u < MU; u++) {
+            const int m = mm + MS * u;
+            const bool ok = jok && m <= L;
+            A[u] = ok ? F[off + (size_t)m * nlat] : make_double2(0.0, 0.0);
+            B[u] = ok ? F[off + dstr + (size_t)m * nlat] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();   // every wavefront is done with its buffer
+        for (int i = L + 1 + lane; i < N - L; i += 64) z[i] = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int u = 0; u < MU; u++) {
+            const int m = mm + MS * u;
+            if (!jok || m > L) continue;
+            double2 x = A[u], y = B[u];
+            if (m == 0) { x.y = 0.0; y.y = 0.0; }
+            aj[m] = make_double2(x.x - y.y, x.y + y.x);
+            if (m > 0) aj[N - m] = make_double2(x.x + y.y, -x.y + y.x);
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    };
+    double acc[3][RU];
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int u = 0; u < RU; u++) acc[p][u] = 0.0;
+    const double* R = Rr + (size_t)wave * N;
+    const double scale = 2.0 * VH_PI / N;
+    for (int sh = 0; sh < 3; sh++) {
+        const int b = 3 * i3 + sh;
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+            const size_t off = ((size_t)b * 6 + 2 * p) * dstr + j0 + jj;
+            const double omA = p == 2 ? 0.5 : 1.0;   // dyad weights: tp counts 1/2, the rest 1
+            double2 gf[RU];
+            fill(Ff, off);
+            if (wave < nj) {
+                fft_passes_ip<N, 0, true>(z, TW, lane);
+#pragma unroll
+                for (int u = 0; u < RU; u++) gf[u] = z[lane + 64 * u];
+            }
+            fill(Fa, off);
+            if (wave < nj) {
+                fft_passes_ip<N, 0, true>(z, TW, lane);
+#pragma unroll
+                for (int u = 0; u < RU; u++) {
+                    const int i = lane + 64 * u;
+                    const double2 g = z[i];
+                    acc[p][u] = fma(omA * g.x, gf[u].x, fma(g.y, gf[u].y, acc[p][u]));
+                    const double M = R[i] * dt;
+                    z[i] = make_double2(-M * g.x, -M * g.y);
+                }
+                wave_lds_sync();
+                fft_passes_ip<N, 0, false>(z, TW, lane);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < MU; u++) {
+                const int m = mm + MS * u;
+                if (!jok || m > L) continue;
+                double2 zm = aj[m], zc = aj[(N - m) % N];
+                zc.y = -zc.y;
+                const double2 Aq = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+                const double2 Dd = csub(zm, zc);
+                const double2 Bq = make_double2(0.5 * Dd.y, -0.5 * Dd.x);
+                Fa[off + (size_t)m * nlat] = make_double2(scale * Aq.x, scale * Aq.y);
+                Fa[off + dstr + (size_t)m * nlat] = make_double2(scale * Bq.x, scale * Bq.y);
+            }
+        }
+    }
+    if (wave < nj) {
+        const int j = j0 + wave;
+        const double f = -(2.0 * VH_PI / N) * s.w[j] * dt;
+        double* out = rate_h + ((size_t)i3 * nlat + j) * N;
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            const int i = lane + 64 * u;
+            out[i] += f * (acc[0][u] + acc[1][u] + acc[2][u]);
+        }
+    }
+}
+
+template <int N>
+static size_t rows_adj_g_lds(const vh_sht& s) {
+    return ((size_t)VH_ROWS_ADJ_RJ * (N + 1) + s.ntw) * sizeof(double2) + (size_t)VH_ROWS_ADJ_RJ * N * sizeof(double);
+}
+template <int N>
+static void rows_adj_g_limit(size_t lds) {
+    static std::atomic<size_t> set{0};
+    if (lds > 65536 && set.load() < lds) {
+        HIPCHECK(hipFuncSetAttribute((const void*)k_t3_rows_adj_g<N, VH_ROWS_ADJ_RJ>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        set.store(lds);
+    }
+}
+
 void vh_t3_pack_adj(hipStream_t st, const vh_dims& d, const vh_dev& v, const int* e3em, int ne3d, const double* n6,
                     const double* tauh, double2* Xa) {
     if (ne3d <= 0) return;
@@ -327,6 +466,20 @@ void vh_t3_pack_adj(hipStream_t st, const vh_dims& d, const vh_dev& v, const int
 void vh_t3_rows_adj(hipStream_t st, const vh_sht& s, double2* Fa, const double2* Ff, int ne3, const double* rate,
                     double dt, double* rate_h) {
     if (ne3 <= 0) return;
+    if (2 * s.L + 1 <= s.nphi && (s.nphi == 256 || s.nphi == 384)) {   // the grouped pass
+        const unsigned g = (unsigned)(((ne3 * ((s.nlat + VH_ROWS_ADJ_RJ - 1) / VH_ROWS_ADJ_RJ) + 7) / 8) * 8);
+        if (s.nphi == 256) {
+            const size_t l = rows_adj_g_lds<256>(s);
+            rows_adj_g_limit<256>(l);
+            k_t3_rows_adj_g<256, VH_ROWS_ADJ_RJ><<<g, 64 * VH_ROWS_ADJ_RJ, l, st>>>(s, Fa, Ff, ne3, rate, dt, rate_h);
+        } else {
+            const size_t l = rows_adj_g_lds<384>(s);
+            rows_adj_g_limit<384>(l);
+            k_t3_rows_adj_g<384, VH_ROWS_ADJ_RJ><<<g, 64 * VH_ROWS_ADJ_RJ, l, st>>>(s, Fa, Ff, ne3, rate, dt, rate_h);
+        }
+        HIPCHECK(hipGetLastError());
+        return;
+    }
     const size_t lds = (size_t)6 * s.nphi * sizeof(double2) + (size_t)s.nphi * sizeof(double);
     if (s.nphi > 64 * RA_U) throw vh_error("3-D adjoint: nphi above 512 is not supported by the row pass");
     const unsigned nblk = (unsigned)(((ne3 * s.nlat + 7) / 8) * 8);

@@ -553,7 +553,7 @@ __global__ void __launch_bounds__(192) k_t3_rows(vh_sht s, const double2* Fin, d
 // transforms latitude j0 + w; the spectra are read and written as whole
 // lines (the RJ latitudes of one order and dyad are RJ x 16 contiguous
 // bytes), so the vector memory unit handles RJ times fewer lines than with
-// one latitude per block. Same arithmetic as k_t3_rows, so the same bits.
+// one latitude per block. The operations of k_t3_rows in the same order.
 // LDS: RJ transform buffers of N points (stride N + 1), the twiddles, RJ rate rows.
 template <int N, int RJ>
 __global__ void __launch_bounds__(64 * RJ) k_t3_rows_g(vh_sht s, const double2* Fin, double2* Fd, int nb, int mode,

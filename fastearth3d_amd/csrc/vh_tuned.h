@@ -33,3 +33,9 @@
 #ifndef VH_ROWS_RJ
 #define VH_ROWS_RJ 8        // latitudes per block of the grouped row pass (one wavefront each)
 #endif
+#ifndef VH_ROWS_ADJ_RJ
+#define VH_ROWS_ADJ_RJ 4    // latitudes per block of the grouped reverse row pass
+#endif
+#ifndef VH_ROWS_ADJ_MINW
+#define VH_ROWS_ADJ_MINW 2  // wavefronts per SIMD the grouped reverse row pass's registers must allow
+#endif

@@ -132,6 +132,18 @@ def test_gradient_vs_gpu_finite_differences(rotation):
     _fd_check(lib.SolidEarth, c, p, ice, wgt, tgt, gp, gi, n_acc, vi, vp, 0.1, 1e-4, 1e-6)
 
 
+@pytest.mark.parametrize("lmax", [64, 96])
+def test_gradient_grouped_row_passes(lmax):
+    """Grid widths 256 and 384: the forward and reverse row passes that take
+    several latitudes per block, against GPU finite differences."""
+    from fastearth3d_amd import lib
+    c, p, ice, wgt, tgt = _case(lmax, 3)
+    assert c["params"].grid()[0] in (256, 384)
+    rsl, n_acc, gp, gi, _ = _gradients(c, p, ice, wgt, tgt, 2)
+    vi, vp = _directions(c, p, ice, 7)
+    _fd_check(lib.SolidEarth, c, p, ice, wgt, tgt, gp, gi, n_acc, vi, vp, 0.1, 1e-4, 1e-6)
+
+
 @pytest.mark.parametrize("rotation", [0, 1])
 def test_gradient_vs_oracle_finite_differences(oracle, rotation):
     """Directional derivatives of the oracle's forward (the reference step restated) against the GPU adjoint."""
