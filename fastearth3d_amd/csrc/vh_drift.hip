@@ -189,6 +189,7 @@ __global__ void __launch_bounds__(DRIFT_THREADS, VH_DRIFT_MINW) k_drift(drift_ar
     extern __shared__ double F[];   // [ncol][rw]
     const int2 job = a.jobs[blockIdx.x];
     const int l = job.x, m0 = job.y;
+    if (l <= 0) return;   // padding of the XCD-grouped job list
     constexpr int rw = RW, ts = RW / 2;
     const int ncol = 2 * a.nmn, nrow = ncol + 3;
     const int nsl = min(ts, l + 1 - m0);
@@ -279,6 +280,7 @@ __global__ void __launch_bounds__(DRIFT_THREADS) k_drift_adj(drift_args a, const
     extern __shared__ double Y[];   // [nrow][rw], then Fh [ncol][rw]
     const int2 job = a.jobs[blockIdx.x];
     const int l = job.x, m0 = job.y;
+    if (l <= 0) return;   // padding of the XCD-grouped job list
     constexpr int rw = RW, ts = RW / 2;
     const int ncol = 2 * a.nmn, nrow = ncol + 3, nk = a.nk;
     double* Fh = Y + (size_t)nrow * rw;
@@ -420,9 +422,22 @@ vh_drift_plan* vh_drift_plan_create(hipStream_t st, const vh_dims& d, const vh_d
         throw vh_error("drift: too many memory nodes for the LDS forcing tile (nmn = " + std::to_string(d.nmn) + ")");
     p->rw = rw; p->ts = rw / 2;   // the MFMA paths (rw >= 16) also need ncol <= 128
     p->lds = (size_t)std::max(ncol, 1) * rw * 8;
+    // jobs (degree, first order): blocks are dealt round-robin to the 8 XCDs
+    // (block b on XCD b % 8), so every job of one degree is placed on one
+    // XCD, whose L2 then fetches that degree's G_l once; degrees go to the
+    // least loaded XCD, largest first; (0, 0) pads the list
+    std::vector<int2> per[8];
+    for (int l = d.L; l >= 1; l--) {
+        int x = 0;
+        for (int q = 1; q < 8; q++)
+            if (per[q].size() < per[x].size()) x = q;
+        for (int m0 = 0; m0 <= l; m0 += p->ts) per[x].push_back(make_int2(l, m0));
+    }
+    size_t nmax = 0;
+    for (const auto& v : per) nmax = std::max(nmax, v.size());
     std::vector<int2> jobs;
-    for (int l = 1; l <= d.L; l++)
-        for (int m0 = 0; m0 <= l; m0 += p->ts) jobs.push_back(make_int2(l, m0));
+    for (size_t k = 0; k < nmax; k++)
+        for (int x = 0; x < 8; x++) jobs.push_back(k < per[x].size() ? per[x][k] : make_int2(0, 0));
     p->njobs = (int)jobs.size();
     HIPCHECK(hipMalloc((void**)&p->jobs, jobs.size() * sizeof(int2)));
     HIPCHECK(hipMemcpy(p->jobs, jobs.data(), jobs.size() * sizeof(int2), hipMemcpyHostToDevice));
