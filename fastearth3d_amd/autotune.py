@@ -45,7 +45,7 @@ KNOBS = {
     "VH_TSH_TB_SYNTH": ("vh_tensor.hip", [2, 3, 4], ["k_tsh_synth"]),
     "VH_TSH_SU": ("vh_tensor.hip", [1, 2, 4, 8], ["k_tsh_synth"]),
     "VH_DRIFT_RW": ("vh_drift.hip", [16, 32, 64], ["k_drift"]),
-    "VH_TCHUNK": ("vh_api.hip", [24, 36, 48, 72], ["k_tsh_synth", "k_t3_rows", "k_tsh_analysis"]),
+    "VH_TCHUNK": ("vh_api.hip", [36, 54, 72, 108], ["k_tsh_synth", "k_t3_rows", "k_tsh_analysis"]),
     "VH_TA_PF": ("vh_tensor.hip", [1, 2, 3, 4], ["k_tsh_analysis"]),
     "VH_TA_JC": ("vh_tensor.hip", [32, 48, 64, 96], ["k_tsh_analysis"]),
     "VH_DRIFT_MINW": ("vh_drift.hip", [1, 4], ["k_drift"]),

@@ -13,7 +13,7 @@
 #define VH_DRIFT_RW 32      // right-hand sides (orders x re/im) per block of the drift solve
 #endif
 #ifndef VH_TCHUNK
-#define VH_TCHUNK 72        // tensor batches per synthesis / row / analysis launch
+#define VH_TCHUNK 108       // tensor batches per synthesis / row / analysis launch
 #endif
 #ifndef VH_TA_PF
 #define VH_TA_PF 1          // k-steps of table fragments the tensor analysis keeps in flight
