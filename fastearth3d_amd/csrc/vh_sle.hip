@@ -1162,10 +1162,12 @@ __global__ void __launch_bounds__(256, MinW) k_sle_solve(sle_solve_args a) {
                 // spectra of the block's next work are loaded first, in flight
                 // over the decision, and staged in LDS after it: an order
                 // block's after START (F) and ROWS (F and FC), a row block's
-                // pair after ORD0 and UPD (F2, used when ROWS follows)
+                // pair after ORD0 and UPD (F2, used when ROWS follows; not in
+                // the instances with a register cap or several items per
+                // block, where its addressing would spill)
                 const int k0 = (int)sct[CT_KIND];
                 const bool opf = own >= npairs && (k0 == SK_START || k0 == SK_ROWS);
-                const bool rpf = own < npairs && (k0 == SK_ORD0 || k0 == SK_UPD) && L < 64 * SLE_RPF;
+                const bool rpf = MinW == 1 && !Multi && own < npairs && (k0 == SK_ORD0 || k0 == SK_UPD) && L < 64 * SLE_RPF;
                 const bool rw = rpf && wave < 2 && 2 * own + wave < nlat;   // this wavefront's row
                 if (opf) fetch_order_dma(own - npairs, k0 == SK_ROWS);
                 if (rw)   // staged beyond the FFT buffers, unused by the decision
