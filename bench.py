@@ -579,7 +579,7 @@ def main():
                          "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
+        print("bench.py: the solid-earth step runs on one GPU (there is no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
