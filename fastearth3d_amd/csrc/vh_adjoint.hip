@@ -387,6 +387,11 @@ __global__ void __launch_bounds__(64 * RJ, VH_ROWS_ADJ_MINW) k_t3_rows_adj_g(vh_
         for (int u = 0; u < RU; u++) acc[p][u] = 0.0;
     const double* R = Rr + (size_t)wave * N;
     const double scale = 2.0 * VH_PI / N;
+    static_assert(fft_last<N>::R == RU, "the last pass's outputs are the lane's longitudes");
+    double2 tl[fft_last<N>::R - 1];   // the last pass's twiddles, from LDS once the copies are in
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    fft_last_twiddles<N>(TW, lane, tl);
     for (int sh = 0; sh < 3; sh++) {
         const int b = 3 * i3 + sh;
 #pragma unroll
@@ -395,24 +400,22 @@ __global__ void __launch_bounds__(64 * RJ, VH_ROWS_ADJ_MINW) k_t3_rows_adj_g(vh_
             const double omA = p == 2 ? 0.5 : 1.0;   // dyad weights: tp counts 1/2, the rest 1
             double2 gf[RU];
             fill(Ff, off);
-            if (wave < nj) {
-                fft_passes_ip<N, 0, true>(z, TW, lane);
-#pragma unroll
-                for (int u = 0; u < RU; u++) gf[u] = z[lane + 64 * u];
-            }
+            if (wave < nj)   // the forward rows to real space: kept in registers
+                fft_ip_reg_last<N, true>(z, TW, tl, lane, [&](int r, int, double2 g) __attribute__((always_inline)) {
+                    gf[r] = g;
+                    return g;
+                });
             fill(Fa, off);
             if (wave < nj) {
-                fft_passes_ip<N, 0, true>(z, TW, lane);
-#pragma unroll
-                for (int u = 0; u < RU; u++) {
-                    const int i = lane + 64 * u;
-                    const double2 g = z[i];
-                    acc[p][u] = fma(omA * g.x, gf[u].x, fma(g.y, gf[u].y, acc[p][u]));
+                // the adjoint rows to real space; the product with the forward
+                // rows and with -rate dt applied as the transform stores
+                fft_ip_reg_last<N, true>(z, TW, tl, lane, [&](int r, int i, double2 g) __attribute__((always_inline)) {
+                    acc[p][r] = fma(omA * g.x, gf[r].x, fma(g.y, gf[r].y, acc[p][r]));
                     const double M = R[i] * dt;
-                    z[i] = make_double2(-M * g.x, -M * g.y);
-                }
+                    return make_double2(-M * g.x, -M * g.y);
+                });
                 wave_lds_sync();
-                fft_passes_ip<N, 0, false>(z, TW, lane);
+                fft_ip_reg_last<N, false>(z, TW, tl, lane, [](int, int, double2 g) __attribute__((always_inline)) { return g; });
             }
             __syncthreads();
 #pragma unroll

@@ -268,6 +268,50 @@ __device__ __forceinline__ void fft_passes_ip(double2* buf, const double2* tw, i
     }
 }
 
+// The in-place transform of length N with its last pass taken apart: passes
+// 0 .. nfac-2 as fft_passes_ip, then the last one (64 butterflies, one per
+// lane, stride 64) with its twiddles held in registers (tl: the lane's
+// R - 1 values, loaded once) and epi(r, i, v) applied to the lane's output r
+// (element i = lane + 64 r) before it is stored. The operations of
+// fft_passes_ip in the same order.
+template <int N, int P, bool INV>
+__device__ __forceinline__ void fft_passes_ip_head(double2* buf, const double2* tw, int lane) {
+    constexpr fft_cplan c = fft_make_cplan(N);
+    if constexpr (P < c.nfac - 1) {
+        fft_pass_ip<N, P, INV>(buf, tw, lane);
+        wave_lds_sync();
+        fft_passes_ip_head<N, P + 1, INV>(buf, tw, lane);
+    }
+}
+template <int N>
+struct fft_last {
+    static constexpr fft_cplan c = fft_make_cplan(N);
+    static constexpr int P = c.nfac - 1, R = c.fac[P], Ns = c.ns[P];
+    static_assert(c.nfac >= 3 && N / R == 64 && Ns == 64, "last pass: one butterfly per lane, no swizzle");
+};
+template <int N>
+__device__ __forceinline__ void fft_last_twiddles(const double2* tw, int lane, double2 (&tl)[fft_last<N>::R - 1]) {
+    constexpr int R = fft_last<N>::R;
+    const double2* tp = tw + fft_last<N>::c.twoff[fft_last<N>::P];
+#pragma unroll
+    for (int r = 1; r < R; r++) tl[r - 1] = tp[(r - 1) * 64 + lane];
+}
+template <int N, bool INV, class Epi>
+__device__ __forceinline__ void fft_ip_reg_last(double2* buf, const double2* tw,
+                                                const double2 (&tl)[fft_last<N>::R - 1], int lane, Epi epi) {
+    constexpr int R = fft_last<N>::R;
+    fft_passes_ip_head<N, 0, INV>(buf, tw, lane);
+    double2 v[R], V[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = buf[lane + r * 64];
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 1; r < R; r++) v[r] = cmul(v[r], cconj_if(tl[r - 1], INV));
+    dft_small<R>(v, V, INV);
+#pragma unroll
+    for (int r = 0; r < R; r++) buf[lane + r * 64] = epi(r, lane + r * 64, V[r]);
+}
+
 // grid widths with a compile-time transform: the default 4 lmax of lmax 64
 // and 192, and 384 (lmax 96 by default, lmax 128 on its 192 x 384 grid)
 #define VH_FFT_FIXED_SIZES(X) X(256) X(384) X(768)
